@@ -1,0 +1,212 @@
+/*
+ * paragraph.h -- C ABI of the MI355X graph-growth library (libparagraph.so).
+ *
+ * The reference (ParAGraph, one CUDA executable) has no library or FFI
+ * layer.  Its hot path sits behind two boundaries, and this ABI replaces the
+ * inner one:
+ *   - the per-year host->GPU call buildOneNodeConnections_timed(ABM*, Graph*,
+ *     new_nodes, ..., year, N, N0, N_final, max_batch, EpochTiming*)
+ *     (src/kernel.cu:1125-1139, declared includes/int2.cuh), which uploads
+ *     the score arrays, rebuilds the CSR, runs Stages 1-4 and downloads the
+ *     new edges every year;
+ *   - the host-side year boundary that inserts those edges and updates the
+ *     degrees (src/kernel.cu:1766-1815, src/graph.cu:537-618).
+ * The outer boundary (the `abm` command line and its edgelist/nodelist files,
+ * src/main.cu:29-114, src/graph.cu:336-447, src/graph.cu:909-1218) is kept by
+ * the `abm` executable built from paragraph_amd/csrc/abm_main.cpp on top of
+ * this ABI.
+ *
+ * Conventions: every function returns 0 on success or a PG_E* code, and
+ * pg_last_error() describes the failure; no C++ exception crosses the ABI.
+ * Structs start with a `size` field (set it to sizeof the struct).  The
+ * context owns all device memory; callers own host buffers.  A context is
+ * used by one host thread at a time.
+ */
+#ifndef PARAGRAPH_H
+#define PARAGRAPH_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PG_ABI_VERSION 1
+
+enum {
+    PG_OK = 0,
+    PG_EINVAL = 1,   /* bad argument or struct size */
+    PG_ENOMEM = 2,   /* device or host allocation failed */
+    PG_EHIP = 3,     /* HIP runtime error */
+    PG_ECAP = 4,     /* a capacity fixed at pg_ctx_create would be exceeded */
+    PG_ESTATE = 5,   /* call out of order (e.g. no graph loaded) */
+    PG_ENODEV = 6    /* no usable gfx950 device */
+};
+
+enum { PG_MODEL_PA = 0, PG_MODEL_ER = 1 };
+
+typedef struct pg_ctx pg_ctx;
+
+typedef struct {
+    uint32_t size;
+    int32_t device;            /* HIP device ordinal */
+    int64_t max_papers;        /* final graph size the context can hold */
+    int64_t max_edges;         /* citation edges the context can hold */
+    int64_t max_new_per_year;  /* papers one year may add */
+    int32_t max_cites;         /* citations per paper, upper bound (<= 64) */
+    int32_t bfs_cap;           /* 2-hop frontier/output cap (<= 4096) */
+    uint64_t seed;             /* Philox key of this simulation */
+} pg_config;
+
+typedef struct {
+    uint32_t size;
+    int32_t model;             /* PG_MODEL_PA or PG_MODEL_ER */
+    int32_t year;              /* calendar year given to the new papers */
+    int32_t year_index;        /* 0-based growth year (Philox counter word) */
+    int64_t n_new;             /* papers added this year */
+    int32_t m;                 /* PA: citations per new paper */
+    int32_t n_nb;              /* PA: of which neighbourhood draws (<= m-1) */
+    float alpha;               /* PA: in-degree exponent */
+    float inv_tau;             /* PA: 1/tau in 1/years (0 disables ageing) */
+    double er_p;               /* ER: link probability */
+} pg_year_params;
+
+typedef struct {
+    uint32_t size;
+    int64_t n_before;          /* papers at the start of the year */
+    int64_t n_after;
+    int64_t edges_added;
+    int64_t near_boundary;     /* PA attempts within 1e-12*Z of a CDF boundary */
+    int64_t shortfall;         /* citations not placed (attempt budget spent) */
+    int64_t pa_attempts;       /* PA draw attempts consumed (phase 1 + refill) */
+    double device_ms;          /* GPU time of the step (hipEvents) */
+} pg_year_stats;
+
+/* Kernel groups reported by pg_profile_read. */
+enum {
+    PG_K_WEIGHTS_CDF = 0,      /* fused weight pass + FP64 CDF */
+    PG_K_PA_DRAW = 1,          /* PA draws without duplicates */
+    PG_K_BFS_NB = 2,           /* 2-hop BFS + neighbourhood draws + refill */
+    PG_K_ER = 3,               /* ER count + write */
+    PG_K_CSR_APPEND = 4,       /* forward append, radix group, backward merge */
+    PG_K_COUNT = 5
+};
+
+/* Returns PG_ABI_VERSION. */
+int pg_version(void);
+
+/* Creates a context on cfg->device and allocates its device arena. */
+int pg_ctx_create(const pg_config* cfg, pg_ctx** out);
+void pg_ctx_destroy(pg_ctx* ctx);
+/* Message of the last failure on ctx (or of the last pg_ctx_create failure
+ * when ctx is NULL).  Valid until the next call on ctx. */
+const char* pg_last_error(const pg_ctx* ctx);
+
+/*
+ * Loads the seed graph: n papers with birth years, log-fitness for
+ * n_logfit >= n papers (entries beyond n are used by papers added later),
+ * and n_edges citations src[i] -> dst[i] (citing -> cited, ids in [0,n)).
+ * Duplicate edges are dropped and self-citations rejected, as the
+ * reference's set-based adjacency does (src/graph.cu:820-826).
+ * Replaces Graph::ParseEdgelist's AddEdge loop plus prepareGraph's CSR build
+ * (src/graph.cu:376-447, src/utils.cu:75-142).
+ */
+int pg_graph_load(pg_ctx* ctx, int64_t n, const int32_t* birth,
+                  const float* logfit, int64_t n_logfit, int64_t n_edges,
+                  const int32_t* src, const int32_t* dst);
+
+/*
+ * Runs one simulated year on the device: PA (weights, FP64 CDF, draws,
+ * 2-hop neighbourhood draws, refill) or ER G(n,p), then the CSR append and
+ * degree update.  Replaces buildOneNodeConnections_timed
+ * (src/kernel.cu:1125-1586) plus the year-boundary edge insertion and
+ * degree update (src/kernel.cu:1766-1815).  stats may be NULL.
+ */
+int pg_year_step(pg_ctx* ctx, const pg_year_params* params, pg_year_stats* stats);
+
+/* Current paper and edge counts. */
+int pg_graph_size(const pg_ctx* ctx, int64_t* n_papers, int64_t* n_edges);
+
+/*
+ * Copies the graph to caller buffers (any pointer may be NULL):
+ * fwd_off/bwd_off [n+1], fwd_adj/bwd_adj [e], in_deg/birth [n],
+ * logfit [n].  Rows are sorted ascending.
+ */
+int pg_graph_download(pg_ctx* ctx, int32_t* fwd_off, int32_t* fwd_adj,
+                      int32_t* bwd_off, int32_t* bwd_adj, int32_t* in_deg,
+                      int32_t* birth, float* logfit);
+
+/*
+ * Copies the last PA year's per-paper draws: cit/kind [n_new*m] in draw
+ * order (kind 0 PA, 1 neighbourhood, 2 PA refill; unused slots -1),
+ * cnt/near [n_new].  Replaces the per-paper D2H of append_device_to_host
+ * (includes/utils.cuh:151-200).  Any pointer may be NULL.
+ */
+int pg_year_citations(pg_ctx* ctx, int32_t* cit, int8_t* kind, int32_t* cnt,
+                      int32_t* near_boundary);
+
+/*
+ * Copies the last year's FP32 weights [n_before] and FP64 CDF [n_before]
+ * (PA years only).  Any pointer may be NULL.
+ */
+int pg_year_weights(pg_ctx* ctx, float* w, double* cdf);
+
+/*
+ * 2-hop neighbourhoods of `count` roots on the current graph, with the
+ * context's cap: out[i*bfs_cap ...] gets the ascending ids, out_len[i] the
+ * count.  Exposes the Stage-1 BFS (src/kernel.cu:17-168,
+ * src/abm.cu:577-883) on its own for testing and tooling.
+ */
+int pg_bfs(pg_ctx* ctx, int64_t count, const int32_t* roots, int32_t* out,
+           int32_t* out_len);
+
+/*
+ * Device-side snapshot of the graph state (CSR, degrees, birth years,
+ * log-fitness): pg_snapshot_save copies the current state into a second
+ * device buffer, pg_snapshot_restore copies it back.  Used to rerun a
+ * simulation from the same seed graph without another host upload.
+ */
+int pg_snapshot_save(pg_ctx* ctx);
+int pg_snapshot_restore(pg_ctx* ctx);
+
+/*
+ * Per-kernel-group timing with hipEvents recorded on the context's stream
+ * around each group (off by default).  pg_profile_read returns, for each
+ * group g < n: the summed device milliseconds, the summed algorithmic HBM
+ * bytes and the number of launches; pg_profile_reset clears.  Algorithmic
+ * bytes count compulsory traffic only:
+ *   WEIGHTS_CDF  20 B per existing paper (12 B read, 8 B FP64 CDF written);
+ *   PA_DRAW      8*ceil(log2 N) B per attempt + 4 B per emitted target;
+ *   BFS_NB       per query, 16 B + 4 B*deg(v) for the root and each capped
+ *                1-hop paper, + 4 B per output id;
+ *   ER           4 B per emitted target + 4 B per new paper;
+ *   CSR_APPEND   4*(dE+dN) forward write + 4*(E_old+N_old) read +
+ *                4*(E_new+N_new) write + 8 B per new edge (degree RMW).
+ */
+int pg_profile_enable(pg_ctx* ctx, int on);
+int pg_profile_read(pg_ctx* ctx, int n, double* ms, double* bytes, int64_t* launches);
+int pg_profile_reset(pg_ctx* ctx);
+
+/* Ceil growth rule of the reference (src/abm.cu:233-240,
+ * src/kernel.cu:1690): ceil(n * rate) in double precision. */
+int64_t pg_growth_count(int64_t n, double rate);
+
+/*
+ * Synthetic seed graph (host, OpenMP, Philox-keyed so every run matches):
+ * n papers with birth years uniform in [year_lo, year_hi], ids sorted by
+ * birth; each cites min(Poisson(mean_cites), #older) distinct older papers
+ * chosen uniformly.  birth[n]; src/dst sized by *n_edges on return: call once
+ * with src == NULL to get the count.  Log-fitness is drawn separately by
+ * pg_seed_logfit.
+ */
+int pg_seed_graph(uint64_t seed, int64_t n, int32_t year_lo, int32_t year_hi,
+                  double mean_cites, int32_t* birth, int64_t* n_edges,
+                  int32_t* src, int32_t* dst);
+
+/* log-fitness phi_i ~ Normal(mu, sigma^2) for papers [first, first+count). */
+int pg_seed_logfit(uint64_t seed, int64_t first, int64_t count, double mu,
+                   double sigma, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,129 @@
+/*
+ * oracle.h -- CPU restatement of ParAGraph's yearly graph-growth step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation in paragraph_amd/: only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it, and only to check or to time the
+ * CPU restatement.  The product path never links or calls it.
+ *
+ * It restates, in plain C with OpenMP over the papers of a year, the model
+ * that the reference's year loop runs (src/kernel.cu:1682-1845) with the
+ * BASELINE formulas (FP32 weights (k+1)^a * exp(-age/tau) * fitness, FP64
+ * CDF, binary-search draws without duplicates, 2-hop neighbourhood draws,
+ * ER G(n,p) by geometric skips, CSR append with degree update).  Every
+ * random number is a Philox4x32-10 output keyed by the simulation seed and
+ * indexed by (paper, attempt, year index, tag), so the GPU path consumes the
+ * identical stream.
+ *
+ * Parity pinning (what checks each function against something other than
+ * itself) is listed in the header comment of oracle.c.
+ */
+#ifndef PG_ORACLE_H
+#define PG_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Philox4x32-10 (Salmon et al., SC'11). */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* 53-bit uniform in [0,1) from two words: ((x0<<21) ^ (x1>>11)) * 2^-53. */
+double or_u53(uint32_t x0, uint32_t x1);
+/* 53-bit uniform in (0,1]. */
+double or_u53_open0(uint32_t x0, uint32_t x1);
+
+/* Deterministic elementary functions (same operation sequence on the GPU). */
+float or_expf(float x);
+float or_logf(float x);
+double or_log(double x);
+double or_log1p(double x);
+
+/* Compressed sparse rows of the citation graph: fwd = papers each paper
+ * cites, bwd = papers citing each paper; every row sorted ascending. */
+typedef struct {
+    int64_t n;
+    const int32_t* fwd_off;
+    const int32_t* fwd_adj;
+    const int32_t* bwd_off;
+    const int32_t* bwd_adj;
+} or_csr;
+
+/* w_i = exp(fma(alpha, log(k_i+1), fma(-(year-birth_i), inv_tau, logfit_i))) */
+void or_weights(int64_t n, const int32_t* in_deg, const int32_t* birth,
+                const float* logfit, int32_t year, float alpha, float inv_tau,
+                float* w_out);
+/* Sequential FP64 inclusive prefix sum of the FP32 weights. */
+void or_cdf(int64_t n, const float* w, double* cdf_out);
+/* First index k with cdf[k] > t, clamped to n-1. */
+int64_t or_upper_bound(const double* cdf, int64_t n, double t);
+
+/* 2-hop undirected neighbourhood of root (root excluded, ascending ids,
+ * 1-hop frontier and output both capped to the cap smallest ids).
+ * Writes at most cap ids to out; returns the count. */
+int32_t or_bfs(const or_csr* g, int32_t root, int32_t cap, int32_t* out);
+
+typedef struct {
+    uint64_t seed;
+    int32_t year_index;   /* 0-based growth year; Philox counter word 2 */
+    int32_t year;         /* calendar year given to the new papers */
+    int64_t n_exist;      /* papers at the start of the year (ids [0,n_exist)) */
+    int64_t n_new;        /* papers added this year (ids n_exist..) */
+    int32_t m;            /* citations per new paper */
+    int32_t n_nb;         /* of which drawn from the root's 2-hop neighbourhood */
+    int32_t cap;          /* BFS frontier / output cap */
+    float alpha;
+    float inv_tau;
+} or_pa_params;
+
+/* One PA year.  Outputs per new paper i: cit[i*m + j] (draw order),
+ * kind[i*m + j] (0 PA, 1 neighbourhood, 2 PA refill), cnt[i],
+ * near[i] = PA attempts whose uniform fell within 1e-12*Z of a CDF boundary.
+ * Returns 0 on success. */
+int or_pa_year(const or_csr* g, const int32_t* in_deg, const int32_t* birth,
+               const float* logfit, const or_pa_params* p, int32_t* cit,
+               int8_t* kind, int32_t* cnt, int32_t* near, int nthreads);
+
+typedef struct {
+    uint64_t seed;
+    int32_t year_index;
+    int64_t n_exist;
+    int64_t n_new;
+    double p;
+} or_er_params;
+
+/* One ER G(n,p) year.  off_out[n_new+1], tgt_out[cap_out].
+ * Returns the edge count, or -1 if cap_out is too small. */
+int64_t or_er_year(const or_er_params* p, int32_t* off_out, int32_t* tgt_out,
+                   int64_t cap_out, int nthreads);
+
+/* CSR append at a year boundary.  New paper i (id n_old+i) cites
+ * new_tgt[new_off[i] .. new_off[i+1]) (distinct, any order).  Output arrays
+ * are sized for n_old+n_new rows and E_old+E_added entries; in_deg_out gets
+ * the new backward row lengths. */
+int or_csr_append(const or_csr* g, int64_t n_new, const int32_t* new_off,
+                  const int32_t* new_tgt, int32_t* fwd_off_out,
+                  int32_t* fwd_adj_out, int32_t* bwd_off_out,
+                  int32_t* bwd_adj_out, int32_t* in_deg_out);
+
+/* FP64 log-likelihood of a PA citation sequence and its gradient.
+ * Years y = 0..n_years-1: papers [0, n_exist[y]) exist with in-degree
+ * snapshot deg + deg_off[y] (length n_exist[y]); new papers
+ * [n_exist[y], n_exist[y]+n_new[y]) carry m slots each in
+ * cit/kind + ev_off[y] and counts cnt + cnt_off[y].  Kinds 0 and 2 are
+ * likelihood events; kind 1 draws only shrink the partition.
+ * grad_phi has one entry per paper (sized n_exist[last]+n_new[last]). */
+int or_pa_loglik_grad(int32_t n_years, const int64_t* n_exist,
+                      const int64_t* n_new, const int32_t* year_of,
+                      const int32_t* deg, const int64_t* deg_off,
+                      const int32_t* cit, const int8_t* kind,
+                      const int64_t* ev_off, const int32_t* cnt,
+                      const int64_t* cnt_off, int32_t m,
+                      const int32_t* birth, const float* logfit, double alpha,
+                      double tau, double* loglik, double* grad_alpha,
+                      double* grad_tau, double* grad_phi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,291 @@
+"""ParAGraph graph-growth step on MI355X: Python binding of libparagraph.so.
+
+The product is the C ABI in ``include/paragraph.h`` (HIP kernels for gfx950
+behind it).  This module is a thin ctypes mirror of that ABI used by the
+tests, the benchmark and the likelihood-fitting harness; it holds no compute
+of its own and raises if the native library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libparagraph.so")
+ABM_PATH = os.path.join(_HERE, "abm")
+
+PG_MODEL_PA = 0
+PG_MODEL_ER = 1
+
+_ERRORS = {1: "EINVAL", 2: "ENOMEM", 3: "EHIP", 4: "ECAP", 5: "ESTATE", 6: "ENODEV"}
+
+
+class PGError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("device", ctypes.c_int32),
+                ("max_papers", ctypes.c_int64), ("max_edges", ctypes.c_int64),
+                ("max_new_per_year", ctypes.c_int64), ("max_cites", ctypes.c_int32),
+                ("bfs_cap", ctypes.c_int32), ("seed", ctypes.c_uint64)]
+
+
+class YearParams(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("model", ctypes.c_int32),
+                ("year", ctypes.c_int32), ("year_index", ctypes.c_int32),
+                ("n_new", ctypes.c_int64), ("m", ctypes.c_int32), ("n_nb", ctypes.c_int32),
+                ("alpha", ctypes.c_float), ("inv_tau", ctypes.c_float),
+                ("er_p", ctypes.c_double)]
+
+
+class YearStats(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("n_before", ctypes.c_int64),
+                ("n_after", ctypes.c_int64), ("edges_added", ctypes.c_int64),
+                ("near_boundary", ctypes.c_int64), ("shortfall", ctypes.c_int64),
+                ("pa_attempts", ctypes.c_int64), ("device_ms", ctypes.c_double)]
+
+
+KERNEL_GROUPS = ["weights_cdf", "pa_draw", "bfs_nb", "er", "csr_append"]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libparagraph.so (built in-tree); raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        i32p, i64p = P(ctypes.c_int32), P(ctypes.c_int64)
+        f32p, f64p, i8p = P(ctypes.c_float), P(ctypes.c_double), P(ctypes.c_int8)
+        vp = ctypes.c_void_p
+        sig = {
+            "pg_version": (ctypes.c_int, []),
+            "pg_ctx_create": (ctypes.c_int, [P(Config), P(vp)]),
+            "pg_ctx_destroy": (None, [vp]),
+            "pg_last_error": (ctypes.c_char_p, [vp]),
+            "pg_graph_load": (ctypes.c_int, [vp, ctypes.c_int64, i32p, f32p, ctypes.c_int64,
+                                             ctypes.c_int64, i32p, i32p]),
+            "pg_year_step": (ctypes.c_int, [vp, P(YearParams), P(YearStats)]),
+            "pg_graph_size": (ctypes.c_int, [vp, i64p, i64p]),
+            "pg_graph_download": (ctypes.c_int, [vp, i32p, i32p, i32p, i32p, i32p, i32p, f32p]),
+            "pg_year_citations": (ctypes.c_int, [vp, i32p, i8p, i32p, i32p]),
+            "pg_year_weights": (ctypes.c_int, [vp, f32p, f64p]),
+            "pg_bfs": (ctypes.c_int, [vp, ctypes.c_int64, i32p, i32p, i32p]),
+            "pg_growth_count": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_double]),
+            "pg_snapshot_save": (ctypes.c_int, [vp]),
+            "pg_snapshot_restore": (ctypes.c_int, [vp]),
+            "pg_profile_enable": (ctypes.c_int, [vp, ctypes.c_int]),
+            "pg_profile_read": (ctypes.c_int, [vp, ctypes.c_int, f64p, f64p, i64p]),
+            "pg_profile_reset": (ctypes.c_int, [vp]),
+            "pg_seed_graph": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int32,
+                                             ctypes.c_int32, ctypes.c_double, i32p, i64p, i32p,
+                                             i32p]),
+            "pg_seed_logfit": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64,
+                                              ctypes.c_double, ctypes.c_double, f32p]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray | None, ctype):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def growth_count(n: int, rate: float) -> int:
+    """ceil(n*rate) in double precision, as the reference grows the graph."""
+    return int(lib().pg_growth_count(int(n), float(rate)))
+
+
+def growth_schedule(n0: int, rate: float, years: int) -> list[int]:
+    out, n = [], n0
+    for _ in range(years):
+        d = growth_count(n, rate)
+        out.append(d)
+        n += d
+    return out
+
+
+@dataclass
+class SeedGraph:
+    birth: np.ndarray      # int32[n]
+    src: np.ndarray        # int32[e] citing
+    dst: np.ndarray        # int32[e] cited
+    logfit: np.ndarray     # float32[n_final]
+
+    @property
+    def n(self) -> int:
+        return int(self.birth.shape[0])
+
+
+def seed_graph(seed: int, n: int, mean_cites: float, n_final: int | None = None,
+               year_lo: int = -20, year_hi: int = 0, sigma: float = 0.5) -> SeedGraph:
+    """Synthetic seed graph plus log-fitness for n_final papers (host, Philox)."""
+    L = lib()
+    birth = np.empty(n, np.int32)
+    ne = ctypes.c_int64(0)
+    rc = L.pg_seed_graph(seed, n, year_lo, year_hi, mean_cites, _p(birth, ctypes.c_int32),
+                         ctypes.byref(ne), None, None)
+    if rc:
+        raise PGError(rc, "pg_seed_graph")
+    src = np.empty(ne.value, np.int32)
+    dst = np.empty(ne.value, np.int32)
+    rc = L.pg_seed_graph(seed, n, year_lo, year_hi, mean_cites, _p(birth, ctypes.c_int32),
+                         ctypes.byref(ne), _p(src, ctypes.c_int32), _p(dst, ctypes.c_int32))
+    if rc:
+        raise PGError(rc, "pg_seed_graph")
+    nf = n if n_final is None else n_final
+    logfit = np.empty(nf, np.float32)
+    rc = L.pg_seed_logfit(seed, 0, nf, 0.0, sigma, _p(logfit, ctypes.c_float))
+    if rc:
+        raise PGError(rc, "pg_seed_logfit")
+    return SeedGraph(birth, src, dst, logfit)
+
+
+class Context:
+    """One simulation on one GPU (wraps pg_ctx)."""
+
+    def __init__(self, max_papers: int, max_edges: int, max_new_per_year: int,
+                 max_cites: int = 32, bfs_cap: int = 4096, seed: int = 1, device: int = 0):
+        L = lib()
+        cfg = Config(ctypes.sizeof(Config), device, max_papers, max_edges, max_new_per_year,
+                     max_cites, bfs_cap, seed)
+        h = ctypes.c_void_p()
+        rc = L.pg_ctx_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc:
+            raise PGError(rc, L.pg_last_error(None).decode())
+        self._h = h
+        self.cfg = cfg
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pg_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc:
+            raise PGError(rc, f"{what}: {lib().pg_last_error(self._h).decode()}")
+
+    def load(self, g: SeedGraph):
+        birth = np.ascontiguousarray(g.birth, np.int32)
+        logfit = np.ascontiguousarray(g.logfit, np.float32)
+        src = np.ascontiguousarray(g.src, np.int32)
+        dst = np.ascontiguousarray(g.dst, np.int32)
+        self._check(lib().pg_graph_load(self._h, birth.shape[0], _p(birth, ctypes.c_int32),
+                                        _p(logfit, ctypes.c_float), logfit.shape[0],
+                                        src.shape[0], _p(src, ctypes.c_int32),
+                                        _p(dst, ctypes.c_int32)), "pg_graph_load")
+
+    def year_pa(self, year: int, year_index: int, n_new: int, m: int, n_nb: int,
+                alpha: float, tau: float) -> YearStats:
+        inv_tau = 0.0 if math.isinf(tau) else 1.0 / tau
+        p = YearParams(ctypes.sizeof(YearParams), PG_MODEL_PA, year, year_index, n_new, m,
+                       n_nb, alpha, inv_tau, 0.0)
+        st = YearStats(ctypes.sizeof(YearStats))
+        self._check(lib().pg_year_step(self._h, ctypes.byref(p), ctypes.byref(st)),
+                    "pg_year_step")
+        return st
+
+    def year_er(self, year: int, year_index: int, n_new: int, p_link: float) -> YearStats:
+        p = YearParams(ctypes.sizeof(YearParams), PG_MODEL_ER, year, year_index, n_new, 0, 0,
+                       0.0, 0.0, p_link)
+        st = YearStats(ctypes.sizeof(YearStats))
+        self._check(lib().pg_year_step(self._h, ctypes.byref(p), ctypes.byref(st)),
+                    "pg_year_step")
+        return st
+
+    def size(self) -> tuple[int, int]:
+        n, e = ctypes.c_int64(), ctypes.c_int64()
+        self._check(lib().pg_graph_size(self._h, ctypes.byref(n), ctypes.byref(e)),
+                    "pg_graph_size")
+        return n.value, e.value
+
+    def download(self) -> dict:
+        n, e = self.size()
+        out = {k: np.empty(s, np.int32) for k, s in
+               [("fwd_off", n + 1), ("fwd_adj", e), ("bwd_off", n + 1), ("bwd_adj", e),
+                ("in_deg", n), ("birth", n)]}
+        out["logfit"] = np.empty(n, np.float32)
+        i32 = ctypes.c_int32
+        self._check(lib().pg_graph_download(
+            self._h, _p(out["fwd_off"], i32), _p(out["fwd_adj"], i32), _p(out["bwd_off"], i32),
+            _p(out["bwd_adj"], i32), _p(out["in_deg"], i32), _p(out["birth"], i32),
+            _p(out["logfit"], ctypes.c_float)), "pg_graph_download")
+        return out
+
+    def citations(self, n_new: int, m: int) -> dict:
+        cit = np.empty(n_new * m, np.int32)
+        kind = np.empty(n_new * m, np.int8)
+        cnt = np.empty(n_new, np.int32)
+        near = np.empty(n_new, np.int32)
+        self._check(lib().pg_year_citations(self._h, _p(cit, ctypes.c_int32),
+                                            _p(kind, ctypes.c_int8), _p(cnt, ctypes.c_int32),
+                                            _p(near, ctypes.c_int32)), "pg_year_citations")
+        return {"cit": cit.reshape(n_new, m), "kind": kind.reshape(n_new, m), "cnt": cnt,
+                "near": near}
+
+    def weights(self, n: int) -> tuple[np.ndarray, np.ndarray]:
+        w = np.empty(n, np.float32)
+        c = np.empty(n, np.float64)
+        self._check(lib().pg_year_weights(self._h, _p(w, ctypes.c_float),
+                                          _p(c, ctypes.c_double)), "pg_year_weights")
+        return w, c
+
+    def bfs(self, roots: np.ndarray) -> list[np.ndarray]:
+        roots = np.ascontiguousarray(roots, np.int32)
+        cap = self.cfg.bfs_cap
+        out = np.empty(roots.shape[0] * cap, np.int32)
+        ln = np.empty(roots.shape[0], np.int32)
+        self._check(lib().pg_bfs(self._h, roots.shape[0], _p(roots, ctypes.c_int32),
+                                 _p(out, ctypes.c_int32), _p(ln, ctypes.c_int32)), "pg_bfs")
+        out = out.reshape(-1, cap)
+        return [out[i, :ln[i]].copy() for i in range(roots.shape[0])]
+
+    def snapshot_save(self):
+        self._check(lib().pg_snapshot_save(self._h), "pg_snapshot_save")
+
+    def snapshot_restore(self):
+        self._check(lib().pg_snapshot_restore(self._h), "pg_snapshot_restore")
+
+    def profile(self, on: bool = True):
+        self._check(lib().pg_profile_enable(self._h, 1 if on else 0), "pg_profile_enable")
+
+    def profile_reset(self):
+        self._check(lib().pg_profile_reset(self._h), "pg_profile_reset")
+
+    def profile_read(self) -> dict:
+        k = len(KERNEL_GROUPS)
+        ms = np.zeros(k, np.float64)
+        by = np.zeros(k, np.float64)
+        n = np.zeros(k, np.int64)
+        self._check(lib().pg_profile_read(self._h, k, _p(ms, ctypes.c_double),
+                                          _p(by, ctypes.c_double), _p(n, ctypes.c_int64)),
+                    "pg_profile_read")
+        return {name: {"ms": float(ms[i]), "bytes": float(by[i]), "launches": int(n[i])}
+                for i, name in enumerate(KERNEL_GROUPS)}

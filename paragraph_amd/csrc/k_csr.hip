@@ -1,0 +1,213 @@
+// k_csr.hip -- device CSR append and degree update at a year boundary.
+//
+// Replaces the reference's host-side year boundary: sorting the new edges
+// and inserting them into std::map<int,std::set<int>> forward/backward
+// adjacency (src/kernel.cu:1766-1811, Graph::AddEdge src/graph.cu:820-826),
+// the in/out-degree update (src/graph.cu:537-618), and the next year's CSR
+// rebuild from scratch (prepareGraph, src/utils.cu:75-142).
+//
+// Forward rows of the new papers are appended in place (sorted).  Backward
+// rows are rebuilt into the other buffer: each old row is followed by the
+// row's new citers.  The new citers are grouped by target with a stable LSD
+// radix sort of the year's (target, citing) pairs; since the pairs arrive in
+// citing order, every group is ascending and every backward row stays sorted.
+#include "pg_internal.h"
+
+namespace pg {
+namespace {
+
+__global__ __launch_bounds__(256) void k_pa_compact(int64_t n_exist, int64_t n_new, int m,
+                                                    const int32_t* __restrict__ cit,
+                                                    const int32_t* __restrict__ cnt,
+                                                    const int64_t* __restrict__ new_off,
+                                                    int64_t fwd_base, int32_t* __restrict__ fwd_off,
+                                                    int32_t* __restrict__ fwd_adj,
+                                                    int32_t* __restrict__ src_of,
+                                                    int32_t* __restrict__ add) {
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n_new) return;  // whole wavefront
+    const int c = cnt[i];
+    const int32_t x = lane < c ? cit[i * m + lane] : 0x7fffffff;
+    int rank = 0;
+    for (int j = 0; j < c; ++j) {
+        int32_t y = __shfl(x, j, 64);
+        rank += (y < x) ? 1 : 0;
+    }
+    if (lane < c) {
+        const int64_t e = new_off[i] + rank;
+        fwd_adj[fwd_base + e] = x;
+        src_of[e] = (int32_t)(n_exist + i);
+        atomicAdd(add + x, 1);
+    }
+    if (lane == 0) fwd_off[n_exist + i + 1] = (int32_t)(fwd_base + new_off[i + 1]);
+}
+
+constexpr int kRT = 256;
+constexpr int kRounds = kTile / kRT;  // 16
+
+__global__ __launch_bounds__(kRT) void k_radix_hist(const uint32_t* __restrict__ keys, int64_t n,
+                                                    int shift, int32_t* __restrict__ hist,
+                                                    int64_t ntiles) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+    for (int r = 0; r < kRounds; ++r) {
+        int64_t idx = base + r * kRT + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = (int32_t)h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kRT) void k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                       const uint32_t* __restrict__ vin,
+                                                       uint32_t* __restrict__ kout,
+                                                       uint32_t* __restrict__ vout, int64_t n,
+                                                       int shift, const int64_t* __restrict__ offs,
+                                                       int64_t ntiles) {
+    __shared__ int64_t s_base[256];
+    __shared__ uint32_t s_cnt[kRT / 64][256];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    s_base[threadIdx.x] = offs[(int64_t)threadIdx.x * ntiles + blockIdx.x];
+    for (int w = 0; w < kRT / 64; ++w) s_cnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+    for (int r = 0; r < kRounds; ++r) {
+        const int64_t idx = base + r * kRT + threadIdx.x;
+        const bool valid = idx < n;
+        const uint32_t k = valid ? kin[idx] : 0u;
+        const uint32_t v = valid ? vin[idx] : 0u;
+        const uint32_t d = (k >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t mb = __ballot(bit);
+            peers &= bit ? mb : ~mb;
+        }
+        const int rank = __popcll(peers & below);
+        if (valid && rank == 0) s_cnt[wid][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            int64_t pos = s_base[d] + rank;
+            for (int w = 0; w < wid; ++w) pos += s_cnt[w][d];
+            kout[pos] = k;
+            vout[pos] = v;
+        }
+        __syncthreads();
+        uint32_t t = 0;
+        for (int w = 0; w < kRT / 64; ++w) { t += s_cnt[w][threadIdx.x]; s_cnt[w][threadIdx.x] = 0; }
+        s_base[threadIdx.x] += t;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bwd_offsets(int64_t n_old, int64_t n_total,
+                                                     const int32_t* __restrict__ old_off,
+                                                     const int64_t* __restrict__ excl,
+                                                     const int32_t* __restrict__ add,
+                                                     int32_t* __restrict__ new_off,
+                                                     int32_t* __restrict__ in_deg) {
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v > n_total) return;
+    const int64_t vv = v < n_old ? v : n_old;
+    new_off[v] = (int32_t)(old_off[vv] + excl[vv]);
+    if (v < n_old) in_deg[v] += add[v];
+    else if (v < n_total) in_deg[v] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_bwd_merge(int64_t n_old, int64_t n_total,
+                                                   const int32_t* __restrict__ old_off,
+                                                   const int32_t* __restrict__ old_adj,
+                                                   const int32_t* __restrict__ new_off,
+                                                   const uint32_t* __restrict__ sorted_src,
+                                                   int32_t* __restrict__ out, int64_t e_new) {
+    const int64_t p0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 16;
+    if (p0 >= e_new) return;
+    int64_t lo = 0, hi = n_total + 1;  // first index with new_off > p0
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (new_off[mid] > p0) hi = mid; else lo = mid + 1;
+    }
+    int64_t v = lo - 1;
+    int64_t end = p0 + 16 < e_new ? p0 + 16 : e_new;
+    int32_t row_beg = new_off[v], row_end = new_off[v + 1];
+    for (int64_t p = p0; p < end; ++p) {
+        while (row_end <= p) { ++v; row_beg = row_end; row_end = new_off[v + 1]; }
+        const int64_t k = p - row_beg;
+        const int64_t ob = old_off[v];
+        const int64_t olen = old_off[v + 1] - ob;  // v < n_old for any non-empty row
+        out[p] = k < olen ? old_adj[ob + k] : (int32_t)sorted_src[(row_beg - ob) + (k - olen)];
+    }
+}
+
+}  // namespace
+
+void launch_pa_compact(hipStream_t s, int64_t n_exist, int64_t n_new, int m, const int32_t* cit,
+                       const int32_t* cnt, const int64_t* new_off, int64_t fwd_base,
+                       int32_t* fwd_off, int32_t* fwd_adj, int32_t* src_of, int32_t* add) {
+    if (n_new <= 0) return;
+    int64_t nb = (n_new + 3) / 4;
+    hipLaunchKernelGGL(k_pa_compact, dim3((uint32_t)nb), dim3(256), 0, s, n_exist, n_new, m, cit,
+                       cnt, new_off, fwd_base, fwd_off, fwd_adj, src_of, add);
+}
+
+int64_t radix_scratch_elems(int64_t n) {
+    int64_t nt = (n + kTile - 1) / kTile;
+    if (nt == 0) nt = 1;
+    int64_t h = 256 * nt;
+    return (h + 1) + scan_scratch_elems(h) + (h / 2 + 1);
+}
+
+void launch_radix_sort_pairs(hipStream_t s, const uint32_t* k_in, const uint32_t* v_in,
+                             uint32_t* k_out, uint32_t* v_out, uint32_t* k_tmp, uint32_t* v_tmp,
+                             int64_t n, int key_bits, int64_t* scratch) {
+    if (n <= 0) return;
+    int64_t nt = (n + kTile - 1) / kTile;
+    int64_t h = 256 * nt;
+    int64_t* offs = scratch;
+    int64_t* scan_scr = scratch + (h + 1);
+    int32_t* hist = reinterpret_cast<int32_t*>(scan_scr + scan_scratch_elems(h));
+    int passes = (key_bits + 7) / 8;
+    if (passes < 1) passes = 1;
+    const uint32_t* ks = k_in;
+    const uint32_t* vs = v_in;
+    for (int p = 0; p < passes; ++p) {
+        const bool to_out = ((passes - 1 - p) % 2) == 0;
+        uint32_t* kd = to_out ? k_out : k_tmp;
+        uint32_t* vd = to_out ? v_out : v_tmp;
+        hipLaunchKernelGGL(k_radix_hist, dim3((uint32_t)nt), dim3(kRT), 0, s, ks, n, 8 * p, hist, nt);
+        launch_excl_scan_i32(s, hist, h, offs, scan_scr);
+        hipLaunchKernelGGL(k_radix_scatter, dim3((uint32_t)nt), dim3(kRT), 0, s, ks, vs, kd, vd, n,
+                           8 * p, offs, nt);
+        ks = kd;
+        vs = vd;
+    }
+}
+
+void launch_bwd_offsets(hipStream_t s, int64_t n_old, int64_t n_new, const int32_t* bwd_off_old,
+                        const int32_t* add, int32_t* bwd_off_new, int32_t* in_deg,
+                        int64_t* scratch) {
+    int64_t* excl = scratch;
+    int64_t* scan_scr = scratch + (n_old + 1);
+    launch_excl_scan_i32(s, add, n_old, excl, scan_scr);
+    const int64_t n_total = n_old + n_new;
+    int64_t nb = (n_total + 1 + 255) / 256;
+    hipLaunchKernelGGL(k_bwd_offsets, dim3((uint32_t)nb), dim3(256), 0, s, n_old, n_total,
+                       bwd_off_old, excl, add, bwd_off_new, in_deg);
+}
+
+void launch_bwd_merge(hipStream_t s, int64_t n_old, int64_t n_total, const int32_t* bwd_off_old,
+                      const int32_t* bwd_adj_old, const int32_t* bwd_off_new,
+                      const uint32_t* sorted_src, int32_t* bwd_adj_new, int64_t e_new) {
+    if (e_new <= 0) return;
+    int64_t threads = (e_new + 15) / 16;
+    int64_t nb = (threads + 255) / 256;
+    hipLaunchKernelGGL(k_bwd_merge, dim3((uint32_t)nb), dim3(256), 0, s, n_old, n_total, bwd_off_old,
+                       bwd_adj_old, bwd_off_new, sorted_src, bwd_adj_new, e_new);
+}
+
+}  // namespace pg

@@ -1,0 +1,649 @@
+// pg_api.hip -- host implementation of the C ABI declared in
+// include/paragraph.h: context and device arena, seed-graph load, the
+// per-year step (replacing buildOneNodeConnections_timed,
+// src/kernel.cu:1125-1586, and the year-boundary update,
+// src/kernel.cu:1766-1815), downloads, and the stand-alone BFS.
+//
+// HBM layout per context (int32 ids and offsets; N = papers, E = edges):
+//   birth, in_deg : int32[N_cap]        logfit : float[N_logfit]
+//   fwd_off       : int32[N_cap+1]      fwd_adj : int32[E_cap]   (appended)
+//   bwd_off x2    : int32[N_cap+1]      bwd_adj x2 : int32[E_cap] (ping-pong)
+//   w : float[N_cap], cdf : double[N_cap]       (this year's PA weights/CDF)
+//   cit/kind : [max_new * max_cites], cnt/near/anext : [max_new]
+//   add : int32[N_cap] (per-target new citations), src_of + radix buffers
+//   sized for one year's edges, and an int64 scratch for scans.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/paragraph.h"
+#include "pg_internal.h"
+
+using namespace pg;
+
+struct pg_ctx {
+    pg_config cfg{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    bool loaded = false;
+    int64_t n = 0, e = 0, n_logfit = 0;
+    int32_t* birth = nullptr;
+    int32_t* in_deg = nullptr;
+    float* logfit = nullptr;
+    int32_t* fwd_off = nullptr;
+    int32_t* fwd_adj = nullptr;
+    int32_t* bwd_off[2] = {nullptr, nullptr};
+    int32_t* bwd_adj[2] = {nullptr, nullptr};
+    int cur = 0;
+    float* w = nullptr;
+    double* cdf = nullptr;
+    double* scan_agg = nullptr;
+    double* scan_incl = nullptr;
+    uint32_t* scan_status = nullptr;
+    uint32_t* scan_ticket = nullptr;
+    uint32_t epoch = 0;
+    int32_t* cit = nullptr;
+    int8_t* kind = nullptr;
+    int32_t* cnt = nullptr;
+    int32_t* near = nullptr;
+    int32_t* anext = nullptr;
+    int64_t* new_off = nullptr;
+    int32_t* add = nullptr;
+    int64_t year_edge_cap = 0;
+    int32_t* src_of = nullptr;
+    uint32_t* k_sorted = nullptr;
+    uint32_t* v_sorted = nullptr;
+    uint32_t* k_tmp = nullptr;
+    uint32_t* v_tmp = nullptr;
+    int64_t* scratch = nullptr;
+    unsigned long long* d_sum = nullptr;
+    int32_t* d_tmp_i32 = nullptr;
+    int64_t d_tmp_cap = 0;
+    int64_t last_n_new = 0, last_n_exist = 0;
+    int last_m = 0, last_model = -1;
+    // per-kernel-group profiling
+    bool prof = false;
+    hipEvent_t pev[PG_K_COUNT][2] = {};
+    double prof_ms[PG_K_COUNT] = {};
+    double prof_bytes[PG_K_COUNT] = {};
+    int64_t prof_calls[PG_K_COUNT] = {};
+    // device snapshot of the graph state
+    bool snap = false;
+    int64_t snap_n = 0, snap_e = 0;
+    int32_t* s_birth = nullptr;
+    int32_t* s_in_deg = nullptr;
+    int32_t* s_fwd_off = nullptr;
+    int32_t* s_fwd_adj = nullptr;
+    int32_t* s_bwd_off = nullptr;
+    int32_t* s_bwd_adj = nullptr;
+};
+
+namespace {
+
+std::string g_create_err;
+
+#define PG_HIP(expr)                                                                    \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess) {                                                         \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(_e);               \
+            return PG_EHIP;                                                             \
+        }                                                                               \
+    } while (0)
+
+int fail(pg_ctx* ctx, int code, const std::string& msg) {
+    ctx->err = msg;
+    return code;
+}
+
+template <typename T>
+hipError_t dalloc(T** p, int64_t count) {
+    if (count <= 0) count = 1;
+    return hipMalloc(reinterpret_cast<void**>(p), (size_t)count * sizeof(T));
+}
+
+void release(pg_ctx* c) {
+    void* ptrs[] = {c->birth, c->in_deg, c->logfit, c->fwd_off, c->fwd_adj, c->bwd_off[0],
+                    c->bwd_off[1], c->bwd_adj[0], c->bwd_adj[1], c->w, c->cdf, c->scan_agg,
+                    c->scan_incl, c->scan_status, c->scan_ticket, c->cit, c->kind, c->cnt,
+                    c->near, c->anext, c->new_off, c->add, c->src_of, c->k_sorted,
+                    c->v_sorted, c->k_tmp, c->v_tmp, c->scratch, c->d_sum, c->d_tmp_i32,
+                    c->s_birth, c->s_in_deg, c->s_fwd_off, c->s_fwd_adj, c->s_bwd_off,
+                    c->s_bwd_adj};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (int k = 0; k < PG_K_COUNT; ++k)
+        for (int j = 0; j < 2; ++j)
+            if (c->pev[k][j]) (void)hipEventDestroy(c->pev[k][j]);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+int64_t scratch_need(int64_t max_papers, int64_t max_new, int64_t year_edges) {
+    int64_t a = (max_papers + 2) + scan_scratch_elems(max_papers + 1);
+    int64_t b = scan_scratch_elems(max_new + 1);
+    int64_t c = radix_scratch_elems(year_edges);
+    return std::max(a, std::max(b, c));
+}
+
+int ensure_year_edges(pg_ctx* ctx, int64_t need) {
+    if (need <= ctx->year_edge_cap) return PG_OK;
+    int64_t cap = std::max(need, ctx->year_edge_cap * 2);
+    (void)hipFree(ctx->src_of); (void)hipFree(ctx->k_sorted); (void)hipFree(ctx->v_sorted);
+    (void)hipFree(ctx->k_tmp); (void)hipFree(ctx->v_tmp); (void)hipFree(ctx->scratch);
+    ctx->src_of = nullptr; ctx->k_sorted = ctx->v_sorted = ctx->k_tmp = ctx->v_tmp = nullptr;
+    ctx->scratch = nullptr;
+    PG_HIP(dalloc(&ctx->src_of, cap));
+    PG_HIP(dalloc(&ctx->k_sorted, cap));
+    PG_HIP(dalloc(&ctx->v_sorted, cap));
+    PG_HIP(dalloc(&ctx->k_tmp, cap));
+    PG_HIP(dalloc(&ctx->v_tmp, cap));
+    PG_HIP(dalloc(&ctx->scratch,
+                  scratch_need(ctx->cfg.max_papers, ctx->cfg.max_new_per_year, cap)));
+    ctx->year_edge_cap = cap;
+    return PG_OK;
+}
+
+int key_bits_for(int64_t n) {
+    int b = 1;
+    while (b < 31 && (int64_t(1) << b) < n) ++b;
+    return b;
+}
+
+CsrView view(const pg_ctx* c) {
+    return CsrView{c->fwd_off, c->fwd_adj, c->bwd_off[c->cur], c->bwd_adj[c->cur], c->n};
+}
+
+}  // namespace
+
+extern "C" {
+
+int pg_version(void) { return PG_ABI_VERSION; }
+
+const char* pg_last_error(const pg_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
+
+int pg_ctx_create(const pg_config* cfg, pg_ctx** out) {
+    g_create_err.clear();
+    if (!cfg || !out || cfg->size != sizeof(pg_config)) {
+        g_create_err = "pg_ctx_create: null argument or pg_config.size mismatch";
+        return PG_EINVAL;
+    }
+    *out = nullptr;
+    if (cfg->max_papers < 1 || cfg->max_papers >= (int64_t(1) << 31) - 1 || cfg->max_edges < 0 ||
+        cfg->max_edges >= (int64_t(1) << 31) - 1 || cfg->max_new_per_year < 0 ||
+        cfg->max_cites < 0 || cfg->max_cites > kMaxCites || cfg->bfs_cap < 1 ||
+        cfg->bfs_cap > kBfsCapMax) {
+        g_create_err = "pg_ctx_create: capacity out of range (papers/edges < 2^31-1, "
+                       "max_cites <= 64, 1 <= bfs_cap <= 4096)";
+        return PG_EINVAL;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || cfg->device < 0 || cfg->device >= ndev) {
+        g_create_err = "pg_ctx_create: no HIP device with that ordinal";
+        return PG_ENODEV;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, cfg->device) != hipSuccess ||
+        std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        g_create_err = std::string("pg_ctx_create: device is not gfx950 (") + prop.gcnArchName + ")";
+        return PG_ENODEV;
+    }
+    pg_ctx* ctx = new (std::nothrow) pg_ctx();
+    if (!ctx) { g_create_err = "pg_ctx_create: out of host memory"; return PG_ENOMEM; }
+    ctx->cfg = *cfg;
+    auto bail = [&](int code) {
+        g_create_err = ctx->err;
+        release(ctx);
+        delete ctx;
+        return code;
+    };
+#define CREATE_HIP(expr)                                                       \
+    do {                                                                       \
+        hipError_t _e = (expr);                                                \
+        if (_e != hipSuccess) {                                                \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(_e);      \
+            return bail(_e == hipErrorOutOfMemory ? PG_ENOMEM : PG_EHIP);      \
+        }                                                                      \
+    } while (0)
+    CREATE_HIP(hipSetDevice(cfg->device));
+    CREATE_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    CREATE_HIP(hipEventCreate(&ctx->ev0));
+    CREATE_HIP(hipEventCreate(&ctx->ev1));
+    const int64_t N = cfg->max_papers, E = std::max<int64_t>(cfg->max_edges, 1);
+    const int64_t D = std::max<int64_t>(cfg->max_new_per_year, 1);
+    const int64_t M = std::max<int64_t>(cfg->max_cites, 1);
+    const int64_t tiles = (N + kTile - 1) / kTile + 1;
+    CREATE_HIP(dalloc(&ctx->birth, N));
+    CREATE_HIP(dalloc(&ctx->in_deg, N));
+    CREATE_HIP(dalloc(&ctx->fwd_off, N + 1));
+    CREATE_HIP(dalloc(&ctx->fwd_adj, E));
+    for (int b = 0; b < 2; ++b) {
+        CREATE_HIP(dalloc(&ctx->bwd_off[b], N + 1));
+        CREATE_HIP(dalloc(&ctx->bwd_adj[b], E));
+    }
+    CREATE_HIP(dalloc(&ctx->w, N));
+    CREATE_HIP(dalloc(&ctx->cdf, N));
+    CREATE_HIP(dalloc(&ctx->scan_agg, tiles));
+    CREATE_HIP(dalloc(&ctx->scan_incl, tiles));
+    CREATE_HIP(dalloc(&ctx->scan_status, tiles));
+    CREATE_HIP(dalloc(&ctx->scan_ticket, 1));
+    CREATE_HIP(hipMemset(ctx->scan_status, 0, (size_t)tiles * sizeof(uint32_t)));
+    CREATE_HIP(hipMemset(ctx->scan_ticket, 0, sizeof(uint32_t)));
+    CREATE_HIP(dalloc(&ctx->cit, D * M));
+    CREATE_HIP(dalloc(&ctx->kind, D * M));
+    CREATE_HIP(dalloc(&ctx->cnt, D));
+    CREATE_HIP(dalloc(&ctx->near, D));
+    CREATE_HIP(dalloc(&ctx->anext, D));
+    CREATE_HIP(dalloc(&ctx->new_off, D + 2));
+    CREATE_HIP(dalloc(&ctx->add, N));
+    CREATE_HIP(dalloc(&ctx->d_sum, 4));
+    for (int k = 0; k < PG_K_COUNT; ++k)
+        for (int j = 0; j < 2; ++j) CREATE_HIP(hipEventCreate(&ctx->pev[k][j]));
+    {
+        int rc = ensure_year_edges(ctx, D * M);
+        if (rc) return bail(rc);
+    }
+#undef CREATE_HIP
+    *out = ctx;
+    return PG_OK;
+}
+
+void pg_ctx_destroy(pg_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->cfg.device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    release(ctx);
+    delete ctx;
+}
+
+int pg_graph_load(pg_ctx* ctx, int64_t n, const int32_t* birth, const float* logfit,
+                  int64_t n_logfit, int64_t n_edges, const int32_t* src, const int32_t* dst) {
+    if (!ctx) return PG_EINVAL;
+    if (n < 1 || !birth || !logfit || n_logfit < n || n_edges < 0 ||
+        (n_edges > 0 && (!src || !dst)))
+        return fail(ctx, PG_EINVAL, "pg_graph_load: bad arguments");
+    if (n > ctx->cfg.max_papers) return fail(ctx, PG_ECAP, "pg_graph_load: n > max_papers");
+    if (n_logfit > ctx->cfg.max_papers) n_logfit = ctx->cfg.max_papers;
+    for (int64_t k = 0; k < n_edges; ++k) {
+        if (src[k] < 0 || src[k] >= n || dst[k] < 0 || dst[k] >= n)
+            return fail(ctx, PG_EINVAL, "pg_graph_load: edge endpoint out of range");
+        if (src[k] == dst[k]) return fail(ctx, PG_EINVAL, "pg_graph_load: self-citation");
+    }
+    // forward CSR by counting sort on src, rows sorted and deduplicated
+    std::vector<int64_t> off(n + 1, 0);
+    for (int64_t k = 0; k < n_edges; ++k) off[src[k] + 1]++;
+    for (int64_t v = 0; v < n; ++v) off[v + 1] += off[v];
+    std::vector<int32_t> adj(n_edges);
+    {
+        std::vector<int64_t> cur(off.begin(), off.end() - 1);
+        for (int64_t k = 0; k < n_edges; ++k) adj[cur[src[k]]++] = dst[k];
+    }
+    std::vector<int32_t> fo(n + 1, 0);
+    int64_t out_e = 0;
+    for (int64_t v = 0; v < n; ++v) {
+        int32_t* b = adj.data() + off[v];
+        int32_t* e = adj.data() + off[v + 1];
+        std::sort(b, e);
+        int32_t* u = std::unique(b, e);
+        for (int32_t* p = b; p < u; ++p) adj[out_e++] = *p;
+        fo[v + 1] = (int32_t)out_e;
+    }
+    adj.resize(out_e);
+    if (out_e > ctx->cfg.max_edges) return fail(ctx, PG_ECAP, "pg_graph_load: edges > max_edges");
+    // backward CSR: transpose, citers ascending
+    std::vector<int32_t> bo(n + 1, 0), badj(out_e), indeg(n, 0);
+    for (int64_t k = 0; k < out_e; ++k) indeg[adj[k]]++;
+    for (int64_t v = 0; v < n; ++v) bo[v + 1] = bo[v] + indeg[v];
+    {
+        std::vector<int32_t> cur(bo.begin(), bo.end() - 1);
+        for (int64_t v = 0; v < n; ++v)
+            for (int32_t k = fo[v]; k < fo[v + 1]; ++k) badj[cur[adj[k]]++] = (int32_t)v;
+    }
+    PG_HIP(hipSetDevice(ctx->cfg.device));
+    if (ctx->logfit) { (void)hipFree(ctx->logfit); ctx->logfit = nullptr; }
+    PG_HIP(dalloc(&ctx->logfit, ctx->cfg.max_papers));
+    hipStream_t s = ctx->stream;
+    PG_HIP(hipMemcpyAsync(ctx->birth, birth, n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    PG_HIP(hipMemcpyAsync(ctx->logfit, logfit, n_logfit * sizeof(float), hipMemcpyHostToDevice, s));
+    PG_HIP(hipMemcpyAsync(ctx->in_deg, indeg.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    PG_HIP(hipMemcpyAsync(ctx->fwd_off, fo.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    if (out_e)
+        PG_HIP(hipMemcpyAsync(ctx->fwd_adj, adj.data(), out_e * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    ctx->cur = 0;
+    PG_HIP(hipMemcpyAsync(ctx->bwd_off[0], bo.data(), (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    if (out_e)
+        PG_HIP(hipMemcpyAsync(ctx->bwd_adj[0], badj.data(), out_e * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    PG_HIP(hipStreamSynchronize(s));
+    ctx->n = n;
+    ctx->e = out_e;
+    ctx->n_logfit = n_logfit;
+    ctx->loaded = true;
+    ctx->last_model = -1;
+    ctx->err.clear();
+    return PG_OK;
+}
+
+int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
+    if (!ctx) return PG_EINVAL;
+    if (!p || p->size != sizeof(pg_year_params) ||
+        (stats && stats->size != sizeof(pg_year_stats)))
+        return fail(ctx, PG_EINVAL, "pg_year_step: null params or struct size mismatch");
+    if (!ctx->loaded) return fail(ctx, PG_ESTATE, "pg_year_step: no graph loaded");
+    const int64_t N = ctx->n, D = p->n_new;
+    if (D < 0) return fail(ctx, PG_EINVAL, "pg_year_step: n_new < 0");
+    if (D > ctx->cfg.max_new_per_year) return fail(ctx, PG_ECAP, "pg_year_step: n_new > max_new_per_year");
+    if (N + D > ctx->cfg.max_papers) return fail(ctx, PG_ECAP, "pg_year_step: graph would exceed max_papers");
+    if (N + D > ctx->n_logfit) return fail(ctx, PG_ECAP, "pg_year_step: log-fitness not supplied for the new papers");
+    if (p->model != PG_MODEL_PA && p->model != PG_MODEL_ER)
+        return fail(ctx, PG_EINVAL, "pg_year_step: unknown model");
+    if (p->model == PG_MODEL_PA) {
+        if (p->m < 0 || p->m > ctx->cfg.max_cites)
+            return fail(ctx, PG_ECAP, "pg_year_step: m exceeds max_cites");
+        if (p->n_nb < 0 || (p->m > 0 && p->n_nb > p->m - 1) || (p->m == 0 && p->n_nb != 0))
+            return fail(ctx, PG_EINVAL, "pg_year_step: n_nb must be in [0, m-1]");
+        if (!(p->alpha == p->alpha) || !(p->inv_tau >= 0.0f))
+            return fail(ctx, PG_EINVAL, "pg_year_step: alpha must be finite and inv_tau >= 0");
+    } else if (!(p->er_p >= 0.0 && p->er_p <= 1.0)) {
+        return fail(ctx, PG_EINVAL, "pg_year_step: er_p must be in [0,1]");
+    }
+    PG_HIP(hipSetDevice(ctx->cfg.device));
+    hipStream_t s = ctx->stream;
+    const uint64_t seed = ctx->cfg.seed;
+    const uint32_t yi = (uint32_t)p->year_index;
+    bool used[PG_K_COUNT] = {};
+    auto pbeg = [&](int k) { if (ctx->prof) (void)hipEventRecord(ctx->pev[k][0], s); };
+    auto pend = [&](int k) { if (ctx->prof) { (void)hipEventRecord(ctx->pev[k][1], s); used[k] = true; } };
+    PG_HIP(hipEventRecord(ctx->ev0, s));
+    int64_t e_year = 0, near_total = 0, attempts = 0;
+    double bfs_bytes = 0.0;
+    if (D > 0) {
+        launch_fill_i32(s, ctx->birth + N, p->year, D);
+        unsigned long long* sums = ctx->d_sum;  // [0] near, [1] attempts, [2] BFS bytes
+        PG_HIP(hipMemsetAsync(sums, 0, 4 * sizeof(unsigned long long), s));
+        if (p->model == PG_MODEL_PA) {
+            const int m = p->m, n_nb = p->n_nb;
+            ++ctx->epoch;
+            ScanState st{ctx->scan_agg, ctx->scan_incl, ctx->scan_status, ctx->scan_ticket, ctx->epoch};
+            pbeg(PG_K_WEIGHTS_CDF);
+            launch_weights_cdf(s, N, ctx->in_deg, ctx->birth, ctx->logfit, p->year, p->alpha,
+                               p->inv_tau, ctx->w, ctx->cdf, st);
+            pend(PG_K_WEIGHTS_CDF);
+            pbeg(PG_K_PA_DRAW);
+            launch_pa_draw(s, N, ctx->cdf, D, m, m - n_nb, seed, yi, ctx->cit, ctx->kind, ctx->cnt,
+                           ctx->near, ctx->anext);
+            pend(PG_K_PA_DRAW);
+            if (n_nb > 0) {
+                pbeg(PG_K_BFS_NB);
+                launch_bfs_nb(s, view(ctx), ctx->w, ctx->cdf, D, m, n_nb, ctx->cfg.bfs_cap, seed, yi,
+                              ctx->cit, ctx->kind, ctx->cnt, ctx->near, ctx->anext, sums + 2);
+                pend(PG_K_BFS_NB);
+            }
+            launch_sum_i32(s, ctx->near, D, sums + 0);
+            launch_sum_i32(s, ctx->anext, D, sums + 1);
+        } else {
+            pbeg(PG_K_ER);
+            launch_er_count(s, N, D, p->er_p, seed, yi, ctx->cnt);
+        }
+        launch_excl_scan_i32(s, ctx->cnt, D, ctx->new_off, ctx->scratch);
+        PG_HIP(hipGetLastError());
+        int64_t tot = 0;
+        unsigned long long hs[4] = {0, 0, 0, 0};
+        PG_HIP(hipMemcpyAsync(&tot, ctx->new_off + D, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        PG_HIP(hipMemcpyAsync(hs, sums, sizeof(hs), hipMemcpyDeviceToHost, s));
+        PG_HIP(hipStreamSynchronize(s));
+        e_year = tot;
+        near_total = (int64_t)hs[0];
+        attempts = (int64_t)hs[1];
+        bfs_bytes = (double)hs[2];
+        if (ctx->e + e_year > ctx->cfg.max_edges)
+            return fail(ctx, PG_ECAP, "pg_year_step: edges would exceed max_edges");
+        {
+            int rc = ensure_year_edges(ctx, e_year);
+            if (rc) return rc;
+        }
+        launch_fill_i32(s, ctx->add, 0, N);
+        if (p->model == PG_MODEL_PA) {
+            pbeg(PG_K_CSR_APPEND);
+            launch_pa_compact(s, N, D, p->m, ctx->cit, ctx->cnt, ctx->new_off, ctx->e, ctx->fwd_off,
+                              ctx->fwd_adj, ctx->src_of, ctx->add);
+        } else {
+            launch_er_write(s, N, D, p->er_p, seed, yi, ctx->new_off, ctx->e, ctx->fwd_off,
+                            ctx->fwd_adj, ctx->src_of, ctx->add);
+            pend(PG_K_ER);
+            pbeg(PG_K_CSR_APPEND);
+        }
+        launch_radix_sort_pairs(s, reinterpret_cast<const uint32_t*>(ctx->fwd_adj + ctx->e),
+                                reinterpret_cast<const uint32_t*>(ctx->src_of), ctx->k_sorted,
+                                ctx->v_sorted, ctx->k_tmp, ctx->v_tmp, e_year, key_bits_for(N),
+                                ctx->scratch);
+        const int nxt = 1 - ctx->cur;
+        launch_bwd_offsets(s, N, D, ctx->bwd_off[ctx->cur], ctx->add, ctx->bwd_off[nxt],
+                           ctx->in_deg, ctx->scratch);
+        launch_bwd_merge(s, N, N + D, ctx->bwd_off[ctx->cur], ctx->bwd_adj[ctx->cur],
+                         ctx->bwd_off[nxt], ctx->v_sorted, ctx->bwd_adj[nxt], ctx->e + e_year);
+        pend(PG_K_CSR_APPEND);
+        PG_HIP(hipGetLastError());
+        ctx->cur = nxt;
+    }
+    PG_HIP(hipEventRecord(ctx->ev1, s));
+    PG_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0.0f;
+    PG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (ctx->prof && D > 0) {
+        const double lg = std::ceil(std::log2((double)std::max<int64_t>(N, 2)));
+        const double e_old = (double)ctx->e;
+        double bytes[PG_K_COUNT] = {};
+        bytes[PG_K_WEIGHTS_CDF] = 20.0 * N;
+        bytes[PG_K_PA_DRAW] = 8.0 * lg * attempts + 4.0 * D * (p->m - p->n_nb);
+        bytes[PG_K_BFS_NB] = bfs_bytes;
+        bytes[PG_K_ER] = 4.0 * e_year + 4.0 * D;
+        bytes[PG_K_CSR_APPEND] = 4.0 * (e_year + D) + 4.0 * (e_old + N) +
+                                 4.0 * (e_old + e_year + N + D) + 8.0 * e_year;
+        for (int k = 0; k < PG_K_COUNT; ++k) {
+            if (!used[k]) continue;
+            float kms = 0.0f;
+            PG_HIP(hipEventElapsedTime(&kms, ctx->pev[k][0], ctx->pev[k][1]));
+            ctx->prof_ms[k] += kms;
+            ctx->prof_bytes[k] += bytes[k];
+            ctx->prof_calls[k] += 1;
+        }
+    }
+    if (stats) {
+        stats->n_before = N;
+        stats->n_after = N + D;
+        stats->edges_added = e_year;
+        stats->near_boundary = near_total;
+        stats->shortfall = (p->model == PG_MODEL_PA) ? D * (int64_t)p->m - e_year : 0;
+        stats->pa_attempts = attempts;
+        stats->device_ms = ms;
+    }
+    ctx->n = N + D;
+    ctx->e += e_year;
+    ctx->last_n_new = D;
+    ctx->last_n_exist = N;
+    ctx->last_m = p->model == PG_MODEL_PA ? p->m : 0;
+    ctx->last_model = p->model;
+    return PG_OK;
+}
+
+int pg_snapshot_save(pg_ctx* ctx) {
+    if (!ctx) return PG_EINVAL;
+    if (!ctx->loaded) return fail(ctx, PG_ESTATE, "pg_snapshot_save: no graph loaded");
+    PG_HIP(hipSetDevice(ctx->cfg.device));
+    if (!ctx->s_birth) {
+        const int64_t N = ctx->cfg.max_papers, E = std::max<int64_t>(ctx->cfg.max_edges, 1);
+        hipError_t r = hipSuccess;
+        if (r == hipSuccess) r = dalloc(&ctx->s_birth, N);
+        if (r == hipSuccess) r = dalloc(&ctx->s_in_deg, N);
+        if (r == hipSuccess) r = dalloc(&ctx->s_fwd_off, N + 1);
+        if (r == hipSuccess) r = dalloc(&ctx->s_fwd_adj, E);
+        if (r == hipSuccess) r = dalloc(&ctx->s_bwd_off, N + 1);
+        if (r == hipSuccess) r = dalloc(&ctx->s_bwd_adj, E);
+        if (r != hipSuccess) return fail(ctx, PG_ENOMEM, "pg_snapshot_save: device allocation failed");
+    }
+    hipStream_t s = ctx->stream;
+    const int64_t n = ctx->n, e = ctx->e;
+    PG_HIP(hipMemcpyAsync(ctx->s_birth, ctx->birth, n * 4, hipMemcpyDeviceToDevice, s));
+    PG_HIP(hipMemcpyAsync(ctx->s_in_deg, ctx->in_deg, n * 4, hipMemcpyDeviceToDevice, s));
+    PG_HIP(hipMemcpyAsync(ctx->s_fwd_off, ctx->fwd_off, (n + 1) * 4, hipMemcpyDeviceToDevice, s));
+    PG_HIP(hipMemcpyAsync(ctx->s_bwd_off, ctx->bwd_off[ctx->cur], (n + 1) * 4, hipMemcpyDeviceToDevice, s));
+    if (e) {
+        PG_HIP(hipMemcpyAsync(ctx->s_fwd_adj, ctx->fwd_adj, e * 4, hipMemcpyDeviceToDevice, s));
+        PG_HIP(hipMemcpyAsync(ctx->s_bwd_adj, ctx->bwd_adj[ctx->cur], e * 4, hipMemcpyDeviceToDevice, s));
+    }
+    PG_HIP(hipStreamSynchronize(s));
+    ctx->snap = true;
+    ctx->snap_n = n;
+    ctx->snap_e = e;
+    return PG_OK;
+}
+
+int pg_snapshot_restore(pg_ctx* ctx) {
+    if (!ctx) return PG_EINVAL;
+    if (!ctx->snap) return fail(ctx, PG_ESTATE, "pg_snapshot_restore: no snapshot saved");
+    PG_HIP(hipSetDevice(ctx->cfg.device));
+    hipStream_t s = ctx->stream;
+    const int64_t n = ctx->snap_n, e = ctx->snap_e;
+    ctx->cur = 0;
+    PG_HIP(hipMemcpyAsync(ctx->birth, ctx->s_birth, n * 4, hipMemcpyDeviceToDevice, s));
+    PG_HIP(hipMemcpyAsync(ctx->in_deg, ctx->s_in_deg, n * 4, hipMemcpyDeviceToDevice, s));
+    PG_HIP(hipMemcpyAsync(ctx->fwd_off, ctx->s_fwd_off, (n + 1) * 4, hipMemcpyDeviceToDevice, s));
+    PG_HIP(hipMemcpyAsync(ctx->bwd_off[0], ctx->s_bwd_off, (n + 1) * 4, hipMemcpyDeviceToDevice, s));
+    if (e) {
+        PG_HIP(hipMemcpyAsync(ctx->fwd_adj, ctx->s_fwd_adj, e * 4, hipMemcpyDeviceToDevice, s));
+        PG_HIP(hipMemcpyAsync(ctx->bwd_adj[0], ctx->s_bwd_adj, e * 4, hipMemcpyDeviceToDevice, s));
+    }
+    PG_HIP(hipStreamSynchronize(s));
+    ctx->n = n;
+    ctx->e = e;
+    ctx->last_model = -1;
+    return PG_OK;
+}
+
+int pg_profile_enable(pg_ctx* ctx, int on) {
+    if (!ctx) return PG_EINVAL;
+    ctx->prof = on != 0;
+    return PG_OK;
+}
+
+int pg_profile_read(pg_ctx* ctx, int n, double* ms, double* bytes, int64_t* launches) {
+    if (!ctx || n < 0) return PG_EINVAL;
+    for (int k = 0; k < n && k < PG_K_COUNT; ++k) {
+        if (ms) ms[k] = ctx->prof_ms[k];
+        if (bytes) bytes[k] = ctx->prof_bytes[k];
+        if (launches) launches[k] = ctx->prof_calls[k];
+    }
+    return PG_OK;
+}
+
+int pg_profile_reset(pg_ctx* ctx) {
+    if (!ctx) return PG_EINVAL;
+    for (int k = 0; k < PG_K_COUNT; ++k) {
+        ctx->prof_ms[k] = 0.0;
+        ctx->prof_bytes[k] = 0.0;
+        ctx->prof_calls[k] = 0;
+    }
+    return PG_OK;
+}
+
+int pg_graph_size(const pg_ctx* ctx, int64_t* n_papers, int64_t* n_edges) {
+    if (!ctx) return PG_EINVAL;
+    if (n_papers) *n_papers = ctx->n;
+    if (n_edges) *n_edges = ctx->e;
+    return PG_OK;
+}
+
+int pg_graph_download(pg_ctx* ctx, int32_t* fwd_off, int32_t* fwd_adj, int32_t* bwd_off,
+                      int32_t* bwd_adj, int32_t* in_deg, int32_t* birth, float* logfit) {
+    if (!ctx) return PG_EINVAL;
+    if (!ctx->loaded) return fail(ctx, PG_ESTATE, "pg_graph_download: no graph loaded");
+    PG_HIP(hipSetDevice(ctx->cfg.device));
+    hipStream_t s = ctx->stream;
+    const int64_t n = ctx->n, e = ctx->e;
+    auto cp = [&](void* dst, const void* src, size_t bytes) {
+        return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s) : hipSuccess;
+    };
+    if (fwd_off) PG_HIP(cp(fwd_off, ctx->fwd_off, (n + 1) * 4));
+    if (fwd_adj) PG_HIP(cp(fwd_adj, ctx->fwd_adj, e * 4));
+    if (bwd_off) PG_HIP(cp(bwd_off, ctx->bwd_off[ctx->cur], (n + 1) * 4));
+    if (bwd_adj) PG_HIP(cp(bwd_adj, ctx->bwd_adj[ctx->cur], e * 4));
+    if (in_deg) PG_HIP(cp(in_deg, ctx->in_deg, n * 4));
+    if (birth) PG_HIP(cp(birth, ctx->birth, n * 4));
+    if (logfit) PG_HIP(cp(logfit, ctx->logfit, n * 4));
+    PG_HIP(hipStreamSynchronize(s));
+    return PG_OK;
+}
+
+int pg_year_citations(pg_ctx* ctx, int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* near_boundary) {
+    if (!ctx) return PG_EINVAL;
+    if (ctx->last_model != PG_MODEL_PA)
+        return fail(ctx, PG_ESTATE, "pg_year_citations: the last step was not a PA year");
+    PG_HIP(hipSetDevice(ctx->cfg.device));
+    hipStream_t s = ctx->stream;
+    const int64_t D = ctx->last_n_new, M = ctx->last_m;
+    if (cit && D * M) PG_HIP(hipMemcpyAsync(cit, ctx->cit, D * M * 4, hipMemcpyDeviceToHost, s));
+    if (kind && D * M) PG_HIP(hipMemcpyAsync(kind, ctx->kind, D * M, hipMemcpyDeviceToHost, s));
+    if (cnt && D) PG_HIP(hipMemcpyAsync(cnt, ctx->cnt, D * 4, hipMemcpyDeviceToHost, s));
+    if (near_boundary && D) PG_HIP(hipMemcpyAsync(near_boundary, ctx->near, D * 4, hipMemcpyDeviceToHost, s));
+    PG_HIP(hipStreamSynchronize(s));
+    return PG_OK;
+}
+
+int pg_year_weights(pg_ctx* ctx, float* w, double* cdf) {
+    if (!ctx) return PG_EINVAL;
+    if (ctx->last_model != PG_MODEL_PA)
+        return fail(ctx, PG_ESTATE, "pg_year_weights: the last step was not a PA year");
+    PG_HIP(hipSetDevice(ctx->cfg.device));
+    hipStream_t s = ctx->stream;
+    const int64_t N = ctx->last_n_exist;
+    if (w) PG_HIP(hipMemcpyAsync(w, ctx->w, N * 4, hipMemcpyDeviceToHost, s));
+    if (cdf) PG_HIP(hipMemcpyAsync(cdf, ctx->cdf, N * 8, hipMemcpyDeviceToHost, s));
+    PG_HIP(hipStreamSynchronize(s));
+    return PG_OK;
+}
+
+int pg_bfs(pg_ctx* ctx, int64_t count, const int32_t* roots, int32_t* out, int32_t* out_len) {
+    if (!ctx) return PG_EINVAL;
+    if (!ctx->loaded) return fail(ctx, PG_ESTATE, "pg_bfs: no graph loaded");
+    if (count < 0 || (count > 0 && (!roots || !out || !out_len)))
+        return fail(ctx, PG_EINVAL, "pg_bfs: bad arguments");
+    if (count == 0) return PG_OK;
+    for (int64_t i = 0; i < count; ++i)
+        if (roots[i] < 0 || roots[i] >= ctx->n) return fail(ctx, PG_EINVAL, "pg_bfs: root out of range");
+    PG_HIP(hipSetDevice(ctx->cfg.device));
+    hipStream_t s = ctx->stream;
+    const int cap = ctx->cfg.bfs_cap;
+    int32_t *d_roots = nullptr, *d_out = nullptr, *d_len = nullptr;
+    PG_HIP(dalloc(&d_roots, count));
+    hipError_t e1 = dalloc(&d_out, count * cap);
+    hipError_t e2 = dalloc(&d_len, count);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        (void)hipFree(d_roots); (void)hipFree(d_out); (void)hipFree(d_len);
+        return fail(ctx, PG_ENOMEM, "pg_bfs: device allocation failed");
+    }
+    hipError_t rc = hipMemcpyAsync(d_roots, roots, count * 4, hipMemcpyHostToDevice, s);
+    if (rc == hipSuccess) {
+        launch_bfs_only(s, view(ctx), count, d_roots, cap, d_out, d_len);
+        rc = hipGetLastError();
+    }
+    if (rc == hipSuccess) rc = hipMemcpyAsync(out, d_out, count * cap * 4, hipMemcpyDeviceToHost, s);
+    if (rc == hipSuccess) rc = hipMemcpyAsync(out_len, d_len, count * 4, hipMemcpyDeviceToHost, s);
+    if (rc == hipSuccess) rc = hipStreamSynchronize(s);
+    (void)hipFree(d_roots); (void)hipFree(d_out); (void)hipFree(d_len);
+    if (rc != hipSuccess) return fail(ctx, PG_EHIP, std::string("pg_bfs: ") + hipGetErrorString(rc));
+    return PG_OK;
+}
+
+int64_t pg_growth_count(int64_t n, double rate) {
+    return (int64_t)std::ceil((double)n * rate);
+}
+
+}  // extern "C"

@@ -1,0 +1,115 @@
+// pg_internal.h -- kernel launchers and device helpers shared by the HIP
+// translation units of libparagraph.so.  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pg_math.h"
+
+namespace pg {
+
+constexpr int kTile = 4096;          // elements per scan / radix tile
+constexpr int kBfsCapMax = 4096;     // largest neighbourhood cap
+constexpr int kMaxCites = 64;        // largest citations per paper
+constexpr int kAttemptsPerCite = 64; // PA / neighbourhood attempt budget
+constexpr double kNearRel = 1e-12;   // boundary-case tolerance (relative)
+
+// First index k in [0,n) with cdf[k] > t, clamped to n-1.
+__device__ __forceinline__ int64_t cdf_upper_bound(const double* __restrict__ cdf, int64_t n, double t) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (cdf[mid] > t) hi = mid; else lo = mid + 1;
+    }
+    return lo < n ? lo : n - 1;
+}
+
+__device__ __forceinline__ bool cdf_near(const double* __restrict__ cdf, int64_t n, int64_t k,
+                                         double t, double eps) {
+    return (k > 0 && t - cdf[k - 1] <= eps) || (k < n - 1 && cdf[k] - t <= eps);
+}
+
+// Decoupled look-back state for the single-pass scans.  status[b] holds
+// epoch*4 + 1 (aggregate published) or epoch*4 + 2 (inclusive published).
+struct ScanState {
+    double* agg;
+    double* incl;
+    uint32_t* status;
+    uint32_t* ticket;
+    uint32_t epoch;
+};
+
+// ---- launchers (stream-ordered, no host synchronisation) ----
+
+// w[i] = pa_weight(...) for i < n; cdf[i] = FP64 inclusive prefix of w.
+void launch_weights_cdf(hipStream_t s, int64_t n, const int32_t* in_deg, const int32_t* birth,
+                        const float* logfit, int32_t year, float alpha, float inv_tau,
+                        float* w, double* cdf, const ScanState& st);
+
+// Exclusive prefix sum of int32 in[0,n) into int64 out[0,n]; out[n] = total.
+// scratch: at least scan_scratch_elems(n) int64.
+int64_t scan_scratch_elems(int64_t n);
+void launch_excl_scan_i32(hipStream_t s, const int32_t* in, int64_t n, int64_t* out, int64_t* scratch);
+
+// PA phase 1: n_pa distinct draws per new paper from the global CDF.
+void launch_pa_draw(hipStream_t s, int64_t n_exist, const double* cdf, int64_t n_new, int m,
+                    int n_pa, uint64_t seed, uint32_t year_index, int32_t* cit, int8_t* kind,
+                    int32_t* cnt, int32_t* near, int32_t* anext);
+
+struct CsrView {
+    const int32_t* fwd_off;
+    const int32_t* fwd_adj;
+    const int32_t* bwd_off;
+    const int32_t* bwd_adj;
+    int64_t n;
+};
+
+// PA phases 2-3: 2-hop neighbourhood of the first pick, neighbourhood
+// draws, refill.  One workgroup per new paper.
+void launch_bfs_nb(hipStream_t s, const CsrView& g, const float* w, const double* cdf,
+                   int64_t n_new, int m, int n_nb, int cap, uint64_t seed, uint32_t year_index,
+                   int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* near, int32_t* anext,
+                   unsigned long long* bytes_out);
+
+// Stand-alone 2-hop neighbourhoods (out[i*cap ...], out_len[i]).
+void launch_bfs_only(hipStream_t s, const CsrView& g, int64_t count, const int32_t* roots, int cap,
+                     int32_t* out, int32_t* out_len);
+
+// ---- CSR append ----
+// Sorted forward rows of the new papers appended at fwd_adj + fwd_base;
+// src_of[e] = citing id; add[t] += 1 per edge.  new_off: int64 exclusive
+// offsets of the per-paper counts (n_new+1).
+void launch_pa_compact(hipStream_t s, int64_t n_exist, int64_t n_new, int m, const int32_t* cit,
+                       const int32_t* cnt, const int64_t* new_off, int64_t fwd_base,
+                       int32_t* fwd_off, int32_t* fwd_adj, int32_t* src_of, int32_t* add);
+
+// Stable LSD radix sort of (key,val) pairs by key (keys < 2^key_bits).
+// Result lands in (k_out, v_out).  Buffers k_tmp/v_tmp same size as n.
+int64_t radix_scratch_elems(int64_t n);
+void launch_radix_sort_pairs(hipStream_t s, const uint32_t* k_in, const uint32_t* v_in,
+                             uint32_t* k_out, uint32_t* v_out, uint32_t* k_tmp, uint32_t* v_tmp,
+                             int64_t n, int key_bits, int64_t* scratch);
+
+// Backward offsets and degrees after adding add[t] citations to paper t:
+// bwd_off_new[v] = bwd_off_old[min(v,n_old)] + excl(add)[v]; in_deg[v] += add[v].
+void launch_bwd_offsets(hipStream_t s, int64_t n_old, int64_t n_new, const int32_t* bwd_off_old,
+                        const int32_t* add, int32_t* bwd_off_new, int32_t* in_deg, int64_t* scratch);
+
+// bwd_adj_new = old rows followed by the row's new citers (sorted_src,
+// grouped by target in ascending citing order).
+void launch_bwd_merge(hipStream_t s, int64_t n_old, int64_t n_total, const int32_t* bwd_off_old,
+                      const int32_t* bwd_adj_old, const int32_t* bwd_off_new,
+                      const uint32_t* sorted_src, int32_t* bwd_adj_new, int64_t e_new);
+
+// ---- ER ----
+void launch_er_count(hipStream_t s, int64_t n_exist, int64_t n_new, double p, uint64_t seed,
+                     uint32_t year_index, int32_t* counts);
+void launch_er_write(hipStream_t s, int64_t n_exist, int64_t n_new, double p, uint64_t seed,
+                     uint32_t year_index, const int64_t* new_off, int64_t fwd_base,
+                     int32_t* fwd_off, int32_t* fwd_adj, int32_t* src_of, int32_t* add);
+
+// Small helpers.
+void launch_fill_i32(hipStream_t s, int32_t* p, int32_t v, int64_t n);
+void launch_sum_i32(hipStream_t s, const int32_t* p, int64_t n, unsigned long long* out);
+
+}  // namespace pg

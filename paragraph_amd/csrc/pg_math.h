@@ -1,0 +1,157 @@
+// pg_math.h -- counter-based RNG and deterministic elementary functions used
+// by every kernel and by the host-side seed generator.
+//
+// Philox4x32-10 follows Salmon et al., "Parallel random numbers: as easy as
+// 1, 2, 3" (SC'11).  It replaces the reference's curand XORWOW streams
+// (src/kernel.cu:233, src/abm.cu:529), which were seeded from the wall clock
+// and so could not be reproduced.  Counter layout: (paper id, attempt,
+// growth-year index, tag); key: the 64-bit simulation seed.
+//
+// exp/log use Cody-Waite reduction with Taylor polynomials (Horner, explicit
+// fma) and the atanh series log(1+f) = 2 atanh(f/(2+f)).  Each operation is
+// spelled out and the build disables FP contraction, so host and gfx950
+// produce the same bits for the same inputs.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PG_HD __host__ __device__ __forceinline__
+
+namespace pg {
+
+enum : uint32_t { TAG_PA = 0, TAG_NB = 1, TAG_ER = 2, TAG_SEED_BIRTH = 3,
+                  TAG_FITNESS = 4, TAG_SEED_COUNT = 5, TAG_SEED_CITE = 6 };
+
+struct u32x4 { uint32_t x, y, z, w; };
+
+PG_HD u32x4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint64_t seed) {
+    uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return u32x4{c0, c1, c2, c3};
+}
+
+// Uniform in [0,1) with 53 random bits.
+PG_HD double u53(uint32_t x0, uint32_t x1) {
+    uint64_t v = ((uint64_t)x0 << 21) ^ (uint64_t)(x1 >> 11);
+    return (double)v * 0x1p-53;
+}
+// Uniform in (0,1].
+PG_HD double u53_open0(uint32_t x0, uint32_t x1) {
+    uint64_t v = ((uint64_t)x0 << 21) ^ (uint64_t)(x1 >> 11);
+    return (double)(v + 1) * 0x1p-53;
+}
+
+PG_HD uint32_t f2u(float x) { return __builtin_bit_cast(uint32_t, x); }
+PG_HD float u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
+PG_HD uint64_t d2u(double x) { return __builtin_bit_cast(uint64_t, x); }
+PG_HD double u2d(uint64_t u) { return __builtin_bit_cast(double, u); }
+
+PG_HD float det_expf(float x) {
+    if (x != x) return x;
+    if (x > 88.7228317f) return __builtin_huge_valf();
+    if (x < -87.3365402f) return 0.0f;
+    float kf = __builtin_rintf(x * 0x1.715476p+0f);
+    float r = __builtin_fmaf(kf, -0x1.62e4p-1f, x);
+    r = __builtin_fmaf(kf, -0x1.7f7d1cp-20f, r);
+    float p = 1.0f / 5040.0f;
+    p = __builtin_fmaf(p, r, 1.0f / 720.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 120.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 24.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 6.0f);
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    int n = (int)kf;
+    int n1 = n / 2;
+    int n2 = n - n1;
+    float s1 = u2f((uint32_t)(n1 + 127) << 23);
+    float s2 = u2f((uint32_t)(n2 + 127) << 23);
+    return (p * s1) * s2;
+}
+
+PG_HD float det_logf(float x) {
+    if (x != x) return x;
+    if (x < 0.0f) return __builtin_nanf("");
+    if (x == 0.0f) return -__builtin_huge_valf();
+    if (x == __builtin_huge_valf()) return x;
+    uint32_t b = f2u(x);
+    int e = 0;
+    if (b < 0x00800000u) { x = x * 8388608.0f; b = f2u(x); e = -23; }
+    e += (int)(b >> 23) - 127;
+    float mm = u2f((b & 0x007fffffu) | 0x3f800000u);
+    if (mm > 1.41421354f) { mm = mm * 0.5f; e += 1; }
+    float f = mm - 1.0f;
+    float s = f / (2.0f + f);
+    float z = s * s;
+    float R = __builtin_fmaf(z, 2.0f / 11.0f, 2.0f / 9.0f);
+    R = __builtin_fmaf(z, R, 2.0f / 7.0f);
+    R = __builtin_fmaf(z, R, 2.0f / 5.0f);
+    R = __builtin_fmaf(z, R, 2.0f / 3.0f);
+    R = z * R;
+    float t = f - R;
+    t = s * t;
+    float lm = f - t;
+    float ef = (float)e;
+    return __builtin_fmaf(ef, 0x1.62e4p-1f, __builtin_fmaf(ef, 0x1.7f7d1cp-20f, lm));
+}
+
+PG_HD double det_log_core(double f) {
+    double s = f / (2.0 + f);
+    double z = s * s;
+    double R = __builtin_fma(z, 2.0 / 21.0, 2.0 / 19.0);
+    R = __builtin_fma(z, R, 2.0 / 17.0);
+    R = __builtin_fma(z, R, 2.0 / 15.0);
+    R = __builtin_fma(z, R, 2.0 / 13.0);
+    R = __builtin_fma(z, R, 2.0 / 11.0);
+    R = __builtin_fma(z, R, 2.0 / 9.0);
+    R = __builtin_fma(z, R, 2.0 / 7.0);
+    R = __builtin_fma(z, R, 2.0 / 5.0);
+    R = __builtin_fma(z, R, 2.0 / 3.0);
+    R = z * R;
+    double t = f - R;
+    t = s * t;
+    return f - t;
+}
+
+PG_HD double det_log(double x) {
+    if (x != x) return x;
+    if (x < 0.0) return __builtin_nan("");
+    if (x == 0.0) return -__builtin_huge_val();
+    if (x == __builtin_huge_val()) return x;
+    uint64_t b = d2u(x);
+    int e = 0;
+    if (b < 0x0010000000000000ull) { x = x * 0x1p52; b = d2u(x); e = -52; }
+    e += (int)(b >> 52) - 1023;
+    double mm = u2d((b & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
+    if (mm > 1.4142135623730951) { mm = mm * 0.5; e += 1; }
+    double lm = det_log_core(mm - 1.0);
+    double ef = (double)e;
+    return __builtin_fma(ef, 6.93147180369123816490e-01,
+                         __builtin_fma(ef, 1.90821492927058770002e-10, lm));
+}
+
+PG_HD double det_log1p(double x) {
+    if (x > -0.29289321881345248 && x < 0.41421356237309515) return det_log_core(x);
+    return det_log(1.0 + x);
+}
+
+// PA weight of one paper: exp(alpha*log(k+1) - age*inv_tau + logfit), FP32.
+PG_HD float pa_weight(int32_t k, int32_t age, float logfit, float alpha, float inv_tau) {
+    float lk = det_logf((float)(k + 1));
+    float t1 = __builtin_fmaf(-(float)age, inv_tau, logfit);
+    float x = __builtin_fmaf(alpha, lk, t1);
+    return det_expf(x);
+}
+
+}  // namespace pg

@@ -1,0 +1,209 @@
+"""GPU-vs-oracle parity through the C ABI (needs an MI355X).
+
+Every comparison starts the oracle from the exact state the GPU started the
+year from (downloaded from the device), so each year is checked on its own:
+  - FP32 weights: bit-exact;
+  - FP64 CDF: within 1e-12 relative (blocked vs sequential summation);
+  - draws: bit-exact, except papers where a PA uniform fell within 1e-12*Z of
+    a CDF boundary (counted by both sides; every mismatching paper must be
+    one of them);
+  - CSR after the append: bit-exact against the oracle's append of the GPU's
+    draws; in-degrees equal the backward row lengths.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import paragraph_amd as pg
+from oracle import binding as ob
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def new_ctx(g: pg.SeedGraph, sched, m, cap=4096, seed=1):
+    n_final = g.n + int(sum(sched))
+    e_cap = int(g.src.size + sum(sched) * max(m, 1) + 16)
+    ctx = pg.Context(n_final, e_cap, max(max(sched), 1), max_cites=max(m, 1), bfs_cap=cap,
+                     seed=seed)
+    ctx.load(g)
+    return ctx
+
+
+def oracle_graph(ctx, logfit_full):
+    return ob.Graph.from_download(ctx.download(), logfit_full)
+
+
+def check_pa_year(ctx, logfit_full, year, yi, D, m, n_nb, alpha, tau, seed, cap):
+    G = oracle_graph(ctx, logfit_full)
+    N = G.n
+    log(f"PA year {year} (idx {yi}): N={N} D={D} m={m} n_nb={n_nb} alpha={alpha} tau={tau} cap={cap}")
+    st = ctx.year_pa(year, yi, D, m, n_nb, alpha, tau)
+    assert st.n_before == N and st.n_after == N + D
+    w_gpu, c_gpu = ctx.weights(N)
+    inv_tau = 0.0 if np.isinf(tau) else 1.0 / tau
+    w_ref = ob.weights(G, year, alpha, inv_tau)
+    assert np.array_equal(w_gpu.view(np.uint32), w_ref.view(np.uint32)), "weights differ"
+    c_ref = ob.cdf(w_ref)
+    assert np.all(np.abs(c_gpu - c_ref) <= 1e-12 * c_ref[-1]), "CDF beyond 1e-12 relative"
+    assert np.all(np.diff(c_gpu) >= -1e-15 * c_ref[-1])
+    gres = ctx.citations(D, m)
+    ref = ob.pa_year(G, seed, yi, year, D, m, n_nb, alpha, tau, cap)
+    same = np.all(gres["cit"] == ref["cit"], axis=1) & (gres["cnt"] == ref["cnt"]) & \
+        np.all(gres["kind"] == ref["kind"], axis=1)
+    bad = np.nonzero(~same)[0]
+    flagged = (gres["near"] > 0) | (ref["near"] > 0)
+    log(f"  edges={st.edges_added} near={st.near_boundary} mismatching papers={bad.size} "
+        f"(flagged {int(flagged[bad].sum()) if bad.size else 0}) device_ms={st.device_ms:.2f}")
+    assert np.all(flagged[bad]), f"unflagged mismatches at papers {bad[~flagged[bad]][:10]}"
+    assert int(gres["near"].sum()) == st.near_boundary
+    assert st.edges_added == int(gres["cnt"].sum())
+    # rows: distinct, existing targets only
+    for i in range(0, D, max(1, D // 200)):
+        row = gres["cit"][i, :gres["cnt"][i]]
+        assert np.unique(row).size == row.size
+        assert row.min() >= 0 and row.max() < N
+    # CSR append: oracle appends the GPU's draws
+    off, tgt = ob.pa_compact(gres)
+    G2 = ob.csr_append(G, off, tgt, logfit_full, year)
+    d = ctx.download()
+    for k in ("fwd_off", "fwd_adj", "bwd_off", "bwd_adj", "in_deg", "birth"):
+        assert np.array_equal(d[k], getattr(G2, k)), f"{k} differs after append"
+    assert np.array_equal(np.diff(d["bwd_off"]), d["in_deg"])
+    return st, bad.size
+
+
+@pytest.mark.parametrize("years", [3])
+def test_pa_config0_years(years):
+    """Config 0: 50,000 papers, Poisson(8), 3 years at 3%, m=10, 30% neighbourhood."""
+    sched = pg.growth_schedule(50000, 0.03, years)
+    g = pg.seed_graph(1, 50000, 8.0, n_final=50000 + sum(sched))
+    ctx = new_ctx(g, sched, 10)
+    total_bad = 0
+    for yi, D in enumerate(sched):
+        _, nb = check_pa_year(ctx, g.logfit, 1 + yi, yi, D, 10, 3, 1.0, 5.0, 1, 4096)
+        total_bad += nb
+    n, e = ctx.size()
+    assert n == 54637
+    assert e == g.src.size + 46370 or e <= g.src.size + 46370
+
+
+def hub_graph(n=20000, hubs=8, seed=7):
+    """Seed graph with a few very highly cited papers (stresses caps/windows)."""
+    rng = np.random.default_rng(seed)
+    birth = np.sort(rng.integers(-20, 1, n)).astype(np.int32)
+    src, dst = [], []
+    for i in range(hubs + 1, n):
+        k = min(6, i)
+        t = rng.choice(i, size=k, replace=False)
+        src += [i] * k
+        dst += list(t)
+        hs = np.nonzero(rng.random(hubs) < 0.45)[0]
+        hs = hs[~np.isin(hs, t)]
+        src += [i] * hs.size
+        dst += list(hs)
+    logfit = (rng.standard_normal(n + 4000) * 0.5).astype(np.float32)
+    return pg.SeedGraph(birth, np.array(src, np.int32), np.array(dst, np.int32), logfit)
+
+
+@pytest.mark.parametrize("cap", [4096, 300, 17])
+def test_pa_hub_graph_caps(cap):
+    g = hub_graph()
+    sched = [600, 600]
+    ctx = new_ctx(g, sched, 12, cap=cap, seed=5)
+    for yi, D in enumerate(sched):
+        check_pa_year(ctx, g.logfit, 1 + yi, yi, D, 12, 5, 1.3, 3.0, 5, cap)
+
+
+@pytest.mark.parametrize("params", [(0.0, float("inf"), 4, 3), (1.5, 1.0, 20, 19),
+                                    (0.5, 10.0, 6, 0), (1.0, 5.0, 1, 0)])
+def test_pa_param_edges(params):
+    alpha, tau, m, n_nb = params
+    g = pg.seed_graph(3, 4000, 5.0, n_final=4500)
+    ctx = new_ctx(g, [300], m, cap=4096, seed=11)
+    check_pa_year(ctx, g.logfit, 1, 0, 300, m, n_nb, alpha, tau, 11, 4096)
+
+
+def test_pa_tiny_graph_exhaustion():
+    """Fewer eligible papers than citations: attempt budget ends the draws."""
+    g = pg.SeedGraph(np.array([-3, -2, -1], np.int32), np.array([1, 2], np.int32),
+                     np.array([0, 1], np.int32), np.zeros(10, np.float32))
+    ctx = new_ctx(g, [4], 5, cap=8, seed=2)
+    st, _ = check_pa_year(ctx, g.logfit, 1, 0, 4, 5, 2, 1.0, 5.0, 2, 8)
+    assert st.shortfall > 0
+
+
+@pytest.mark.parametrize("cap", [4096, 64, 1])
+def test_bfs_sets(cap):
+    g = hub_graph(n=8000, hubs=5, seed=3)
+    ctx = new_ctx(g, [1], 1, cap=cap)
+    G = oracle_graph(ctx, g.logfit)
+    rng = np.random.default_rng(0)
+    roots = np.concatenate([np.arange(0, 8), rng.integers(0, G.n, 120)]).astype(np.int32)
+    log(f"BFS cap={cap} roots={roots.size}")
+    got = ctx.bfs(roots)
+    for r, s in zip(roots, got):
+        ref = ob.bfs(G, int(r), cap)
+        assert np.array_equal(s, ref), f"root {r}: {s[:8]} vs {ref[:8]}"
+
+
+def test_bfs_isolated_and_path():
+    n = 12
+    birth = np.zeros(n, np.int32)
+    src = np.array([1, 2, 3, 4, 6], np.int32)   # path 0-1-2-3-4, 5 isolated, 6-0
+    dst = np.array([0, 1, 2, 3, 0], np.int32)
+    g = pg.SeedGraph(birth, src, dst, np.zeros(n, np.float32))
+    ctx = new_ctx(g, [1], 1, cap=4096)
+    G = oracle_graph(ctx, g.logfit)
+    roots = np.arange(n, dtype=np.int32)
+    for r, s in zip(roots, ctx.bfs(roots)):
+        assert np.array_equal(s, ob.bfs(G, int(r), 4096))
+    assert ctx.bfs(np.array([5], np.int32))[0].size == 0
+    assert list(ctx.bfs(np.array([2], np.int32))[0]) == [0, 1, 3, 4]
+
+
+@pytest.mark.parametrize("p_link", [5e-4, 0.0, 1.0])
+def test_er_edge_sets(p_link):
+    n0 = 20000 if 0 < p_link < 1 else 60
+    D = 600 if 0 < p_link < 1 else 7
+    g = pg.SeedGraph(np.zeros(n0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32),
+                     np.zeros(n0 + 2 * D, np.float32))
+    cap_e = int(n0 * D * 2 * max(p_link, 1e-9) + 5000) if p_link < 1 else n0 * 2 * D + 2 * D * D
+    ctx = pg.Context(n0 + 2 * D, cap_e, D, max_cites=1, bfs_cap=16, seed=9)
+    ctx.load(g)
+    N = n0
+    for yi in range(2):
+        log(f"ER year {yi}: N={N} D={D} p={p_link}")
+        G = oracle_graph(ctx, g.logfit)
+        st = ctx.year_er(1 + yi, yi, D, p_link)
+        off, tgt = ob.er_year(N, D, p_link, 9, yi)
+        assert st.edges_added == tgt.size
+        d = ctx.download()
+        fo = d["fwd_off"]
+        got = d["fwd_adj"][fo[N]:fo[N + D]]
+        assert np.array_equal(got, tgt)
+        assert np.array_equal(fo[N:N + D + 1] - fo[N], off)
+        G2 = ob.csr_append(G, off, tgt, g.logfit, 1 + yi)
+        for k in ("fwd_off", "fwd_adj", "bwd_off", "bwd_adj", "in_deg"):
+            assert np.array_equal(d[k], getattr(G2, k)), k
+        N += D
+
+
+def test_snapshot_restore_reproduces():
+    sched = [500, 515]
+    g = pg.seed_graph(4, 5000, 6.0, n_final=5000 + sum(sched))
+    ctx = new_ctx(g, sched, 8, seed=4)
+    ctx.snapshot_save()
+    runs = []
+    for _ in range(2):
+        for yi, D in enumerate(sched):
+            ctx.year_pa(1 + yi, yi, D, 8, 2, 1.0, 5.0)
+        runs.append(ctx.download())
+        ctx.snapshot_restore()
+    for k in runs[0]:
+        assert np.array_equal(runs[0][k], runs[1][k]), k
