@@ -60,6 +60,10 @@ def lib() -> ctypes.CDLL:
             "or_logf": (ctypes.c_float, [ctypes.c_float]),
             "or_log": (ctypes.c_double, [ctypes.c_double]),
             "or_log1p": (ctypes.c_double, [ctypes.c_double]),
+            "or_vec_expf": (None, [ctypes.c_int64, _f32p, _f32p]),
+            "or_vec_logf": (None, [ctypes.c_int64, _f32p, _f32p]),
+            "or_vec_log": (None, [ctypes.c_int64, _f64p, _f64p]),
+            "or_vec_log1p": (None, [ctypes.c_int64, _f64p, _f64p]),
             "or_weights": (None, [ctypes.c_int64, _i32p, _i32p, _f32p, ctypes.c_int32,
                                   ctypes.c_float, ctypes.c_float, _f32p]),
             "or_cdf": (None, [ctypes.c_int64, _f32p, _f64p]),
@@ -266,3 +270,16 @@ def loglik_grad(n_exist, n_new, year_of, deg_list, cit_list, kind_list, cnt_list
     if rc:
         raise RuntimeError(f"or_pa_loglik_grad failed: {rc}")
     return ll.value, ga.value, gt.value, gphi
+
+
+def vec(fn: str, x: np.ndarray) -> np.ndarray:
+    """Elementwise oracle function: expf/logf (float32) or log/log1p (float64)."""
+    if fn in ("expf", "logf"):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.empty_like(x)
+        getattr(lib(), "or_vec_" + fn)(x.size, _p(x, ctypes.c_float), _p(y, ctypes.c_float))
+    else:
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty_like(x)
+        getattr(lib(), "or_vec_" + fn)(x.size, _p(x, ctypes.c_double), _p(y, ctypes.c_double))
+    return y

@@ -349,6 +349,26 @@ int32_t or_bfs(const or_csr* g, int32_t root, int32_t cap, int32_t* out) {
 #define PA_ATTEMPTS_PER_CITE 64
 #define NEAR_REL 1e-12
 
+/* Neighbourhood draws use an exact fixed-point CDF: weight w becomes
+ * floor(w * 2^(49 - e)) with e = the binary exponent field of the largest
+ * weight in the neighbourhood (saturated at 2^50), prefix sums are uint64,
+ * and a draw takes the first entry whose prefix exceeds
+ * floor(v * Z / 2^53), v the 53-bit Philox integer. */
+static inline double nb_scale(uint32_t max_bits) {
+    int e = (int)(max_bits >> 23) - 127;
+    return u2d((uint64_t)(1023 + 49 - e) << 52);
+}
+
+static inline uint64_t nb_fixed(float w, double scale) {
+    double q = (double)w * scale;
+    return q < 0x1p50 ? (uint64_t)q : (uint64_t)1 << 50;
+}
+
+static inline uint64_t nb_target(uint32_t x0, uint32_t x1, uint64_t Z) {
+    uint64_t v = ((uint64_t)x0 << 21) ^ (uint64_t)(x1 >> 11);
+    return (uint64_t)(((unsigned __int128)v * Z) >> 53);
+}
+
 static inline int in_list(const int32_t* a, int n, int32_t x) {
     for (int i = 0; i < n; ++i) if (a[i] == x) return 1;
     return 0;
@@ -381,7 +401,7 @@ int or_pa_year(const or_csr* g, const int32_t* in_deg, const int32_t* birth,
     {
         bfs_buf bb = {0};
         int32_t* S = (int32_t*)malloc((size_t)p->cap * sizeof(int32_t));
-        double* D = (double*)malloc((size_t)p->cap * sizeof(double));
+        uint64_t* Q = (uint64_t*)malloc((size_t)p->cap * sizeof(uint64_t));
 #pragma omp for schedule(dynamic, 64)
         for (int64_t i = 0; i < p->n_new; ++i) {
             const uint32_t gid = (uint32_t)(N + i);
@@ -407,15 +427,29 @@ int or_pa_year(const or_csr* g, const int32_t* in_deg, const int32_t* birth,
                     for (int32_t s = 0; s < ns; ++s)
                         if (!in_list(row, c, S[s])) { row[c] = S[s]; kr[c] = 1; ++c; }
                 } else {
-                    double acc = 0.0;
-                    for (int32_t s = 0; s < ns; ++s) { acc += (double)w[S[s]]; D[s] = acc; }
-                    const double Zs = acc;
+                    /* neighbourhood CDF in 64-bit fixed point, scaled so the
+                     * largest weight of S maps below 2^50: exact and
+                     * independent of summation order */
+                    uint32_t mb = 0;
+                    for (int32_t s = 0; s < ns; ++s) {
+                        uint32_t b = f2u(w[S[s]]);
+                        if (b > mb) mb = b;
+                    }
+                    const double scale = nb_scale(mb);
+                    uint64_t acc = 0;
+                    for (int32_t s = 0; s < ns; ++s) { acc += nb_fixed(w[S[s]], scale); Q[s] = acc; }
+                    const uint64_t Zs = acc;
                     int got = 0;
                     for (int bt = 0; got < n_nb && bt < b_max; ++bt) {
                         draw4(p->seed, gid, (uint32_t)bt, yi, 1u, r4);
-                        double t = or_u53(r4[0], r4[1]) * Zs;
-                        int64_t s = or_upper_bound(D, ns, t);
-                        int32_t x = S[s];
+                        const uint64_t t = nb_target(r4[0], r4[1], Zs);
+                        int32_t lo = 0, hi = ns;
+                        while (lo < hi) {
+                            int32_t mid = (lo + hi) >> 1;
+                            if (Q[mid] > t) hi = mid; else lo = mid + 1;
+                        }
+                        if (lo >= ns) lo = ns - 1;
+                        int32_t x = S[lo];
                         if (!in_list(row, c, x)) { row[c] = x; kr[c] = 1; ++c; ++got; }
                     }
                 }
@@ -433,7 +467,7 @@ int or_pa_year(const or_csr* g, const int32_t* in_deg, const int32_t* birth,
             cnt[i] = c;
             near[i] = nr;
         }
-        free(S); free(D);
+        free(S); free(Q);
         buf_free(&bb);
     }
     free(w); free(C);
@@ -607,4 +641,18 @@ int or_pa_loglik_grad(int32_t n_years, const int64_t* n_exist,
     *loglik = ll; *grad_alpha = ga; *grad_tau = gt;
     free(w); free(L); free(later);
     return 0;
+}
+
+/* Vector forms of the elementary functions (for the ulp sweeps in tests). */
+void or_vec_expf(int64_t n, const float* x, float* y) {
+    for (int64_t i = 0; i < n; ++i) y[i] = or_expf(x[i]);
+}
+void or_vec_logf(int64_t n, const float* x, float* y) {
+    for (int64_t i = 0; i < n; ++i) y[i] = or_logf(x[i]);
+}
+void or_vec_log(int64_t n, const double* x, double* y) {
+    for (int64_t i = 0; i < n; ++i) y[i] = or_log(x[i]);
+}
+void or_vec_log1p(int64_t n, const double* x, double* y) {
+    for (int64_t i = 0; i < n; ++i) y[i] = or_log1p(x[i]);
 }

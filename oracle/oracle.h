@@ -38,6 +38,10 @@ float or_expf(float x);
 float or_logf(float x);
 double or_log(double x);
 double or_log1p(double x);
+void or_vec_expf(int64_t n, const float* x, float* y);
+void or_vec_logf(int64_t n, const float* x, float* y);
+void or_vec_log(int64_t n, const double* x, double* y);
+void or_vec_log1p(int64_t n, const double* x, double* y);
 
 /* Compressed sparse rows of the citation graph: fwd = papers each paper
  * cites, bwd = papers citing each paper; every row sorted ascending. */

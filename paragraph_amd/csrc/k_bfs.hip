@@ -1,25 +1,34 @@
-// k_bfs.hip -- 2-hop neighbourhood of a root paper and the neighbourhood
-// draws of PA (phases 2 and 3 of a new paper).
+// k_bfs.hip -- 2-hop neighbourhoods and the neighbourhood draws of PA
+// (phases 2 and 3 of a new paper).
 //
 // Replaces Stage 1 of the reference (kernelCallStage1 / _warped,
 // src/kernel.cu:17-168, bodies GetOneAndTwoHopNeighborhood[_Warp],
 // src/abm.cu:577-883), which cleared and scanned three whole-graph bitmaps
-// per query (Theta(N) per paper), plus the neighbourhood sampling of
-// Stage 2/3 (src/abm.cu:887-956) and the shortfall fill of Stage 4
-// (src/abm.cu:958-994).
+// per query (Theta(N) per paper), the neighbourhood sampling of Stage 2/3
+// (src/abm.cu:887-956) and the shortfall fill of Stage 4 (src/abm.cu:958-994).
 //
 // Semantics (identical to the oracle): F1 = (fwd(root) U bwd(root)) \ {root};
 // F1c = its `cap` smallest ids; F2 = (U_{v in F1c} fwd(v) U bwd(v)) \ F1 \
 // {root}; S = the `cap` smallest ids of F1c U F2, ascending.
 //
-// One 256-thread workgroup per query.  Per-query work is proportional to the
-// adjacency it touches, not to N: the candidate ids of each level are
-// gathered into a 4096-entry LDS window, bitonic-sorted and deduplicated in
-// LDS.  When a level has more than 4096 candidate entries, the id range is
-// cut into windows [lo,hi) chosen by per-row binary search so that each
-// window fits, and windows are processed in ascending order until the cap is
-// reached.  Rows of the CSR are sorted, so every row's part in a window is a
-// contiguous slice.
+// The neighbourhood depends only on the root and on the graph, which is
+// frozen for the year, and PA roots concentrate on hubs (late in a 10-year
+// run thousands of papers share one root).  So the year's roots are sorted
+// and deduplicated, S and its CDF are built once per distinct root
+// (k_bfs_root), and each paper then draws from its root's S (k_nb_draw).
+//
+// k_bfs_root, one 512-thread workgroup per distinct root: each level's
+// candidate ids are the rows of the CSR (sorted); the id range is swept in
+// ascending windows [lo,hi) sized so that a window holds at most 4096 row
+// entries, every row keeps a cursor in LDS, a row's part of a window is found
+// by galloping search, long parts are copied by whole wavefronts, and the
+// window is bitonic-sorted, deduplicated and filtered in LDS, until `cap`
+// ids are found.  Per-query work follows the adjacency touched below the
+// final threshold, not N.
+//
+// The neighbourhood CDF is exact 64-bit fixed point (weights scaled so the
+// largest maps below 2^50), built by an integer block scan, so it is
+// bit-identical to the oracle's sequential loop.
 #include <climits>
 
 #include "pg_internal.h"
@@ -27,27 +36,31 @@
 namespace pg {
 namespace {
 
-constexpr int kBT = 256;             // threads per query
-constexpr int kWin = 4096;           // LDS window entries
+constexpr int kBT = 512;                    // threads per distinct root
+constexpr int kWaves = kBT / 64;
+constexpr int kWin = 4096;                  // LDS window entries
+constexpr int kMaxRows = 1 + 2 * kBfsCapMax;
+constexpr int kLongSeg = 24;                // row parts copied by a whole wave
+constexpr int kMaxSegs = kWin / kLongSeg + 1;
 
 struct BfsLds {
-    // big[0, kBfsCapMax): 1-hop list; big[kBfsCapMax, 2*kBfsCapMax): window.
-    // After the BFS the whole array is reused as double[kBfsCapMax] (the
-    // neighbourhood CDF) with the gathered FP32 weights in its upper half.
-    int32_t big[2 * kBfsCapMax];
-    int32_t S[kBfsCapMax];
-    int32_t cited[kMaxCites];
-    int64_t red[kBT / 64];
-    int32_t iscan[kBT / 64];
-    int64_t tot[3];      // candidate entries per level (first window)
-    int32_t nout1;       // |F1c| of the current query
+    int32_t f1[kBfsCapMax];       // capped 1-hop list (ascending)
+    int32_t win[kWin];            // window
+    int32_t cursor[kMaxRows];     // per-row position
+    int32_t seg_row[kMaxSegs];
+    int32_t seg_beg[kMaxSegs];
+    int32_t seg_len[kMaxSegs];
+    int64_t red[kWaves];
+    int64_t tot[3];               // row entries per level
+    int32_t nseg;
     int32_t fill;
-    int32_t lo, hi, nout;
-    int32_t flag;
+    int32_t nout;
+    int32_t n1c;
+    int64_t consumed;
 };
 
-// One LDS object per workgroup; every helper below addresses it directly so
-// the compiler keeps LDS address-space accesses (ds_*) throughout.
+// One LDS object per workgroup; helpers address it directly so the
+// compiler keeps LDS (ds_*) accesses.
 __shared__ BfsLds L;
 
 __device__ __forceinline__ int lower_bound_i32(const int32_t* p, int b, int e, int32_t x) {
@@ -58,35 +71,48 @@ __device__ __forceinline__ int lower_bound_i32(const int32_t* p, int b, int e, i
     return b;
 }
 
+// First index in [b,e) with p[idx] >= x, searching forward from b.
+__device__ __forceinline__ int gallop_lower_bound(const int32_t* p, int b, int e, int32_t x) {
+    if (b >= e || p[b] >= x) return b;
+    int step = 1, lo = b;  // p[lo] < x
+    while (true) {
+        int nx = lo + step;
+        if (nx >= e) return lower_bound_i32(p, lo + 1, e, x);
+        if (p[nx] >= x) return lower_bound_i32(p, lo + 1, nx, x);
+        lo = nx;
+        step <<= 1;
+    }
+}
+
 __device__ __forceinline__ bool row_contains(const int32_t* p, int b, int e, int32_t x) {
     int k = lower_bound_i32(p, b, e, x);
     return k < e && p[k] == x;
 }
 
-__device__ int64_t block_sum_i64(int64_t v) {
+__device__ int64_t block_sum(int64_t v) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     __syncthreads();
     if (lane == 0) L.red[wid] = v;
     __syncthreads();
     int64_t t = 0;
-    for (int k = 0; k < kBT / 64; ++k) t += L.red[k];
+    for (int k = 0; k < kWaves; ++k) t += L.red[k];
     return t;
 }
 
-// Exclusive scan of one int per thread (block-wide); *total gets the sum.
-__device__ int block_excl_scan_i32(int v, int* total) {
+// Block-wide exclusive scan of one int64 per thread; *total gets the sum.
+__device__ int64_t block_excl_scan(int64_t v, int64_t* total) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    int x = v;
+    int64_t x = v;
     for (int off = 1; off < 64; off <<= 1) {
-        int y = __shfl_up(x, off, 64);
+        int64_t y = __shfl_up(x, off, 64);
         if (lane >= off) x += y;
     }
     __syncthreads();
-    if (lane == 63) L.iscan[wid] = x;
+    if (lane == 63) L.red[wid] = x;
     __syncthreads();
-    int pre = 0, tot = 0;
-    for (int k = 0; k < kBT / 64; ++k) { if (k < wid) pre += L.iscan[k]; tot += L.iscan[k]; }
+    int64_t pre = 0, tot = 0;
+    for (int k = 0; k < kWaves; ++k) { if (k < wid) pre += L.red[k]; tot += L.red[k]; }
     *total = tot;
     return pre + x - v;
 }
@@ -118,35 +144,20 @@ struct Query {
 
 __device__ __forceinline__ int num_rows(const Query& q) { return q.level == 1 ? 2 : 1 + 2 * q.n1c; }
 
-__device__ __forceinline__ void row_of(const Query& q, const int32_t* f1, int r, const int32_t*& p,
-                                       int& b, int& e) {
+__device__ __forceinline__ void row_of(const Query& q, int r, const int32_t*& p, int& b, int& e) {
     int32_t v;
     int d;
     if (q.level == 1) {
         v = q.root; d = r;
     } else if (r == 0) {
-        p = f1; b = 0; e = q.n1c; return;
+        p = L.f1; b = 0; e = q.n1c; return;
     } else {
-        v = f1[(r - 1) >> 1]; d = (r - 1) & 1;
+        v = L.f1[(r - 1) >> 1]; d = (r - 1) & 1;
     }
     const int32_t* off = d ? q.g.bwd_off : q.g.fwd_off;
     p = d ? q.g.bwd_adj : q.g.fwd_adj;
     b = off[v];
     e = off[v + 1];
-}
-
-// Number of row entries with id in [lo, hi) (hi == INT_MAX: no upper bound).
-__device__ int64_t count_range(const Query& q, const int32_t* f1, int32_t lo, int32_t hi) {
-    int64_t acc = 0;
-    const int nr = num_rows(q);
-    for (int r = threadIdx.x; r < nr; r += kBT) {
-        const int32_t* p; int b, e;
-        row_of(q, f1, r, p, b, e);
-        int a = lo > 0 ? lower_bound_i32(p, b, e, lo) : b;
-        int z = hi != INT_MAX ? lower_bound_i32(p, a, e, hi) : e;
-        acc += z - a;
-    }
-    return block_sum_i64(acc);
 }
 
 __device__ __forceinline__ bool keep_id(const Query& q, int32_t x) {
@@ -159,232 +170,215 @@ __device__ __forceinline__ bool keep_id(const Query& q, int32_t x) {
     return true;
 }
 
-// Collects the `cap` smallest distinct kept ids of the level's rows into
-// L.S (to_s) or the 1-hop list L.big (ascending).  Kept out of line: inlining
-// it into both kernels crashes the ROCm 7.2 optimiser.  Returns the count.  All threads of the block call it.
-__device__ __attribute__((noinline)) int collect(const Query& q, bool to_s, int cap) {
-    const int32_t n = (int32_t)q.g.n;
-    const int32_t* f1 = L.big;
-    int32_t* win = L.big + kBfsCapMax;
-    int32_t* out = to_s ? L.S : L.big;
-    if (threadIdx.x == 0) { L.lo = 0; L.nout = 0; L.tot[q.level] = 0; }
+// Sort the window, drop duplicates and filtered ids, append to out[] (up to
+// cap).  All threads call it.
+__device__ void flush_window(const Query& q, int nw, int32_t* out, int cap) {
+    int P = 1;
+    while (P < nw) P <<= 1;
+    for (int k = nw + threadIdx.x; k < P; k += kBT) L.win[k] = INT_MAX;
     __syncthreads();
-    for (;;) {
-        const int32_t lo = L.lo;
-        const int64_t rem = count_range(q, f1, lo, INT_MAX);
-        if (lo == 0 && threadIdx.x == 0) L.tot[q.level] = rem;
-        if (rem == 0) break;
-        int32_t hi;
-        if (rem <= kWin) {
-            hi = INT_MAX;
-        } else {
-            int32_t a = lo + 1, z = n;  // largest a with count(lo, a) <= kWin
-            if (count_range(q, f1, lo, a) > kWin) {
-                hi = a;                  // single id window (truncated below)
-            } else {
-                while (z - a > 1) {
-                    int32_t mid = a + (z - a) / 2;
-                    if (count_range(q, f1, lo, mid) <= kWin) a = mid; else z = mid;
-                }
-                hi = a;
-            }
+    if (nw > 1) bitonic_sort(L.win, P);
+    const int chunk = (nw + kBT - 1) / kBT;
+    const int s0 = threadIdx.x * chunk;
+    const int s1 = min(nw, s0 + chunk);
+    int mine = 0;
+    for (int k = s0; k < s1; ++k) {
+        int32_t x = L.win[k];
+        if ((k == 0 || L.win[k - 1] != x) && keep_id(q, x)) ++mine;
+    }
+    int64_t tot;
+    int64_t pos = block_excl_scan(mine, &tot) + L.nout;
+    for (int k = s0; k < s1; ++k) {
+        int32_t x = L.win[k];
+        if ((k == 0 || L.win[k - 1] != x) && keep_id(q, x)) {
+            if (pos < cap) out[pos] = x;
+            ++pos;
         }
-        // gather [lo, hi) into the window
-        if (threadIdx.x == 0) L.fill = 0;
-        __syncthreads();
-        {
-            const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-            const int nr = num_rows(q);
-            for (int r = wid; r < nr; r += kBT / 64) {
-                const int32_t* p; int b, e;
-                row_of(q, f1, r, p, b, e);
-                int a = lo > 0 ? lower_bound_i32(p, b, e, lo) : b;
-                int z = hi != INT_MAX ? lower_bound_i32(p, a, e, hi) : e;
-                int len = z - a;
-                if (len <= 0) continue;
-                int base = 0;
-                if (lane == 0) base = atomicAdd(&L.fill, len);
-                base = __shfl(base, 0, 64);
-                for (int k = lane; k < len; k += 64) {
-                    int pos = base + k;
-                    if (pos < kWin) win[pos] = p[a + k];
-                }
-            }
-        }
-        __syncthreads();
-        const int nw = L.fill < kWin ? L.fill : kWin;
-        int P = 1;
-        while (P < nw) P <<= 1;
-        for (int k = nw + threadIdx.x; k < P; k += kBT) win[k] = INT_MAX;
-        __syncthreads();
-        if (nw > 1) bitonic_sort(win, P);
-        // unique + filter, ordered compaction
-        const int chunk = (nw + kBT - 1) / kBT;
-        const int s0 = threadIdx.x * chunk;
-        const int s1 = min(nw, s0 + chunk);
-        int mine = 0;
-        for (int k = s0; k < s1; ++k) {
-            int32_t x = win[k];
-            if ((k == 0 || win[k - 1] != x) && keep_id(q, x)) ++mine;
-        }
-        int tot;
-        int pos = block_excl_scan_i32(mine, &tot) + L.nout;
-        for (int k = s0; k < s1; ++k) {
-            int32_t x = win[k];
-            if ((k == 0 || win[k - 1] != x) && keep_id(q, x)) {
-                if (pos < cap) out[pos] = x;
-                ++pos;
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int no = L.nout + tot;
-            L.nout = no < cap ? no : cap;
-            L.lo = hi;
-        }
-        __syncthreads();
-        if (L.nout >= cap || hi == INT_MAX || hi >= n) break;
     }
     __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t no = L.nout + tot;
+        L.nout = (int32_t)(no < cap ? no : cap);
+    }
+    __syncthreads();
+}
+
+// Collects the `cap` smallest distinct kept ids of the level's rows into
+// out[] (ascending); returns the count.  All threads call it.  Kept out of
+// line: inlining it into the kernels crashes the ROCm 7.2 optimiser.
+__device__ __attribute__((noinline)) int collect(const Query& q, int32_t* out, int cap) {
+    const int32_t n = (int32_t)q.g.n;
+    const int nr = num_rows(q);
+    int64_t mine = 0;
+    for (int r = threadIdx.x; r < nr; r += kBT) {
+        const int32_t* p; int b, e;
+        row_of(q, r, p, b, e);
+        L.cursor[r] = b;
+        mine += e - b;
+    }
+    const int64_t total = block_sum(mine);
+    if (threadIdx.x == 0) { L.nout = 0; L.tot[q.level] = total; L.consumed = 0; }
+    __syncthreads();
+    if (total == 0) return 0;
+    if (total <= kWin) {
+        // everything fits in one window
+        if (threadIdx.x == 0) L.fill = 0;
+        __syncthreads();
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        for (int r = wid; r < nr; r += kWaves) {
+            const int32_t* p; int b, e;
+            row_of(q, r, p, b, e);
+            int len = e - b;
+            if (len <= 0) continue;
+            int base = 0;
+            if (lane == 0) base = atomicAdd(&L.fill, len);
+            base = __shfl(base, 0, 64);
+            for (int k = lane; k < len; k += 64) L.win[base + k] = p[b + k];
+        }
+        __syncthreads();
+        flush_window(q, (int)total, out, cap);
+        return L.nout;
+    }
+    // Sweep ascending id windows; delta starts from a uniform-density guess
+    // and adapts (halved on overflow, doubled when a window is sparse).
+    int64_t delta = (int64_t)(kWin / 4) * n / total;
+    if (delta < 1) delta = 1;
+    int32_t lo = 0;
+    for (;;) {
+        const int32_t hi = (lo + delta >= n) ? INT_MAX : (int32_t)(lo + delta);
+        // count entries in [lo, hi) without moving the cursors
+        int64_t cnt = 0;
+        for (int r = threadIdx.x; r < nr; r += kBT) {
+            const int32_t* p; int b, e;
+            row_of(q, r, p, b, e);
+            const int c = L.cursor[r];
+            const int z = hi == INT_MAX ? e : gallop_lower_bound(p, c, e, hi);
+            cnt += z - c;
+        }
+        const int64_t wtot = block_sum(cnt);
+        if (wtot > kWin && (hi == INT_MAX || hi - lo > 1)) {
+            delta = (hi == INT_MAX ? (int64_t)n - lo : delta) / 2;
+            if (delta < 1) delta = 1;
+            continue;
+        }
+        // gather (a single-id window may hold more than kWin copies of one
+        // id; the extra copies are dropped, which is harmless)
+        if (threadIdx.x == 0) { L.fill = 0; L.nseg = 0; }
+        __syncthreads();
+        for (int r = threadIdx.x; r < nr; r += kBT) {
+            const int32_t* p; int b, e;
+            row_of(q, r, p, b, e);
+            const int c = L.cursor[r];
+            const int z = hi == INT_MAX ? e : gallop_lower_bound(p, c, e, hi);
+            const int len = z - c;
+            if (len > 0) {
+                const int base = atomicAdd(&L.fill, len);
+                if (len >= kLongSeg && base < kWin) {
+                    const int s = atomicAdd(&L.nseg, 1);
+                    L.seg_row[s] = r; L.seg_beg[s] = c; L.seg_len[s] = base;
+                } else {
+                    for (int k = 0; k < len; ++k)
+                        if (base + k < kWin) L.win[base + k] = p[c + k];
+                }
+            }
+            L.cursor[r] = z;
+        }
+        __syncthreads();
+        {
+            // long row parts: one wavefront each, lanes copy in parallel
+            const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+            const int nseg = L.nseg;
+            for (int s = wid; s < nseg; s += kWaves) {
+                const int r = L.seg_row[s];
+                const int c = L.seg_beg[s];
+                const int base = L.seg_len[s];
+                const int32_t* p; int b, e;
+                row_of(q, r, p, b, e);
+                const int z = L.cursor[r];
+                for (int k = lane; k < z - c; k += 64)
+                    if (base + k < kWin) L.win[base + k] = p[c + k];
+            }
+        }
+        __syncthreads();
+        const int fill = L.fill;
+        const int nw = fill < kWin ? fill : kWin;
+        flush_window(q, nw, out, cap);
+        if (threadIdx.x == 0) L.consumed += fill;
+        __syncthreads();
+        if (L.nout >= cap || hi == INT_MAX || L.consumed >= total) break;
+        lo = hi;
+        if (fill < kWin / 4) delta *= 2;
+    }
     return L.nout;
 }
 
-// Builds S for `root` into L.S; returns |S|.
-__device__ int two_hop(const CsrView& g, int32_t root, int cap) {
-    int32_t* f1 = L.big;
+// Builds S for `root` into out[] (global); returns |S|.
+__device__ int two_hop(const CsrView& g, int32_t root, int cap, int32_t* out) {
     Query q{g, root, 1, 0, false, 0};
-    const int n1c = collect(q, false, cap);
-    if (threadIdx.x == 0) L.nout1 = n1c;
+    const int n1c = collect(q, L.f1, cap);
     q.level = 2;
     q.n1c = n1c;
-    q.check_f1 = false;
-    q.f1max = 0;
     if (n1c == cap) {
-        q.f1max = f1[cap - 1];
+        q.f1max = L.f1[cap - 1];
         const int32_t* fo = g.fwd_off; const int32_t* bo = g.bwd_off;
         int32_t mx = -1;
         if (fo[root + 1] > fo[root]) mx = max(mx, g.fwd_adj[fo[root + 1] - 1]);
         if (bo[root + 1] > bo[root]) mx = max(mx, g.bwd_adj[bo[root + 1] - 1]);
         q.check_f1 = mx > q.f1max;
     }
-    return collect(q, true, cap);
-}
-
-__device__ __forceinline__ bool in_cited(const int32_t* c, int n, int32_t x) {
-    for (int j = 0; j < n; ++j) if (c[j] == x) return true;
-    return false;
-}
-
-__global__ __launch_bounds__(kBT) void k_bfs_nb(CsrView g, const float* __restrict__ w,
-                                                const double* __restrict__ cdf, int m, int n_nb,
-                                                int cap, uint64_t seed, uint32_t yi,
-                                                int32_t* __restrict__ cit, int8_t* __restrict__ kind,
-                                                int32_t* __restrict__ cnt, int32_t* __restrict__ near,
-                                                int32_t* __restrict__ anext,
-                                                unsigned long long* __restrict__ bytes_out) {
-    const int64_t i = blockIdx.x;
-    const int64_t N = g.n;
-    const uint32_t gid = (uint32_t)(N + i);
-    const int c0 = cnt[i];
-    if (c0 <= 0 || n_nb <= 0) return;  // uniform across the block
-    for (int j = threadIdx.x; j < c0; j += kBT) L.cited[j] = cit[i * m + j];
+    if (threadIdx.x == 0) L.n1c = n1c;
     __syncthreads();
-    const int ns = two_hop(g, L.cited[0], cap);
+    return collect(q, out, cap);
+}
+
+__device__ __forceinline__ double pow2d(int k) { return u2d((uint64_t)(1023 + k) << 52); }
+
+// Fixed-point image of a weight given the neighbourhood's scale.
+__device__ __forceinline__ uint64_t nb_fixed(float w, double scale) {
+    double q = (double)w * scale;
+    return q < 0x1p50 ? (uint64_t)q : (uint64_t)1 << 50;
+}
+
+// k_bfs_root: S and its fixed-point CDF for each distinct root.
+__global__ __launch_bounds__(kBT) void k_bfs_root(CsrView g, const float* __restrict__ w,
+                                                  const int32_t* __restrict__ uroot,
+                                                  const int64_t* __restrict__ n_unique_ptr, int cap,
+                                                  int32_t* __restrict__ S_buf,
+                                                  uint64_t* __restrict__ Q_buf,
+                                                  int32_t* __restrict__ ns_buf,
+                                                  unsigned long long* __restrict__ bytes_out) {
+    const int64_t u = blockIdx.x;
+    if (u >= *n_unique_ptr) return;
+    const int32_t root = uroot[u];
+    int32_t* S = S_buf + u * cap;
+    const int ns = two_hop(g, root, cap, S);
     if (threadIdx.x == 0) {
-        // algorithmic bytes: offsets + adjacency of the root and of each
-        // capped 1-hop paper, plus the output ids
-        const int64_t b = 16 * (1 + (int64_t)L.nout1) + 4 * (L.tot[1] + L.tot[2] - L.nout1) +
+        ns_buf[u] = ns;
+        const int64_t b = 16 * (1 + (int64_t)L.n1c) + 4 * (L.tot[1] + L.tot[2] - L.n1c) +
                           4 * (int64_t)ns;
         atomicAdd(bytes_out, (unsigned long long)b);
     }
-
-    // available = S \ cited
-    int mine = 0;
-    for (int s = threadIdx.x; s < ns; s += kBT) mine += in_cited(L.cited, c0, L.S[s]) ? 0 : 1;
-    const int64_t avail = block_sum_i64(mine);
-    int c = c0;
-    if (avail <= n_nb) {
-        // take every available neighbour, ascending
-        const int chunk = (ns + kBT - 1) / kBT;
-        const int s0 = threadIdx.x * chunk, s1 = min(ns, s0 + chunk);
-        int cntm = 0;
-        for (int s = s0; s < s1; ++s) cntm += in_cited(L.cited, c0, L.S[s]) ? 0 : 1;
-        int tot;
-        int pos = block_excl_scan_i32(cntm, &tot) + c0;
-        for (int s = s0; s < s1; ++s) {
-            int32_t x = L.S[s];
-            if (!in_cited(L.cited, c0, x)) {
-                cit[i * m + pos] = x;
-                kind[i * m + pos] = 1;
-                ++pos;
-            }
-        }
-        __syncthreads();
-        for (int s = s0, p2 = pos - cntm; s < s1; ++s) {
-            int32_t x = L.S[s];
-            if (!in_cited(L.cited, c0, x)) L.cited[p2++] = x;
-        }
-        __syncthreads();
-        c = c0 + tot;
-    } else {
-        float* wv = reinterpret_cast<float*>(L.big + kBfsCapMax);
-        double* D = reinterpret_cast<double*>(L.big);
-        for (int s = threadIdx.x; s < ns; s += kBT) wv[s] = w[L.S[s]];
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            // Sequential FP64 prefix in ascending id order (in place: D[s]
-            // only overwrites weights that have already been read).
-            double acc = 0.0;
-            for (int s = 0; s < ns; ++s) { acc += (double)wv[s]; D[s] = acc; }
-            const double Zs = acc;
-            const int b_max = kAttemptsPerCite * n_nb;
-            int got = 0;
-            for (int bt = 0; got < n_nb && bt < b_max; ++bt) {
-                u32x4 r = philox(gid, (uint32_t)bt, yi, TAG_NB, seed);
-                double t = u53(r.x, r.y) * Zs;
-                int lo = 0, hi = ns;
-                while (lo < hi) {
-                    int mid = (lo + hi) >> 1;
-                    if (D[mid] > t) hi = mid; else lo = mid + 1;
-                }
-                if (lo >= ns) lo = ns - 1;
-                int32_t x = L.S[lo];
-                if (!in_cited(L.cited, c, x)) {
-                    L.cited[c] = x;
-                    cit[i * m + c] = x;
-                    kind[i * m + c] = 1;
-                    ++c;
-                    ++got;
-                }
-            }
-            L.fill = c;
-        }
-        __syncthreads();
-        c = L.fill;
-    }
-    if (threadIdx.x == 0) {
-        // refill with PA draws, continuing the attempt sequence
-        const double Z = cdf[N - 1];
-        const double eps = kNearRel * Z;
-        const int a_max = kAttemptsPerCite * m;
-        int a = anext[i];
-        int nr = 0;
-        while (c < m && a < a_max) {
-            u32x4 r = philox(gid, (uint32_t)a, yi, TAG_PA, seed);
-            double t = u53(r.x, r.y) * Z;
-            int64_t k = cdf_upper_bound(cdf, N, t);
-            if (cdf_near(cdf, N, k, t, eps)) ++nr;
-            ++a;
-            if (!in_cited(L.cited, c, (int32_t)k)) {
-                L.cited[c] = (int32_t)k;
-                cit[i * m + c] = (int32_t)k;
-                kind[i * m + c] = 2;
-                ++c;
-            }
-        }
-        cnt[i] = c;
-        near[i] += nr;
-        anext[i] = a;
+    __syncthreads();
+    // fixed-point CDF: scale from the largest weight's exponent field
+    const int per = (ns + kBT - 1) / kBT;
+    const int s0 = threadIdx.x * per, s1 = min(ns, s0 + per);
+    uint32_t mb = 0;
+    for (int s = s0; s < s1; ++s) mb = max(mb, f2u(w[S[s]]));
+    for (int off = 32; off > 0; off >>= 1) mb = max(mb, (uint32_t)__shfl_down((int)mb, off, 64));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) L.win[threadIdx.x >> 6] = (int32_t)mb;
+    __syncthreads();
+    uint32_t gmb = 0;
+    for (int k = 0; k < kWaves; ++k) gmb = max(gmb, (uint32_t)L.win[k]);
+    const double scale = pow2d(49 - ((int)(gmb >> 23) - 127));
+    int64_t acc = 0;
+    for (int s = s0; s < s1; ++s) acc += (int64_t)nb_fixed(w[S[s]], scale);
+    int64_t tot;
+    int64_t run = block_excl_scan(acc, &tot);
+    uint64_t* Q = Q_buf + u * cap;
+    for (int s = s0; s < s1; ++s) {
+        run += (int64_t)nb_fixed(w[S[s]], scale);
+        Q[s] = (uint64_t)run;
     }
 }
 
@@ -394,20 +388,156 @@ __global__ __launch_bounds__(kBT) void k_bfs_only(CsrView g, const int32_t* __re
     const int64_t i = blockIdx.x;
     const int32_t root = roots[i];
     int ns = 0;
-    if (root >= 0 && root < g.n) ns = two_hop(g, root, cap);
-    for (int s = threadIdx.x; s < ns; s += kBT) out[i * cap + s] = L.S[s];
+    if (root >= 0 && root < g.n) ns = two_hop(g, root, cap, out + i * cap);
     if (threadIdx.x == 0) out_len[i] = ns;
+}
+
+// ---- root deduplication ----
+
+__global__ void k_root_keys(int64_t n_exist, int64_t n_new, int m, const int32_t* __restrict__ cit,
+                            const int32_t* __restrict__ cnt, uint32_t* __restrict__ keys,
+                            uint32_t* __restrict__ vals) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_new) return;
+    keys[i] = cnt[i] > 0 ? (uint32_t)cit[i * m] : (uint32_t)n_exist;
+    vals[i] = (uint32_t)i;
+}
+
+__global__ void k_root_flags(int64_t n_exist, int64_t n, const uint32_t* __restrict__ keys,
+                             int32_t* __restrict__ flags) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t k = keys[j];
+    flags[j] = (k < (uint32_t)n_exist && (j == 0 || keys[j - 1] != k)) ? 1 : 0;
+}
+
+__global__ void k_root_assign(int64_t n_exist, int64_t n, const uint32_t* __restrict__ keys,
+                              const uint32_t* __restrict__ vals, const int32_t* __restrict__ flags,
+                              const int64_t* __restrict__ excl, int32_t* __restrict__ root_slot,
+                              int32_t* __restrict__ uroot) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t k = keys[j];
+    if (k >= (uint32_t)n_exist) { root_slot[vals[j]] = -1; return; }
+    const int64_t uid = excl[j] + flags[j] - 1;
+    root_slot[vals[j]] = (int32_t)uid;
+    if (flags[j]) uroot[uid] = (int32_t)k;
+}
+
+// ---- per-paper neighbourhood draws and refill ----
+
+__device__ __forceinline__ bool row_has(const int32_t* row, int n, int32_t x) {
+    for (int j = 0; j < n; ++j) if (row[j] == x) return true;
+    return false;
+}
+
+__global__ __launch_bounds__(256) void k_nb_draw(int64_t n_exist, int64_t n_new, int m, int n_nb,
+                                                 int cap, uint64_t seed, uint32_t yi,
+                                                 const double* __restrict__ cdf,
+                                                 const int32_t* __restrict__ root_slot,
+                                                 const int32_t* __restrict__ S_buf,
+                                                 const uint64_t* __restrict__ Q_buf,
+                                                 const int32_t* __restrict__ ns_buf,
+                                                 int32_t* __restrict__ cit, int8_t* __restrict__ kind,
+                                                 int32_t* __restrict__ cnt, int32_t* __restrict__ near,
+                                                 int32_t* __restrict__ anext) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_new) return;
+    const int c0 = cnt[i];
+    if (c0 <= 0) return;
+    const uint32_t gid = (uint32_t)(n_exist + i);
+    int32_t* row = cit + i * m;
+    int8_t* kr = kind + i * m;
+    const int64_t u = root_slot[i];
+    const int ns = ns_buf[u];
+    const int32_t* S = S_buf + u * cap;
+    const uint64_t* Q = Q_buf + u * cap;
+    // cited papers inside S (binary search each cited id in S)
+    int inS = 0;
+    for (int j = 0; j < c0; ++j) {
+        const int32_t x = row[j];
+        int lo = 0, hi = ns;
+        while (lo < hi) { int mid = (lo + hi) >> 1; if (S[mid] < x) lo = mid + 1; else hi = mid; }
+        inS += (lo < ns && S[lo] == x) ? 1 : 0;
+    }
+    int c = c0;
+    const int avail = ns - inS;
+    if (avail <= n_nb) {
+        for (int s = 0; s < ns; ++s) {
+            const int32_t x = S[s];
+            if (!row_has(row, c0, x)) { row[c] = x; kr[c] = 1; ++c; }
+        }
+    } else {
+        const uint64_t Z = Q[ns - 1];
+        const int b_max = kAttemptsPerCite * n_nb;
+        int got = 0;
+        for (int bt = 0; got < n_nb && bt < b_max; ++bt) {
+            u32x4 r = philox(gid, (uint32_t)bt, yi, TAG_NB, seed);
+            const uint64_t v = ((uint64_t)r.x << 21) ^ (uint64_t)(r.y >> 11);
+            const uint64_t t = (__umul64hi(v, Z) << 11) | ((v * Z) >> 53);
+            int lo = 0, hi = ns;
+            while (lo < hi) { int mid = (lo + hi) >> 1; if (Q[mid] > t) hi = mid; else lo = mid + 1; }
+            if (lo >= ns) lo = ns - 1;
+            const int32_t x = S[lo];
+            if (!row_has(row, c, x)) { row[c] = x; kr[c] = 1; ++c; ++got; }
+        }
+    }
+    // refill with PA draws, continuing the attempt sequence
+    const double Zg = cdf[n_exist - 1];
+    const double eps = kNearRel * Zg;
+    const int a_max = kAttemptsPerCite * m;
+    int a = anext[i];
+    int nr = 0;
+    while (c < m && a < a_max) {
+        u32x4 r = philox(gid, (uint32_t)a, yi, TAG_PA, seed);
+        const double t = u53(r.x, r.y) * Zg;
+        const int64_t k = cdf_upper_bound(cdf, n_exist, t);
+        if (cdf_near(cdf, n_exist, k, t, eps)) ++nr;
+        ++a;
+        if (!row_has(row, c, (int32_t)k)) { row[c] = (int32_t)k; kr[c] = 2; ++c; }
+    }
+    cnt[i] = c;
+    near[i] += nr;
+    anext[i] = a;
 }
 
 }  // namespace
 
-void launch_bfs_nb(hipStream_t s, const CsrView& g, const float* w, const double* cdf, int64_t n_new,
-                   int m, int n_nb, int cap, uint64_t seed, uint32_t year_index, int32_t* cit,
-                   int8_t* kind, int32_t* cnt, int32_t* near, int32_t* anext,
-                   unsigned long long* bytes_out) {
+void launch_root_dedup(hipStream_t s, int64_t n_exist, int64_t n_new, int m, const int32_t* cit,
+                       const int32_t* cnt, uint32_t* keys, uint32_t* vals, uint32_t* k_sorted,
+                       uint32_t* v_sorted, uint32_t* k_tmp, uint32_t* v_tmp, int32_t* flags,
+                       int64_t* excl, int32_t* root_slot, int32_t* uroot, int key_bits,
+                       int64_t* scratch) {
+    if (n_new <= 0) return;
+    const int64_t nb = (n_new + 255) / 256;
+    hipLaunchKernelGGL(k_root_keys, dim3((uint32_t)nb), dim3(256), 0, s, n_exist, n_new, m, cit, cnt,
+                       keys, vals);
+    launch_radix_sort_pairs(s, keys, vals, k_sorted, v_sorted, k_tmp, v_tmp, n_new, key_bits,
+                            scratch);
+    hipLaunchKernelGGL(k_root_flags, dim3((uint32_t)nb), dim3(256), 0, s, n_exist, n_new, k_sorted,
+                       flags);
+    launch_excl_scan_i32(s, flags, n_new, excl, scratch);
+    hipLaunchKernelGGL(k_root_assign, dim3((uint32_t)nb), dim3(256), 0, s, n_exist, n_new, k_sorted,
+                       v_sorted, flags, excl, root_slot, uroot);
+}
+
+void launch_bfs_root(hipStream_t s, const CsrView& g, const float* w, const int32_t* uroot,
+                     const int64_t* n_unique, int64_t max_unique, int cap, int32_t* S_buf,
+                     uint64_t* Q_buf, int32_t* ns_buf, unsigned long long* bytes_out) {
+    if (max_unique <= 0) return;
+    hipLaunchKernelGGL(k_bfs_root, dim3((uint32_t)max_unique), dim3(kBT), 0, s, g, w, uroot,
+                       n_unique, cap, S_buf, Q_buf, ns_buf, bytes_out);
+}
+
+void launch_nb_draw(hipStream_t s, int64_t n_exist, int64_t n_new, int m, int n_nb, int cap,
+                    uint64_t seed, uint32_t year_index, const double* cdf, const int32_t* root_slot,
+                    const int32_t* S_buf, const uint64_t* Q_buf, const int32_t* ns_buf,
+                    int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* near, int32_t* anext) {
     if (n_new <= 0 || n_nb <= 0) return;
-    hipLaunchKernelGGL(k_bfs_nb, dim3((uint32_t)n_new), dim3(kBT), 0, s, g, w, cdf, m, n_nb, cap,
-                       seed, year_index, cit, kind, cnt, near, anext, bytes_out);
+    const int64_t nb = (n_new + 255) / 256;
+    hipLaunchKernelGGL(k_nb_draw, dim3((uint32_t)nb), dim3(256), 0, s, n_exist, n_new, m, n_nb, cap,
+                       seed, year_index, cdf, root_slot, S_buf, Q_buf, ns_buf, cit, kind, cnt, near,
+                       anext);
 }
 
 void launch_bfs_only(hipStream_t s, const CsrView& g, int64_t count, const int32_t* roots, int cap,

@@ -81,6 +81,15 @@ struct pg_ctx {
     int32_t* s_fwd_adj = nullptr;
     int32_t* s_bwd_off = nullptr;
     int32_t* s_bwd_adj = nullptr;
+    // neighbourhoods of the year's distinct roots
+    int32_t* S_buf = nullptr;
+    uint64_t* Q_buf = nullptr;
+    int32_t* ns_buf = nullptr;
+    uint32_t* rk[6] = {};          // root keys/vals: in, sorted, tmp
+    int32_t* r_flags = nullptr;
+    int64_t* r_excl = nullptr;
+    int32_t* root_slot = nullptr;
+    int32_t* uroot = nullptr;
 };
 
 namespace {
@@ -114,7 +123,9 @@ void release(pg_ctx* c) {
                     c->near, c->anext, c->new_off, c->add, c->src_of, c->k_sorted,
                     c->v_sorted, c->k_tmp, c->v_tmp, c->scratch, c->d_sum, c->d_tmp_i32,
                     c->s_birth, c->s_in_deg, c->s_fwd_off, c->s_fwd_adj, c->s_bwd_off,
-                    c->s_bwd_adj};
+                    c->s_bwd_adj, c->S_buf, c->Q_buf, c->ns_buf, c->rk[0], c->rk[1],
+                    c->rk[2], c->rk[3], c->rk[4], c->rk[5], c->r_flags, c->r_excl,
+                    c->root_slot, c->uroot};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int k = 0; k < PG_K_COUNT; ++k)
@@ -154,6 +165,23 @@ int key_bits_for(int64_t n) {
     int b = 1;
     while (b < 31 && (int64_t(1) << b) < n) ++b;
     return b;
+}
+
+int ensure_nb_buffers(pg_ctx* ctx) {
+    if (ctx->S_buf) return PG_OK;
+    const int64_t D = std::max<int64_t>(ctx->cfg.max_new_per_year, 1);
+    const int64_t cap = ctx->cfg.bfs_cap;
+    hipError_t r = dalloc(&ctx->S_buf, D * cap);
+    if (r == hipSuccess) r = dalloc(&ctx->Q_buf, D * cap);
+    if (r == hipSuccess) r = dalloc(&ctx->ns_buf, D);
+    for (int k = 0; k < 6 && r == hipSuccess; ++k) r = dalloc(&ctx->rk[k], D);
+    if (r == hipSuccess) r = dalloc(&ctx->r_flags, D);
+    if (r == hipSuccess) r = dalloc(&ctx->r_excl, D + 1);
+    if (r == hipSuccess) r = dalloc(&ctx->root_slot, D);
+    if (r == hipSuccess) r = dalloc(&ctx->uroot, D);
+    if (r != hipSuccess)
+        return fail(ctx, PG_ENOMEM, "neighbourhood buffers: device allocation failed");
+    return PG_OK;
 }
 
 CsrView view(const pg_ctx* c) {
@@ -381,9 +409,19 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
                            ctx->near, ctx->anext);
             pend(PG_K_PA_DRAW);
             if (n_nb > 0) {
+                {
+                    int rc = ensure_nb_buffers(ctx);
+                    if (rc) return rc;
+                }
                 pbeg(PG_K_BFS_NB);
-                launch_bfs_nb(s, view(ctx), ctx->w, ctx->cdf, D, m, n_nb, ctx->cfg.bfs_cap, seed, yi,
-                              ctx->cit, ctx->kind, ctx->cnt, ctx->near, ctx->anext, sums + 2);
+                launch_root_dedup(s, N, D, m, ctx->cit, ctx->cnt, ctx->rk[0], ctx->rk[1], ctx->rk[2],
+                                  ctx->rk[3], ctx->rk[4], ctx->rk[5], ctx->r_flags, ctx->r_excl,
+                                  ctx->root_slot, ctx->uroot, key_bits_for(N + 1), ctx->scratch);
+                launch_bfs_root(s, view(ctx), ctx->w, ctx->uroot, ctx->r_excl + D, D,
+                                ctx->cfg.bfs_cap, ctx->S_buf, ctx->Q_buf, ctx->ns_buf, sums + 2);
+                launch_nb_draw(s, N, D, m, n_nb, ctx->cfg.bfs_cap, seed, yi, ctx->cdf,
+                               ctx->root_slot, ctx->S_buf, ctx->Q_buf, ctx->ns_buf, ctx->cit,
+                               ctx->kind, ctx->cnt, ctx->near, ctx->anext);
                 pend(PG_K_BFS_NB);
             }
             launch_sum_i32(s, ctx->near, D, sums + 0);

@@ -64,12 +64,24 @@ struct CsrView {
     int64_t n;
 };
 
-// PA phases 2-3: 2-hop neighbourhood of the first pick, neighbourhood
-// draws, refill.  One workgroup per new paper.
-void launch_bfs_nb(hipStream_t s, const CsrView& g, const float* w, const double* cdf,
-                   int64_t n_new, int m, int n_nb, int cap, uint64_t seed, uint32_t year_index,
-                   int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* near, int32_t* anext,
-                   unsigned long long* bytes_out);
+// PA phases 2-3.  Roots (first PA pick of each paper) are sorted and
+// deduplicated: root_slot[i] = distinct-root index of paper i (-1 if none),
+// uroot[u] = root id, n_unique = excl[n_new].
+void launch_root_dedup(hipStream_t s, int64_t n_exist, int64_t n_new, int m, const int32_t* cit,
+                       const int32_t* cnt, uint32_t* keys, uint32_t* vals, uint32_t* k_sorted,
+                       uint32_t* v_sorted, uint32_t* k_tmp, uint32_t* v_tmp, int32_t* flags,
+                       int64_t* excl, int32_t* root_slot, int32_t* uroot, int key_bits,
+                       int64_t* scratch);
+// 2-hop neighbourhood S (S_buf[u*cap...], ns_buf[u]) and its fixed-point
+// CDF (Q_buf[u*cap...]) of every distinct root; grid max_unique blocks.
+void launch_bfs_root(hipStream_t s, const CsrView& g, const float* w, const int32_t* uroot,
+                     const int64_t* n_unique, int64_t max_unique, int cap, int32_t* S_buf,
+                     uint64_t* Q_buf, int32_t* ns_buf, unsigned long long* bytes_out);
+// Per paper: neighbourhood draws from its root's S, then PA refill.
+void launch_nb_draw(hipStream_t s, int64_t n_exist, int64_t n_new, int m, int n_nb, int cap,
+                    uint64_t seed, uint32_t year_index, const double* cdf, const int32_t* root_slot,
+                    const int32_t* S_buf, const uint64_t* Q_buf, const int32_t* ns_buf,
+                    int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* near, int32_t* anext);
 
 // Stand-alone 2-hop neighbourhoods (out[i*cap ...], out_len[i]).
 void launch_bfs_only(hipStream_t s, const CsrView& g, int64_t count, const int32_t* roots, int cap,

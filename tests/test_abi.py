@@ -37,12 +37,8 @@ def test_version_and_growth_rule():
 
 def test_create_without_device_fails_cleanly():
     import pytest
-    try:
-        import torch
-        if torch.cuda.is_available():
-            pytest.skip("a device is present")
-    except Exception:
-        pass
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a device is present")
     cfg = pg.Config(ctypes.sizeof(pg.Config), 0, 100, 100, 10, 4, 16, 1)
     h = ctypes.c_void_p()
     rc = pg.lib().pg_ctx_create(ctypes.byref(cfg), ctypes.byref(h))
