@@ -339,16 +339,18 @@ __device__ __forceinline__ uint64_t nb_fixed(float w, double scale) {
     return q < 0x1p50 ? (uint64_t)q : (uint64_t)1 << 50;
 }
 
-// k_bfs_root: S and its fixed-point CDF for each distinct root.
+// k_bfs_root: S and its fixed-point CDF for each distinct root that the
+// wavefront path (k_bfs_small) handed over in big_list.
 __global__ __launch_bounds__(kBT) void k_bfs_root(CsrView g, const float* __restrict__ w,
                                                   const int32_t* __restrict__ uroot,
-                                                  const int64_t* __restrict__ n_unique_ptr, int cap,
+                                                  const int32_t* __restrict__ big_list,
+                                                  const int32_t* __restrict__ big_count, int cap,
                                                   int32_t* __restrict__ S_buf,
                                                   uint64_t* __restrict__ Q_buf,
                                                   int32_t* __restrict__ ns_buf,
                                                   unsigned long long* __restrict__ bytes_out) {
-    const int64_t u = blockIdx.x;
-    if (u >= *n_unique_ptr) return;
+    if ((int)blockIdx.x >= *big_count) return;
+    const int64_t u = big_list[blockIdx.x];
     const int32_t root = uroot[u];
     int32_t* S = S_buf + u * cap;
     const int ns = two_hop(g, root, cap, S);
@@ -379,6 +381,174 @@ __global__ __launch_bounds__(kBT) void k_bfs_root(CsrView g, const float* __rest
     for (int s = s0; s < s1; ++s) {
         run += (int64_t)nb_fixed(w[S[s]], scale);
         Q[s] = (uint64_t)run;
+    }
+}
+
+
+// ---- wavefront path for small neighbourhoods ----
+//
+// One 64-lane wavefront per distinct root whose 1-hop rows hold at most
+// kSmallF1 entries and whose 2-hop candidate list (F1c plus the rows of
+// F1c) at most kSmallWin entries; ~10 KB of LDS, so 16 roots share a CU.
+// Larger roots are appended to big_list for k_bfs_root.  Same result as
+// the workgroup path: sort, drop duplicates (ballot/popcount compaction),
+// filter, keep the cap smallest.
+constexpr int kSmallF1 = 512;
+constexpr int kSmallWin = 2048;
+
+struct SmallLds {
+    int32_t f1[kSmallF1];
+    int32_t win[kSmallWin];
+};
+
+__device__ void wave_bitonic(int32_t* s, int P) {
+    const int lane = threadIdx.x;
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = lane; i < P; i += 64) {
+                int ixj = i ^ j;
+                if (ixj > i) {
+                    int32_t a = s[i], b = s[ixj];
+                    bool up = (i & k) == 0;
+                    if ((a > b) == up) { s[i] = b; s[ixj] = a; }
+                }
+            }
+            __syncthreads();  // one wavefront per workgroup: a wave barrier
+        }
+    }
+}
+
+// Ordered in-place compaction of the sorted s[0,n): keeps the first copy of
+// each id for which keep(x) holds, at most `limit` of them; returns count.
+template <typename Keep>
+__device__ int wave_unique_compact(int32_t* s, int n, int limit, Keep keep) {
+    const int lane = threadIdx.x;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int out = 0;
+    int32_t carry = INT_MIN;  // last element of the previous chunk
+    for (int base = 0; base < n && out < limit; base += 64) {
+        const int i = base + lane;
+        const int32_t x = i < n ? s[i] : INT_MAX;
+        int32_t prev = __shfl_up(x, 1, 64);
+        if (lane == 0) prev = carry;
+        const bool k = i < n && x != prev && x != INT_MAX && keep(x);
+        const uint64_t mask = __ballot(k);
+        const int pos = out + __popcll(mask & below);
+        carry = __shfl(x, 63, 64);
+        __syncthreads();
+        if (k && pos < limit) s[pos] = x;
+        __syncthreads();
+        out += __popcll(mask);
+    }
+    return out < limit ? out : limit;
+}
+
+__global__ __launch_bounds__(64) void k_bfs_small(CsrView g, const float* __restrict__ w,
+                                                  const int32_t* __restrict__ uroot,
+                                                  const int64_t* __restrict__ n_unique_ptr, int cap,
+                                                  int32_t* __restrict__ S_buf,
+                                                  uint64_t* __restrict__ Q_buf,
+                                                  int32_t* __restrict__ ns_buf,
+                                                  int32_t* __restrict__ big_list,
+                                                  int32_t* __restrict__ big_count,
+                                                  unsigned long long* __restrict__ bytes_out) {
+    __shared__ SmallLds Ls;
+    const int64_t u = blockIdx.x;
+    if (u >= *n_unique_ptr) return;
+    const int lane = threadIdx.x;
+    const int32_t root = uroot[u];
+    const int32_t fb = g.fwd_off[root], fe = g.fwd_off[root + 1];
+    const int32_t bb = g.bwd_off[root], be = g.bwd_off[root + 1];
+    const int df = fe - fb, d1 = df + (be - bb);
+    if (d1 > kSmallF1) {
+        if (lane == 0) big_list[atomicAdd(big_count, 1)] = (int32_t)u;
+        return;
+    }
+    // 1-hop: both rows, sorted, deduplicated, root dropped, capped
+    int P1 = 1;
+    while (P1 < d1) P1 <<= 1;
+    for (int k = lane; k < P1; k += 64)
+        Ls.f1[k] = k < df ? g.fwd_adj[fb + k] : (k < d1 ? g.bwd_adj[bb + k - df] : INT_MAX);
+    __syncthreads();
+    if (d1 > 1) wave_bitonic(Ls.f1, P1);
+    const int n1 = wave_unique_compact(Ls.f1, d1, kSmallF1, [&](int32_t x) { return x != root; });
+    const int n1c = n1 < cap ? n1 : cap;
+    // 2-hop candidate count
+    int64_t t = 0;
+    for (int q = lane; q < n1c; q += 64) {
+        const int32_t v = Ls.f1[q];
+        t += (g.fwd_off[v + 1] - g.fwd_off[v]) + (g.bwd_off[v + 1] - g.bwd_off[v]);
+    }
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    const int64_t tot2 = t + n1c;
+    if (tot2 > kSmallWin) {
+        if (lane == 0) big_list[atomicAdd(big_count, 1)] = (int32_t)u;
+        return;
+    }
+    const bool check_f1 = n1 > n1c;
+    const int32_t f1max = n1c > 0 ? Ls.f1[n1c - 1] : -1;
+    // gather F1c and the rows of F1c (one row pair per lane, 64 at a time)
+    for (int q = lane; q < n1c; q += 64) Ls.win[q] = Ls.f1[q];
+    int run = n1c;
+    for (int q0 = 0; q0 < n1c; q0 += 64) {
+        const int q = q0 + lane;
+        int32_t a = 0, b = 0, c = 0, d = 0;
+        if (q < n1c) {
+            const int32_t v = Ls.f1[q];
+            a = g.fwd_off[v]; b = g.fwd_off[v + 1]; c = g.bwd_off[v]; d = g.bwd_off[v + 1];
+        }
+        const int len = (b - a) + (d - c);
+        int x = len;
+        for (int off = 1; off < 64; off <<= 1) {
+            int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        const int pos = run + x - len;
+        for (int k = 0; k < b - a; ++k) Ls.win[pos + k] = g.fwd_adj[a + k];
+        for (int k = 0; k < d - c; ++k) Ls.win[pos + (b - a) + k] = g.bwd_adj[c + k];
+        run += __shfl(x, 63, 64);
+    }
+    int P2 = 1;
+    while (P2 < (int)tot2) P2 <<= 1;
+    for (int k = (int)tot2 + lane; k < P2; k += 64) Ls.win[k] = INT_MAX;
+    __syncthreads();
+    if (tot2 > 1) wave_bitonic(Ls.win, P2);
+    const int ns = wave_unique_compact(Ls.win, (int)tot2, cap, [&](int32_t x) {
+        if (x == root) return false;
+        if (check_f1 && x > f1max) {
+            if (row_contains(g.fwd_adj, fb, fe, x)) return false;
+            if (row_contains(g.bwd_adj, bb, be, x)) return false;
+        }
+        return true;
+    });
+    int32_t* S = S_buf + u * cap;
+    uint64_t* Q = Q_buf + u * cap;
+    uint32_t mb = 0;
+    for (int s = lane; s < ns; s += 64) {
+        const int32_t x = Ls.win[s];
+        S[s] = x;
+        mb = max(mb, f2u(w[x]));
+    }
+    for (int off = 32; off > 0; off >>= 1) mb = max(mb, (uint32_t)__shfl_xor((int)mb, off, 64));
+    const double scale = pow2d(49 - ((int)(mb >> 23) - 127));
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    (void)below;
+    uint64_t carry = 0;
+    for (int s0 = 0; s0 < ns; s0 += 64) {
+        const int s = s0 + lane;
+        uint64_t v = s < ns ? nb_fixed(w[Ls.win[s]], scale) : 0;
+        uint64_t x = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            uint64_t y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (s < ns) Q[s] = carry + x;
+        carry += __shfl(x, 63, 64);
+    }
+    if (lane == 0) {
+        ns_buf[u] = ns;
+        const int64_t by = 16 * (1 + (int64_t)n1c) + 4 * ((int64_t)d1 + tot2 - n1c) + 4 * (int64_t)ns;
+        atomicAdd(bytes_out, (unsigned long long)by);
     }
 }
 
@@ -523,10 +693,13 @@ void launch_root_dedup(hipStream_t s, int64_t n_exist, int64_t n_new, int m, con
 
 void launch_bfs_root(hipStream_t s, const CsrView& g, const float* w, const int32_t* uroot,
                      const int64_t* n_unique, int64_t max_unique, int cap, int32_t* S_buf,
-                     uint64_t* Q_buf, int32_t* ns_buf, unsigned long long* bytes_out) {
+                     uint64_t* Q_buf, int32_t* ns_buf, int32_t* big_list, int32_t* big_count,
+                     unsigned long long* bytes_out) {
     if (max_unique <= 0) return;
+    hipLaunchKernelGGL(k_bfs_small, dim3((uint32_t)max_unique), dim3(64), 0, s, g, w, uroot,
+                       n_unique, cap, S_buf, Q_buf, ns_buf, big_list, big_count, bytes_out);
     hipLaunchKernelGGL(k_bfs_root, dim3((uint32_t)max_unique), dim3(kBT), 0, s, g, w, uroot,
-                       n_unique, cap, S_buf, Q_buf, ns_buf, bytes_out);
+                       big_list, big_count, cap, S_buf, Q_buf, ns_buf, bytes_out);
 }
 
 void launch_nb_draw(hipStream_t s, int64_t n_exist, int64_t n_new, int m, int n_nb, int cap,

@@ -90,6 +90,7 @@ struct pg_ctx {
     int64_t* r_excl = nullptr;
     int32_t* root_slot = nullptr;
     int32_t* uroot = nullptr;
+    int32_t* big_list = nullptr;
 };
 
 namespace {
@@ -125,7 +126,7 @@ void release(pg_ctx* c) {
                     c->s_birth, c->s_in_deg, c->s_fwd_off, c->s_fwd_adj, c->s_bwd_off,
                     c->s_bwd_adj, c->S_buf, c->Q_buf, c->ns_buf, c->rk[0], c->rk[1],
                     c->rk[2], c->rk[3], c->rk[4], c->rk[5], c->r_flags, c->r_excl,
-                    c->root_slot, c->uroot};
+                    c->root_slot, c->uroot, c->big_list};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int k = 0; k < PG_K_COUNT; ++k)
@@ -179,6 +180,7 @@ int ensure_nb_buffers(pg_ctx* ctx) {
     if (r == hipSuccess) r = dalloc(&ctx->r_excl, D + 1);
     if (r == hipSuccess) r = dalloc(&ctx->root_slot, D);
     if (r == hipSuccess) r = dalloc(&ctx->uroot, D);
+    if (r == hipSuccess) r = dalloc(&ctx->big_list, D);
     if (r != hipSuccess)
         return fail(ctx, PG_ENOMEM, "neighbourhood buffers: device allocation failed");
     return PG_OK;
@@ -417,8 +419,10 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
                 launch_root_dedup(s, N, D, m, ctx->cit, ctx->cnt, ctx->rk[0], ctx->rk[1], ctx->rk[2],
                                   ctx->rk[3], ctx->rk[4], ctx->rk[5], ctx->r_flags, ctx->r_excl,
                                   ctx->root_slot, ctx->uroot, key_bits_for(N + 1), ctx->scratch);
+                int32_t* big_count = reinterpret_cast<int32_t*>(sums + 3);
                 launch_bfs_root(s, view(ctx), ctx->w, ctx->uroot, ctx->r_excl + D, D,
-                                ctx->cfg.bfs_cap, ctx->S_buf, ctx->Q_buf, ctx->ns_buf, sums + 2);
+                                ctx->cfg.bfs_cap, ctx->S_buf, ctx->Q_buf, ctx->ns_buf,
+                                ctx->big_list, big_count, sums + 2);
                 launch_nb_draw(s, N, D, m, n_nb, ctx->cfg.bfs_cap, seed, yi, ctx->cdf,
                                ctx->root_slot, ctx->S_buf, ctx->Q_buf, ctx->ns_buf, ctx->cit,
                                ctx->kind, ctx->cnt, ctx->near, ctx->anext);

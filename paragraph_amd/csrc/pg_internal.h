@@ -73,10 +73,13 @@ void launch_root_dedup(hipStream_t s, int64_t n_exist, int64_t n_new, int m, con
                        int64_t* excl, int32_t* root_slot, int32_t* uroot, int key_bits,
                        int64_t* scratch);
 // 2-hop neighbourhood S (S_buf[u*cap...], ns_buf[u]) and its fixed-point
-// CDF (Q_buf[u*cap...]) of every distinct root; grid max_unique blocks.
+// CDF (Q_buf[u*cap...]) of every distinct root: a wavefront per small root,
+// a workgroup per large one (big_list/big_count: scratch, *big_count = 0 on
+// entry); grids of max_unique blocks.
 void launch_bfs_root(hipStream_t s, const CsrView& g, const float* w, const int32_t* uroot,
                      const int64_t* n_unique, int64_t max_unique, int cap, int32_t* S_buf,
-                     uint64_t* Q_buf, int32_t* ns_buf, unsigned long long* bytes_out);
+                     uint64_t* Q_buf, int32_t* ns_buf, int32_t* big_list, int32_t* big_count,
+                     unsigned long long* bytes_out);
 // Per paper: neighbourhood draws from its root's S, then PA refill.
 void launch_nb_draw(hipStream_t s, int64_t n_exist, int64_t n_new, int m, int n_nb, int cap,
                     uint64_t seed, uint32_t year_index, const double* cdf, const int32_t* root_slot,
