@@ -160,6 +160,16 @@ int pg_bfs(pg_ctx* ctx, int64_t count, const int32_t* roots, int32_t* out,
            int32_t* out_len);
 
 /*
+ * pg_bfs with per-root timing of the device work: ticks[3*i] = wall-clock
+ * ticks (hipDeviceAttributeWallClockRate per ms) to the end of the 1-hop
+ * level, ticks[3*i+1] = to the end of the query, ticks[3*i+2] = 2-hop
+ * candidate entries.  A tooling entry point with no reference counterpart
+ * (the reference times whole stages, includes/epoch_profiler.cuh:40-51).
+ */
+int pg_bfs_timed(pg_ctx* ctx, int64_t count, const int32_t* roots, int32_t* out,
+                 int32_t* out_len, int64_t* ticks);
+
+/*
  * Device-side snapshot of the graph state (CSR, degrees, birth years,
  * log-fitness): pg_snapshot_save copies the current state into a second
  * device buffer, pg_snapshot_restore copies it back.  Used to rerun a

@@ -82,6 +82,7 @@ def lib() -> ctypes.CDLL:
             "pg_year_citations": (ctypes.c_int, [vp, i32p, i8p, i32p, i32p]),
             "pg_year_weights": (ctypes.c_int, [vp, f32p, f64p]),
             "pg_bfs": (ctypes.c_int, [vp, ctypes.c_int64, i32p, i32p, i32p]),
+            "pg_bfs_timed": (ctypes.c_int, [vp, ctypes.c_int64, i32p, i32p, i32p, i64p]),
             "pg_growth_count": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_double]),
             "pg_snapshot_save": (ctypes.c_int, [vp]),
             "pg_snapshot_restore": (ctypes.c_int, [vp]),
@@ -256,6 +257,19 @@ class Context:
         self._check(lib().pg_year_weights(self._h, _p(w, ctypes.c_float),
                                           _p(c, ctypes.c_double)), "pg_year_weights")
         return w, c
+
+    def bfs_timed(self, roots: np.ndarray):
+        """Neighbourhoods plus per-root (ticks to 1-hop end, ticks total,
+        2-hop candidate entries) from the workgroup path."""
+        roots = np.ascontiguousarray(roots, np.int32)
+        cap = self.cfg.bfs_cap
+        out = np.empty(roots.shape[0] * cap, np.int32)
+        ln = np.empty(roots.shape[0], np.int32)
+        tk = np.empty(roots.shape[0] * 3, np.int64)
+        self._check(lib().pg_bfs_timed(self._h, roots.shape[0], _p(roots, ctypes.c_int32),
+                                       _p(out, ctypes.c_int32), _p(ln, ctypes.c_int32),
+                                       _p(tk, ctypes.c_int64)), "pg_bfs_timed")
+        return ln, tk.reshape(-1, 3)
 
     def bfs(self, roots: np.ndarray) -> list[np.ndarray]:
         roots = np.ascontiguousarray(roots, np.int32)

@@ -36,27 +36,33 @@
 namespace pg {
 namespace {
 
-constexpr int kBT = 512;                    // threads per distinct root
+constexpr int kBT = 512;                    // threads per large root
 constexpr int kWaves = kBT / 64;
-constexpr int kWin = 4096;                  // LDS window entries
+constexpr int kWin = 7168;                  // LDS buffer: sort window or bitmap words
+constexpr int kSortMax = 2048;              // totals up to this are sorted, above it swept
+constexpr int kBmBits = kWin * 32;          // ids per bitmap window
 constexpr int kMaxRows = 1 + 2 * kBfsCapMax;
-constexpr int kLongSeg = 24;                // row parts copied by a whole wave
-constexpr int kMaxSegs = kWin / kLongSeg + 1;
+constexpr int kLongSeg = 48;                // row parts copied by a whole wave
+constexpr int kMaxSegs = kSortMax / kLongSeg + 1;
+constexpr int kLongRow = 192;               // rows longer than this use wave search
+constexpr int kMaxLong = 256;
 
 struct BfsLds {
     int32_t f1[kBfsCapMax];       // capped 1-hop list (ascending)
-    int32_t win[kWin];            // window
+    int32_t win[kWin];            // sort window, or bitmap of one id window
     int32_t cursor[kMaxRows];     // per-row position
     int32_t seg_row[kMaxSegs];
     int32_t seg_beg[kMaxSegs];
     int32_t seg_len[kMaxSegs];
+    int32_t lrow[kMaxLong];       // rows searched/copied by whole wavefronts
     int64_t red[kWaves];
     int64_t tot[3];               // row entries per level
+    int32_t nlong;
     int32_t nseg;
     int32_t fill;
     int32_t nout;
     int32_t n1c;
-    int64_t consumed;
+    int64_t t_level1;             // wall clock after the 1-hop level
 };
 
 // One LDS object per workgroup; helpers address it directly so the
@@ -84,6 +90,30 @@ __device__ __forceinline__ int gallop_lower_bound(const int32_t* p, int b, int e
     }
 }
 
+// First index in [b,e) with p[idx] >= x, found by the 64 lanes of a
+// wavefront together: each round samples 64 split points with one load per
+// lane and keeps the sub-range holding the answer (a 100k-entry row takes
+// three rounds of loads instead of 17 dependent ones).  All lanes call it
+// with the same arguments and get the same result.
+__device__ int wave_lower_bound(const int32_t* p, int b, int e, int32_t x) {
+    // (the count of samples below x is summed with shuffles: a ballot here
+    // trips a register-class error in the ROCm 7.2 backend)
+    const int lane = threadIdx.x & 63;
+    while (e - b > 64) {
+        const int64_t len = e - b;
+        const int pos = b + (int)((len * (lane + 1)) / 65);
+        int k = (p[pos] < x) ? 1 : 0;  // samples below x form a prefix
+        for (int off = 32; off > 0; off >>= 1) k += __shfl_xor(k, off, 64);
+        const int nb = k == 0 ? b : b + (int)((len * k) / 65) + 1;
+        const int ne = k == 64 ? e : b + (int)((len * (k + 1)) / 65) + 1;
+        b = nb;
+        e = ne;
+    }
+    int k = ((b + lane < e) && p[b + lane] < x) ? 1 : 0;
+    for (int off = 32; off > 0; off >>= 1) k += __shfl_xor(k, off, 64);
+    return b + k;
+}
+
 __device__ __forceinline__ bool row_contains(const int32_t* p, int b, int e, int32_t x) {
     int k = lower_bound_i32(p, b, e, x);
     return k < e && p[k] == x;
@@ -98,6 +128,17 @@ __device__ int64_t block_sum(int64_t v) {
     int64_t t = 0;
     for (int k = 0; k < kWaves; ++k) t += L.red[k];
     return t;
+}
+
+__device__ int32_t block_min(int32_t v) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_down(v, off, 64));
+    __syncthreads();
+    if (lane == 0) L.red[wid] = v;
+    __syncthreads();
+    int64_t t = INT_MAX;
+    for (int k = 0; k < kWaves; ++k) t = min(t, L.red[k]);
+    return (int32_t)t;
 }
 
 // Block-wide exclusive scan of one int64 per thread; *total gets the sum.
@@ -204,27 +245,47 @@ __device__ void flush_window(const Query& q, int nw, int32_t* out, int cap) {
 }
 
 // Collects the `cap` smallest distinct kept ids of the level's rows into
-// out[] (ascending); returns the count.  All threads call it.  Kept out of
-// line: inlining it into the kernels crashes the ROCm 7.2 optimiser.
+// out[] (ascending); returns the count.  All threads call it.  Up to
+// kSortMax row entries are gathered and bitonic-sorted in LDS; larger lists
+// are swept through the id range in bitmap windows (each entry touched
+// once, ids extracted in order, empty stretches skipped), below the bound
+// derived from long rows.  Rows longer than kLongRow are searched and read
+// by whole wavefronts.  Kept out of line: inlining it into the kernels
+// crashes the ROCm 7.2 optimiser.
 __device__ __attribute__((noinline)) int collect(const Query& q, int32_t* out, int cap) {
     const int32_t n = (int32_t)q.g.n;
     const int nr = num_rows(q);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (threadIdx.x == 0) L.nlong = 0;
+    __syncthreads();
     int64_t mine = 0;
+    int32_t tb = INT_MAX;
     for (int r = threadIdx.x; r < nr; r += kBT) {
         const int32_t* p; int b, e;
         row_of(q, r, p, b, e);
         L.cursor[r] = b;
         mine += e - b;
+        // a row with more than cap entries holds at least cap kept ids (only
+        // the root can be dropped from it) among its first cap+1, so no id
+        // above its entry [cap] can be in the answer
+        if (e - b > cap) tb = min(tb, p[b + cap]);
+        if (e - b > kLongRow) {
+            const int s = atomicAdd(&L.nlong, 1);
+            if (s < kMaxLong) L.lrow[s] = r;
+        }
     }
     const int64_t total = block_sum(mine);
-    if (threadIdx.x == 0) { L.nout = 0; L.tot[q.level] = total; L.consumed = 0; }
+    tb = block_min(tb);
+    if (threadIdx.x == 0) {
+        L.nout = 0; L.tot[q.level] = total;
+        if (L.nlong > kMaxLong) L.nlong = -1;  // list overflow: every row per thread
+    }
     __syncthreads();
     if (total == 0) return 0;
-    if (total <= kWin) {
-        // everything fits in one window
+    if (total <= kSortMax) {
+        // one sorted window
         if (threadIdx.x == 0) L.fill = 0;
         __syncthreads();
-        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
         for (int r = wid; r < nr; r += kWaves) {
             const int32_t* p; int b, e;
             row_of(q, r, p, b, e);
@@ -239,75 +300,74 @@ __device__ __attribute__((noinline)) int collect(const Query& q, int32_t* out, i
         flush_window(q, (int)total, out, cap);
         return L.nout;
     }
-    // Sweep ascending id windows; delta starts from a uniform-density guess
-    // and adapts (halved on overflow, doubled when a window is sparse).
-    int64_t delta = (int64_t)(kWin / 4) * n / total;
-    if (delta < 1) delta = 1;
-    int32_t lo = 0;
-    for (;;) {
-        const int32_t hi = (lo + delta >= n) ? INT_MAX : (int32_t)(lo + delta);
-        // count entries in [lo, hi) without moving the cursors
-        int64_t cnt = 0;
+    const bool long_ok = L.nlong >= 0;
+    const int nlong = long_ok ? L.nlong : 0;
+    // with a capped 1-hop list, F1c alone holds cap kept ids, all <= f1max;
+    // below the bound the F1 \ F1c filter never applies (those ids exceed
+    // f1max), so only the root is cleared from the bitmap
+    if (q.level == 2 && q.n1c == cap && q.n1c > 0) tb = min(tb, q.f1max);
+    const int64_t span = (tb == INT_MAX) ? (int64_t)n : (int64_t)tb + 1;
+    uint32_t* bm = reinterpret_cast<uint32_t*>(L.win);
+    int64_t lo64 = 0;
+    while (lo64 < span) {
+        const int32_t lo = (int32_t)lo64;
+        const int32_t hi = (int32_t)min(lo64 + kBmBits, span);
+        for (int k = threadIdx.x; k < kWin; k += kBT) bm[k] = 0u;
+        __syncthreads();
+        int32_t nxt = INT_MAX;  // smallest unconsumed id at or above hi
         for (int r = threadIdx.x; r < nr; r += kBT) {
             const int32_t* p; int b, e;
             row_of(q, r, p, b, e);
-            const int c = L.cursor[r];
-            const int z = hi == INT_MAX ? e : gallop_lower_bound(p, c, e, hi);
-            cnt += z - c;
+            if (long_ok && e - b > kLongRow) continue;
+            int c = L.cursor[r];
+            for (; c < e; ++c) {
+                const int32_t x = p[c];
+                if (x >= hi) { nxt = min(nxt, x); break; }
+                atomicOr(&bm[(x - lo) >> 5], 1u << ((x - lo) & 31));
+            }
+            L.cursor[r] = c;
         }
-        const int64_t wtot = block_sum(cnt);
-        if (wtot > kWin && (hi == INT_MAX || hi - lo > 1)) {
-            delta = (hi == INT_MAX ? (int64_t)n - lo : delta) / 2;
-            if (delta < 1) delta = 1;
-            continue;
-        }
-        // gather (a single-id window may hold more than kWin copies of one
-        // id; the extra copies are dropped, which is harmless)
-        if (threadIdx.x == 0) { L.fill = 0; L.nseg = 0; }
-        __syncthreads();
-        for (int r = threadIdx.x; r < nr; r += kBT) {
+        for (int s = wid; s < nlong; s += kWaves) {
+            const int r = L.lrow[s];
             const int32_t* p; int b, e;
             row_of(q, r, p, b, e);
             const int c = L.cursor[r];
-            const int z = hi == INT_MAX ? e : gallop_lower_bound(p, c, e, hi);
-            const int len = z - c;
-            if (len > 0) {
-                const int base = atomicAdd(&L.fill, len);
-                if (len >= kLongSeg && base < kWin) {
-                    const int s = atomicAdd(&L.nseg, 1);
-                    L.seg_row[s] = r; L.seg_beg[s] = c; L.seg_len[s] = base;
-                } else {
-                    for (int k = 0; k < len; ++k)
-                        if (base + k < kWin) L.win[base + k] = p[c + k];
-                }
+            const int z = wave_lower_bound(p, c, e, hi);
+            for (int k = c + lane; k < z; k += 64) {
+                const int32_t x = p[k];
+                atomicOr(&bm[(x - lo) >> 5], 1u << ((x - lo) & 31));
             }
-            L.cursor[r] = z;
+            if (lane == 0) {
+                L.cursor[r] = z;
+                if (z < e) nxt = min(nxt, p[z]);
+            }
         }
+        nxt = block_min(nxt);
+        if (threadIdx.x == 0 && q.root >= lo && q.root < hi)
+            bm[(q.root - lo) >> 5] &= ~(1u << ((q.root - lo) & 31));
         __syncthreads();
-        {
-            // long row parts: one wavefront each, lanes copy in parallel
-            const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-            const int nseg = L.nseg;
-            for (int s = wid; s < nseg; s += kWaves) {
-                const int r = L.seg_row[s];
-                const int c = L.seg_beg[s];
-                const int base = L.seg_len[s];
-                const int32_t* p; int b, e;
-                row_of(q, r, p, b, e);
-                const int z = L.cursor[r];
-                for (int k = lane; k < z - c; k += 64)
-                    if (base + k < kWin) L.win[base + k] = p[c + k];
+        constexpr int kWordsPer = kWin / kBT;
+        const int w0 = threadIdx.x * kWordsPer;
+        int cntw = 0;
+        for (int k = 0; k < kWordsPer; ++k) cntw += __popc(bm[w0 + k]);
+        int64_t tot;
+        int64_t pos = block_excl_scan(cntw, &tot) + L.nout;
+        for (int k = 0; k < kWordsPer && pos < cap; ++k) {
+            uint32_t bits = bm[w0 + k];
+            while (bits && pos < cap) {
+                const int bit = __builtin_ctz(bits);
+                bits &= bits - 1;
+                out[pos++] = lo + ((w0 + k) << 5) + bit;
             }
         }
         __syncthreads();
-        const int fill = L.fill;
-        const int nw = fill < kWin ? fill : kWin;
-        flush_window(q, nw, out, cap);
-        if (threadIdx.x == 0) L.consumed += fill;
+        if (threadIdx.x == 0) {
+            const int64_t no = L.nout + tot;
+            L.nout = (int32_t)(no < cap ? no : cap);
+        }
         __syncthreads();
-        if (L.nout >= cap || hi == INT_MAX || L.consumed >= total) break;
-        lo = hi;
-        if (fill < kWin / 4) delta *= 2;
+        if (L.nout >= cap || nxt == INT_MAX) break;
+        lo64 = nxt;  // skip the empty stretch
     }
     return L.nout;
 }
@@ -326,7 +386,7 @@ __device__ int two_hop(const CsrView& g, int32_t root, int cap, int32_t* out) {
         if (bo[root + 1] > bo[root]) mx = max(mx, g.bwd_adj[bo[root + 1] - 1]);
         q.check_f1 = mx > q.f1max;
     }
-    if (threadIdx.x == 0) L.n1c = n1c;
+    if (threadIdx.x == 0) { L.n1c = n1c; L.t_level1 = wall_clock64(); }
     __syncthreads();
     return collect(q, out, cap);
 }
@@ -341,7 +401,7 @@ __device__ __forceinline__ uint64_t nb_fixed(float w, double scale) {
 
 // k_bfs_root: S and its fixed-point CDF for each distinct root that the
 // wavefront path (k_bfs_small) handed over in big_list.
-__global__ __launch_bounds__(kBT) void k_bfs_root(CsrView g, const float* __restrict__ w,
+__global__ __launch_bounds__(kBT, 4) void k_bfs_root(CsrView g, const float* __restrict__ w,
                                                   const int32_t* __restrict__ uroot,
                                                   const int32_t* __restrict__ big_list,
                                                   const int32_t* __restrict__ big_count, int cap,
@@ -552,14 +612,23 @@ __global__ __launch_bounds__(64) void k_bfs_small(CsrView g, const float* __rest
     }
 }
 
-__global__ __launch_bounds__(kBT) void k_bfs_only(CsrView g, const int32_t* __restrict__ roots,
+__global__ __launch_bounds__(kBT, 4) void k_bfs_only(CsrView g, const int32_t* __restrict__ roots,
                                                   int cap, int32_t* __restrict__ out,
-                                                  int32_t* __restrict__ out_len) {
+                                                  int32_t* __restrict__ out_len,
+                                                  int64_t* __restrict__ ticks) {
     const int64_t i = blockIdx.x;
     const int32_t root = roots[i];
+    const int64_t t0 = wall_clock64();
     int ns = 0;
     if (root >= 0 && root < g.n) ns = two_hop(g, root, cap, out + i * cap);
-    if (threadIdx.x == 0) out_len[i] = ns;
+    if (threadIdx.x == 0) {
+        out_len[i] = ns;
+        if (ticks) {
+            ticks[3 * i + 0] = L.t_level1 - t0;
+            ticks[3 * i + 1] = wall_clock64() - t0;
+            ticks[3 * i + 2] = L.tot[2];
+        }
+    }
 }
 
 // ---- root deduplication ----
@@ -714,10 +783,10 @@ void launch_nb_draw(hipStream_t s, int64_t n_exist, int64_t n_new, int m, int n_
 }
 
 void launch_bfs_only(hipStream_t s, const CsrView& g, int64_t count, const int32_t* roots, int cap,
-                     int32_t* out, int32_t* out_len) {
+                     int32_t* out, int32_t* out_len, int64_t* ticks) {
     if (count <= 0) return;
     hipLaunchKernelGGL(k_bfs_only, dim3((uint32_t)count), dim3(kBT), 0, s, g, roots, cap, out,
-                       out_len);
+                       out_len, ticks);
 }
 
 }  // namespace pg

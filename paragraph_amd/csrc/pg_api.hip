@@ -652,7 +652,8 @@ int pg_year_weights(pg_ctx* ctx, float* w, double* cdf) {
     return PG_OK;
 }
 
-int pg_bfs(pg_ctx* ctx, int64_t count, const int32_t* roots, int32_t* out, int32_t* out_len) {
+int pg_bfs_timed(pg_ctx* ctx, int64_t count, const int32_t* roots, int32_t* out, int32_t* out_len,
+                 int64_t* ticks) {
     if (!ctx) return PG_EINVAL;
     if (!ctx->loaded) return fail(ctx, PG_ESTATE, "pg_bfs: no graph loaded");
     if (count < 0 || (count > 0 && (!roots || !out || !out_len)))
@@ -664,24 +665,32 @@ int pg_bfs(pg_ctx* ctx, int64_t count, const int32_t* roots, int32_t* out, int32
     hipStream_t s = ctx->stream;
     const int cap = ctx->cfg.bfs_cap;
     int32_t *d_roots = nullptr, *d_out = nullptr, *d_len = nullptr;
-    PG_HIP(dalloc(&d_roots, count));
-    hipError_t e1 = dalloc(&d_out, count * cap);
-    hipError_t e2 = dalloc(&d_len, count);
-    if (e1 != hipSuccess || e2 != hipSuccess) {
-        (void)hipFree(d_roots); (void)hipFree(d_out); (void)hipFree(d_len);
+    int64_t* d_ticks = nullptr;
+    hipError_t rc = dalloc(&d_roots, count);
+    if (rc == hipSuccess) rc = dalloc(&d_out, count * cap);
+    if (rc == hipSuccess) rc = dalloc(&d_len, count);
+    if (rc == hipSuccess && ticks) rc = dalloc(&d_ticks, 3 * count);
+    if (rc != hipSuccess) {
+        (void)hipFree(d_roots); (void)hipFree(d_out); (void)hipFree(d_len); (void)hipFree(d_ticks);
         return fail(ctx, PG_ENOMEM, "pg_bfs: device allocation failed");
     }
-    hipError_t rc = hipMemcpyAsync(d_roots, roots, count * 4, hipMemcpyHostToDevice, s);
+    rc = hipMemcpyAsync(d_roots, roots, count * 4, hipMemcpyHostToDevice, s);
     if (rc == hipSuccess) {
-        launch_bfs_only(s, view(ctx), count, d_roots, cap, d_out, d_len);
+        launch_bfs_only(s, view(ctx), count, d_roots, cap, d_out, d_len, d_ticks);
         rc = hipGetLastError();
     }
     if (rc == hipSuccess) rc = hipMemcpyAsync(out, d_out, count * cap * 4, hipMemcpyDeviceToHost, s);
     if (rc == hipSuccess) rc = hipMemcpyAsync(out_len, d_len, count * 4, hipMemcpyDeviceToHost, s);
+    if (rc == hipSuccess && ticks)
+        rc = hipMemcpyAsync(ticks, d_ticks, 3 * count * 8, hipMemcpyDeviceToHost, s);
     if (rc == hipSuccess) rc = hipStreamSynchronize(s);
-    (void)hipFree(d_roots); (void)hipFree(d_out); (void)hipFree(d_len);
+    (void)hipFree(d_roots); (void)hipFree(d_out); (void)hipFree(d_len); (void)hipFree(d_ticks);
     if (rc != hipSuccess) return fail(ctx, PG_EHIP, std::string("pg_bfs: ") + hipGetErrorString(rc));
     return PG_OK;
+}
+
+int pg_bfs(pg_ctx* ctx, int64_t count, const int32_t* roots, int32_t* out, int32_t* out_len) {
+    return pg_bfs_timed(ctx, count, roots, out, out_len, nullptr);
 }
 
 int64_t pg_growth_count(int64_t n, double rate) {

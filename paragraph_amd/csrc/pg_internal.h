@@ -86,9 +86,11 @@ void launch_nb_draw(hipStream_t s, int64_t n_exist, int64_t n_new, int m, int n_
                     const int32_t* S_buf, const uint64_t* Q_buf, const int32_t* ns_buf,
                     int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* near, int32_t* anext);
 
-// Stand-alone 2-hop neighbourhoods (out[i*cap ...], out_len[i]).
+// Stand-alone 2-hop neighbourhoods (out[i*cap ...], out_len[i]) by the
+// workgroup path; ticks (nullable) gets per root the wall-clock ticks to the
+// end of the 1-hop level, to the end, and the 2-hop candidate entries.
 void launch_bfs_only(hipStream_t s, const CsrView& g, int64_t count, const int32_t* roots, int cap,
-                     int32_t* out, int32_t* out_len);
+                     int32_t* out, int32_t* out_len, int64_t* ticks);
 
 // ---- CSR append ----
 // Sorted forward rows of the new papers appended at fwd_adj + fwd_base;

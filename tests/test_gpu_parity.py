@@ -207,3 +207,33 @@ def test_snapshot_restore_reproduces():
         ctx.snapshot_restore()
     for k in runs[0]:
         assert np.array_equal(runs[0][k], runs[1][k]), k
+
+
+def multi_hub_graph(n=100_000, hubs=30, p=0.03, seed=12):
+    """Paper n-1 cites `hubs` hubs (ids 0..hubs-1), each cited by ~p*n papers
+    spread over the whole id range: its level-2 list has ~90k entries and no
+    row longer than the cap, which takes the bitmap-window path."""
+    rng = np.random.default_rng(seed)
+    src, dst = [], []
+    for h in range(hubs):
+        cit = np.nonzero(rng.random(n - 1 - hubs) < p)[0] + hubs
+        src.append(cit); dst.append(np.full(cit.size, h))
+    src.append(np.full(hubs, n - 1)); dst.append(np.arange(hubs))
+    for i in range(hubs + 1, n - 1, 7):   # some ordinary citations too
+        src.append(np.array([i])); dst.append(np.array([rng.integers(hubs, i)]))
+    src = np.concatenate(src).astype(np.int32)
+    dst = np.concatenate(dst).astype(np.int32)
+    birth = np.zeros(n, np.int32)
+    return pg.SeedGraph(birth, src, dst, np.zeros(n + 10, np.float32))
+
+
+@pytest.mark.parametrize("cap", [4096, 1000])
+def test_bfs_bitmap_window_path(cap):
+    g = multi_hub_graph()
+    ctx = new_ctx(g, [1], 1, cap=cap)
+    G = oracle_graph(ctx, g.logfit)
+    roots = np.array([g.n - 1, 0, 1, 29, 31, 5000], np.int32)
+    log(f"bitmap-path BFS cap={cap}")
+    for r, s in zip(roots, ctx.bfs(roots)):
+        ref = ob.bfs(G, int(r), cap)
+        assert np.array_equal(s, ref), f"root {r}: {s[:8]} vs {ref[:8]} ({s.size} vs {ref.size})"
