@@ -57,6 +57,24 @@ def cpu_baseline(G0, sched, m, n_nb, alpha, tau, seed, cap):
                       f"with {cores} OpenMP threads: {dt:.2f} s"}
 
 
+def copy_calibration(nbytes: int = 1 << 30) -> None:
+    """One plain device-to-device copy of nbytes (HIP runtime copy kernel):
+    the reference point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE
+    counters to bytes (the copy reads and writes nbytes each)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": nbytes}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,7 +83,12 @@ def main():
     ap.add_argument("--n0", type=int, default=2_000_000)
     ap.add_argument("--years", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(every GPU box here has one GPU).", file=sys.stderr)

@@ -85,10 +85,11 @@ typedef struct {
 enum {
     PG_K_WEIGHTS_CDF = 0,      /* fused weight pass + FP64 CDF */
     PG_K_PA_DRAW = 1,          /* PA draws without duplicates */
-    PG_K_BFS_NB = 2,           /* 2-hop BFS + neighbourhood draws + refill */
-    PG_K_ER = 3,               /* ER count + write */
-    PG_K_CSR_APPEND = 4,       /* forward append, radix group, backward merge */
-    PG_K_COUNT = 5
+    PG_K_BFS = 2,              /* 2-hop BFS of the distinct roots (both paths) */
+    PG_K_NB_DRAW = 3,          /* root dedup + neighbourhood draws + refill */
+    PG_K_ER = 4,               /* ER count + write */
+    PG_K_CSR_APPEND = 5,       /* forward append, radix group, backward merge */
+    PG_K_COUNT = 6
 };
 
 /* Returns PG_ABI_VERSION. */
@@ -186,8 +187,9 @@ int pg_snapshot_restore(pg_ctx* ctx);
  * bytes count compulsory traffic only:
  *   WEIGHTS_CDF  20 B per existing paper (12 B read, 8 B FP64 CDF written);
  *   PA_DRAW      8*ceil(log2 N) B per attempt + 4 B per emitted target;
- *   BFS_NB       per query, 16 B + 4 B*deg(v) for the root and each capped
- *                1-hop paper, + 4 B per output id;
+ *   BFS          per distinct root, 16 B + 4 B*deg(v) for the root and each
+ *                capped 1-hop paper, + 4 B per output id;
+ *   NB_DRAW      20 B per new paper + 8 B per citation slot;
  *   ER           4 B per emitted target + 4 B per new paper;
  *   CSR_APPEND   4*(dE+dN) forward write + 4*(E_old+N_old) read +
  *                4*(E_new+N_new) write + 8 B per new edge (degree RMW).

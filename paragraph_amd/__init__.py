@@ -52,7 +52,7 @@ class YearStats(ctypes.Structure):
                 ("pa_attempts", ctypes.c_int64), ("device_ms", ctypes.c_double)]
 
 
-KERNEL_GROUPS = ["weights_cdf", "pa_draw", "bfs_nb", "er", "csr_append"]
+KERNEL_GROUPS = ["weights_cdf", "pa_draw", "bfs", "nb_draw", "er", "csr_append"]
 
 
 _lib = None

@@ -691,13 +691,16 @@ __global__ __launch_bounds__(256) void k_nb_draw(int64_t n_exist, int64_t n_new,
     const int ns = ns_buf[u];
     const int32_t* S = S_buf + u * cap;
     const uint64_t* Q = Q_buf + u * cap;
-    // cited papers inside S (binary search each cited id in S)
+    // cited papers inside S (binary search each cited id in S); only needed
+    // when S is small enough that taking all of S \ cited is possible
     int inS = 0;
-    for (int j = 0; j < c0; ++j) {
-        const int32_t x = row[j];
-        int lo = 0, hi = ns;
-        while (lo < hi) { int mid = (lo + hi) >> 1; if (S[mid] < x) lo = mid + 1; else hi = mid; }
-        inS += (lo < ns && S[lo] == x) ? 1 : 0;
+    if (ns <= c0 + n_nb) {
+        for (int j = 0; j < c0; ++j) {
+            const int32_t x = row[j];
+            int lo = 0, hi = ns;
+            while (lo < hi) { int mid = (lo + hi) >> 1; if (S[mid] < x) lo = mid + 1; else hi = mid; }
+            inS += (lo < ns && S[lo] == x) ? 1 : 0;
+        }
     }
     int c = c0;
     const int avail = ns - inS;

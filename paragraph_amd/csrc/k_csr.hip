@@ -119,28 +119,78 @@ __global__ __launch_bounds__(256) void k_bwd_offsets(int64_t n_old, int64_t n_to
     else if (v < n_total) in_deg[v] = 0;
 }
 
+constexpr int kMergeOut = 4096;   // outputs per block (16 per thread)
+
+__device__ __forceinline__ int64_t row_of_pos(const int32_t* off, int64_t lo, int64_t hi, int64_t p) {
+    // largest v in [lo, hi) with off[v] <= p
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (off[mid] <= p) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Each block writes kMergeOut consecutive positions of the new backward
+// adjacency.  The new and old offsets of the rows those positions span are
+// staged in LDS once per block, so a thread locates its first row with a
+// short LDS search instead of a search over all N offsets.
 __global__ __launch_bounds__(256) void k_bwd_merge(int64_t n_old, int64_t n_total,
                                                    const int32_t* __restrict__ old_off,
                                                    const int32_t* __restrict__ old_adj,
                                                    const int32_t* __restrict__ new_off,
                                                    const uint32_t* __restrict__ sorted_src,
                                                    int32_t* __restrict__ out, int64_t e_new) {
-    const int64_t p0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 16;
-    if (p0 >= e_new) return;
-    int64_t lo = 0, hi = n_total + 1;  // first index with new_off > p0
-    while (lo < hi) {
-        int64_t mid = (lo + hi) >> 1;
-        if (new_off[mid] > p0) hi = mid; else lo = mid + 1;
+    __shared__ int32_t s_new[kMergeOut + 1];
+    __shared__ int32_t s_old[kMergeOut + 1];
+    __shared__ int64_t s_rlo, s_nrows;
+    const int64_t pb = (int64_t)blockIdx.x * kMergeOut;
+    if (pb >= e_new) return;
+    const int64_t pe = (pb + kMergeOut < e_new ? pb + kMergeOut : e_new) - 1;  // last position
+    if (threadIdx.x == 0) {
+        const int64_t r0 = row_of_pos(new_off, 0, n_total + 1, pb);
+        const int64_t r1 = row_of_pos(new_off, r0, n_total + 1, pe);
+        s_rlo = r0;
+        s_nrows = r1 - r0 + 1;
     }
-    int64_t v = lo - 1;
-    int64_t end = p0 + 16 < e_new ? p0 + 16 : e_new;
-    int32_t row_beg = new_off[v], row_end = new_off[v + 1];
-    for (int64_t p = p0; p < end; ++p) {
-        while (row_end <= p) { ++v; row_beg = row_end; row_end = new_off[v + 1]; }
-        const int64_t k = p - row_beg;
-        const int64_t ob = old_off[v];
-        const int64_t olen = old_off[v + 1] - ob;  // v < n_old for any non-empty row
-        out[p] = k < olen ? old_adj[ob + k] : (int32_t)sorted_src[(row_beg - ob) + (k - olen)];
+    __syncthreads();
+    const int64_t rlo = s_rlo, nrows = s_nrows;
+    const bool staged = nrows <= kMergeOut;
+    if (staged) {
+        for (int64_t k = threadIdx.x; k <= nrows; k += 256) {
+            s_new[k] = new_off[rlo + k];
+            const int64_t v = rlo + k;
+            s_old[k] = old_off[v < n_old ? v : n_old];
+        }
+    }
+    __syncthreads();
+    const int64_t p0 = pb + (int64_t)threadIdx.x * 16;
+    if (p0 > pe) return;
+    const int64_t end = p0 + 16 <= pe + 1 ? p0 + 16 : pe + 1;
+    if (staged) {
+        int64_t lo = 0, hi = nrows;  // local row index
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (s_new[mid] <= p0) lo = mid; else hi = mid;
+        }
+        int64_t v = lo;
+        int32_t row_beg = s_new[v], row_end = s_new[v + 1];
+        for (int64_t p = p0; p < end; ++p) {
+            while (row_end <= p) { ++v; row_beg = row_end; row_end = s_new[v + 1]; }
+            const int64_t k = p - row_beg;
+            const int64_t ob = s_old[v];
+            const int64_t olen = s_old[v + 1] - ob;
+            out[p] = k < olen ? old_adj[ob + k] : (int32_t)sorted_src[(row_beg - ob) + (k - olen)];
+        }
+    } else {
+        int64_t v = row_of_pos(new_off, rlo, rlo + nrows, p0);
+        int32_t row_beg = new_off[v], row_end = new_off[v + 1];
+        for (int64_t p = p0; p < end; ++p) {
+            while (row_end <= p) { ++v; row_beg = row_end; row_end = new_off[v + 1]; }
+            const int64_t k = p - row_beg;
+            const int64_t ob = old_off[v];
+            const int64_t olen = old_off[v + 1] - ob;  // v < n_old for any non-empty row
+            out[p] = k < olen ? old_adj[ob + k] : (int32_t)sorted_src[(row_beg - ob) + (k - olen)];
+        }
     }
 }
 
@@ -204,8 +254,7 @@ void launch_bwd_merge(hipStream_t s, int64_t n_old, int64_t n_total, const int32
                       const int32_t* bwd_adj_old, const int32_t* bwd_off_new,
                       const uint32_t* sorted_src, int32_t* bwd_adj_new, int64_t e_new) {
     if (e_new <= 0) return;
-    int64_t threads = (e_new + 15) / 16;
-    int64_t nb = (threads + 255) / 256;
+    int64_t nb = (e_new + kMergeOut - 1) / kMergeOut;
     hipLaunchKernelGGL(k_bwd_merge, dim3((uint32_t)nb), dim3(256), 0, s, n_old, n_total, bwd_off_old,
                        bwd_adj_old, bwd_off_new, sorted_src, bwd_adj_new, e_new);
 }

@@ -10,11 +10,12 @@
 //
 // The CDF is deterministic: each 4096-element tile is summed in a fixed
 // order (sequential within a thread, Kogge-Stone across a wavefront,
-// sequential across the four wavefronts); tiles take tickets in launch order
-// and look back to the nearest tile that has published its inclusive
-// prefix, then add the intervening tile aggregates *forward*.  Forward
-// addition from any published inclusive prefix reproduces the same
-// left-to-right chain of tile sums, so the result does not depend on timing.
+// sequential across the four wavefronts), the tile totals are chained left
+// to right, and every element is its tile's exclusive prefix plus its local
+// value.  Three launches (tile pass, tile-total chain, down-sweep) keep
+// inter-block communication at kernel boundaries: a single-pass look-back
+// needs a device-scope release per tile, which on this part writes back L2
+// and serialises the tiles.
 #include "pg_internal.h"
 
 namespace pg {
@@ -24,61 +25,45 @@ namespace {
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = kTile / kScanThreads;  // 16
 
-__device__ __forceinline__ uint32_t ld_acquire(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_release(uint32_t* p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Look-back over the 64 nearest predecessor tiles (wave 0 only): waits until
-// one of them has published its inclusive prefix and every tile after it
-// its aggregate, then leaves the start value and those aggregates in LDS.
-struct LookBack {
-    double vals[64];
-    double start;
-    int from;
+// Local (in-tile) inclusive values of one thread's 16 weights, in the fixed
+// summation order; *tile_last gets the tile total for thread 255.
+struct TileScan {
+    double v[kScanItems];
 };
 
-__device__ void lookback_collect(const ScanState st, uint32_t vb, LookBack* lb) {
-    const int lane = threadIdx.x & 63;
-    const uint32_t want_agg = st.epoch * 4u + 1u;
-    const uint32_t want_inc = st.epoch * 4u + 2u;
-    for (;;) {
-        int64_t k = (int64_t)vb - 1 - lane;
-        uint32_t stt = k >= 0 ? ld_acquire(st.status + k) : want_inc;
-        bool inc = (stt == want_inc);
-        bool ready = inc || stt == want_agg;
-        uint64_t inc_mask = __ballot(inc);
-        uint64_t rdy_mask = __ballot(ready);
-        int first = inc_mask ? __builtin_ctzll(inc_mask) : 64;
-        uint64_t need = (first >= 64) ? ~0ull : ((1ull << first) - 1ull);
-        if (first < 64 && (rdy_mask & need) == need) {
-            if (lane < first) lb->vals[lane] = st.agg[vb - 1 - lane];
-            if (lane == first) {
-                lb->start = (k >= 0) ? st.incl[k] : 0.0;
-                lb->from = first;
-            }
-            return;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-}
-
-__global__ __launch_bounds__(kScanThreads) void k_weights_cdf(
-    int64_t n, const int32_t* __restrict__ in_deg, const int32_t* __restrict__ birth,
-    const float* __restrict__ logfit, int32_t year, float alpha, float inv_tau,
-    float* __restrict__ w_out, double* __restrict__ cdf_out, ScanState st) {
-    __shared__ uint32_t s_vb;
-    __shared__ double s_wave[4];
-    __shared__ double s_excl;
-    if (threadIdx.x == 0) s_vb = atomicAdd(st.ticket, 1u);
-    __syncthreads();
-    const uint32_t vb = s_vb;
+__device__ __forceinline__ TileScan tile_local_scan(const float* wv, double* s_wave) {
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
-    const int64_t base = (int64_t)vb * kTile + (int64_t)threadIdx.x * kScanItems;
+    double s[kScanItems];
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) { acc += (double)wv[j]; s[j] = acc; }
+    double x = acc;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        double y = __shfl_up(x, off, 64);
+        if (lane >= off) x = y + x;
+    }
+    double ex = __shfl_up(x, 1, 64);
+    if (lane == 0) ex = 0.0;
+    if (lane == 63) s_wave[wid] = x;
+    __syncthreads();
+    double wpre = 0.0;
+    for (int k = 0; k < wid; ++k) wpre += s_wave[k];
+    const double texcl = (wid == 0) ? ex : wpre + ex;
+    TileScan t;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) t.v[j] = texcl + s[j];
+    return t;
+}
 
+// Pass 1: weights (stored) and each tile's total.
+__global__ __launch_bounds__(kScanThreads) void k_weights_tiles(
+    int64_t n, const int32_t* __restrict__ in_deg, const int32_t* __restrict__ birth,
+    const float* __restrict__ logfit, int32_t year, float alpha, float inv_tau,
+    float* __restrict__ w_out, double* __restrict__ tile_total) {
+    __shared__ double s_wave[4];
+    const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kScanItems;
     float wv[kScanItems];
     if (base + kScanItems <= n) {
         const int4* kd = reinterpret_cast<const int4*>(in_deg + base);
@@ -109,69 +94,66 @@ __global__ __launch_bounds__(kScanThreads) void k_weights_cdf(
             }
         }
     }
-    double s[kScanItems];
-    double acc = 0.0;
-#pragma unroll
-    for (int j = 0; j < kScanItems; ++j) { acc += (double)wv[j]; s[j] = acc; }
-    // Kogge-Stone inclusive scan of thread totals across the wavefront.
-    double x = acc;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        double y = __shfl_up(x, off, 64);
-        if (lane >= off) x = y + x;
-    }
-    double ex = __shfl_up(x, 1, 64);
-    if (lane == 0) ex = 0.0;
-    if (lane == 63) s_wave[wid] = x;
-    __syncthreads();
-    double wpre = 0.0;
-    for (int k = 0; k < wid; ++k) wpre += s_wave[k];
-    const double texcl = (wid == 0) ? ex : wpre + ex;
-    double v[kScanItems];
-#pragma unroll
-    for (int j = 0; j < kScanItems; ++j) v[j] = texcl + s[j];
+    TileScan t = tile_local_scan(wv, s_wave);
+    if (threadIdx.x == kScanThreads - 1) tile_total[blockIdx.x] = t.v[kScanItems - 1];
+}
 
-    // Publish this tile's aggregate (value at its last element).
-    if (threadIdx.x == kScanThreads - 1) {
-        if (vb == 0) {
-            st.incl[0] = v[kScanItems - 1];
-            __threadfence();
-            st_release(st.status + 0, st.epoch * 4u + 2u);
-        } else {
-            st.agg[vb] = v[kScanItems - 1];
-            __threadfence();
-            st_release(st.status + vb, st.epoch * 4u + 1u);
+// Pass 2: exclusive prefix of the tile totals, chained left to right by one
+// lane from LDS (the loads and stores of each 4096-tile chunk are spread
+// over the block).
+__global__ __launch_bounds__(256) void k_tile_chain(double* __restrict__ tile_total, int64_t ntiles) {
+    __shared__ double s_t[kTile];
+    __shared__ double s_carry;
+    if (threadIdx.x == 0) s_carry = 0.0;
+    for (int64_t c0 = 0; c0 < ntiles; c0 += kTile) {
+        const int64_t cn = (ntiles - c0 < kTile) ? ntiles - c0 : kTile;
+        for (int64_t k = threadIdx.x; k < cn; k += 256) s_t[k] = tile_total[c0 + k];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double e = s_carry;
+            for (int64_t k = 0; k < cn; ++k) {
+                const double t = s_t[k];
+                s_t[k] = e;
+                e = e + t;
+            }
+            s_carry = e;
         }
+        __syncthreads();
+        for (int64_t k = threadIdx.x; k < cn; k += 256) tile_total[c0 + k] = s_t[k];
+        __syncthreads();
     }
-    __shared__ LookBack s_lb;
-    if (vb != 0 && wid == 0) lookback_collect(st, vb, &s_lb);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double e = 0.0;
-        if (vb != 0) {
-            e = s_lb.start;
-            for (int q = s_lb.from - 1; q >= 0; --q) e += s_lb.vals[q];  // oldest first
+}
+
+// Pass 3: every element's CDF value (tile prefix + local value).
+__global__ __launch_bounds__(kScanThreads) void k_cdf_down(int64_t n, const float* __restrict__ w,
+                                                          const double* __restrict__ tile_excl,
+                                                          double* __restrict__ cdf_out) {
+    __shared__ double s_wave[4];
+    const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kScanItems;
+    float wv[kScanItems];
+    if (base + kScanItems <= n) {
+        const float4* wi = reinterpret_cast<const float4*>(w + base);
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; ++q) {
+            float4 f = wi[q];
+            wv[4 * q] = f.x; wv[4 * q + 1] = f.y; wv[4 * q + 2] = f.z; wv[4 * q + 3] = f.w;
         }
-        s_excl = e;
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) wv[j] = base + j < n ? w[base + j] : 0.0f;
     }
-    __syncthreads();
-    const double e = s_excl;
-    if (vb != 0 && threadIdx.x == kScanThreads - 1) {
-        st.incl[vb] = e + v[kScanItems - 1];
-        __threadfence();
-        st_release(st.status + vb, st.epoch * 4u + 2u);
-    }
+    TileScan t = tile_local_scan(wv, s_wave);
+    const double e = tile_excl[blockIdx.x];
     if (base + kScanItems <= n) {
         double2* co = reinterpret_cast<double2*>(cdf_out + base);
 #pragma unroll
         for (int q = 0; q < kScanItems / 2; ++q)
-            co[q] = make_double2(e + v[2 * q], e + v[2 * q + 1]);
+            co[q] = make_double2(e + t.v[2 * q], e + t.v[2 * q + 1]);
     } else {
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j)
-            if (base + j < n) cdf_out[base + j] = e + v[j];
+            if (base + j < n) cdf_out[base + j] = e + t.v[j];
     }
-    if (threadIdx.x == 0 && vb == gridDim.x - 1) atomicExch(st.ticket, 0u);
 }
 
 // ---- integer exclusive scan (reduce, scan of tile sums, down-sweep) ----
@@ -266,11 +248,14 @@ __global__ void k_sum_i32(const int32_t* __restrict__ p, int64_t n, unsigned lon
 
 void launch_weights_cdf(hipStream_t s, int64_t n, const int32_t* in_deg, const int32_t* birth,
                         const float* logfit, int32_t year, float alpha, float inv_tau, float* w,
-                        double* cdf, const ScanState& st) {
+                        double* cdf, double* tile_scratch) {
     if (n <= 0) return;
     int64_t nb = (n + kTile - 1) / kTile;
-    hipLaunchKernelGGL(k_weights_cdf, dim3((uint32_t)nb), dim3(kScanThreads), 0, s, n, in_deg,
-                       birth, logfit, year, alpha, inv_tau, w, cdf, st);
+    hipLaunchKernelGGL(k_weights_tiles, dim3((uint32_t)nb), dim3(kScanThreads), 0, s, n, in_deg,
+                       birth, logfit, year, alpha, inv_tau, w, tile_scratch);
+    hipLaunchKernelGGL(k_tile_chain, dim3(1), dim3(256), 0, s, tile_scratch, nb);
+    hipLaunchKernelGGL(k_cdf_down, dim3((uint32_t)nb), dim3(kScanThreads), 0, s, n, w, tile_scratch,
+                       cdf);
 }
 
 int64_t scan_scratch_elems(int64_t n) { return (n + kTile - 1) / kTile + 2; }

@@ -43,10 +43,6 @@ struct pg_ctx {
     float* w = nullptr;
     double* cdf = nullptr;
     double* scan_agg = nullptr;
-    double* scan_incl = nullptr;
-    uint32_t* scan_status = nullptr;
-    uint32_t* scan_ticket = nullptr;
-    uint32_t epoch = 0;
     int32_t* cit = nullptr;
     int8_t* kind = nullptr;
     int32_t* cnt = nullptr;
@@ -69,6 +65,7 @@ struct pg_ctx {
     // per-kernel-group profiling
     bool prof = false;
     hipEvent_t pev[PG_K_COUNT][2] = {};
+    hipEvent_t pev2[PG_K_COUNT][2] = {};
     double prof_ms[PG_K_COUNT] = {};
     double prof_bytes[PG_K_COUNT] = {};
     int64_t prof_calls[PG_K_COUNT] = {};
@@ -120,7 +117,7 @@ hipError_t dalloc(T** p, int64_t count) {
 void release(pg_ctx* c) {
     void* ptrs[] = {c->birth, c->in_deg, c->logfit, c->fwd_off, c->fwd_adj, c->bwd_off[0],
                     c->bwd_off[1], c->bwd_adj[0], c->bwd_adj[1], c->w, c->cdf, c->scan_agg,
-                    c->scan_incl, c->scan_status, c->scan_ticket, c->cit, c->kind, c->cnt,
+                    c->cit, c->kind, c->cnt,
                     c->near, c->anext, c->new_off, c->add, c->src_of, c->k_sorted,
                     c->v_sorted, c->k_tmp, c->v_tmp, c->scratch, c->d_sum, c->d_tmp_i32,
                     c->s_birth, c->s_in_deg, c->s_fwd_off, c->s_fwd_adj, c->s_bwd_off,
@@ -130,8 +127,10 @@ void release(pg_ctx* c) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int k = 0; k < PG_K_COUNT; ++k)
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < 2; ++j) {
             if (c->pev[k][j]) (void)hipEventDestroy(c->pev[k][j]);
+            if (c->pev2[k][j]) (void)hipEventDestroy(c->pev2[k][j]);
+        }
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -262,11 +261,6 @@ int pg_ctx_create(const pg_config* cfg, pg_ctx** out) {
     CREATE_HIP(dalloc(&ctx->w, N));
     CREATE_HIP(dalloc(&ctx->cdf, N));
     CREATE_HIP(dalloc(&ctx->scan_agg, tiles));
-    CREATE_HIP(dalloc(&ctx->scan_incl, tiles));
-    CREATE_HIP(dalloc(&ctx->scan_status, tiles));
-    CREATE_HIP(dalloc(&ctx->scan_ticket, 1));
-    CREATE_HIP(hipMemset(ctx->scan_status, 0, (size_t)tiles * sizeof(uint32_t)));
-    CREATE_HIP(hipMemset(ctx->scan_ticket, 0, sizeof(uint32_t)));
     CREATE_HIP(dalloc(&ctx->cit, D * M));
     CREATE_HIP(dalloc(&ctx->kind, D * M));
     CREATE_HIP(dalloc(&ctx->cnt, D));
@@ -276,7 +270,10 @@ int pg_ctx_create(const pg_config* cfg, pg_ctx** out) {
     CREATE_HIP(dalloc(&ctx->add, N));
     CREATE_HIP(dalloc(&ctx->d_sum, 4));
     for (int k = 0; k < PG_K_COUNT; ++k)
-        for (int j = 0; j < 2; ++j) CREATE_HIP(hipEventCreate(&ctx->pev[k][j]));
+        for (int j = 0; j < 2; ++j) {
+            CREATE_HIP(hipEventCreate(&ctx->pev[k][j]));
+            CREATE_HIP(hipEventCreate(&ctx->pev2[k][j]));
+        }
     {
         int rc = ensure_year_edges(ctx, D * M);
         if (rc) return bail(rc);
@@ -389,8 +386,12 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
     const uint64_t seed = ctx->cfg.seed;
     const uint32_t yi = (uint32_t)p->year_index;
     bool used[PG_K_COUNT] = {};
+    bool used2[PG_K_COUNT] = {};
     auto pbeg = [&](int k) { if (ctx->prof) (void)hipEventRecord(ctx->pev[k][0], s); };
     auto pend = [&](int k) { if (ctx->prof) { (void)hipEventRecord(ctx->pev[k][1], s); used[k] = true; } };
+    // second interval of a group within the same step (NB_DRAW: dedup, then draws)
+    auto pbeg2 = [&](int k) { if (ctx->prof) (void)hipEventRecord(ctx->pev2[k][0], s); };
+    auto pend2 = [&](int k) { if (ctx->prof) { (void)hipEventRecord(ctx->pev2[k][1], s); used2[k] = true; } };
     PG_HIP(hipEventRecord(ctx->ev0, s));
     int64_t e_year = 0, near_total = 0, attempts = 0;
     double bfs_bytes = 0.0;
@@ -400,11 +401,9 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
         PG_HIP(hipMemsetAsync(sums, 0, 4 * sizeof(unsigned long long), s));
         if (p->model == PG_MODEL_PA) {
             const int m = p->m, n_nb = p->n_nb;
-            ++ctx->epoch;
-            ScanState st{ctx->scan_agg, ctx->scan_incl, ctx->scan_status, ctx->scan_ticket, ctx->epoch};
             pbeg(PG_K_WEIGHTS_CDF);
             launch_weights_cdf(s, N, ctx->in_deg, ctx->birth, ctx->logfit, p->year, p->alpha,
-                               p->inv_tau, ctx->w, ctx->cdf, st);
+                               p->inv_tau, ctx->w, ctx->cdf, ctx->scan_agg);
             pend(PG_K_WEIGHTS_CDF);
             pbeg(PG_K_PA_DRAW);
             launch_pa_draw(s, N, ctx->cdf, D, m, m - n_nb, seed, yi, ctx->cit, ctx->kind, ctx->cnt,
@@ -415,18 +414,22 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
                     int rc = ensure_nb_buffers(ctx);
                     if (rc) return rc;
                 }
-                pbeg(PG_K_BFS_NB);
+                pbeg(PG_K_NB_DRAW);
                 launch_root_dedup(s, N, D, m, ctx->cit, ctx->cnt, ctx->rk[0], ctx->rk[1], ctx->rk[2],
                                   ctx->rk[3], ctx->rk[4], ctx->rk[5], ctx->r_flags, ctx->r_excl,
                                   ctx->root_slot, ctx->uroot, key_bits_for(N + 1), ctx->scratch);
+                pend(PG_K_NB_DRAW);
                 int32_t* big_count = reinterpret_cast<int32_t*>(sums + 3);
+                pbeg(PG_K_BFS);
                 launch_bfs_root(s, view(ctx), ctx->w, ctx->uroot, ctx->r_excl + D, D,
                                 ctx->cfg.bfs_cap, ctx->S_buf, ctx->Q_buf, ctx->ns_buf,
                                 ctx->big_list, big_count, sums + 2);
+                pend(PG_K_BFS);
+                pbeg2(PG_K_NB_DRAW);
                 launch_nb_draw(s, N, D, m, n_nb, ctx->cfg.bfs_cap, seed, yi, ctx->cdf,
                                ctx->root_slot, ctx->S_buf, ctx->Q_buf, ctx->ns_buf, ctx->cit,
                                ctx->kind, ctx->cnt, ctx->near, ctx->anext);
-                pend(PG_K_BFS_NB);
+                pend2(PG_K_NB_DRAW);
             }
             launch_sum_i32(s, ctx->near, D, sums + 0);
             launch_sum_i32(s, ctx->anext, D, sums + 1);
@@ -485,7 +488,8 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
         double bytes[PG_K_COUNT] = {};
         bytes[PG_K_WEIGHTS_CDF] = 20.0 * N;
         bytes[PG_K_PA_DRAW] = 8.0 * lg * attempts + 4.0 * D * (p->m - p->n_nb);
-        bytes[PG_K_BFS_NB] = bfs_bytes;
+        bytes[PG_K_BFS] = bfs_bytes;
+        bytes[PG_K_NB_DRAW] = 20.0 * D + 8.0 * D * p->m;
         bytes[PG_K_ER] = 4.0 * e_year + 4.0 * D;
         bytes[PG_K_CSR_APPEND] = 4.0 * (e_year + D) + 4.0 * (e_old + N) +
                                  4.0 * (e_old + e_year + N + D) + 8.0 * e_year;
@@ -493,6 +497,11 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
             if (!used[k]) continue;
             float kms = 0.0f;
             PG_HIP(hipEventElapsedTime(&kms, ctx->pev[k][0], ctx->pev[k][1]));
+            if (used2[k]) {
+                float k2 = 0.0f;
+                PG_HIP(hipEventElapsedTime(&k2, ctx->pev2[k][0], ctx->pev2[k][1]));
+                kms += k2;
+            }
             ctx->prof_ms[k] += kms;
             ctx->prof_bytes[k] += bytes[k];
             ctx->prof_calls[k] += 1;

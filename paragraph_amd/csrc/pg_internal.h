@@ -29,22 +29,13 @@ __device__ __forceinline__ bool cdf_near(const double* __restrict__ cdf, int64_t
     return (k > 0 && t - cdf[k - 1] <= eps) || (k < n - 1 && cdf[k] - t <= eps);
 }
 
-// Decoupled look-back state for the single-pass scans.  status[b] holds
-// epoch*4 + 1 (aggregate published) or epoch*4 + 2 (inclusive published).
-struct ScanState {
-    double* agg;
-    double* incl;
-    uint32_t* status;
-    uint32_t* ticket;
-    uint32_t epoch;
-};
-
 // ---- launchers (stream-ordered, no host synchronisation) ----
 
 // w[i] = pa_weight(...) for i < n; cdf[i] = FP64 inclusive prefix of w.
+// tile_scratch: ceil(n/4096) doubles.
 void launch_weights_cdf(hipStream_t s, int64_t n, const int32_t* in_deg, const int32_t* birth,
                         const float* logfit, int32_t year, float alpha, float inv_tau,
-                        float* w, double* cdf, const ScanState& st);
+                        float* w, double* cdf, double* tile_scratch);
 
 // Exclusive prefix sum of int32 in[0,n) into int64 out[0,n]; out[n] = total.
 // scratch: at least scan_scratch_elems(n) int64.
