@@ -1,1 +1,393 @@
-int main(){return 0;}
+// abm_main.cpp -- the `abm` command-line driver on top of libparagraph.so.
+//
+// Drop-in for the reference's `abm` executable (src/main.cu:22-175): the same
+// flags (src/main.cu:29-114; booleans accept true/1/yes, src/main.cu:43-45),
+// the same inputs -- nodelist `node,year` remapped to dense ids in ascending
+// original-id order (src/graph.cu:336-373), edgelist `citing,cited` with
+// unknown ids dropped (src/graph.cu:376-447), delimiter detected from the
+// first line as ',', tab, then space (includes/graph.cuh:41-53), `#` lines
+// skipped -- and the same outputs: `#source,target` edgelist rows in source
+// order with targets ascending (src/graph.cu:909-1003) and the 15-column
+// auxiliary CSV (src/graph.cu:1057-1218).  New papers get original ids
+// max_id+1, ... (src/kernel.cu:1638, 1711-1719), and each year adds
+// ceil(N * growth_rate) papers (src/abm.cu:233-240).
+//
+// The growth model is the PA/ER model of this library (weights
+// (k+1)^a * exp(-age/tau) * fitness, FP64 CDF, 2-hop neighbourhood draws, or
+// ER G(n,p)); flags that drive the reference's score-mixing model
+// (--preferential-weight, --recency-weight, --fitness-weight,
+// --same-year-proportion, --fully-random-citations, --planted-nodes,
+// --recency-probabilities contents) are parsed and reported as not used.
+// The process exits non-zero on any failure (the reference always exits 0,
+// src/main.cu:169).
+#include <algorithm>
+#include <charconv>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/paragraph.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Args {
+    std::map<std::string, std::string> kv;
+    bool has(const std::string& k) const { return kv.count(k) != 0; }
+    std::string str(const std::string& k, const std::string& def = "") const {
+        auto it = kv.find(k);
+        return it == kv.end() ? def : it->second;
+    }
+    double num(const std::string& k, double def) const {
+        auto it = kv.find(k);
+        return it == kv.end() ? def : std::stod(it->second);
+    }
+    bool flag(const std::string& k, bool def) const {
+        auto it = kv.find(k);
+        if (it == kv.end()) return def;
+        const std::string& v = it->second;
+        return v == "true" || v == "1" || v == "yes";
+    }
+};
+
+const char* kKnown[] = {
+    // reference flags (src/main.cu:29-114)
+    "--edgelist", "--nodelist", "--out-degree-bag", "--recency-probabilities", "--use-warp-bfs",
+    "--use-batching", "--use-multistage-kernel", "--max-batch-size", "--planted-nodes",
+    "--preferential-weight", "--recency-weight", "--fitness-weight", "--alpha",
+    "--fully-random-citations", "--growth-rate", "--num-cycles", "--same-year-proportion",
+    "--output-file", "--auxiliary-information-file", "--log-file", "--num-processors",
+    "--log-level",
+    // ER flags of the reference's gpu-er build (results/slurm-8640738.out:553-556)
+    "--model", "--er-probability", "--er-edges-per-node",
+    // this library's model parameters
+    "--pa-exponent", "--age-decay", "--neighbourhood-fraction", "--citations-per-paper",
+    "--seed", "--bfs-cap", "--fitness-sigma", "--device"};
+
+const char* kRequired[] = {"--edgelist", "--nodelist", "--out-degree-bag",
+                           "--recency-probabilities", "--growth-rate", "--num-cycles",
+                           "--same-year-proportion", "--output-file",
+                           "--auxiliary-information-file", "--log-file"};
+
+void usage() {
+    std::fprintf(stderr,
+                 "usage: abm --edgelist F --nodelist F --out-degree-bag F --recency-probabilities F\n"
+                 "           --growth-rate R --num-cycles Y --same-year-proportion P\n"
+                 "           --output-file F --auxiliary-information-file F --log-file F\n"
+                 "           [--model pa|er-gnp] [--er-probability p] [--er-edges-per-node k]\n"
+                 "           [--pa-exponent a] [--age-decay tau] [--neighbourhood-fraction f | --alpha f]\n"
+                 "           [--citations-per-paper m] [--seed s] [--bfs-cap c] [--fitness-sigma s]\n");
+}
+
+Args parse(int argc, char** argv) {
+    Args a;
+    for (int i = 1; i < argc; ++i) {
+        std::string k = argv[i];
+        if (k == "-h" || k == "--help") { usage(); std::exit(0); }
+        bool known = false;
+        for (const char* kk : kKnown) known = known || k == kk;
+        if (!known) throw std::runtime_error("unknown argument " + k);
+        if (i + 1 >= argc) throw std::runtime_error("missing value for " + k);
+        a.kv[k] = argv[++i];
+    }
+    for (const char* r : kRequired)
+        if (!a.has(r)) throw std::runtime_error(std::string(r) + ": required.");
+    return a;
+}
+
+char detect_delimiter(const std::string& path) {
+    std::ifstream f(path);
+    if (!f) throw std::runtime_error("cannot open " + path);
+    std::string line;
+    std::getline(f, line);
+    if (line.find(',') != std::string::npos) return ',';
+    if (line.find('\t') != std::string::npos) return '\t';
+    if (line.find(' ') != std::string::npos) return ' ';
+    throw std::invalid_argument("Could not detect filetype for " + path);
+}
+
+std::string slurp(const std::string& path) {
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
+    if (!f) throw std::runtime_error("cannot open " + path);
+    std::string s;
+    s.resize((size_t)f.tellg());
+    f.seekg(0);
+    f.read(&s[0], (std::streamsize)s.size());
+    return s;
+}
+
+// Splits `text` into lines and the first two delimiter-separated fields.
+template <typename F>
+void for_each_pair(const std::string& text, char delim, F&& fn) {
+    const char* p = text.data();
+    const char* end = p + text.size();
+    while (p < end) {
+        const char* le = p;
+        while (le < end && *le != '\n' && *le != '\r') ++le;
+        if (le > p && *p != '#') {
+            const char* f1 = p;
+            while (f1 < le && *f1 != delim) ++f1;
+            const char* s2 = f1 < le ? f1 + 1 : f1;
+            const char* f2 = s2;
+            while (f2 < le && *f2 != delim) ++f2;
+            long long a = 0, b = 0;
+            auto r1 = std::from_chars(p, f1, a);
+            auto r2 = std::from_chars(s2, f2, b);
+            if (r1.ec != std::errc() || r2.ec != std::errc())
+                throw std::runtime_error("malformed line: " + std::string(p, le));
+            fn(a, b);
+        }
+        p = le;
+        while (p < end && (*p == '\n' || *p == '\r')) ++p;
+    }
+}
+
+struct Logger {
+    std::ofstream f;
+    int level;
+    Clock::time_point t0 = Clock::now();
+    void info(const std::string& m) { write("[INFO]", m); }
+    void debug(const std::string& m) { if (level >= 1) write("[DEBUG]", m); }
+    void write(const char* tag, const std::string& m) {
+        if (level < 0 || !f) return;
+        double s = std::chrono::duration<double>(Clock::now() - t0).count();
+        f << tag << " " << m << " (elapsed " << s << " s)\n";
+        f.flush();
+    }
+};
+
+void check(int rc, pg_ctx* ctx, const char* what) {
+    if (rc != PG_OK)
+        throw std::runtime_error(std::string(what) + " failed (" + std::to_string(rc) + "): " +
+                                 pg_last_error(ctx));
+}
+
+void append_int(std::string& buf, long long v) {
+    char tmp[24];
+    auto r = std::to_chars(tmp, tmp + sizeof(tmp), v);
+    buf.append(tmp, r.ptr);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    try {
+        const Args a = parse(argc, argv);
+        Logger log;
+        log.level = (int)a.num("--log-level", 1) - 1;
+        log.f.open(a.str("--log-file"));
+        const std::string model = a.str("--model", "pa");
+        if (model != "pa" && model != "er-gnp")
+            throw std::runtime_error("--model must be pa or er-gnp");
+        const double rate = a.num("--growth-rate", 0.0);
+        const int years = (int)a.num("--num-cycles", 0);
+        if (!(rate >= 0.0) || years < 0) throw std::runtime_error("bad --growth-rate/--num-cycles");
+        for (const char* k : {"--preferential-weight", "--recency-weight", "--fitness-weight",
+                              "--same-year-proportion", "--fully-random-citations", "--planted-nodes",
+                              "--use-warp-bfs", "--use-batching", "--max-batch-size",
+                              "--use-multistage-kernel", "--num-processors"})
+            if (a.has(k))
+                std::fprintf(stderr, "abm: %s is a parameter of the reference's score-mixing "
+                                     "model and is not used by this growth model\n", k);
+
+        // ---- nodelist: dense ids in ascending original-id order ----
+        std::map<long long, int> node_year;
+        {
+            const std::string path = a.str("--nodelist");
+            const char d = detect_delimiter(path);
+            for_each_pair(slurp(path), d, [&](long long id, long long y) { node_year[id] = (int)y; });
+        }
+        if (node_year.empty()) throw std::runtime_error("empty nodelist");
+        const int64_t n0 = (int64_t)node_year.size();
+        std::vector<long long> orig(n0);
+        std::vector<int32_t> birth(n0);
+        std::unordered_map<long long, int32_t> seq;
+        seq.reserve((size_t)n0 * 2);
+        {
+            int32_t s = 0;
+            for (const auto& [id, y] : node_year) { orig[s] = id; birth[s] = y; seq[id] = s; ++s; }
+        }
+        int max_year = birth[0];
+        for (int32_t y : birth) max_year = std::max(max_year, y);
+        // ---- edgelist: edges between known papers ----
+        std::vector<int32_t> src, dst;
+        {
+            const std::string path = a.str("--edgelist");
+            const char d = detect_delimiter(path);
+            for_each_pair(slurp(path), d, [&](long long u, long long v) {
+                auto iu = seq.find(u), iv = seq.find(v);
+                if (iu != seq.end() && iv != seq.end() && iu->second != iv->second) {
+                    src.push_back(iu->second);
+                    dst.push_back(iv->second);
+                }
+            });
+        }
+        // ---- citations per new paper (out-degree bag `idx,outdeg`) ----
+        int m = (int)a.num("--citations-per-paper", -1);
+        if (m < 0 && model == "pa") {
+            std::vector<long long> bag;
+            const std::string path = a.str("--out-degree-bag");
+            const char d = detect_delimiter(path);
+            for_each_pair(slurp(path), d, [&](long long, long long k) { bag.push_back(k); });
+            if (bag.empty()) throw std::runtime_error("empty out-degree bag");
+            for (long long k : bag)
+                if (k != bag[0])
+                    throw std::runtime_error("this growth model draws a fixed number of citations "
+                                             "per paper; the out-degree bag holds several values, "
+                                             "pass --citations-per-paper");
+            m = (int)bag[0];
+        }
+        if (model == "er-gnp") m = 0;
+        if (m > 64) throw std::runtime_error("at most 64 citations per paper");
+        const double f_nb = a.has("--neighbourhood-fraction") ? a.num("--neighbourhood-fraction", 0.3)
+                            : (a.num("--alpha", -1.0) >= 0.0 ? a.num("--alpha", -1.0) : 0.3);
+        int n_nb = (int)std::floor(f_nb * m + 0.5);
+        if (m > 0 && n_nb > m - 1) n_nb = m - 1;
+        if (n_nb < 0) n_nb = 0;
+        const double alpha = a.num("--pa-exponent", 1.0);
+        const double tau = a.num("--age-decay", 5.0);
+        const uint64_t seed = (uint64_t)a.num("--seed", 1);
+        double p_er = a.num("--er-probability", -1.0);
+        // ---- schedule and capacities ----
+        std::vector<int64_t> sched;
+        int64_t n = n0, max_new = 0;
+        for (int y = 0; y < years; ++y) {
+            int64_t dn = pg_growth_count(n, rate);
+            sched.push_back(dn);
+            max_new = std::max(max_new, dn);
+            n += dn;
+        }
+        const int64_t n_final = n;
+        if (model == "er-gnp" && p_er < 0.0) {
+            const double k = a.num("--er-edges-per-node", -1.0);
+            if (k < 0.0) throw std::runtime_error("--model er-gnp needs --er-probability or --er-edges-per-node");
+            p_er = k / (double)n0;
+        }
+        int64_t e_cap = (int64_t)src.size() + (n_final - n0) * (int64_t)std::max(m, 1) + 16;
+        if (model == "er-gnp")
+            e_cap = (int64_t)src.size() + (int64_t)(1.2 * p_er * (double)n_final * (double)(n_final - n0)) + 1000000;
+        std::vector<float> logfit(n_final);
+        if (pg_seed_logfit(seed, 0, n_final, 0.0, a.num("--fitness-sigma", 0.5), logfit.data()))
+            throw std::runtime_error("fitness draw failed");
+        pg_config cfg{};
+        cfg.size = sizeof(cfg);
+        cfg.device = (int)a.num("--device", 0);
+        cfg.max_papers = n_final;
+        cfg.max_edges = e_cap;
+        cfg.max_new_per_year = std::max<int64_t>(max_new, 1);
+        cfg.max_cites = std::max(m, 1);
+        cfg.bfs_cap = (int)a.num("--bfs-cap", 4096);
+        cfg.seed = seed;
+        pg_ctx* ctx = nullptr;
+        check(pg_ctx_create(&cfg, &ctx), nullptr, "pg_ctx_create");
+        check(pg_graph_load(ctx, n0, birth.data(), logfit.data(), n_final, (int64_t)src.size(),
+                            src.data(), dst.data()), ctx, "pg_graph_load");
+        log.info("loaded " + std::to_string(n0) + " papers and " + std::to_string(src.size()) +
+                 " citations");
+        // ---- year loop ----
+        const auto t_loop = Clock::now();
+        int64_t total_edges = 0;
+        std::vector<int32_t> roots(n_final - n0, -1);
+        for (int y = 0; y < years; ++y) {
+            pg_year_params p{};
+            p.size = sizeof(p);
+            p.model = model == "pa" ? PG_MODEL_PA : PG_MODEL_ER;
+            p.year = max_year + 1 + y;
+            p.year_index = y;
+            p.n_new = sched[y];
+            p.m = m;
+            p.n_nb = n_nb;
+            p.alpha = (float)alpha;
+            p.inv_tau = std::isinf(tau) ? 0.0f : (float)(1.0 / tau);
+            p.er_p = p_er;
+            pg_year_stats st{};
+            st.size = sizeof(st);
+            int64_t nb_before = 0, e_unused = 0;
+            pg_graph_size(ctx, &nb_before, &e_unused);
+            check(pg_year_step(ctx, &p, &st), ctx, "pg_year_step");
+            if (model == "pa" && m > 0) {
+                std::vector<int32_t> cit((size_t)sched[y] * m);
+                check(pg_year_citations(ctx, cit.data(), nullptr, nullptr, nullptr), ctx,
+                      "pg_year_citations");
+                for (int64_t i = 0; i < sched[y]; ++i) roots[nb_before - n0 + i] = cit[(size_t)i * m];
+            }
+            total_edges += st.edges_added;
+            char msg[256];
+            std::snprintf(msg, sizeof(msg),
+                          "Year %d: N=%lld new=%lld edges=%lld near_boundary=%lld shortfall=%lld "
+                          "device_ms=%.2f", p.year, (long long)st.n_before, (long long)sched[y],
+                          (long long)st.edges_added, (long long)st.near_boundary,
+                          (long long)st.shortfall, st.device_ms);
+            log.info(msg);
+            std::printf("%s\n", msg);
+        }
+        const double loop_s = std::chrono::duration<double>(Clock::now() - t_loop).count();
+        std::printf("Growth: %d years, %lld new citation edges in %.3f s (%.3e edges/s)\n", years,
+                    (long long)total_edges, loop_s, loop_s > 0 ? total_edges / loop_s : 0.0);
+        // ---- outputs ----
+        int64_t nn = 0, ne = 0;
+        pg_graph_size(ctx, &nn, &ne);
+        std::vector<int32_t> fo(nn + 1), fa(ne), bo(nn + 1), indeg(nn), by(nn);
+        check(pg_graph_download(ctx, fo.data(), fa.data(), bo.data(), nullptr, indeg.data(),
+                                by.data(), nullptr), ctx, "pg_graph_download");
+        pg_ctx_destroy(ctx);
+        long long next_id = orig[n0 - 1] + 1;
+        orig.resize(nn);
+        for (int64_t v = n0; v < nn; ++v) orig[v] = next_id++;
+        {
+            std::string buf;
+            buf.reserve((size_t)ne * 16 + 32);
+            buf += "#source,target\n";
+            for (int64_t v = 0; v < nn; ++v) {
+                for (int32_t k = fo[v]; k < fo[v + 1]; ++k) {
+                    append_int(buf, orig[v]);
+                    buf.push_back(',');
+                    append_int(buf, orig[fa[k]]);
+                    buf.push_back('\n');
+                }
+            }
+            std::ofstream out(a.str("--output-file"), std::ios::binary);
+            out.write(buf.data(), (std::streamsize)buf.size());
+            if (!out) throw std::runtime_error("cannot write --output-file");
+        }
+        {
+            std::string buf;
+            buf += "node_id,type,year,pa_weight,rec_weight,fit_weight,fit_lag_duration,"
+                   "fit_peak_value,fit_peak_duration,alpha,in_degree,out_degree,"
+                   "assigned_out_degree,planted_nodes_line_number,generator_node_string\n";
+            char row[320];
+            for (int64_t v = 0; v < nn; ++v) {
+                const int outd = fo[v + 1] - fo[v];
+                int len;
+                if (v < n0) {
+                    len = std::snprintf(row, sizeof(row), "%lld,seed,%d,-1,-1,-1,0,0,0,-1,%d,%d,-1,-1,no_generators\n",
+                                        orig[v], by[v], indeg[v], outd);
+                } else {
+                    const int32_t r = roots[v - n0];
+                    const long long ro = r >= 0 ? orig[r] : -1;
+                    len = std::snprintf(row, sizeof(row), "%lld,agent,%d,%.10g,%.10g,%.10g,0,0,0,%.10g,%d,%d,%d,-1,%lld\n",
+                                        orig[v], by[v], alpha, tau, std::exp((double)logfit[v]),
+                                        f_nb, indeg[v], outd, m, ro);
+                }
+                buf.append(row, (size_t)len);
+            }
+            std::ofstream out(a.str("--auxiliary-information-file"), std::ios::binary);
+            out.write(buf.data(), (std::streamsize)buf.size());
+            if (!out) throw std::runtime_error("cannot write --auxiliary-information-file");
+        }
+        log.info("wrote " + std::to_string(ne) + " edges for " + std::to_string(nn) + " papers");
+        return 0;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "abm: %s\n", e.what());
+        return 1;
+    }
+}

@@ -1,0 +1,71 @@
+"""`abm` end to end on the GPU: reference file formats, and the graph it
+writes equals the oracle's replay of the same years."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import paragraph_amd as pg
+from oracle import binding as ob
+
+pytestmark = pytest.mark.gpu
+
+
+def test_abm_writes_reference_formats_and_matches_oracle(tmp_path):
+    d = str(tmp_path)
+    n0, years, rate, m, f_nb = 3000, 3, 0.05, 8, 0.25
+    g = pg.seed_graph(5, n0, 6.0)
+    ids = np.arange(n0) * 3 + 100  # sparse original ids
+    rng = np.random.default_rng(0)
+    order = rng.permutation(n0)
+    with open(os.path.join(d, "nodes.csv"), "w") as f:
+        f.write("#node,year\n")
+        for i in order:
+            f.write(f"{ids[i]},{g.birth[i] + 2000}\n")
+    with open(os.path.join(d, "edges.tsv"), "w") as f:
+        for s, t in zip(g.src, g.dst):
+            f.write(f"{ids[s]}\t{ids[t]}\n")
+        f.write("999999\t100\n")  # unknown id: dropped
+    open(os.path.join(d, "bag.csv"), "w").write(f"0,{m}\n")
+    open(os.path.join(d, "rec.csv"), "w").write("0,1\n")
+    j = lambda f: os.path.join(d, f)
+    args = [pg.ABM_PATH, "--edgelist", j("edges.tsv"), "--nodelist", j("nodes.csv"),
+            "--out-degree-bag", j("bag.csv"), "--recency-probabilities", j("rec.csv"),
+            "--growth-rate", str(rate), "--num-cycles", str(years), "--same-year-proportion", "0",
+            "--output-file", j("out.csv"), "--auxiliary-information-file", j("aux.csv"),
+            "--log-file", j("log.txt"), "--alpha", str(f_nb), "--pa-exponent", "1.1",
+            "--age-decay", "4", "--seed", "7"]
+    print("running abm", flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    lines = open(j("out.csv")).read().splitlines()
+    assert lines[0] == "#source,target"
+    e = np.array([list(map(int, l.split(","))) for l in lines[1:]], np.int64)
+    # oracle replay on dense ids (ascending original ids = dense order here)
+    birth = (g.birth + 2000).astype(np.int32)
+    sched = pg.growth_schedule(n0, rate, years)
+    lf = np.empty(n0 + sum(sched), np.float32)
+    pg.lib().pg_seed_logfit(7, 0, lf.size, 0.0, 0.5, lf.ctypes.data_as(pg.ctypes.POINTER(pg.ctypes.c_float)))
+    G = ob.Graph.from_edges(n0, g.src, g.dst, birth, lf)
+    near = 0
+    n_nb = min(int(np.floor(f_nb * m + 0.5)), m - 1)
+    for y, D in enumerate(sched):
+        res = ob.pa_year(G, 7, y, int(birth.max()) + 1 + y, D, m, n_nb, 1.1, 4.0)
+        near += int(res["near"].sum())
+        off, tgt = ob.pa_compact(res)
+        G = ob.csr_append(G, off, tgt, lf, int(birth.max()) + 1 + y)
+    orig = np.concatenate([ids, ids[-1] + 1 + np.arange(G.n - n0)])
+    src = np.repeat(np.arange(G.n), np.diff(G.fwd_off))
+    ref = np.stack([orig[src], orig[G.fwd_adj]], 1)
+    if near == 0:
+        assert np.array_equal(e, ref)
+    else:
+        assert e.shape == ref.shape
+    aux = open(j("aux.csv")).read().splitlines()
+    assert aux[0].startswith("node_id,type,year,pa_weight")
+    assert len(aux) == 1 + G.n
+    row = aux[1].split(",")
+    assert row[1] == "seed" and row[-1] == "no_generators" and len(row) == 15
+    row = aux[-1].split(",")
+    assert row[1] == "agent" and len(row) == 15 and int(row[11]) == m
