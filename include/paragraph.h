@@ -89,7 +89,9 @@ enum {
     PG_K_NB_DRAW = 3,          /* root dedup + neighbourhood draws + refill */
     PG_K_ER = 4,               /* ER count + write */
     PG_K_CSR_APPEND = 5,       /* forward append, radix group, backward merge */
-    PG_K_COUNT = 6
+    PG_K_LIK = 6,              /* pg_pa_loglik evaluations (all passes) */
+    PG_K_LIK_GRAD = 7,         /* pg_pa_loglik_grad evaluations (all passes) */
+    PG_K_COUNT = 8
 };
 
 /* Returns PG_ABI_VERSION. */
@@ -192,11 +194,94 @@ int pg_snapshot_restore(pg_ctx* ctx);
  *   NB_DRAW      20 B per new paper + 8 B per citation slot;
  *   ER           4 B per emitted target + 4 B per new paper;
  *   CSR_APPEND   4*(dE+dN) forward write + 4*(E_old+N_old) read +
- *                4*(E_new+N_new) write + 8 B per new edge (degree RMW).
+ *                4*(E_new+N_new) write + 8 B per new edge (degree RMW);
+ *   LIK          4 B per paper-year (in-degree snapshot) + 8 B per paper
+ *                (birth, log-fitness) + 4 B per citing paper (count) +
+ *                17 B per filled slot (target, kind, 12 B gathered);
+ *   LIK_GRAD     two paper passes: 8 B per paper-year + 40 B per paper
+ *                (2x birth and log-fitness, dL/dphi written then updated)
+ *                + 4 B per citing paper + 33 B per filled slot (17 B as
+ *                LIK + 16 B dL/dphi atomic update).
  */
 int pg_profile_enable(pg_ctx* ctx, int on);
 int pg_profile_read(pg_ctx* ctx, int n, double* ms, double* bytes, int64_t* launches);
 int pg_profile_reset(pg_ctx* ctx);
+
+/*
+ * ---- Citation sequences and the PA likelihood ----
+ * No reference counterpart.  A context holds one citation sequence on the
+ * device: per year y, the in-degree snapshot of the papers [0, N_y) that
+ * exist at the start of the year and the year's PA-model draws (m slots per
+ * new paper, in draw order, kinds 0 PA / 1 neighbourhood / 2 PA refill, as
+ * pg_year_citations returns them).  Recording fills it from pg_year_step;
+ * pg_seq_append_year fills it from host arrays.
+ *
+ * log L = sum over slots of kind 0 or 2 of  x_t - log(Z_y - P)
+ *   x_j = alpha*log(k_j+1) - (year_y - birth_j)/tau + phi_j,  w_j = exp(x_j)
+ *   Z_y = sum_{j < N_y} w_j,  P = sum of w over the same paper's earlier slots
+ * with x and w in FP32 (the simulator's weight arithmetic, alpha and 1/tau
+ * rounded to float) and Z, P, log L and all gradients in FP64.
+ */
+typedef struct {
+    uint32_t size;
+    int32_t n_years;
+    int32_t m;                 /* slots per new paper */
+    int64_t n_papers;          /* papers the sequence refers to (phi length) */
+    int64_t paper_years;       /* sum over years of N_y */
+    int64_t new_papers;        /* sum over years of the new papers */
+} pg_seq_info;
+
+typedef struct {
+    uint32_t size;
+    double alpha;              /* used when alpha_tau_dev is NULL */
+    double tau;                /* years, > 0 */
+    const double* alpha_tau_dev; /* optional DEVICE {alpha, tau}: read by the
+                                    kernels, so no host round trip */
+    const float* logfit_dev;   /* DEVICE float[n_papers]: phi, the parameters */
+    void* stream;              /* hipStream_t to run on; NULL = the context's */
+} pg_lik_params;
+
+typedef struct {
+    uint32_t size;
+    double* scalars_dev;       /* optional DEVICE double[3]: log L, dL/dalpha,
+                                  dL/dtau (the last two 0 for pg_pa_loglik) */
+    double* grad_logfit_dev;   /* pg_pa_loglik_grad: DEVICE double[n_papers],
+                                  dL/dphi (overwritten) */
+    double* host3;             /* optional HOST double[3] as scalars_dev; when
+                                  set, the call waits for the result */
+} pg_lik_out;
+
+/* Allocates (or re-allocates) and clears the context's sequence store. */
+int pg_seq_reserve(pg_ctx* ctx, int32_t m, int64_t max_paper_years,
+                   int64_t max_new_papers, int64_t max_papers);
+/* on != 0: every later PA pg_year_step appends its year (the in-degree
+ * snapshot before the step, the draws after it, the births).  ER years are
+ * refused while recording. */
+int pg_seq_record(pg_ctx* ctx, int on);
+/* Appends one year from host arrays: deg[n_exist], cit/kind[n_new*m],
+ * cnt[n_new].  Targets must be in [0, n_exist), kinds in {0,1,2}, counts in
+ * [0, m]. */
+int pg_seq_append_year(pg_ctx* ctx, int32_t year, int64_t n_exist, int64_t n_new,
+                       const int32_t* deg, const int32_t* cit, const int8_t* kind,
+                       const int32_t* cnt);
+/* Births of papers [0, n_papers) (host array); n_papers must cover every
+ * year's N_y + new papers.  Recording sets them itself. */
+int pg_seq_set_birth(pg_ctx* ctx, int64_t n_papers, const int32_t* birth);
+int pg_seq_get_info(pg_ctx* ctx, pg_seq_info* info);
+/* Copies the sequence to host buffers sized from pg_seq_info (any may be
+ * NULL): per-year tables, then deg[paper_years], cit/kind[new_papers*m],
+ * cnt[new_papers], birth[n_papers]. */
+int pg_seq_download(pg_ctx* ctx, int64_t* n_exist, int64_t* n_new, int32_t* year_of,
+                    int32_t* deg, int32_t* cit, int8_t* kind, int32_t* cnt,
+                    int32_t* birth);
+/* log L of the sequence.  Asynchronous on the chosen stream unless
+ * out->host3 is set.  Calls on one context must not overlap. */
+int pg_pa_loglik(pg_ctx* ctx, const pg_lik_params* p, pg_lik_out* out);
+/* log L and its gradient.  The backward paper pass recomputes every weight
+ * w_j(y) and the partition sums are recomputed per call (nothing is kept
+ * from pg_pa_loglik).  dL/dphi uses FP64 atomics (device memory only); log L
+ * and the scalar gradients are reduced in a fixed order. */
+int pg_pa_loglik_grad(pg_ctx* ctx, const pg_lik_params* p, pg_lik_out* out);
 
 /* Ceil growth rule of the reference (src/abm.cu:233-240,
  * src/kernel.cu:1690): ceil(n * rate) in double precision. */

@@ -79,6 +79,10 @@ def lib() -> ctypes.CDLL:
                                                  _i64p, _i32p, _i8p, _i64p, _i32p, _i64p,
                                                  ctypes.c_int32, _i32p, _f32p, ctypes.c_double,
                                                  ctypes.c_double, _f64p, _f64p, _f64p, _f64p]),
+            "or_pa_loglik_grad_w32": (ctypes.c_int, [ctypes.c_int32, _i64p, _i64p, _i32p, _i32p,
+                                                     _i64p, _i32p, _i8p, _i64p, _i32p, _i64p,
+                                                     ctypes.c_int32, _i32p, _f32p, ctypes.c_double,
+                                                     ctypes.c_double, _f64p, _f64p, _f64p, _f64p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -238,8 +242,11 @@ def pa_compact(res: dict) -> tuple[np.ndarray, np.ndarray]:
 
 
 def loglik_grad(n_exist, n_new, year_of, deg_list, cit_list, kind_list, cnt_list, m, birth,
-                logfit, alpha, tau):
-    """FP64 log-likelihood and gradient over per-year arrays (lists of numpy)."""
+                logfit, alpha, tau, w32=False):
+    """Log-likelihood and gradient over per-year arrays (lists of numpy).
+
+    w32=False: FP64 throughout.  w32=True: the GPU likelihood's arithmetic
+    (FP32 weights, FP64 sums)."""
     Y = len(n_exist)
     ne = np.ascontiguousarray(n_exist, np.int64)
     nn = np.ascontiguousarray(n_new, np.int64)
@@ -259,14 +266,15 @@ def loglik_grad(n_exist, n_new, year_of, deg_list, cit_list, kind_list, cnt_list
     ll, ga, gt = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
     birth = np.ascontiguousarray(birth, np.int32)
     logfit = np.ascontiguousarray(logfit, np.float32)
-    rc = lib().or_pa_loglik_grad(Y, _p(ne, ctypes.c_int64), _p(nn, ctypes.c_int64),
-                                 _p(yo, ctypes.c_int32), _p(deg, ctypes.c_int32),
-                                 _p(deg_off, ctypes.c_int64), _p(cit, ctypes.c_int32),
-                                 _p(kind, ctypes.c_int8), _p(ev_off, ctypes.c_int64),
-                                 _p(cnt, ctypes.c_int32), _p(cnt_off, ctypes.c_int64), m,
-                                 _p(birth, ctypes.c_int32), _p(logfit, ctypes.c_float),
-                                 alpha, tau, ctypes.byref(ll), ctypes.byref(ga),
-                                 ctypes.byref(gt), _p(gphi, ctypes.c_double))
+    fn = lib().or_pa_loglik_grad_w32 if w32 else lib().or_pa_loglik_grad
+    rc = fn(Y, _p(ne, ctypes.c_int64), _p(nn, ctypes.c_int64),
+            _p(yo, ctypes.c_int32), _p(deg, ctypes.c_int32),
+            _p(deg_off, ctypes.c_int64), _p(cit, ctypes.c_int32),
+            _p(kind, ctypes.c_int8), _p(ev_off, ctypes.c_int64),
+            _p(cnt, ctypes.c_int32), _p(cnt_off, ctypes.c_int64), m,
+            _p(birth, ctypes.c_int32), _p(logfit, ctypes.c_float),
+            alpha, tau, ctypes.byref(ll), ctypes.byref(ga),
+            ctypes.byref(gt), _p(gphi, ctypes.c_double))
     if rc:
         raise RuntimeError(f"or_pa_loglik_grad failed: {rc}")
     return ll.value, ga.value, gt.value, gphi

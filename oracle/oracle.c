@@ -577,27 +577,36 @@ int or_csr_append(const or_csr* g, int64_t n_new, const int32_t* new_off,
 }
 
 /* ---------------------------------------------------------------------- */
-/* PA log-likelihood and gradient in FP64 (no reference counterpart).       */
+/* PA log-likelihood and gradient (no reference counterpart).               */
 /* log L = sum over PA draws d of [x_t - log(Z_y - sum_{s in B_d} w_s)],    */
 /* x_i = alpha*log(k_i+1) - age_i/tau + phi_i, w = exp(x), B_d = papers the */
 /* same citing paper cited before draw d.                                   */
+/* w32 = 0: every quantity in FP64 (libm exp/log): the smooth function that */
+/*          finite differences check.                                       */
+/* w32 = 1: L, x and w are the simulator's FP32 values (weight_one's        */
+/*          arithmetic with alpha and 1/tau rounded to float); sums,        */
+/*          partitions and gradients stay FP64 and log(D) is or_log.  This  */
+/*          is the GPU likelihood's arithmetic, up to summation order.      */
 /* ---------------------------------------------------------------------- */
-int or_pa_loglik_grad(int32_t n_years, const int64_t* n_exist,
-                      const int64_t* n_new, const int32_t* year_of,
-                      const int32_t* deg, const int64_t* deg_off,
-                      const int32_t* cit, const int8_t* kind,
-                      const int64_t* ev_off, const int32_t* cnt,
-                      const int64_t* cnt_off, int32_t m,
-                      const int32_t* birth, const float* logfit, double alpha,
-                      double tau, double* loglik, double* grad_alpha,
-                      double* grad_tau, double* grad_phi) {
+static int loglik_impl(int32_t n_years, const int64_t* n_exist,
+                       const int64_t* n_new, const int32_t* year_of,
+                       const int32_t* deg, const int64_t* deg_off,
+                       const int32_t* cit, const int8_t* kind,
+                       const int64_t* ev_off, const int32_t* cnt,
+                       const int64_t* cnt_off, int32_t m,
+                       const int32_t* birth, const float* logfit, double alpha,
+                       double tau, int w32, double* loglik, double* grad_alpha,
+                       double* grad_tau, double* grad_phi) {
     int64_t nmax = 0;
     for (int y = 0; y < n_years; ++y)
         if (n_exist[y] + n_new[y] > nmax) nmax = n_exist[y] + n_new[y];
-    double* w = (double*)malloc((size_t)nmax * sizeof(double));
-    double* L = (double*)malloc((size_t)nmax * sizeof(double));
+    double* w = (double*)malloc((size_t)(nmax > 0 ? nmax : 1) * sizeof(double));
+    double* L = (double*)malloc((size_t)(nmax > 0 ? nmax : 1) * sizeof(double));
+    double* X = (double*)malloc((size_t)(nmax > 0 ? nmax : 1) * sizeof(double));
     double* later = (double*)malloc((size_t)(m > 0 ? m : 1) * sizeof(double));
     memset(grad_phi, 0, (size_t)nmax * sizeof(double));
+    const float alpha_f = (float)alpha;
+    const float inv_tau_f = (float)(1.0 / tau);
     double ll = 0.0, ga = 0.0, gt = 0.0;
     const double tau2 = tau * tau;
     for (int y = 0; y < n_years; ++y) {
@@ -605,9 +614,19 @@ int or_pa_loglik_grad(int32_t n_years, const int64_t* n_exist,
         const int32_t* k = deg + deg_off[y];
         double Z = 0.0, SL = 0.0, Sa = 0.0;
         for (int64_t j = 0; j < N; ++j) {
-            double age = (double)(year_of[y] - birth[j]);
-            L[j] = log((double)k[j] + 1.0);
-            w[j] = exp(alpha * L[j] - age / tau + (double)logfit[j]);
+            const int32_t age_i = year_of[y] - birth[j];
+            const double age = (double)age_i;
+            if (w32) {
+                const float lf = or_logf((float)(k[j] + 1));
+                const float xf = fmaf(alpha_f, lf, fmaf(-(float)age_i, inv_tau_f, logfit[j]));
+                L[j] = (double)lf;
+                X[j] = (double)xf;
+                w[j] = (double)or_expf(xf);
+            } else {
+                L[j] = log((double)k[j] + 1.0);
+                X[j] = alpha * L[j] - age / tau + (double)logfit[j];
+                w[j] = exp(X[j]);
+            }
             Z += w[j]; SL += w[j] * L[j]; Sa += w[j] * age;
         }
         double A = 0.0;
@@ -622,8 +641,7 @@ int or_pa_loglik_grad(int32_t n_years, const int64_t* n_exist,
                 const double age_t = (double)(year_of[y] - birth[t]);
                 if (kr[j] != 1) {
                     double D = Z - Pw;
-                    double xt = alpha * L[t] - age_t / tau + (double)logfit[t];
-                    ll += xt - log(D);
+                    ll += X[t] - (w32 ? or_log(D) : log(D));
                     A += 1.0 / D;
                     ga += L[t] + PwL / D;
                     gt += age_t / tau2 + Pwa / (tau2 * D);
@@ -639,11 +657,38 @@ int or_pa_loglik_grad(int32_t n_years, const int64_t* n_exist,
         for (int64_t j = 0; j < N; ++j) grad_phi[j] -= A * w[j];
     }
     *loglik = ll; *grad_alpha = ga; *grad_tau = gt;
-    free(w); free(L); free(later);
+    free(w); free(L); free(X); free(later);
     return 0;
 }
 
-/* Vector forms of the elementary functions (for the ulp sweeps in tests). */
+int or_pa_loglik_grad(int32_t n_years, const int64_t* n_exist,
+                      const int64_t* n_new, const int32_t* year_of,
+                      const int32_t* deg, const int64_t* deg_off,
+                      const int32_t* cit, const int8_t* kind,
+                      const int64_t* ev_off, const int32_t* cnt,
+                      const int64_t* cnt_off, int32_t m,
+                      const int32_t* birth, const float* logfit, double alpha,
+                      double tau, double* loglik, double* grad_alpha,
+                      double* grad_tau, double* grad_phi) {
+    return loglik_impl(n_years, n_exist, n_new, year_of, deg, deg_off, cit, kind, ev_off, cnt,
+                       cnt_off, m, birth, logfit, alpha, tau, 0, loglik, grad_alpha, grad_tau,
+                       grad_phi);
+}
+
+int or_pa_loglik_grad_w32(int32_t n_years, const int64_t* n_exist,
+                          const int64_t* n_new, const int32_t* year_of,
+                          const int32_t* deg, const int64_t* deg_off,
+                          const int32_t* cit, const int8_t* kind,
+                          const int64_t* ev_off, const int32_t* cnt,
+                          const int64_t* cnt_off, int32_t m,
+                          const int32_t* birth, const float* logfit, double alpha,
+                          double tau, double* loglik, double* grad_alpha,
+                          double* grad_tau, double* grad_phi) {
+    return loglik_impl(n_years, n_exist, n_new, year_of, deg, deg_off, cit, kind, ev_off, cnt,
+                       cnt_off, m, birth, logfit, alpha, tau, 1, loglik, grad_alpha, grad_tau,
+                       grad_phi);
+}
+
 void or_vec_expf(int64_t n, const float* x, float* y) {
     for (int64_t i = 0; i < n; ++i) y[i] = or_expf(x[i]);
 }

@@ -127,6 +127,19 @@ int or_pa_loglik_grad(int32_t n_years, const int64_t* n_exist,
                       double tau, double* loglik, double* grad_alpha,
                       double* grad_tau, double* grad_phi);
 
+/* Same sequence and outputs, with the GPU likelihood's arithmetic: L, x and
+ * w are FP32 (the simulator's weight formula with alpha and 1/tau rounded to
+ * float), partition sums and gradients FP64, log(D) = or_log. */
+int or_pa_loglik_grad_w32(int32_t n_years, const int64_t* n_exist,
+                          const int64_t* n_new, const int32_t* year_of,
+                          const int32_t* deg, const int64_t* deg_off,
+                          const int32_t* cit, const int8_t* kind,
+                          const int64_t* ev_off, const int32_t* cnt,
+                          const int64_t* cnt_off, int32_t m,
+                          const int32_t* birth, const float* logfit, double alpha,
+                          double tau, double* loglik, double* grad_alpha,
+                          double* grad_tau, double* grad_phi);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,7 +52,25 @@ class YearStats(ctypes.Structure):
                 ("pa_attempts", ctypes.c_int64), ("device_ms", ctypes.c_double)]
 
 
-KERNEL_GROUPS = ["weights_cdf", "pa_draw", "bfs", "nb_draw", "er", "csr_append"]
+class SeqInfo(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("n_years", ctypes.c_int32), ("m", ctypes.c_int32),
+                ("n_papers", ctypes.c_int64), ("paper_years", ctypes.c_int64),
+                ("new_papers", ctypes.c_int64)]
+
+
+class LikParams(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("alpha", ctypes.c_double), ("tau", ctypes.c_double),
+                ("alpha_tau_dev", ctypes.c_void_p), ("logfit_dev", ctypes.c_void_p),
+                ("stream", ctypes.c_void_p)]
+
+
+class LikOut(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("scalars_dev", ctypes.c_void_p),
+                ("grad_logfit_dev", ctypes.c_void_p), ("host3", ctypes.c_void_p)]
+
+
+KERNEL_GROUPS = ["weights_cdf", "pa_draw", "bfs", "nb_draw", "er", "csr_append", "lik",
+                 "lik_grad"]
 
 
 _lib = None
@@ -64,6 +82,16 @@ def lib() -> ctypes.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        # One HIP runtime per process.  PyTorch-ROCm ships its own
+        # libamdhip64/libhsa-runtime64; loaded first, it satisfies this
+        # library's libamdhip64.so.7 dependency, so torch tensors and this
+        # library share one runtime.  Loaded the other way round, the process
+        # would map a second runtime from /opt/rocm and torch would find no
+        # device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         P = ctypes.POINTER
         i32p, i64p = P(ctypes.c_int32), P(ctypes.c_int64)
@@ -94,6 +122,17 @@ def lib() -> ctypes.CDLL:
                                              i32p]),
             "pg_seed_logfit": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64,
                                               ctypes.c_double, ctypes.c_double, f32p]),
+            "pg_seq_reserve": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64,
+                                              ctypes.c_int64]),
+            "pg_seq_record": (ctypes.c_int, [vp, ctypes.c_int]),
+            "pg_seq_append_year": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int64,
+                                                  ctypes.c_int64, i32p, i32p, i8p, i32p]),
+            "pg_seq_set_birth": (ctypes.c_int, [vp, ctypes.c_int64, i32p]),
+            "pg_seq_get_info": (ctypes.c_int, [vp, P(SeqInfo)]),
+            "pg_seq_download": (ctypes.c_int, [vp, i64p, i64p, i32p, i32p, i32p, i8p, i32p,
+                                               i32p]),
+            "pg_pa_loglik": (ctypes.c_int, [vp, P(LikParams), P(LikOut)]),
+            "pg_pa_loglik_grad": (ctypes.c_int, [vp, P(LikParams), P(LikOut)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -280,6 +319,85 @@ class Context:
                                  _p(out, ctypes.c_int32), _p(ln, ctypes.c_int32)), "pg_bfs")
         out = out.reshape(-1, cap)
         return [out[i, :ln[i]].copy() for i in range(roots.shape[0])]
+
+    # ---- citation sequence and PA likelihood ----
+    def seq_reserve(self, m: int, max_paper_years: int, max_new_papers: int,
+                    max_papers: int | None = None):
+        mp = self.cfg.max_papers if max_papers is None else max_papers
+        self._check(lib().pg_seq_reserve(self._h, m, max_paper_years, max_new_papers, mp),
+                    "pg_seq_reserve")
+
+    def seq_record(self, on: bool = True):
+        self._check(lib().pg_seq_record(self._h, 1 if on else 0), "pg_seq_record")
+
+    def seq_append_year(self, year: int, deg: np.ndarray, cit: np.ndarray, kind: np.ndarray,
+                        cnt: np.ndarray):
+        deg = np.ascontiguousarray(deg, np.int32)
+        cit = np.ascontiguousarray(cit, np.int32)
+        kind = np.ascontiguousarray(kind, np.int8)
+        cnt = np.ascontiguousarray(cnt, np.int32)
+        self._check(lib().pg_seq_append_year(self._h, year, deg.shape[0], cnt.shape[0],
+                                             _p(deg, ctypes.c_int32), _p(cit, ctypes.c_int32),
+                                             _p(kind, ctypes.c_int8), _p(cnt, ctypes.c_int32)),
+                    "pg_seq_append_year")
+
+    def seq_set_birth(self, birth: np.ndarray):
+        birth = np.ascontiguousarray(birth, np.int32)
+        self._check(lib().pg_seq_set_birth(self._h, birth.shape[0], _p(birth, ctypes.c_int32)),
+                    "pg_seq_set_birth")
+
+    def seq_info(self) -> SeqInfo:
+        info = SeqInfo(ctypes.sizeof(SeqInfo))
+        self._check(lib().pg_seq_get_info(self._h, ctypes.byref(info)), "pg_seq_get_info")
+        return info
+
+    def seq_download(self) -> dict:
+        """The sequence as per-year lists (the oracle likelihood's inputs)."""
+        info = self.seq_info()
+        Y, m = info.n_years, info.m
+        n_exist = np.zeros(Y, np.int64)
+        n_new = np.zeros(Y, np.int64)
+        year_of = np.zeros(Y, np.int32)
+        deg = np.zeros(info.paper_years, np.int32)
+        cit = np.zeros(info.new_papers * m, np.int32)
+        kind = np.zeros(info.new_papers * m, np.int8)
+        cnt = np.zeros(info.new_papers, np.int32)
+        birth = np.zeros(info.n_papers, np.int32)
+        self._check(lib().pg_seq_download(self._h, _p(n_exist, ctypes.c_int64),
+                                          _p(n_new, ctypes.c_int64), _p(year_of, ctypes.c_int32),
+                                          _p(deg, ctypes.c_int32), _p(cit, ctypes.c_int32),
+                                          _p(kind, ctypes.c_int8), _p(cnt, ctypes.c_int32),
+                                          _p(birth, ctypes.c_int32)), "pg_seq_download")
+        out = dict(n_exist=list(n_exist), n_new=list(n_new), year_of=list(year_of), deg=[],
+                   cit=[], kind=[], cnt=[], birth=birth, m=m)
+        d0 = e0 = 0
+        for y in range(Y):
+            N, D = int(n_exist[y]), int(n_new[y])
+            out["deg"].append(deg[d0:d0 + N])
+            out["cit"].append(cit[e0 * m:(e0 + D) * m].reshape(D, m))
+            out["kind"].append(kind[e0 * m:(e0 + D) * m].reshape(D, m))
+            out["cnt"].append(cnt[e0:e0 + D])
+            d0 += N
+            e0 += D
+        return out
+
+    def _lik(self, fn: str, alpha: float, tau: float, logfit_dev: int, *, grad_dev: int = 0,
+             scalars_dev: int = 0, alpha_tau_dev: int = 0, stream: int = 0, host: bool = True):
+        p = LikParams(ctypes.sizeof(LikParams), float(alpha), float(tau), alpha_tau_dev or None,
+                      logfit_dev or None, stream or None)
+        h3 = (ctypes.c_double * 3)()
+        o = LikOut(ctypes.sizeof(LikOut), scalars_dev or None, grad_dev or None,
+                   ctypes.cast(h3, ctypes.c_void_p) if host else None)
+        self._check(getattr(lib(), fn)(self._h, ctypes.byref(p), ctypes.byref(o)), fn)
+        return (h3[0], h3[1], h3[2]) if host else None
+
+    def pa_loglik(self, alpha: float, tau: float, logfit_dev: int, **kw):
+        """log L (device pointers as ints); returns (log L, 0, 0) when host=True."""
+        return self._lik("pg_pa_loglik", alpha, tau, logfit_dev, **kw)
+
+    def pa_loglik_grad(self, alpha: float, tau: float, logfit_dev: int, grad_dev: int, **kw):
+        """log L, dL/dalpha, dL/dtau (host=True) and dL/dphi into grad_dev (FP64)."""
+        return self._lik("pg_pa_loglik_grad", alpha, tau, logfit_dev, grad_dev=grad_dev, **kw)
 
     def snapshot_save(self):
         self._check(lib().pg_snapshot_save(self._h), "pg_snapshot_save")

@@ -21,98 +21,15 @@
 #include <vector>
 
 #include "../../include/paragraph.h"
+#include "pg_ctx.h"
 #include "pg_internal.h"
 
 using namespace pg;
 
-struct pg_ctx {
-    pg_config cfg{};
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::string err;
-    bool loaded = false;
-    int64_t n = 0, e = 0, n_logfit = 0;
-    int32_t* birth = nullptr;
-    int32_t* in_deg = nullptr;
-    float* logfit = nullptr;
-    int32_t* fwd_off = nullptr;
-    int32_t* fwd_adj = nullptr;
-    int32_t* bwd_off[2] = {nullptr, nullptr};
-    int32_t* bwd_adj[2] = {nullptr, nullptr};
-    int cur = 0;
-    float* w = nullptr;
-    double* cdf = nullptr;
-    double* scan_agg = nullptr;
-    int32_t* cit = nullptr;
-    int8_t* kind = nullptr;
-    int32_t* cnt = nullptr;
-    int32_t* near = nullptr;
-    int32_t* anext = nullptr;
-    int64_t* new_off = nullptr;
-    int32_t* add = nullptr;
-    int64_t year_edge_cap = 0;
-    int32_t* src_of = nullptr;
-    uint32_t* k_sorted = nullptr;
-    uint32_t* v_sorted = nullptr;
-    uint32_t* k_tmp = nullptr;
-    uint32_t* v_tmp = nullptr;
-    int64_t* scratch = nullptr;
-    unsigned long long* d_sum = nullptr;
-    int32_t* d_tmp_i32 = nullptr;
-    int64_t d_tmp_cap = 0;
-    int64_t last_n_new = 0, last_n_exist = 0;
-    int last_m = 0, last_model = -1;
-    // per-kernel-group profiling
-    bool prof = false;
-    hipEvent_t pev[PG_K_COUNT][2] = {};
-    hipEvent_t pev2[PG_K_COUNT][2] = {};
-    double prof_ms[PG_K_COUNT] = {};
-    double prof_bytes[PG_K_COUNT] = {};
-    int64_t prof_calls[PG_K_COUNT] = {};
-    // device snapshot of the graph state
-    bool snap = false;
-    int64_t snap_n = 0, snap_e = 0;
-    int32_t* s_birth = nullptr;
-    int32_t* s_in_deg = nullptr;
-    int32_t* s_fwd_off = nullptr;
-    int32_t* s_fwd_adj = nullptr;
-    int32_t* s_bwd_off = nullptr;
-    int32_t* s_bwd_adj = nullptr;
-    // neighbourhoods of the year's distinct roots
-    int32_t* S_buf = nullptr;
-    uint64_t* Q_buf = nullptr;
-    int32_t* ns_buf = nullptr;
-    uint32_t* rk[6] = {};          // root keys/vals: in, sorted, tmp
-    int32_t* r_flags = nullptr;
-    int64_t* r_excl = nullptr;
-    int32_t* root_slot = nullptr;
-    int32_t* uroot = nullptr;
-    int32_t* big_list = nullptr;
-};
 
 namespace {
 
 std::string g_create_err;
-
-#define PG_HIP(expr)                                                                    \
-    do {                                                                                \
-        hipError_t _e = (expr);                                                         \
-        if (_e != hipSuccess) {                                                         \
-            ctx->err = std::string(#expr) + ": " + hipGetErrorString(_e);               \
-            return PG_EHIP;                                                             \
-        }                                                                               \
-    } while (0)
-
-int fail(pg_ctx* ctx, int code, const std::string& msg) {
-    ctx->err = msg;
-    return code;
-}
-
-template <typename T>
-hipError_t dalloc(T** p, int64_t count) {
-    if (count <= 0) count = 1;
-    return hipMalloc(reinterpret_cast<void**>(p), (size_t)count * sizeof(T));
-}
 
 void release(pg_ctx* c) {
     void* ptrs[] = {c->birth, c->in_deg, c->logfit, c->fwd_off, c->fwd_adj, c->bwd_off[0],
@@ -134,6 +51,7 @@ void release(pg_ctx* c) {
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    seq_release(c);
 }
 
 int64_t scratch_need(int64_t max_papers, int64_t max_new, int64_t year_edges) {
@@ -382,6 +300,10 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
         return fail(ctx, PG_EINVAL, "pg_year_step: er_p must be in [0,1]");
     }
     PG_HIP(hipSetDevice(ctx->cfg.device));
+    {
+        int rc = seq_record_snapshot(ctx, p->year, N, D, p->m, p->model);
+        if (rc) return rc;
+    }
     hipStream_t s = ctx->stream;
     const uint64_t seed = ctx->cfg.seed;
     const uint32_t yi = (uint32_t)p->year_index;
@@ -479,6 +401,10 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
         ctx->cur = nxt;
     }
     PG_HIP(hipEventRecord(ctx->ev1, s));
+    {
+        int rc = seq_record_draws(ctx, e_year);
+        if (rc) return rc;
+    }
     PG_HIP(hipEventSynchronize(ctx->ev1));
     float ms = 0.0f;
     PG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));

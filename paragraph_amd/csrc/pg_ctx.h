@@ -1,0 +1,149 @@
+// pg_ctx.h -- the context object behind the C ABI (private to the library).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/paragraph.h"
+
+// A citation sequence held on the device for the PA likelihood: per year,
+// the in-degree snapshot at the start of the year and the year's draws
+// (m slots per new paper, in draw order).
+struct pg_seq_store {
+    bool reserved = false;
+    int32_t m = 0;
+    int64_t cap_paper_years = 0, cap_new = 0, cap_papers = 0;
+    // host copies of the per-year tables
+    std::vector<int64_t> n_exist, n_new, deg_off, new_off;
+    std::vector<int32_t> year_of;
+    int64_t n_papers = 0;        // papers in the sequence (ids 0..n_papers-1)
+    int64_t paper_years = 0;     // sum of n_exist
+    int64_t new_total = 0;       // sum of n_new
+    int64_t filled = 0;          // filled slots (sum of cnt)
+    // the year pg_year_step has snapshotted and whose draws follow
+    bool pending = false;
+    int32_t pending_year = 0;
+    int64_t pending_n_exist = 0, pending_n_new = 0;
+    // device arrays
+    int32_t* deg = nullptr;      // [cap_paper_years] in-degree snapshots
+    int32_t* cit = nullptr;      // [cap_new * m] targets, draw order
+    int8_t* kind = nullptr;      // [cap_new * m] 0 PA, 1 neighbourhood, 2 PA refill
+    int32_t* cnt = nullptr;      // [cap_new] filled slots per new paper
+    int32_t* birth = nullptr;    // [cap_papers]
+    // device copies of the per-year tables and the launch tables
+    bool tables_dirty = true;
+    int64_t* d_n_exist = nullptr;
+    int64_t* d_n_new = nullptr;
+    int64_t* d_deg_off = nullptr;
+    int64_t* d_new_off = nullptr;
+    int32_t* d_year_of = nullptr;
+    int32_t* d_ev_block = nullptr;   // event block -> year, then first new paper
+    int32_t* d_ev_year_blk = nullptr;// year -> first event block (n_years+1)
+    int64_t n_ev_blocks = 0;
+    int32_t ev_group = 0;            // lanes per new paper in the event kernel
+    int64_t n_paper_blocks = 0;
+    double* part = nullptr;          // per-block partial sums
+    int64_t part_cap = 0;
+    double* year_sums = nullptr;     // per year: Z, A
+    int64_t table_cap_years = 0;
+    int64_t table_cap_blocks = 0;
+    double* d_scalars = nullptr;     // loglik, d/dalpha, d/dtau
+};
+
+struct pg_ctx {
+    pg_config cfg{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    bool loaded = false;
+    int64_t n = 0, e = 0, n_logfit = 0;
+    int32_t* birth = nullptr;
+    int32_t* in_deg = nullptr;
+    float* logfit = nullptr;
+    int32_t* fwd_off = nullptr;
+    int32_t* fwd_adj = nullptr;
+    int32_t* bwd_off[2] = {nullptr, nullptr};
+    int32_t* bwd_adj[2] = {nullptr, nullptr};
+    int cur = 0;
+    float* w = nullptr;
+    double* cdf = nullptr;
+    double* scan_agg = nullptr;
+    int32_t* cit = nullptr;
+    int8_t* kind = nullptr;
+    int32_t* cnt = nullptr;
+    int32_t* near = nullptr;
+    int32_t* anext = nullptr;
+    int64_t* new_off = nullptr;
+    int32_t* add = nullptr;
+    int64_t year_edge_cap = 0;
+    int32_t* src_of = nullptr;
+    uint32_t* k_sorted = nullptr;
+    uint32_t* v_sorted = nullptr;
+    uint32_t* k_tmp = nullptr;
+    uint32_t* v_tmp = nullptr;
+    int64_t* scratch = nullptr;
+    unsigned long long* d_sum = nullptr;
+    int32_t* d_tmp_i32 = nullptr;
+    int64_t d_tmp_cap = 0;
+    int64_t last_n_new = 0, last_n_exist = 0;
+    int last_m = 0, last_model = -1;
+    // per-kernel-group profiling
+    bool prof = false;
+    hipEvent_t pev[PG_K_COUNT][2] = {};
+    hipEvent_t pev2[PG_K_COUNT][2] = {};
+    double prof_ms[PG_K_COUNT] = {};
+    double prof_bytes[PG_K_COUNT] = {};
+    int64_t prof_calls[PG_K_COUNT] = {};
+    // device snapshot of the graph state
+    bool snap = false;
+    int64_t snap_n = 0, snap_e = 0;
+    int32_t* s_birth = nullptr;
+    int32_t* s_in_deg = nullptr;
+    int32_t* s_fwd_off = nullptr;
+    int32_t* s_fwd_adj = nullptr;
+    int32_t* s_bwd_off = nullptr;
+    int32_t* s_bwd_adj = nullptr;
+    // neighbourhoods of the year's distinct roots
+    int32_t* S_buf = nullptr;
+    uint64_t* Q_buf = nullptr;
+    int32_t* ns_buf = nullptr;
+    uint32_t* rk[6] = {};          // root keys/vals: in, sorted, tmp
+    int32_t* r_flags = nullptr;
+    int64_t* r_excl = nullptr;
+    int32_t* root_slot = nullptr;
+    int32_t* uroot = nullptr;
+    int32_t* big_list = nullptr;
+    // citation-sequence store for the likelihood; recording appends each PA year
+    pg_seq_store seq;
+    bool seq_recording = false;
+};
+
+// Host helpers shared by the ABI source files.
+#define PG_HIP(expr)                                                                    \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess) {                                                         \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(_e);               \
+            return PG_EHIP;                                                             \
+        }                                                                               \
+    } while (0)
+
+inline int fail(pg_ctx* ctx, int code, const std::string& msg) {
+    ctx->err = msg;
+    return code;
+}
+
+template <typename T>
+inline hipError_t dalloc(T** p, int64_t count) {
+    if (count <= 0) count = 1;
+    return hipMalloc(reinterpret_cast<void**>(p), (size_t)count * sizeof(T));
+}
+
+// Sequence store (pg_lik.hip): frees its device memory; appends the PA year
+// that pg_year_step is about to run (in-degree snapshot) and has just drawn.
+void seq_release(pg_ctx* ctx);
+int seq_record_snapshot(pg_ctx* ctx, int32_t year, int64_t n_exist, int64_t n_new, int32_t m,
+                        int model);
+int seq_record_draws(pg_ctx* ctx, int64_t filled);

@@ -120,4 +120,53 @@ void launch_er_write(hipStream_t s, int64_t n_exist, int64_t n_new, double p, ui
 void launch_fill_i32(hipStream_t s, int32_t* p, int32_t v, int64_t n);
 void launch_sum_i32(hipStream_t s, const int32_t* p, int64_t n, unsigned long long* out);
 
+// ---- PA likelihood (k_lik.hip) ----
+constexpr int kLikMaxYears = 256;   // years per sequence (LDS per-year partials)
+constexpr int kLikYearCols = 5;     // per year: Z, A, log L, alpha terms, tau terms
+constexpr int kLikSlotPapers = 64;  // citing papers per slot-pass block
+
+// A citation sequence in device memory.  Year y: papers [0, n_exist[y])
+// exist, with in-degrees deg[deg_off[y] + j]; citing papers new_off[y] ..
+// new_off[y] + n_new[y] - 1 (sequence-wide index p) fill cit/kind[p*m + s]
+// for s < cnt[p].  Tables n_exist .. year_of are device arrays.
+struct LikSeq {
+    int32_t n_years;
+    int32_t m;
+    int64_t n_papers;
+    const int64_t* n_exist;
+    const int64_t* n_new;
+    const int64_t* deg_off;
+    const int64_t* new_off;
+    const int32_t* year_of;
+    const int32_t* deg;
+    const int32_t* cit;
+    const int8_t* kind;
+    const int32_t* cnt;
+    const int32_t* birth;
+};
+
+struct LikArgs {
+    double alpha, tau;        // used when at_dev is null
+    const double* at_dev;     // optional device {alpha, tau}
+    const float* logfit;      // device [n_papers]
+};
+
+int lik_group_lanes(int m);
+int64_t lik_paper_blocks(int64_t n_papers);
+// Paper pass (per-block partial Z_y; zeroes grad_phi if non-null), then Z_y.
+void launch_lik_forward_sums(hipStream_t s, const LikSeq& q, const LikArgs& a, int64_t nblk_p,
+                             double* part_p, double* year_sums, double* grad_phi);
+// Slot pass (log L; with grad also A_y, the alpha/tau slot terms and the
+// dL/dphi scatter), then the per-year sums.
+void launch_lik_slots(hipStream_t s, bool grad, int G, int64_t nblk_e, const int32_t* ev_block,
+                      const int32_t* year_blk, const LikSeq& q, const LikArgs& a,
+                      double* year_sums, double* grad_phi, double* part_e);
+// Paper pass with A_y: dL/dphi -= sum_y A_y w_j(y); partials of A-weighted sums.
+void launch_lik_backward(hipStream_t s, const LikSeq& q, const LikArgs& a, int64_t nblk_p,
+                         const double* year_sums, double* grad_phi, double* part_p);
+// {log L, dL/dalpha, dL/dtau} -> out[3] (and out2[3] if non-null).
+void launch_lik_final(hipStream_t s, int n_years, bool grad, int64_t nblk_p,
+                      const double* part_p, const LikArgs& a, const double* year_sums,
+                      double* out, double* out2);
+
 }  // namespace pg

@@ -1,0 +1,404 @@
+// pg_lik.hip -- host side of the citation-sequence store and the PA
+// likelihood entry points declared in include/paragraph.h (no reference
+// counterpart).  Device layout of the store (one per context):
+//   deg   : int32[paper-years]  in-degree snapshots, year after year
+//   cit   : int32[new papers * m], kind : int8[new papers * m]
+//   cnt   : int32[new papers],  birth : int32[papers]
+//   per-year tables n_exist/n_new/deg_off/new_off (int64), year_of (int32)
+//   slot-pass block table (year, first paper) and per-year block ranges
+//   partials: Y*nP (paper pass) + 4*nE (slot pass) + 2*nP (backward pass)
+//   doubles, per-year sums [Y][5], the scalar result [3].
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/paragraph.h"
+#include "pg_ctx.h"
+#include "pg_internal.h"
+
+using namespace pg;
+
+namespace {
+
+void free_ptr(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+void seq_free_tables(pg_seq_store& q) {
+    free_ptr(q.d_n_exist); free_ptr(q.d_n_new); free_ptr(q.d_deg_off); free_ptr(q.d_new_off);
+    free_ptr(q.d_year_of); free_ptr(q.d_ev_block); free_ptr(q.d_ev_year_blk);
+    free_ptr(q.part); free_ptr(q.year_sums); free_ptr(q.d_scalars);
+    q.d_n_exist = q.d_n_new = q.d_deg_off = q.d_new_off = nullptr;
+    q.d_year_of = q.d_ev_block = q.d_ev_year_blk = nullptr;
+    q.part = q.year_sums = q.d_scalars = nullptr;
+    q.part_cap = q.table_cap_years = q.table_cap_blocks = 0;
+    q.tables_dirty = true;
+}
+
+void seq_clear(pg_seq_store& q) {
+    q.n_exist.clear(); q.n_new.clear(); q.deg_off.clear(); q.new_off.clear(); q.year_of.clear();
+    q.n_papers = q.paper_years = q.new_total = q.filled = 0;
+    q.pending = false;
+    q.tables_dirty = true;
+}
+
+// Per-year tables, slot-pass block table and partial buffers on the device.
+int seq_prepare(pg_ctx* ctx) {
+    pg_seq_store& q = ctx->seq;
+    if (!q.tables_dirty) return PG_OK;
+    const int64_t Y = (int64_t)q.n_exist.size();
+    const int64_t nP = lik_paper_blocks(q.n_papers);
+    std::vector<int32_t> ev_block, year_blk(Y + 1, 0);
+    for (int64_t y = 0; y < Y; ++y) {
+        year_blk[y] = (int32_t)(ev_block.size() / 2);
+        for (int64_t f = 0; f < q.n_new[y]; f += kLikSlotPapers) {
+            ev_block.push_back((int32_t)y);
+            ev_block.push_back((int32_t)f);
+        }
+    }
+    year_blk[Y] = (int32_t)(ev_block.size() / 2);
+    const int64_t nE = (int64_t)ev_block.size() / 2;
+    const int64_t part_need = Y * nP + 4 * nE + 2 * nP + 1;
+    if (Y + 1 > q.table_cap_years || nE > q.table_cap_blocks || part_need > q.part_cap) {
+        const int64_t ycap = std::max<int64_t>(Y + 1, 2 * q.table_cap_years);
+        const int64_t bcap = std::max<int64_t>(std::max<int64_t>(nE, 1), 2 * q.table_cap_blocks);
+        const int64_t pcap = std::max<int64_t>(part_need, 2 * q.part_cap);
+        seq_free_tables(q);
+        hipError_t r = dalloc(&q.d_n_exist, ycap);
+        if (r == hipSuccess) r = dalloc(&q.d_n_new, ycap);
+        if (r == hipSuccess) r = dalloc(&q.d_deg_off, ycap);
+        if (r == hipSuccess) r = dalloc(&q.d_new_off, ycap);
+        if (r == hipSuccess) r = dalloc(&q.d_year_of, ycap);
+        if (r == hipSuccess) r = dalloc(&q.d_ev_year_blk, ycap + 1);
+        if (r == hipSuccess) r = dalloc(&q.d_ev_block, 2 * bcap);
+        if (r == hipSuccess) r = dalloc(&q.part, pcap);
+        if (r == hipSuccess) r = dalloc(&q.year_sums, ycap * kLikYearCols);
+        if (r == hipSuccess) r = dalloc(&q.d_scalars, 4);
+        if (r != hipSuccess) {
+            seq_free_tables(q);
+            return fail(ctx, PG_ENOMEM, "likelihood tables: device allocation failed");
+        }
+        q.table_cap_years = ycap;
+        q.table_cap_blocks = bcap;
+        q.part_cap = pcap;
+    }
+    if (Y > 0) {
+        PG_HIP(hipMemcpy(q.d_n_exist, q.n_exist.data(), Y * 8, hipMemcpyHostToDevice));
+        PG_HIP(hipMemcpy(q.d_n_new, q.n_new.data(), Y * 8, hipMemcpyHostToDevice));
+        PG_HIP(hipMemcpy(q.d_deg_off, q.deg_off.data(), Y * 8, hipMemcpyHostToDevice));
+        PG_HIP(hipMemcpy(q.d_new_off, q.new_off.data(), Y * 8, hipMemcpyHostToDevice));
+        PG_HIP(hipMemcpy(q.d_year_of, q.year_of.data(), Y * 4, hipMemcpyHostToDevice));
+    }
+    PG_HIP(hipMemcpy(q.d_ev_year_blk, year_blk.data(), (Y + 1) * 4, hipMemcpyHostToDevice));
+    if (nE > 0)
+        PG_HIP(hipMemcpy(q.d_ev_block, ev_block.data(), nE * 8, hipMemcpyHostToDevice));
+    q.n_ev_blocks = nE;
+    q.n_paper_blocks = nP;
+    q.ev_group = lik_group_lanes(q.m);
+    q.tables_dirty = false;
+    return PG_OK;
+}
+
+LikSeq seq_view(const pg_seq_store& q) {
+    LikSeq v;
+    v.n_years = (int32_t)q.n_exist.size();
+    v.m = q.m;
+    v.n_papers = q.n_papers;
+    v.n_exist = q.d_n_exist;
+    v.n_new = q.d_n_new;
+    v.deg_off = q.d_deg_off;
+    v.new_off = q.d_new_off;
+    v.year_of = q.d_year_of;
+    v.deg = q.deg;
+    v.cit = q.cit;
+    v.kind = q.kind;
+    v.cnt = q.cnt;
+    v.birth = q.birth;
+    return v;
+}
+
+// Checks a new year against the store's capacities.
+int seq_check_year(pg_ctx* ctx, const char* fn, int64_t n_exist, int64_t n_new) {
+    const pg_seq_store& q = ctx->seq;
+    if (!q.reserved) return fail(ctx, PG_ESTATE, std::string(fn) + ": no sequence reserved");
+    if (n_exist < 0 || n_new < 0) return fail(ctx, PG_EINVAL, std::string(fn) + ": negative size");
+    if ((int64_t)q.n_exist.size() + 1 > kLikMaxYears)
+        return fail(ctx, PG_ECAP, std::string(fn) + ": more than 256 years");
+    if (q.paper_years + n_exist > q.cap_paper_years || q.new_total + n_new > q.cap_new ||
+        n_exist + n_new > q.cap_papers)
+        return fail(ctx, PG_ECAP, std::string(fn) + ": sequence capacity exceeded");
+    return PG_OK;
+}
+
+void seq_commit_year(pg_seq_store& q, int32_t year, int64_t n_exist, int64_t n_new) {
+    q.n_exist.push_back(n_exist);
+    q.n_new.push_back(n_new);
+    q.deg_off.push_back(q.paper_years);
+    q.new_off.push_back(q.new_total);
+    q.year_of.push_back(year);
+    q.paper_years += n_exist;
+    q.new_total += n_new;
+    q.tables_dirty = true;
+}
+
+// Algorithmic HBM bytes of one evaluation (the formulas in paragraph.h).
+double lik_bytes(const pg_seq_store& q, bool grad) {
+    const double PY = (double)q.paper_years, P = (double)q.n_papers, NP = (double)q.new_total,
+                 S = (double)q.filled;
+    return grad ? 8.0 * PY + 40.0 * P + 4.0 * NP + 33.0 * S
+                : 4.0 * PY + 8.0 * P + 4.0 * NP + 17.0 * S;
+}
+
+int lik_eval(pg_ctx* ctx, const pg_lik_params* p, pg_lik_out* out, bool grad) {
+    const char* fn = grad ? "pg_pa_loglik_grad" : "pg_pa_loglik";
+    if (!ctx) return PG_EINVAL;
+    if (!p || p->size != sizeof(pg_lik_params) || !out || out->size != sizeof(pg_lik_out))
+        return fail(ctx, PG_EINVAL, std::string(fn) + ": null argument or struct size mismatch");
+    pg_seq_store& q = ctx->seq;
+    if (!q.reserved) return fail(ctx, PG_ESTATE, std::string(fn) + ": no sequence");
+    if (!p->alpha_tau_dev && !(std::isfinite(p->alpha) && p->tau > 0.0 && std::isfinite(p->tau)))
+        return fail(ctx, PG_EINVAL, std::string(fn) + ": alpha must be finite and tau > 0");
+    if (q.n_papers > 0 && !p->logfit_dev)
+        return fail(ctx, PG_EINVAL, std::string(fn) + ": logfit_dev is NULL");
+    if (grad && q.n_papers > 0 && !out->grad_logfit_dev)
+        return fail(ctx, PG_EINVAL, std::string(fn) + ": grad_logfit_dev is NULL");
+    for (size_t y = 0; y < q.n_exist.size(); ++y)
+        if (q.n_exist[y] + q.n_new[y] > q.n_papers)
+            return fail(ctx, PG_ESTATE, std::string(fn) + ": births not set for every paper");
+    PG_HIP(hipSetDevice(ctx->cfg.device));
+    {
+        int rc = seq_prepare(ctx);
+        if (rc) return rc;
+    }
+    hipStream_t s = p->stream ? static_cast<hipStream_t>(p->stream) : ctx->stream;
+    const int grp = grad ? PG_K_LIK_GRAD : PG_K_LIK;
+    if (ctx->prof) PG_HIP(hipEventRecord(ctx->pev[grp][0], s));
+    const LikSeq v = seq_view(q);
+    LikArgs a;
+    a.alpha = p->alpha;
+    a.tau = p->tau;
+    a.at_dev = p->alpha_tau_dev;
+    a.logfit = p->logfit_dev;
+    const int64_t nP = q.n_paper_blocks, nE = q.n_ev_blocks, Y = v.n_years;
+    double* part_p = q.part;
+    double* part_e = q.part + Y * nP;
+    double* part_b = part_e + 4 * nE;
+    double* gphi = grad ? out->grad_logfit_dev : nullptr;
+    launch_lik_forward_sums(s, v, a, nP, part_p, q.year_sums, gphi);
+    launch_lik_slots(s, grad, q.ev_group, nE, q.d_ev_block, q.d_ev_year_blk, v, a, q.year_sums,
+                     gphi, part_e);
+    if (grad) launch_lik_backward(s, v, a, nP, q.year_sums, gphi, part_b);
+    launch_lik_final(s, v.n_years, grad, nP, part_b, a, q.year_sums, q.d_scalars,
+                     out->scalars_dev);
+    PG_HIP(hipGetLastError());
+    if (ctx->prof) {
+        PG_HIP(hipEventRecord(ctx->pev[grp][1], s));
+        PG_HIP(hipEventSynchronize(ctx->pev[grp][1]));
+        float ms = 0.0f;
+        PG_HIP(hipEventElapsedTime(&ms, ctx->pev[grp][0], ctx->pev[grp][1]));
+        ctx->prof_ms[grp] += ms;
+        ctx->prof_bytes[grp] += lik_bytes(q, grad);
+        ctx->prof_calls[grp] += 1;
+    }
+    if (out->host3) {
+        PG_HIP(hipMemcpyAsync(out->host3, q.d_scalars, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+        PG_HIP(hipStreamSynchronize(s));
+    }
+    return PG_OK;
+}
+
+}  // namespace
+
+void seq_release(pg_ctx* ctx) {
+    pg_seq_store& q = ctx->seq;
+    free_ptr(q.deg); free_ptr(q.cit); free_ptr(q.kind); free_ptr(q.cnt); free_ptr(q.birth);
+    q.deg = q.cit = q.cnt = q.birth = nullptr;
+    q.kind = nullptr;
+    seq_free_tables(q);
+    seq_clear(q);
+    q.reserved = false;
+}
+
+int seq_record_snapshot(pg_ctx* ctx, int32_t year, int64_t n_exist, int64_t n_new, int32_t m,
+                        int model) {
+    ctx->seq.pending = false;
+    if (!ctx->seq_recording) return PG_OK;
+    if (model != PG_MODEL_PA)
+        return fail(ctx, PG_ESTATE, "pg_year_step: sequence recording holds PA years only");
+    if (m != ctx->seq.m)
+        return fail(ctx, PG_EINVAL, "pg_year_step: m differs from the recorded sequence's m");
+    {
+        int rc = seq_check_year(ctx, "pg_year_step (recording)", n_exist, n_new);
+        if (rc) return rc;
+    }
+    pg_seq_store& q = ctx->seq;
+    if (n_exist > 0)
+        PG_HIP(hipMemcpyAsync(q.deg + q.paper_years, ctx->in_deg, n_exist * 4,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+    q.pending_year = year;
+    q.pending_n_exist = n_exist;
+    q.pending_n_new = n_new;
+    q.pending = true;
+    return PG_OK;
+}
+
+int seq_record_draws(pg_ctx* ctx, int64_t filled) {
+    pg_seq_store& q = ctx->seq;
+    if (!ctx->seq_recording || !q.pending) return PG_OK;
+    q.pending = false;
+    const int64_t D = q.pending_n_new, N = q.pending_n_exist, m = q.m;
+    hipStream_t s = ctx->stream;
+    if (D > 0 && m > 0) {
+        PG_HIP(hipMemcpyAsync(q.cit + q.new_total * m, ctx->cit, D * m * 4,
+                              hipMemcpyDeviceToDevice, s));
+        PG_HIP(hipMemcpyAsync(q.kind + q.new_total * m, ctx->kind, D * m,
+                              hipMemcpyDeviceToDevice, s));
+    }
+    if (D > 0)
+        PG_HIP(hipMemcpyAsync(q.cnt + q.new_total, ctx->cnt, D * 4, hipMemcpyDeviceToDevice, s));
+    if (N + D > 0)
+        PG_HIP(hipMemcpyAsync(q.birth, ctx->birth, (N + D) * 4, hipMemcpyDeviceToDevice, s));
+    seq_commit_year(q, q.pending_year, N, D);
+    q.filled += filled;
+    q.n_papers = std::max<int64_t>(q.n_papers, N + D);
+    return PG_OK;
+}
+
+extern "C" {
+
+int pg_seq_reserve(pg_ctx* ctx, int32_t m, int64_t max_paper_years, int64_t max_new_papers,
+                   int64_t max_papers) {
+    if (!ctx) return PG_EINVAL;
+    if (m < 0 || m > kMaxCites || max_paper_years < 0 || max_new_papers < 0 || max_papers < 0)
+        return fail(ctx, PG_EINVAL, "pg_seq_reserve: need 0 <= m <= 64 and non-negative sizes");
+    PG_HIP(hipSetDevice(ctx->cfg.device));
+    PG_HIP(hipStreamSynchronize(ctx->stream));
+    seq_release(ctx);
+    pg_seq_store& q = ctx->seq;
+    hipError_t r = dalloc(&q.deg, max_paper_years);
+    if (r == hipSuccess) r = dalloc(&q.cit, max_new_papers * std::max<int64_t>(m, 1));
+    if (r == hipSuccess) r = dalloc(&q.kind, max_new_papers * std::max<int64_t>(m, 1));
+    if (r == hipSuccess) r = dalloc(&q.cnt, max_new_papers);
+    if (r == hipSuccess) r = dalloc(&q.birth, max_papers);
+    if (r != hipSuccess) {
+        seq_release(ctx);
+        return fail(ctx, PG_ENOMEM, "pg_seq_reserve: device allocation failed");
+    }
+    q.m = m;
+    q.cap_paper_years = max_paper_years;
+    q.cap_new = max_new_papers;
+    q.cap_papers = max_papers;
+    q.reserved = true;
+    return PG_OK;
+}
+
+int pg_seq_record(pg_ctx* ctx, int on) {
+    if (!ctx) return PG_EINVAL;
+    if (on && !ctx->seq.reserved) return fail(ctx, PG_ESTATE, "pg_seq_record: no sequence reserved");
+    ctx->seq_recording = on != 0;
+    return PG_OK;
+}
+
+int pg_seq_append_year(pg_ctx* ctx, int32_t year, int64_t n_exist, int64_t n_new,
+                       const int32_t* deg, const int32_t* cit, const int8_t* kind,
+                       const int32_t* cnt) {
+    if (!ctx) return PG_EINVAL;
+    {
+        int rc = seq_check_year(ctx, "pg_seq_append_year", n_exist, n_new);
+        if (rc) return rc;
+    }
+    pg_seq_store& q = ctx->seq;
+    const int64_t m = q.m;
+    if ((n_exist > 0 && !deg) || (n_new > 0 && (!cnt || (m > 0 && (!cit || !kind)))))
+        return fail(ctx, PG_EINVAL, "pg_seq_append_year: null array");
+    for (int64_t j = 0; j < n_exist; ++j)
+        if (deg[j] < 0) return fail(ctx, PG_EINVAL, "pg_seq_append_year: negative in-degree");
+    for (int64_t i = 0; i < n_new; ++i) {
+        if (cnt[i] < 0 || cnt[i] > m)
+            return fail(ctx, PG_EINVAL, "pg_seq_append_year: count outside [0, m]");
+        for (int64_t sl = 0; sl < cnt[i]; ++sl) {
+            const int32_t t = cit[i * m + sl];
+            const int8_t k = kind[i * m + sl];
+            if (t < 0 || t >= n_exist)
+                return fail(ctx, PG_EINVAL, "pg_seq_append_year: target outside [0, n_exist)");
+            if (k < 0 || k > 2)
+                return fail(ctx, PG_EINVAL, "pg_seq_append_year: kind outside {0,1,2}");
+        }
+    }
+    PG_HIP(hipSetDevice(ctx->cfg.device));
+    if (n_exist > 0)
+        PG_HIP(hipMemcpy(q.deg + q.paper_years, deg, n_exist * 4, hipMemcpyHostToDevice));
+    if (n_new > 0 && m > 0) {
+        PG_HIP(hipMemcpy(q.cit + q.new_total * m, cit, n_new * m * 4, hipMemcpyHostToDevice));
+        PG_HIP(hipMemcpy(q.kind + q.new_total * m, kind, n_new * m, hipMemcpyHostToDevice));
+    }
+    if (n_new > 0) PG_HIP(hipMemcpy(q.cnt + q.new_total, cnt, n_new * 4, hipMemcpyHostToDevice));
+    for (int64_t i = 0; i < n_new; ++i) q.filled += cnt[i];
+    seq_commit_year(q, year, n_exist, n_new);
+    return PG_OK;
+}
+
+int pg_seq_set_birth(pg_ctx* ctx, int64_t n_papers, const int32_t* birth) {
+    if (!ctx) return PG_EINVAL;
+    pg_seq_store& q = ctx->seq;
+    if (!q.reserved) return fail(ctx, PG_ESTATE, "pg_seq_set_birth: no sequence reserved");
+    if (n_papers < 0 || n_papers > q.cap_papers || (n_papers > 0 && !birth))
+        return fail(ctx, PG_EINVAL, "pg_seq_set_birth: bad size or null array");
+    for (size_t y = 0; y < q.n_exist.size(); ++y)
+        if (q.n_exist[y] + q.n_new[y] > n_papers)
+            return fail(ctx, PG_EINVAL, "pg_seq_set_birth: fewer papers than the sequence uses");
+    PG_HIP(hipSetDevice(ctx->cfg.device));
+    if (n_papers > 0) PG_HIP(hipMemcpy(q.birth, birth, n_papers * 4, hipMemcpyHostToDevice));
+    q.n_papers = n_papers;
+    q.tables_dirty = true;
+    return PG_OK;
+}
+
+int pg_seq_get_info(pg_ctx* ctx, pg_seq_info* info) {
+    if (!ctx) return PG_EINVAL;
+    if (!info || info->size != sizeof(pg_seq_info))
+        return fail(ctx, PG_EINVAL, "pg_seq_get_info: null info or struct size mismatch");
+    const pg_seq_store& q = ctx->seq;
+    info->n_years = (int32_t)q.n_exist.size();
+    info->m = q.m;
+    info->n_papers = q.n_papers;
+    info->paper_years = q.paper_years;
+    info->new_papers = q.new_total;
+    return PG_OK;
+}
+
+int pg_seq_download(pg_ctx* ctx, int64_t* n_exist, int64_t* n_new, int32_t* year_of,
+                    int32_t* deg, int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* birth) {
+    if (!ctx) return PG_EINVAL;
+    const pg_seq_store& q = ctx->seq;
+    if (!q.reserved) return fail(ctx, PG_ESTATE, "pg_seq_download: no sequence reserved");
+    PG_HIP(hipSetDevice(ctx->cfg.device));
+    PG_HIP(hipStreamSynchronize(ctx->stream));
+    const size_t Y = q.n_exist.size();
+    if (n_exist && Y) std::memcpy(n_exist, q.n_exist.data(), Y * 8);
+    if (n_new && Y) std::memcpy(n_new, q.n_new.data(), Y * 8);
+    if (year_of && Y) std::memcpy(year_of, q.year_of.data(), Y * 4);
+    const int64_t m = q.m;
+    if (deg && q.paper_years)
+        PG_HIP(hipMemcpy(deg, q.deg, q.paper_years * 4, hipMemcpyDeviceToHost));
+    if (cit && q.new_total * m)
+        PG_HIP(hipMemcpy(cit, q.cit, q.new_total * m * 4, hipMemcpyDeviceToHost));
+    if (kind && q.new_total * m)
+        PG_HIP(hipMemcpy(kind, q.kind, q.new_total * m, hipMemcpyDeviceToHost));
+    if (cnt && q.new_total) PG_HIP(hipMemcpy(cnt, q.cnt, q.new_total * 4, hipMemcpyDeviceToHost));
+    if (birth && q.n_papers)
+        PG_HIP(hipMemcpy(birth, q.birth, q.n_papers * 4, hipMemcpyDeviceToHost));
+    return PG_OK;
+}
+
+int pg_pa_loglik(pg_ctx* ctx, const pg_lik_params* p, pg_lik_out* out) {
+    return lik_eval(ctx, p, out, false);
+}
+
+int pg_pa_loglik_grad(pg_ctx* ctx, const pg_lik_params* p, pg_lik_out* out) {
+    return lik_eval(ctx, p, out, true);
+}
+
+}  // extern "C"

@@ -1,0 +1,232 @@
+"""GPU PA likelihood and gradient through the C ABI (needs an MI355X).
+
+Checks, for sequences from the oracle's year step and from the GPU's own
+recorded simulation:
+  - against the oracle with the same arithmetic (FP32 weights, FP64 sums,
+    or_pa_loglik_grad_w32): log L within 1e-12 relative, the scalar
+    gradients and dL/dphi within 1e-9 (summation order and FP64 atomics are
+    the only differences);
+  - against the FP64 oracle gradient (whose formulas the CPU suite pins by
+    central differences to 1e-5): relative error <= 1e-5 for dL/dalpha,
+    dL/dtau and the dL/dphi vector (2-norm);
+  - the torch.autograd.Function against the C entry and against stock
+    PyTorch autograd in FP64 (1e-5);
+  - bit-reproducible log L and scalar gradients across calls;
+  - error paths.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import paragraph_amd as pg
+from oracle import binding as ob
+from test_oracle_loglik import simulate
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def seq_ctx(seq, birth, m):
+    n = int(birth.size)
+    ctx = pg.Context(n, 16, 16, max_cites=max(m, 1), seed=1)
+    ctx.seq_reserve(m, int(sum(seq["n_exist"])), int(sum(seq["n_new"])), n)
+    for y in range(len(seq["n_exist"])):
+        ctx.seq_append_year(int(seq["year_of"][y]), seq["deg"][y], seq["cit"][y],
+                            seq["kind"][y], seq["cnt"][y])
+    ctx.seq_set_birth(birth)
+    return ctx
+
+
+def gpu_eval(ctx, alpha, tau, logfit):
+    import torch
+    phi = torch.as_tensor(np.ascontiguousarray(logfit, np.float32), device="cuda")
+    grad = torch.empty(phi.numel(), dtype=torch.float64, device="cuda")
+    ll, ga, gt = ctx.pa_loglik_grad(alpha, tau, phi.data_ptr(), grad.data_ptr())
+    ll_f = ctx.pa_loglik(alpha, tau, phi.data_ptr())[0]
+    return ll, ga, gt, grad.cpu().numpy(), ll_f
+
+
+def oracle_eval(seq, birth, logfit, m, alpha, tau, w32):
+    return ob.loglik_grad(seq["n_exist"], seq["n_new"], seq["year_of"], seq["deg"], seq["cit"],
+                          seq["kind"], seq["cnt"], m, birth, logfit, alpha, tau, w32=w32)
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+@pytest.mark.parametrize("case", [
+    dict(n0=300, years=3, rate=0.1, m=5, n_nb=2, alpha=1.2, tau=4.0, seed=7),
+    dict(n0=500, years=4, rate=0.08, m=3, n_nb=0, alpha=0.7, tau=2.0, seed=11),
+    dict(n0=400, years=2, rate=0.1, m=40, n_nb=10, alpha=1.0, tau=6.0, seed=3),
+    dict(n0=2000, years=3, rate=0.05, m=20, n_nb=6, alpha=1.2, tau=4.0, seed=5),
+])
+def test_gpu_matches_oracle(case):
+    log("likelihood case", case)
+    seq, birth, logfit, m = simulate(**case)
+    ctx = seq_ctx(seq, birth, m)
+    for alpha, tau in [(1.0, 5.0), (1.25, 4.0), (0.5, 2.5)]:
+        ll, ga, gt, gphi, ll_f = gpu_eval(ctx, alpha, tau, logfit)
+        r = oracle_eval(seq, birth, logfit, m, alpha, tau, w32=True)
+        log(f"  alpha={alpha} tau={tau}: ll={ll:.12g} ref={r[0]:.12g} ga={ga:.9g} ref={r[1]:.9g} "
+            f"gt={gt:.9g} ref={r[2]:.9g}")
+        assert ll == ll_f, "pg_pa_loglik and pg_pa_loglik_grad disagree on log L"
+        assert rel(ll, r[0]) <= 1e-12
+        scale_a = max(abs(r[1]), 1.0)
+        scale_t = max(abs(r[2]), 1.0)
+        assert abs(ga - r[1]) <= 1e-9 * scale_a
+        assert abs(gt - r[2]) <= 1e-9 * scale_t
+        assert np.all(np.abs(gphi - r[3]) <= 1e-9 * np.maximum(1.0, np.abs(r[3])))
+        # FP64 gradient: the FP32 weights move it by far less than 1e-5
+        d = oracle_eval(seq, birth, logfit, m, alpha, tau, w32=False)
+        assert rel(ll, d[0]) <= 1e-6
+        assert rel(ga, d[1]) <= 1e-5, (ga, d[1])
+        assert rel(gt, d[2]) <= 1e-5, (gt, d[2])
+        assert np.linalg.norm(gphi - d[3]) <= 1e-5 * np.linalg.norm(d[3])
+
+
+def test_uniform_closed_form_and_reproducible():
+    seq, birth, logfit, m = simulate(n0=300, years=3, rate=0.1, m=5, n_nb=2)
+    ctx = seq_ctx(seq, birth, m)
+    zero = np.zeros_like(logfit)
+    ll, ga, gt, gphi, _ = gpu_eval(ctx, 0.0, 1e300, zero)   # 1/tau rounds to 0 in FP32
+    ref = 0.0
+    for N, cit, kind, cnt in zip(seq["n_exist"], seq["cit"], seq["kind"], seq["cnt"]):
+        for row, kr, c in zip(cit, kind, cnt):
+            for j in range(c):
+                if kr[j] != 1:
+                    ref -= math.log(N - j)
+    assert rel(ll, ref) <= 1e-13
+    a = gpu_eval(ctx, 1.1, 3.0, logfit)
+    b = gpu_eval(ctx, 1.1, 3.0, logfit)
+    assert a[:3] == b[:3], "log L / scalar gradients not bit-reproducible"
+    assert np.allclose(a[3], b[3], rtol=1e-13, atol=1e-12)
+
+
+def test_recorded_gpu_sequence():
+    """Record a GPU simulation; the store must equal the per-year downloads,
+    and the likelihood of it must match the oracle's."""
+    n0, years, rate, m, n_nb = 5000, 3, 0.05, 10, 3
+    sched = pg.growth_schedule(n0, rate, years)
+    n_final = n0 + sum(sched)
+    g = pg.seed_graph(seed=2, n=n0, mean_cites=8.0, n_final=n_final)
+    ctx = pg.Context(n_final, g.src.size + n_final * m, max(sched), max_cites=m, seed=2)
+    ctx.load(g)
+    paper_years = sum(n0 + sum(sched[:y]) for y in range(years))
+    ctx.seq_reserve(m, paper_years, sum(sched))
+    ctx.seq_record(True)
+    want = dict(deg=[], cit=[], kind=[], cnt=[])
+    for yi, D in enumerate(sched):
+        want["deg"].append(ctx.download()["in_deg"].copy())
+        ctx.year_pa(1 + yi, yi, D, m, n_nb, 1.2, 4.0)
+        c = ctx.citations(D, m)
+        for k in ("cit", "kind", "cnt"):
+            want[k].append(c[k])
+    ctx.seq_record(False)
+    got = ctx.seq_download()
+    assert got["n_exist"] == [n0 + sum(sched[:y]) for y in range(years)]
+    assert got["n_new"] == sched and got["year_of"] == [1, 2, 3]
+    for y in range(years):
+        assert np.array_equal(got["deg"][y], want["deg"][y])
+        cnt = want["cnt"][y]
+        mask = np.arange(m)[None, :] < cnt[:, None]
+        assert np.array_equal(got["cnt"][y], cnt)
+        assert np.array_equal(got["cit"][y][mask], want["cit"][y][mask])
+        assert np.array_equal(got["kind"][y][mask], want["kind"][y][mask])
+    dl = ctx.download()
+    assert np.array_equal(got["birth"], dl["birth"])
+    logfit = g.logfit[:n_final]
+    for alpha, tau in [(1.2, 4.0), (1.0, 5.0)]:
+        ll, ga, gt, gphi, _ = gpu_eval(ctx, alpha, tau, logfit)
+        r = oracle_eval(got, got["birth"], logfit, m, alpha, tau, w32=True)
+        log(f"recorded: alpha={alpha} tau={tau} ll={ll:.12g} ref={r[0]:.12g}")
+        assert rel(ll, r[0]) <= 1e-12
+        assert abs(ga - r[1]) <= 1e-9 * max(abs(r[1]), 1.0)
+        assert abs(gt - r[2]) <= 1e-9 * max(abs(r[2]), 1.0)
+        assert np.all(np.abs(gphi - r[3]) <= 1e-9 * np.maximum(1.0, np.abs(r[3])))
+
+
+def test_autograd_function_and_plain_torch():
+    import torch
+    from paragraph_amd.likelihood import PALogLikelihood, PlainSequence, loglik_plain_torch
+    seq, birth, logfit, m = simulate(n0=800, years=3, rate=0.08, m=8, n_nb=2, seed=9)
+    ctx = seq_ctx(seq, birth, m)
+    at = torch.tensor([1.1, 4.5], dtype=torch.float64, device="cuda", requires_grad=True)
+    phi = torch.as_tensor(logfit, device="cuda").clone().requires_grad_(True)
+    ll = PALogLikelihood.apply(ctx, at, phi)
+    ll.backward()
+    ref = ctx.pa_loglik_grad(1.1, 4.5, phi.detach().data_ptr(),
+                             torch.empty(phi.numel(), dtype=torch.float64,
+                                         device="cuda").data_ptr())
+    assert float(ll.detach()) == ref[0]
+    assert at.grad[0].item() == ref[1] and at.grad[1].item() == ref[2]
+    # stock PyTorch autograd in FP64
+    s = dict(seq)
+    s["birth"], s["m"] = birth, m
+    ps = PlainSequence(s, "cuda")
+    at2 = torch.tensor([1.1, 4.5], dtype=torch.float64, device="cuda", requires_grad=True)
+    phi2 = torch.as_tensor(logfit, device="cuda").to(torch.float64).requires_grad_(True)
+    ll2 = loglik_plain_torch(ps, at2, phi2)
+    ll2.backward()
+    log(f"autograd: ll={float(ll.detach()):.12g} plain={float(ll2.detach()):.12g} "
+        f"ga={at.grad[0].item():.9g} plain={at2.grad[0].item():.9g}")
+    assert rel(float(ll.detach()), float(ll2.detach())) <= 1e-6
+    assert rel(at.grad[0].item(), at2.grad[0].item()) <= 1e-5
+    assert rel(at.grad[1].item(), at2.grad[1].item()) <= 1e-5
+    g1 = phi.grad.double().cpu().numpy()
+    g2 = phi2.grad.cpu().numpy()
+    assert np.linalg.norm(g1 - g2) <= 1e-5 * np.linalg.norm(g2)
+
+
+def test_fit_adam_moves_towards_generating_parameters():
+    import torch
+    from paragraph_amd.likelihood import fit_adam
+    seq, birth, logfit, m = simulate(n0=3000, years=4, rate=0.1, m=10, n_nb=3, alpha=1.2,
+                                     tau=4.0, seed=21)
+    ctx = seq_ctx(seq, birth, m)
+    n_ev = sum(int(((np.arange(m)[None, :] < c[:, None]) & (k != 1)).sum())
+               for c, k in zip(seq["cnt"], seq["kind"]))
+    phi0 = torch.as_tensor(logfit, device="cuda")
+    at, _, hist = fit_adam(ctx, phi0, 1.0, 5.0, steps=150, lr=0.02, fit_logfit=False,
+                           per_event=1.0 / n_ev)
+    h = hist.cpu().numpy()
+    a, t = at.cpu().numpy()
+    log(f"fit: ll {h[0]:.6g} -> {h[-1]:.6g}; alpha {a:.4f} tau {t:.4f}")
+    assert h[-1] > h[0]
+    assert abs(a - 1.2) < abs(1.0 - 1.2)
+    assert abs(t - 4.0) < abs(5.0 - 4.0)
+
+
+def test_error_paths():
+    import torch
+    ctx = pg.Context(100, 16, 16, max_cites=4, seed=1)
+    phi = torch.zeros(100, dtype=torch.float32, device="cuda")
+    with pytest.raises(pg.PGError, match="ESTATE"):
+        ctx.pa_loglik(1.0, 5.0, phi.data_ptr())
+    ctx.seq_reserve(2, 100, 10, 100)
+    deg = np.zeros(5, np.int32)
+    with pytest.raises(pg.PGError, match="EINVAL"):   # target outside [0, n_exist)
+        ctx.seq_append_year(1, deg, np.array([[0, 5]], np.int32), np.zeros((1, 2), np.int8),
+                            np.array([2], np.int32))
+    with pytest.raises(pg.PGError, match="EINVAL"):   # kind outside {0,1,2}
+        ctx.seq_append_year(1, deg, np.array([[0, 1]], np.int32), np.full((1, 2), 3, np.int8),
+                            np.array([2], np.int32))
+    ctx.seq_append_year(1, deg, np.array([[0, 1]], np.int32), np.zeros((1, 2), np.int8),
+                        np.array([2], np.int32))
+    with pytest.raises(pg.PGError, match="ESTATE"):   # births not set
+        ctx.pa_loglik(1.0, 5.0, phi.data_ptr())
+    ctx.seq_set_birth(np.zeros(6, np.int32))
+    with pytest.raises(pg.PGError, match="EINVAL"):
+        ctx.pa_loglik(1.0, -1.0, phi.data_ptr())
+    with pytest.raises(pg.PGError, match="ECAP"):
+        ctx.seq_append_year(2, np.zeros(200, np.int32), np.zeros((0, 2), np.int32),
+                            np.zeros((0, 2), np.int8), np.zeros(0, np.int32))
+    # 5 equal weights, two PA picks: log L = -log 5 - log 4
+    ll = ctx.pa_loglik(0.0, 1e300, phi.data_ptr())[0]
+    assert abs(ll - (-math.log(5) - math.log(4))) <= 1e-15

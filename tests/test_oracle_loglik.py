@@ -92,3 +92,27 @@ def test_gradient_matches_central_differences():
         lm = logfit.copy(); lm[j] -= np.float32(hp)
         fd = (loglik(seq, birth, lp, m, alpha, tau)[0] - loglik(seq, birth, lm, m, alpha, tau)[0]) / (2 * hp)
         assert abs(gphi[j] - fd) <= 1e-5 * max(1.0, abs(fd)), j
+
+
+def test_w32_mode_closed_form_and_fp64_agreement():
+    """The FP32-weight mode (the GPU's arithmetic) gives exact unit weights at
+    alpha=0, 1/tau -> 0, phi=0, and stays within 1e-5 of the FP64 gradient."""
+    seq, birth, logfit, m = simulate()
+    ref = 0.0
+    for N, cit, kind, cnt in zip(seq["n_exist"], seq["cit"], seq["kind"], seq["cnt"]):
+        for row, kr, c in zip(cit, kind, cnt):
+            for j in range(c):
+                if kr[j] != 1:
+                    ref -= math.log(N - j)
+    ll, ga, gt, _ = ob.loglik_grad(seq["n_exist"], seq["n_new"], seq["year_of"], seq["deg"],
+                                   seq["cit"], seq["kind"], seq["cnt"], m, birth,
+                                   np.zeros_like(logfit), 0.0, 1e300, w32=True)
+    assert abs(ll - ref) <= 1e-13 * abs(ref)
+    for alpha, tau in [(1.25, 4.0), (1.0, 5.0), (0.5, 2.5)]:
+        a = ob.loglik_grad(seq["n_exist"], seq["n_new"], seq["year_of"], seq["deg"], seq["cit"],
+                           seq["kind"], seq["cnt"], m, birth, logfit, alpha, tau, w32=True)
+        d = loglik(seq, birth, logfit, m, alpha, tau)
+        assert abs(a[0] - d[0]) <= 1e-6 * abs(d[0])
+        assert abs(a[1] - d[1]) <= 1e-5 * abs(d[1])
+        assert abs(a[2] - d[2]) <= 1e-5 * abs(d[2])
+        assert np.linalg.norm(a[3] - d[3]) <= 1e-5 * np.linalg.norm(d[3])
