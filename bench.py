@@ -1,4 +1,5 @@
-"""Benchmark: citation edges/s of the 10-year PA growth on one MI355X.
+"""Benchmark: citation edges/s of the 10-year PA growth on one MI355X, and
+likelihood-gradient evaluations/s on a recorded 30-year sequence.
 
 Workload (one step): the 2,000,000-paper synthetic seed graph (birth years
 uniform in [-20,0], Poisson(10) citations to uniformly chosen older papers,
@@ -57,6 +58,178 @@ def cpu_baseline(G0, sched, m, n_nb, alpha, tau, seed, cap):
                       f"with {cores} OpenMP threads: {dt:.2f} s"}
 
 
+def likelihood_bench(args) -> dict:
+    """Likelihood-gradient evaluations per second on the 30-year sequence.
+
+    Sequence: a 1,000,000-paper seed (Poisson(8) citations, births uniform
+    in [-20,0], lognormal(0,0.5) fitness) grown 30 years at 3% per year with
+    20 citations per paper (14 PA + 6 neighbourhood), alpha=1.2, tau=4,
+    Philox seed 4, recorded on the device: 1,427,286 citing papers,
+    28,545,720 slots, 2,427,286 log-fitness parameters.  Timed: the fused
+    pg_pa_loglik_grad (log L, dL/dalpha, dL/dtau, dL/dphi) at alpha=1,
+    tau=5, back to back on one stream; beside it the same gradient with stock
+    PyTorch autograd (FP64) on the same GPU, and the FP64 C oracle on one
+    host core.  Also checks the full-size gradient against the oracle."""
+    import torch
+    from paragraph_amd.likelihood import (PALogLikelihood, PlainSequence, fit_adam,
+                                          loglik_plain_torch)
+    n0, years, rate, m, n_nb = 1_000_000, 30, 0.03, 20, 6
+    alpha_g, tau_g, seed = 1.2, 4.0, 4
+    sched = pg.growth_schedule(n0, rate, years)
+    n_final = n0 + sum(sched)
+    t0 = time.perf_counter()
+    g = pg.seed_graph(seed, n0, 8.0, n_final=n_final)
+    ctx = pg.Context(n_final, g.src.size + sum(sched) * m + 16, max(sched), max_cites=m,
+                     bfs_cap=4096, seed=seed)
+    ctx.load(g)
+    paper_years, n = 0, n0
+    for d in sched:
+        paper_years += n
+        n += d
+    ctx.seq_reserve(m, paper_years, sum(sched))
+    ctx.seq_record(True)
+    for yi, d in enumerate(sched):
+        ctx.year_pa(1 + yi, yi, d, m, n_nb, alpha_g, tau_g)
+    ctx.seq_record(False)
+    info = ctx.seq_info()
+    eprint(f"likelihood sequence: {info.n_years} years, {info.paper_years} paper-years, "
+           f"{info.new_papers} citing papers, {info.n_papers} papers "
+           f"({time.perf_counter() - t0:.1f} s)")
+    dev = torch.device("cuda", 0)
+    phi = torch.from_numpy(np.ascontiguousarray(g.logfit[:n_final])).to(dev)
+    grad = torch.empty(n_final, dtype=torch.float64, device=dev)
+    sc = torch.empty(3, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    a0, t0_ = 1.0, 5.0
+
+    def fused():
+        ctx.pa_loglik_grad(a0, t0_, phi.data_ptr(), grad.data_ptr(), scalars_dev=sc.data_ptr(),
+                           stream=stream, host=False)
+
+    for _ in range(3):
+        fused()
+    torch.cuda.synchronize()
+    K = args.lik_evals
+    t1 = time.perf_counter()
+    for _ in range(K):
+        fused()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t1
+    evals_s = K / dt
+    # device time and algorithmic bytes per evaluation (events around the passes)
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(20):
+        fused()
+    prof = ctx.profile_read()["lik_grad"]
+    ctx.profile(False)
+    dev_ms = prof["ms"] / prof["launches"]
+    bytes_eval = prof["bytes"] / prof["launches"]
+    achieved = bytes_eval / (dev_ms * 1e-3) / 1e9
+    ll, ga, gt = (float(x) for x in sc.cpu())
+    eprint(f"fused gradient: {evals_s:.1f} evals/s ({1e3 / evals_s:.3f} ms wall, {dev_ms:.3f} ms "
+           f"device, {bytes_eval / 1e9:.3f} GB algorithmic, {achieved:.0f} GB/s); "
+           f"log L {ll:.10g} dL/dalpha {ga:.8g} dL/dtau {gt:.8g}")
+    # torch.autograd.Function (forward + backward, device-resident parameters)
+    at = torch.tensor([a0, t0_], dtype=torch.float64, device=dev, requires_grad=True)
+    phr = phi.clone().requires_grad_(True)
+    K2 = max(K // 4, 5)
+    for i in range(K2 + 2):
+        if i == 2:
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+        at.grad = None
+        phr.grad = None
+        PALogLikelihood.apply(ctx, at, phr).backward()
+    torch.cuda.synchronize()
+    autograd_s = K2 / (time.perf_counter() - t2)
+    # plain PyTorch autograd, FP64
+    seq = ctx.seq_download()
+    ps = PlainSequence(seq, dev)
+    at2 = torch.tensor([a0, t0_], dtype=torch.float64, device=dev, requires_grad=True)
+    ph2 = phi.to(torch.float64).requires_grad_(True)
+    K3 = 3
+    for i in range(K3 + 1):
+        if i == 1:
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+        at2.grad = None
+        ph2.grad = None
+        ll2 = loglik_plain_torch(ps, at2, ph2)
+        ll2.backward()
+    torch.cuda.synchronize()
+    plain_s = K3 / (time.perf_counter() - t3)
+    g_plain = ph2.grad.cpu().numpy()
+    ga2, gt2 = at2.grad.cpu().numpy()
+    eprint(f"autograd.Function: {autograd_s:.1f} evals/s; plain PyTorch autograd (FP64): "
+           f"{plain_s:.2f} evals/s; plain log L {float(ll2.detach()):.10g}")
+    g_gpu = grad.cpu().numpy()
+    res = {
+        "metric": "likelihood-grad evals/s (30y sequence, 1M seed)",
+        "value": evals_s,
+        "unit": "gradient evaluations/s",
+        "ms_per_eval": 1e3 / evals_s,
+        "device_ms_per_eval": dev_ms,
+        "vs_plain": evals_s / plain_s,
+        "plain_torch_evals_per_s": plain_s,
+        "autograd_function_evals_per_s": autograd_s,
+        "dtype": "f32 weights, f64 partition sums and gradients",
+        "config": {"workload": "30y PA sequence at 3%/y from a 1,000,000-paper seed, m=20 "
+                               "(14 PA + 6 neighbourhood), generated at alpha=1.2, tau=4; "
+                               "gradient at alpha=1, tau=5",
+                   "paper_years": info.paper_years, "citing_papers": info.new_papers,
+                   "slots": info.new_papers * m, "params": info.n_papers + 2},
+        "roofline": {"bound": "hbm", "kernel": "pg_pa_loglik_grad (all passes)",
+                     "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None},
+        "vs_plain_torch_rel_err": {
+            "dalpha": abs(ga - ga2) / abs(ga2), "dtau": abs(gt - gt2) / abs(gt2),
+            "dphi_l2": float(np.linalg.norm(g_gpu - g_plain) / np.linalg.norm(g_plain))},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import binding as ob
+        kw = dict(n_exist=seq["n_exist"], n_new=seq["n_new"], year_of=seq["year_of"],
+                  deg_list=seq["deg"], cit_list=seq["cit"], kind_list=seq["kind"],
+                  cnt_list=seq["cnt"], m=m, birth=seq["birth"], logfit=g.logfit[:n_final],
+                  alpha=a0, tau=t0_)
+        t4 = time.perf_counter()
+        d = ob.loglik_grad(**kw, w32=False)
+        cpu_s = time.perf_counter() - t4
+        w = ob.loglik_grad(**kw, w32=True)
+        res["cpu_baseline"] = {"value": 1.0 / cpu_s, "unit": "gradient evaluations/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"one FP64 gradient evaluation of the full sequence on "
+                                         f"the C oracle, one thread: {cpu_s:.2f} s"}
+        res["vs_oracle_fp64_rel_err"] = {
+            "loglik": abs(ll - d[0]) / abs(d[0]), "dalpha": abs(ga - d[1]) / abs(d[1]),
+            "dtau": abs(gt - d[2]) / abs(d[2]),
+            "dphi_l2": float(np.linalg.norm(g_gpu - d[3]) / np.linalg.norm(d[3]))}
+        res["vs_oracle_same_arith_rel_err"] = {
+            "loglik": abs(ll - w[0]) / abs(w[0]), "dalpha": abs(ga - w[1]) / abs(w[1]),
+            "dtau": abs(gt - w[2]) / abs(w[2]),
+            "dphi_max": float(np.max(np.abs(g_gpu - w[3]) / np.maximum(1.0, np.abs(w[3]))))}
+        eprint(f"oracle: {cpu_s:.2f} s per FP64 evaluation; errors {res['vs_oracle_fp64_rel_err']}"
+               f" (FP64), {res['vs_oracle_same_arith_rel_err']} (same arithmetic)")
+    # Adam from alpha=1, tau=5 (log-fitness held at its true values)
+    n_ev = int(sum(((np.arange(m)[None, :] < c[:, None]) & (k != 1)).sum()
+                   for c, k in zip(seq["cnt"], seq["kind"])))
+    torch.cuda.synchronize()
+    t5 = time.perf_counter()
+    at_fit, _, hist = fit_adam(ctx, phi, 1.0, 5.0, steps=args.fit_steps, lr=0.02,
+                               fit_logfit=False, per_event=1.0 / n_ev)
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t5
+    af, tf = (float(x) for x in at_fit.cpu())
+    h = hist.cpu().numpy()
+    res["fit"] = {"steps": args.fit_steps, "seconds": fit_s, "alpha": af, "tau": tf,
+                  "start": [1.0, 5.0], "generating": [alpha_g, tau_g],
+                  "loglik_first": float(h[0]), "loglik_last": float(h[-1])}
+    eprint(f"Adam fit ({args.fit_steps} steps, {fit_s:.2f} s): alpha {af:.4f} tau {tf:.4f}, "
+           f"log L {h[0]:.10g} -> {h[-1]:.10g}")
+    ctx.close()
+    return res
+
+
 def copy_calibration(nbytes: int = 1 << 30) -> None:
     """One plain device-to-device copy of nbytes (HIP runtime copy kernel):
     the reference point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE
@@ -85,9 +258,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
+    ap.add_argument("--lik-evals", type=int, default=200,
+                    help="timed likelihood-gradient evaluations")
+    ap.add_argument("--fit-steps", type=int, default=100)
+    ap.add_argument("--no-lik", action="store_true", help="skip the likelihood part")
+    ap.add_argument("--lik-only", action="store_true",
+                    help="only the likelihood part (its JSON object on stdout)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.lik_only:
+        print(json.dumps(likelihood_bench(args)), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -168,13 +350,15 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None},
     }
+    ctx.close()
     if not args.no_cpu_baseline:
-        ctx.close()
         try:
             out["cpu_baseline"] = cpu_baseline(G0, sched, m, n_nb, alpha, tau, seed, cap)
         except Exception as exc:  # reported, never fatal for the GPU number
             eprint(f"cpu baseline failed: {exc}")
             out["cpu_baseline"] = None
+    if not args.no_lik:
+        out["likelihood"] = likelihood_bench(args)
     print(json.dumps(out), flush=True)
 
 

@@ -198,10 +198,11 @@ int pg_snapshot_restore(pg_ctx* ctx);
  *   LIK          4 B per paper-year (in-degree snapshot) + 8 B per paper
  *                (birth, log-fitness) + 4 B per citing paper (count) +
  *                17 B per filled slot (target, kind, 12 B gathered);
- *   LIK_GRAD     two paper passes: 8 B per paper-year + 40 B per paper
- *                (2x birth and log-fitness, dL/dphi written then updated)
- *                + 4 B per citing paper + 33 B per filled slot (17 B as
- *                LIK + 16 B dL/dphi atomic update).
+ *   LIK_GRAD     two paper passes: 8 B per paper-year + 44 B per paper
+ *                (2x birth and log-fitness, segment offset, per-paper share
+ *                sum written and read, dL/dphi written) + 4 B per citing
+ *                paper + 33 B per filled slot (17 B as LIK + 8 B share
+ *                written + 8 B sorted position and share read back).
  */
 int pg_profile_enable(pg_ctx* ctx, int on);
 int pg_profile_read(pg_ctx* ctx, int n, double* ms, double* bytes, int64_t* launches);
@@ -238,7 +239,9 @@ typedef struct {
     const double* alpha_tau_dev; /* optional DEVICE {alpha, tau}: read by the
                                     kernels, so no host round trip */
     const float* logfit_dev;   /* DEVICE float[n_papers]: phi, the parameters */
-    void* stream;              /* hipStream_t to run on; NULL = the context's */
+    void* stream;              /* hipStream_t to run on when stream_given != 0
+                                  (NULL is then the null stream) */
+    int32_t stream_given;      /* 0: run on the context's own stream */
 } pg_lik_params;
 
 typedef struct {
@@ -279,8 +282,9 @@ int pg_seq_download(pg_ctx* ctx, int64_t* n_exist, int64_t* n_new, int32_t* year
 int pg_pa_loglik(pg_ctx* ctx, const pg_lik_params* p, pg_lik_out* out);
 /* log L and its gradient.  The backward paper pass recomputes every weight
  * w_j(y) and the partition sums are recomputed per call (nothing is kept
- * from pg_pa_loglik).  dL/dphi uses FP64 atomics (device memory only); log L
- * and the scalar gradients are reduced in a fixed order. */
+ * from pg_pa_loglik).  Every sum has a fixed order (dL/dphi via the slots
+ * sorted by target, built once per sequence), so repeated calls return the
+ * same bits. */
 int pg_pa_loglik_grad(pg_ctx* ctx, const pg_lik_params* p, pg_lik_out* out);
 
 /* Ceil growth rule of the reference (src/abm.cu:233-240,

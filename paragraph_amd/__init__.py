@@ -61,7 +61,7 @@ class SeqInfo(ctypes.Structure):
 class LikParams(ctypes.Structure):
     _fields_ = [("size", ctypes.c_uint32), ("alpha", ctypes.c_double), ("tau", ctypes.c_double),
                 ("alpha_tau_dev", ctypes.c_void_p), ("logfit_dev", ctypes.c_void_p),
-                ("stream", ctypes.c_void_p)]
+                ("stream", ctypes.c_void_p), ("stream_given", ctypes.c_int32)]
 
 
 class LikOut(ctypes.Structure):
@@ -382,9 +382,12 @@ class Context:
         return out
 
     def _lik(self, fn: str, alpha: float, tau: float, logfit_dev: int, *, grad_dev: int = 0,
-             scalars_dev: int = 0, alpha_tau_dev: int = 0, stream: int = 0, host: bool = True):
+             scalars_dev: int = 0, alpha_tau_dev: int = 0, stream: int | None = None,
+             host: bool = True):
+        """stream: a hipStream_t handle (0 = the null stream), or None for the
+        context's own stream."""
         p = LikParams(ctypes.sizeof(LikParams), float(alpha), float(tau), alpha_tau_dev or None,
-                      logfit_dev or None, stream or None)
+                      logfit_dev or None, stream or None, 0 if stream is None else 1)
         h3 = (ctypes.c_double * 3)()
         o = LikOut(ctypes.sizeof(LikOut), scalars_dev or None, grad_dev or None,
                    ctypes.cast(h3, ctypes.c_void_p) if host else None)

@@ -19,19 +19,24 @@
 // with PL, Pa the w*L and w*age sums over earlier slots and `later` the sum
 // of 1/D over the same paper's later PA slots.
 //
-// Passes (one evaluation; every reduction has a fixed order, so log L and
-// the two scalar gradients are bit-reproducible; dL/dphi takes FP64 atomics
-// from the slot pass and may differ in the last bits between runs):
-//   k_lik_papers<false>  paper-major: per block and year, the partial Z_y;
-//                        zeroes dL/dphi
+// Passes of one evaluation.  Every reduction has a fixed order, so log L
+// and the whole gradient are bit-reproducible from call to call:
+//   k_lik_papers<false>  paper-major, kLikPPT papers per thread: per block
+//                        and year, the partial Z_y
 //   k_lik_year_z         Z_y = fixed-order sum of the block partials
-//   k_lik_slots<GRAD,G>  G lanes per citing paper, one lane per slot:
-//                        exclusive group scans give P (and PL, Pa); per
-//                        block partials of log L (and A, alpha, tau terms);
-//                        dL/dphi scatter
+//   k_lik_cite<GRAD,G,K> G lanes per citing paper, K slots per lane:
+//                        D = Z_y - P per PA slot; per block partials of
+//                        log L (and A, alpha, tau terms); GRAD: each slot's
+//                        dL/dphi_t share [PA slot] + w_t * later, stored at
+//                        the slot's position in the by-target order
 //   k_lik_year_ev        per year: log L, A_y and the slot terms
-//   k_lik_papers<true>   recomputes w_j(y) (never stored): dL/dphi_j -=
-//                        sum_y A_y w_j(y); partials of the A-weighted sums
+//   k_lik_segsum         the shares summed per target paper over the slots
+//                        sorted by target (sorted once per sequence): runs
+//                        inside a 2048-slot chunk are final, runs crossing a
+//                        chunk edge leave per-chunk carries
+//   k_lik_papers<true>   recomputes w_j(y) (never stored): dL/dphi_j =
+//                        slot shares - sum_y A_y w_j(y); partials of the
+//                        A-weighted sums
 //   k_lik_final          combines into {log L, dL/dalpha, dL/dtau}
 #include <hip/hip_runtime.h>
 
@@ -94,78 +99,90 @@ __device__ inline double block_sum_d(double v, double* red) {
     return t;
 }
 
-template <int G>
-__device__ inline double grp_incl_scan(double v, int lane) {
-#pragma unroll
-    for (int d = 1; d < G; d <<= 1) {
-        const double u = __shfl_up(v, d, G);
-        if (lane >= d) v += u;
-    }
-    return v;
+// Slot shares of paper j from the segment sums (k_lik_segsum's outputs).
+__device__ inline double slot_share(int64_t j, const int32_t* __restrict__ seg_off,
+                                    const double* __restrict__ evt,
+                                    const double* __restrict__ carry) {
+    const int64_t o0 = seg_off[j], o1 = seg_off[j + 1];
+    if (o0 == o1) return 0.0;
+    const int64_t b0 = o0 / kLikSegChunk, b1 = (o1 - 1) / kLikSegChunk;
+    if (b0 == b1) return evt[j];
+    double e = carry[2 * b0 + 1];
+    for (int64_t b = b0 + 1; b < b1; ++b) e += carry[2 * b];
+    return e + carry[2 * b1];
 }
 
-template <int G>
-__device__ inline double grp_excl_scan(double v, int lane) {
-    const double inc = grp_incl_scan<G>(v, lane);
-    const double e = __shfl_up(inc, 1, G);
-    return lane ? e : 0.0;
-}
-
-// Paper-major pass.  One thread per paper j in [0, n_papers); loops over
-// the years in which j exists (j < N_y).
-//   BWD = false: part[y * nblk + block] = sum of the block's w_j(y); zeroes
-//                grad_phi.
-//   BWD = true : grad_phi[j] -= sum_y A_y w_j(y); part[q * nblk + block] for
-//                q = 0: sum A_y w L, q = 1: sum A_y w age.
+// Paper-major pass.  Thread t of block B holds papers
+// j = B*kLikBT*kLikPPT + r*kLikBT + t (r < kLikPPT) and loops over the years
+// in which they exist (j < N_y).
+//   BWD = false: part[y * nblk + B] = sum of the block's w_j(y).
+//   BWD = true : grad_phi[j] = slot shares - sum_y A_y w_j(y);
+//                part[B], part[nblk + B] = sums of A_y w L and A_y w age.
 template <bool BWD>
 __global__ __launch_bounds__(kLikBT) void k_lik_papers(LikSeq q, LikArgs a,
                                                        const double* __restrict__ year_sums,
-                                                       double* __restrict__ grad_phi,
+                                                       LikSegs sg, double* __restrict__ grad_phi,
                                                        double* __restrict__ part) {
     __shared__ double s_red[kLikWaves][kLikMaxYears];
     __shared__ double s_red2[kLikWaves];
     const LikScal s = lik_scal(a);
     const int64_t nblk = gridDim.x;
-    const int64_t j0 = (int64_t)blockIdx.x * kLikBT;
-    const int64_t j = j0 + threadIdx.x;
-    const bool in = j < q.n_papers;
+    const int64_t j0 = (int64_t)blockIdx.x * (kLikBT * kLikPPT);
     const int wave = threadIdx.x >> 6;
-    float phi = 0.0f;
-    int32_t b = 0;
-    if (in) {
-        phi = a.logfit[j];
-        b = q.birth[j];
-        if (!BWD && grad_phi) grad_phi[j] = 0.0;
+    float phi[kLikPPT];
+    int32_t b[kLikPPT];
+    double g[kLikPPT];
+#pragma unroll
+    for (int r = 0; r < kLikPPT; ++r) {
+        const int64_t j = j0 + r * kLikBT + threadIdx.x;
+        phi[r] = 0.0f;
+        b[r] = 0;
+        g[r] = 0.0;
+        if (j < q.n_papers) {
+            phi[r] = a.logfit[j];
+            b[r] = q.birth[j];
+        }
     }
-    double g = 0.0, gL = 0.0, ga = 0.0;
+    double gL = 0.0, ga = 0.0;
     for (int y = 0; y < q.n_years; ++y) {
         const int64_t Ny = q.n_exist[y];
         if (j0 >= Ny) {                  // block-uniform: no paper of the block exists
             if (!BWD && threadIdx.x == 0) part[(int64_t)y * nblk + blockIdx.x] = 0.0;
             continue;
         }
-        double wv = 0.0, Lv = 0.0, av = 0.0;
-        if (j < Ny) {
-            const int32_t k = q.deg[q.deg_off[y] + j];
-            const int32_t age = q.year_of[y] - b;
-            float L, x;
-            wv = (double)lik_weight(k, age, phi, s, L, x);
-            Lv = (double)L;
-            av = (double)age;
+        const int64_t doff = q.deg_off[y];
+        const int32_t year = q.year_of[y];
+        const double Ay = BWD ? year_sums[y * kLikYearCols + 1] : 0.0;
+        double zacc = 0.0;
+#pragma unroll
+        for (int r = 0; r < kLikPPT; ++r) {
+            const int64_t j = j0 + r * kLikBT + threadIdx.x;
+            if (j < Ny) {
+                const int32_t k = q.deg[doff + j];
+                const int32_t age = year - b[r];
+                float L, x;
+                const double wv = (double)lik_weight(k, age, phi[r], s, L, x);
+                if (BWD) {
+                    const double aw = Ay * wv;
+                    g[r] += aw;
+                    gL += aw * (double)L;
+                    ga += aw * (double)age;
+                } else {
+                    zacc += wv;
+                }
+            }
         }
-        if (BWD) {
-            const double Ay = year_sums[y * kLikYearCols + 1];
-            const double aw = Ay * wv;
-            g += aw;
-            gL += aw * Lv;
-            ga += aw * av;
-        } else {
-            const double ws = wave_sum_d(wv);
+        if (!BWD) {
+            const double ws = wave_sum_d(zacc);
             if ((threadIdx.x & 63) == 0) s_red[wave][y] = ws;
         }
     }
     if (BWD) {
-        if (in && grad_phi) grad_phi[j] -= g;
+#pragma unroll
+        for (int r = 0; r < kLikPPT; ++r) {
+            const int64_t j = j0 + r * kLikBT + threadIdx.x;
+            if (j < q.n_papers) grad_phi[j] = slot_share(j, sg.seg_off, sg.evt, sg.carry) - g[r];
+        }
         const double tL = block_sum_d(gL, s_red2);
         const double tA = block_sum_d(ga, s_red2);
         if (threadIdx.x == 0) {
@@ -195,72 +212,127 @@ __global__ __launch_bounds__(kLikBT) void k_lik_year_z(int64_t nblk, const doubl
     if (threadIdx.x == 0) year_sums[y * kLikYearCols + 0] = t;
 }
 
-// Slot pass.  Each block holds kLikSlotPapers consecutive citing papers of
-// one year (ev_block[2*blk] = year, ev_block[2*blk+1] = first paper within
-// the year); G lanes per paper, lane l = slot l.  Partials per block, column
-// c at part[c * nblk + blk]: 0 log L, 1 A, 2 alpha terms, 3 tau terms.
-template <bool GRAD, int G>
-__global__ __launch_bounds__(kLikBT) void k_lik_slots(LikSeq q, LikArgs a,
-                                                      const int32_t* __restrict__ ev_block,
-                                                      const double* __restrict__ year_sums,
-                                                      double* __restrict__ grad_phi,
-                                                      double* __restrict__ part) {
+// Slot pass.  G lanes per citing paper (G consecutive lanes of a wave),
+// lane r owning the contiguous slots [r*K, r*K + K) of its paper; block B
+// holds kLikBT/G consecutive citing papers of one year starting at
+// ev_block[2B+1] (year ev_block[2B]).
+//   loop 1  per slot, independent: target, gathers, FP32 weight, x, L, age
+//           (a slot past the count re-reads the paper's slot 0, which is
+//           valid, and is masked off: no branches, the loads overlap);
+//   chain   the draw-order prefix P of the weights, added slot by slot in
+//           the oracle's order: lane r starts from lane r-1's end value;
+//   loop 2  D = Z_y - P per PA slot: log L, and (GRAD) 1/D;
+//   GRAD    later(q) = sum of 1/D over the paper's PA slots after q (suffix
+//           over the lanes, then within the lane in reverse); each slot's
+//           dL/dphi share [PA slot] + w_t * later goes to its by-target
+//           position share[pos_t[slot]]; the alpha/tau prefix terms are
+//           sum_j PL_j/D_j = sum_i w_i L_i later_i (likewise for age).
+// Partials per block, column c at part[c * nblk + B]: 0 log L, 1 A,
+// 2 alpha terms, 3 tau terms.
+template <bool GRAD, int G, int K>
+__global__ __launch_bounds__(kLikBT) void k_lik_cite(LikSeq q, LikArgs a,
+                                                     const int32_t* __restrict__ ev_block,
+                                                     const double* __restrict__ year_sums,
+                                                     double* __restrict__ share,
+                                                     double* __restrict__ part) {
     __shared__ double red[kLikWaves];
-    constexpr int kPer = kLikBT / G;                 // papers per iteration
-    constexpr int kIters = kLikSlotPapers / kPer;
     const LikScal s = lik_scal(a);
     const int64_t nblk = gridDim.x;
     const int y = ev_block[2 * blockIdx.x];
-    const int64_t first = ev_block[2 * blockIdx.x + 1];
+    const int lg = threadIdx.x % G;                   // lane within the paper's group
+    const int64_t i = (int64_t)ev_block[2 * blockIdx.x + 1] + threadIdx.x / G;
     const int64_t Dy = q.n_new[y];
-    const int64_t base = q.new_off[y];
     const int64_t doff = q.deg_off[y];
     const int32_t year = q.year_of[y];
     const double Z = year_sums[y * kLikYearCols + 0];
-    const int lane = threadIdx.x % G;
-    const int m = q.m;
-    double ll = 0.0, Aacc = 0.0, gaacc = 0.0, gtacc = 0.0;
-    for (int it = 0; it < kIters; ++it) {
-        const int64_t i = first + (int64_t)it * kPer + threadIdx.x / G;
-        const bool paper = i < Dy;
-        const int32_t c = paper ? q.cnt[base + i] : 0;
-        const bool have = lane < c;
-        int32_t t = 0;
-        int8_t kd = 1;
-        double wd = 0.0, Ld = 0.0, ad = 0.0, xd = 0.0;
-        if (have) {
-            const int64_t slot = (base + i) * m + lane;
-            t = q.cit[slot];
-            kd = q.kind[slot];
+    const int64_t NP = q.n_cite;
+    const int64_t P = q.new_off[y] + i;               // sequence-wide citing index
+    const int c = i < Dy ? q.cnt[P] : 0;
+    const int j0 = lg * K;                            // first slot of this lane
+    float wr[K], Lr[K];
+    int32_t ar[K];
+    uint32_t evm = 0, vm = 0;
+    double ll = 0.0, gaacc = 0.0, gtacc = 0.0;
+    if (c > 0) {
+#pragma unroll
+        for (int e = 0; e < K; ++e) {
+            const int j = j0 + e;
+            const bool v = j < c;
+            const int64_t qi = (int64_t)(v ? j : 0) * NP + P;
+            const int32_t t = q.cit_t[qi];
+            const bool ev = v && q.kind_t[qi] != 1;
             const int32_t k = q.deg[doff + t];
             const int32_t age = year - q.birth[t];
             float L, x;
-            wd = (double)lik_weight(k, age, a.logfit[t], s, L, x);
-            Ld = (double)L;
-            ad = (double)age;
-            xd = (double)x;
-        }
-        const bool ev = have && kd != 1;
-        const double P = grp_excl_scan<G>(wd, lane);
-        const double Dv = Z - P;
-        if (ev) ll += xd - det_log(Dv);
-        if (GRAD) {
-            const double PL = grp_excl_scan<G>(wd * Ld, lane);
-            const double Pa = grp_excl_scan<G>(wd * ad, lane);
-            const double inv = ev ? 1.0 / Dv : 0.0;
-            if (ev) {
-                Aacc += inv;
-                gaacc += Ld + PL * inv;
-                gtacc += (ad + Pa * inv) * s.itau2;
-            }
-            const double inc = grp_incl_scan<G>(inv, lane);
-            const double tot = __shfl(inc, G - 1, G);
-            const double later = tot - inc;
-            if (have) {
-                const double gv = (ev ? 1.0 : 0.0) + wd * later;
-                unsafeAtomicAdd(grad_phi + t, gv);
+            const float w = lik_weight(k, age, a.logfit[t], s, L, x);
+            wr[e] = v ? w : 0.0f;
+            Lr[e] = L;
+            ar[e] = age;
+            evm |= uint32_t(ev) << e;
+            vm |= uint32_t(v) << e;
+            ll += ev ? (double)x : 0.0;
+            if (GRAD) {
+                gaacc += ev ? (double)L : 0.0;
+                gtacc += ev ? (double)age : 0.0;
             }
         }
+    } else {
+#pragma unroll
+        for (int e = 0; e < K; ++e) {
+            wr[e] = 0.0f;
+            Lr[e] = 0.0f;
+            ar[e] = 0;
+        }
+    }
+    // prefix of the weights in slot order, chained across the lanes
+    double Pw[K];
+    double start = 0.0;
+#pragma unroll
+    for (int r = 0; r < G; ++r) {
+        double p = start;
+#pragma unroll
+        for (int e = 0; e < K; ++e) {
+            if (lg == r) Pw[e] = p;
+            p += (double)wr[e];
+        }
+        if (G > 1) {
+            const double nxt = __shfl(p, (threadIdx.x & 63) - lg + r, 64);
+            if (lg == r + 1) start = nxt;
+        }
+    }
+    double inv[K];
+    double tot = 0.0;
+#pragma unroll
+    for (int e = 0; e < K; ++e) {
+        const bool ev = (evm >> e) & 1;
+        const double Dv = Z - Pw[e];
+        ll -= ev ? det_log(Dv) : 0.0;
+        inv[e] = (GRAD && ev) ? 1.0 / Dv : 0.0;
+        tot += inv[e];
+    }
+    double Aacc = 0.0;
+    if (GRAD) {
+        Aacc = tot;
+        // sum of tot over the later lanes of the paper
+        double suf = tot;
+#pragma unroll
+        for (int d = 1; d < G; d <<= 1) {
+            const double u = __shfl_down(suf, d, G);
+            if (lg + d < G) suf += u;
+        }
+        double later = suf - tot;
+        double sL = 0.0, sa = 0.0;
+#pragma unroll
+        for (int e = K - 1; e >= 0; --e) {
+            const double wl = (double)wr[e] * later;
+            sL += wl * (double)Lr[e];
+            sa += wl * (double)ar[e];
+            if ((vm >> e) & 1)
+                share[q.pos_t[(int64_t)(j0 + e) * NP + P]] = (double)((evm >> e) & 1) + wl;
+            later += inv[e];
+        }
+        gaacc += sL;
+        gtacc = (gtacc + sa) * s.itau2;
     }
     const double tl = block_sum_d(ll, red);
     if (threadIdx.x == 0) part[blockIdx.x] = tl;
@@ -292,6 +364,138 @@ __global__ __launch_bounds__(kLikBT) void k_lik_year_ev(int64_t nblk, int ncol,
         const double t = block_sum_d(v, red);
         const int dst = c == 0 ? 2 : (c == 1 ? 1 : c + 1);
         if (threadIdx.x == 0) year_sums[y * kLikYearCols + dst] = t;
+    }
+}
+
+// Sequence preparation: key = target of each filled slot (n_papers for an
+// empty slot, so empty slots sort last), value = slot index.
+__global__ __launch_bounds__(kLikBT) void k_lik_slot_keys(int64_t T, int m, int64_t n_papers,
+                                                          const int32_t* __restrict__ cnt,
+                                                          const int32_t* __restrict__ cit,
+                                                          uint32_t* __restrict__ keys,
+                                                          uint32_t* __restrict__ vals) {
+    const int64_t i = (int64_t)blockIdx.x * kLikBT + threadIdx.x;
+    if (i >= T) return;
+    const int64_t p = i / m;
+    const int64_t sl = i - p * m;
+    keys[i] = sl < cnt[p] ? (uint32_t)cit[i] : (uint32_t)n_papers;
+    vals[i] = (uint32_t)i;
+}
+
+// seg_off[j] = first sorted position with key >= j, j in [0, n_papers].
+__global__ __launch_bounds__(kLikBT) void k_lik_seg_off(int64_t n_papers, int64_t T,
+                                                        const uint32_t* __restrict__ skey,
+                                                        int32_t* __restrict__ seg_off) {
+    const int64_t j = (int64_t)blockIdx.x * kLikBT + threadIdx.x;
+    if (j > n_papers) return;
+    int64_t lo = 0, hi = T;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)skey[mid] < j) lo = mid + 1;
+        else hi = mid;
+    }
+    seg_off[j] = (int32_t)lo;
+}
+
+// Slot-major copies (slot s of citing paper p at s*NP + p): targets (0 for
+// an empty slot), kinds (1 for an empty slot, so it is never a PA slot) and
+// the by-target sorted position of every slot (pos_t[s*NP+p] = i where
+// perm[i] = p*m + s).
+__global__ __launch_bounds__(kLikBT) void k_lik_slot_major(int64_t T, int m, int64_t NP,
+                                                           const int32_t* __restrict__ cnt,
+                                                           const int32_t* __restrict__ cit,
+                                                           const int8_t* __restrict__ kind,
+                                                           const uint32_t* __restrict__ perm,
+                                                           int32_t* __restrict__ cit_t,
+                                                           int8_t* __restrict__ kind_t,
+                                                           uint32_t* __restrict__ pos_t) {
+    const int64_t i = (int64_t)blockIdx.x * kLikBT + threadIdx.x;
+    if (i >= T) return;
+    const int64_t p = i / m, sl = i - p * m;
+    const bool filled = sl < cnt[p];
+    cit_t[sl * NP + p] = filled ? cit[i] : 0;
+    kind_t[sl * NP + p] = filled ? kind[i] : (int8_t)1;
+    const uint32_t qv = perm[i];
+    const int64_t pp = qv / m, ss = qv - pp * m;
+    pos_t[ss * NP + pp] = (uint32_t)i;
+}
+
+// Per-target sums of the slot shares (stored in by-target order) over sorted
+// positions [0, S) in chunks
+// of kLikSegChunk (kLikSegPer per thread).  A run of equal keys that starts
+// and ends inside the chunk is final: evt[key] = its sum.  A run that
+// continues from the previous chunk gives carry[2*chunk] (its part in this
+// chunk), one that continues into the next chunk gives carry[2*chunk+1].
+__global__ __launch_bounds__(kLikBT) void k_lik_segsum(int64_t S, const uint32_t* __restrict__ skey,
+                                                       const double* __restrict__ share,
+                                                       double* __restrict__ evt,
+                                                       double* __restrict__ carry) {
+    __shared__ double s_val[kLikBT];
+    __shared__ int s_flag[kLikBT];
+    const int64_t c0 = (int64_t)blockIdx.x * kLikSegChunk;
+    const int64_t c1 = c0 + kLikSegChunk < S ? c0 + kLikSegChunk : S;
+    const int64_t i0 = c0 + (int64_t)threadIdx.x * kLikSegPer;
+    uint32_t key[kLikSegPer];
+    double val[kLikSegPer];
+    const uint32_t kprev = (i0 > 0 && i0 - 1 < S) ? skey[i0 - 1] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int e = 0; e < kLikSegPer; ++e) {
+        const int64_t i = i0 + e;
+        key[e] = i < c1 ? skey[i] : 0xFFFFFFFFu;
+        val[e] = i < c1 ? share[i] : 0.0;
+    }
+    const uint32_t knext = (i0 + kLikSegPer < c1) ? skey[i0 + kLikSegPer] : 0xFFFFFFFFu;
+    // thread aggregate: sum of the trailing run, and whether a run starts here
+    // (the chunk's first element always starts a run for the scan)
+    int head_any = (i0 == c0) ? 1 : 0;
+    double tail = 0.0;
+#pragma unroll
+    for (int e = 0; e < kLikSegPer; ++e) {
+        const uint32_t kp = e ? key[e - 1] : kprev;
+        const bool head = (e == 0 && i0 == c0) || key[e] != kp;
+        if (head) {
+            head_any = 1;
+            tail = 0.0;
+        }
+        tail += val[e];
+    }
+    // inclusive segmented scan of (head_any, tail) over the block's threads
+    // (Hillis-Steele in LDS, fixed order)
+    s_val[threadIdx.x] = tail;
+    s_flag[threadIdx.x] = head_any;
+    __syncthreads();
+    for (int d = 1; d < kLikBT; d <<= 1) {
+        double v = s_val[threadIdx.x];
+        int f = s_flag[threadIdx.x];
+        if ((int)threadIdx.x >= d && !f) {
+            v = s_val[threadIdx.x - d] + v;
+            f = s_flag[threadIdx.x - d];
+        }
+        __syncthreads();
+        s_val[threadIdx.x] = v;
+        s_flag[threadIdx.x] = f;
+        __syncthreads();
+    }
+    double run = threadIdx.x > 0 ? s_val[threadIdx.x - 1] : 0.0;  // carry into this thread
+    const uint32_t kfirst = skey[c0];
+    const bool cont_in = c0 > 0 && skey[c0 - 1] == kfirst;
+    const uint32_t klast = skey[c1 - 1];
+    const bool cont_out = c1 < S && skey[c1] == klast;
+#pragma unroll
+    for (int e = 0; e < kLikSegPer; ++e) {
+        const int64_t i = i0 + e;
+        if (i >= c1) break;
+        const uint32_t kp = e ? key[e - 1] : kprev;
+        if ((e == 0 && i0 == c0) || key[e] != kp) run = 0.0;
+        run += val[e];
+        const uint32_t kn = e + 1 < kLikSegPer ? key[e + 1] : knext;
+        if (i + 1 == c1 || kn != key[e]) {            // last element of its run in the chunk
+            const bool starts = !(cont_in && key[e] == kfirst);
+            const bool ends = !(cont_out && key[e] == klast);
+            if (starts && ends) evt[key[e]] = run;
+            if (!starts) carry[2 * blockIdx.x] = run;
+            if (!ends) carry[2 * blockIdx.x + 1] = run;
+        }
     }
 }
 
@@ -334,43 +538,83 @@ __global__ __launch_bounds__(kLikBT) void k_lik_final(int n_years, int grad, int
     }
 }
 
+template <bool GRAD, int G, int K>
+void cite1(hipStream_t st, int64_t nblk, const LikSeq& q, const LikArgs& a,
+           const int32_t* ev_block, const double* year_sums, double* share, double* part) {
+    k_lik_cite<GRAD, G, K><<<(unsigned)nblk, kLikBT, 0, st>>>(q, a, ev_block, year_sums, share,
+                                                               part);
+}
+
+template <bool GRAD, int G>
+void cite_k(hipStream_t st, int K, int64_t nblk, const LikSeq& q, const LikArgs& a,
+            const int32_t* ev_block, const double* year_sums, double* share, double* part) {
+    switch (K) {
+        case 4: cite1<GRAD, G, 4>(st, nblk, q, a, ev_block, year_sums, share, part); break;
+        case 5: cite1<GRAD, G, 5>(st, nblk, q, a, ev_block, year_sums, share, part); break;
+        case 6: cite1<GRAD, G, 6>(st, nblk, q, a, ev_block, year_sums, share, part); break;
+        case 7: cite1<GRAD, G, 7>(st, nblk, q, a, ev_block, year_sums, share, part); break;
+        default: cite1<GRAD, G, 8>(st, nblk, q, a, ev_block, year_sums, share, part); break;
+    }
+}
+
 template <bool GRAD>
-void launch_slots(hipStream_t st, int G, int64_t nblk, const LikSeq& q, const LikArgs& a,
-                  const int32_t* ev_block, const double* year_sums, double* grad_phi,
-                  double* part) {
-    const dim3 grid((unsigned)nblk), block(kLikBT);
+void launch_cite(hipStream_t st, int G, int K, int64_t nblk, const LikSeq& q, const LikArgs& a,
+                 const int32_t* ev_block, const double* year_sums, double* share, double* part) {
     switch (G) {
-        case 4: k_lik_slots<GRAD, 4><<<grid, block, 0, st>>>(q, a, ev_block, year_sums, grad_phi, part); break;
-        case 8: k_lik_slots<GRAD, 8><<<grid, block, 0, st>>>(q, a, ev_block, year_sums, grad_phi, part); break;
-        case 16: k_lik_slots<GRAD, 16><<<grid, block, 0, st>>>(q, a, ev_block, year_sums, grad_phi, part); break;
-        case 32: k_lik_slots<GRAD, 32><<<grid, block, 0, st>>>(q, a, ev_block, year_sums, grad_phi, part); break;
-        default: k_lik_slots<GRAD, 64><<<grid, block, 0, st>>>(q, a, ev_block, year_sums, grad_phi, part); break;
+        case 1: cite_k<GRAD, 1>(st, K, nblk, q, a, ev_block, year_sums, share, part); break;
+        case 2: cite_k<GRAD, 2>(st, K, nblk, q, a, ev_block, year_sums, share, part); break;
+        case 4: cite_k<GRAD, 4>(st, K, nblk, q, a, ev_block, year_sums, share, part); break;
+        default: cite_k<GRAD, 8>(st, K, nblk, q, a, ev_block, year_sums, share, part); break;
     }
 }
 
 }  // namespace
 
-int lik_group_lanes(int m) {
-    int G = 4;
-    while (G < m && G < 64) G <<= 1;
-    return G;
+LikCiteShape lik_cite_shape(int m) {
+    LikCiteShape sh;
+    sh.G = 1;
+    while (sh.G * 8 < m) sh.G <<= 1;                  // m <= 64: G <= 8
+    sh.K = (m + sh.G - 1) / sh.G;
+    if (sh.K < 4) sh.K = 4;
+    return sh;
 }
 
-int64_t lik_paper_blocks(int64_t n_papers) { return (n_papers + kLikBT - 1) / kLikBT; }
+int64_t lik_paper_blocks(int64_t n_papers) {
+    return (n_papers + kLikBT * kLikPPT - 1) / (kLikBT * kLikPPT);
+}
+
+int64_t lik_seg_chunks(int64_t S) { return (S + kLikSegChunk - 1) / kLikSegChunk; }
+
+void launch_lik_sort_slots(hipStream_t st, int64_t T, int m, int64_t n_papers,
+                           const int32_t* cnt, const int32_t* cit, const int8_t* kind,
+                           uint32_t* keys, uint32_t* vals, uint32_t* skey, uint32_t* perm,
+                           int32_t* cit_t, int8_t* kind_t, uint32_t* pos_t, uint32_t* k_tmp,
+                           uint32_t* v_tmp, int key_bits, int64_t* scratch, int32_t* seg_off) {
+    if (T > 0) {
+        const unsigned g = (unsigned)((T + kLikBT - 1) / kLikBT);
+        k_lik_slot_keys<<<g, kLikBT, 0, st>>>(T, m, n_papers, cnt, cit, keys, vals);
+        launch_radix_sort_pairs(st, keys, vals, skey, perm, k_tmp, v_tmp, T, key_bits, scratch);
+        k_lik_slot_major<<<g, kLikBT, 0, st>>>(T, m, T / m, cnt, cit, kind, perm, cit_t, kind_t,
+                                               pos_t);
+    }
+    k_lik_seg_off<<<(unsigned)((n_papers + 1 + kLikBT - 1) / kLikBT), kLikBT, 0, st>>>(
+        n_papers, T, skey, seg_off);
+}
 
 void launch_lik_forward_sums(hipStream_t st, const LikSeq& q, const LikArgs& a, int64_t nblk_p,
-                             double* part_p, double* year_sums, double* grad_phi) {
+                             double* part_p, double* year_sums) {
     if (nblk_p > 0)
-        k_lik_papers<false><<<(unsigned)nblk_p, kLikBT, 0, st>>>(q, a, year_sums, grad_phi, part_p);
+        k_lik_papers<false><<<(unsigned)nblk_p, kLikBT, 0, st>>>(q, a, year_sums, LikSegs{},
+                                                                 nullptr, part_p);
     if (q.n_years > 0) k_lik_year_z<<<q.n_years, kLikBT, 0, st>>>(nblk_p, part_p, year_sums);
 }
 
-void launch_lik_slots(hipStream_t st, bool grad, int G, int64_t nblk_e, const int32_t* ev_block,
-                      const int32_t* year_blk, const LikSeq& q, const LikArgs& a,
-                      double* year_sums, double* grad_phi, double* part_e) {
+void launch_lik_slots(hipStream_t st, bool grad, LikCiteShape sh, int64_t nblk_e,
+                      const int32_t* ev_block, const int32_t* year_blk, const LikSeq& q,
+                      const LikArgs& a, double* year_sums, double* share, double* part_e) {
     if (nblk_e > 0) {
-        if (grad) launch_slots<true>(st, G, nblk_e, q, a, ev_block, year_sums, grad_phi, part_e);
-        else launch_slots<false>(st, G, nblk_e, q, a, ev_block, year_sums, grad_phi, part_e);
+        if (grad) launch_cite<true>(st, sh.G, sh.K, nblk_e, q, a, ev_block, year_sums, share, part_e);
+        else launch_cite<false>(st, sh.G, sh.K, nblk_e, q, a, ev_block, year_sums, share, part_e);
     }
     if (q.n_years > 0)
         k_lik_year_ev<<<q.n_years, kLikBT, 0, st>>>(nblk_e, grad ? 4 : 1, year_blk, part_e,
@@ -378,9 +622,15 @@ void launch_lik_slots(hipStream_t st, bool grad, int G, int64_t nblk_e, const in
 }
 
 void launch_lik_backward(hipStream_t st, const LikSeq& q, const LikArgs& a, int64_t nblk_p,
-                         const double* year_sums, double* grad_phi, double* part_p) {
+                         const double* year_sums, int64_t S, const uint32_t* skey,
+                         const double* share, const LikSegs& sg, double* grad_phi,
+                         double* part_p) {
+    if (S > 0)
+        k_lik_segsum<<<(unsigned)lik_seg_chunks(S), kLikBT, 0, st>>>(S, skey, share, sg.evt,
+                                                                      sg.carry);
     if (nblk_p > 0)
-        k_lik_papers<true><<<(unsigned)nblk_p, kLikBT, 0, st>>>(q, a, year_sums, grad_phi, part_p);
+        k_lik_papers<true><<<(unsigned)nblk_p, kLikBT, 0, st>>>(q, a, year_sums, sg, grad_phi,
+                                                                part_p);
 }
 
 void launch_lik_final(hipStream_t st, int n_years, bool grad, int64_t nblk_p,

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/paragraph.h"
+#include "pg_internal.h"
 
 // A citation sequence held on the device for the PA likelihood: per year,
 // the in-degree snapshot at the start of the year and the year's draws
@@ -42,7 +43,7 @@ struct pg_seq_store {
     int32_t* d_ev_block = nullptr;   // event block -> year, then first new paper
     int32_t* d_ev_year_blk = nullptr;// year -> first event block (n_years+1)
     int64_t n_ev_blocks = 0;
-    int32_t ev_group = 0;            // lanes per new paper in the event kernel
+    pg::LikCiteShape cite_shape{1, 4};  // slot-pass lanes per paper and slots per lane
     int64_t n_paper_blocks = 0;
     double* part = nullptr;          // per-block partial sums
     int64_t part_cap = 0;
@@ -50,6 +51,16 @@ struct pg_seq_store {
     int64_t table_cap_years = 0;
     int64_t table_cap_blocks = 0;
     double* d_scalars = nullptr;     // loglik, d/dalpha, d/dtau
+    // slots sorted by target (built when the tables are rebuilt)
+    uint32_t* skey = nullptr;        // [new * m] slot targets, sorted
+    uint32_t* pos_t = nullptr;       // [new * m] by-target position, slot-major
+    int32_t* cit_t = nullptr;        // [new * m] targets, slot-major
+    int8_t* kind_t = nullptr;        // [new * m] kinds, slot-major
+    int32_t* seg_off = nullptr;      // [cap_papers + 1]
+    double* share = nullptr;         // [filled] per-slot dL/dphi shares, by-target order
+    double* evt = nullptr;           // [cap_papers]
+    double* carry = nullptr;         // [2 * chunks]
+    int64_t carry_cap = 0;
 };
 
 struct pg_ctx {

@@ -123,7 +123,9 @@ void launch_sum_i32(hipStream_t s, const int32_t* p, int64_t n, unsigned long lo
 // ---- PA likelihood (k_lik.hip) ----
 constexpr int kLikMaxYears = 256;   // years per sequence (LDS per-year partials)
 constexpr int kLikYearCols = 5;     // per year: Z, A, log L, alpha terms, tau terms
-constexpr int kLikSlotPapers = 64;  // citing papers per slot-pass block
+constexpr int kLikPPT = 4;          // papers per thread in the paper passes
+constexpr int kLikSegPer = 8;       // sorted slots per thread in k_lik_segsum
+constexpr int kLikSegChunk = 256 * kLikSegPer;
 
 // A citation sequence in device memory.  Year y: papers [0, n_exist[y])
 // exist, with in-degrees deg[deg_off[y] + j]; citing papers new_off[y] ..
@@ -143,6 +145,10 @@ struct LikSeq {
     const int8_t* kind;
     const int32_t* cnt;
     const int32_t* birth;
+    int64_t n_cite;           // citing papers (sum of n_new)
+    const int32_t* cit_t;     // slot-major copies: slot s of paper p at s*n_cite + p
+    const int8_t* kind_t;
+    const uint32_t* pos_t;    // by-target sorted position of each slot
 };
 
 struct LikArgs {
@@ -151,19 +157,46 @@ struct LikArgs {
     const float* logfit;      // device [n_papers]
 };
 
-int lik_group_lanes(int m);
+// Per-target sums of the slot shares: seg_off[n_papers+1] over the slots
+// sorted by target, evt[n_papers] (runs inside one chunk), carry[2*chunks].
+struct LikSegs {
+    const int32_t* seg_off;
+    double* evt;
+    double* carry;
+};
+
+// Slot-pass shape for m slots per citing paper: G lanes per paper (a power
+// of two, G*8 >= m), K = max(4, ceil(m/G)) slots per lane; a block of 256
+// threads holds 256/G citing papers.
+struct LikCiteShape {
+    int G, K;
+};
+LikCiteShape lik_cite_shape(int m);
 int64_t lik_paper_blocks(int64_t n_papers);
-// Paper pass (per-block partial Z_y; zeroes grad_phi if non-null), then Z_y.
+int64_t lik_seg_chunks(int64_t S);
+// Once per sequence: the T = new_papers*m slots sorted by target (empty
+// slots last): skey = sorted keys, seg_off per paper, and the slot-major
+// copies cit_t/kind_t/pos_t.  keys/vals/perm/k_tmp/v_tmp: T each; scratch:
+// radix_scratch_elems(T).
+void launch_lik_sort_slots(hipStream_t s, int64_t T, int m, int64_t n_papers,
+                           const int32_t* cnt, const int32_t* cit, const int8_t* kind,
+                           uint32_t* keys, uint32_t* vals, uint32_t* skey, uint32_t* perm,
+                           int32_t* cit_t, int8_t* kind_t, uint32_t* pos_t, uint32_t* k_tmp,
+                           uint32_t* v_tmp, int key_bits, int64_t* scratch, int32_t* seg_off);
+// Paper pass (per-block partial Z_y), then Z_y.
 void launch_lik_forward_sums(hipStream_t s, const LikSeq& q, const LikArgs& a, int64_t nblk_p,
-                             double* part_p, double* year_sums, double* grad_phi);
-// Slot pass (log L; with grad also A_y, the alpha/tau slot terms and the
-// dL/dphi scatter), then the per-year sums.
-void launch_lik_slots(hipStream_t s, bool grad, int G, int64_t nblk_e, const int32_t* ev_block,
-                      const int32_t* year_blk, const LikSeq& q, const LikArgs& a,
-                      double* year_sums, double* grad_phi, double* part_e);
-// Paper pass with A_y: dL/dphi -= sum_y A_y w_j(y); partials of A-weighted sums.
+                             double* part_p, double* year_sums);
+// Slot pass (log L; with grad also A_y, the alpha/tau slot terms and each
+// slot's dL/dphi share into share[pos_t[slot]]), then the per-year sums.
+void launch_lik_slots(hipStream_t s, bool grad, LikCiteShape sh, int64_t nblk_e,
+                      const int32_t* ev_block, const int32_t* year_blk, const LikSeq& q,
+                      const LikArgs& a, double* year_sums, double* share, double* part_e);
+// Segment sums of the shares over the S filled sorted slots, then the paper
+// pass with A_y: dL/dphi = shares - sum_y A_y w_j(y); A-weighted partials.
 void launch_lik_backward(hipStream_t s, const LikSeq& q, const LikArgs& a, int64_t nblk_p,
-                         const double* year_sums, double* grad_phi, double* part_p);
+                         const double* year_sums, int64_t S, const uint32_t* skey,
+                         const double* share, const LikSegs& sg, double* grad_phi,
+                         double* part_p);
 // {log L, dL/dalpha, dL/dtau} -> out[3] (and out2[3] if non-null).
 void launch_lik_final(hipStream_t s, int n_years, bool grad, int64_t nblk_p,
                       const double* part_p, const LikArgs& a, const double* year_sums,

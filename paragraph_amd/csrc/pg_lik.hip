@@ -30,6 +30,13 @@ void seq_free_tables(pg_seq_store& q) {
     free_ptr(q.d_n_exist); free_ptr(q.d_n_new); free_ptr(q.d_deg_off); free_ptr(q.d_new_off);
     free_ptr(q.d_year_of); free_ptr(q.d_ev_block); free_ptr(q.d_ev_year_blk);
     free_ptr(q.part); free_ptr(q.year_sums); free_ptr(q.d_scalars);
+    free_ptr(q.skey); free_ptr(q.pos_t); free_ptr(q.cit_t); free_ptr(q.kind_t);
+    free_ptr(q.seg_off); free_ptr(q.share); free_ptr(q.evt); free_ptr(q.carry);
+    q.skey = q.pos_t = nullptr;
+    q.cit_t = q.seg_off = nullptr;
+    q.kind_t = nullptr;
+    q.share = q.evt = q.carry = nullptr;
+    q.carry_cap = 0;
     q.d_n_exist = q.d_n_new = q.d_deg_off = q.d_new_off = nullptr;
     q.d_year_of = q.d_ev_block = q.d_ev_year_blk = nullptr;
     q.part = q.year_sums = q.d_scalars = nullptr;
@@ -50,10 +57,12 @@ int seq_prepare(pg_ctx* ctx) {
     if (!q.tables_dirty) return PG_OK;
     const int64_t Y = (int64_t)q.n_exist.size();
     const int64_t nP = lik_paper_blocks(q.n_papers);
+    const LikCiteShape sh = lik_cite_shape(q.m);
+    const int64_t per_block = 256 / sh.G;
     std::vector<int32_t> ev_block, year_blk(Y + 1, 0);
     for (int64_t y = 0; y < Y; ++y) {
         year_blk[y] = (int32_t)(ev_block.size() / 2);
-        for (int64_t f = 0; f < q.n_new[y]; f += kLikSlotPapers) {
+        for (int64_t f = 0; f < q.n_new[y]; f += per_block) {
             ev_block.push_back((int32_t)y);
             ev_block.push_back((int32_t)f);
         }
@@ -96,7 +105,49 @@ int seq_prepare(pg_ctx* ctx) {
         PG_HIP(hipMemcpy(q.d_ev_block, ev_block.data(), nE * 8, hipMemcpyHostToDevice));
     q.n_ev_blocks = nE;
     q.n_paper_blocks = nP;
-    q.ev_group = lik_group_lanes(q.m);
+    q.cite_shape = sh;
+    // slots sorted by target, for the per-paper sums of the dL/dphi shares
+    const int64_t T = q.new_total * q.m;
+    free_ptr(q.skey); free_ptr(q.pos_t); free_ptr(q.cit_t); free_ptr(q.kind_t);
+    free_ptr(q.seg_off); free_ptr(q.share); free_ptr(q.evt); free_ptr(q.carry);
+    q.skey = q.pos_t = nullptr;
+    q.cit_t = q.seg_off = nullptr;
+    q.kind_t = nullptr;
+    q.share = q.evt = q.carry = nullptr;
+    const int64_t nchunk = lik_seg_chunks(q.filled);
+    uint32_t *keys = nullptr, *vals = nullptr, *k_tmp = nullptr, *v_tmp = nullptr,
+             *perm = nullptr;
+    int64_t* scratch = nullptr;
+    hipError_t r = dalloc(&q.skey, T);
+    if (r == hipSuccess) r = dalloc(&q.pos_t, T);
+    if (r == hipSuccess) r = dalloc(&q.cit_t, T);
+    if (r == hipSuccess) r = dalloc(&q.kind_t, T);
+    if (r == hipSuccess) r = dalloc(&q.seg_off, q.n_papers + 1);
+    if (r == hipSuccess) r = dalloc(&q.share, q.filled);
+    if (r == hipSuccess) r = dalloc(&perm, T);
+    if (r == hipSuccess) r = dalloc(&q.evt, q.n_papers);
+    if (r == hipSuccess) r = dalloc(&q.carry, 2 * nchunk);
+    if (r == hipSuccess) r = dalloc(&keys, T);
+    if (r == hipSuccess) r = dalloc(&vals, T);
+    if (r == hipSuccess) r = dalloc(&k_tmp, T);
+    if (r == hipSuccess) r = dalloc(&v_tmp, T);
+    if (r == hipSuccess) r = dalloc(&scratch, radix_scratch_elems(T));
+    if (r == hipSuccess) {
+        int bits = 1;
+        while (bits < 31 && (int64_t(1) << bits) <= q.n_papers) ++bits;
+        launch_lik_sort_slots(ctx->stream, T, q.m, q.n_papers, q.cnt, q.cit, q.kind, keys, vals,
+                              q.skey, perm, q.cit_t, q.kind_t, q.pos_t, k_tmp, v_tmp, bits,
+                              scratch, q.seg_off);
+        r = hipGetLastError();
+        if (r == hipSuccess) r = hipStreamSynchronize(ctx->stream);
+    }
+    free_ptr(keys); free_ptr(vals); free_ptr(k_tmp); free_ptr(v_tmp); free_ptr(scratch);
+    free_ptr(perm);
+    if (r != hipSuccess) {
+        seq_free_tables(q);
+        return fail(ctx, PG_EHIP, std::string("likelihood slot sort: ") + hipGetErrorString(r));
+    }
+    q.carry_cap = 2 * nchunk;
     q.tables_dirty = false;
     return PG_OK;
 }
@@ -116,6 +167,10 @@ LikSeq seq_view(const pg_seq_store& q) {
     v.kind = q.kind;
     v.cnt = q.cnt;
     v.birth = q.birth;
+    v.n_cite = q.new_total;
+    v.cit_t = q.cit_t;
+    v.kind_t = q.kind_t;
+    v.pos_t = q.pos_t;
     return v;
 }
 
@@ -147,7 +202,7 @@ void seq_commit_year(pg_seq_store& q, int32_t year, int64_t n_exist, int64_t n_n
 double lik_bytes(const pg_seq_store& q, bool grad) {
     const double PY = (double)q.paper_years, P = (double)q.n_papers, NP = (double)q.new_total,
                  S = (double)q.filled;
-    return grad ? 8.0 * PY + 40.0 * P + 4.0 * NP + 33.0 * S
+    return grad ? 8.0 * PY + 44.0 * P + 4.0 * NP + 33.0 * S
                 : 4.0 * PY + 8.0 * P + 4.0 * NP + 17.0 * S;
 }
 
@@ -172,7 +227,7 @@ int lik_eval(pg_ctx* ctx, const pg_lik_params* p, pg_lik_out* out, bool grad) {
         int rc = seq_prepare(ctx);
         if (rc) return rc;
     }
-    hipStream_t s = p->stream ? static_cast<hipStream_t>(p->stream) : ctx->stream;
+    hipStream_t s = p->stream_given ? static_cast<hipStream_t>(p->stream) : ctx->stream;
     const int grp = grad ? PG_K_LIK_GRAD : PG_K_LIK;
     if (ctx->prof) PG_HIP(hipEventRecord(ctx->pev[grp][0], s));
     const LikSeq v = seq_view(q);
@@ -185,11 +240,14 @@ int lik_eval(pg_ctx* ctx, const pg_lik_params* p, pg_lik_out* out, bool grad) {
     double* part_p = q.part;
     double* part_e = q.part + Y * nP;
     double* part_b = part_e + 4 * nE;
-    double* gphi = grad ? out->grad_logfit_dev : nullptr;
-    launch_lik_forward_sums(s, v, a, nP, part_p, q.year_sums, gphi);
-    launch_lik_slots(s, grad, q.ev_group, nE, q.d_ev_block, q.d_ev_year_blk, v, a, q.year_sums,
-                     gphi, part_e);
-    if (grad) launch_lik_backward(s, v, a, nP, q.year_sums, gphi, part_b);
+    launch_lik_forward_sums(s, v, a, nP, part_p, q.year_sums);
+    launch_lik_slots(s, grad, q.cite_shape, nE, q.d_ev_block, q.d_ev_year_blk, v, a, q.year_sums,
+                     q.share, part_e);
+    if (grad) {
+        const LikSegs sg{q.seg_off, q.evt, q.carry};
+        launch_lik_backward(s, v, a, nP, q.year_sums, q.filled, q.skey, q.share, sg,
+                            out->grad_logfit_dev, part_b);
+    }
     launch_lik_final(s, v.n_years, grad, nP, part_b, a, q.year_sums, q.d_scalars,
                      out->scalars_dev);
     PG_HIP(hipGetLastError());
@@ -273,6 +331,9 @@ int pg_seq_reserve(pg_ctx* ctx, int32_t m, int64_t max_paper_years, int64_t max_
     if (!ctx) return PG_EINVAL;
     if (m < 0 || m > kMaxCites || max_paper_years < 0 || max_new_papers < 0 || max_papers < 0)
         return fail(ctx, PG_EINVAL, "pg_seq_reserve: need 0 <= m <= 64 and non-negative sizes");
+    if (max_new_papers * std::max<int64_t>(m, 1) >= (int64_t(1) << 31) ||
+        max_papers >= (int64_t(1) << 31) - 1)
+        return fail(ctx, PG_ECAP, "pg_seq_reserve: slots and papers must stay below 2^31");
     PG_HIP(hipSetDevice(ctx->cfg.device));
     PG_HIP(hipStreamSynchronize(ctx->stream));
     seq_release(ctx);
