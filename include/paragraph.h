@@ -287,6 +287,80 @@ int pg_pa_loglik(pg_ctx* ctx, const pg_lik_params* p, pg_lik_out* out);
  * same bits. */
 int pg_pa_loglik_grad(pg_ctx* ctx, const pg_lik_params* p, pg_lik_out* out);
 
+/*
+ * ---- Batched ensembles ----
+ * No reference counterpart (the reference runs one simulation per process).
+ * Many independent PA simulations -- each with its own seed graph,
+ * parameters and Philox key -- advance together: every kernel of a year step
+ * covers all of them, and a whole run (all years) is captured once as a
+ * hipGraph and replayed.  Simulation s reproduces, bit for bit, a plain
+ * context loaded with the same seed graph and stepped with the same
+ * parameters (year = first_year + y, year_index = y).
+ */
+typedef struct pg_ensemble pg_ensemble;
+
+typedef struct {
+    uint32_t size;
+    int32_t device;
+    int32_t n_sims;
+    int32_t n_years;           /* growth years per run */
+    int32_t m;                 /* citations per new paper (every simulation) */
+    int32_t bfs_cap;           /* 2-hop cap (<= 4096) */
+    int32_t first_year;        /* calendar year of growth year 0 */
+    double growth_rate;        /* new papers per year: ceil(N * rate) */
+} pg_ensemble_config;
+
+typedef struct {
+    uint32_t size;
+    int32_t n_nb;              /* neighbourhood draws per paper (<= m-1) */
+    int64_t n0;                /* seed papers */
+    int64_t seed_edges;        /* bound on the distinct seed citations loaded */
+    float alpha;               /* in-degree exponent */
+    float inv_tau;             /* 1/tau in 1/years */
+    uint64_t seed;             /* Philox key */
+} pg_sim_params;
+
+typedef struct {
+    uint32_t size;
+    int64_t edges_added;       /* over all simulations and years */
+    int64_t near_boundary;
+    int64_t shortfall;
+    int64_t pa_attempts;
+    double device_ms;          /* GPU time of the run (hipEvents) */
+} pg_ensemble_stats;
+
+/* Launch shapes that change speed, never results. */
+typedef struct {
+    uint32_t size;
+    int32_t pa_lanes;          /* PA attempts evaluated together per paper:
+                                  8, 16, 32, 64 (0 = by citations per paper) */
+    int32_t lik_lanes;         /* likelihood slot pass: lanes per citing paper
+                                  1, 2, 4, 8 (0 = by citations per paper) */
+    int32_t reserved;
+} pg_tuning;
+
+int pg_ensemble_create(const pg_ensemble_config* cfg, const pg_sim_params* sims,
+                       pg_ensemble** out);
+void pg_ensemble_destroy(pg_ensemble* ens);
+/* Error text of the last failed call on ens (NULL: of pg_ensemble_create). */
+const char* pg_ensemble_error(const pg_ensemble* ens);
+/* Seed graph of simulation `sim` (loaded in order 0, 1, ...): n == n0
+ * papers, logfit for all of its final papers; after the last one the seed
+ * state is kept for pg_ensemble_restore. */
+int pg_ensemble_load(pg_ensemble* ens, int32_t sim, int64_t n, const int32_t* birth,
+                     const float* logfit, int64_t n_edges, const int32_t* src,
+                     const int32_t* dst);
+/* All years for all simulations, from the seed state.  use_graph != 0:
+ * captured once into a hipGraph, replayed on later runs. */
+int pg_ensemble_run(pg_ensemble* ens, int32_t use_graph, pg_ensemble_stats* stats);
+int pg_ensemble_restore(pg_ensemble* ens);
+int pg_ensemble_sim_size(pg_ensemble* ens, int32_t sim, int64_t* n_papers, int64_t* n_edges);
+/* One simulation's CSR in its local ids (buffers sized from sim_size). */
+int pg_ensemble_download(pg_ensemble* ens, int32_t sim, int32_t* fwd_off, int32_t* fwd_adj,
+                         int32_t* bwd_off, int32_t* bwd_adj, int32_t* in_deg,
+                         int32_t* birth);
+int pg_ensemble_set_tuning(pg_ensemble* ens, const pg_tuning* t);
+
 /* Ceil growth rule of the reference (src/abm.cu:233-240,
  * src/kernel.cu:1690): ceil(n * rate) in double precision. */
 int64_t pg_growth_count(int64_t n, double rate);

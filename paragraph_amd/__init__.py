@@ -69,6 +69,29 @@ class LikOut(ctypes.Structure):
                 ("grad_logfit_dev", ctypes.c_void_p), ("host3", ctypes.c_void_p)]
 
 
+class EnsembleConfig(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("device", ctypes.c_int32), ("n_sims", ctypes.c_int32),
+                ("n_years", ctypes.c_int32), ("m", ctypes.c_int32), ("bfs_cap", ctypes.c_int32),
+                ("first_year", ctypes.c_int32), ("growth_rate", ctypes.c_double)]
+
+
+class SimParams(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("n_nb", ctypes.c_int32), ("n0", ctypes.c_int64),
+                ("seed_edges", ctypes.c_int64), ("alpha", ctypes.c_float),
+                ("inv_tau", ctypes.c_float), ("seed", ctypes.c_uint64)]
+
+
+class EnsembleStats(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("edges_added", ctypes.c_int64),
+                ("near_boundary", ctypes.c_int64), ("shortfall", ctypes.c_int64),
+                ("pa_attempts", ctypes.c_int64), ("device_ms", ctypes.c_double)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("pa_lanes", ctypes.c_int32),
+                ("lik_lanes", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 KERNEL_GROUPS = ["weights_cdf", "pa_draw", "bfs", "nb_draw", "er", "csr_append", "lik",
                  "lik_grad"]
 
@@ -131,6 +154,17 @@ def lib() -> ctypes.CDLL:
             "pg_seq_get_info": (ctypes.c_int, [vp, P(SeqInfo)]),
             "pg_seq_download": (ctypes.c_int, [vp, i64p, i64p, i32p, i32p, i32p, i8p, i32p,
                                                i32p]),
+            "pg_ensemble_create": (ctypes.c_int, [P(EnsembleConfig), P(SimParams), P(vp)]),
+            "pg_ensemble_destroy": (None, [vp]),
+            "pg_ensemble_error": (ctypes.c_char_p, [vp]),
+            "pg_ensemble_load": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int64, i32p, f32p,
+                                                ctypes.c_int64, i32p, i32p]),
+            "pg_ensemble_run": (ctypes.c_int, [vp, ctypes.c_int32, P(EnsembleStats)]),
+            "pg_ensemble_restore": (ctypes.c_int, [vp]),
+            "pg_ensemble_sim_size": (ctypes.c_int, [vp, ctypes.c_int32, i64p, i64p]),
+            "pg_ensemble_download": (ctypes.c_int, [vp, ctypes.c_int32, i32p, i32p, i32p, i32p,
+                                                    i32p, i32p]),
+            "pg_ensemble_set_tuning": (ctypes.c_int, [vp, P(Tuning)]),
             "pg_pa_loglik": (ctypes.c_int, [vp, P(LikParams), P(LikOut)]),
             "pg_pa_loglik_grad": (ctypes.c_int, [vp, P(LikParams), P(LikOut)]),
         }
@@ -424,3 +458,81 @@ class Context:
                     "pg_profile_read")
         return {name: {"ms": float(ms[i]), "bytes": float(by[i]), "launches": int(n[i])}
                 for i, name in enumerate(KERNEL_GROUPS)}
+
+
+class Ensemble:
+    """Many independent PA simulations advanced together (wraps pg_ensemble).
+
+    sims: list of dicts with n0, alpha, tau, n_nb, seed and seed_edges (an
+    upper bound of the distinct seed citations that load() will receive).
+    """
+
+    def __init__(self, sims: list, n_years: int, m: int, rate: float = 0.03,
+                 bfs_cap: int = 4096, first_year: int = 1, device: int = 0):
+        L = lib()
+        cfg = EnsembleConfig(ctypes.sizeof(EnsembleConfig), device, len(sims), n_years, m,
+                             bfs_cap, first_year, rate)
+        arr = (SimParams * len(sims))()
+        for k, sp in enumerate(sims):
+            tau = sp["tau"]
+            arr[k] = SimParams(ctypes.sizeof(SimParams), int(sp["n_nb"]), int(sp["n0"]),
+                               int(sp["seed_edges"]), float(sp["alpha"]),
+                               0.0 if math.isinf(tau) else 1.0 / tau, int(sp["seed"]))
+        h = ctypes.c_void_p()
+        rc = L.pg_ensemble_create(ctypes.byref(cfg), arr, ctypes.byref(h))
+        if rc:
+            raise PGError(rc, L.pg_ensemble_error(None).decode())
+        self._h = h
+        self.cfg = cfg
+        self.sims = sims
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pg_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc:
+            raise PGError(rc, f"{what}: {lib().pg_ensemble_error(self._h).decode()}")
+
+    def load(self, sim: int, g: SeedGraph):
+        birth = np.ascontiguousarray(g.birth, np.int32)
+        logfit = np.ascontiguousarray(g.logfit, np.float32)
+        src = np.ascontiguousarray(g.src, np.int32)
+        dst = np.ascontiguousarray(g.dst, np.int32)
+        self._check(lib().pg_ensemble_load(self._h, sim, birth.shape[0], _p(birth, ctypes.c_int32),
+                                           _p(logfit, ctypes.c_float), src.shape[0],
+                                           _p(src, ctypes.c_int32), _p(dst, ctypes.c_int32)),
+                    "pg_ensemble_load")
+
+    def run(self, use_graph: bool = True) -> EnsembleStats:
+        st = EnsembleStats(ctypes.sizeof(EnsembleStats))
+        self._check(lib().pg_ensemble_run(self._h, 1 if use_graph else 0, ctypes.byref(st)),
+                    "pg_ensemble_run")
+        return st
+
+    def restore(self):
+        self._check(lib().pg_ensemble_restore(self._h), "pg_ensemble_restore")
+
+    def set_tuning(self, pa_lanes: int = 0, lik_lanes: int = 0):
+        t = Tuning(ctypes.sizeof(Tuning), pa_lanes, lik_lanes, 0)
+        self._check(lib().pg_ensemble_set_tuning(self._h, ctypes.byref(t)),
+                    "pg_ensemble_set_tuning")
+
+    def download(self, sim: int) -> dict:
+        n, e = ctypes.c_int64(), ctypes.c_int64()
+        self._check(lib().pg_ensemble_sim_size(self._h, sim, ctypes.byref(n), ctypes.byref(e)),
+                    "pg_ensemble_sim_size")
+        n, e = n.value, e.value
+        out = {k: np.empty(sz, np.int32) for k, sz in
+               [("fwd_off", n + 1), ("fwd_adj", e), ("bwd_off", n + 1), ("bwd_adj", e),
+                ("in_deg", n), ("birth", n)]}
+        i32 = ctypes.c_int32
+        self._check(lib().pg_ensemble_download(
+            self._h, sim, _p(out["fwd_off"], i32), _p(out["fwd_adj"], i32),
+            _p(out["bwd_off"], i32), _p(out["bwd_adj"], i32), _p(out["in_deg"], i32),
+            _p(out["birth"], i32)), "pg_ensemble_download")
+        return out

@@ -670,8 +670,11 @@ __device__ __forceinline__ bool row_has(const int32_t* row, int n, int32_t x) {
     return false;
 }
 
-__global__ __launch_bounds__(256) void k_nb_draw(int64_t n_exist, int64_t n_new, int m, int n_nb,
-                                                 int cap, uint64_t seed, uint32_t yi,
+// BATCH: the paper's simulation (batch table) supplies n_exist, n_nb, the
+// CDF region and the Philox key; ids in rows and in S are global.
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_nb_draw(Batch bt, int64_t n_exist, int64_t n_new, int m,
+                                                 int n_nb, int cap, uint64_t seed, uint32_t yi,
                                                  const double* __restrict__ cdf,
                                                  const int32_t* __restrict__ root_slot,
                                                  const int32_t* __restrict__ S_buf,
@@ -681,10 +684,22 @@ __global__ __launch_bounds__(256) void k_nb_draw(int64_t n_exist, int64_t n_new,
                                                  int32_t* __restrict__ cnt, int32_t* __restrict__ near,
                                                  int32_t* __restrict__ anext) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (BATCH) n_new = bt.n_new;
     if (i >= n_new) return;
+    int64_t local = i, base = 0;
+    if (BATCH) {
+        const BatchSim& sm = bt.sims[batch_sim_of_new(bt, i)];
+        n_exist = sm.n_exist;
+        n_nb = sm.n_nb;
+        base = sm.base;
+        cdf += base;
+        seed = sm.seed;
+        local = i - sm.new_first;
+        if (n_nb <= 0) return;
+    }
     const int c0 = cnt[i];
     if (c0 <= 0) return;
-    const uint32_t gid = (uint32_t)(n_exist + i);
+    const uint32_t gid = (uint32_t)(n_exist + local);
     int32_t* row = cit + i * m;
     int8_t* kr = kind + i * m;
     const int64_t u = root_slot[i];
@@ -736,7 +751,8 @@ __global__ __launch_bounds__(256) void k_nb_draw(int64_t n_exist, int64_t n_new,
         const int64_t k = cdf_upper_bound(cdf, n_exist, t);
         if (cdf_near(cdf, n_exist, k, t, eps)) ++nr;
         ++a;
-        if (!row_has(row, c, (int32_t)k)) { row[c] = (int32_t)k; kr[c] = 2; ++c; }
+        const int32_t x = (int32_t)(k + base);
+        if (!row_has(row, c, x)) { row[c] = x; kr[c] = 2; ++c; }
     }
     cnt[i] = c;
     near[i] += nr;
@@ -780,9 +796,20 @@ void launch_nb_draw(hipStream_t s, int64_t n_exist, int64_t n_new, int m, int n_
                     int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* near, int32_t* anext) {
     if (n_new <= 0 || n_nb <= 0) return;
     const int64_t nb = (n_new + 255) / 256;
-    hipLaunchKernelGGL(k_nb_draw, dim3((uint32_t)nb), dim3(256), 0, s, n_exist, n_new, m, n_nb, cap,
-                       seed, year_index, cdf, root_slot, S_buf, Q_buf, ns_buf, cit, kind, cnt, near,
-                       anext);
+    hipLaunchKernelGGL(k_nb_draw<false>, dim3((uint32_t)nb), dim3(256), 0, s, Batch{}, n_exist,
+                       n_new, m, n_nb, cap, seed, year_index, cdf, root_slot, S_buf, Q_buf, ns_buf,
+                       cit, kind, cnt, near, anext);
+}
+
+void launch_nb_draw_batch(hipStream_t s, const Batch& b, int m, int cap, uint32_t year_index,
+                          const double* cdf, const int32_t* root_slot, const int32_t* S_buf,
+                          const uint64_t* Q_buf, const int32_t* ns_buf, int32_t* cit, int8_t* kind,
+                          int32_t* cnt, int32_t* near, int32_t* anext) {
+    if (b.n_new <= 0) return;
+    const int64_t nb = (b.n_new + 255) / 256;
+    hipLaunchKernelGGL(k_nb_draw<true>, dim3((uint32_t)nb), dim3(256), 0, s, b, (int64_t)0,
+                       b.n_new, m, 0, cap, (uint64_t)0, year_index, cdf, root_slot, S_buf, Q_buf,
+                       ns_buf, cit, kind, cnt, near, anext);
 }
 
 void launch_bfs_only(hipStream_t s, const CsrView& g, int64_t count, const int32_t* roots, int cap,

@@ -16,17 +16,32 @@
 namespace pg {
 namespace {
 
-__global__ __launch_bounds__(256) void k_pa_compact(int64_t n_exist, int64_t n_new, int m,
+// BATCH: paper i's simulation (batch table) appends its row at the end of
+// that simulation's forward region, fwd_off[base + n_exist], and the
+// (target, citing) pair also goes to the year's contiguous list `stage`
+// (the radix sort's keys) at new_off[i] + rank.
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_pa_compact(Batch bt, int64_t n_exist, int64_t n_new, int m,
                                                     const int32_t* __restrict__ cit,
                                                     const int32_t* __restrict__ cnt,
                                                     const int64_t* __restrict__ new_off,
                                                     int64_t fwd_base, int32_t* __restrict__ fwd_off,
                                                     int32_t* __restrict__ fwd_adj,
                                                     int32_t* __restrict__ src_of,
-                                                    int32_t* __restrict__ add) {
+                                                    int32_t* __restrict__ add,
+                                                    int32_t* __restrict__ stage) {
     const int lane = threadIdx.x & 63;
     const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (BATCH) n_new = bt.n_new;
     if (i >= n_new) return;  // whole wavefront
+    int64_t id0 = n_exist + i;        // global id of paper i
+    int64_t off0 = 0;                 // batch offset of the simulation's first new paper
+    if (BATCH) {
+        const BatchSim& sm = bt.sims[batch_sim_of_new(bt, i)];
+        id0 = sm.base + sm.n_exist + (i - sm.new_first);
+        fwd_base = fwd_off[sm.base + sm.n_exist];
+        off0 = new_off[sm.new_first];
+    }
     const int c = cnt[i];
     const int32_t x = lane < c ? cit[i * m + lane] : 0x7fffffff;
     int rank = 0;
@@ -36,11 +51,29 @@ __global__ __launch_bounds__(256) void k_pa_compact(int64_t n_exist, int64_t n_n
     }
     if (lane < c) {
         const int64_t e = new_off[i] + rank;
-        fwd_adj[fwd_base + e] = x;
-        src_of[e] = (int32_t)(n_exist + i);
+        fwd_adj[fwd_base + e - off0] = x;
+        src_of[e] = (int32_t)id0;
+        if (BATCH) stage[e] = x;
         atomicAdd(add + x, 1);
     }
-    if (lane == 0) fwd_off[n_exist + i + 1] = (int32_t)(fwd_base + new_off[i + 1]);
+    if (lane == 0) fwd_off[id0 + 1] = (int32_t)(fwd_base + new_off[i + 1] - off0);
+}
+
+// Birth year of every new paper of the year (batched).
+__global__ __launch_bounds__(256) void k_birth_batch(Batch bt, int32_t* __restrict__ birth) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= bt.n_new) return;
+    const BatchSim& sm = bt.sims[batch_sim_of_new(bt, i)];
+    birth[sm.base + sm.n_exist + (i - sm.new_first)] = bt.year;
+}
+
+// keys[e] = v for e in [*from, to): pads a sort input whose length is only
+// known on the device.
+__global__ __launch_bounds__(256) void k_fill_tail_u32(uint32_t* __restrict__ keys,
+                                                       const int64_t* __restrict__ from, int64_t to,
+                                                       uint32_t v) {
+    int64_t e = *from + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; e < to; e += (int64_t)gridDim.x * blockDim.x) keys[e] = v;
 }
 
 constexpr int kRT = 256;
@@ -143,6 +176,7 @@ __global__ __launch_bounds__(256) void k_bwd_merge(int64_t n_old, int64_t n_tota
     __shared__ int32_t s_new[kMergeOut + 1];
     __shared__ int32_t s_old[kMergeOut + 1];
     __shared__ int64_t s_rlo, s_nrows;
+    if (e_new < 0) e_new = new_off[n_total];  // total only known on the device
     const int64_t pb = (int64_t)blockIdx.x * kMergeOut;
     if (pb >= e_new) return;
     const int64_t pe = (pb + kMergeOut < e_new ? pb + kMergeOut : e_new) - 1;  // last position
@@ -201,8 +235,33 @@ void launch_pa_compact(hipStream_t s, int64_t n_exist, int64_t n_new, int m, con
                        int32_t* fwd_off, int32_t* fwd_adj, int32_t* src_of, int32_t* add) {
     if (n_new <= 0) return;
     int64_t nb = (n_new + 3) / 4;
-    hipLaunchKernelGGL(k_pa_compact, dim3((uint32_t)nb), dim3(256), 0, s, n_exist, n_new, m, cit,
-                       cnt, new_off, fwd_base, fwd_off, fwd_adj, src_of, add);
+    hipLaunchKernelGGL(k_pa_compact<false>, dim3((uint32_t)nb), dim3(256), 0, s, Batch{}, n_exist,
+                       n_new, m, cit, cnt, new_off, fwd_base, fwd_off, fwd_adj, src_of, add,
+                       (int32_t*)nullptr);
+}
+
+void launch_pa_compact_batch(hipStream_t s, const Batch& b, int m, const int32_t* cit,
+                             const int32_t* cnt, const int64_t* new_off, int32_t* fwd_off,
+                             int32_t* fwd_adj, int32_t* src_of, int32_t* add, int32_t* stage) {
+    if (b.n_new <= 0) return;
+    int64_t nb = (b.n_new + 3) / 4;
+    hipLaunchKernelGGL(k_pa_compact<true>, dim3((uint32_t)nb), dim3(256), 0, s, b, (int64_t)0,
+                       b.n_new, m, cit, cnt, new_off, (int64_t)0, fwd_off, fwd_adj, src_of, add,
+                       stage);
+}
+
+void launch_birth_batch(hipStream_t s, const Batch& b, int32_t* birth) {
+    if (b.n_new <= 0) return;
+    hipLaunchKernelGGL(k_birth_batch, dim3((uint32_t)((b.n_new + 255) / 256)), dim3(256), 0, s, b,
+                       birth);
+}
+
+void launch_fill_tail_u32(hipStream_t s, uint32_t* keys, const int64_t* from, int64_t to,
+                          uint32_t v) {
+    if (to <= 0) return;
+    int64_t nb = (to + 255) / 256;
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(k_fill_tail_u32, dim3((uint32_t)nb), dim3(256), 0, s, keys, from, to, v);
 }
 
 int64_t radix_scratch_elems(int64_t n) {
@@ -252,9 +311,11 @@ void launch_bwd_offsets(hipStream_t s, int64_t n_old, int64_t n_new, const int32
 
 void launch_bwd_merge(hipStream_t s, int64_t n_old, int64_t n_total, const int32_t* bwd_off_old,
                       const int32_t* bwd_adj_old, const int32_t* bwd_off_new,
-                      const uint32_t* sorted_src, int32_t* bwd_adj_new, int64_t e_new) {
-    if (e_new <= 0) return;
-    int64_t nb = (e_new + kMergeOut - 1) / kMergeOut;
+                      const uint32_t* sorted_src, int32_t* bwd_adj_new, int64_t e_new,
+                      int64_t e_bound) {
+    const int64_t e_grid = e_new >= 0 ? e_new : e_bound;
+    if (e_grid <= 0) return;
+    int64_t nb = (e_grid + kMergeOut - 1) / kMergeOut;
     hipLaunchKernelGGL(k_bwd_merge, dim3((uint32_t)nb), dim3(256), 0, s, n_old, n_total, bwd_off_old,
                        bwd_adj_old, bwd_off_new, sorted_src, bwd_adj_new, e_new);
 }

@@ -18,10 +18,14 @@
 namespace pg {
 namespace {
 
-template <int G>
-__global__ __launch_bounds__(256) void k_pa_draw(int64_t n_exist, const double* __restrict__ cdf,
-                                                 int64_t n_new, int m, int n_pa, uint64_t seed,
-                                                 uint32_t yi, int a_max, int32_t* __restrict__ cit,
+// BATCH: paper i of the year's batch order belongs to the simulation found in
+// the batch table, which supplies n_exist, the CDF region, the Philox key
+// and n_pa; picks are written as global ids (region base + local id).
+template <int G, bool BATCH>
+__global__ __launch_bounds__(256) void k_pa_draw(Batch bt, int64_t n_exist,
+                                                 const double* __restrict__ cdf, int64_t n_new,
+                                                 int m, int n_pa, uint64_t seed, uint32_t yi,
+                                                 int a_max, int32_t* __restrict__ cit,
                                                  int8_t* __restrict__ kind, int32_t* __restrict__ cnt,
                                                  int32_t* __restrict__ near,
                                                  int32_t* __restrict__ anext) {
@@ -34,8 +38,20 @@ __global__ __launch_bounds__(256) void k_pa_draw(int64_t n_exist, const double* 
     const int shift = (lane / G) * G;  // first lane of this group in the wavefront
     const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << shift);
     const int64_t i = (int64_t)blockIdx.x * kGroups + grp;
+    int64_t local = i;
+    int64_t base = 0;
+    if (BATCH) {
+        n_new = bt.n_new;
+        const BatchSim& sm = bt.sims[batch_sim_of_new(bt, i < n_new ? i : n_new - 1)];
+        n_exist = sm.n_exist;
+        base = sm.base;
+        cdf += base;
+        seed = sm.seed;
+        n_pa = sm.n_pa;
+        local = i - sm.new_first;
+    }
     const bool live = i < n_new;
-    const uint32_t gid = (uint32_t)(n_exist + i);
+    const uint32_t gid = (uint32_t)(n_exist + local);
     const double Z = cdf[n_exist - 1];
     const double eps = kNearRel * Z;
 
@@ -67,7 +83,7 @@ __global__ __launch_bounds__(256) void k_pa_draw(int64_t n_exist, const double* 
         const int pos = c + __popcll(acc_mask & below);
         const bool take = !dup && pos < n_pa;
         if (take) {
-            cit[i * m + pos] = pick;
+            cit[i * m + pos] = (int32_t)(pick + base);
             kind[i * m + pos] = 0;
             s_acc[grp][pos] = pick;
         }
@@ -104,30 +120,43 @@ __global__ __launch_bounds__(256) void k_pa_draw(int64_t n_exist, const double* 
     }
 }
 
+template <bool BATCH>
+void pa_draw_g(hipStream_t s, int G, const Batch& bt, int64_t n_exist, const double* cdf,
+               int64_t n_new, int m, int n_pa, uint64_t seed, uint32_t year_index, int32_t* cit,
+               int8_t* kind, int32_t* cnt, int32_t* near, int32_t* anext) {
+    const int a_max = kAttemptsPerCite * m;
+    const int64_t per = 256 / G;
+    const dim3 grid((uint32_t)((n_new + per - 1) / per)), block(256);
+    switch (G) {
+        case 8: hipLaunchKernelGGL((k_pa_draw<8, BATCH>), grid, block, 0, s, bt, n_exist, cdf, n_new,
+                                   m, n_pa, seed, year_index, a_max, cit, kind, cnt, near, anext); break;
+        case 16: hipLaunchKernelGGL((k_pa_draw<16, BATCH>), grid, block, 0, s, bt, n_exist, cdf, n_new,
+                                    m, n_pa, seed, year_index, a_max, cit, kind, cnt, near, anext); break;
+        case 32: hipLaunchKernelGGL((k_pa_draw<32, BATCH>), grid, block, 0, s, bt, n_exist, cdf, n_new,
+                                    m, n_pa, seed, year_index, a_max, cit, kind, cnt, near, anext); break;
+        default: hipLaunchKernelGGL((k_pa_draw<64, BATCH>), grid, block, 0, s, bt, n_exist, cdf, n_new,
+                                    m, n_pa, seed, year_index, a_max, cit, kind, cnt, near, anext); break;
+    }
+}
+
 }  // namespace
+
+int pa_draw_lanes(int n_pa) { return n_pa <= 8 ? 8 : n_pa <= 16 ? 16 : n_pa <= 32 ? 32 : 64; }
 
 void launch_pa_draw(hipStream_t s, int64_t n_exist, const double* cdf, int64_t n_new, int m,
                     int n_pa, uint64_t seed, uint32_t year_index, int32_t* cit, int8_t* kind,
-                    int32_t* cnt, int32_t* near, int32_t* anext) {
+                    int32_t* cnt, int32_t* near, int32_t* anext, int lanes) {
     if (n_new <= 0) return;
-    const int a_max = kAttemptsPerCite * m;
-    if (n_pa <= 8) {
-        int64_t nb = (n_new + 31) / 32;
-        hipLaunchKernelGGL(k_pa_draw<8>, dim3((uint32_t)nb), dim3(256), 0, s, n_exist, cdf, n_new,
-                           m, n_pa, seed, year_index, a_max, cit, kind, cnt, near, anext);
-    } else if (n_pa <= 16) {
-        int64_t nb = (n_new + 15) / 16;
-        hipLaunchKernelGGL(k_pa_draw<16>, dim3((uint32_t)nb), dim3(256), 0, s, n_exist, cdf, n_new,
-                           m, n_pa, seed, year_index, a_max, cit, kind, cnt, near, anext);
-    } else if (n_pa <= 32) {
-        int64_t nb = (n_new + 7) / 8;
-        hipLaunchKernelGGL(k_pa_draw<32>, dim3((uint32_t)nb), dim3(256), 0, s, n_exist, cdf, n_new,
-                           m, n_pa, seed, year_index, a_max, cit, kind, cnt, near, anext);
-    } else {
-        int64_t nb = (n_new + 3) / 4;
-        hipLaunchKernelGGL(k_pa_draw<64>, dim3((uint32_t)nb), dim3(256), 0, s, n_exist, cdf, n_new,
-                           m, n_pa, seed, year_index, a_max, cit, kind, cnt, near, anext);
-    }
+    const int G = lanes > 0 ? lanes : pa_draw_lanes(n_pa);
+    pa_draw_g<false>(s, G, Batch{}, n_exist, cdf, n_new, m, n_pa, seed, year_index, cit, kind, cnt,
+                     near, anext);
+}
+
+void launch_pa_draw_batch(hipStream_t s, const Batch& b, int m, uint32_t year_index, int lanes,
+                          const double* cdf, int32_t* cit, int8_t* kind, int32_t* cnt,
+                          int32_t* near, int32_t* anext) {
+    if (b.n_new <= 0) return;
+    pa_draw_g<true>(s, lanes, b, 0, cdf, b.n_new, m, 0, 0, year_index, cit, kind, cnt, near, anext);
 }
 
 }  // namespace pg

@@ -57,13 +57,44 @@ __device__ __forceinline__ TileScan tile_local_scan(const float* wv, double* s_w
     return t;
 }
 
+// Tile of this block: plain (tile blockIdx.x of [0, n)) or batched (the
+// simulation holding batch tile blockIdx.x; n becomes its region end and
+// alpha/inv_tau its own).
+struct TileAt {
+    int64_t start, end;
+    float alpha, inv_tau;
+};
+
+template <bool BATCH>
+__device__ __forceinline__ TileAt tile_at(const Batch& b, int64_t n, float alpha, float inv_tau) {
+    TileAt t;
+    if (BATCH) {
+        const BatchSim& sm = b.sims[batch_sim_of_tile(b, blockIdx.x)];
+        t.start = sm.base + ((int64_t)blockIdx.x - sm.tile_first) * kTile;
+        t.end = sm.base + sm.n_exist;
+        t.alpha = sm.alpha;
+        t.inv_tau = sm.inv_tau;
+    } else {
+        t.start = (int64_t)blockIdx.x * kTile;
+        t.end = n;
+        t.alpha = alpha;
+        t.inv_tau = inv_tau;
+    }
+    return t;
+}
+
 // Pass 1: weights (stored) and each tile's total.
+template <bool BATCH>
 __global__ __launch_bounds__(kScanThreads) void k_weights_tiles(
-    int64_t n, const int32_t* __restrict__ in_deg, const int32_t* __restrict__ birth,
-    const float* __restrict__ logfit, int32_t year, float alpha, float inv_tau,
-    float* __restrict__ w_out, double* __restrict__ tile_total) {
+    Batch bt, int64_t n_plain, const int32_t* __restrict__ in_deg,
+    const int32_t* __restrict__ birth, const float* __restrict__ logfit, int32_t year,
+    float alpha_plain, float inv_tau_plain, float* __restrict__ w_out,
+    double* __restrict__ tile_total) {
     __shared__ double s_wave[4];
-    const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kScanItems;
+    const TileAt ta = tile_at<BATCH>(bt, n_plain, alpha_plain, inv_tau_plain);
+    const int64_t n = ta.end;
+    const float alpha = ta.alpha, inv_tau = ta.inv_tau;
+    const int64_t base = ta.start + (int64_t)threadIdx.x * kScanItems;
     float wv[kScanItems];
     if (base + kScanItems <= n) {
         const int4* kd = reinterpret_cast<const int4*>(in_deg + base);
@@ -100,10 +131,17 @@ __global__ __launch_bounds__(kScanThreads) void k_weights_tiles(
 
 // Pass 2: exclusive prefix of the tile totals, chained left to right by one
 // lane from LDS (the loads and stores of each 4096-tile chunk are spread
-// over the block).
-__global__ __launch_bounds__(256) void k_tile_chain(double* __restrict__ tile_total, int64_t ntiles) {
+// over the block).  Batched: block s chains simulation s's tiles.
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_tile_chain(Batch bt, double* __restrict__ tile_total,
+                                                    int64_t ntiles) {
     __shared__ double s_t[kTile];
     __shared__ double s_carry;
+    if (BATCH) {
+        const BatchSim& sm = bt.sims[blockIdx.x];
+        tile_total += sm.tile_first;
+        ntiles = (sm.n_exist + kTile - 1) / kTile;
+    }
     if (threadIdx.x == 0) s_carry = 0.0;
     for (int64_t c0 = 0; c0 < ntiles; c0 += kTile) {
         const int64_t cn = (ntiles - c0 < kTile) ? ntiles - c0 : kTile;
@@ -125,11 +163,15 @@ __global__ __launch_bounds__(256) void k_tile_chain(double* __restrict__ tile_to
 }
 
 // Pass 3: every element's CDF value (tile prefix + local value).
-__global__ __launch_bounds__(kScanThreads) void k_cdf_down(int64_t n, const float* __restrict__ w,
+template <bool BATCH>
+__global__ __launch_bounds__(kScanThreads) void k_cdf_down(Batch bt, int64_t n_plain,
+                                                          const float* __restrict__ w,
                                                           const double* __restrict__ tile_excl,
                                                           double* __restrict__ cdf_out) {
     __shared__ double s_wave[4];
-    const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kScanItems;
+    const TileAt ta = tile_at<BATCH>(bt, n_plain, 0.0f, 0.0f);
+    const int64_t n = ta.end;
+    const int64_t base = ta.start + (int64_t)threadIdx.x * kScanItems;
     float wv[kScanItems];
     if (base + kScanItems <= n) {
         const float4* wi = reinterpret_cast<const float4*>(w + base);
@@ -244,18 +286,39 @@ __global__ void k_sum_i32(const int32_t* __restrict__ p, int64_t n, unsigned lon
     if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
 }
 
+__global__ void k_acc_i64(const int64_t* __restrict__ src, unsigned long long* dst) {
+    if (threadIdx.x == 0) *dst += (unsigned long long)*src;
+}
+
 }  // namespace
+
+void launch_acc_i64(hipStream_t s, const int64_t* src, unsigned long long* dst) {
+    hipLaunchKernelGGL(k_acc_i64, dim3(1), dim3(64), 0, s, src, dst);
+}
 
 void launch_weights_cdf(hipStream_t s, int64_t n, const int32_t* in_deg, const int32_t* birth,
                         const float* logfit, int32_t year, float alpha, float inv_tau, float* w,
                         double* cdf, double* tile_scratch) {
     if (n <= 0) return;
     int64_t nb = (n + kTile - 1) / kTile;
-    hipLaunchKernelGGL(k_weights_tiles, dim3((uint32_t)nb), dim3(kScanThreads), 0, s, n, in_deg,
-                       birth, logfit, year, alpha, inv_tau, w, tile_scratch);
-    hipLaunchKernelGGL(k_tile_chain, dim3(1), dim3(256), 0, s, tile_scratch, nb);
-    hipLaunchKernelGGL(k_cdf_down, dim3((uint32_t)nb), dim3(kScanThreads), 0, s, n, w, tile_scratch,
-                       cdf);
+    const Batch none{};
+    hipLaunchKernelGGL(k_weights_tiles<false>, dim3((uint32_t)nb), dim3(kScanThreads), 0, s, none,
+                       n, in_deg, birth, logfit, year, alpha, inv_tau, w, tile_scratch);
+    hipLaunchKernelGGL(k_tile_chain<false>, dim3(1), dim3(256), 0, s, none, tile_scratch, nb);
+    hipLaunchKernelGGL(k_cdf_down<false>, dim3((uint32_t)nb), dim3(kScanThreads), 0, s, none, n, w,
+                       tile_scratch, cdf);
+}
+
+void launch_weights_cdf_batch(hipStream_t s, const Batch& b, const int32_t* in_deg,
+                              const int32_t* birth, const float* logfit, float* w, double* cdf,
+                              double* tile_scratch) {
+    if (b.n_tiles <= 0) return;
+    hipLaunchKernelGGL(k_weights_tiles<true>, dim3((uint32_t)b.n_tiles), dim3(kScanThreads), 0, s,
+                       b, (int64_t)0, in_deg, birth, logfit, b.year, 0.0f, 0.0f, w, tile_scratch);
+    hipLaunchKernelGGL(k_tile_chain<true>, dim3((uint32_t)b.n_sims), dim3(256), 0, s, b,
+                       tile_scratch, (int64_t)0);
+    hipLaunchKernelGGL(k_cdf_down<true>, dim3((uint32_t)b.n_tiles), dim3(kScanThreads), 0, s, b,
+                       (int64_t)0, w, tile_scratch, cdf);
 }
 
 int64_t scan_scratch_elems(int64_t n) { return (n + kTile - 1) / kTile + 2; }

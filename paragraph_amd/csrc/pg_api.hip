@@ -85,6 +85,13 @@ int key_bits_for(int64_t n) {
     return b;
 }
 
+
+CsrView view(const pg_ctx* c) {
+    return CsrView{c->fwd_off, c->fwd_adj, c->bwd_off[c->cur], c->bwd_adj[c->cur], c->n};
+}
+
+}  // namespace
+
 int ensure_nb_buffers(pg_ctx* ctx) {
     if (ctx->S_buf) return PG_OK;
     const int64_t D = std::max<int64_t>(ctx->cfg.max_new_per_year, 1);
@@ -102,12 +109,6 @@ int ensure_nb_buffers(pg_ctx* ctx) {
         return fail(ctx, PG_ENOMEM, "neighbourhood buffers: device allocation failed");
     return PG_OK;
 }
-
-CsrView view(const pg_ctx* c) {
-    return CsrView{c->fwd_off, c->fwd_adj, c->bwd_off[c->cur], c->bwd_adj[c->cur], c->n};
-}
-
-}  // namespace
 
 extern "C" {
 

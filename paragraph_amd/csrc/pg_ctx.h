@@ -152,6 +152,9 @@ inline hipError_t dalloc(T** p, int64_t count) {
     return hipMalloc(reinterpret_cast<void**>(p), (size_t)count * sizeof(T));
 }
 
+// Neighbourhood buffers sized by cfg.max_new_per_year and cfg.bfs_cap.
+int ensure_nb_buffers(pg_ctx* ctx);
+
 // Sequence store (pg_lik.hip): frees its device memory; appends the PA year
 // that pg_year_step is about to run (in-degree snapshot) and has just drawn.
 void seq_release(pg_ctx* ctx);

@@ -29,6 +29,48 @@ __device__ __forceinline__ bool cdf_near(const double* __restrict__ cdf, int64_t
     return (k > 0 && t - cdf[k - 1] <= eps) || (k < n - 1 && cdf[k] - t <= eps);
 }
 
+// ---- batched simulations (ensemble) ----
+// Every simulation of an ensemble owns a region of one global paper-id
+// space starting at `base` (a multiple of kTile, so its CDF tiles and bits
+// are those of a plain run); ids are local + base.  One BatchSim per
+// simulation and year, in region order.
+struct BatchSim {
+    int64_t base;          // global id of the simulation's paper 0
+    int64_t n_exist;       // papers at the start of the year
+    int64_t new_first;     // its first new paper in the year's batch order
+    int64_t n_new;
+    int64_t tile_first;    // its first CDF tile in the year's tile order
+    uint64_t seed;         // Philox key
+    float alpha, inv_tau;
+    int32_t n_pa, n_nb;
+};
+
+struct Batch {
+    const BatchSim* sims;  // device table [n_sims]
+    int32_t n_sims;
+    int32_t year;
+    int64_t n_tiles;       // CDF tiles of the year over all simulations
+    int64_t n_new;         // new papers of the year over all simulations
+};
+
+// Simulation of batch-order new paper p / tile t (binary search).
+__device__ __forceinline__ int batch_sim_of_new(const Batch& b, int64_t p) {
+    int lo = 0, hi = b.n_sims - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b.sims[mid].new_first <= p) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+__device__ __forceinline__ int batch_sim_of_tile(const Batch& b, int64_t t) {
+    int lo = 0, hi = b.n_sims - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b.sims[mid].tile_first <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 // ---- launchers (stream-ordered, no host synchronisation) ----
 
 // w[i] = pa_weight(...) for i < n; cdf[i] = FP64 inclusive prefix of w.
@@ -37,15 +79,29 @@ void launch_weights_cdf(hipStream_t s, int64_t n, const int32_t* in_deg, const i
                         const float* logfit, int32_t year, float alpha, float inv_tau,
                         float* w, double* cdf, double* tile_scratch);
 
+// Batched form: per simulation of the year, w and the FP64 CDF over its
+// papers [base, base + n_exist), with its own alpha/inv_tau; tile_scratch:
+// b.n_tiles doubles.
+void launch_weights_cdf_batch(hipStream_t s, const Batch& b, const int32_t* in_deg,
+                              const int32_t* birth, const float* logfit, float* w, double* cdf,
+                              double* tile_scratch);
+
 // Exclusive prefix sum of int32 in[0,n) into int64 out[0,n]; out[n] = total.
 // scratch: at least scan_scratch_elems(n) int64.
 int64_t scan_scratch_elems(int64_t n);
 void launch_excl_scan_i32(hipStream_t s, const int32_t* in, int64_t n, int64_t* out, int64_t* scratch);
 
 // PA phase 1: n_pa distinct draws per new paper from the global CDF.
+// lanes: attempts evaluated together per paper (8/16/32/64; 0 = by n_pa).
+int pa_draw_lanes(int n_pa);
 void launch_pa_draw(hipStream_t s, int64_t n_exist, const double* cdf, int64_t n_new, int m,
                     int n_pa, uint64_t seed, uint32_t year_index, int32_t* cit, int8_t* kind,
-                    int32_t* cnt, int32_t* near, int32_t* anext);
+                    int32_t* cnt, int32_t* near, int32_t* anext, int lanes = 0);
+// Batched: every simulation's new papers of the year (cdf: the global FP64
+// CDF array; targets are written as global ids).
+void launch_pa_draw_batch(hipStream_t s, const Batch& b, int m, uint32_t year_index, int lanes,
+                          const double* cdf, int32_t* cit, int8_t* kind, int32_t* cnt,
+                          int32_t* near, int32_t* anext);
 
 struct CsrView {
     const int32_t* fwd_off;
@@ -77,6 +133,11 @@ void launch_nb_draw(hipStream_t s, int64_t n_exist, int64_t n_new, int m, int n_
                     const int32_t* S_buf, const uint64_t* Q_buf, const int32_t* ns_buf,
                     int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* near, int32_t* anext);
 
+void launch_nb_draw_batch(hipStream_t s, const Batch& b, int m, int cap, uint32_t year_index,
+                          const double* cdf, const int32_t* root_slot, const int32_t* S_buf,
+                          const uint64_t* Q_buf, const int32_t* ns_buf, int32_t* cit, int8_t* kind,
+                          int32_t* cnt, int32_t* near, int32_t* anext);
+
 // Stand-alone 2-hop neighbourhoods (out[i*cap ...], out_len[i]) by the
 // workgroup path; ticks (nullable) gets per root the wall-clock ticks to the
 // end of the 1-hop level, to the end, and the 2-hop candidate entries.
@@ -104,10 +165,22 @@ void launch_bwd_offsets(hipStream_t s, int64_t n_old, int64_t n_new, const int32
                         const int32_t* add, int32_t* bwd_off_new, int32_t* in_deg, int64_t* scratch);
 
 // bwd_adj_new = old rows followed by the row's new citers (sorted_src,
-// grouped by target in ascending citing order).
+// grouped by target in ascending citing order).  e_new < 0: the edge total
+// is read on the device (bwd_off_new[n_total]) and e_bound sizes the grid.
 void launch_bwd_merge(hipStream_t s, int64_t n_old, int64_t n_total, const int32_t* bwd_off_old,
                       const int32_t* bwd_adj_old, const int32_t* bwd_off_new,
-                      const uint32_t* sorted_src, int32_t* bwd_adj_new, int64_t e_new);
+                      const uint32_t* sorted_src, int32_t* bwd_adj_new, int64_t e_new,
+                      int64_t e_bound = 0);
+
+// Batched forward append (rows at the end of each simulation's region; the
+// year's (target, citing) pairs also listed contiguously in stage/src_of).
+void launch_pa_compact_batch(hipStream_t s, const Batch& b, int m, const int32_t* cit,
+                             const int32_t* cnt, const int64_t* new_off, int32_t* fwd_off,
+                             int32_t* fwd_adj, int32_t* src_of, int32_t* add, int32_t* stage);
+void launch_birth_batch(hipStream_t s, const Batch& b, int32_t* birth);
+// keys[e] = v for e in [*from, to) (from read on the device).
+void launch_fill_tail_u32(hipStream_t s, uint32_t* keys, const int64_t* from, int64_t to,
+                          uint32_t v);
 
 // ---- ER ----
 void launch_er_count(hipStream_t s, int64_t n_exist, int64_t n_new, double p, uint64_t seed,
@@ -119,6 +192,8 @@ void launch_er_write(hipStream_t s, int64_t n_exist, int64_t n_new, double p, ui
 // Small helpers.
 void launch_fill_i32(hipStream_t s, int32_t* p, int32_t v, int64_t n);
 void launch_sum_i32(hipStream_t s, const int32_t* p, int64_t n, unsigned long long* out);
+// *dst += *src (one thread).
+void launch_acc_i64(hipStream_t s, const int64_t* src, unsigned long long* dst);
 
 // ---- PA likelihood (k_lik.hip) ----
 constexpr int kLikMaxYears = 256;   // years per sequence (LDS per-year partials)

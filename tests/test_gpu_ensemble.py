@@ -1,0 +1,120 @@
+"""Batched ensemble against plain runs (needs an MI355X).
+
+Simulation s of an ensemble must reproduce, bit for bit, a plain context
+loaded with the same seed graph and stepped with the same parameters:
+forward and backward CSR, in-degrees and births after every run, for the
+eager path and for the captured hipGraph and its replays.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import paragraph_amd as pg
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+SIMS = [
+    dict(n0=3000, alpha=1.0, tau=5.0, n_nb=3, seed=11),
+    dict(n0=8000, alpha=0.6, tau=2.0, n_nb=0, seed=12),
+    dict(n0=500, alpha=1.4, tau=8.0, n_nb=5, seed=13),
+    dict(n0=20000, alpha=1.1, tau=3.0, n_nb=2, seed=14),
+    dict(n0=4096, alpha=0.9, tau=6.0, n_nb=4, seed=15),
+    dict(n0=12000, alpha=1.2, tau=4.0, n_nb=3, seed=16),
+]
+M, YEARS, RATE = 10, 3, 0.05
+
+
+def seed_graphs(sims):
+    out = []
+    for k, sp in enumerate(sims):
+        sched = pg.growth_schedule(sp["n0"], RATE, YEARS)
+        g = pg.seed_graph(100 + k, sp["n0"], 8.0, n_final=sp["n0"] + sum(sched))
+        out.append((g, sched))
+    return out
+
+
+def plain_run(sp, g, sched):
+    n_final = g.n + sum(sched)
+    ctx = pg.Context(n_final, g.src.size + sum(sched) * M + 16, max(sched), max_cites=M,
+                     seed=sp["seed"])
+    ctx.load(g)
+    edges = 0
+    for yi, d in enumerate(sched):
+        st = ctx.year_pa(1 + yi, yi, d, M, sp["n_nb"], sp["alpha"], sp["tau"])
+        edges += st.edges_added
+    out = ctx.download()
+    ctx.close()
+    return out, edges
+
+
+def make_ensemble(sims, graphs):
+    spec = [dict(sp, seed_edges=int(g.src.size)) for sp, (g, _) in zip(sims, graphs)]
+    ens = pg.Ensemble(spec, YEARS, M, RATE)
+    for k, (g, _) in enumerate(graphs):
+        ens.load(k, g)
+    return ens
+
+
+def check_same(ens, refs):
+    for k, ref in enumerate(refs):
+        got = ens.download(k)
+        for key in ("fwd_off", "fwd_adj", "bwd_off", "bwd_adj", "in_deg", "birth"):
+            assert np.array_equal(got[key], ref[key]), f"sim {k}: {key} differs"
+
+
+def test_ensemble_equals_plain_runs_eager_and_graph():
+    graphs = seed_graphs(SIMS)
+    refs, total = [], 0
+    for sp, (g, sched) in zip(SIMS, graphs):
+        r, e = plain_run(sp, g, sched)
+        refs.append(r)
+        total += e
+    log(f"plain runs: {total} edges")
+    ens = make_ensemble(SIMS, graphs)
+    st = ens.run(use_graph=False)
+    log(f"eager: {st.edges_added} edges, {st.device_ms:.2f} ms")
+    assert st.edges_added == total
+    check_same(ens, refs)
+    for rep in range(3):
+        ens.restore()
+        st = ens.run(use_graph=True)
+        log(f"graph run {rep}: {st.edges_added} edges, {st.device_ms:.2f} ms")
+        assert st.edges_added == total
+        check_same(ens, refs)
+
+
+def test_ensemble_tuning_does_not_change_results():
+    sims = SIMS[:3]
+    graphs = seed_graphs(sims)
+    refs = [plain_run(sp, g, sched)[0] for sp, (g, sched) in zip(sims, graphs)]
+    ens = make_ensemble(sims, graphs)
+    for lanes in (8, 16, 32, 64):
+        ens.set_tuning(pa_lanes=lanes)
+        ens.restore()
+        ens.run(use_graph=True)
+        check_same(ens, refs)
+
+
+def test_ensemble_errors():
+    with pytest.raises(pg.PGError, match="EINVAL"):
+        pg.Ensemble([dict(n0=100, alpha=1.0, tau=5.0, n_nb=12, seed=1, seed_edges=10)], 2, 10)
+    sims = SIMS[:2]
+    graphs = seed_graphs(sims)
+    spec = [dict(sp, seed_edges=int(g.src.size)) for sp, (g, _) in zip(sims, graphs)]
+    ens = pg.Ensemble(spec, YEARS, M, RATE)
+    with pytest.raises(pg.PGError, match="ESTATE"):   # out of order
+        ens.load(1, graphs[1][0])
+    with pytest.raises(pg.PGError, match="ESTATE"):   # not loaded
+        ens.run()
+    ens.load(0, graphs[0][0])
+    ens.load(1, graphs[1][0])
+    ens.run(use_graph=False)
+    with pytest.raises(pg.PGError, match="ESTATE"):   # must restore first
+        ens.run()
