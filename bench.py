@@ -230,6 +230,98 @@ def likelihood_bench(args) -> dict:
     return res
 
 
+def ensemble_params(n_sims: int, m: int = 20):
+    """The ensemble's simulations: seed sizes log-uniform in [10k, 500k],
+    alpha ~ U[0.5, 1.5], tau ~ U[1, 10] years, neighbourhood fraction
+    ~ U[0, 0.5] (n_nb = round(f*m), at most m-1), Philox keys 0..n_sims-1.
+    Drawn from numpy's PCG64 with seed 3."""
+    rng = np.random.default_rng(3)
+    out = []
+    for s in range(n_sims):
+        n0 = int(round(math.exp(rng.uniform(math.log(1e4), math.log(5e5)))))
+        alpha = float(rng.uniform(0.5, 1.5))
+        tau = float(rng.uniform(1.0, 10.0))
+        f_nb = float(rng.uniform(0.0, 0.5))
+        out.append(dict(n0=n0, alpha=alpha, tau=tau, n_nb=min(int(math.floor(f_nb * m + 0.5)), m - 1),
+                        seed=s))
+    return out
+
+
+def ensemble_bench(args) -> dict:
+    """Batched ensemble: n_sims PA simulations (seed graphs as the 2M-paper
+    workload's: Poisson(10) citations, births uniform in [-20,0],
+    lognormal(0,0.5) fitness), each grown 10 years at 3% with 20 citations
+    per paper, run together (one hipGraph replay for the whole run) and, for
+    comparison, one after another as plain runs (their 10 year steps timed;
+    context setup and loading excluded).  Both must add the same edges."""
+    m, years, rate = 20, 10, 0.03
+    sims = ensemble_params(args.ens_sims, m)
+    t0 = time.perf_counter()
+    graphs = []
+    for s, sp in enumerate(sims):
+        sched = pg.growth_schedule(sp["n0"], rate, years)
+        g = pg.seed_graph(sp["seed"], sp["n0"], 10.0, n_final=sp["n0"] + sum(sched))
+        graphs.append((g, sched))
+        sp["seed_edges"] = int(g.src.size)
+    n_seed = sum(sp["n0"] for sp in sims)
+    n_new = sum(sum(sc) for _, sc in graphs)
+    eprint(f"ensemble: {len(sims)} simulations, {n_seed} seed papers, {n_new} new papers "
+           f"(seed graphs {time.perf_counter() - t0:.1f} s)")
+    t1 = time.perf_counter()
+    ens = pg.Ensemble(sims, years, m, rate)
+    for s, (g, _) in enumerate(graphs):
+        ens.load(s, g)
+    eprint(f"ensemble loaded ({time.perf_counter() - t1:.1f} s)")
+    st = ens.run(use_graph=False)
+    eager_ms = st.device_ms
+    ens.restore()
+    st = ens.run(use_graph=True)       # captures the graph
+    capture_ms = st.device_ms
+    walls, devs = [], []
+    for _ in range(args.ens_reps):
+        ens.restore()
+        t2 = time.perf_counter()
+        st = ens.run(use_graph=True)
+        walls.append(time.perf_counter() - t2)
+        devs.append(st.device_ms)
+    edges = st.edges_added
+    wall = float(np.median(walls))
+    dev = float(np.median(devs))
+    eprint(f"ensemble run: {edges} edges, {wall * 1e3:.1f} ms wall ({dev:.1f} ms device; eager "
+           f"{eager_ms:.1f} ms, first graph run {capture_ms:.1f} ms), {edges / wall:.3e} edges/s, "
+           f"near-boundary {st.near_boundary}, shortfall {st.shortfall}")
+    ens.close()
+    # the same simulations as plain runs, one after another
+    plain_s, plain_edges = 0.0, 0
+    for sp, (g, sched) in zip(sims, graphs):
+        ctx = pg.Context(g.n + sum(sched), g.src.size + sum(sched) * m + 16, max(sched),
+                         max_cites=m, bfs_cap=4096, seed=sp["seed"])
+        ctx.load(g)
+        t3 = time.perf_counter()
+        for yi, d in enumerate(sched):
+            plain_edges += ctx.year_pa(1 + yi, yi, d, m, sp["n_nb"], sp["alpha"], sp["tau"]).edges_added
+        plain_s += time.perf_counter() - t3
+        ctx.close()
+    eprint(f"plain runs: {plain_edges} edges in {plain_s:.3f} s ({plain_edges / plain_s:.3e} edges/s)")
+    if plain_edges != edges:
+        raise RuntimeError(f"ensemble added {edges} edges, plain runs {plain_edges}")
+    return {
+        "metric": f"ensemble citation edges/s ({len(sims)} simulations, 10y)",
+        "value": edges / wall,
+        "unit": "citation edges/s",
+        "ms_per_run": wall * 1e3,
+        "device_ms_per_run": dev,
+        "plain_sequential_edges_per_s": plain_edges / plain_s,
+        "plain_sequential_s": plain_s,
+        "vs_plain": (edges / wall) / (plain_edges / plain_s),
+        "config": {"workload": f"{len(sims)} PA simulations, seeds log-uniform in [1e4, 5e5] "
+                               f"papers, alpha~U[0.5,1.5], tau~U[1,10], f_nb~U[0,0.5], m=20, "
+                               f"10y at 3%/y, Philox keys 0..{len(sims) - 1}",
+                   "seed_papers": n_seed, "new_papers": n_new, "edges_added": edges,
+                   "near_boundary_attempts": st.near_boundary, "hipgraph": True},
+    }
+
+
 def copy_calibration(nbytes: int = 1 << 30) -> None:
     """One plain device-to-device copy of nbytes (HIP runtime copy kernel):
     the reference point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE
@@ -262,6 +354,11 @@ def main():
                     help="timed likelihood-gradient evaluations")
     ap.add_argument("--fit-steps", type=int, default=100)
     ap.add_argument("--no-lik", action="store_true", help="skip the likelihood part")
+    ap.add_argument("--no-ens", action="store_true", help="skip the ensemble part")
+    ap.add_argument("--ens-only", action="store_true",
+                    help="only the ensemble part (its JSON object on stdout)")
+    ap.add_argument("--ens-sims", type=int, default=256)
+    ap.add_argument("--ens-reps", type=int, default=3)
     ap.add_argument("--lik-only", action="store_true",
                     help="only the likelihood part (its JSON object on stdout)")
     args = ap.parse_args()
@@ -270,6 +367,9 @@ def main():
         return
     if args.lik_only:
         print(json.dumps(likelihood_bench(args)), flush=True)
+        return
+    if args.ens_only:
+        print(json.dumps(ensemble_bench(args)), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -359,6 +459,8 @@ def main():
             out["cpu_baseline"] = None
     if not args.no_lik:
         out["likelihood"] = likelihood_bench(args)
+    if not args.no_ens:
+        out["ensemble"] = ensemble_bench(args)
     print(json.dumps(out), flush=True)
 
 

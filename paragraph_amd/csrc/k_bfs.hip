@@ -547,7 +547,11 @@ __global__ __launch_bounds__(64) void k_bfs_small(CsrView g, const float* __rest
     }
     const bool check_f1 = n1 > n1c;
     const int32_t f1max = n1c > 0 ? Ls.f1[n1c - 1] : -1;
-    // gather F1c and the rows of F1c (one row pair per lane, 64 at a time)
+    // gather F1c and the rows of F1c, 64 row pairs at a time: lane q owns
+    // row pair q (start `pos` in win); the wave then fills the chunk's
+    // positions lane-strided, each position finding its row pair by a
+    // binary search over the lanes' starts, so consecutive lanes read
+    // consecutive row entries
     for (int q = lane; q < n1c; q += 64) Ls.win[q] = Ls.f1[q];
     int run = n1c;
     for (int q0 = 0; q0 < n1c; q0 += 64) {
@@ -557,16 +561,31 @@ __global__ __launch_bounds__(64) void k_bfs_small(CsrView g, const float* __rest
             const int32_t v = Ls.f1[q];
             a = g.fwd_off[v]; b = g.fwd_off[v + 1]; c = g.bwd_off[v]; d = g.bwd_off[v + 1];
         }
-        const int len = (b - a) + (d - c);
+        const int fl = b - a;
+        const int len = fl + (d - c);
         int x = len;
         for (int off = 1; off < 64; off <<= 1) {
             int y = __shfl_up(x, off, 64);
             if (lane >= off) x += y;
         }
-        const int pos = run + x - len;
-        for (int k = 0; k < b - a; ++k) Ls.win[pos + k] = g.fwd_adj[a + k];
-        for (int k = 0; k < d - c; ++k) Ls.win[pos + (b - a) + k] = g.bwd_adj[c + k];
-        run += __shfl(x, 63, 64);
+        const int pos = x - len;                     // start within the chunk
+        const int total = __shfl(x, 63, 64);
+        for (int p0 = 0; p0 < total; p0 += 64) {
+            const int pp = p0 + lane;
+            // last lane whose start is <= pp (starts ascend across lanes)
+            int r = 0;
+#pragma unroll
+            for (int st = 32; st > 0; st >>= 1) {
+                const int sp = __shfl(pos, r + st, 64);
+                if (sp <= pp) r += st;
+            }
+            const int k = pp - __shfl(pos, r, 64);
+            const int32_t ar = __shfl(a, r, 64);
+            const int flr = __shfl(fl, r, 64);
+            const int32_t cr = __shfl(c, r, 64);
+            if (pp < total) Ls.win[run + pp] = k < flr ? g.fwd_adj[ar + k] : g.bwd_adj[cr + k - flr];
+        }
+        run += total;
     }
     int P2 = 1;
     while (P2 < (int)tot2) P2 <<= 1;

@@ -165,8 +165,12 @@ __device__ __forceinline__ int64_t row_of_pos(const int32_t* off, int64_t lo, in
 
 // Each block writes kMergeOut consecutive positions of the new backward
 // adjacency.  The new and old offsets of the rows those positions span are
-// staged in LDS once per block, so a thread locates its first row with a
-// short LDS search instead of a search over all N offsets.
+// staged in LDS; every position's row comes from a row-start table (each
+// row marks its first position, a prefix maximum fills the rest), and then
+// consecutive threads write consecutive positions, so reads of old rows,
+// of the sorted new citers and the writes are all coalesced.  Blocks whose
+// positions span more than kMergeOut rows (long runs of empty rows) search
+// the offsets per position instead.
 __global__ __launch_bounds__(256) void k_bwd_merge(int64_t n_old, int64_t n_total,
                                                    const int32_t* __restrict__ old_off,
                                                    const int32_t* __restrict__ old_adj,
@@ -175,11 +179,14 @@ __global__ __launch_bounds__(256) void k_bwd_merge(int64_t n_old, int64_t n_tota
                                                    int32_t* __restrict__ out, int64_t e_new) {
     __shared__ int32_t s_new[kMergeOut + 1];
     __shared__ int32_t s_old[kMergeOut + 1];
+    __shared__ int32_t s_row[kMergeOut];
+    __shared__ int32_t s_max[256];
     __shared__ int64_t s_rlo, s_nrows;
     if (e_new < 0) e_new = new_off[n_total];  // total only known on the device
     const int64_t pb = (int64_t)blockIdx.x * kMergeOut;
     if (pb >= e_new) return;
     const int64_t pe = (pb + kMergeOut < e_new ? pb + kMergeOut : e_new) - 1;  // last position
+    const int cnt = (int)(pe - pb + 1);
     if (threadIdx.x == 0) {
         const int64_t r0 = row_of_pos(new_off, 0, n_total + 1, pb);
         const int64_t r1 = row_of_pos(new_off, r0, n_total + 1, pe);
@@ -188,41 +195,59 @@ __global__ __launch_bounds__(256) void k_bwd_merge(int64_t n_old, int64_t n_tota
     }
     __syncthreads();
     const int64_t rlo = s_rlo, nrows = s_nrows;
-    const bool staged = nrows <= kMergeOut;
-    if (staged) {
+    if (nrows <= kMergeOut) {
         for (int64_t k = threadIdx.x; k <= nrows; k += 256) {
             s_new[k] = new_off[rlo + k];
             const int64_t v = rlo + k;
             s_old[k] = old_off[v < n_old ? v : n_old];
         }
-    }
-    __syncthreads();
-    const int64_t p0 = pb + (int64_t)threadIdx.x * 16;
-    if (p0 > pe) return;
-    const int64_t end = p0 + 16 <= pe + 1 ? p0 + 16 : pe + 1;
-    if (staged) {
-        int64_t lo = 0, hi = nrows;  // local row index
-        while (hi - lo > 1) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (s_new[mid] <= p0) lo = mid; else hi = mid;
+        for (int j = threadIdx.x; j < kMergeOut; j += 256) s_row[j] = 0;
+        __syncthreads();
+        // row k starts at local position s_new[k] - pb; of several rows
+        // starting there (empty ones), the last holds the position
+        for (int64_t k = threadIdx.x + 1; k < nrows; k += 256) {
+            const int64_t st = s_new[k] - pb;
+            if (st >= 0 && st < cnt) atomicMax(&s_row[st], (int)k);
         }
-        int64_t v = lo;
-        int32_t row_beg = s_new[v], row_end = s_new[v + 1];
-        for (int64_t p = p0; p < end; ++p) {
-            while (row_end <= p) { ++v; row_beg = row_end; row_end = s_new[v + 1]; }
+        __syncthreads();
+        // prefix maximum over the positions: 16 per thread, then across threads
+        constexpr int kPer = kMergeOut / 256;
+        const int j0 = threadIdx.x * kPer;
+        int mx = 0;
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) mx = max(mx, s_row[j0 + q]);
+        s_max[threadIdx.x] = mx;
+        __syncthreads();
+        for (int d = 1; d < 256; d <<= 1) {
+            const int v = (int)threadIdx.x >= d ? s_max[threadIdx.x - d] : 0;
+            __syncthreads();
+            s_max[threadIdx.x] = max(s_max[threadIdx.x], v);
+            __syncthreads();
+        }
+        int run = threadIdx.x > 0 ? s_max[threadIdx.x - 1] : 0;
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            run = max(run, s_row[j0 + q]);
+            s_row[j0 + q] = run;
+        }
+        __syncthreads();
+        for (int j = threadIdx.x; j < cnt; j += 256) {
+            const int64_t p = pb + j;
+            const int v = s_row[j];
+            const int32_t row_beg = s_new[v];
             const int64_t k = p - row_beg;
             const int64_t ob = s_old[v];
             const int64_t olen = s_old[v + 1] - ob;
             out[p] = k < olen ? old_adj[ob + k] : (int32_t)sorted_src[(row_beg - ob) + (k - olen)];
         }
     } else {
-        int64_t v = row_of_pos(new_off, rlo, rlo + nrows, p0);
-        int32_t row_beg = new_off[v], row_end = new_off[v + 1];
-        for (int64_t p = p0; p < end; ++p) {
-            while (row_end <= p) { ++v; row_beg = row_end; row_end = new_off[v + 1]; }
+        for (int j = threadIdx.x; j < cnt; j += 256) {
+            const int64_t p = pb + j;
+            const int64_t v = row_of_pos(new_off, rlo, rlo + nrows, p);
+            const int32_t row_beg = new_off[v];
             const int64_t k = p - row_beg;
-            const int64_t ob = old_off[v];
-            const int64_t olen = old_off[v + 1] - ob;  // v < n_old for any non-empty row
+            const int64_t ob = old_off[v < n_old ? v : n_old];
+            const int64_t olen = old_off[v + 1 < n_old ? v + 1 : n_old] - ob;
             out[p] = k < olen ? old_adj[ob + k] : (int32_t)sorted_src[(row_beg - ob) + (k - olen)];
         }
     }
