@@ -158,15 +158,31 @@ __device__ int64_t block_excl_scan(int64_t v, int64_t* total) {
     return pre + x - v;
 }
 
+// Bitonic sort of s[0,P) (P a power of two <= kSortMax) by the workgroup:
+// a stage's P/2 compare-exchange pairs are spread over the threads, each
+// loading all of its pairs before comparing.
 __device__ void bitonic_sort(int32_t* s, int P) {
+    constexpr int kPairs = kSortMax / (2 * kBT);
+    const int np = P >> 1;
     for (int k = 2; k <= P; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < P; i += kBT) {
-                int ixj = i ^ j;
-                if (ixj > i) {
-                    int32_t a = s[i], b = s[ixj];
-                    bool up = (i & k) == 0;
-                    if ((a > b) == up) { s[i] = b; s[ixj] = a; }
+            int32_t A[kPairs], B[kPairs];
+#pragma unroll
+            for (int r = 0; r < kPairs; ++r) {
+                const int t = threadIdx.x + r * kBT;
+                if (t < np) {
+                    const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+                    A[r] = s[i];
+                    B[r] = s[i + j];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kPairs; ++r) {
+                const int t = threadIdx.x + r * kBT;
+                if (t < np) {
+                    const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+                    const bool up = (i & k) == 0;
+                    if ((A[r] > B[r]) == up) { s[i] = B[r]; s[i + j] = A[r]; }
                 }
             }
             __syncthreads();
@@ -421,11 +437,21 @@ __global__ __launch_bounds__(kBT, 4) void k_bfs_root(CsrView g, const float* __r
         atomicAdd(bytes_out, (unsigned long long)b);
     }
     __syncthreads();
-    // fixed-point CDF: scale from the largest weight's exponent field
+    // fixed-point CDF: scale from the largest weight's exponent field.  Each
+    // thread's (at most kMaxPer) weights are gathered once, loads issued
+    // together, and kept in registers.
+    constexpr int kMaxPer = kBfsCapMax / kBT;
     const int per = (ns + kBT - 1) / kBT;
     const int s0 = threadIdx.x * per, s1 = min(ns, s0 + per);
+    int32_t xs[kMaxPer];
+    float wv[kMaxPer];
+#pragma unroll
+    for (int e = 0; e < kMaxPer; ++e) xs[e] = s0 + e < s1 ? S[s0 + e] : 0;
+#pragma unroll
+    for (int e = 0; e < kMaxPer; ++e) wv[e] = s0 + e < s1 ? w[xs[e]] : 0.0f;
     uint32_t mb = 0;
-    for (int s = s0; s < s1; ++s) mb = max(mb, f2u(w[S[s]]));
+#pragma unroll
+    for (int e = 0; e < kMaxPer; ++e) mb = max(mb, f2u(wv[e]));
     for (int off = 32; off > 0; off >>= 1) mb = max(mb, (uint32_t)__shfl_down((int)mb, off, 64));
     __syncthreads();
     if ((threadIdx.x & 63) == 0) L.win[threadIdx.x >> 6] = (int32_t)mb;
@@ -434,13 +460,17 @@ __global__ __launch_bounds__(kBT, 4) void k_bfs_root(CsrView g, const float* __r
     for (int k = 0; k < kWaves; ++k) gmb = max(gmb, (uint32_t)L.win[k]);
     const double scale = pow2d(49 - ((int)(gmb >> 23) - 127));
     int64_t acc = 0;
-    for (int s = s0; s < s1; ++s) acc += (int64_t)nb_fixed(w[S[s]], scale);
+#pragma unroll
+    for (int e = 0; e < kMaxPer; ++e) acc += s0 + e < s1 ? (int64_t)nb_fixed(wv[e], scale) : 0;
     int64_t tot;
     int64_t run = block_excl_scan(acc, &tot);
     uint64_t* Q = Q_buf + u * cap;
-    for (int s = s0; s < s1; ++s) {
-        run += (int64_t)nb_fixed(w[S[s]], scale);
-        Q[s] = (uint64_t)run;
+#pragma unroll
+    for (int e = 0; e < kMaxPer; ++e) {
+        if (s0 + e < s1) {
+            run += (int64_t)nb_fixed(wv[e], scale);
+            Q[s0 + e] = (uint64_t)run;
+        }
     }
 }
 
@@ -461,21 +491,70 @@ struct SmallLds {
     int32_t win[kSmallWin];
 };
 
-__device__ void wave_bitonic(int32_t* s, int P) {
-    const int lane = threadIdx.x;
+// Bitonic sort of s[0,P) (P a power of two <= kSmallWin) by one wavefront,
+// in registers: element i lives in register e = i / 64 of lane i % 64 (so
+// the loads and stores are conflict-free and the result lands back in
+// natural order).  Compare distances below 64 are lane exchanges
+// (shuffle-xor), larger ones pairs of registers within a lane; the network
+// is fully unrolled for E = P / 64 registers per lane.
+template <int E>
+__device__ __forceinline__ void wave_sort_reg(int32_t* s) {
+    constexpr int P = 64 * E;
+    const int lane = threadIdx.x & 63;
+    int32_t v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = s[e * 64 + lane];
+#pragma unroll
     for (int k = 2; k <= P; k <<= 1) {
+#pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = lane; i < P; i += 64) {
-                int ixj = i ^ j;
-                if (ixj > i) {
-                    int32_t a = s[i], b = s[ixj];
-                    bool up = (i & k) == 0;
-                    if ((a > b) == up) { s[i] = b; s[ixj] = a; }
+            if (j < 64) {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int i = e * 64 + lane;
+                    const int32_t o = __shfl_xor(v[e], j, 64);
+                    const bool up = (i & k) == 0;
+                    const bool lower = (lane & j) == 0;
+                    v[e] = (lower == up) ? min(v[e], o) : max(v[e], o);
+                }
+            } else {
+                constexpr int dummy = 0;
+                (void)dummy;
+                const int je = j / 64;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    if ((e & je) == 0) {
+                        const int i = e * 64 + lane;
+                        const bool up = (i & k) == 0;
+                        const int32_t x = v[e], y = v[e + je];
+                        v[e] = up ? min(x, y) : max(x, y);
+                        v[e + je] = up ? max(x, y) : min(x, y);
+                    }
                 }
             }
-            __syncthreads();  // one wavefront per workgroup: a wave barrier
         }
     }
+#pragma unroll
+    for (int e = 0; e < E; ++e) s[e * 64 + lane] = v[e];
+}
+
+// s[0,P), P a power of two: padded to 64 if smaller (with INT_MAX), sorted.
+__device__ void wave_bitonic(int32_t* s, int P) {
+    const int lane = threadIdx.x;
+    if (P < 64) {
+        if (lane >= P) s[lane] = INT_MAX;
+        P = 64;
+    }
+    __syncthreads();
+    switch (P) {
+        case 64: wave_sort_reg<1>(s); break;
+        case 128: wave_sort_reg<2>(s); break;
+        case 256: wave_sort_reg<4>(s); break;
+        case 512: wave_sort_reg<8>(s); break;
+        case 1024: wave_sort_reg<16>(s); break;
+        default: wave_sort_reg<32>(s); break;
+    }
+    __syncthreads();
 }
 
 // Ordered in-place compaction of the sorted s[0,n): keeps the first copy of
@@ -503,7 +582,7 @@ __device__ int wave_unique_compact(int32_t* s, int n, int limit, Keep keep) {
     return out < limit ? out : limit;
 }
 
-__global__ __launch_bounds__(64) void k_bfs_small(CsrView g, const float* __restrict__ w,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_bfs_small(CsrView g, const float* __restrict__ w,
                                                   const int32_t* __restrict__ uroot,
                                                   const int64_t* __restrict__ n_unique_ptr, int cap,
                                                   int32_t* __restrict__ S_buf,
@@ -533,11 +612,17 @@ __global__ __launch_bounds__(64) void k_bfs_small(CsrView g, const float* __rest
     if (d1 > 1) wave_bitonic(Ls.f1, P1);
     const int n1 = wave_unique_compact(Ls.f1, d1, kSmallF1, [&](int32_t x) { return x != root; });
     const int n1c = n1 < cap ? n1 : cap;
-    // 2-hop candidate count
+    // 2-hop candidate count: the F1c rows' offsets, all loads of a lane
+    // issued together
+    constexpr int kChunks = kSmallF1 / 64;
     int64_t t = 0;
-    for (int q = lane; q < n1c; q += 64) {
-        const int32_t v = Ls.f1[q];
-        t += (g.fwd_off[v + 1] - g.fwd_off[v]) + (g.bwd_off[v + 1] - g.bwd_off[v]);
+#pragma unroll
+    for (int ch = 0; ch < kChunks; ++ch) {
+        const int q = ch * 64 + lane;
+        if (q < n1c) {
+            const int32_t v = Ls.f1[q];
+            t += (g.fwd_off[v + 1] - g.fwd_off[v]) + (g.bwd_off[v + 1] - g.bwd_off[v]);
+        }
     }
     for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
     const int64_t tot2 = t + n1c;
@@ -548,12 +633,13 @@ __global__ __launch_bounds__(64) void k_bfs_small(CsrView g, const float* __rest
     const bool check_f1 = n1 > n1c;
     const int32_t f1max = n1c > 0 ? Ls.f1[n1c - 1] : -1;
     // gather F1c and the rows of F1c, 64 row pairs at a time: lane q owns
-    // row pair q (start `pos` in win); the wave then fills the chunk's
-    // positions lane-strided, each position finding its row pair by a
-    // binary search over the lanes' starts, so consecutive lanes read
-    // consecutive row entries
+    // row pair q (start `pos` in the chunk); the chunk's positions are filled
+    // lane-strided, kGather at a time with their loads issued together, each
+    // position finding its row pair by a binary search over the lanes'
+    // starts, so consecutive lanes read consecutive row entries
     for (int q = lane; q < n1c; q += 64) Ls.win[q] = Ls.f1[q];
     int run = n1c;
+    constexpr int kGather = 8;
     for (int q0 = 0; q0 < n1c; q0 += 64) {
         const int q = q0 + lane;
         int32_t a = 0, b = 0, c = 0, d = 0;
@@ -570,20 +656,30 @@ __global__ __launch_bounds__(64) void k_bfs_small(CsrView g, const float* __rest
         }
         const int pos = x - len;                     // start within the chunk
         const int total = __shfl(x, 63, 64);
-        for (int p0 = 0; p0 < total; p0 += 64) {
-            const int pp = p0 + lane;
-            // last lane whose start is <= pp (starts ascend across lanes)
-            int r = 0;
+        for (int p0 = 0; p0 < total; p0 += 64 * kGather) {
+            int32_t val[kGather];
 #pragma unroll
-            for (int st = 32; st > 0; st >>= 1) {
-                const int sp = __shfl(pos, r + st, 64);
-                if (sp <= pp) r += st;
+            for (int e = 0; e < kGather; ++e) {
+                const int pp = p0 + e * 64 + lane;
+                // last lane whose start is <= pp (starts ascend across lanes)
+                int r = 0;
+#pragma unroll
+                for (int st = 32; st > 0; st >>= 1) {
+                    const int sp = __shfl(pos, r + st, 64);
+                    if (sp <= pp) r += st;
+                }
+                const int k = pp - __shfl(pos, r, 64);
+                const int32_t ar = __shfl(a, r, 64);
+                const int flr = __shfl(fl, r, 64);
+                const int32_t cr = __shfl(c, r, 64);
+                val[e] = 0;
+                if (pp < total) val[e] = k < flr ? g.fwd_adj[ar + k] : g.bwd_adj[cr + k - flr];
             }
-            const int k = pp - __shfl(pos, r, 64);
-            const int32_t ar = __shfl(a, r, 64);
-            const int flr = __shfl(fl, r, 64);
-            const int32_t cr = __shfl(c, r, 64);
-            if (pp < total) Ls.win[run + pp] = k < flr ? g.fwd_adj[ar + k] : g.bwd_adj[cr + k - flr];
+#pragma unroll
+            for (int e = 0; e < kGather; ++e) {
+                const int pp = p0 + e * 64 + lane;
+                if (pp < total) Ls.win[run + pp] = val[e];
+            }
         }
         run += total;
     }
@@ -600,28 +696,57 @@ __global__ __launch_bounds__(64) void k_bfs_small(CsrView g, const float* __rest
         }
         return true;
     });
+    // S, and its fixed-point CDF from the weights.  Pass 1 writes S and
+    // gathers the weights (kWb per lane per round, loads issued together)
+    // into the LDS slots of their ids; pass 2 reads four consecutive
+    // weights per lane, sums them locally and scans the lane totals.
     int32_t* S = S_buf + u * cap;
     uint64_t* Q = Q_buf + u * cap;
+    constexpr int kWb = 4;
     uint32_t mb = 0;
-    for (int s = lane; s < ns; s += 64) {
-        const int32_t x = Ls.win[s];
-        S[s] = x;
-        mb = max(mb, f2u(w[x]));
+    for (int s0 = 0; s0 < ns; s0 += 64 * kWb) {
+        int32_t xs[kWb];
+        float wb[kWb];
+#pragma unroll
+        for (int e = 0; e < kWb; ++e) {
+            const int s2 = s0 + e * 64 + lane;
+            xs[e] = s2 < ns ? Ls.win[s2] : 0;
+        }
+#pragma unroll
+        for (int e = 0; e < kWb; ++e) wb[e] = s0 + e * 64 + lane < ns ? w[xs[e]] : 0.0f;
+#pragma unroll
+        for (int e = 0; e < kWb; ++e) {
+            const int s2 = s0 + e * 64 + lane;
+            if (s2 < ns) {
+                S[s2] = xs[e];
+                Ls.win[s2] = (int32_t)f2u(wb[e]);
+                mb = max(mb, f2u(wb[e]));
+            }
+        }
     }
     for (int off = 32; off > 0; off >>= 1) mb = max(mb, (uint32_t)__shfl_xor((int)mb, off, 64));
     const double scale = pow2d(49 - ((int)(mb >> 23) - 127));
-    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    (void)below;
+    __syncthreads();
     uint64_t carry = 0;
-    for (int s0 = 0; s0 < ns; s0 += 64) {
-        const int s = s0 + lane;
-        uint64_t v = s < ns ? nb_fixed(w[Ls.win[s]], scale) : 0;
-        uint64_t x = v;
+    for (int s0 = 0; s0 < ns; s0 += 256) {
+        const int b0 = s0 + lane * 4;
+        uint64_t q[4];
+        uint64_t acc = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int s2 = b0 + e;
+            acc += s2 < ns ? nb_fixed(__uint_as_float((uint32_t)Ls.win[s2]), scale) : 0;
+            q[e] = acc;
+        }
+        uint64_t x = acc;
         for (int off = 1; off < 64; off <<= 1) {
             uint64_t y = __shfl_up(x, off, 64);
             if (lane >= off) x += y;
         }
-        if (s < ns) Q[s] = carry + x;
+        const uint64_t ex = carry + x - acc;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if (b0 + e < ns) Q[b0 + e] = ex + q[e];
         carry += __shfl(x, 63, 64);
     }
     if (lane == 0) {
