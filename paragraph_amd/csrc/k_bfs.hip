@@ -814,8 +814,20 @@ __device__ __forceinline__ bool row_has(const int32_t* row, int n, int32_t x) {
     return false;
 }
 
+// Neighbourhood draws and PA refill, kNbG lanes per paper.  After the PA
+// phase the paper's row holds c0 picks (kept in LDS for the duplicate
+// checks).  If S \ row has at most n_nb ids, all of them are taken in S
+// order (lanes take consecutive S entries, a ballot keeps the order);
+// otherwise draw attempts b = 0, 1, ... each pick S[upper_bound(Q, t)], and
+// the lanes evaluate kNbG consecutive attempts at once: attempt b is taken
+// iff its pick is neither in the row nor the pick of an earlier attempt of
+// the same round -- the sequential redraw-on-duplicate rule.  The refill
+// continues the paper's PA attempt sequence (one lane; it runs only when
+// the neighbourhood could not supply n_nb ids).
 // BATCH: the paper's simulation (batch table) supplies n_exist, n_nb, the
 // CDF region and the Philox key; ids in rows and in S are global.
+constexpr int kNbG = 8;
+
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_nb_draw(Batch bt, int64_t n_exist, int64_t n_new, int m,
                                                  int n_nb, int cap, uint64_t seed, uint32_t yi,
@@ -827,11 +839,20 @@ __global__ __launch_bounds__(256) void k_nb_draw(Batch bt, int64_t n_exist, int6
                                                  int32_t* __restrict__ cit, int8_t* __restrict__ kind,
                                                  int32_t* __restrict__ cnt, int32_t* __restrict__ near,
                                                  int32_t* __restrict__ anext) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int G = kNbG;
+    constexpr int kGroups = 256 / G;
+    __shared__ int32_t s_row[kGroups][kMaxCites];
+    __shared__ int32_t s_pick[256];
+    const int grp = threadIdx.x / G;
+    const int gl = threadIdx.x % G;
+    const int lane = threadIdx.x & 63;
+    const int shift = (lane / G) * G;
+    const uint64_t gmask = ((1ull << G) - 1ull) << shift;
+    const uint64_t below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    const int64_t i = (int64_t)blockIdx.x * kGroups + grp;
     if (BATCH) n_new = bt.n_new;
-    if (i >= n_new) return;
     int64_t local = i, base = 0;
-    if (BATCH) {
+    if (BATCH && i < n_new) {
         const BatchSim& sm = bt.sims[batch_sim_of_new(bt, i)];
         n_exist = sm.n_exist;
         n_nb = sm.n_nb;
@@ -839,68 +860,131 @@ __global__ __launch_bounds__(256) void k_nb_draw(Batch bt, int64_t n_exist, int6
         cdf += base;
         seed = sm.seed;
         local = i - sm.new_first;
-        if (n_nb <= 0) return;
     }
-    const int c0 = cnt[i];
-    if (c0 <= 0) return;
+    const int c0 = (i < n_new && n_nb > 0) ? cnt[i] : 0;
+    const bool live = c0 > 0;
     const uint32_t gid = (uint32_t)(n_exist + local);
     int32_t* row = cit + i * m;
     int8_t* kr = kind + i * m;
-    const int64_t u = root_slot[i];
-    const int ns = ns_buf[u];
-    const int32_t* S = S_buf + u * cap;
-    const uint64_t* Q = Q_buf + u * cap;
-    // cited papers inside S (binary search each cited id in S); only needed
-    // when S is small enough that taking all of S \ cited is possible
+    int ns = 0;
+    const int32_t* S = S_buf;
+    const uint64_t* Q = Q_buf;
+    if (live) {
+        const int64_t u = root_slot[i];
+        ns = ns_buf[u];
+        S = S_buf + u * cap;
+        Q = Q_buf + u * cap;
+        for (int j = gl; j < c0; j += G) s_row[grp][j] = row[j];
+    }
+    __syncthreads();
+    // PA picks inside S (only needed when taking all of S \ row is possible)
     int inS = 0;
-    if (ns <= c0 + n_nb) {
-        for (int j = 0; j < c0; ++j) {
-            const int32_t x = row[j];
+    if (live && ns <= c0 + n_nb) {
+        for (int j = gl; j < c0; j += G) {
+            const int32_t x = s_row[grp][j];
             int lo = 0, hi = ns;
             while (lo < hi) { int mid = (lo + hi) >> 1; if (S[mid] < x) lo = mid + 1; else hi = mid; }
             inS += (lo < ns && S[lo] == x) ? 1 : 0;
         }
     }
+    for (int off = 1; off < G; off <<= 1) inS += __shfl_xor(inS, off, 64);
+    const bool take_all = live && ns - inS <= n_nb;
     int c = c0;
-    const int avail = ns - inS;
-    if (avail <= n_nb) {
-        for (int s = 0; s < ns; ++s) {
-            const int32_t x = S[s];
-            if (!row_has(row, c0, x)) { row[c] = x; kr[c] = 1; ++c; }
+    // take all of S \ row, in S order
+    bool act = take_all;
+    int sb = 0;
+    while (__syncthreads_or(act)) {
+        bool keep = false;
+        int32_t x = 0;
+        if (act) {
+            const int sidx = sb + gl;
+            if (sidx < ns) {
+                x = S[sidx];
+                keep = true;
+                for (int j = 0; j < c0; ++j) keep &= s_row[grp][j] != x;
+            }
         }
-    } else {
-        const uint64_t Z = Q[ns - 1];
-        const int b_max = kAttemptsPerCite * n_nb;
-        int got = 0;
-        for (int bt = 0; got < n_nb && bt < b_max; ++bt) {
-            u32x4 r = philox(gid, (uint32_t)bt, yi, TAG_NB, seed);
+        const uint64_t km = __ballot(keep) & gmask;
+        if (keep) {
+            const int pos = c + __popcll(km & below);
+            row[pos] = x;
+            kr[pos] = 1;
+            s_row[grp][pos] = x;
+        }
+        if (act) {
+            c += __popcll(km);
+            sb += G;
+            if (sb >= ns) act = false;
+        }
+    }
+    // rejection draws over the neighbourhood's fixed-point CDF
+    bool need = live && !take_all;
+    const uint64_t Zq = (live && ns > 0) ? Q[ns - 1] : 0;
+    const int b_max = kAttemptsPerCite * n_nb;
+    int got = 0, b0 = 0;
+    while (__syncthreads_or(need)) {
+        const int b = b0 + gl;
+        const bool valid = need && b < b_max;
+        int32_t pick = -1;
+        if (valid) {
+            const u32x4 r = philox(gid, (uint32_t)b, yi, TAG_NB, seed);
             const uint64_t v = ((uint64_t)r.x << 21) ^ (uint64_t)(r.y >> 11);
-            const uint64_t t = (__umul64hi(v, Z) << 11) | ((v * Z) >> 53);
+            const uint64_t t = (__umul64hi(v, Zq) << 11) | ((v * Zq) >> 53);
             int lo = 0, hi = ns;
             while (lo < hi) { int mid = (lo + hi) >> 1; if (Q[mid] > t) hi = mid; else lo = mid + 1; }
             if (lo >= ns) lo = ns - 1;
-            const int32_t x = S[lo];
-            if (!row_has(row, c, x)) { row[c] = x; kr[c] = 1; ++c; ++got; }
+            pick = S[lo];
+        }
+        s_pick[threadIdx.x] = pick;
+        __syncthreads();
+        bool dup = !valid;
+        if (valid) {
+            const int g0 = grp * G;
+            for (int q = 0; q < gl && !dup; ++q) dup = s_pick[g0 + q] == pick;
+            for (int q = 0; q < c && !dup; ++q) dup = s_row[grp][q] == pick;
+        }
+        const uint64_t acc = __ballot(!dup) & gmask;
+        const int pos = got + __popcll(acc & below);
+        const bool take = !dup && pos < n_nb;
+        __syncthreads();
+        if (take) {
+            row[c + pos - got] = pick;
+            kr[c + pos - got] = 1;
+            s_row[grp][c + pos - got] = pick;
+        }
+        if (need) {
+            const int add = min(__popcll(acc), n_nb - got);
+            got += add;
+            c += add;
+            b0 += G;
+            if (got >= n_nb || b0 >= b_max) need = false;
         }
     }
     // refill with PA draws, continuing the attempt sequence
-    const double Zg = cdf[n_exist - 1];
-    const double eps = kNearRel * Zg;
-    const int a_max = kAttemptsPerCite * m;
-    int a = anext[i];
-    int nr = 0;
-    while (c < m && a < a_max) {
-        u32x4 r = philox(gid, (uint32_t)a, yi, TAG_PA, seed);
-        const double t = u53(r.x, r.y) * Zg;
-        const int64_t k = cdf_upper_bound(cdf, n_exist, t);
-        if (cdf_near(cdf, n_exist, k, t, eps)) ++nr;
-        ++a;
-        const int32_t x = (int32_t)(k + base);
-        if (!row_has(row, c, x)) { row[c] = x; kr[c] = 2; ++c; }
+    if (live && gl == 0) {
+        const double Zg = cdf[n_exist - 1];
+        const double eps = kNearRel * Zg;
+        const int a_max = kAttemptsPerCite * m;
+        int a = anext[i];
+        int nr = 0;
+        while (c < m && a < a_max) {
+            u32x4 r = philox(gid, (uint32_t)a, yi, TAG_PA, seed);
+            const double t = u53(r.x, r.y) * Zg;
+            const int64_t k = cdf_upper_bound(cdf, n_exist, t);
+            if (cdf_near(cdf, n_exist, k, t, eps)) ++nr;
+            ++a;
+            const int32_t x = (int32_t)(k + base);
+            if (!row_has(s_row[grp], c, x)) {
+                row[c] = x;
+                kr[c] = 2;
+                s_row[grp][c] = x;
+                ++c;
+            }
+        }
+        cnt[i] = c;
+        near[i] += nr;
+        anext[i] = a;
     }
-    cnt[i] = c;
-    near[i] += nr;
-    anext[i] = a;
 }
 
 }  // namespace
@@ -939,7 +1023,7 @@ void launch_nb_draw(hipStream_t s, int64_t n_exist, int64_t n_new, int m, int n_
                     const int32_t* S_buf, const uint64_t* Q_buf, const int32_t* ns_buf,
                     int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* near, int32_t* anext) {
     if (n_new <= 0 || n_nb <= 0) return;
-    const int64_t nb = (n_new + 255) / 256;
+    const int64_t nb = (n_new + 256 / kNbG - 1) / (256 / kNbG);
     hipLaunchKernelGGL(k_nb_draw<false>, dim3((uint32_t)nb), dim3(256), 0, s, Batch{}, n_exist,
                        n_new, m, n_nb, cap, seed, year_index, cdf, root_slot, S_buf, Q_buf, ns_buf,
                        cit, kind, cnt, near, anext);
@@ -950,7 +1034,7 @@ void launch_nb_draw_batch(hipStream_t s, const Batch& b, int m, int cap, uint32_
                           const uint64_t* Q_buf, const int32_t* ns_buf, int32_t* cit, int8_t* kind,
                           int32_t* cnt, int32_t* near, int32_t* anext) {
     if (b.n_new <= 0) return;
-    const int64_t nb = (b.n_new + 255) / 256;
+    const int64_t nb = (b.n_new + 256 / kNbG - 1) / (256 / kNbG);
     hipLaunchKernelGGL(k_nb_draw<true>, dim3((uint32_t)nb), dim3(256), 0, s, b, (int64_t)0,
                        b.n_new, m, 0, cap, (uint64_t)0, year_index, cdf, root_slot, S_buf, Q_buf,
                        ns_buf, cit, kind, cnt, near, anext);
