@@ -21,3 +21,34 @@ def test_u53_range():
     assert L.or_u53(0xffffffff, 0xffffffff) == 1.0 - 2.0 ** -53
     assert L.or_u53_open0(0, 0) == 2.0 ** -53
     assert L.or_u53_open0(0xffffffff, 0xffffffff) == 1.0
+
+
+def test_matches_rocrand_engine():
+    """Against rocRAND's Philox4x32-10 engine (an independent implementation,
+    compiled for the host by oracle/Makefile) on random counters and keys,
+    including the counter shapes the model uses (paper, attempt, year, tag)."""
+    import os
+    import subprocess
+
+    import numpy as np
+    import pytest
+    exe = os.path.join(os.path.dirname(ob.__file__), "rocrand_philox")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/rocrand_philox not built (needs hipcc)")
+    rng = np.random.default_rng(5)
+    cases = []
+    for _ in range(3000):
+        c = [int(rng.integers(0, 2**32)), int(rng.integers(0, 2**30)),
+             int(rng.integers(0, 2**32)), int(rng.integers(0, 2**32))]
+        k = [int(rng.integers(0, 2**32)), int(rng.integers(0, 2**32))]
+        cases.append((c, k))
+    for paper in (0, 1, 2_000_000, 2_687_836):
+        for attempt in (0, 1, 63, 1279):
+            for yi, tag in ((0, 0), (9, 1), (29, 2), (0xFFFFFFFF, 3)):
+                cases.append(([paper, attempt, yi, tag], [1, 0]))
+    inp = "".join(f"{c[0]} {c[1]} {c[2]} {c[3]} {k[0]} {k[1]}\n" for c, k in cases)
+    out = subprocess.run([exe], input=inp, capture_output=True, text=True, check=True).stdout
+    got = [tuple(int(x) for x in line.split()) for line in out.splitlines()]
+    assert len(got) == len(cases)
+    for (c, k), g in zip(cases, got):
+        assert tuple(ob.philox(c, k)) == g, (c, k)
