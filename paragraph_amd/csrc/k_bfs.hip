@@ -39,7 +39,7 @@ namespace {
 constexpr int kBT = 512;                    // threads per large root
 constexpr int kWaves = kBT / 64;
 constexpr int kWin = 7168;                  // LDS buffer: sort window or bitmap words
-constexpr int kSortMax = 2048;              // totals up to this are sorted, above it swept
+constexpr int kSortMax = 4096;              // totals up to this are sorted, above it swept
 constexpr int kBmBits = kWin * 32;          // ids per bitmap window
 constexpr int kMaxRows = 1 + 2 * kBfsCapMax;
 constexpr int kLongSeg = 48;                // row parts copied by a whole wave
