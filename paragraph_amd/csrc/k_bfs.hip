@@ -266,9 +266,9 @@ __device__ void flush_window(const Query& q, int nw, int32_t* out, int cap) {
 // are swept through the id range in bitmap windows (each entry touched
 // once, ids extracted in order, empty stretches skipped), below the bound
 // derived from long rows.  Rows longer than kLongRow are searched and read
-// by whole wavefronts.  Kept out of line: inlining it into the kernels
-// crashes the ROCm 7.2 optimiser.
-__device__ __attribute__((noinline)) int collect(const Query& q, int32_t* out, int cap) {
+// by whole wavefronts.  It has one call site (two_hop) in one kernel
+// (k_bfs_root): out-of-line calls cost a scratch stack frame per lane.
+__device__ __forceinline__ int collect(const Query& q, int32_t* out, int cap) {
     const int32_t n = (int32_t)q.g.n;
     const int nr = num_rows(q);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -388,23 +388,30 @@ __device__ __attribute__((noinline)) int collect(const Query& q, int32_t* out, i
     return L.nout;
 }
 
-// Builds S for `root` into out[] (global); returns |S|.
-__device__ int two_hop(const CsrView& g, int32_t root, int cap, int32_t* out) {
+// Builds S for `root` into out[] (global); returns |S|.  The two levels
+// share one call site of collect, so it is inlined once.
+__device__ __forceinline__ int two_hop(const CsrView& g, int32_t root, int cap, int32_t* out) {
     Query q{g, root, 1, 0, false, 0};
-    const int n1c = collect(q, L.f1, cap);
-    q.level = 2;
-    q.n1c = n1c;
-    if (n1c == cap) {
-        q.f1max = L.f1[cap - 1];
-        const int32_t* fo = g.fwd_off; const int32_t* bo = g.bwd_off;
-        int32_t mx = -1;
-        if (fo[root + 1] > fo[root]) mx = max(mx, g.fwd_adj[fo[root + 1] - 1]);
-        if (bo[root + 1] > bo[root]) mx = max(mx, g.bwd_adj[bo[root + 1] - 1]);
-        q.check_f1 = mx > q.f1max;
+    int n = 0;
+#pragma nounroll
+    for (int level = 1; level <= 2; ++level) {
+        n = collect(q, level == 1 ? L.f1 : out, cap);
+        if (level == 1) {
+            q.level = 2;
+            q.n1c = n;
+            if (n == cap) {
+                q.f1max = L.f1[cap - 1];
+                const int32_t* fo = g.fwd_off; const int32_t* bo = g.bwd_off;
+                int32_t mx = -1;
+                if (fo[root + 1] > fo[root]) mx = max(mx, g.fwd_adj[fo[root + 1] - 1]);
+                if (bo[root + 1] > bo[root]) mx = max(mx, g.bwd_adj[bo[root + 1] - 1]);
+                q.check_f1 = mx > q.f1max;
+            }
+            if (threadIdx.x == 0) { L.n1c = n; L.t_level1 = wall_clock64(); }
+            __syncthreads();
+        }
     }
-    if (threadIdx.x == 0) { L.n1c = n1c; L.t_level1 = wall_clock64(); }
-    __syncthreads();
-    return collect(q, out, cap);
+    return n;
 }
 
 __device__ __forceinline__ double pow2d(int k) { return u2d((uint64_t)(1023 + k) << 52); }
@@ -417,6 +424,10 @@ __device__ __forceinline__ uint64_t nb_fixed(float w, double scale) {
 
 // k_bfs_root: S and its fixed-point CDF for each distinct root that the
 // wavefront path (k_bfs_small) handed over in big_list.
+// Stand-alone mode (only_roots non-null): block i builds S for
+// only_roots[i] into S_buf + i*cap, its size into only_len[i] and, with
+// ticks, the wall-clock ticks to the end of the 1-hop level, to the end, and
+// the 2-hop candidate entries; no CDF.
 __global__ __launch_bounds__(kBT, 4) void k_bfs_root(CsrView g, const float* __restrict__ w,
                                                   const int32_t* __restrict__ uroot,
                                                   const int32_t* __restrict__ big_list,
@@ -424,12 +435,29 @@ __global__ __launch_bounds__(kBT, 4) void k_bfs_root(CsrView g, const float* __r
                                                   int32_t* __restrict__ S_buf,
                                                   uint64_t* __restrict__ Q_buf,
                                                   int32_t* __restrict__ ns_buf,
-                                                  unsigned long long* __restrict__ bytes_out) {
-    if ((int)blockIdx.x >= *big_count) return;
-    const int64_t u = big_list[blockIdx.x];
-    const int32_t root = uroot[u];
+                                                  unsigned long long* __restrict__ bytes_out,
+                                                  const int32_t* __restrict__ only_roots,
+                                                  int32_t* __restrict__ only_len,
+                                                  int64_t* __restrict__ ticks) {
+    const bool only = only_roots != nullptr;
+    if (!only && (int)blockIdx.x >= *big_count) return;
+    const int64_t u = only ? (int64_t)blockIdx.x : big_list[blockIdx.x];
+    const int32_t root = only ? only_roots[u] : uroot[u];
     int32_t* S = S_buf + u * cap;
-    const int ns = two_hop(g, root, cap, S);
+    const int64_t t0 = wall_clock64();
+    const bool valid = root >= 0 && root < g.n;
+    const int ns = valid ? two_hop(g, root, cap, S) : 0;
+    if (only) {
+        if (threadIdx.x == 0) {
+            only_len[u] = ns;
+            if (ticks) {
+                ticks[3 * u + 0] = valid ? L.t_level1 - t0 : 0;
+                ticks[3 * u + 1] = wall_clock64() - t0;
+                ticks[3 * u + 2] = valid ? L.tot[2] : 0;
+            }
+        }
+        return;
+    }
     if (threadIdx.x == 0) {
         ns_buf[u] = ns;
         const int64_t b = 16 * (1 + (int64_t)L.n1c) + 4 * (L.tot[1] + L.tot[2] - L.n1c) +
@@ -539,7 +567,7 @@ __device__ __forceinline__ void wave_sort_reg(int32_t* s) {
 }
 
 // s[0,P), P a power of two: padded to 64 if smaller (with INT_MAX), sorted.
-__device__ void wave_bitonic(int32_t* s, int P) {
+__device__ __forceinline__ void wave_bitonic(int32_t* s, int P) {
     const int lane = threadIdx.x;
     if (P < 64) {
         if (lane >= P) s[lane] = INT_MAX;
@@ -753,25 +781,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         ns_buf[u] = ns;
         const int64_t by = 16 * (1 + (int64_t)n1c) + 4 * ((int64_t)d1 + tot2 - n1c) + 4 * (int64_t)ns;
         atomicAdd(bytes_out, (unsigned long long)by);
-    }
-}
-
-__global__ __launch_bounds__(kBT, 4) void k_bfs_only(CsrView g, const int32_t* __restrict__ roots,
-                                                  int cap, int32_t* __restrict__ out,
-                                                  int32_t* __restrict__ out_len,
-                                                  int64_t* __restrict__ ticks) {
-    const int64_t i = blockIdx.x;
-    const int32_t root = roots[i];
-    const int64_t t0 = wall_clock64();
-    int ns = 0;
-    if (root >= 0 && root < g.n) ns = two_hop(g, root, cap, out + i * cap);
-    if (threadIdx.x == 0) {
-        out_len[i] = ns;
-        if (ticks) {
-            ticks[3 * i + 0] = L.t_level1 - t0;
-            ticks[3 * i + 1] = wall_clock64() - t0;
-            ticks[3 * i + 2] = L.tot[2];
-        }
     }
 }
 
@@ -1015,7 +1024,8 @@ void launch_bfs_root(hipStream_t s, const CsrView& g, const float* w, const int3
     hipLaunchKernelGGL(k_bfs_small, dim3((uint32_t)max_unique), dim3(64), 0, s, g, w, uroot,
                        n_unique, cap, S_buf, Q_buf, ns_buf, big_list, big_count, bytes_out);
     hipLaunchKernelGGL(k_bfs_root, dim3((uint32_t)max_unique), dim3(kBT), 0, s, g, w, uroot,
-                       big_list, big_count, cap, S_buf, Q_buf, ns_buf, bytes_out);
+                       big_list, big_count, cap, S_buf, Q_buf, ns_buf, bytes_out,
+                       (const int32_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr);
 }
 
 void launch_nb_draw(hipStream_t s, int64_t n_exist, int64_t n_new, int m, int n_nb, int cap,
@@ -1043,8 +1053,10 @@ void launch_nb_draw_batch(hipStream_t s, const Batch& b, int m, int cap, uint32_
 void launch_bfs_only(hipStream_t s, const CsrView& g, int64_t count, const int32_t* roots, int cap,
                      int32_t* out, int32_t* out_len, int64_t* ticks) {
     if (count <= 0) return;
-    hipLaunchKernelGGL(k_bfs_only, dim3((uint32_t)count), dim3(kBT), 0, s, g, roots, cap, out,
-                       out_len, ticks);
+    hipLaunchKernelGGL(k_bfs_root, dim3((uint32_t)count), dim3(kBT), 0, s, g, (const float*)nullptr,
+                       (const int32_t*)nullptr, (const int32_t*)nullptr, (const int32_t*)nullptr,
+                       cap, out, (uint64_t*)nullptr, (int32_t*)nullptr,
+                       (unsigned long long*)nullptr, roots, out_len, ticks);
 }
 
 }  // namespace pg
