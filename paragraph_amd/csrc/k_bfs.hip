@@ -299,18 +299,53 @@ __device__ __forceinline__ int collect(const Query& q, int32_t* out, int cap) {
     __syncthreads();
     if (total == 0) return 0;
     if (total <= kSortMax) {
-        // one sorted window
+        // one sorted window.  Each wave gathers 64 rows at a time: lane k
+        // holds row r0 + k, one atomic reserves the chunk's space, and the
+        // chunk's positions are filled lane-strided, each finding its row by
+        // a binary search over the lanes' starts (consecutive lanes read
+        // consecutive entries; four loads in flight per lane)
         if (threadIdx.x == 0) L.fill = 0;
         __syncthreads();
-        for (int r = wid; r < nr; r += kWaves) {
-            const int32_t* p; int b, e;
-            row_of(q, r, p, b, e);
-            int len = e - b;
-            if (len <= 0) continue;
+        for (int r0 = wid * 64; r0 < nr; r0 += kBT) {
+            const int r = r0 + lane;
+            const int32_t* p = nullptr;
+            int b = 0, e = 0;
+            if (r < nr) row_of(q, r, p, b, e);
+            const int len = e - b;
+            int x = len;
+            for (int off = 1; off < 64; off <<= 1) {
+                const int y = __shfl_up(x, off, 64);
+                if (lane >= off) x += y;
+            }
+            const int pos = x - len;
+            const int chunk = __shfl(x, 63, 64);
+            if (chunk == 0) continue;
             int base = 0;
-            if (lane == 0) base = atomicAdd(&L.fill, len);
+            if (lane == 0) base = atomicAdd(&L.fill, chunk);
             base = __shfl(base, 0, 64);
-            for (int k = lane; k < len; k += 64) L.win[base + k] = p[b + k];
+            const uint64_t pa = (uint64_t)(p + b);
+            constexpr int kG = 4;
+            for (int p0 = 0; p0 < chunk; p0 += 64 * kG) {
+                int32_t val[kG];
+#pragma unroll
+                for (int g2 = 0; g2 < kG; ++g2) {
+                    const int pp = p0 + g2 * 64 + lane;
+                    int rr = 0;
+#pragma unroll
+                    for (int st = 32; st > 0; st >>= 1) {
+                        const int sp = __shfl(pos, rr + st, 64);
+                        if (sp <= pp) rr += st;
+                    }
+                    const int k = pp - __shfl(pos, rr, 64);
+                    const uint64_t src = __shfl(pa, rr, 64);
+                    val[g2] = pp < chunk ? reinterpret_cast<const int32_t*>(src)[k] : 0;
+                }
+#pragma unroll
+                for (int g2 = 0; g2 < kG; ++g2) {
+                    const int pp = p0 + g2 * 64 + lane;
+                    if (pp < chunk) L.win[base + pp] = val[g2];
+                }
+            }
         }
         __syncthreads();
         flush_window(q, (int)total, out, cap);
@@ -331,17 +366,43 @@ __device__ __forceinline__ int collect(const Query& q, int32_t* out, int cap) {
         for (int k = threadIdx.x; k < kWin; k += kBT) bm[k] = 0u;
         __syncthreads();
         int32_t nxt = INT_MAX;  // smallest unconsumed id at or above hi
-        for (int r = threadIdx.x; r < nr; r += kBT) {
-            const int32_t* p; int b, e;
-            row_of(q, r, p, b, e);
-            if (long_ok && e - b > kLongRow) continue;
-            int c = L.cursor[r];
-            for (; c < e; ++c) {
-                const int32_t x = p[c];
-                if (x >= hi) { nxt = min(nxt, x); break; }
-                atomicOr(&bm[(x - lo) >> 5], 1u << ((x - lo) & 31));
+        // a thread's rows kRB at a time: their offsets, then their next
+        // entries, are loaded together before any row is walked
+        constexpr int kRB = 2;
+        for (int r0 = threadIdx.x; r0 < nr; r0 += kBT * kRB) {
+            const int32_t* pp[kRB];
+            int cc[kRB], ee[kRB];
+            bool mine[kRB];
+            int32_t x0[kRB];
+#pragma unroll
+            for (int k = 0; k < kRB; ++k) {
+                const int r = r0 + k * kBT;
+                pp[k] = nullptr;
+                cc[k] = ee[k] = 0;
+                mine[k] = false;
+                if (r < nr) {
+                    int b;
+                    row_of(q, r, pp[k], b, ee[k]);
+                    mine[k] = !(long_ok && ee[k] - b > kLongRow);
+                    cc[k] = L.cursor[r];
+                }
             }
-            L.cursor[r] = c;
+#pragma unroll
+            for (int k = 0; k < kRB; ++k)
+                x0[k] = (mine[k] && cc[k] < ee[k]) ? pp[k][cc[k]] : INT_MAX;
+#pragma unroll
+            for (int k = 0; k < kRB; ++k) {
+                if (!mine[k]) continue;
+                int c = cc[k];
+                int32_t x = x0[k];
+                while (c < ee[k]) {
+                    if (x >= hi) { nxt = min(nxt, x); break; }
+                    atomicOr(&bm[(x - lo) >> 5], 1u << ((x - lo) & 31));
+                    ++c;
+                    if (c < ee[k]) x = pp[k][c];
+                }
+                L.cursor[r0 + k * kBT] = c;
+            }
         }
         for (int s = wid; s < nlong; s += kWaves) {
             const int r = L.lrow[s];
