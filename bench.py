@@ -358,6 +358,8 @@ def main():
     ap.add_argument("--ens-only", action="store_true",
                     help="only the ensemble part (its JSON object on stdout)")
     ap.add_argument("--ens-sims", type=int, default=256)
+    ap.add_argument("--bfs-mode", type=int, default=0,
+                    help="2-hop workgroup path: 0 staged where it fits, 1 sweep only")
     ap.add_argument("--ens-reps", type=int, default=3)
     ap.add_argument("--lik-only", action="store_true",
                     help="only the likelihood part (its JSON object on stdout)")
@@ -383,6 +385,7 @@ def main():
     eprint(f"seed graph: {n0} papers, {g.src.size} edges ({time.perf_counter() - t0:.1f} s)")
     e_cap = g.src.size + sum(sched) * m + 16
     ctx = pg.Context(n_final, e_cap, max(sched), max_cites=m, bfs_cap=cap, seed=seed)
+    ctx.set_tuning(bfs_mode=args.bfs_mode)
     ctx.load(g)
     ctx.snapshot_save()
     G0 = None

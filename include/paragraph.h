@@ -335,8 +335,13 @@ typedef struct {
     int32_t pa_lanes;          /* PA attempts evaluated together per paper:
                                   8, 16, 32, 64 (0 = by citations per paper) */
     int32_t lik_lanes;         /* likelihood slot pass: lanes per citing paper
-                                  1, 2, 4, 8 (0 = by citations per paper) */
-    int32_t reserved;
+                                  1, 2, 4, 8 with 8 * lanes >= citations per
+                                  paper (0 = the fewest such) */
+    int32_t bfs_mode;          /* workgroup 2-hop path for candidate lists
+                                  above 4096 entries: 0 = staged in id
+                                  buckets up to 65536 entries, swept above;
+                                  1 = always swept; 2 = staged whenever the
+                                  list fits a staging slot */
 } pg_tuning;
 
 int pg_ensemble_create(const pg_ensemble_config* cfg, const pg_sim_params* sims,
@@ -360,6 +365,10 @@ int pg_ensemble_download(pg_ensemble* ens, int32_t sim, int32_t* fwd_off, int32_
                          int32_t* bwd_off, int32_t* bwd_adj, int32_t* in_deg,
                          int32_t* birth);
 int pg_ensemble_set_tuning(pg_ensemble* ens, const pg_tuning* t);
+/* The same launch shapes for a plain context (pg_year_pa, pg_bfs,
+ * pg_pa_loglik[_grad]).  No reference counterpart: the reference fixes its
+ * launch shapes at compile time (src/kernel.cu:1600-1640). */
+int pg_set_tuning(pg_ctx* ctx, const pg_tuning* t);
 
 /* Ceil growth rule of the reference (src/abm.cu:233-240,
  * src/kernel.cu:1690): ceil(n * rate) in double precision. */

@@ -89,7 +89,7 @@ class EnsembleStats(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("size", ctypes.c_uint32), ("pa_lanes", ctypes.c_int32),
-                ("lik_lanes", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("lik_lanes", ctypes.c_int32), ("bfs_mode", ctypes.c_int32)]
 
 
 KERNEL_GROUPS = ["weights_cdf", "pa_draw", "bfs", "nb_draw", "er", "csr_append", "lik",
@@ -165,6 +165,7 @@ def lib() -> ctypes.CDLL:
             "pg_ensemble_download": (ctypes.c_int, [vp, ctypes.c_int32, i32p, i32p, i32p, i32p,
                                                     i32p, i32p]),
             "pg_ensemble_set_tuning": (ctypes.c_int, [vp, P(Tuning)]),
+            "pg_set_tuning": (ctypes.c_int, [vp, P(Tuning)]),
             "pg_pa_loglik": (ctypes.c_int, [vp, P(LikParams), P(LikOut)]),
             "pg_pa_loglik_grad": (ctypes.c_int, [vp, P(LikParams), P(LikOut)]),
         }
@@ -436,6 +437,13 @@ class Context:
         """log L, dL/dalpha, dL/dtau (host=True) and dL/dphi into grad_dev (FP64)."""
         return self._lik("pg_pa_loglik_grad", alpha, tau, logfit_dev, grad_dev=grad_dev, **kw)
 
+    def set_tuning(self, pa_lanes: int = 0, lik_lanes: int = 0, bfs_mode: int = 0):
+        """Launch shapes (speed only): PA lanes per paper, likelihood lanes
+        per citing paper, 2-hop workgroup path mode (0 staged up to 65536
+        candidate entries, 1 always swept, 2 staged whenever it fits)."""
+        t = Tuning(ctypes.sizeof(Tuning), pa_lanes, lik_lanes, bfs_mode)
+        self._check(lib().pg_set_tuning(self._h, ctypes.byref(t)), "pg_set_tuning")
+
     def snapshot_save(self):
         self._check(lib().pg_snapshot_save(self._h), "pg_snapshot_save")
 
@@ -517,8 +525,9 @@ class Ensemble:
     def restore(self):
         self._check(lib().pg_ensemble_restore(self._h), "pg_ensemble_restore")
 
-    def set_tuning(self, pa_lanes: int = 0, lik_lanes: int = 0):
-        t = Tuning(ctypes.sizeof(Tuning), pa_lanes, lik_lanes, 0)
+    def set_tuning(self, pa_lanes: int = 0, lik_lanes: int = 0, bfs_mode: int = 0):
+        """Launch shapes (speed only): PA lanes per paper, 2-hop path mode."""
+        t = Tuning(ctypes.sizeof(Tuning), pa_lanes, lik_lanes, bfs_mode)
         self._check(lib().pg_ensemble_set_tuning(self._h, ctypes.byref(t)),
                     "pg_ensemble_set_tuning")
 

@@ -42,26 +42,27 @@ constexpr int kWin = 7168;                  // LDS buffer: sort window or bitmap
 constexpr int kSortMax = 4096;              // totals up to this are sorted, above it swept
 constexpr int kBmBits = kWin * 32;          // ids per bitmap window
 constexpr int kMaxRows = 1 + 2 * kBfsCapMax;
-constexpr int kLongSeg = 48;                // row parts copied by a whole wave
-constexpr int kMaxSegs = kSortMax / kLongSeg + 1;
 constexpr int kLongRow = 192;               // rows longer than this use wave search
 constexpr int kMaxLong = 256;
+constexpr int kMaxBuckets = 2048;           // staged path: id buckets (in cursor[])
+constexpr int kMaxBucketShift = 17;         // a bucket must fit one bitmap window
+constexpr int kPiece = 256;                 // staged path: long-row part per wave step
 
 struct BfsLds {
     int32_t f1[kBfsCapMax];       // capped 1-hop list (ascending)
     int32_t win[kWin];            // sort window, or bitmap of one id window
-    int32_t cursor[kMaxRows];     // per-row position
-    int32_t seg_row[kMaxSegs];
-    int32_t seg_beg[kMaxSegs];
-    int32_t seg_len[kMaxSegs];
+    int32_t cursor[kMaxRows];     // per-row position, or (staged path) bucket ends
     int32_t lrow[kMaxLong];       // rows searched/copied by whole wavefronts
+    int32_t lpre[kMaxLong + 1];   // staged path: long rows' first piece; bits 16-17 the row kind
+    int32_t lbeg[kMaxLong];       // staged path: long rows' first entry
     int64_t red[kWaves];
     int64_t tot[3];               // row entries per level
     int32_t nlong;
-    int32_t nseg;
+    int32_t slot;                 // staging slot held by this workgroup, or -1
     int32_t fill;
     int32_t nout;
     int32_t n1c;
+    int64_t t_start;              // wall clock at the workgroup's start
     int64_t t_level1;             // wall clock after the 1-hop level
 };
 
@@ -69,7 +70,15 @@ struct BfsLds {
 // compiler keeps LDS (ds_*) accesses.
 __shared__ BfsLds L;
 
-__device__ __forceinline__ int lower_bound_i32(const int32_t* p, int b, int e, int32_t x) {
+// Global-memory views: graph rows and outputs are read and written through
+// these, so the compiler emits global (not flat) accesses.  A flat access
+// also counts against the LDS wait counter, and every LDS operation or
+// shuffle that follows it waits for the memory access to complete.
+typedef __attribute__((address_space(1))) const int32_t g_cint;
+typedef __attribute__((address_space(1))) int32_t g_int;
+#define GROW(p) ((g_cint*)(p))
+
+__device__ __forceinline__ int lower_bound_i32(g_cint* p, int b, int e, int32_t x) {
     while (b < e) {
         int mid = (b + e) >> 1;
         if (p[mid] < x) b = mid + 1; else e = mid;
@@ -78,7 +87,7 @@ __device__ __forceinline__ int lower_bound_i32(const int32_t* p, int b, int e, i
 }
 
 // First index in [b,e) with p[idx] >= x, searching forward from b.
-__device__ __forceinline__ int gallop_lower_bound(const int32_t* p, int b, int e, int32_t x) {
+__device__ __forceinline__ int gallop_lower_bound(g_cint* p, int b, int e, int32_t x) {
     if (b >= e || p[b] >= x) return b;
     int step = 1, lo = b;  // p[lo] < x
     while (true) {
@@ -95,7 +104,7 @@ __device__ __forceinline__ int gallop_lower_bound(const int32_t* p, int b, int e
 // lane and keeps the sub-range holding the answer (a 100k-entry row takes
 // three rounds of loads instead of 17 dependent ones).  All lanes call it
 // with the same arguments and get the same result.
-__device__ int wave_lower_bound(const int32_t* p, int b, int e, int32_t x) {
+__device__ __forceinline__ int wave_lower_bound(g_cint* p, int b, int e, int32_t x) {
     // (the count of samples below x is summed with shuffles: a ballot here
     // trips a register-class error in the ROCm 7.2 backend)
     const int lane = threadIdx.x & 63;
@@ -114,7 +123,7 @@ __device__ int wave_lower_bound(const int32_t* p, int b, int e, int32_t x) {
     return b + k;
 }
 
-__device__ __forceinline__ bool row_contains(const int32_t* p, int b, int e, int32_t x) {
+__device__ __forceinline__ bool row_contains(g_cint* p, int b, int e, int32_t x) {
     int k = lower_bound_i32(p, b, e, x);
     return k < e && p[k] == x;
 }
@@ -158,6 +167,69 @@ __device__ int64_t block_excl_scan(int64_t v, int64_t* total) {
     return pre + x - v;
 }
 
+// The same for one int32 per thread.
+__device__ int block_excl_scan32(int v, int* total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    __syncthreads();
+    if (lane == 63) L.red[wid] = x;
+    __syncthreads();
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k) {
+        const int t = (int)L.red[k];
+        if (k < wid) pre += t;
+        tot += t;
+    }
+    *total = tot;
+    return pre + x - v;
+}
+
+// block_excl_scan32 without the barrier that protects L.red from the
+// previous scan's readers: for callers with a barrier since that scan.
+__device__ __forceinline__ int block_excl_scan32_nb(int v, int* total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) L.red[wid] = x;
+    __syncthreads();
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k) {
+        const int t = (int)L.red[k];
+        if (k < wid) pre += t;
+        tot += t;
+    }
+    *total = tot;
+    return pre + x - v;
+}
+
+// Wavefront-cooperative LDS counter add: every lane with key >= 0 adds 1 to
+// ctr[key]; a run of consecutive lanes with one key issues one atomic for
+// the run.  All 64 lanes call it.  With RET, each such lane gets the
+// counter's old value plus its rank in the run (the slot it reserved).
+template <bool RET>
+__device__ __forceinline__ int run_add(int32_t* ctr, int key) {
+    const int lane = threadIdx.x & 63;
+    const int up = __shfl_up(key, 1, 64);
+    const uint64_t bnd = __ballot(lane == 0 || up != key);  // run starts
+    const uint64_t upto = bnd & ((2ull << lane) - 1);
+    const uint64_t above = bnd & ~((2ull << lane) - 1);
+    const int start = 63 - __builtin_clzll(upto);
+    const int end = above ? __builtin_ctzll(above) : 64;
+    int base = 0;
+    if (key >= 0 && start == lane) base = atomicAdd(&ctr[key], end - lane);
+    if (!RET) return 0;
+    return __shfl(base, start, 64) + (lane - start);
+}
+
 // Bitonic sort of s[0,P) (P a power of two <= kSortMax) by the workgroup:
 // a stage's P/2 compare-exchange pairs are spread over the threads, each
 // loading all of its pairs before comparing.
@@ -197,39 +269,55 @@ struct Query {
     int n1c;            // |F1c| (level 2)
     bool check_f1;      // level 2: F1 was capped, so filter F1 \ F1c
     int32_t f1max;
+    BfsStage st;
+    g_int* S;           // level-2 output (level 1 writes L.f1)
+    bool out_lds;       // output into L.f1 (level 1; staged level 2, copied to S)
 };
 
 __device__ __forceinline__ int num_rows(const Query& q) { return q.level == 1 ? 2 : 1 + 2 * q.n1c; }
 
-__device__ __forceinline__ void row_of(const Query& q, int r, const int32_t*& p, int& b, int& e) {
+// Level 2's row 0 is the F1c list in LDS; every other row is a CSR row.
+__device__ __forceinline__ bool is_f1_row(const Query& q, int r) { return q.level == 2 && r == 0; }
+
+// CSR row r (not the F1c row): entries p[b,e) of the forward or backward
+// adjacency.
+__device__ __forceinline__ void graph_row(const Query& q, int r, g_cint*& p, int& b, int& e) {
     int32_t v;
     int d;
     if (q.level == 1) {
         v = q.root; d = r;
-    } else if (r == 0) {
-        p = L.f1; b = 0; e = q.n1c; return;
     } else {
         v = L.f1[(r - 1) >> 1]; d = (r - 1) & 1;
     }
-    const int32_t* off = d ? q.g.bwd_off : q.g.fwd_off;
-    p = d ? q.g.bwd_adj : q.g.fwd_adj;
+    g_cint* off = GROW(d ? q.g.bwd_off : q.g.fwd_off);
+    p = GROW(d ? q.g.bwd_adj : q.g.fwd_adj);
     b = off[v];
     e = off[v + 1];
+}
+
+// Row r's bounds (the F1c row: [0, n1c) of L.f1).
+__device__ __forceinline__ void row_bounds(const Query& q, int r, g_cint*& p, int& b, int& e) {
+    if (is_f1_row(q, r)) { p = nullptr; b = 0; e = q.n1c; return; }
+    graph_row(q, r, p, b, e);
+}
+
+__device__ __forceinline__ void put_out(const Query& q, int pos, int32_t x) {
+    if (q.out_lds) L.f1[pos] = x; else q.S[pos] = x;
 }
 
 __device__ __forceinline__ bool keep_id(const Query& q, int32_t x) {
     if (x == q.root) return false;
     if (q.level == 2 && q.check_f1 && x > q.f1max) {
-        const int32_t* fo = q.g.fwd_off; const int32_t* bo = q.g.bwd_off;
-        if (row_contains(q.g.fwd_adj, fo[q.root], fo[q.root + 1], x)) return false;
-        if (row_contains(q.g.bwd_adj, bo[q.root], bo[q.root + 1], x)) return false;
+        g_cint* fo = GROW(q.g.fwd_off); g_cint* bo = GROW(q.g.bwd_off);
+        if (row_contains(GROW(q.g.fwd_adj), fo[q.root], fo[q.root + 1], x)) return false;
+        if (row_contains(GROW(q.g.bwd_adj), bo[q.root], bo[q.root + 1], x)) return false;
     }
     return true;
 }
 
-// Sort the window, drop duplicates and filtered ids, append to out[] (up to
-// cap).  All threads call it.
-__device__ void flush_window(const Query& q, int nw, int32_t* out, int cap) {
+// Sort the window, drop duplicates and filtered ids, append to the output
+// (up to cap).  All threads call it.
+__device__ void flush_window(const Query& q, int nw, int cap) {
     int P = 1;
     while (P < nw) P <<= 1;
     for (int k = nw + threadIdx.x; k < P; k += kBT) L.win[k] = INT_MAX;
@@ -248,7 +336,7 @@ __device__ void flush_window(const Query& q, int nw, int32_t* out, int cap) {
     for (int k = s0; k < s1; ++k) {
         int32_t x = L.win[k];
         if ((k == 0 || L.win[k - 1] != x) && keep_id(q, x)) {
-            if (pos < cap) out[pos] = x;
+            if (pos < cap) put_out(q, (int)pos, x);
             ++pos;
         }
     }
@@ -260,27 +348,225 @@ __device__ void flush_window(const Query& q, int nw, int32_t* out, int cap) {
     __syncthreads();
 }
 
-// Collects the `cap` smallest distinct kept ids of the level's rows into
-// out[] (ascending); returns the count.  All threads call it.  Up to
-// kSortMax row entries are gathered and bitonic-sorted in LDS; larger lists
-// are swept through the id range in bitmap windows (each entry touched
-// once, ids extracted in order, empty stretches skipped), below the bound
-// derived from long rows.  Rows longer than kLongRow are searched and read
-// by whole wavefronts.  It has one call site (two_hop) in one kernel
-// (k_bfs_root): out-of-line calls cost a scratch stack frame per lane.
-__device__ __forceinline__ int collect(const Query& q, int32_t* out, int cap) {
+// Appends the set bits of bm[0,nw) (ids lo + bit index) in ascending order
+// to the output after the L.nout already there, up to cap; updates L.nout.
+// Each thread takes a contiguous run of words.  All threads call it.
+// Each thread holds kXW consecutive words in registers (four 16-byte LDS
+// loads), so kWin / kXW threads cover a full window.
+constexpr int kXW = 16;
+static_assert(kWin % kXW == 0 && kWin / kXW <= kBT, "bitmap window per thread");
+__device__ __forceinline__ void extract_bits(const Query& q, const uint32_t* bm, int nw, int32_t lo,
+                                             int cap) {
+    const int w0 = threadIdx.x * kXW;
+    uint32_t wv[kXW];
+    if (w0 < nw) {
+#pragma unroll
+        for (int k = 0; k < kXW / 4; ++k) {
+            const uint4 v = reinterpret_cast<const uint4*>(bm + w0)[k];
+            wv[4 * k] = v.x; wv[4 * k + 1] = v.y; wv[4 * k + 2] = v.z; wv[4 * k + 3] = v.w;
+        }
+#pragma unroll
+        for (int k = 0; k < kXW; ++k)
+            if (w0 + k >= nw) wv[k] = 0u;  // past the window: stale words
+    } else {
+#pragma unroll
+        for (int k = 0; k < kXW; ++k) wv[k] = 0u;
+    }
+    int cntw = 0;
+#pragma unroll
+    for (int k = 0; k < kXW; ++k) cntw += __popc(wv[k]);
+    int tot;
+    int pos = block_excl_scan32(cntw, &tot) + L.nout;
+    if (cntw > 0 && pos < cap) {
+#pragma unroll
+        for (int k = 0; k < kXW; ++k) {
+            uint32_t bits = wv[k];
+            while (bits && pos < cap) {
+                const int bit = __builtin_ctz(bits);
+                bits &= bits - 1;
+                put_out(q, pos++, lo + ((w0 + k) << 5) + bit);
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int no = L.nout + tot;
+        L.nout = no < cap ? no : cap;
+    }
+    __syncthreads();
+}
+
+// Visits entries of CSR rows 64 rows per wavefront: lane k holds row r0+k
+// (its first entry and length, lengths cut by the caller), the chunk's
+// positions are spread lane-strided, each lane finding its row by a binary
+// search over the lanes' starts, so consecutive lanes read consecutive
+// entries with kG loads in flight per lane.  visit(x, pos, ok) gets the entry
+// and its position in the chunk, and is called by all lanes together with
+// ok = whether the lane holds an entry; returns the chunk's total.
+template <typename Visit>
+__device__ __forceinline__ int chunk_rows(g_cint* pbase, int b, int len, Visit visit) {
+    const int lane = threadIdx.x & 63;
+    int x = len;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    const int pos = x - len;
+    const int chunk = __shfl(x, 63, 64);
+    const uint64_t pa = (uint64_t)(pbase + b);
+    constexpr int kG = 8;
+    for (int p0 = 0; p0 < chunk; p0 += 64 * kG) {
+        int32_t val[kG];
+#pragma unroll
+        for (int g2 = 0; g2 < kG; ++g2) {
+            val[g2] = 0;
+            if (p0 + g2 * 64 >= chunk) continue;  // uniform: past the chunk
+            const int pp = p0 + g2 * 64 + lane;
+            int rr = 0;
+#pragma unroll
+            for (int st = 32; st > 0; st >>= 1) {
+                const int sp = __shfl(pos, rr + st, 64);
+                if (sp <= pp) rr += st;
+            }
+            const int k = pp - __shfl(pos, rr, 64);
+            const uint64_t src = __shfl(pa, rr, 64);
+            if (pp < chunk) val[g2] = reinterpret_cast<g_cint*>(src)[k];
+        }
+#pragma unroll
+        for (int g2 = 0; g2 < kG; ++g2) {
+            if (p0 + g2 * 64 >= chunk) continue;
+            const int pp = p0 + g2 * 64 + lane;
+            visit(val[g2], pp, pp < chunk);
+        }
+    }
+    return chunk;
+}
+
+// Staged windows: as extract_bits, with the count kept by the caller (nout,
+// the same in every thread; returns the new count) and the words read
+// zeroed for the next window.  One barrier: the caller has one between the
+// window's bitmap writes and this call, which also separates it from the
+// previous call's reads of L.red.
+__device__ __forceinline__ int extract_bits_staged(const Query& q, uint32_t* bm, int nw, int32_t lo,
+                                                   int cap, int nout) {
+    const int w0 = threadIdx.x * kXW;
+    uint32_t wv[kXW];
+    if (w0 < nw) {
+#pragma unroll
+        for (int k = 0; k < kXW / 4; ++k) {
+            const uint4 v = reinterpret_cast<const uint4*>(bm + w0)[k];
+            wv[4 * k] = v.x; wv[4 * k + 1] = v.y; wv[4 * k + 2] = v.z; wv[4 * k + 3] = v.w;
+        }
+#pragma unroll
+        for (int k = 0; k < kXW / 4; ++k) reinterpret_cast<uint4*>(bm + w0)[k] = make_uint4(0, 0, 0, 0);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kXW; ++k) wv[k] = 0u;
+    }
+    int cntw = 0;
+#pragma unroll
+    for (int k = 0; k < kXW; ++k) cntw += __popc(wv[k]);
+    int tot;
+    int pos = block_excl_scan32_nb(cntw, &tot) + nout;
+    if (cntw > 0 && pos < cap) {
+#pragma unroll
+        for (int k = 0; k < kXW; ++k) {
+            uint32_t bits = wv[k];
+            while (bits && pos < cap) {
+                const int bit = __builtin_ctz(bits);
+                bits &= bits - 1;
+                put_out(q, pos++, lo + ((w0 + k) << 5) + bit);
+            }
+        }
+    }
+    const int no = nout + tot;
+    return no < cap ? no : cap;
+}
+
+// Staged path: visits every entry of the level's rows, each row cut to its
+// first cap+1 entries (a longer row's entry [cap] already bounds the
+// answer): the F1c list from LDS, CSR rows up to kLongRow entries through
+// chunk_rows, longer ones (L.lrow, pieces in L.lpre/L.lbeg) kPiece entries
+// per wavefront step.  All threads call it; visit(x, ok) is called by whole
+// wavefronts, ok telling the lanes that hold an entry.
+template <typename Visit>
+__device__ __forceinline__ void stage_rows(const Query& q, int nr, int cap, bool long_ok,
+                                           int nlong, int npieces, Visit visit) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (q.level == 2)
+        for (int k0 = 0; k0 < q.n1c; k0 += kBT) {
+            const int k = k0 + (int)threadIdx.x;
+            visit(k < q.n1c ? L.f1[k] : 0, k < q.n1c);
+        }
+    for (int r0 = wid * 64; r0 < nr; r0 += kBT) {
+        const int r = r0 + lane;
+        g_cint* p = GROW(q.g.fwd_adj);
+        int b = 0, e = 0;
+        if (r < nr && !is_f1_row(q, r)) graph_row(q, r, p, b, e);
+        int len = min(e - b, cap + 1);
+        if (long_ok && e - b > kLongRow) len = 0;
+        chunk_rows(p, b, len, [&](int32_t x, int, bool ok) { visit(x, ok); });
+    }
+    // kPP pieces per wavefront step: kPP * kPiece / 64 loads in flight per lane
+    constexpr int kPP = 4;
+    constexpr int kPL = kPiece / 64;
+    for (int pc0 = wid; pc0 < npieces; pc0 += kPP * kWaves) {
+        int32_t val[kPP][kPL];
+        int lenr[kPP], off[kPP];
+#pragma unroll
+        for (int j = 0; j < kPP; ++j) {
+            const int pc = pc0 + j * kWaves;
+            lenr[j] = 0;
+            off[j] = 0;
+            if (pc < npieces) {
+                int a = 0, z = nlong;  // long row with first piece <= pc < next row's
+                while (z - a > 1) {
+                    const int mid = (a + z) >> 1;
+                    if ((L.lpre[mid] & 0xffff) <= pc) a = mid; else z = mid;
+                }
+                const int pj = L.lpre[a];
+                g_cint* p = GROW(((pj >> 16) & 1) ? q.g.bwd_adj : q.g.fwd_adj) + L.lbeg[a];
+                lenr[j] = pj >> 18;
+                off[j] = (pc - (pj & 0xffff)) * kPiece;
+#pragma unroll
+                for (int g2 = 0; g2 < kPL; ++g2) {
+                    const int k = off[j] + g2 * 64 + lane;
+                    val[j][g2] = k < lenr[j] ? p[k] : 0;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kPP; ++j)
+#pragma unroll
+            for (int g2 = 0; g2 < kPL; ++g2)
+                visit(val[j][g2], off[j] + g2 * 64 + lane < lenr[j]);
+    }
+}
+
+// Collects the `cap` smallest distinct kept ids of the level's rows into the
+// output (level 1: L.f1, level 2: q.S; ascending); returns the count.  All
+// threads call it.  Up to kSortMax row entries are gathered and
+// bitonic-sorted in LDS.  Larger lists below the answer bound (from rows
+// longer than cap) are staged in a pool slot bucketed by id when they fit,
+// else swept through the id range in bitmap windows; either way ids come out
+// of LDS bitmaps in ascending order.  It has one call site (two_hop) in one
+// kernel (k_bfs_root): out-of-line calls cost a scratch stack frame per lane.
+__device__ __forceinline__ int collect(Query& q, int cap) {
     const int32_t n = (int32_t)q.g.n;
     const int nr = num_rows(q);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (threadIdx.x == 0) L.nlong = 0;
     __syncthreads();
+    // row entries (bits 26 and up) and entries cut at cap+1 per row (bits
+    // 0-25: at most kMaxRows * (kBfsCapMax + 1) < 2^26)
     int64_t mine = 0;
     int32_t tb = INT_MAX;
     for (int r = threadIdx.x; r < nr; r += kBT) {
-        const int32_t* p; int b, e;
-        row_of(q, r, p, b, e);
+        g_cint* p; int b, e;
+        row_bounds(q, r, p, b, e);
         L.cursor[r] = b;
-        mine += e - b;
+        mine += ((int64_t)(e - b) << 26) + min(e - b, cap + 1);
+        if (is_f1_row(q, r)) continue;  // at most cap entries, in LDS
         // a row with more than cap entries holds at least cap kept ids (only
         // the root can be dropped from it) among its first cap+1, so no id
         // above its entry [cap] can be in the answer
@@ -290,75 +576,176 @@ __device__ __forceinline__ int collect(const Query& q, int32_t* out, int cap) {
             if (s < kMaxLong) L.lrow[s] = r;
         }
     }
-    const int64_t total = block_sum(mine);
+    const int64_t packed = block_sum(mine);
+    const int64_t total = packed >> 26;
+    const int64_t total_cut = packed & ((int64_t(1) << 26) - 1);
     tb = block_min(tb);
-    if (threadIdx.x == 0) {
-        L.nout = 0; L.tot[q.level] = total;
-        if (L.nlong > kMaxLong) L.nlong = -1;  // list overflow: every row per thread
-    }
-    __syncthreads();
-    if (total == 0) return 0;
-    if (total <= kSortMax) {
-        // one sorted window.  Each wave gathers 64 rows at a time: lane k
-        // holds row r0 + k, one atomic reserves the chunk's space, and the
-        // chunk's positions are filled lane-strided, each finding its row by
-        // a binary search over the lanes' starts (consecutive lanes read
-        // consecutive entries; four loads in flight per lane)
-        if (threadIdx.x == 0) L.fill = 0;
-        __syncthreads();
-        for (int r0 = wid * 64; r0 < nr; r0 += kBT) {
-            const int r = r0 + lane;
-            const int32_t* p = nullptr;
-            int b = 0, e = 0;
-            if (r < nr) row_of(q, r, p, b, e);
-            const int len = e - b;
-            int x = len;
-            for (int off = 1; off < 64; off <<= 1) {
-                const int y = __shfl_up(x, off, 64);
-                if (lane >= off) x += y;
-            }
-            const int pos = x - len;
-            const int chunk = __shfl(x, 63, 64);
-            if (chunk == 0) continue;
-            int base = 0;
-            if (lane == 0) base = atomicAdd(&L.fill, chunk);
-            base = __shfl(base, 0, 64);
-            const uint64_t pa = (uint64_t)(p + b);
-            constexpr int kG = 4;
-            for (int p0 = 0; p0 < chunk; p0 += 64 * kG) {
-                int32_t val[kG];
-#pragma unroll
-                for (int g2 = 0; g2 < kG; ++g2) {
-                    const int pp = p0 + g2 * 64 + lane;
-                    int rr = 0;
-#pragma unroll
-                    for (int st = 32; st > 0; st >>= 1) {
-                        const int sp = __shfl(pos, rr + st, 64);
-                        if (sp <= pp) rr += st;
-                    }
-                    const int k = pp - __shfl(pos, rr, 64);
-                    const uint64_t src = __shfl(pa, rr, 64);
-                    val[g2] = pp < chunk ? reinterpret_cast<const int32_t*>(src)[k] : 0;
-                }
-#pragma unroll
-                for (int g2 = 0; g2 < kG; ++g2) {
-                    const int pp = p0 + g2 * 64 + lane;
-                    if (pp < chunk) L.win[base + pp] = val[g2];
-                }
-            }
-        }
-        __syncthreads();
-        flush_window(q, (int)total, out, cap);
-        return L.nout;
-    }
-    const bool long_ok = L.nlong >= 0;
-    const int nlong = long_ok ? L.nlong : 0;
     // with a capped 1-hop list, F1c alone holds cap kept ids, all <= f1max;
     // below the bound the F1 \ F1c filter never applies (those ids exceed
     // f1max), so only the root is cleared from the bitmap
     if (q.level == 2 && q.n1c == cap && q.n1c > 0) tb = min(tb, q.f1max);
     const int64_t span = (tb == INT_MAX) ? (int64_t)n : (int64_t)tb + 1;
+    int sh = 0;
+    while (((span - 1) >> sh) >= kMaxBuckets) ++sh;
+    const bool want_stage = total > kSortMax && total <= q.st.max_total &&
+                            total_cut <= q.st.cap_entries && sh <= kMaxBucketShift &&
+                            q.st.n_slots > 0;
+    if (threadIdx.x == 0) {
+        L.nout = 0; L.tot[q.level] = total;
+        if (L.nlong > kMaxLong) L.nlong = -1;  // list overflow: every row per thread
+        if (want_stage && L.slot < 0) {
+            // first free slot from a block-dependent start; one pass at most
+            for (int t = 0; t < q.st.n_slots; ++t) {
+                const int k = (int)((blockIdx.x + (uint32_t)t) % (uint32_t)q.st.n_slots);
+                if (atomicCAS(&q.st.busy[k], 0, 1) == 0) { L.slot = k; break; }
+            }
+            __threadfence();
+        }
+    }
+    __syncthreads();
+    if (total == 0) return 0;
+    if (total <= kSortMax) {
+        // one sorted window: the F1c list first, then the CSR rows, a
+        // wavefront's chunk placed by one LDS atomic
+        const int f1n = q.level == 2 ? q.n1c : 0;
+        for (int k = threadIdx.x; k < f1n; k += kBT) L.win[k] = L.f1[k];
+        if (threadIdx.x == 0) L.fill = f1n;
+        __syncthreads();
+        for (int r0 = wid * 64; r0 < nr; r0 += kBT) {
+            const int r = r0 + lane;
+            g_cint* p = GROW(q.g.fwd_adj);
+            int b = 0, e = 0;
+            if (r < nr && !is_f1_row(q, r)) graph_row(q, r, p, b, e);
+            int len = e - b;
+            int x = len;
+            for (int off = 1; off < 64; off <<= 1) x += __shfl_xor(x, off, 64);
+            if (x == 0) continue;
+            int base = 0;
+            if (lane == 0) base = atomicAdd(&L.fill, x);
+            base = __shfl(base, 0, 64);
+            chunk_rows(p, b, len, [&](int32_t v, int pp, bool ok) {
+                if (ok) L.win[base + pp] = v;
+            });
+        }
+        __syncthreads();
+        flush_window(q, (int)total, cap);
+        return L.nout;
+    }
+    const bool long_ok = L.nlong >= 0;
+    const int nlong = long_ok ? L.nlong : 0;
     uint32_t* bm = reinterpret_cast<uint32_t*>(L.win);
+    if (want_stage && L.slot >= 0) {
+        // staged: the entries below the bound are counted per id bucket
+        // (width 2^sh), then written bucket by bucket into the slot, and
+        // the windows are runs of whole buckets, each reading its own
+        // entries only
+        g_int* buf = (g_int*)(q.st.buf + (int64_t)L.slot * q.st.cap_entries);
+        const int nbk = (int)((span - 1) >> sh) + 1;
+        int32_t* E = L.cursor;  // bucket counts, then starts, then ends
+        int npieces;
+        {
+            // (an opaque copy of the thread index: left visible, the LDS
+            // addresses below are hoisted out of the level loop and spilled)
+            int tid = (int)threadIdx.x;
+            asm volatile("" : "+v"(tid));
+            int cnt = 0, pk = 0, bb = 0, ln = 0;
+            if (tid < nlong) {
+                const int r = L.lrow[tid];
+                g_cint* p; int b, e;
+                graph_row(q, r, p, b, e);
+                ln = min(e - b, cap + 1);
+                cnt = (ln + kPiece - 1) / kPiece;
+                pk = q.level == 1 ? r : (r - 1) & 1;
+                bb = b;
+            }
+            int64_t tp;
+            const int pre = (int)block_excl_scan(cnt, &tp);
+            if (tid < nlong) {
+                L.lpre[tid] = pre | (pk << 16) | (ln << 18);
+                L.lbeg[tid] = bb;
+            }
+            npieces = (int)tp;
+        }
+        for (int k = threadIdx.x; k < nbk; k += kBT) E[k] = 0;
+        for (int k = threadIdx.x; k < kWin; k += kBT) bm[k] = 0u;  // each window re-zeroes what it used
+        __syncthreads();
+        stage_rows(q, nr, cap, long_ok, nlong, npieces, [&](int32_t x, bool ok) {
+            run_add<false>(E, ok && x < span ? x >> sh : -1);
+        });
+        __syncthreads();
+        {
+            constexpr int kBPT = kMaxBuckets / kBT;
+            int tid = (int)threadIdx.x;  // opaque, as above
+            asm volatile("" : "+v"(tid));
+            const int b0 = tid * kBPT;
+            const int b1 = min(nbk, b0 + kBPT);
+            int sum = 0;
+            for (int k = b0; k < b1; ++k) sum += E[k];
+            int64_t tt;
+            int run = (int)block_excl_scan(sum, &tt);
+            for (int k = b0; k < b1; ++k) {
+                const int c = E[k];
+                E[k] = run;
+                run += c;
+            }
+        }
+        __syncthreads();
+        stage_rows(q, nr, cap, long_ok, nlong, npieces, [&](int32_t x, bool ok) {
+            const bool in = ok && x < span;
+            const int at = run_add<true>(E, in ? x >> sh : -1);
+            if (in) buf[at] = x;
+        });
+        __syncthreads();
+        // every entry, the F1c list's too, is in the slot now: level 2's
+        // output is built in L.f1 and copied out once (a barrier after
+        // global stores waits for them to complete)
+        const bool copy_out = !q.out_lds;
+        q.out_lds = true;
+        // E[b] is now the end of bucket b.  Smallest bucket >= b0 holding an
+        // entry (nbk if none):
+        auto next_nonempty = [&](int b0) {
+            const int st = b0 > 0 ? E[b0 - 1] : 0;
+            int a = b0, z = nbk;
+            while (a < z) {
+                const int mid = (a + z) >> 1;
+                if (E[mid] > st) z = mid; else a = mid + 1;
+            }
+            return a;
+        };
+        const int bpw = kBmBits >> sh;
+        int cur = next_nonempty(0);
+        int no = 0;
+        while (cur < nbk) {
+            const int wend = min(cur + bpw, nbk);
+            const int e0 = cur > 0 ? E[cur - 1] : 0, e1 = E[wend - 1];
+            const int32_t lo = cur << sh;
+            const int nw = (int)((((int64_t)(wend - cur) << sh) + 31) >> 5);
+            constexpr int kU = 4;  // loads in flight per thread
+            for (int k0 = e0 + threadIdx.x; k0 < e1; k0 += kU * kBT) {
+                int32_t xv[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) xv[u] = k0 + u * kBT < e1 ? buf[k0 + u * kBT] : q.root;
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int32_t x = xv[u];
+                    if (x != q.root) atomicOr(&bm[(x - lo) >> 5], 1u << ((x - lo) & 31));
+                }
+            }
+            __syncthreads();
+            no = extract_bits_staged(q, bm, nw, lo, cap, no);
+            if (no >= cap) break;
+            cur = next_nonempty(wend);
+        }
+        __syncthreads();  // the output in L.f1 is complete
+        if (copy_out) {
+            int tid = (int)threadIdx.x;  // opaque, as above
+            asm volatile("" : "+v"(tid));
+            for (int k = tid; k < no; k += kBT) q.S[k] = L.f1[k];
+            __syncthreads();
+            q.out_lds = false;
+        }
+        return no;
+    }
     int64_t lo64 = 0;
     while (lo64 < span) {
         const int32_t lo = (int32_t)lo64;
@@ -366,23 +753,39 @@ __device__ __forceinline__ int collect(const Query& q, int32_t* out, int cap) {
         for (int k = threadIdx.x; k < kWin; k += kBT) bm[k] = 0u;
         __syncthreads();
         int32_t nxt = INT_MAX;  // smallest unconsumed id at or above hi
+        if (q.level == 2) {
+            // the F1c list's part of the window, found by a search in LDS
+            const int c = L.cursor[0];
+            int a = c, z = q.n1c;
+            while (a < z) {
+                const int mid = (a + z) >> 1;
+                if (L.f1[mid] < hi) a = mid + 1; else z = mid;
+            }
+            for (int k = c + threadIdx.x; k < a; k += kBT) {
+                const int32_t x = L.f1[k];
+                atomicOr(&bm[(x - lo) >> 5], 1u << ((x - lo) & 31));
+            }
+            if (a < q.n1c) nxt = L.f1[a];
+            __syncthreads();  // every thread has read cursor[0]
+            if (threadIdx.x == 0) L.cursor[0] = a;
+        }
         // a thread's rows kRB at a time: their offsets, then their next
         // entries, are loaded together before any row is walked
         constexpr int kRB = 2;
         for (int r0 = threadIdx.x; r0 < nr; r0 += kBT * kRB) {
-            const int32_t* pp[kRB];
+            g_cint* pp[kRB];
             int cc[kRB], ee[kRB];
             bool mine[kRB];
             int32_t x0[kRB];
 #pragma unroll
             for (int k = 0; k < kRB; ++k) {
                 const int r = r0 + k * kBT;
-                pp[k] = nullptr;
+                pp[k] = GROW(q.g.fwd_adj);
                 cc[k] = ee[k] = 0;
                 mine[k] = false;
-                if (r < nr) {
+                if (r < nr && !is_f1_row(q, r)) {
                     int b;
-                    row_of(q, r, pp[k], b, ee[k]);
+                    graph_row(q, r, pp[k], b, ee[k]);
                     mine[k] = !(long_ok && ee[k] - b > kLongRow);
                     cc[k] = L.cursor[r];
                 }
@@ -406,8 +809,8 @@ __device__ __forceinline__ int collect(const Query& q, int32_t* out, int cap) {
         }
         for (int s = wid; s < nlong; s += kWaves) {
             const int r = L.lrow[s];
-            const int32_t* p; int b, e;
-            row_of(q, r, p, b, e);
+            g_cint* p; int b, e;
+            graph_row(q, r, p, b, e);
             const int c = L.cursor[r];
             const int z = wave_lower_bound(p, c, e, hi);
             for (int k = c + lane; k < z; k += 64) {
@@ -423,26 +826,7 @@ __device__ __forceinline__ int collect(const Query& q, int32_t* out, int cap) {
         if (threadIdx.x == 0 && q.root >= lo && q.root < hi)
             bm[(q.root - lo) >> 5] &= ~(1u << ((q.root - lo) & 31));
         __syncthreads();
-        constexpr int kWordsPer = kWin / kBT;
-        const int w0 = threadIdx.x * kWordsPer;
-        int cntw = 0;
-        for (int k = 0; k < kWordsPer; ++k) cntw += __popc(bm[w0 + k]);
-        int64_t tot;
-        int64_t pos = block_excl_scan(cntw, &tot) + L.nout;
-        for (int k = 0; k < kWordsPer && pos < cap; ++k) {
-            uint32_t bits = bm[w0 + k];
-            while (bits && pos < cap) {
-                const int bit = __builtin_ctz(bits);
-                bits &= bits - 1;
-                out[pos++] = lo + ((w0 + k) << 5) + bit;
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const int64_t no = L.nout + tot;
-            L.nout = (int32_t)(no < cap ? no : cap);
-        }
-        __syncthreads();
+        extract_bits(q, bm, kWin, lo, cap);
         if (L.nout >= cap || nxt == INT_MAX) break;
         lo64 = nxt;  // skip the empty stretch
     }
@@ -451,21 +835,23 @@ __device__ __forceinline__ int collect(const Query& q, int32_t* out, int cap) {
 
 // Builds S for `root` into out[] (global); returns |S|.  The two levels
 // share one call site of collect, so it is inlined once.
-__device__ __forceinline__ int two_hop(const CsrView& g, int32_t root, int cap, int32_t* out) {
-    Query q{g, root, 1, 0, false, 0};
+__device__ __forceinline__ int two_hop(const CsrView& g, int32_t root, int cap, int32_t* out,
+                                       const BfsStage& st) {
+    Query q{g, root, 1, 0, false, 0, st, (g_int*)out, true};
     int n = 0;
 #pragma nounroll
     for (int level = 1; level <= 2; ++level) {
-        n = collect(q, level == 1 ? L.f1 : out, cap);
+        n = collect(q, cap);
         if (level == 1) {
             q.level = 2;
+            q.out_lds = false;
             q.n1c = n;
             if (n == cap) {
                 q.f1max = L.f1[cap - 1];
-                const int32_t* fo = g.fwd_off; const int32_t* bo = g.bwd_off;
+                g_cint* fo = GROW(g.fwd_off); g_cint* bo = GROW(g.bwd_off);
                 int32_t mx = -1;
-                if (fo[root + 1] > fo[root]) mx = max(mx, g.fwd_adj[fo[root + 1] - 1]);
-                if (bo[root + 1] > bo[root]) mx = max(mx, g.bwd_adj[bo[root + 1] - 1]);
+                if (fo[root + 1] > fo[root]) mx = max(mx, GROW(g.fwd_adj)[fo[root + 1] - 1]);
+                if (bo[root + 1] > bo[root]) mx = max(mx, GROW(g.bwd_adj)[bo[root + 1] - 1]);
                 q.check_f1 = mx > q.f1max;
             }
             if (threadIdx.x == 0) { L.n1c = n; L.t_level1 = wall_clock64(); }
@@ -499,21 +885,29 @@ __global__ __launch_bounds__(kBT, 4) void k_bfs_root(CsrView g, const float* __r
                                                   unsigned long long* __restrict__ bytes_out,
                                                   const int32_t* __restrict__ only_roots,
                                                   int32_t* __restrict__ only_len,
-                                                  int64_t* __restrict__ ticks) {
+                                                  int64_t* __restrict__ ticks, BfsStage stage) {
     const bool only = only_roots != nullptr;
     if (!only && (int)blockIdx.x >= *big_count) return;
     const int64_t u = only ? (int64_t)blockIdx.x : big_list[blockIdx.x];
     const int32_t root = only ? only_roots[u] : uroot[u];
     int32_t* S = S_buf + u * cap;
-    const int64_t t0 = wall_clock64();
     const bool valid = root >= 0 && root < g.n;
-    const int ns = valid ? two_hop(g, root, cap, S) : 0;
+    if (threadIdx.x == 0) {
+        L.slot = -1;
+        L.t_start = wall_clock64();  // kept in LDS, not in a register across two_hop
+    }
+    const int ns = valid ? two_hop(g, root, cap, S, stage) : 0;
+    if (threadIdx.x == 0 && L.slot >= 0) {
+        // every access to the slot ended before two_hop's last barrier
+        __threadfence();
+        atomicExch(&stage.busy[L.slot], 0);
+    }
     if (only) {
         if (threadIdx.x == 0) {
             only_len[u] = ns;
             if (ticks) {
-                ticks[3 * u + 0] = valid ? L.t_level1 - t0 : 0;
-                ticks[3 * u + 1] = wall_clock64() - t0;
+                ticks[3 * u + 0] = valid ? L.t_level1 - L.t_start : 0;
+                ticks[3 * u + 1] = wall_clock64() - L.t_start;
                 ticks[3 * u + 2] = valid ? L.tot[2] : 0;
             }
         }
@@ -780,8 +1174,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const int ns = wave_unique_compact(Ls.win, (int)tot2, cap, [&](int32_t x) {
         if (x == root) return false;
         if (check_f1 && x > f1max) {
-            if (row_contains(g.fwd_adj, fb, fe, x)) return false;
-            if (row_contains(g.bwd_adj, bb, be, x)) return false;
+            if (row_contains(GROW(g.fwd_adj), fb, fe, x)) return false;
+            if (row_contains(GROW(g.bwd_adj), bb, be, x)) return false;
         }
         return true;
     });
@@ -1080,13 +1474,13 @@ void launch_root_dedup(hipStream_t s, int64_t n_exist, int64_t n_new, int m, con
 void launch_bfs_root(hipStream_t s, const CsrView& g, const float* w, const int32_t* uroot,
                      const int64_t* n_unique, int64_t max_unique, int cap, int32_t* S_buf,
                      uint64_t* Q_buf, int32_t* ns_buf, int32_t* big_list, int32_t* big_count,
-                     unsigned long long* bytes_out) {
+                     unsigned long long* bytes_out, const BfsStage& stage) {
     if (max_unique <= 0) return;
     hipLaunchKernelGGL(k_bfs_small, dim3((uint32_t)max_unique), dim3(64), 0, s, g, w, uroot,
                        n_unique, cap, S_buf, Q_buf, ns_buf, big_list, big_count, bytes_out);
     hipLaunchKernelGGL(k_bfs_root, dim3((uint32_t)max_unique), dim3(kBT), 0, s, g, w, uroot,
                        big_list, big_count, cap, S_buf, Q_buf, ns_buf, bytes_out,
-                       (const int32_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr);
+                       (const int32_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr, stage);
 }
 
 void launch_nb_draw(hipStream_t s, int64_t n_exist, int64_t n_new, int m, int n_nb, int cap,
@@ -1111,13 +1505,27 @@ void launch_nb_draw_batch(hipStream_t s, const Batch& b, int m, int cap, uint32_
                        ns_buf, cit, kind, cnt, near, anext);
 }
 
+BfsStage bfs_stage_view(const BfsStage& pool, int mode) {
+    BfsStage v = pool;
+    v.max_total = mode == 1 ? 0 : mode == 2 ? INT64_MAX : kBfsStageDefaultMax;
+    return v;
+}
+
+int bfs_stage_slots(int device) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bfs_root, kBT, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return 0;
+    return 2 * std::max(per_cu, 1) * std::max(cus, 1);
+}
+
 void launch_bfs_only(hipStream_t s, const CsrView& g, int64_t count, const int32_t* roots, int cap,
-                     int32_t* out, int32_t* out_len, int64_t* ticks) {
+                     int32_t* out, int32_t* out_len, int64_t* ticks, const BfsStage& stage) {
     if (count <= 0) return;
     hipLaunchKernelGGL(k_bfs_root, dim3((uint32_t)count), dim3(kBT), 0, s, g, (const float*)nullptr,
                        (const int32_t*)nullptr, (const int32_t*)nullptr, (const int32_t*)nullptr,
                        cap, out, (uint64_t*)nullptr, (int32_t*)nullptr,
-                       (unsigned long long*)nullptr, roots, out_len, ticks);
+                       (unsigned long long*)nullptr, roots, out_len, ticks, stage);
 }
 
 }  // namespace pg

@@ -570,10 +570,15 @@ void launch_cite(hipStream_t st, int G, int K, int64_t nblk, const LikSeq& q, co
 
 }  // namespace
 
-LikCiteShape lik_cite_shape(int m) {
+bool lik_lanes_ok(int m, int lanes) {
+    return (lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8) && lanes * 8 >= m;
+}
+
+LikCiteShape lik_cite_shape(int m, int lanes) {
     LikCiteShape sh;
     sh.G = 1;
     while (sh.G * 8 < m) sh.G <<= 1;                  // m <= 64: G <= 8
+    if (lanes > 0 && lik_lanes_ok(m, lanes)) sh.G = lanes;
     sh.K = (m + sh.G - 1) / sh.G;
     if (sh.K < 4) sh.K = 4;
     return sh;

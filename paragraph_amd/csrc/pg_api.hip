@@ -40,7 +40,7 @@ void release(pg_ctx* c) {
                     c->s_birth, c->s_in_deg, c->s_fwd_off, c->s_fwd_adj, c->s_bwd_off,
                     c->s_bwd_adj, c->S_buf, c->Q_buf, c->ns_buf, c->rk[0], c->rk[1],
                     c->rk[2], c->rk[3], c->rk[4], c->rk[5], c->r_flags, c->r_excl,
-                    c->root_slot, c->uroot, c->big_list};
+                    c->root_slot, c->uroot, c->big_list, c->stage.buf, c->stage.busy};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int k = 0; k < PG_K_COUNT; ++k)
@@ -92,8 +92,27 @@ CsrView view(const pg_ctx* c) {
 
 }  // namespace
 
+int ensure_bfs_stage(pg_ctx* ctx) {
+    if (ctx->stage.buf) return PG_OK;
+    const int slots = bfs_stage_slots(ctx->cfg.device);
+    if (slots <= 0) return PG_OK;  // no pool: every root takes the unstaged sweep
+    BfsStage st;
+    hipError_t r = dalloc(&st.buf, (int64_t)slots * kBfsStageEntries);
+    if (r == hipSuccess) r = dalloc(&st.busy, slots);
+    if (r == hipSuccess) r = hipMemset(st.busy, 0, (size_t)slots * sizeof(int32_t));
+    if (r != hipSuccess) {
+        (void)hipFree(st.buf); (void)hipFree(st.busy);
+        return fail(ctx, PG_ENOMEM, "2-hop staging pool: device allocation failed");
+    }
+    st.n_slots = slots;
+    st.cap_entries = kBfsStageEntries;
+    ctx->stage = st;
+    return PG_OK;
+}
+
 int ensure_nb_buffers(pg_ctx* ctx) {
     if (ctx->S_buf) return PG_OK;
+    if (int rc = ensure_bfs_stage(ctx)) return rc;
     const int64_t D = std::max<int64_t>(ctx->cfg.max_new_per_year, 1);
     const int64_t cap = ctx->cfg.bfs_cap;
     hipError_t r = dalloc(&ctx->S_buf, D * cap);
@@ -330,7 +349,7 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
             pend(PG_K_WEIGHTS_CDF);
             pbeg(PG_K_PA_DRAW);
             launch_pa_draw(s, N, ctx->cdf, D, m, m - n_nb, seed, yi, ctx->cit, ctx->kind, ctx->cnt,
-                           ctx->near, ctx->anext);
+                           ctx->near, ctx->anext, ctx->tune_pa_lanes);
             pend(PG_K_PA_DRAW);
             if (n_nb > 0) {
                 {
@@ -346,7 +365,8 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
                 pbeg(PG_K_BFS);
                 launch_bfs_root(s, view(ctx), ctx->w, ctx->uroot, ctx->r_excl + D, D,
                                 ctx->cfg.bfs_cap, ctx->S_buf, ctx->Q_buf, ctx->ns_buf,
-                                ctx->big_list, big_count, sums + 2);
+                                ctx->big_list, big_count, sums + 2,
+                                bfs_stage_view(ctx->stage, ctx->tune_bfs_mode));
                 pend(PG_K_BFS);
                 pbeg2(PG_K_NB_DRAW);
                 launch_nb_draw(s, N, D, m, n_nb, ctx->cfg.bfs_cap, seed, yi, ctx->cdf,
@@ -588,6 +608,25 @@ int pg_year_weights(pg_ctx* ctx, float* w, double* cdf) {
     return PG_OK;
 }
 
+int pg_set_tuning(pg_ctx* ctx, const pg_tuning* t) {
+    if (!ctx) return PG_EINVAL;
+    if (!t || t->size != sizeof(pg_tuning))
+        return fail(ctx, PG_EINVAL, "pg_set_tuning: null or struct size mismatch");
+    if (t->pa_lanes != 0 && t->pa_lanes != 8 && t->pa_lanes != 16 && t->pa_lanes != 32 &&
+        t->pa_lanes != 64)
+        return fail(ctx, PG_EINVAL, "pg_set_tuning: pa_lanes must be 0, 8, 16, 32 or 64");
+    if (t->lik_lanes != 0 && t->lik_lanes != 1 && t->lik_lanes != 2 && t->lik_lanes != 4 &&
+        t->lik_lanes != 8)
+        return fail(ctx, PG_EINVAL, "pg_set_tuning: lik_lanes must be 0, 1, 2, 4 or 8");
+    if (t->bfs_mode < 0 || t->bfs_mode > 2)
+        return fail(ctx, PG_EINVAL, "pg_set_tuning: bfs_mode must be 0, 1 or 2");
+    if (t->lik_lanes != ctx->tune_lik_lanes) ctx->seq.tables_dirty = true;
+    ctx->tune_pa_lanes = t->pa_lanes;
+    ctx->tune_lik_lanes = t->lik_lanes;
+    ctx->tune_bfs_mode = t->bfs_mode;
+    return PG_OK;
+}
+
 int pg_bfs_timed(pg_ctx* ctx, int64_t count, const int32_t* roots, int32_t* out, int32_t* out_len,
                  int64_t* ticks) {
     if (!ctx) return PG_EINVAL;
@@ -598,6 +637,7 @@ int pg_bfs_timed(pg_ctx* ctx, int64_t count, const int32_t* roots, int32_t* out,
     for (int64_t i = 0; i < count; ++i)
         if (roots[i] < 0 || roots[i] >= ctx->n) return fail(ctx, PG_EINVAL, "pg_bfs: root out of range");
     PG_HIP(hipSetDevice(ctx->cfg.device));
+    if (int rcs = ensure_bfs_stage(ctx)) return rcs;
     hipStream_t s = ctx->stream;
     const int cap = ctx->cfg.bfs_cap;
     int32_t *d_roots = nullptr, *d_out = nullptr, *d_len = nullptr;
@@ -612,7 +652,8 @@ int pg_bfs_timed(pg_ctx* ctx, int64_t count, const int32_t* roots, int32_t* out,
     }
     rc = hipMemcpyAsync(d_roots, roots, count * 4, hipMemcpyHostToDevice, s);
     if (rc == hipSuccess) {
-        launch_bfs_only(s, view(ctx), count, d_roots, cap, d_out, d_len, d_ticks);
+        launch_bfs_only(s, view(ctx), count, d_roots, cap, d_out, d_len, d_ticks,
+                        bfs_stage_view(ctx->stage, ctx->tune_bfs_mode));
         rc = hipGetLastError();
     }
     if (rc == hipSuccess) rc = hipMemcpyAsync(out, d_out, count * cap * 4, hipMemcpyDeviceToHost, s);

@@ -126,6 +126,11 @@ struct pg_ctx {
     int32_t* root_slot = nullptr;
     int32_t* uroot = nullptr;
     int32_t* big_list = nullptr;
+    pg::BfsStage stage;           // staging pool of the workgroup 2-hop path
+    // launch shapes (pg_set_tuning): 0 = default
+    int tune_pa_lanes = 0;
+    int tune_lik_lanes = 0;
+    int tune_bfs_mode = 0;
     // citation-sequence store for the likelihood; recording appends each PA year
     pg_seq_store seq;
     bool seq_recording = false;
@@ -154,6 +159,8 @@ inline hipError_t dalloc(T** p, int64_t count) {
 
 // Neighbourhood buffers sized by cfg.max_new_per_year and cfg.bfs_cap.
 int ensure_nb_buffers(pg_ctx* ctx);
+// Staging pool of the workgroup 2-hop path (allocated once per context).
+int ensure_bfs_stage(pg_ctx* ctx);
 
 // Sequence store (pg_lik.hip): frees its device memory; appends the PA year
 // that pg_year_step is about to run (in-degree snapshot) and has just drawn.

@@ -108,7 +108,8 @@ void enqueue_year(pg_ensemble* ens, int y, hipStream_t s) {
         launch_fill_i32(s, big_count, 0, 1);
         const CsrView g{c->fwd_off, c->fwd_adj, c->bwd_off[c->cur], c->bwd_adj[c->cur], TOT};
         launch_bfs_root(s, g, c->w, c->uroot, c->r_excl + Dn, Dn, c->cfg.bfs_cap, c->S_buf,
-                        c->Q_buf, c->ns_buf, c->big_list, big_count, cnt + 2);
+                        c->Q_buf, c->ns_buf, c->big_list, big_count, cnt + 2,
+                        bfs_stage_view(c->stage, c->tune_bfs_mode));
         launch_nb_draw_batch(s, b, m, c->cfg.bfs_cap, (uint32_t)y, c->cdf, c->root_slot,
                              c->S_buf, c->Q_buf, c->ns_buf, c->cit, c->kind, c->cnt, c->near,
                              c->anext);
@@ -450,8 +451,11 @@ int pg_ensemble_set_tuning(pg_ensemble* ens, const pg_tuning* t) {
     if (t->pa_lanes != 0 && t->pa_lanes != 8 && t->pa_lanes != 16 && t->pa_lanes != 32 &&
         t->pa_lanes != 64)
         return efail(ens, PG_EINVAL, "pg_ensemble_set_tuning: pa_lanes must be 0, 8, 16, 32 or 64");
-    if (t->pa_lanes != ens->pa_lanes) drop_graph(ens);
+    if (t->bfs_mode < 0 || t->bfs_mode > 2)
+        return efail(ens, PG_EINVAL, "pg_ensemble_set_tuning: bfs_mode must be 0, 1 or 2");
+    if (t->pa_lanes != ens->pa_lanes || t->bfs_mode != ens->ctx->tune_bfs_mode) drop_graph(ens);
     ens->pa_lanes = t->pa_lanes;
+    ens->ctx->tune_bfs_mode = t->bfs_mode;
     return PG_OK;
 }
 

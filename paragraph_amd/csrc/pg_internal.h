@@ -111,6 +111,30 @@ struct CsrView {
     int64_t n;
 };
 
+// Staging pool of the workgroup 2-hop path: n_slots slots of cap_entries
+// int32 ids in buf; busy[k] is 0 while slot k is free (zeroed once at
+// allocation; each workgroup that takes a slot clears its flag when done).
+// A workgroup holds at most one slot, so n_slots above the number of
+// resident workgroups always leaves one free; a workgroup that finds none,
+// or whose candidate list does not fit, takes the unstaged sweep.
+struct BfsStage {
+    int32_t* buf = nullptr;
+    int32_t* busy = nullptr;
+    int32_t n_slots = 0;
+    int32_t cap_entries = 0;
+    int64_t max_total = 0;   // larger candidate lists (row entries) are swept
+};
+// Slots for one device: twice the resident workgroups of the path.
+int bfs_stage_slots(int device);
+constexpr int32_t kBfsStageEntries = 1 << 17;
+// Default bound on the candidate entries a root stages: above it the sweep
+// was faster on the 2M-paper citation graphs (few long hub rows, where the
+// sweep reads only the rows' heads).
+constexpr int64_t kBfsStageDefaultMax = 1 << 16;
+// The pool view that a launch gets for tuning mode m (0 default bound,
+// 1 never staged, 2 staged whenever the list fits a slot).
+BfsStage bfs_stage_view(const BfsStage& pool, int mode);
+
 // PA phases 2-3.  Roots (first PA pick of each paper) are sorted and
 // deduplicated: root_slot[i] = distinct-root index of paper i (-1 if none),
 // uroot[u] = root id, n_unique = excl[n_new].
@@ -126,7 +150,7 @@ void launch_root_dedup(hipStream_t s, int64_t n_exist, int64_t n_new, int m, con
 void launch_bfs_root(hipStream_t s, const CsrView& g, const float* w, const int32_t* uroot,
                      const int64_t* n_unique, int64_t max_unique, int cap, int32_t* S_buf,
                      uint64_t* Q_buf, int32_t* ns_buf, int32_t* big_list, int32_t* big_count,
-                     unsigned long long* bytes_out);
+                     unsigned long long* bytes_out, const BfsStage& stage);
 // Per paper: neighbourhood draws from its root's S, then PA refill.
 void launch_nb_draw(hipStream_t s, int64_t n_exist, int64_t n_new, int m, int n_nb, int cap,
                     uint64_t seed, uint32_t year_index, const double* cdf, const int32_t* root_slot,
@@ -142,7 +166,7 @@ void launch_nb_draw_batch(hipStream_t s, const Batch& b, int m, int cap, uint32_
 // workgroup path; ticks (nullable) gets per root the wall-clock ticks to the
 // end of the 1-hop level, to the end, and the 2-hop candidate entries.
 void launch_bfs_only(hipStream_t s, const CsrView& g, int64_t count, const int32_t* roots, int cap,
-                     int32_t* out, int32_t* out_len, int64_t* ticks);
+                     int32_t* out, int32_t* out_len, int64_t* ticks, const BfsStage& stage);
 
 // ---- CSR append ----
 // Sorted forward rows of the new papers appended at fwd_adj + fwd_base;
@@ -246,7 +270,9 @@ struct LikSegs {
 struct LikCiteShape {
     int G, K;
 };
-LikCiteShape lik_cite_shape(int m);
+LikCiteShape lik_cite_shape(int m, int lanes = 0);
+// Whether `lanes` lanes per citing paper can hold m slots (lanes 1,2,4,8).
+bool lik_lanes_ok(int m, int lanes);
 int64_t lik_paper_blocks(int64_t n_papers);
 int64_t lik_seg_chunks(int64_t S);
 // Once per sequence: the T = new_papers*m slots sorted by target (empty

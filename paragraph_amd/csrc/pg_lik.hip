@@ -57,7 +57,7 @@ int seq_prepare(pg_ctx* ctx) {
     if (!q.tables_dirty) return PG_OK;
     const int64_t Y = (int64_t)q.n_exist.size();
     const int64_t nP = lik_paper_blocks(q.n_papers);
-    const LikCiteShape sh = lik_cite_shape(q.m);
+    const LikCiteShape sh = lik_cite_shape(q.m, ctx->tune_lik_lanes);
     const int64_t per_block = 256 / sh.G;
     std::vector<int32_t> ev_block, year_blk(Y + 1, 0);
     for (int64_t y = 0; y < Y; ++y) {

@@ -138,10 +138,12 @@ def test_pa_tiny_graph_exhaustion():
     assert st.shortfall > 0
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("cap", [4096, 64, 1])
-def test_bfs_sets(cap):
+def test_bfs_sets(cap, mode):
     g = hub_graph(n=8000, hubs=5, seed=3)
     ctx = new_ctx(g, [1], 1, cap=cap)
+    ctx.set_tuning(bfs_mode=mode)
     G = oracle_graph(ctx, g.logfit)
     rng = np.random.default_rng(0)
     roots = np.concatenate([np.arange(0, 8), rng.integers(0, G.n, 120)]).astype(np.int32)
@@ -227,13 +229,33 @@ def multi_hub_graph(n=100_000, hubs=30, p=0.03, seed=12):
     return pg.SeedGraph(birth, src, dst, np.zeros(n + 10, np.float32))
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("cap", [4096, 1000])
-def test_bfs_bitmap_window_path(cap):
+def test_bfs_bitmap_window_path(cap, mode):
+    """Large candidate lists (~90k entries): mode 2 stages them in id buckets
+    (long hub rows read in pieces), modes 0 and 1 sweep the rows window by
+    window; the smaller roots' lists are staged in mode 0."""
     g = multi_hub_graph()
     ctx = new_ctx(g, [1], 1, cap=cap)
+    ctx.set_tuning(bfs_mode=mode)
     G = oracle_graph(ctx, g.logfit)
     roots = np.array([g.n - 1, 0, 1, 29, 31, 5000], np.int32)
-    log(f"bitmap-path BFS cap={cap}")
+    log(f"bitmap-path BFS cap={cap} mode={mode}")
     for r, s in zip(roots, ctx.bfs(roots)):
         ref = ob.bfs(G, int(r), cap)
+        assert np.array_equal(s, ref), f"root {r}: {s[:8]} vs {ref[:8]} ({s.size} vs {ref.size})"
+
+
+def test_bfs_stage_overflow():
+    """60 hubs of ~5000 citers each: the root's level-2 list, cut at cap+1
+    entries per row, exceeds the staging slot (131072 ids), so the staged
+    mode falls back to the sweep for it and stages the smaller roots."""
+    g = multi_hub_graph(hubs=60, p=0.05, seed=13)
+    ctx = new_ctx(g, [1], 1, cap=4096)
+    ctx.set_tuning(bfs_mode=2)
+    G = oracle_graph(ctx, g.logfit)
+    roots = np.array([g.n - 1, 0, 59, 61, 777], np.int32)
+    log("staging-overflow BFS")
+    for r, s in zip(roots, ctx.bfs(roots)):
+        ref = ob.bfs(G, int(r), 4096)
         assert np.array_equal(s, ref), f"root {r}: {s[:8]} vs {ref[:8]} ({s.size} vs {ref.size})"
