@@ -102,6 +102,14 @@ def likelihood_bench(args) -> dict:
     stream = torch.cuda.current_stream(dev).cuda_stream
     a0, t0_ = 1.0, 5.0
 
+    tune = None
+    if not args.no_tune:
+        tune = ctx.lik_autotune(a0, t0_, phi.data_ptr(), reps=5)
+        eprint(f"likelihood autotune: chosen {tune['chosen']['lik_lanes']} lanes "
+               f"{tune['chosen_ms']:.3f} ms (default {tune['default_ms']:.3f} ms, results within "
+               f"{tune['max_rel_dev']:.1e}); " +
+               ", ".join(f"{t['lik_lanes']}: {t['ms']:.3f}" for t in tune["trials"]))
+
     def fused():
         ctx.pa_loglik_grad(a0, t0_, phi.data_ptr(), grad.data_ptr(), scalars_dev=sc.data_ptr(),
                            stream=stream, host=False)
@@ -226,6 +234,7 @@ def likelihood_bench(args) -> dict:
                   "loglik_first": float(h[0]), "loglik_last": float(h[-1])}
     eprint(f"Adam fit ({args.fit_steps} steps, {fit_s:.2f} s): alpha {af:.4f} tau {tf:.4f}, "
            f"log L {h[0]:.10g} -> {h[-1]:.10g}")
+    res["autotune"] = tune
     ctx.close()
     return res
 
@@ -272,6 +281,13 @@ def ensemble_bench(args) -> dict:
     for s, (g, _) in enumerate(graphs):
         ens.load(s, g)
     eprint(f"ensemble loaded ({time.perf_counter() - t1:.1f} s)")
+    tune = None
+    if not args.no_tune:
+        t4 = time.perf_counter()
+        tune = ens.autotune(reps=args.tune_reps)   # leaves the seed state
+        eprint(f"autotune ({time.perf_counter() - t4:.1f} s): chosen {tune['chosen']} "
+               f"{tune['chosen_ms']:.1f} ms, defaults {tune['default_ms']:.1f} ms; " +
+               ", ".join(f"{t['pa_lanes']}/{t['bfs_mode']}: {t['ms']:.1f}" for t in tune["trials"]))
     st = ens.run(use_graph=False)
     eager_ms = st.device_ms
     ens.restore()
@@ -319,6 +335,7 @@ def ensemble_bench(args) -> dict:
                                f"10y at 3%/y, Philox keys 0..{len(sims) - 1}",
                    "seed_papers": n_seed, "new_papers": n_new, "edges_added": edges,
                    "near_boundary_attempts": st.near_boundary, "hipgraph": True},
+        "autotune": tune,
     }
 
 
@@ -358,6 +375,9 @@ def main():
     ap.add_argument("--ens-only", action="store_true",
                     help="only the ensemble part (its JSON object on stdout)")
     ap.add_argument("--ens-sims", type=int, default=256)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuners (ensemble, likelihood)")
+    ap.add_argument("--tune-reps", type=int, default=2)
     ap.add_argument("--bfs-mode", type=int, default=0,
                     help="2-hop workgroup path: 0 staged where it fits, 1 sweep only")
     ap.add_argument("--ens-reps", type=int, default=3)

@@ -370,6 +370,40 @@ int pg_ensemble_set_tuning(pg_ensemble* ens, const pg_tuning* t);
  * launch shapes at compile time (src/kernel.cu:1600-1640). */
 int pg_set_tuning(pg_ctx* ctx, const pg_tuning* t);
 
+/*
+ * ---- Autotuner ----
+ * No reference counterpart (the reference fixes its launch shapes at compile
+ * time, src/kernel.cu:1600-1640).  Times candidate launch shapes on the
+ * caller's own workload with hipEvents -- one untimed run per candidate
+ * (graph capture, table rebuilds), then `reps` timed runs, median -- and
+ * keeps the fastest.
+ */
+#define PG_TUNE_MAX_TRIALS 16
+typedef struct {
+    uint32_t size;
+    int32_t n_trials;
+    pg_tuning trial[PG_TUNE_MAX_TRIALS];
+    double trial_ms[PG_TUNE_MAX_TRIALS];  /* median device ms per candidate */
+    pg_tuning chosen;                      /* now set on the ensemble/context */
+    double chosen_ms;
+    double default_ms;                     /* the default shapes' median */
+    double max_rel_dev;                    /* pg_lik_autotune: largest relative
+                                              difference of log L, dL/dalpha,
+                                              dL/dtau from the default shapes'
+                                              (summation order); 0 for ensembles */
+} pg_tune_report;
+
+/* Ensemble: PA lanes per paper (8, 16, 32, 64) with the default 2-hop mode,
+ * then the 2-hop modes (0, 1, 2) with the best lanes; every trial is a whole
+ * graph-captured run from the seed state.  Candidates must reproduce the
+ * same edge, attempt and near-boundary counts (PG_ESTATE otherwise).  Leaves
+ * the ensemble in its seed state with the chosen shapes. */
+int pg_ensemble_autotune(pg_ensemble* ens, int32_t reps, pg_tune_report* report);
+/* Likelihood: lanes per citing paper in the slot pass (1, 2, 4, 8 with
+ * 8 * lanes >= m), each trial one pg_pa_loglik_grad at p's parameters on the
+ * context's stream.  Leaves the context with the chosen shape. */
+int pg_lik_autotune(pg_ctx* ctx, const pg_lik_params* p, int32_t reps, pg_tune_report* report);
+
 /* Ceil growth rule of the reference (src/abm.cu:233-240,
  * src/kernel.cu:1690): ceil(n * rate) in double precision. */
 int64_t pg_growth_count(int64_t n, double rate);

@@ -91,6 +91,25 @@ class Tuning(ctypes.Structure):
     _fields_ = [("size", ctypes.c_uint32), ("pa_lanes", ctypes.c_int32),
                 ("lik_lanes", ctypes.c_int32), ("bfs_mode", ctypes.c_int32)]
 
+    def as_dict(self) -> dict:
+        return {"pa_lanes": self.pa_lanes, "lik_lanes": self.lik_lanes, "bfs_mode": self.bfs_mode}
+
+
+TUNE_MAX_TRIALS = 16
+
+
+class TuneReport(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("n_trials", ctypes.c_int32),
+                ("trial", Tuning * TUNE_MAX_TRIALS), ("trial_ms", ctypes.c_double * TUNE_MAX_TRIALS),
+                ("chosen", Tuning), ("chosen_ms", ctypes.c_double), ("default_ms", ctypes.c_double),
+                ("max_rel_dev", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {"trials": [dict(self.trial[k].as_dict(), ms=self.trial_ms[k])
+                           for k in range(self.n_trials)],
+                "chosen": self.chosen.as_dict(), "chosen_ms": self.chosen_ms,
+                "default_ms": self.default_ms, "max_rel_dev": self.max_rel_dev}
+
 
 KERNEL_GROUPS = ["weights_cdf", "pa_draw", "bfs", "nb_draw", "er", "csr_append", "lik",
                  "lik_grad"]
@@ -166,6 +185,8 @@ def lib() -> ctypes.CDLL:
                                                     i32p, i32p]),
             "pg_ensemble_set_tuning": (ctypes.c_int, [vp, P(Tuning)]),
             "pg_set_tuning": (ctypes.c_int, [vp, P(Tuning)]),
+            "pg_ensemble_autotune": (ctypes.c_int, [vp, ctypes.c_int32, P(TuneReport)]),
+            "pg_lik_autotune": (ctypes.c_int, [vp, P(LikParams), ctypes.c_int32, P(TuneReport)]),
             "pg_pa_loglik": (ctypes.c_int, [vp, P(LikParams), P(LikOut)]),
             "pg_pa_loglik_grad": (ctypes.c_int, [vp, P(LikParams), P(LikOut)]),
         }
@@ -437,6 +458,17 @@ class Context:
         """log L, dL/dalpha, dL/dtau (host=True) and dL/dphi into grad_dev (FP64)."""
         return self._lik("pg_pa_loglik_grad", alpha, tau, logfit_dev, grad_dev=grad_dev, **kw)
 
+    def lik_autotune(self, alpha: float, tau: float, logfit_dev: int, reps: int = 3) -> dict:
+        """Times the likelihood's lanes-per-citing-paper candidates on this
+        sequence and keeps the fastest; returns the report."""
+        p = LikParams(ctypes.sizeof(LikParams), float(alpha), float(tau), None,
+                      logfit_dev or None, None, 0)
+        r = TuneReport()
+        r.size = ctypes.sizeof(TuneReport)
+        self._check(lib().pg_lik_autotune(self._h, ctypes.byref(p), int(reps), ctypes.byref(r)),
+                    "pg_lik_autotune")
+        return r.as_dict()
+
     def set_tuning(self, pa_lanes: int = 0, lik_lanes: int = 0, bfs_mode: int = 0):
         """Launch shapes (speed only): PA lanes per paper, likelihood lanes
         per citing paper, 2-hop workgroup path mode (0 staged up to 65536
@@ -530,6 +562,15 @@ class Ensemble:
         t = Tuning(ctypes.sizeof(Tuning), pa_lanes, lik_lanes, bfs_mode)
         self._check(lib().pg_ensemble_set_tuning(self._h, ctypes.byref(t)),
                     "pg_ensemble_set_tuning")
+
+    def autotune(self, reps: int = 2) -> dict:
+        """Times PA lanes and 2-hop modes on whole graph-captured runs of this
+        ensemble, keeps the fastest and returns the report (seed state kept)."""
+        r = TuneReport()
+        r.size = ctypes.sizeof(TuneReport)
+        self._check(lib().pg_ensemble_autotune(self._h, int(reps), ctypes.byref(r)),
+                    "pg_ensemble_autotune")
+        return r.as_dict()
 
     def download(self, sim: int) -> dict:
         n, e = ctypes.c_int64(), ctypes.c_int64()

@@ -460,3 +460,83 @@ int pg_ensemble_set_tuning(pg_ensemble* ens, const pg_tuning* t) {
 }
 
 }  // extern "C"
+
+namespace {
+
+double median_of(std::vector<double> v) {
+    std::sort(v.begin(), v.end());
+    const size_t n = v.size();
+    return n == 0 ? 0.0 : (n % 2 ? v[n / 2] : 0.5 * (v[n / 2 - 1] + v[n / 2]));
+}
+
+}  // namespace
+
+extern "C" {
+
+int pg_ensemble_autotune(pg_ensemble* ens, int32_t reps, pg_tune_report* report) {
+    if (!ens) return PG_EINVAL;
+    if (report && report->size != sizeof(pg_tune_report))
+        return efail(ens, PG_EINVAL, "pg_ensemble_autotune: pg_tune_report.size mismatch");
+    if (reps < 1 || reps > 64) return efail(ens, PG_EINVAL, "pg_ensemble_autotune: reps in [1, 64]");
+    if (!ens->ready) return efail(ens, PG_ESTATE, "pg_ensemble_autotune: not all simulations loaded");
+    std::vector<pg_tuning> cand;
+    std::vector<double> ms;
+    pg_ensemble_stats ref{};
+    bool have_ref = false;
+    // one candidate: untimed run (capture), then reps timed runs from the seed state
+    auto trial = [&](const pg_tuning& t) -> int {
+        int rc = pg_ensemble_set_tuning(ens, &t);
+        if (rc) return rc;
+        std::vector<double> v;
+        for (int r = 0; r <= reps; ++r) {
+            if ((rc = pg_ensemble_restore(ens))) return rc;
+            pg_ensemble_stats st{};
+            st.size = sizeof st;
+            if ((rc = pg_ensemble_run(ens, 1, &st))) return rc;
+            if (!have_ref) { ref = st; have_ref = true; }
+            if (st.edges_added != ref.edges_added || st.near_boundary != ref.near_boundary ||
+                st.pa_attempts != ref.pa_attempts)
+                return efail(ens, PG_ESTATE, "pg_ensemble_autotune: a launch shape changed the run");
+            if (r > 0) v.push_back(st.device_ms);
+        }
+        cand.push_back(t);
+        ms.push_back(median_of(v));
+        return PG_OK;
+    };
+    auto shape = [](int lanes, int mode) {
+        pg_tuning t{};
+        t.size = sizeof t;
+        t.pa_lanes = lanes;
+        t.bfs_mode = mode;
+        return t;
+    };
+    const int lanes_c[] = {8, 16, 32, 64};
+    const int def_lanes = pa_draw_lanes(ens->cfg.m);
+    double default_ms = 0.0;
+    for (int l : lanes_c) {
+        if (int rc = trial(shape(l, 0))) return rc;
+        if (l == def_lanes) default_ms = ms.back();
+    }
+    size_t best = std::min_element(ms.begin(), ms.end()) - ms.begin();
+    const int best_lanes = cand[best].pa_lanes;
+    for (int mode : {1, 2})
+        if (int rc = trial(shape(best_lanes, mode))) return rc;
+    best = std::min_element(ms.begin(), ms.end()) - ms.begin();
+    const pg_tuning chosen = cand[best];
+    if (int rc = pg_ensemble_set_tuning(ens, &chosen)) return rc;
+    if (int rc = pg_ensemble_restore(ens)) return rc;
+    if (report) {
+        report->n_trials = (int32_t)std::min<size_t>(cand.size(), PG_TUNE_MAX_TRIALS);
+        for (int k = 0; k < report->n_trials; ++k) {
+            report->trial[k] = cand[k];
+            report->trial_ms[k] = ms[k];
+        }
+        report->chosen = chosen;
+        report->chosen_ms = ms[best];
+        report->default_ms = default_ms;
+        report->max_rel_dev = 0.0;
+    }
+    return PG_OK;
+}
+
+}  // extern "C"

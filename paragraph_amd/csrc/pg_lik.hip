@@ -463,3 +463,88 @@ int pg_pa_loglik_grad(pg_ctx* ctx, const pg_lik_params* p, pg_lik_out* out) {
 }
 
 }  // extern "C"
+
+extern "C" {
+
+int pg_lik_autotune(pg_ctx* ctx, const pg_lik_params* p, int32_t reps, pg_tune_report* report) {
+    if (!ctx) return PG_EINVAL;
+    if (!p || p->size != sizeof(pg_lik_params) || (report && report->size != sizeof(pg_tune_report)))
+        return fail(ctx, PG_EINVAL, "pg_lik_autotune: null argument or struct size mismatch");
+    if (reps < 1 || reps > 64) return fail(ctx, PG_EINVAL, "pg_lik_autotune: reps in [1, 64]");
+    pg_seq_store& q = ctx->seq;
+    if (!q.reserved) return fail(ctx, PG_ESTATE, "pg_lik_autotune: no sequence");
+    PG_HIP(hipSetDevice(ctx->cfg.device));
+    pg_lik_params pp = *p;
+    pp.stream = nullptr;
+    pp.stream_given = 0;  // the context's stream, timed by its events
+    double* grad = nullptr;
+    if (dalloc(&grad, std::max<int64_t>(q.n_papers, 1)) != hipSuccess)
+        return fail(ctx, PG_ENOMEM, "pg_lik_autotune: device allocation failed");
+    struct Freer { double* g; ~Freer() { (void)hipFree(g); } } freer{grad};
+    const int saved = ctx->tune_lik_lanes;
+    const int def_lanes = lik_cite_shape(q.m, 0).G;
+    std::vector<int> cand;
+    std::vector<double> ms;
+    double ref[3] = {0, 0, 0}, max_dev = 0.0, default_ms = 0.0;
+    for (int lanes : {1, 2, 4, 8}) {
+        if (!lik_lanes_ok(q.m, lanes)) continue;
+        if (lanes != ctx->tune_lik_lanes) q.tables_dirty = true;
+        ctx->tune_lik_lanes = lanes;
+        std::vector<double> v;
+        for (int r = 0; r <= reps; ++r) {
+            double h[3];
+            pg_lik_out out{};
+            out.size = sizeof out;
+            out.grad_logfit_dev = grad;
+            out.host3 = r == 0 ? h : nullptr;
+            PG_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+            if (int rc = pg_pa_loglik_grad(ctx, &pp, &out)) {
+                ctx->tune_lik_lanes = saved;
+                q.tables_dirty = true;
+                return rc;
+            }
+            PG_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+            PG_HIP(hipEventSynchronize(ctx->ev1));
+            if (r == 0) {  // untimed: table rebuild; results compared
+                if (cand.empty()) {
+                    for (int k = 0; k < 3; ++k) ref[k] = h[k];
+                } else {
+                    for (int k = 0; k < 3; ++k) {
+                        const double d = std::fabs(h[k] - ref[k]) / std::max(std::fabs(ref[k]), 1e-300);
+                        max_dev = std::max(max_dev, d);
+                    }
+                }
+                continue;
+            }
+            float t = 0.0f;
+            PG_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+            v.push_back(t);
+        }
+        std::sort(v.begin(), v.end());
+        cand.push_back(lanes);
+        ms.push_back(v.size() % 2 ? v[v.size() / 2] : 0.5 * (v[v.size() / 2 - 1] + v[v.size() / 2]));
+        if (lanes == def_lanes) default_ms = ms.back();
+    }
+    const size_t best = std::min_element(ms.begin(), ms.end()) - ms.begin();
+    if (ctx->tune_lik_lanes != cand[best]) q.tables_dirty = true;
+    ctx->tune_lik_lanes = cand[best];
+    if (report) {
+        report->n_trials = (int32_t)cand.size();
+        for (size_t k = 0; k < cand.size(); ++k) {
+            pg_tuning t{};
+            t.size = sizeof t;
+            t.pa_lanes = ctx->tune_pa_lanes;
+            t.lik_lanes = cand[k];
+            t.bfs_mode = ctx->tune_bfs_mode;
+            report->trial[k] = t;
+            report->trial_ms[k] = ms[k];
+        }
+        report->chosen = report->trial[best];
+        report->chosen_ms = ms[best];
+        report->default_ms = default_ms;
+        report->max_rel_dev = max_dev;
+    }
+    return PG_OK;
+}
+
+}  // extern "C"

@@ -102,6 +102,24 @@ def test_ensemble_tuning_does_not_change_results():
         check_same(ens, refs)
 
 
+def test_ensemble_autotune_keeps_results():
+    """The autotuner times 4 PA lane counts and 2 more 2-hop modes on whole
+    captured runs, keeps the fastest, and leaves the seed state: a run with
+    the chosen shapes reproduces the plain runs bit for bit."""
+    sims = SIMS[:4]
+    graphs = seed_graphs(sims)
+    refs = [plain_run(sp, g, sched)[0] for sp, (g, sched) in zip(sims, graphs)]
+    ens = make_ensemble(sims, graphs)
+    rep = ens.autotune(reps=2)
+    log("autotune", rep["chosen"], f"{rep['chosen_ms']:.3f} ms (default {rep['default_ms']:.3f} ms)")
+    assert len(rep["trials"]) == 6
+    assert rep["chosen_ms"] == min(t["ms"] for t in rep["trials"])
+    assert rep["chosen"]["pa_lanes"] in (8, 16, 32, 64) and rep["chosen"]["bfs_mode"] in (0, 1, 2)
+    assert rep["default_ms"] > 0
+    ens.run(use_graph=True)   # the autotuner left the seed state
+    check_same(ens, refs)
+
+
 def test_ensemble_errors():
     with pytest.raises(pg.PGError, match="EINVAL"):
         pg.Ensemble([dict(n0=100, alpha=1.0, tau=5.0, n_nb=12, seed=1, seed_edges=10)], 2, 10)

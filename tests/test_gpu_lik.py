@@ -203,6 +203,25 @@ def test_fit_adam_moves_towards_generating_parameters():
     assert abs(t - 4.0) < abs(5.0 - 4.0)
 
 
+def test_lik_autotune():
+    """Lanes per citing paper: every valid candidate is timed, the results
+    agree with the default shape's to 1e-12 (only the FP64 summation order
+    differs), and the chosen shape reproduces the oracle."""
+    case = dict(n0=2000, years=3, rate=0.05, m=7, n_nb=2, alpha=1.2, tau=4.0, seed=5)
+    seq, birth, logfit, m = simulate(**case)
+    ctx = seq_ctx(seq, birth, m)
+    import torch
+    phi = torch.as_tensor(np.ascontiguousarray(logfit, np.float32), device="cuda")
+    rep = ctx.lik_autotune(1.1, 4.5, phi.data_ptr(), reps=2)
+    log("lik autotune", rep)
+    assert [t["lik_lanes"] for t in rep["trials"]] == [1, 2, 4, 8]   # 8 * lanes >= 7
+    assert rep["max_rel_dev"] <= 1e-12
+    assert rep["chosen"]["lik_lanes"] in (1, 2, 4, 8)
+    ll, ga, gt, gphi, _ = gpu_eval(ctx, 1.1, 4.5, logfit)
+    r = oracle_eval(seq, birth, logfit, m, 1.1, 4.5, w32=True)
+    assert rel(ll, r[0]) <= 1e-12 and rel(ga, r[1]) <= 1e-9 and rel(gt, r[2]) <= 1e-9
+
+
 def test_error_paths():
     import torch
     ctx = pg.Context(100, 16, 16, max_cites=4, seed=1)
