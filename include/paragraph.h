@@ -197,7 +197,9 @@ int pg_snapshot_restore(pg_ctx* ctx);
  *                4*(E_new+N_new) write + 8 B per new edge (degree RMW);
  *   LIK          4 B per paper-year (in-degree snapshot) + 8 B per paper
  *                (birth, log-fitness) + 4 B per citing paper (count) +
- *                17 B per filled slot (target, kind, 12 B gathered);
+ *                17 B per filled slot (target, kind, the 8 B {log(k+1),
+ *                age} record fixed by the sequence, 4 B log-fitness
+ *                gathered);
  *   LIK_GRAD     two paper passes: 8 B per paper-year + 44 B per paper
  *                (2x birth and log-fitness, segment offset, per-paper share
  *                sum written and read, dL/dphi written) + 4 B per citing

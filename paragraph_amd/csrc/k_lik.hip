@@ -79,6 +79,14 @@ __device__ inline float lik_weight(int32_t k, int32_t age, float phi, const LikS
     return det_expf(x);
 }
 
+// The same from the target's L = log(k+1) and age (as float, exact below
+// 2^24), precomputed per slot: identical arithmetic.
+__device__ inline float lik_weight_la(float L, float age, float phi, const LikScal& s, float& x) {
+    const float t1 = __builtin_fmaf(-age, s.inv_tau_f, phi);
+    x = __builtin_fmaf(s.alpha_f, L, t1);
+    return det_expf(x);
+}
+
 __device__ inline double wave_sum_d(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -249,8 +257,7 @@ __global__ __launch_bounds__(kLikBT) void k_lik_cite(LikSeq q, LikArgs a,
     const int64_t P = q.new_off[y] + i;               // sequence-wide citing index
     const int c = i < Dy ? q.cnt[P] : 0;
     const int j0 = lg * K;                            // first slot of this lane
-    float wr[K], Lr[K];
-    int32_t ar[K];
+    float wr[K], Lr[K], ar[K];
     uint32_t evm = 0, vm = 0;
     double ll = 0.0, gaacc = 0.0, gtacc = 0.0;
     if (c > 0) {
@@ -261,19 +268,18 @@ __global__ __launch_bounds__(kLikBT) void k_lik_cite(LikSeq q, LikArgs a,
             const int64_t qi = (int64_t)(v ? j : 0) * NP + P;
             const int32_t t = q.cit_t[qi];
             const bool ev = v && q.kind_t[qi] != 1;
-            const int32_t k = q.deg[doff + t];
-            const int32_t age = year - q.birth[t];
-            float L, x;
-            const float w = lik_weight(k, age, a.logfit[t], s, L, x);
+            const float2 la = q.la_t[qi];
+            float x;
+            const float w = lik_weight_la(la.x, la.y, a.logfit[t], s, x);
             wr[e] = v ? w : 0.0f;
-            Lr[e] = L;
-            ar[e] = age;
+            Lr[e] = la.x;
+            ar[e] = la.y;
             evm |= uint32_t(ev) << e;
             vm |= uint32_t(v) << e;
             ll += ev ? (double)x : 0.0;
             if (GRAD) {
-                gaacc += ev ? (double)L : 0.0;
-                gtacc += ev ? (double)age : 0.0;
+                gaacc += ev ? (double)la.x : 0.0;
+                gtacc += ev ? (double)la.y : 0.0;
             }
         }
     } else {
@@ -281,7 +287,7 @@ __global__ __launch_bounds__(kLikBT) void k_lik_cite(LikSeq q, LikArgs a,
         for (int e = 0; e < K; ++e) {
             wr[e] = 0.0f;
             Lr[e] = 0.0f;
-            ar[e] = 0;
+            ar[e] = 0.0f;
         }
     }
     // prefix of the weights in slot order, chained across the lanes
@@ -568,7 +574,35 @@ void launch_cite(hipStream_t st, int G, int K, int64_t nblk, const LikSeq& q, co
     }
 }
 
+// Sequence preparation: {log(k+1), age} of slot s's target in its year
+// (k = the target's in-degree at the start of the year), slot-major like
+// cit_t; zero for empty slots.  Fixed by the sequence.
+__global__ __launch_bounds__(kLikBT) void k_lik_slot_la(LikSeq q, float2* __restrict__ la_t) {
+    const int64_t NP = q.n_cite;
+    const int64_t idx = (int64_t)blockIdx.x * kLikBT + threadIdx.x;
+    if (idx >= NP * q.m) return;
+    const int64_t j = idx / NP, P = idx - j * NP;
+    int a = 0, z = q.n_years;  // year y with new_off[y] <= P < new_off[y+1]
+    while (z - a > 1) {
+        const int mid = (a + z) >> 1;
+        if (q.new_off[mid] <= P) a = mid; else z = mid;
+    }
+    float2 r = make_float2(0.0f, 0.0f);
+    if (j < q.cnt[P]) {
+        const int32_t t = q.cit_t[idx];
+        r.x = det_logf((float)(q.deg[q.deg_off[a] + t] + 1));
+        r.y = (float)(q.year_of[a] - q.birth[t]);
+    }
+    la_t[idx] = r;
+}
+
 }  // namespace
+
+void launch_lik_slot_la(hipStream_t st, const LikSeq& q, float2* la_t) {
+    const int64_t T = q.n_cite * q.m;
+    if (T <= 0) return;
+    k_lik_slot_la<<<(unsigned)((T + kLikBT - 1) / kLikBT), kLikBT, 0, st>>>(q, la_t);
+}
 
 bool lik_lanes_ok(int m, int lanes) {
     return (lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8) && lanes * 8 >= m;

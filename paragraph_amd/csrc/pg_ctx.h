@@ -56,6 +56,7 @@ struct pg_seq_store {
     uint32_t* pos_t = nullptr;       // [new * m] by-target position, slot-major
     int32_t* cit_t = nullptr;        // [new * m] targets, slot-major
     int8_t* kind_t = nullptr;        // [new * m] kinds, slot-major
+    float2* la_t = nullptr;          // [new * m] {log(k+1), age} of each slot's target, slot-major
     int32_t* seg_off = nullptr;      // [cap_papers + 1]
     double* share = nullptr;         // [filled] per-slot dL/dphi shares, by-target order
     double* evt = nullptr;           // [cap_papers]
