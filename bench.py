@@ -27,6 +27,44 @@ import paragraph_amd as pg  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
 
+# Kernels of each timed group, as they appear in the rocprofv3 summaries.
+GROUP_KERNELS = {
+    "bfs": ["k_bfs_small", "k_bfs_root"],
+    "lik_grad": ["k_lik_papers<false>", "k_lik_year_z", "k_lik_cite<true, 4, 5>", "k_lik_year_ev",
+                 "k_lik_segsum", "k_lik_papers<true>", "k_lik_final"],
+}
+
+
+def measured_traffic(summary: str, group: str):
+    """Memory-side bytes per launch of a kernel group: the sum over its
+    kernels of the copy-calibrated FETCH_SIZE + WRITE_SIZE per dispatch in
+    the committed rocprofv3 summary profiles/<summary> (two --pmc runs of the
+    same workload).  None when the summary or a kernel row is missing."""
+    path = os.path.join(ROOT, "profiles", summary)
+    if not os.path.exists(path):
+        return None
+    rows = {}
+    for line in open(path):
+        c = [x.strip() for x in line.strip().strip("|").split("|")]
+        if len(c) == 4 and c[1].isdigit():
+            try:
+                rows[c[0]] = (float(c[2]) + float(c[3])) * 1e6
+            except ValueError:
+                pass
+    ks = GROUP_KERNELS.get(group, [])
+    if not ks or any(k not in rows for k in ks):
+        return None
+    return sum(rows[k] for k in ks)
+
+
+def roofline(kernel: str, achieved: float, summary: str, group: str) -> dict:
+    t = measured_traffic(summary, group)
+    return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": PEAK_HBM_GBS,
+            "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": t,
+            "traffic_note": None if t is None else
+            f"bytes per launch of {', '.join(GROUP_KERNELS[group])}: FETCH_SIZE + WRITE_SIZE "
+            f"per dispatch from profiles/{summary} (rocprofv3 --pmc, copy-calibrated)"}
+
 
 def eprint(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -187,9 +225,8 @@ def likelihood_bench(args) -> dict:
                                "gradient at alpha=1, tau=5",
                    "paper_years": info.paper_years, "citing_papers": info.new_papers,
                    "slots": info.new_papers * m, "params": info.n_papers + 2},
-        "roofline": {"bound": "hbm", "kernel": "pg_pa_loglik_grad (all passes)",
-                     "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None},
+        "roofline": roofline("pg_pa_loglik_grad (all passes)", achieved,
+                             "likelihood_30y_1M_kernels.md", "lik_grad"),
         "vs_plain_torch_rel_err": {
             "dalpha": abs(ga - ga2) / abs(ga2), "dtau": abs(gt - gt2) / abs(gt2),
             "dphi_l2": float(np.linalg.norm(g_gpu - g_plain) / np.linalg.norm(g_plain))},
@@ -470,8 +507,7 @@ def main():
                    "near_boundary_attempts": total_near,
                    "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in prof.items()
                                           if v["launches"]}},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None},
+        "roofline": roofline(dom, achieved, "pa_10y_2M_kernels.md", dom),
     }
     ctx.close()
     if not args.no_cpu_baseline:
