@@ -96,6 +96,91 @@ def cpu_baseline(G0, sched, m, n_nb, alpha, tau, seed, cap):
                       f"with {cores} OpenMP threads: {dt:.2f} s"}
 
 
+def er_bench(args) -> dict:
+    """ER G(n,p) growth: 1,000,000 isolated seed papers, 10 years at 3% per
+    year, each new paper linked to each existing paper with p = 1e-5
+    (geometric skips), Philox seed 1.  Timed: the 10 device year steps (edge
+    formation + CSR append), the seed restored from a device snapshot between
+    steps outside the timed region.  Year 0's edge set is compared with the
+    oracle's, and the oracle times the whole workload on the host cores."""
+    n0, years, rate, p, seed = 1_000_000, 10, 0.03, 1e-5, 1
+    sched = pg.growth_schedule(n0, rate, years)
+    n_final = n0 + sum(sched)
+    none = np.zeros(0, np.int32)
+    g = pg.SeedGraph(np.zeros(n0, np.int32), none, none, np.zeros(n_final, np.float32))
+    mean_e, n = 0.0, n0
+    for d in sched:
+        mean_e += n * d * p
+        n += d
+    e_cap = int(mean_e * 1.2 + 200_000)
+    ctx = pg.Context(n_final, e_cap, max(sched), max_cites=1, bfs_cap=16, seed=seed)
+    ctx.load(g)
+    ctx.snapshot_save()
+
+    def run():
+        e = 0
+        for yi, d in enumerate(sched):
+            e += ctx.year_er(1 + yi, yi, d, p).edges_added
+        return e
+
+    run()                      # warm-up
+    ctx.snapshot_restore()
+    # year 0 against the oracle: the new papers' forward rows
+    from oracle import binding as ob
+    ctx.year_er(1, 0, sched[0], p)
+    G = ctx.download()
+    off_ref, tgt_ref = ob.er_year(n0, sched[0], p, seed, 0)
+    fo = G["fwd_off"][n0:n0 + sched[0] + 1].astype(np.int64)
+    tgt_gpu = G["fwd_adj"][fo[0]:fo[-1]]
+    ok = (np.array_equal(fo - fo[0], off_ref.astype(np.int64)) and
+          np.array_equal(tgt_gpu, tgt_ref))
+    if not ok:
+        raise RuntimeError("ER year 0 differs from the oracle")
+    ctx.snapshot_restore()
+    ctx.profile_reset()
+    ctx.profile(True)
+    walls, edges = [], 0
+    for _ in range(max(args.steps, 1)):
+        t0 = time.perf_counter()
+        edges = run()
+        walls.append(time.perf_counter() - t0)
+        ctx.snapshot_restore()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    ctx.close()
+    wall = float(np.median(walls))
+    er = prof["er"]
+    achieved = (er["bytes"] / max(er["launches"], 1)) / (er["ms"] / max(er["launches"], 1) * 1e-3) / 1e9
+    eprint(f"ER: {edges} edges in {wall * 1e3:.2f} ms ({edges / wall:.3e} edges/s); "
+           f"er kernels {er['ms'] / len(walls):.2f} ms, csr_append "
+           f"{prof['csr_append']['ms'] / len(walls):.2f} ms per run; year 0 equals the oracle")
+    res = {"metric": "ER citation edges/s (10y, 1M isolated seed, p=1e-5)", "value": edges / wall,
+           "unit": "citation edges/s", "ms_per_step": wall * 1e3, "edges_per_step": edges,
+           "year0_equals_oracle": ok,
+           "kernel_ms_per_step": {k: v["ms"] / len(walls) for k, v in prof.items() if v["launches"]},
+           "roofline": {"bound": "hbm", "kernel": "er", "achieved": achieved, "peak": PEAK_HBM_GBS,
+                        "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                        "traffic_note": "4 B per target + 4 B per new paper: about 16 MB per run, "
+                                        "latency- and FP64-log-bound"}}
+    if not args.no_cpu_baseline:
+        cores = len(os.sched_getaffinity(0))
+        t0 = time.perf_counter()
+        ce, n = 0, n0
+        for yi, d in enumerate(sched):
+            _, tgt = ob.er_year(n, d, p, seed, yi, threads=cores)
+            ce += tgt.size
+            n += d
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": ce / dt, "unit": "citation edges/s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"the edge formation of all {years} years ({ce} edges) on "
+                                         f"the C oracle with {cores} OpenMP threads: {dt:.2f} s "
+                                         f"(no CSR append)"}
+        if ce != edges:
+            raise RuntimeError(f"ER oracle {ce} edges, GPU {edges}")
+    return res
+
+
 def likelihood_bench(args) -> dict:
     """Likelihood-gradient evaluations per second on the 30-year sequence.
 
@@ -409,6 +494,8 @@ def main():
     ap.add_argument("--fit-steps", type=int, default=100)
     ap.add_argument("--no-lik", action="store_true", help="skip the likelihood part")
     ap.add_argument("--no-ens", action="store_true", help="skip the ensemble part")
+    ap.add_argument("--no-er", action="store_true", help="skip the ER part")
+    ap.add_argument("--er-only", action="store_true", help="only the ER part")
     ap.add_argument("--ens-only", action="store_true",
                     help="only the ensemble part (its JSON object on stdout)")
     ap.add_argument("--ens-sims", type=int, default=256)
@@ -429,6 +516,9 @@ def main():
         return
     if args.ens_only:
         print(json.dumps(ensemble_bench(args)), flush=True)
+        return
+    if args.er_only:
+        print(json.dumps(er_bench(args)), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -520,6 +610,8 @@ def main():
         out["likelihood"] = likelihood_bench(args)
     if not args.no_ens:
         out["ensemble"] = ensemble_bench(args)
+    if not args.no_er:
+        out["er"] = er_bench(args)
     print(json.dumps(out), flush=True)
 
 
