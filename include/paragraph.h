@@ -39,7 +39,8 @@ enum {
     PG_EHIP = 3,     /* HIP runtime error */
     PG_ECAP = 4,     /* a capacity fixed at pg_ctx_create would be exceeded */
     PG_ESTATE = 5,   /* call out of order (e.g. no graph loaded) */
-    PG_ENODEV = 6    /* no usable gfx950 device */
+    PG_ENODEV = 6,   /* no usable gfx950 device */
+    PG_EIO = 7       /* checkpoint file unreadable, unwritable or malformed */
 };
 
 enum { PG_MODEL_PA = 0, PG_MODEL_ER = 1 };
@@ -180,6 +181,34 @@ int pg_bfs_timed(pg_ctx* ctx, int64_t count, const int32_t* roots, int32_t* out,
  */
 int pg_snapshot_save(pg_ctx* ctx);
 int pg_snapshot_restore(pg_ctx* ctx);
+
+/*
+ * Year steps completed since pg_graph_load (or in the loaded checkpoint),
+ * and the last step's year and year index (-1: none).  Snapshots keep them.
+ */
+int pg_get_progress(pg_ctx* ctx, int32_t* steps_done, int32_t* last_year,
+                    int32_t* last_year_index);
+
+/*
+ * Exact year-granular checkpoint and resume.  No reference counterpart (the
+ * reference keeps no state between runs).  pg_checkpoint_save writes the
+ * graph state -- both CSRs, in-degrees, birth years, the log-fitness of
+ * every paper supplied, the Philox seed and the progress above -- plus an
+ * optional caller blob (user, user_bytes) to `path` (through a temporary
+ * file renamed into place), with an FNV-1a checksum.  pg_checkpoint_load
+ * reads it into a context created with the same seed and capacities at
+ * least as large; the caller's blob goes to user[0, user_cap) and its size
+ * to *user_bytes (nullable).  Log-fitness the context already holds for
+ * papers beyond the checkpoint's (pg_graph_load of a longer run of the same
+ * seed) is kept, so such a run can resume from a shorter run's checkpoint.  Philox draws depend only on (seed, paper,
+ * attempt, year index), so the loaded context continues bit for bit like
+ * the one that saved.  The likelihood sequence store is not part of it.
+ * PG_EIO on a file error, bad magic, version or checksum; PG_EINVAL on a
+ * seed mismatch; PG_ECAP when the state does not fit the context.
+ */
+int pg_checkpoint_save(pg_ctx* ctx, const char* path, const void* user, int64_t user_bytes);
+int pg_checkpoint_load(pg_ctx* ctx, const char* path, void* user, int64_t user_cap,
+                       int64_t* user_bytes);
 
 /*
  * Per-kernel-group timing with hipEvents recorded on the context's stream

@@ -21,7 +21,7 @@ ABM_PATH = os.path.join(_HERE, "abm")
 PG_MODEL_PA = 0
 PG_MODEL_ER = 1
 
-_ERRORS = {1: "EINVAL", 2: "ENOMEM", 3: "EHIP", 4: "ECAP", 5: "ESTATE", 6: "ENODEV"}
+_ERRORS = {1: "EINVAL", 2: "ENOMEM", 3: "EHIP", 4: "ECAP", 5: "ESTATE", 6: "ENODEV", 7: "EIO"}
 
 
 class PGError(RuntimeError):
@@ -156,6 +156,9 @@ def lib() -> ctypes.CDLL:
             "pg_growth_count": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_double]),
             "pg_snapshot_save": (ctypes.c_int, [vp]),
             "pg_snapshot_restore": (ctypes.c_int, [vp]),
+            "pg_get_progress": (ctypes.c_int, [vp, i32p, i32p, i32p]),
+            "pg_checkpoint_save": (ctypes.c_int, [vp, ctypes.c_char_p, vp, ctypes.c_int64]),
+            "pg_checkpoint_load": (ctypes.c_int, [vp, ctypes.c_char_p, vp, ctypes.c_int64, i64p]),
             "pg_profile_enable": (ctypes.c_int, [vp, ctypes.c_int]),
             "pg_profile_read": (ctypes.c_int, [vp, ctypes.c_int, f64p, f64p, i64p]),
             "pg_profile_reset": (ctypes.c_int, [vp]),
@@ -481,6 +484,27 @@ class Context:
 
     def snapshot_restore(self):
         self._check(lib().pg_snapshot_restore(self._h), "pg_snapshot_restore")
+
+    def progress(self) -> tuple[int, int, int]:
+        """(year steps done, last year, last year index)."""
+        a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        self._check(lib().pg_get_progress(self._h, ctypes.byref(a), ctypes.byref(b),
+                                          ctypes.byref(c)), "pg_get_progress")
+        return a.value, b.value, c.value
+
+    def checkpoint_save(self, path: str, user: bytes = b""):
+        """Writes the graph state, progress and `user` to `path`."""
+        buf = ctypes.create_string_buffer(user, len(user)) if user else None
+        self._check(lib().pg_checkpoint_save(self._h, os.fsencode(path), buf, len(user)),
+                    "pg_checkpoint_save")
+
+    def checkpoint_load(self, path: str, user_cap: int = 1 << 20) -> bytes:
+        """Replaces the graph state by the checkpoint's; returns its user blob."""
+        buf = ctypes.create_string_buffer(max(user_cap, 1))
+        nb = ctypes.c_int64()
+        self._check(lib().pg_checkpoint_load(self._h, os.fsencode(path), buf, user_cap,
+                                             ctypes.byref(nb)), "pg_checkpoint_load")
+        return buf.raw[:nb.value]
 
     def profile(self, on: bool = True):
         self._check(lib().pg_profile_enable(self._h, 1 if on else 0), "pg_profile_enable")

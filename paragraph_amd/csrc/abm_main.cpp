@@ -71,7 +71,9 @@ const char* kKnown[] = {
     "--model", "--er-probability", "--er-edges-per-node",
     // this library's model parameters
     "--pa-exponent", "--age-decay", "--neighbourhood-fraction", "--citations-per-paper",
-    "--seed", "--bfs-cap", "--fitness-sigma", "--device"};
+    "--seed", "--bfs-cap", "--fitness-sigma", "--device",
+    // exact year-granular checkpoint / resume (no reference counterpart)
+    "--checkpoint-file", "--resume-from"};
 
 const char* kRequired[] = {"--edgelist", "--nodelist", "--out-degree-bag",
                            "--recency-probabilities", "--growth-rate", "--num-cycles",
@@ -85,7 +87,8 @@ void usage() {
                  "           --output-file F --auxiliary-information-file F --log-file F\n"
                  "           [--model pa|er-gnp] [--er-probability p] [--er-edges-per-node k]\n"
                  "           [--pa-exponent a] [--age-decay tau] [--neighbourhood-fraction f | --alpha f]\n"
-                 "           [--citations-per-paper m] [--seed s] [--bfs-cap c] [--fitness-sigma s]\n");
+                 "           [--citations-per-paper m] [--seed s] [--bfs-cap c] [--fitness-sigma s]\n"
+                 "           [--checkpoint-file F (written after every year)] [--resume-from F]\n");
 }
 
 Args parse(int argc, char** argv) {
@@ -293,11 +296,27 @@ int main(int argc, char** argv) {
                             src.data(), dst.data()), ctx, "pg_graph_load");
         log.info("loaded " + std::to_string(n0) + " papers and " + std::to_string(src.size()) +
                  " citations");
+        std::vector<int32_t> roots(n_final - n0, -1);
+        int y_start = 0;
+        if (a.has("--resume-from")) {
+            // the checkpoint replaces the loaded state; its blob is the roots so far
+            int64_t ub = 0;
+            check(pg_checkpoint_load(ctx, a.str("--resume-from").c_str(), roots.data(),
+                                     (int64_t)roots.size() * 4, &ub), ctx, "pg_checkpoint_load");
+            int32_t done = 0, last_year = 0, last_yi = -1;
+            pg_get_progress(ctx, &done, &last_year, &last_yi);
+            int64_t grown = 0;
+            for (int y = 0; y < std::min(done, years); ++y) grown += sched[y];
+            if (done > years || last_yi != done - 1 || (done > 0 && last_year != max_year + done) ||
+                ub != grown * 4)
+                throw std::runtime_error("--resume-from: the checkpoint is not of this run");
+            y_start = done;
+            log.info("resumed after " + std::to_string(done) + " years");
+        }
         // ---- year loop ----
         const auto t_loop = Clock::now();
         int64_t total_edges = 0;
-        std::vector<int32_t> roots(n_final - n0, -1);
-        for (int y = 0; y < years; ++y) {
+        for (int y = y_start; y < years; ++y) {
             pg_year_params p{};
             p.size = sizeof(p);
             p.model = model == "pa" ? PG_MODEL_PA : PG_MODEL_ER;
@@ -329,9 +348,12 @@ int main(int argc, char** argv) {
                           (long long)st.shortfall, st.device_ms);
             log.info(msg);
             std::printf("%s\n", msg);
+            if (a.has("--checkpoint-file"))  // roots of the papers grown so far as the blob
+                check(pg_checkpoint_save(ctx, a.str("--checkpoint-file").c_str(), roots.data(),
+                                         (nb_before + sched[y] - n0) * 4), ctx, "pg_checkpoint_save");
         }
         const double loop_s = std::chrono::duration<double>(Clock::now() - t_loop).count();
-        std::printf("Growth: %d years, %lld new citation edges in %.3f s (%.3e edges/s)\n", years,
+        std::printf("Growth: %d years, %lld new citation edges in %.3f s (%.3e edges/s)\n", years - y_start,
                     (long long)total_edges, loop_s, loop_s > 0 ? total_edges / loop_s : 0.0);
         // ---- outputs ----
         int64_t nn = 0, ne = 0;

@@ -250,8 +250,6 @@ __global__ __launch_bounds__(kLikBT) void k_lik_cite(LikSeq q, LikArgs a,
     const int lg = threadIdx.x % G;                   // lane within the paper's group
     const int64_t i = (int64_t)ev_block[2 * blockIdx.x + 1] + threadIdx.x / G;
     const int64_t Dy = q.n_new[y];
-    const int64_t doff = q.deg_off[y];
-    const int32_t year = q.year_of[y];
     const double Z = year_sums[y * kLikYearCols + 0];
     const int64_t NP = q.n_cite;
     const int64_t P = q.new_off[y] + i;               // sequence-wide citing index

@@ -290,8 +290,12 @@ int pg_graph_load(pg_ctx* ctx, int64_t n, const int32_t* birth, const float* log
     ctx->n = n;
     ctx->e = out_e;
     ctx->n_logfit = n_logfit;
+    ctx->logfit_owned = true;
     ctx->loaded = true;
     ctx->last_model = -1;
+    ctx->steps_done = 0;
+    ctx->last_year = 0;
+    ctx->last_year_index = -1;
     ctx->err.clear();
     return PG_OK;
 }
@@ -469,6 +473,9 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
     ctx->last_n_exist = N;
     ctx->last_m = p->model == PG_MODEL_PA ? p->m : 0;
     ctx->last_model = p->model;
+    ctx->steps_done += 1;
+    ctx->last_year = p->year;
+    ctx->last_year_index = p->year_index;
     return PG_OK;
 }
 
@@ -501,6 +508,9 @@ int pg_snapshot_save(pg_ctx* ctx) {
     ctx->snap = true;
     ctx->snap_n = n;
     ctx->snap_e = e;
+    ctx->snap_steps = ctx->steps_done;
+    ctx->snap_last_year = ctx->last_year;
+    ctx->snap_last_year_index = ctx->last_year_index;
     return PG_OK;
 }
 
@@ -523,6 +533,9 @@ int pg_snapshot_restore(pg_ctx* ctx) {
     ctx->n = n;
     ctx->e = e;
     ctx->last_model = -1;
+    ctx->steps_done = ctx->snap_steps;
+    ctx->last_year = ctx->snap_last_year;
+    ctx->last_year_index = ctx->snap_last_year_index;
     return PG_OK;
 }
 

@@ -71,6 +71,7 @@ struct pg_ctx {
     std::string err;
     bool loaded = false;
     int64_t n = 0, e = 0, n_logfit = 0;
+    bool logfit_owned = false;        // logfit[0, n_logfit) holds supplied values
     int32_t* birth = nullptr;
     int32_t* in_deg = nullptr;
     float* logfit = nullptr;
@@ -111,6 +112,10 @@ struct pg_ctx {
     // device snapshot of the graph state
     bool snap = false;
     int64_t snap_n = 0, snap_e = 0;
+    // year steps completed since the graph was loaded, and the last one's
+    // year and year index (kept by snapshots and checkpoints)
+    int32_t steps_done = 0, last_year = 0, last_year_index = -1;
+    int32_t snap_steps = 0, snap_last_year = 0, snap_last_year_index = -1;
     int32_t* s_birth = nullptr;
     int32_t* s_in_deg = nullptr;
     int32_t* s_fwd_off = nullptr;
