@@ -28,7 +28,6 @@ __global__ __launch_bounds__(256) void k_pa_compact(Batch bt, int64_t n_exist, i
                                                     int64_t fwd_base, int32_t* __restrict__ fwd_off,
                                                     int32_t* __restrict__ fwd_adj,
                                                     int32_t* __restrict__ src_of,
-                                                    int32_t* __restrict__ add,
                                                     int32_t* __restrict__ stage) {
     const int lane = threadIdx.x & 63;
     const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -54,7 +53,6 @@ __global__ __launch_bounds__(256) void k_pa_compact(Batch bt, int64_t n_exist, i
         fwd_adj[fwd_base + e - off0] = x;
         src_of[e] = (int32_t)id0;
         if (BATCH) stage[e] = x;
-        atomicAdd(add + x, 1);
     }
     if (lane == 0) fwd_off[id0 + 1] = (int32_t)(fwd_base + new_off[i + 1] - off0);
 }
@@ -138,18 +136,39 @@ __global__ __launch_bounds__(kRT) void k_radix_scatter(const uint32_t* __restric
     }
 }
 
-__global__ __launch_bounds__(256) void k_bwd_offsets(int64_t n_old, int64_t n_total,
-                                                     const int32_t* __restrict__ old_off,
-                                                     const int64_t* __restrict__ excl,
-                                                     const int32_t* __restrict__ add,
-                                                     int32_t* __restrict__ new_off,
-                                                     int32_t* __restrict__ in_deg) {
-    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// First position in keys[0, n) (ascending) holding a key >= v.
+__device__ __forceinline__ int64_t lower_bound_u32(const uint32_t* __restrict__ keys, int64_t n,
+                                                   uint32_t v) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// New backward offsets and in-degrees from the year's targets sorted
+// ascending (keys[0, n_keys); padding keys >= n_total allowed): the new
+// citations of v are the run of v in keys, so excl[v] = lower_bound(v) and
+// the in-degree grows by lower_bound(v+1) - lower_bound(v).  No atomics:
+// hub targets receive thousands of citations a year.
+__global__ __launch_bounds__(256) void k_bwd_offsets_sorted(int64_t n_old, int64_t n_total,
+                                                            const int32_t* __restrict__ old_off,
+                                                            const uint32_t* __restrict__ keys,
+                                                            int64_t n_keys,
+                                                            int32_t* __restrict__ new_off,
+                                                            int32_t* __restrict__ in_deg) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v > n_total) return;
     const int64_t vv = v < n_old ? v : n_old;
-    new_off[v] = (int32_t)(old_off[vv] + excl[vv]);
-    if (v < n_old) in_deg[v] += add[v];
-    else if (v < n_total) in_deg[v] = 0;
+    const int64_t p0 = lower_bound_u32(keys, n_keys, (uint32_t)vv);
+    new_off[v] = (int32_t)(old_off[vv] + p0);
+    if (v < n_old) {
+        const int64_t p1 = lower_bound_u32(keys, n_keys, (uint32_t)(v + 1));
+        in_deg[v] += (int32_t)(p1 - p0);
+    } else if (v < n_total) {
+        in_deg[v] = 0;
+    }
 }
 
 constexpr int kMergeOut = 4096;   // outputs per block (16 per thread)
@@ -257,21 +276,21 @@ __global__ __launch_bounds__(256) void k_bwd_merge(int64_t n_old, int64_t n_tota
 
 void launch_pa_compact(hipStream_t s, int64_t n_exist, int64_t n_new, int m, const int32_t* cit,
                        const int32_t* cnt, const int64_t* new_off, int64_t fwd_base,
-                       int32_t* fwd_off, int32_t* fwd_adj, int32_t* src_of, int32_t* add) {
+                       int32_t* fwd_off, int32_t* fwd_adj, int32_t* src_of) {
     if (n_new <= 0) return;
     int64_t nb = (n_new + 3) / 4;
     hipLaunchKernelGGL(k_pa_compact<false>, dim3((uint32_t)nb), dim3(256), 0, s, Batch{}, n_exist,
-                       n_new, m, cit, cnt, new_off, fwd_base, fwd_off, fwd_adj, src_of, add,
+                       n_new, m, cit, cnt, new_off, fwd_base, fwd_off, fwd_adj, src_of,
                        (int32_t*)nullptr);
 }
 
 void launch_pa_compact_batch(hipStream_t s, const Batch& b, int m, const int32_t* cit,
                              const int32_t* cnt, const int64_t* new_off, int32_t* fwd_off,
-                             int32_t* fwd_adj, int32_t* src_of, int32_t* add, int32_t* stage) {
+                             int32_t* fwd_adj, int32_t* src_of, int32_t* stage) {
     if (b.n_new <= 0) return;
     int64_t nb = (b.n_new + 3) / 4;
     hipLaunchKernelGGL(k_pa_compact<true>, dim3((uint32_t)nb), dim3(256), 0, s, b, (int64_t)0,
-                       b.n_new, m, cit, cnt, new_off, (int64_t)0, fwd_off, fwd_adj, src_of, add,
+                       b.n_new, m, cit, cnt, new_off, (int64_t)0, fwd_off, fwd_adj, src_of,
                        stage);
 }
 
@@ -322,17 +341,15 @@ void launch_radix_sort_pairs(hipStream_t s, const uint32_t* k_in, const uint32_t
     }
 }
 
-void launch_bwd_offsets(hipStream_t s, int64_t n_old, int64_t n_new, const int32_t* bwd_off_old,
-                        const int32_t* add, int32_t* bwd_off_new, int32_t* in_deg,
-                        int64_t* scratch) {
-    int64_t* excl = scratch;
-    int64_t* scan_scr = scratch + (n_old + 1);
-    launch_excl_scan_i32(s, add, n_old, excl, scan_scr);
+void launch_bwd_offsets_sorted(hipStream_t s, int64_t n_old, int64_t n_new,
+                               const int32_t* bwd_off_old, const uint32_t* sorted_keys,
+                               int64_t n_keys, int32_t* bwd_off_new, int32_t* in_deg) {
     const int64_t n_total = n_old + n_new;
     int64_t nb = (n_total + 1 + 255) / 256;
-    hipLaunchKernelGGL(k_bwd_offsets, dim3((uint32_t)nb), dim3(256), 0, s, n_old, n_total,
-                       bwd_off_old, excl, add, bwd_off_new, in_deg);
+    hipLaunchKernelGGL(k_bwd_offsets_sorted, dim3((uint32_t)nb), dim3(256), 0, s, n_old, n_total,
+                       bwd_off_old, sorted_keys, n_keys, bwd_off_new, in_deg);
 }
+
 
 void launch_bwd_merge(hipStream_t s, int64_t n_old, int64_t n_total, const int32_t* bwd_off_old,
                       const int32_t* bwd_adj_old, const int32_t* bwd_off_new,

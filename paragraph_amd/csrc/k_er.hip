@@ -18,13 +18,12 @@ namespace {
 
 template <bool kWrite>
 __device__ __forceinline__ int64_t er_paper(int64_t n_exist, int64_t i, double p, double c,
-                                            uint64_t seed, uint32_t yi, int32_t* out,
-                                            int32_t* add) {
+                                            uint64_t seed, uint32_t yi, int32_t* out) {
     if (p <= 0.0) return 0;
     const uint32_t gid = (uint32_t)(n_exist + i);
     if (p >= 1.0) {
         if (kWrite) {
-            for (int64_t t = 0; t < n_exist; ++t) { out[t] = (int32_t)t; atomicAdd(add + t, 1); }
+            for (int64_t t = 0; t < n_exist; ++t) out[t] = (int32_t)t;
         }
         return n_exist;
     }
@@ -37,10 +36,7 @@ __device__ __forceinline__ int64_t er_paper(int64_t n_exist, int64_t i, double p
         double nxt = pos + 1.0 + gap;
         if (nxt >= (double)n_exist) break;
         pos = nxt;
-        if (kWrite) {
-            out[e] = (int32_t)pos;
-            atomicAdd(add + (int64_t)pos, 1);
-        }
+        if (kWrite) out[e] = (int32_t)pos;
         ++e;
     }
     return e;
@@ -51,7 +47,7 @@ __global__ __launch_bounds__(256) void k_er_count(int64_t n_exist, int64_t n_new
                                                   int32_t* __restrict__ counts) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_new) return;
-    counts[i] = (int32_t)er_paper<false>(n_exist, i, p, c, seed, yi, nullptr, nullptr);
+    counts[i] = (int32_t)er_paper<false>(n_exist, i, p, c, seed, yi, nullptr);
 }
 
 __global__ __launch_bounds__(256) void k_er_write(int64_t n_exist, int64_t n_new, double p, double c,
@@ -59,12 +55,11 @@ __global__ __launch_bounds__(256) void k_er_write(int64_t n_exist, int64_t n_new
                                                   const int64_t* __restrict__ new_off,
                                                   int64_t fwd_base, int32_t* __restrict__ fwd_off,
                                                   int32_t* __restrict__ fwd_adj,
-                                                  int32_t* __restrict__ src_of,
-                                                  int32_t* __restrict__ add) {
+                                                  int32_t* __restrict__ src_of) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_new) return;
     const int64_t b = new_off[i];
-    const int64_t e = er_paper<true>(n_exist, i, p, c, seed, yi, fwd_adj + fwd_base + b, add);
+    const int64_t e = er_paper<true>(n_exist, i, p, c, seed, yi, fwd_adj + fwd_base + b);
     for (int64_t k = 0; k < e; ++k) src_of[b + k] = (int32_t)(n_exist + i);
     fwd_off[n_exist + i + 1] = (int32_t)(fwd_base + new_off[i + 1]);
 }
@@ -82,12 +77,12 @@ void launch_er_count(hipStream_t s, int64_t n_exist, int64_t n_new, double p, ui
 
 void launch_er_write(hipStream_t s, int64_t n_exist, int64_t n_new, double p, uint64_t seed,
                      uint32_t year_index, const int64_t* new_off, int64_t fwd_base,
-                     int32_t* fwd_off, int32_t* fwd_adj, int32_t* src_of, int32_t* add) {
+                     int32_t* fwd_off, int32_t* fwd_adj, int32_t* src_of) {
     if (n_new <= 0) return;
     const double c = det_log1p(-p);
     int64_t nb = (n_new + 255) / 256;
     hipLaunchKernelGGL(k_er_write, dim3((uint32_t)nb), dim3(256), 0, s, n_exist, n_new, p, c, seed,
-                       year_index, new_off, fwd_base, fwd_off, fwd_adj, src_of, add);
+                       year_index, new_off, fwd_base, fwd_off, fwd_adj, src_of);
 }
 
 }  // namespace pg

@@ -35,7 +35,7 @@ void release(pg_ctx* c) {
     void* ptrs[] = {c->birth, c->in_deg, c->logfit, c->fwd_off, c->fwd_adj, c->bwd_off[0],
                     c->bwd_off[1], c->bwd_adj[0], c->bwd_adj[1], c->w, c->cdf, c->scan_agg,
                     c->cit, c->kind, c->cnt,
-                    c->near, c->anext, c->new_off, c->add, c->src_of, c->k_sorted,
+                    c->near, c->anext, c->new_off, c->src_of, c->k_sorted,
                     c->v_sorted, c->k_tmp, c->v_tmp, c->scratch, c->d_sum, c->d_tmp_i32,
                     c->s_birth, c->s_in_deg, c->s_fwd_off, c->s_fwd_adj, c->s_bwd_off,
                     c->s_bwd_adj, c->S_buf, c->Q_buf, c->ns_buf, c->rk[0], c->rk[1],
@@ -205,7 +205,6 @@ int pg_ctx_create(const pg_config* cfg, pg_ctx** out) {
     CREATE_HIP(dalloc(&ctx->near, D));
     CREATE_HIP(dalloc(&ctx->anext, D));
     CREATE_HIP(dalloc(&ctx->new_off, D + 2));
-    CREATE_HIP(dalloc(&ctx->add, N));
     CREATE_HIP(dalloc(&ctx->d_sum, 4));
     for (int k = 0; k < PG_K_COUNT; ++k)
         for (int j = 0; j < 2; ++j) {
@@ -401,14 +400,13 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
             int rc = ensure_year_edges(ctx, e_year);
             if (rc) return rc;
         }
-        launch_fill_i32(s, ctx->add, 0, N);
         if (p->model == PG_MODEL_PA) {
             pbeg(PG_K_CSR_APPEND);
             launch_pa_compact(s, N, D, p->m, ctx->cit, ctx->cnt, ctx->new_off, ctx->e, ctx->fwd_off,
-                              ctx->fwd_adj, ctx->src_of, ctx->add);
+                              ctx->fwd_adj, ctx->src_of);
         } else {
             launch_er_write(s, N, D, p->er_p, seed, yi, ctx->new_off, ctx->e, ctx->fwd_off,
-                            ctx->fwd_adj, ctx->src_of, ctx->add);
+                            ctx->fwd_adj, ctx->src_of);
             pend(PG_K_ER);
             pbeg(PG_K_CSR_APPEND);
         }
@@ -417,8 +415,8 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
                                 ctx->v_sorted, ctx->k_tmp, ctx->v_tmp, e_year, key_bits_for(N),
                                 ctx->scratch);
         const int nxt = 1 - ctx->cur;
-        launch_bwd_offsets(s, N, D, ctx->bwd_off[ctx->cur], ctx->add, ctx->bwd_off[nxt],
-                           ctx->in_deg, ctx->scratch);
+        launch_bwd_offsets_sorted(s, N, D, ctx->bwd_off[ctx->cur], ctx->k_sorted, e_year,
+                                  ctx->bwd_off[nxt], ctx->in_deg);
         launch_bwd_merge(s, N, N + D, ctx->bwd_off[ctx->cur], ctx->bwd_adj[ctx->cur],
                          ctx->bwd_off[nxt], ctx->v_sorted, ctx->bwd_adj[nxt], ctx->e + e_year);
         pend(PG_K_CSR_APPEND);

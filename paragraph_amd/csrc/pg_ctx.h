@@ -89,7 +89,6 @@ struct pg_ctx {
     int32_t* near = nullptr;
     int32_t* anext = nullptr;
     int64_t* new_off = nullptr;
-    int32_t* add = nullptr;
     int64_t year_edge_cap = 0;
     int32_t* src_of = nullptr;
     uint32_t* k_sorted = nullptr;

@@ -118,9 +118,8 @@ void enqueue_year(pg_ensemble* ens, int y, hipStream_t s) {
     launch_sum_i32(s, c->anext, Dn, cnt + 1);
     launch_excl_scan_i32(s, c->cnt, Dn, c->new_off, c->scratch);
     launch_acc_i64(s, c->new_off + Dn, cnt + 4);
-    launch_fill_i32(s, c->add, 0, TOT);
     launch_pa_compact_batch(s, b, m, c->cit, c->cnt, c->new_off, c->fwd_off, c->fwd_adj,
-                            c->src_of, c->add, ens->stage);
+                            c->src_of, ens->stage);
     const int64_t UB = Dn * m;
     launch_fill_tail_u32(s, reinterpret_cast<uint32_t*>(ens->stage), c->new_off + Dn, UB,
                          (uint32_t)TOT);
@@ -128,8 +127,8 @@ void enqueue_year(pg_ensemble* ens, int y, hipStream_t s) {
                             reinterpret_cast<const uint32_t*>(c->src_of), c->k_sorted, c->v_sorted,
                             c->k_tmp, c->v_tmp, UB, key_bits(TOT + 1), c->scratch);
     const int nxt = 1 - c->cur;
-    launch_bwd_offsets(s, TOT, 0, c->bwd_off[c->cur], c->add, c->bwd_off[nxt], c->in_deg,
-                       c->scratch);
+    launch_bwd_offsets_sorted(s, TOT, 0, c->bwd_off[c->cur], c->k_sorted, UB, c->bwd_off[nxt],
+                              c->in_deg);
     launch_bwd_merge(s, TOT, TOT, c->bwd_off[c->cur], c->bwd_adj[c->cur], c->bwd_off[nxt],
                      c->v_sorted, c->bwd_adj[nxt], -1, ens->ETOT);
     c->cur = nxt;

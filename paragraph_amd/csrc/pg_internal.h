@@ -174,7 +174,7 @@ void launch_bfs_only(hipStream_t s, const CsrView& g, int64_t count, const int32
 // offsets of the per-paper counts (n_new+1).
 void launch_pa_compact(hipStream_t s, int64_t n_exist, int64_t n_new, int m, const int32_t* cit,
                        const int32_t* cnt, const int64_t* new_off, int64_t fwd_base,
-                       int32_t* fwd_off, int32_t* fwd_adj, int32_t* src_of, int32_t* add);
+                       int32_t* fwd_off, int32_t* fwd_adj, int32_t* src_of);
 
 // Stable LSD radix sort of (key,val) pairs by key (keys < 2^key_bits).
 // Result lands in (k_out, v_out).  Buffers k_tmp/v_tmp same size as n.
@@ -183,10 +183,13 @@ void launch_radix_sort_pairs(hipStream_t s, const uint32_t* k_in, const uint32_t
                              uint32_t* k_out, uint32_t* v_out, uint32_t* k_tmp, uint32_t* v_tmp,
                              int64_t n, int key_bits, int64_t* scratch);
 
-// Backward offsets and degrees after adding add[t] citations to paper t:
-// bwd_off_new[v] = bwd_off_old[min(v,n_old)] + excl(add)[v]; in_deg[v] += add[v].
-void launch_bwd_offsets(hipStream_t s, int64_t n_old, int64_t n_new, const int32_t* bwd_off_old,
-                        const int32_t* add, int32_t* bwd_off_new, int32_t* in_deg, int64_t* scratch);
+// Backward offsets and degrees after the year's citations, from their targets
+// sorted ascending (keys[0, n_keys); padding keys >= n_old + n_new allowed):
+// bwd_off_new[v] = bwd_off_old[min(v,n_old)] + #keys < v; in_deg[v] += #keys == v.
+void launch_bwd_offsets_sorted(hipStream_t s, int64_t n_old, int64_t n_new,
+                               const int32_t* bwd_off_old, const uint32_t* sorted_keys,
+                               int64_t n_keys, int32_t* bwd_off_new, int32_t* in_deg);
+
 
 // bwd_adj_new = old rows followed by the row's new citers (sorted_src,
 // grouped by target in ascending citing order).  e_new < 0: the edge total
@@ -200,7 +203,7 @@ void launch_bwd_merge(hipStream_t s, int64_t n_old, int64_t n_total, const int32
 // year's (target, citing) pairs also listed contiguously in stage/src_of).
 void launch_pa_compact_batch(hipStream_t s, const Batch& b, int m, const int32_t* cit,
                              const int32_t* cnt, const int64_t* new_off, int32_t* fwd_off,
-                             int32_t* fwd_adj, int32_t* src_of, int32_t* add, int32_t* stage);
+                             int32_t* fwd_adj, int32_t* src_of, int32_t* stage);
 void launch_birth_batch(hipStream_t s, const Batch& b, int32_t* birth);
 // keys[e] = v for e in [*from, to) (from read on the device).
 void launch_fill_tail_u32(hipStream_t s, uint32_t* keys, const int64_t* from, int64_t to,
@@ -211,7 +214,7 @@ void launch_er_count(hipStream_t s, int64_t n_exist, int64_t n_new, double p, ui
                      uint32_t year_index, int32_t* counts);
 void launch_er_write(hipStream_t s, int64_t n_exist, int64_t n_new, double p, uint64_t seed,
                      uint32_t year_index, const int64_t* new_off, int64_t fwd_base,
-                     int32_t* fwd_off, int32_t* fwd_adj, int32_t* src_of, int32_t* add);
+                     int32_t* fwd_off, int32_t* fwd_adj, int32_t* src_of);
 
 // Small helpers.
 void launch_fill_i32(hipStream_t s, int32_t* p, int32_t v, int64_t n);
