@@ -171,7 +171,7 @@ __global__ __launch_bounds__(256) void k_bwd_offsets_sorted(int64_t n_old, int64
     }
 }
 
-constexpr int kMergeOut = 4096;   // outputs per block (16 per thread)
+constexpr int kMergeOut = 2048;   // outputs per block (8 per thread)
 
 __device__ __forceinline__ int64_t row_of_pos(const int32_t* off, int64_t lo, int64_t hi, int64_t p) {
     // largest v in [lo, hi) with off[v] <= p
@@ -182,6 +182,26 @@ __device__ __forceinline__ int64_t row_of_pos(const int32_t* off, int64_t lo, in
     return lo;
 }
 
+// Rows spanned by each merge block: part[2b] = row of the block's first
+// position, part[2b+1] = row of its last (one thread per block, so the
+// binary searches run in parallel instead of at the head of every block).
+__global__ __launch_bounds__(256) void k_merge_part(int64_t n_total, const int32_t* __restrict__ new_off,
+                                                    int64_t e_new, int64_t nblk,
+                                                    int64_t* __restrict__ part) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblk) return;
+    if (e_new < 0) e_new = new_off[n_total];
+    const int64_t pb = b * kMergeOut;
+    if (pb >= e_new) {
+        part[2 * b] = part[2 * b + 1] = 0;
+        return;
+    }
+    const int64_t pe = (pb + kMergeOut < e_new ? pb + kMergeOut : e_new) - 1;
+    const int64_t r0 = row_of_pos(new_off, 0, n_total + 1, pb);
+    part[2 * b] = r0;
+    part[2 * b + 1] = row_of_pos(new_off, r0, n_total + 1, pe);
+}
+
 // Each block writes kMergeOut consecutive positions of the new backward
 // adjacency.  The new and old offsets of the rows those positions span are
 // staged in LDS; every position's row comes from a row-start table (each
@@ -189,17 +209,19 @@ __device__ __forceinline__ int64_t row_of_pos(const int32_t* off, int64_t lo, in
 // consecutive threads write consecutive positions, so reads of old rows,
 // of the sorted new citers and the writes are all coalesced.  Blocks whose
 // positions span more than kMergeOut rows (long runs of empty rows) search
-// the offsets per position instead.
+// the offsets per position instead.  part (nullable) holds the blocks' row
+// ranges from k_merge_part.
 __global__ __launch_bounds__(256) void k_bwd_merge(int64_t n_old, int64_t n_total,
                                                    const int32_t* __restrict__ old_off,
                                                    const int32_t* __restrict__ old_adj,
                                                    const int32_t* __restrict__ new_off,
                                                    const uint32_t* __restrict__ sorted_src,
-                                                   int32_t* __restrict__ out, int64_t e_new) {
+                                                   int32_t* __restrict__ out, int64_t e_new,
+                                                   const int64_t* __restrict__ part) {
     __shared__ int32_t s_new[kMergeOut + 1];
     __shared__ int32_t s_old[kMergeOut + 1];
     __shared__ int32_t s_row[kMergeOut];
-    __shared__ int32_t s_max[256];
+    __shared__ int32_t s_max[4];
     __shared__ int64_t s_rlo, s_nrows;
     if (e_new < 0) e_new = new_off[n_total];  // total only known on the device
     const int64_t pb = (int64_t)blockIdx.x * kMergeOut;
@@ -207,8 +229,14 @@ __global__ __launch_bounds__(256) void k_bwd_merge(int64_t n_old, int64_t n_tota
     const int64_t pe = (pb + kMergeOut < e_new ? pb + kMergeOut : e_new) - 1;  // last position
     const int cnt = (int)(pe - pb + 1);
     if (threadIdx.x == 0) {
-        const int64_t r0 = row_of_pos(new_off, 0, n_total + 1, pb);
-        const int64_t r1 = row_of_pos(new_off, r0, n_total + 1, pe);
+        int64_t r0, r1;
+        if (part) {
+            r0 = part[2 * blockIdx.x];
+            r1 = part[2 * blockIdx.x + 1];
+        } else {
+            r0 = row_of_pos(new_off, 0, n_total + 1, pb);
+            r1 = row_of_pos(new_off, r0, n_total + 1, pe);
+        }
         s_rlo = r0;
         s_nrows = r1 - r0 + 1;
     }
@@ -229,24 +257,32 @@ __global__ __launch_bounds__(256) void k_bwd_merge(int64_t n_old, int64_t n_tota
             if (st >= 0 && st < cnt) atomicMax(&s_row[st], (int)k);
         }
         __syncthreads();
-        // prefix maximum over the positions: 16 per thread, then across threads
+        // prefix maximum over the positions: kPer per thread, across the
+        // lanes of a wavefront, then across the four wavefronts
         constexpr int kPer = kMergeOut / 256;
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
         const int j0 = threadIdx.x * kPer;
+        int loc[kPer];
         int mx = 0;
 #pragma unroll
-        for (int q = 0; q < kPer; ++q) mx = max(mx, s_row[j0 + q]);
-        s_max[threadIdx.x] = mx;
-        __syncthreads();
-        for (int d = 1; d < 256; d <<= 1) {
-            const int v = (int)threadIdx.x >= d ? s_max[threadIdx.x - d] : 0;
-            __syncthreads();
-            s_max[threadIdx.x] = max(s_max[threadIdx.x], v);
-            __syncthreads();
+        for (int q = 0; q < kPer; ++q) {
+            loc[q] = s_row[j0 + q];
+            mx = max(mx, loc[q]);
         }
-        int run = threadIdx.x > 0 ? s_max[threadIdx.x - 1] : 0;
+        int inc = mx;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int u = __shfl_up(inc, d, 64);
+            if (lane >= d) inc = max(inc, u);
+        }
+        if (lane == 63) s_max[wid] = inc;
+        __syncthreads();
+        int run = __shfl_up(inc, 1, 64);
+        if (lane == 0) run = 0;
+        for (int w = 0; w < wid; ++w) run = max(run, s_max[w]);
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
-            run = max(run, s_row[j0 + q]);
+            run = max(run, loc[q]);
             s_row[j0 + q] = run;
         }
         __syncthreads();
@@ -354,12 +390,18 @@ void launch_bwd_offsets_sorted(hipStream_t s, int64_t n_old, int64_t n_new,
 void launch_bwd_merge(hipStream_t s, int64_t n_old, int64_t n_total, const int32_t* bwd_off_old,
                       const int32_t* bwd_adj_old, const int32_t* bwd_off_new,
                       const uint32_t* sorted_src, int32_t* bwd_adj_new, int64_t e_new,
-                      int64_t e_bound) {
+                      int64_t e_bound, int64_t* part, int64_t part_cap) {
     const int64_t e_grid = e_new >= 0 ? e_new : e_bound;
     if (e_grid <= 0) return;
     int64_t nb = (e_grid + kMergeOut - 1) / kMergeOut;
+    if (part && 2 * nb <= part_cap)
+        hipLaunchKernelGGL(k_merge_part, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, s, n_total,
+                           bwd_off_new, e_new, nb, part);
+    else
+        part = nullptr;
     hipLaunchKernelGGL(k_bwd_merge, dim3((uint32_t)nb), dim3(256), 0, s, n_old, n_total, bwd_off_old,
-                       bwd_adj_old, bwd_off_new, sorted_src, bwd_adj_new, e_new);
+                       bwd_adj_old, bwd_off_new, sorted_src, bwd_adj_new, e_new,
+                       (const int64_t*)part);
 }
 
 }  // namespace pg

@@ -417,8 +417,10 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
         const int nxt = 1 - ctx->cur;
         launch_bwd_offsets_sorted(s, N, D, ctx->bwd_off[ctx->cur], ctx->k_sorted, e_year,
                                   ctx->bwd_off[nxt], ctx->in_deg);
+        // the scratch holds at least max_papers + 2 int64 (its scan part)
         launch_bwd_merge(s, N, N + D, ctx->bwd_off[ctx->cur], ctx->bwd_adj[ctx->cur],
-                         ctx->bwd_off[nxt], ctx->v_sorted, ctx->bwd_adj[nxt], ctx->e + e_year);
+                         ctx->bwd_off[nxt], ctx->v_sorted, ctx->bwd_adj[nxt], ctx->e + e_year, 0,
+                         ctx->scratch, ctx->cfg.max_papers + 2);
         pend(PG_K_CSR_APPEND);
         PG_HIP(hipGetLastError());
         ctx->cur = nxt;

@@ -130,7 +130,7 @@ void enqueue_year(pg_ensemble* ens, int y, hipStream_t s) {
     launch_bwd_offsets_sorted(s, TOT, 0, c->bwd_off[c->cur], c->k_sorted, UB, c->bwd_off[nxt],
                               c->in_deg);
     launch_bwd_merge(s, TOT, TOT, c->bwd_off[c->cur], c->bwd_adj[c->cur], c->bwd_off[nxt],
-                     c->v_sorted, c->bwd_adj[nxt], -1, ens->ETOT);
+                     c->v_sorted, c->bwd_adj[nxt], -1, ens->ETOT, c->scratch, c->cfg.max_papers + 2);
     c->cur = nxt;
 }
 

@@ -190,14 +190,14 @@ void launch_bwd_offsets_sorted(hipStream_t s, int64_t n_old, int64_t n_new,
                                const int32_t* bwd_off_old, const uint32_t* sorted_keys,
                                int64_t n_keys, int32_t* bwd_off_new, int32_t* in_deg);
 
-
 // bwd_adj_new = old rows followed by the row's new citers (sorted_src,
 // grouped by target in ascending citing order).  e_new < 0: the edge total
 // is read on the device (bwd_off_new[n_total]) and e_bound sizes the grid.
+// part (nullable, part_cap int64): scratch for the blocks' row ranges.
 void launch_bwd_merge(hipStream_t s, int64_t n_old, int64_t n_total, const int32_t* bwd_off_old,
                       const int32_t* bwd_adj_old, const int32_t* bwd_off_new,
                       const uint32_t* sorted_src, int32_t* bwd_adj_new, int64_t e_new,
-                      int64_t e_bound = 0);
+                      int64_t e_bound, int64_t* part, int64_t part_cap);
 
 // Batched forward append (rows at the end of each simulation's region; the
 // year's (target, citing) pairs also listed contiguously in stage/src_of).
