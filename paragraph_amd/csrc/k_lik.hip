@@ -70,17 +70,9 @@ __device__ inline LikScal lik_scal(const LikArgs& a) {
     return s;
 }
 
-// The simulator's weight arithmetic (pa_weight), keeping L and x.
-__device__ inline float lik_weight(int32_t k, int32_t age, float phi, const LikScal& s, float& L,
-                                   float& x) {
-    L = det_logf((float)(k + 1));
-    const float t1 = __builtin_fmaf(-(float)age, s.inv_tau_f, phi);
-    x = __builtin_fmaf(s.alpha_f, L, t1);
-    return det_expf(x);
-}
-
-// The same from the target's L = log(k+1) and age (as float, exact below
-// 2^24), precomputed per slot: identical arithmetic.
+// The simulator's weight arithmetic (pa_weight) from L = log(k+1), which is
+// precomputed per paper-year and per slot with the simulator's det_logf, and
+// the age (as float, exact below 2^24); keeps x = log w.
 __device__ inline float lik_weight_la(float L, float age, float phi, const LikScal& s, float& x) {
     const float t1 = __builtin_fmaf(-age, s.inv_tau_f, phi);
     x = __builtin_fmaf(s.alpha_f, L, t1);
@@ -166,10 +158,10 @@ __global__ __launch_bounds__(kLikBT) void k_lik_papers(LikSeq q, LikArgs a,
         for (int r = 0; r < kLikPPT; ++r) {
             const int64_t j = j0 + r * kLikBT + threadIdx.x;
             if (j < Ny) {
-                const int32_t k = q.deg[doff + j];
+                const float L = q.Lpy[doff + j];
                 const int32_t age = year - b[r];
-                float L, x;
-                const double wv = (double)lik_weight(k, age, phi[r], s, L, x);
+                float x;
+                const double wv = (double)lik_weight_la(L, (float)age, phi[r], s, x);
                 if (BWD) {
                     const double aw = Ay * wv;
                     g[r] += aw;
@@ -595,6 +587,17 @@ __global__ __launch_bounds__(kLikBT) void k_lik_slot_la(LikSeq q, float2* __rest
 }
 
 }  // namespace
+
+__global__ __launch_bounds__(kLikBT) void k_lik_deg_log(const int32_t* __restrict__ deg, int64_t n,
+                                                        float* __restrict__ Lpy) {
+    const int64_t i = (int64_t)blockIdx.x * kLikBT + threadIdx.x;
+    if (i < n) Lpy[i] = det_logf((float)(deg[i] + 1));
+}
+
+void launch_lik_deg_log(hipStream_t st, const int32_t* deg, int64_t n, float* Lpy) {
+    if (n <= 0) return;
+    k_lik_deg_log<<<(unsigned)((n + kLikBT - 1) / kLikBT), kLikBT, 0, st>>>(deg, n, Lpy);
+}
 
 void launch_lik_slot_la(hipStream_t st, const LikSeq& q, float2* la_t) {
     const int64_t T = q.n_cite * q.m;

@@ -57,6 +57,7 @@ struct pg_seq_store {
     int32_t* cit_t = nullptr;        // [new * m] targets, slot-major
     int8_t* kind_t = nullptr;        // [new * m] kinds, slot-major
     float2* la_t = nullptr;          // [new * m] {log(k+1), age} of each slot's target, slot-major
+    float* Lpy = nullptr;            // [paper_years] log(k+1) of every in-degree snapshot
     int32_t* seg_off = nullptr;      // [cap_papers + 1]
     double* share = nullptr;         // [filled] per-slot dL/dphi shares, by-target order
     double* evt = nullptr;           // [cap_papers]

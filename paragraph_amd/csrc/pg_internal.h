@@ -252,6 +252,7 @@ struct LikSeq {
     const int8_t* kind_t;
     const uint32_t* pos_t;    // by-target sorted position of each slot
     const float2* la_t;       // {log(k+1), age} of each slot's target (fixed by the sequence)
+    const float* Lpy;         // log(k+1) of every paper-year (fixed by the sequence)
 };
 
 struct LikArgs {
@@ -277,6 +278,8 @@ struct LikCiteShape {
 LikCiteShape lik_cite_shape(int m, int lanes = 0);
 // la_t[s] for every slot s of the slot-major layout (zero for empty slots).
 void launch_lik_slot_la(hipStream_t st, const LikSeq& q, float2* la_t);
+// Lpy[i] = log(deg[i] + 1) for the n paper-years.
+void launch_lik_deg_log(hipStream_t st, const int32_t* deg, int64_t n, float* Lpy);
 // Whether `lanes` lanes per citing paper can hold m slots (lanes 1,2,4,8).
 bool lik_lanes_ok(int m, int lanes);
 int64_t lik_paper_blocks(int64_t n_papers);

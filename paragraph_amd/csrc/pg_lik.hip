@@ -32,11 +32,12 @@ void seq_free_tables(pg_seq_store& q) {
     free_ptr(q.part); free_ptr(q.year_sums); free_ptr(q.d_scalars);
     free_ptr(q.skey); free_ptr(q.pos_t); free_ptr(q.cit_t); free_ptr(q.kind_t);
     free_ptr(q.seg_off); free_ptr(q.share); free_ptr(q.evt); free_ptr(q.carry);
-    free_ptr(q.la_t);
+    free_ptr(q.la_t); free_ptr(q.Lpy);
     q.skey = q.pos_t = nullptr;
     q.cit_t = q.seg_off = nullptr;
     q.kind_t = nullptr;
     q.la_t = nullptr;
+    q.Lpy = nullptr;
     q.share = q.evt = q.carry = nullptr;
     q.carry_cap = 0;
     q.d_n_exist = q.d_n_new = q.d_deg_off = q.d_new_off = nullptr;
@@ -114,11 +115,12 @@ int seq_prepare(pg_ctx* ctx) {
     const int64_t T = q.new_total * q.m;
     free_ptr(q.skey); free_ptr(q.pos_t); free_ptr(q.cit_t); free_ptr(q.kind_t);
     free_ptr(q.seg_off); free_ptr(q.share); free_ptr(q.evt); free_ptr(q.carry);
-    free_ptr(q.la_t);
+    free_ptr(q.la_t); free_ptr(q.Lpy);
     q.skey = q.pos_t = nullptr;
     q.cit_t = q.seg_off = nullptr;
     q.kind_t = nullptr;
     q.la_t = nullptr;
+    q.Lpy = nullptr;
     q.share = q.evt = q.carry = nullptr;
     const int64_t nchunk = lik_seg_chunks(q.filled);
     uint32_t *keys = nullptr, *vals = nullptr, *k_tmp = nullptr, *v_tmp = nullptr,
@@ -129,6 +131,7 @@ int seq_prepare(pg_ctx* ctx) {
     if (r == hipSuccess) r = dalloc(&q.cit_t, T);
     if (r == hipSuccess) r = dalloc(&q.kind_t, T);
     if (r == hipSuccess) r = dalloc(&q.la_t, T);
+    if (r == hipSuccess) r = dalloc(&q.Lpy, std::max<int64_t>(q.paper_years, 1));
     if (r == hipSuccess) r = dalloc(&q.seg_off, q.n_papers + 1);
     if (r == hipSuccess) r = dalloc(&q.share, q.filled);
     if (r == hipSuccess) r = dalloc(&perm, T);
@@ -148,6 +151,7 @@ int seq_prepare(pg_ctx* ctx) {
         r = hipGetLastError();
         if (r == hipSuccess) {
             launch_lik_slot_la(ctx->stream, seq_view(q), q.la_t);
+            launch_lik_deg_log(ctx->stream, q.deg, q.paper_years, q.Lpy);
             r = hipGetLastError();
         }
         if (r == hipSuccess) r = hipStreamSynchronize(ctx->stream);
@@ -183,6 +187,7 @@ LikSeq seq_view(const pg_seq_store& q) {
     v.kind_t = q.kind_t;
     v.pos_t = q.pos_t;
     v.la_t = q.la_t;
+    v.Lpy = q.Lpy;
     return v;
 }
 
