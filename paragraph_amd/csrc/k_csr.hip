@@ -136,38 +136,51 @@ __global__ __launch_bounds__(kRT) void k_radix_scatter(const uint32_t* __restric
     }
 }
 
-// First position in keys[0, n) (ascending) holding a key >= v.
-__device__ __forceinline__ int64_t lower_bound_u32(const uint32_t* __restrict__ keys, int64_t n,
-                                                   uint32_t v) {
-    int64_t lo = 0, hi = n;
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (keys[mid] < v) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-}
-
 // New backward offsets and in-degrees from the year's targets sorted
 // ascending (keys[0, n_keys); padding keys >= n_total allowed): the new
 // citations of v are the run of v in keys, so excl[v] = lower_bound(v) and
 // the in-degree grows by lower_bound(v+1) - lower_bound(v).  No atomics:
-// hub targets receive thousands of citations a year.
+// hub targets receive thousands of citations a year.  Each thread takes
+// kOffPer consecutive papers: one binary search for the first, then each
+// paper's run of keys is skipped by galloping.
+constexpr int kOffPer = 8;
 __global__ __launch_bounds__(256) void k_bwd_offsets_sorted(int64_t n_old, int64_t n_total,
                                                             const int32_t* __restrict__ old_off,
                                                             const uint32_t* __restrict__ keys,
                                                             int64_t n_keys,
                                                             int32_t* __restrict__ new_off,
                                                             int32_t* __restrict__ in_deg) {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v > n_total) return;
-    const int64_t vv = v < n_old ? v : n_old;
-    const int64_t p0 = lower_bound_u32(keys, n_keys, (uint32_t)vv);
-    new_off[v] = (int32_t)(old_off[vv] + p0);
-    if (v < n_old) {
-        const int64_t p1 = lower_bound_u32(keys, n_keys, (uint32_t)(v + 1));
-        in_deg[v] += (int32_t)(p1 - p0);
-    } else if (v < n_total) {
-        in_deg[v] = 0;
+    const int64_t v0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * kOffPer;
+    if (v0 > n_total) return;
+    int64_t lo = 0, hi = n_keys;  // first key >= v0 (keys < 2^32 - 1: v0 fits)
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < (uint32_t)v0) lo = mid + 1; else hi = mid;
+    }
+    int64_t p = lo;
+    const int64_t v1 = v0 + kOffPer <= n_total ? v0 + kOffPer : n_total + 1;
+    for (int64_t v = v0; v < v1; ++v) {
+        const int64_t vv = v < n_old ? v : n_old;
+        // p = #keys < v (keys below v0 were skipped by the search)
+        new_off[v] = (int32_t)(old_off[vv] + p);
+        // end of v's run by galloping (a hub's run is thousands of keys)
+        int64_t q = p;
+        if (q < n_keys && keys[q] == (uint32_t)v) {
+            int64_t step = 1, last = q;  // keys[last] == v
+            while (last + step < n_keys && keys[last + step] == (uint32_t)v) {
+                last += step;
+                step <<= 1;
+            }
+            int64_t l = last + 1, h = last + step < n_keys ? last + step : n_keys;
+            while (l < h) {  // first index in [l, h) with key != v (> v)
+                const int64_t mid = (l + h) >> 1;
+                if (keys[mid] == (uint32_t)v) l = mid + 1; else h = mid;
+            }
+            q = l;
+        }
+        if (v < n_old) in_deg[v] += (int32_t)(q - p);
+        else if (v < n_total) in_deg[v] = 0;
+        p = q;
     }
 }
 
@@ -381,7 +394,8 @@ void launch_bwd_offsets_sorted(hipStream_t s, int64_t n_old, int64_t n_new,
                                const int32_t* bwd_off_old, const uint32_t* sorted_keys,
                                int64_t n_keys, int32_t* bwd_off_new, int32_t* in_deg) {
     const int64_t n_total = n_old + n_new;
-    int64_t nb = (n_total + 1 + 255) / 256;
+    const int64_t nthr = (n_total + 1 + kOffPer - 1) / kOffPer;
+    int64_t nb = (nthr + 255) / 256;
     hipLaunchKernelGGL(k_bwd_offsets_sorted, dim3((uint32_t)nb), dim3(256), 0, s, n_old, n_total,
                        bwd_off_old, sorted_keys, n_keys, bwd_off_new, in_deg);
 }
