@@ -1216,7 +1216,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // weights per lane, sums them locally and scans the lane totals.
     int32_t* S = S_buf + u * cap;
     uint64_t* Q = Q_buf + u * cap;
-    constexpr int kWb = 4;
+    constexpr int kWb = 16;
     uint32_t mb = 0;
     for (int s0 = 0; s0 < ns; s0 += 64 * kWb) {
         int32_t xs[kWb];
