@@ -1271,35 +1271,61 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 }
 
 // ---- root deduplication ----
+//
+// A paper's root is its first PA pick (none without picks).  Each paper sets
+// its root's bit in a bitmap over the paper ids (zero between years); the
+// distinct roots are numbered in ascending id order by a prefix sum of the
+// words' popcounts (so consecutive blocks of the BFS take nearby roots), and
+// every paper reads its root's number from the bitmap.  A few passes over
+// the year's papers and the N/32 words instead of a sort of the papers.
 
-__global__ void k_root_keys(int64_t n_exist, int64_t n_new, int m, const int32_t* __restrict__ cit,
-                            const int32_t* __restrict__ cnt, uint32_t* __restrict__ keys,
-                            uint32_t* __restrict__ vals) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ int32_t paper_root(int64_t n_exist, int64_t i, int m,
+                                              const int32_t* cit, const int32_t* cnt) {
+    if (cnt[i] <= 0) return -1;
+    const int32_t r = cit[i * m];
+    return (r >= 0 && r < n_exist) ? r : -1;
+}
+
+__global__ void k_root_mark(int64_t n_exist, int64_t n_new, int m, const int32_t* __restrict__ cit,
+                            const int32_t* __restrict__ cnt, uint32_t* __restrict__ bits) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_new) return;
-    keys[i] = cnt[i] > 0 ? (uint32_t)cit[i * m] : (uint32_t)n_exist;
-    vals[i] = (uint32_t)i;
+    const int32_t r = paper_root(n_exist, i, m, cit, cnt);
+    if (r >= 0) atomicOr(&bits[r >> 5], 1u << (r & 31));
 }
 
-__global__ void k_root_flags(int64_t n_exist, int64_t n, const uint32_t* __restrict__ keys,
-                             int32_t* __restrict__ flags) {
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const uint32_t k = keys[j];
-    flags[j] = (k < (uint32_t)n_exist && (j == 0 || keys[j - 1] != k)) ? 1 : 0;
+__global__ void k_root_popc(int64_t nw, const uint32_t* __restrict__ bits,
+                            int32_t* __restrict__ wcount) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < nw) wcount[w] = __popc(bits[w]);
 }
 
-__global__ void k_root_assign(int64_t n_exist, int64_t n, const uint32_t* __restrict__ keys,
-                              const uint32_t* __restrict__ vals, const int32_t* __restrict__ flags,
-                              const int64_t* __restrict__ excl, int32_t* __restrict__ root_slot,
+__global__ void k_root_number(int64_t n_exist, int64_t n_new, int m, const int32_t* __restrict__ cit,
+                              const int32_t* __restrict__ cnt, const uint32_t* __restrict__ bits,
+                              const int64_t* __restrict__ wprefix, int64_t nw,
+                              int64_t* __restrict__ n_unique, int32_t* __restrict__ root_slot,
                               int32_t* __restrict__ uroot) {
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const uint32_t k = keys[j];
-    if (k >= (uint32_t)n_exist) { root_slot[vals[j]] = -1; return; }
-    const int64_t uid = excl[j] + flags[j] - 1;
-    root_slot[vals[j]] = (int32_t)uid;
-    if (flags[j]) uroot[uid] = (int32_t)k;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *n_unique = wprefix[nw];
+    if (i >= n_new) return;
+    const int32_t r = paper_root(n_exist, i, m, cit, cnt);
+    if (r < 0) {
+        root_slot[i] = -1;
+        return;
+    }
+    const uint32_t below = bits[r >> 5] & ((1u << (r & 31)) - 1u);
+    const int32_t u = (int32_t)(wprefix[r >> 5] + __popc(below));
+    root_slot[i] = u;
+    uroot[u] = r;  // every paper of the root writes the same value
+}
+
+__global__ void k_root_release(int64_t n_exist, int64_t n_new, int m,
+                               const int32_t* __restrict__ cit, const int32_t* __restrict__ cnt,
+                               uint32_t* __restrict__ bits) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_new) return;
+    const int32_t r = paper_root(n_exist, i, m, cit, cnt);
+    if (r >= 0) bits[r >> 5] = 0u;
 }
 
 // ---- per-paper neighbourhood draws and refill ----
@@ -1485,21 +1511,21 @@ __global__ __launch_bounds__(256) void k_nb_draw(Batch bt, int64_t n_exist, int6
 }  // namespace
 
 void launch_root_dedup(hipStream_t s, int64_t n_exist, int64_t n_new, int m, const int32_t* cit,
-                       const int32_t* cnt, uint32_t* keys, uint32_t* vals, uint32_t* k_sorted,
-                       uint32_t* v_sorted, uint32_t* k_tmp, uint32_t* v_tmp, int32_t* flags,
-                       int64_t* excl, int32_t* root_slot, int32_t* uroot, int key_bits,
-                       int64_t* scratch) {
-    if (n_new <= 0) return;
-    const int64_t nb = (n_new + 255) / 256;
-    hipLaunchKernelGGL(k_root_keys, dim3((uint32_t)nb), dim3(256), 0, s, n_exist, n_new, m, cit, cnt,
-                       keys, vals);
-    launch_radix_sort_pairs(s, keys, vals, k_sorted, v_sorted, k_tmp, v_tmp, n_new, key_bits,
-                            scratch);
-    hipLaunchKernelGGL(k_root_flags, dim3((uint32_t)nb), dim3(256), 0, s, n_exist, n_new, k_sorted,
-                       flags);
-    launch_excl_scan_i32(s, flags, n_new, excl, scratch);
-    hipLaunchKernelGGL(k_root_assign, dim3((uint32_t)nb), dim3(256), 0, s, n_exist, n_new, k_sorted,
-                       v_sorted, flags, excl, root_slot, uroot);
+                       const int32_t* cnt, uint32_t* root_bits, int32_t* wcount, int64_t* wprefix,
+                       int64_t* scratch, int64_t* n_unique, int32_t* root_slot, int32_t* uroot) {
+    const int64_t nw = (n_exist + 31) / 32;
+    const int64_t nb = (std::max<int64_t>(n_new, 1) + 255) / 256;
+    if (n_new > 0)
+        hipLaunchKernelGGL(k_root_mark, dim3((uint32_t)nb), dim3(256), 0, s, n_exist, n_new, m, cit,
+                           cnt, root_bits);
+    hipLaunchKernelGGL(k_root_popc, dim3((uint32_t)((nw + 255) / 256)), dim3(256), 0, s, nw,
+                       root_bits, wcount);
+    launch_excl_scan_i32(s, wcount, nw, wprefix, scratch);
+    hipLaunchKernelGGL(k_root_number, dim3((uint32_t)nb), dim3(256), 0, s, n_exist, n_new, m, cit,
+                       cnt, root_bits, wprefix, nw, n_unique, root_slot, uroot);
+    if (n_new > 0)
+        hipLaunchKernelGGL(k_root_release, dim3((uint32_t)nb), dim3(256), 0, s, n_exist, n_new, m,
+                           cit, cnt, root_bits);
 }
 
 void launch_bfs_root(hipStream_t s, const CsrView& g, const float* w, const int32_t* uroot,

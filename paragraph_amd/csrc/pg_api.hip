@@ -38,8 +38,8 @@ void release(pg_ctx* c) {
                     c->near, c->anext, c->new_off, c->src_of, c->k_sorted,
                     c->v_sorted, c->k_tmp, c->v_tmp, c->scratch, c->d_sum, c->d_tmp_i32,
                     c->s_birth, c->s_in_deg, c->s_fwd_off, c->s_fwd_adj, c->s_bwd_off,
-                    c->s_bwd_adj, c->S_buf, c->Q_buf, c->ns_buf, c->rk[0], c->rk[1],
-                    c->rk[2], c->rk[3], c->rk[4], c->rk[5], c->r_flags, c->r_excl,
+                    c->s_bwd_adj, c->S_buf, c->Q_buf, c->ns_buf, c->root_bits, c->r_wcount,
+                    c->r_wprefix, c->r_excl,
                     c->root_slot, c->uroot, c->big_list, c->stage.buf, c->stage.busy};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -118,8 +118,11 @@ int ensure_nb_buffers(pg_ctx* ctx) {
     hipError_t r = dalloc(&ctx->S_buf, D * cap);
     if (r == hipSuccess) r = dalloc(&ctx->Q_buf, D * cap);
     if (r == hipSuccess) r = dalloc(&ctx->ns_buf, D);
-    for (int k = 0; k < 6 && r == hipSuccess; ++k) r = dalloc(&ctx->rk[k], D);
-    if (r == hipSuccess) r = dalloc(&ctx->r_flags, D);
+    const int64_t nw = ctx->cfg.max_papers / 32 + 1;
+    if (r == hipSuccess) r = dalloc(&ctx->root_bits, nw);
+    if (r == hipSuccess) r = hipMemset(ctx->root_bits, 0, (size_t)nw * sizeof(uint32_t));
+    if (r == hipSuccess) r = dalloc(&ctx->r_wcount, nw);
+    if (r == hipSuccess) r = dalloc(&ctx->r_wprefix, nw + 1);
     if (r == hipSuccess) r = dalloc(&ctx->r_excl, D + 1);
     if (r == hipSuccess) r = dalloc(&ctx->root_slot, D);
     if (r == hipSuccess) r = dalloc(&ctx->uroot, D);
@@ -360,9 +363,9 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
                     if (rc) return rc;
                 }
                 pbeg(PG_K_NB_DRAW);
-                launch_root_dedup(s, N, D, m, ctx->cit, ctx->cnt, ctx->rk[0], ctx->rk[1], ctx->rk[2],
-                                  ctx->rk[3], ctx->rk[4], ctx->rk[5], ctx->r_flags, ctx->r_excl,
-                                  ctx->root_slot, ctx->uroot, key_bits_for(N + 1), ctx->scratch);
+                launch_root_dedup(s, N, D, m, ctx->cit, ctx->cnt, ctx->root_bits, ctx->r_wcount,
+                                  ctx->r_wprefix, ctx->scratch, ctx->r_excl + D, ctx->root_slot,
+                                  ctx->uroot);
                 pend(PG_K_NB_DRAW);
                 int32_t* big_count = reinterpret_cast<int32_t*>(sums + 3);
                 pbeg(PG_K_BFS);

@@ -126,9 +126,10 @@ struct pg_ctx {
     int32_t* S_buf = nullptr;
     uint64_t* Q_buf = nullptr;
     int32_t* ns_buf = nullptr;
-    uint32_t* rk[6] = {};          // root keys/vals: in, sorted, tmp
-    int32_t* r_flags = nullptr;
-    int64_t* r_excl = nullptr;
+    uint32_t* root_bits = nullptr; // [max_papers/32] roots of the year, zero between years
+    int32_t* r_wcount = nullptr;   // [max_papers/32] popcounts of root_bits
+    int64_t* r_wprefix = nullptr;  // [max_papers/32 + 1] their exclusive prefix
+    int64_t* r_excl = nullptr;     // [max_new + 1]: [max_new] = distinct roots
     int32_t* root_slot = nullptr;
     int32_t* uroot = nullptr;
     int32_t* big_list = nullptr;

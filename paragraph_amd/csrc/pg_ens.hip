@@ -101,9 +101,8 @@ void enqueue_year(pg_ensemble* ens, int y, hipStream_t s) {
     bool any_nb = false;
     for (int si = 0; si < ens->cfg.n_sims; ++si) any_nb |= ens->sims[si].n_nb > 0;
     if (any_nb) {
-        launch_root_dedup(s, TOT, Dn, m, c->cit, c->cnt, c->rk[0], c->rk[1], c->rk[2], c->rk[3],
-                          c->rk[4], c->rk[5], c->r_flags, c->r_excl, c->root_slot, c->uroot,
-                          key_bits(TOT + 1), c->scratch);
+        launch_root_dedup(s, TOT, Dn, m, c->cit, c->cnt, c->root_bits, c->r_wcount, c->r_wprefix,
+                          c->scratch, c->r_excl + Dn, c->root_slot, c->uroot);
         int32_t* big_count = reinterpret_cast<int32_t*>(cnt + 3);
         launch_fill_i32(s, big_count, 0, 1);
         const CsrView g{c->fwd_off, c->fwd_adj, c->bwd_off[c->cur], c->bwd_adj[c->cur], TOT};

@@ -135,14 +135,15 @@ constexpr int64_t kBfsStageDefaultMax = 1 << 16;
 // 1 never staged, 2 staged whenever the list fits a slot).
 BfsStage bfs_stage_view(const BfsStage& pool, int mode);
 
-// PA phases 2-3.  Roots (first PA pick of each paper) are sorted and
-// deduplicated: root_slot[i] = distinct-root index of paper i (-1 if none),
-// uroot[u] = root id, n_unique = excl[n_new].
+// PA phases 2-3.  Roots (first PA pick of each paper) are deduplicated and
+// numbered in ascending id order: root_slot[i] = distinct-root index u of
+// paper i (-1 if none), uroot[u] = root id, *n_unique = number of distinct
+// roots (on the device).  root_bits (ceil(n_exist/32) words) must be zero on
+// entry and is again on exit; wcount/wprefix: ceil(n_exist/32) int32 and
+// that + 1 int64; scratch: scan scratch for that many.
 void launch_root_dedup(hipStream_t s, int64_t n_exist, int64_t n_new, int m, const int32_t* cit,
-                       const int32_t* cnt, uint32_t* keys, uint32_t* vals, uint32_t* k_sorted,
-                       uint32_t* v_sorted, uint32_t* k_tmp, uint32_t* v_tmp, int32_t* flags,
-                       int64_t* excl, int32_t* root_slot, int32_t* uroot, int key_bits,
-                       int64_t* scratch);
+                       const int32_t* cnt, uint32_t* root_bits, int32_t* wcount, int64_t* wprefix,
+                       int64_t* scratch, int64_t* n_unique, int32_t* root_slot, int32_t* uroot);
 // 2-hop neighbourhood S (S_buf[u*cap...], ns_buf[u]) and its fixed-point
 // CDF (Q_buf[u*cap...]) of every distinct root: a wavefront per small root,
 // a workgroup per large one (big_list/big_count: scratch, *big_count = 0 on
