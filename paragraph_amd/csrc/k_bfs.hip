@@ -870,7 +870,7 @@ __device__ __forceinline__ uint64_t nb_fixed(float w, double scale) {
 }
 
 // k_bfs_root: S and its fixed-point CDF for each distinct root that the
-// wavefront path (k_bfs_small) handed over in big_list.
+// wavefront paths (k_bfs_small, k_bfs_mid) handed over in big_list.
 // Stand-alone mode (only_roots non-null): block i builds S for
 // only_roots[i] into S_buf + i*cap, its size into only_len[i] and, with
 // ticks, the wall-clock ticks to the end of the 1-hop level, to the end, and
@@ -958,20 +958,24 @@ __global__ __launch_bounds__(kBT, 4) void k_bfs_root(CsrView g, const float* __r
 }
 
 
-// ---- wavefront path for small neighbourhoods ----
+// ---- wavefront paths for small and medium neighbourhoods ----
 //
-// One 64-lane wavefront per distinct root whose 1-hop rows hold at most
-// kSmallF1 entries and whose 2-hop candidate list (F1c plus the rows of
-// F1c) at most kSmallWin entries; ~10 KB of LDS, so 16 roots share a CU.
-// Larger roots are appended to big_list for k_bfs_root.  Same result as
-// the workgroup path: sort, drop duplicates (ballot/popcount compaction),
+// One 64-lane wavefront per distinct root whose 1-hop rows hold at most F1
+// entries and whose 2-hop candidate list (F1c plus the rows of F1c) at most
+// WIN entries.  The small tier (512, 2048; 10 KB of LDS, 16 roots per CU)
+// takes every root and hands larger ones to the medium tier (1024, 4096;
+// 20 KB, 8 per CU), which hands the rest to k_bfs_root.  Same result as the
+// workgroup path: sort, drop duplicates (ballot/popcount compaction),
 // filter, keep the cap smallest.
 constexpr int kSmallF1 = 512;
 constexpr int kSmallWin = 2048;
+constexpr int kMidF1 = 1024;
+constexpr int kMidWin = 4096;
 
-struct SmallLds {
-    int32_t f1[kSmallF1];
-    int32_t win[kSmallWin];
+template <int F1, int WIN>
+struct WaveLds {
+    int32_t f1[F1];
+    int32_t win[WIN];
 };
 
 // Value of lane (lane ^ J) without the LDS crossbar: DPP quad permutes and
@@ -1000,7 +1004,7 @@ __device__ __forceinline__ int32_t xor_lane(int32_t v) {
     }
 }
 
-// Bitonic sort of s[0,P) (P a power of two <= kSmallWin) by one wavefront,
+// Bitonic sort of s[0,P) (P a power of two <= 4096) by one wavefront,
 // in registers: element i lives in register e = i / 64 of lane i % 64 (so
 // the loads and stores are conflict-free and the result lands back in
 // natural order).  Compare distances below 64 are lane exchanges (DPP and
@@ -1052,7 +1056,9 @@ __device__ __forceinline__ void wave_sort_reg(int32_t* s) {
     for (int e = 0; e < E; ++e) s[e * 64 + lane] = v[e];
 }
 
-// s[0,P), P a power of two: padded to 64 if smaller (with INT_MAX), sorted.
+// s[0,P), P a power of two <= WIN: padded to 64 if smaller (with INT_MAX),
+// sorted.
+template <int WIN>
 __device__ __forceinline__ void wave_bitonic(int32_t* s, int P) {
     const int lane = threadIdx.x;
     if (P < 64) {
@@ -1066,7 +1072,10 @@ __device__ __forceinline__ void wave_bitonic(int32_t* s, int P) {
         case 256: wave_sort_reg<4>(s); break;
         case 512: wave_sort_reg<8>(s); break;
         case 1024: wave_sort_reg<16>(s); break;
-        default: wave_sort_reg<32>(s); break;
+        case 2048: wave_sort_reg<32>(s); break;
+        default:
+            if constexpr (WIN >= 4096) wave_sort_reg<64>(s);
+            break;
     }
     __syncthreads();
 }
@@ -1096,25 +1105,36 @@ __device__ int wave_unique_compact(int32_t* s, int n, int limit, Keep keep) {
     return out < limit ? out : limit;
 }
 
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_bfs_small(CsrView g, const float* __restrict__ w,
-                                                  const int32_t* __restrict__ uroot,
-                                                  const int64_t* __restrict__ n_unique_ptr, int cap,
-                                                  int32_t* __restrict__ S_buf,
-                                                  uint64_t* __restrict__ Q_buf,
-                                                  int32_t* __restrict__ ns_buf,
-                                                  int32_t* __restrict__ big_list,
-                                                  int32_t* __restrict__ big_count,
-                                                  unsigned long long* __restrict__ bytes_out) {
-    __shared__ SmallLds Ls;
-    const int64_t u = blockIdx.x;
-    if (u >= *n_unique_ptr) return;
+// The body of both tiers.  FROM_LIST: block i takes root u = in_list[i]
+// (i < *in_count) instead of u = i (< *n_unique_ptr).  Roots over the
+// tier's limits are appended to out_list.
+template <int F1, int WIN, bool FROM_LIST>
+__device__ __forceinline__ void bfs_wave(CsrView g, const float* __restrict__ w,
+                                         const int32_t* __restrict__ uroot,
+                                         const int64_t* __restrict__ n_unique_ptr,
+                                         const int32_t* __restrict__ in_list,
+                                         const int32_t* __restrict__ in_count, int cap,
+                                         int32_t* __restrict__ S_buf, uint64_t* __restrict__ Q_buf,
+                                         int32_t* __restrict__ ns_buf,
+                                         int32_t* __restrict__ out_list,
+                                         int32_t* __restrict__ out_count,
+                                         unsigned long long* __restrict__ bytes_out) {
+    __shared__ WaveLds<F1, WIN> Ls;
+    int64_t u;
+    if (FROM_LIST) {
+        if ((int)blockIdx.x >= *in_count) return;
+        u = in_list[blockIdx.x];
+    } else {
+        u = blockIdx.x;
+        if (u >= *n_unique_ptr) return;
+    }
     const int lane = threadIdx.x;
     const int32_t root = uroot[u];
     const int32_t fb = g.fwd_off[root], fe = g.fwd_off[root + 1];
     const int32_t bb = g.bwd_off[root], be = g.bwd_off[root + 1];
     const int df = fe - fb, d1 = df + (be - bb);
-    if (d1 > kSmallF1) {
-        if (lane == 0) big_list[atomicAdd(big_count, 1)] = (int32_t)u;
+    if (d1 > F1) {
+        if (lane == 0) out_list[atomicAdd(out_count, 1)] = (int32_t)u;
         return;
     }
     // 1-hop: both rows, sorted, deduplicated, root dropped, capped
@@ -1123,12 +1143,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     for (int k = lane; k < P1; k += 64)
         Ls.f1[k] = k < df ? g.fwd_adj[fb + k] : (k < d1 ? g.bwd_adj[bb + k - df] : INT_MAX);
     __syncthreads();
-    if (d1 > 1) wave_bitonic(Ls.f1, P1);
-    const int n1 = wave_unique_compact(Ls.f1, d1, kSmallF1, [&](int32_t x) { return x != root; });
+    if (d1 > 1) wave_bitonic<F1>(Ls.f1, P1);
+    const int n1 = wave_unique_compact(Ls.f1, d1, F1, [&](int32_t x) { return x != root; });
     const int n1c = n1 < cap ? n1 : cap;
     // 2-hop candidate count: the F1c rows' offsets, all loads of a lane
     // issued together
-    constexpr int kChunks = kSmallF1 / 64;
+    constexpr int kChunks = F1 / 64;
     int64_t t = 0;
 #pragma unroll
     for (int ch = 0; ch < kChunks; ++ch) {
@@ -1140,8 +1160,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
     for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
     const int64_t tot2 = t + n1c;
-    if (tot2 > kSmallWin) {
-        if (lane == 0) big_list[atomicAdd(big_count, 1)] = (int32_t)u;
+    if (tot2 > WIN) {
+        if (lane == 0) out_list[atomicAdd(out_count, 1)] = (int32_t)u;
         return;
     }
     const bool check_f1 = n1 > n1c;
@@ -1201,7 +1221,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     while (P2 < (int)tot2) P2 <<= 1;
     for (int k = (int)tot2 + lane; k < P2; k += 64) Ls.win[k] = INT_MAX;
     __syncthreads();
-    if (tot2 > 1) wave_bitonic(Ls.win, P2);
+    if (tot2 > 1) wave_bitonic<WIN>(Ls.win, P2);
     const int ns = wave_unique_compact(Ls.win, (int)tot2, cap, [&](int32_t x) {
         if (x == root) return false;
         if (check_f1 && x > f1max) {
@@ -1268,6 +1288,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const int64_t by = 16 * (1 + (int64_t)n1c) + 4 * ((int64_t)d1 + tot2 - n1c) + 4 * (int64_t)ns;
         atomicAdd(bytes_out, (unsigned long long)by);
     }
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_bfs_small(
+    CsrView g, const float* __restrict__ w, const int32_t* __restrict__ uroot,
+    const int64_t* __restrict__ n_unique_ptr, int cap, int32_t* __restrict__ S_buf,
+    uint64_t* __restrict__ Q_buf, int32_t* __restrict__ ns_buf, int32_t* __restrict__ out_list,
+    int32_t* __restrict__ out_count, unsigned long long* __restrict__ bytes_out) {
+    bfs_wave<kSmallF1, kSmallWin, false>(g, w, uroot, n_unique_ptr, nullptr, nullptr, cap, S_buf,
+                                         Q_buf, ns_buf, out_list, out_count, bytes_out);
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_bfs_mid(
+    CsrView g, const float* __restrict__ w, const int32_t* __restrict__ uroot,
+    const int32_t* __restrict__ in_list, const int32_t* __restrict__ in_count, int cap,
+    int32_t* __restrict__ S_buf, uint64_t* __restrict__ Q_buf, int32_t* __restrict__ ns_buf,
+    int32_t* __restrict__ out_list, int32_t* __restrict__ out_count,
+    unsigned long long* __restrict__ bytes_out) {
+    bfs_wave<kMidF1, kMidWin, true>(g, w, uroot, nullptr, in_list, in_count, cap, S_buf, Q_buf,
+                                    ns_buf, out_list, out_count, bytes_out);
 }
 
 // ---- root deduplication ----
@@ -1533,10 +1572,14 @@ void launch_bfs_root(hipStream_t s, const CsrView& g, const float* w, const int3
                      uint64_t* Q_buf, int32_t* ns_buf, int32_t* big_list, int32_t* big_count,
                      unsigned long long* bytes_out, const BfsStage& stage) {
     if (max_unique <= 0) return;
+    int32_t* mid_list = big_list + max_unique;
     hipLaunchKernelGGL(k_bfs_small, dim3((uint32_t)max_unique), dim3(64), 0, s, g, w, uroot,
                        n_unique, cap, S_buf, Q_buf, ns_buf, big_list, big_count, bytes_out);
+    hipLaunchKernelGGL(k_bfs_mid, dim3((uint32_t)max_unique), dim3(64), 0, s, g, w, uroot,
+                       big_list, big_count, cap, S_buf, Q_buf, ns_buf, mid_list, big_count + 1,
+                       bytes_out);
     hipLaunchKernelGGL(k_bfs_root, dim3((uint32_t)max_unique), dim3(kBT), 0, s, g, w, uroot,
-                       big_list, big_count, cap, S_buf, Q_buf, ns_buf, bytes_out,
+                       mid_list, big_count + 1, cap, S_buf, Q_buf, ns_buf, bytes_out,
                        (const int32_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr, stage);
 }
 

@@ -126,7 +126,7 @@ int ensure_nb_buffers(pg_ctx* ctx) {
     if (r == hipSuccess) r = dalloc(&ctx->r_excl, D + 1);
     if (r == hipSuccess) r = dalloc(&ctx->root_slot, D);
     if (r == hipSuccess) r = dalloc(&ctx->uroot, D);
-    if (r == hipSuccess) r = dalloc(&ctx->big_list, D);
+    if (r == hipSuccess) r = dalloc(&ctx->big_list, 2 * D);
     if (r != hipSuccess)
         return fail(ctx, PG_ENOMEM, "neighbourhood buffers: device allocation failed");
     return PG_OK;

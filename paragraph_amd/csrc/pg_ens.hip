@@ -104,7 +104,7 @@ void enqueue_year(pg_ensemble* ens, int y, hipStream_t s) {
         launch_root_dedup(s, TOT, Dn, m, c->cit, c->cnt, c->root_bits, c->r_wcount, c->r_wprefix,
                           c->scratch, c->r_excl + Dn, c->root_slot, c->uroot);
         int32_t* big_count = reinterpret_cast<int32_t*>(cnt + 3);
-        launch_fill_i32(s, big_count, 0, 1);
+        launch_fill_i32(s, big_count, 0, 2);
         const CsrView g{c->fwd_off, c->fwd_adj, c->bwd_off[c->cur], c->bwd_adj[c->cur], TOT};
         launch_bfs_root(s, g, c->w, c->uroot, c->r_excl + Dn, Dn, c->cfg.bfs_cap, c->S_buf,
                         c->Q_buf, c->ns_buf, c->big_list, big_count, cnt + 2,

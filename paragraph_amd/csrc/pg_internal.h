@@ -145,9 +145,9 @@ void launch_root_dedup(hipStream_t s, int64_t n_exist, int64_t n_new, int m, con
                        const int32_t* cnt, uint32_t* root_bits, int32_t* wcount, int64_t* wprefix,
                        int64_t* scratch, int64_t* n_unique, int32_t* root_slot, int32_t* uroot);
 // 2-hop neighbourhood S (S_buf[u*cap...], ns_buf[u]) and its fixed-point
-// CDF (Q_buf[u*cap...]) of every distinct root: a wavefront per small root,
-// a workgroup per large one (big_list/big_count: scratch, *big_count = 0 on
-// entry); grids of max_unique blocks.
+// CDF (Q_buf[u*cap...]) of every distinct root: a wavefront per small or
+// medium root, a workgroup per large one (big_list: scratch of 2*max_unique;
+// big_count[0..1] = 0 on entry); grids of max_unique blocks.
 void launch_bfs_root(hipStream_t s, const CsrView& g, const float* w, const int32_t* uroot,
                      const int64_t* n_unique, int64_t max_unique, int cap, int32_t* S_buf,
                      uint64_t* Q_buf, int32_t* ns_buf, int32_t* big_list, int32_t* big_count,

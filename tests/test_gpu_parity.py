@@ -138,6 +138,61 @@ def test_pa_tiny_graph_exhaustion():
     assert st.shortfall > 0
 
 
+def tier_graph(n=12000, big=6, mid=40, seed=13):
+    """Seed graph whose papers spread over the three 2-hop paths: 6 papers
+    cited by ~7% of the others (in-degree ~800), 40 by ~2% (~240), each paper
+    also citing 3 random earlier ones."""
+    rng = np.random.default_rng(seed)
+    birth = np.sort(rng.integers(-20, 1, n)).astype(np.int32)
+    hubs = np.arange(big + mid)
+    pb = np.r_[np.full(big, 0.07), np.full(mid, 0.02)]
+    src, dst = [], []
+    for i in range(big + mid + 1, n):
+        t = rng.choice(i, size=3, replace=False)
+        hs = hubs[rng.random(big + mid) < pb]
+        hs = hs[~np.isin(hs, t)]
+        src += [i] * (3 + hs.size)
+        dst += list(t) + list(hs)
+    logfit = (rng.standard_normal(n + 2000) * 0.5).astype(np.float32)
+    return pg.SeedGraph(birth, np.array(src, np.int32), np.array(dst, np.int32), logfit)
+
+
+def bfs_tier_counts(d, roots, cap):
+    """Distinct roots per 2-hop path: [one wavefront with a 512/2048-entry
+    window, one wavefront with 1024/4096, one workgroup], by the 1-hop row
+    entries and the 2-hop candidate entries (F1c plus the rows of F1c)."""
+    fo, fa, bo, ba = d["fwd_off"], d["fwd_adj"], d["bwd_off"], d["bwd_adj"]
+    deg = np.diff(fo) + np.diff(bo)
+    out = np.zeros(3, np.int64)
+    for r in np.unique(roots):
+        f1 = np.unique(np.r_[fa[fo[r]:fo[r + 1]], ba[bo[r]:bo[r + 1]]])
+        f1c = f1[f1 != r][:cap]
+        d1, tot2 = deg[r], deg[f1c].sum() + f1c.size
+        out[0 if (d1 <= 512 and tot2 <= 2048) else 1 if (d1 <= 1024 and tot2 <= 4096) else 2] += 1
+    return out
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("cap", [4096, 64])
+def test_pa_bfs_tiers(cap, mode):
+    """Year steps whose roots take all three 2-hop paths (with cap 64 the
+    1024-entry wavefront also sees 1-hop rows beyond the cap)."""
+    g = tier_graph()
+    sched = [900, 900]
+    ctx = new_ctx(g, sched, 10, cap=cap, seed=9)
+    ctx.set_tuning(bfs_mode=mode)
+    for yi, D in enumerate(sched):
+        before = ctx.download()
+        check_pa_year(ctx, g.logfit, 1 + yi, yi, D, 10, 4, 1.2, 4.0, 9, cap)
+        gres = ctx.citations(D, 10)
+        roots = gres["cit"][(gres["cnt"] > 0) & (gres["kind"][:, 0] == 0), 0]
+        tiers = bfs_tier_counts(before, roots, cap)
+        log(f"  distinct roots per path: {tiers.tolist()}")
+        assert tiers[0] > 0 and tiers[1] > 0
+        if cap == 4096:
+            assert tiers[2] > 0
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("cap", [4096, 64, 1])
 def test_bfs_sets(cap, mode):
