@@ -29,7 +29,7 @@ PEAK_HBM_GBS = 8000.0
 
 # Kernels of each timed group, as they appear in the rocprofv3 summaries.
 GROUP_KERNELS = {
-    "bfs": ["k_bfs_small", "k_bfs_root"],
+    "bfs": ["k_bfs_small", "k_bfs_mid", "k_bfs_root"],
     "lik_grad": ["k_lik_papers<false>", "k_lik_year_z", "k_lik_cite<true, 4, 5>", "k_lik_year_ev",
                  "k_lik_segsum", "k_lik_papers<true>", "k_lik_final"],
 }

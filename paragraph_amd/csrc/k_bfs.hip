@@ -39,7 +39,10 @@ namespace {
 constexpr int kBT = 512;                    // threads per large root
 constexpr int kWaves = kBT / 64;
 constexpr int kWin = 7168;                  // LDS buffer: sort window or bitmap words
-constexpr int kSortMax = 4096;              // totals up to this are sorted, above it swept
+constexpr int kSortMax = 4096;              // totals up to this are always radix-sorted
+constexpr int kRadixMax = 7168;             // radix path: entries sorted in LDS (L.win)
+constexpr int kRadixRounds = kRadixMax / kBT;
+constexpr int kHistBuckets = 1024;          // radix path: buckets of the cut histogram
 constexpr int kBmBits = kWin * 32;          // ids per bitmap window
 constexpr int kMaxRows = 1 + 2 * kBfsCapMax;
 constexpr int kLongRow = 192;               // rows longer than this use wave search
@@ -51,7 +54,8 @@ constexpr int kPiece = 256;                 // staged path: long-row part per wa
 struct BfsLds {
     int32_t f1[kBfsCapMax];       // capped 1-hop list (ascending)
     int32_t win[kWin];            // sort window, or bitmap of one id window
-    int32_t cursor[kMaxRows];     // per-row position, or (staged path) bucket ends
+    int32_t cursor[kMaxRows];     // per-row position, (staged path) bucket ends, or
+                                  // (radix path) the second buffer and the counters
     int32_t lrow[kMaxLong];       // rows searched/copied by whole wavefronts
     int32_t lpre[kMaxLong + 1];   // staged path: long rows' first piece; bits 16-17 the row kind
     int32_t lbeg[kMaxLong];       // staged path: long rows' first entry
@@ -62,6 +66,8 @@ struct BfsLds {
     int32_t fill;
     int32_t nout;
     int32_t n1c;
+    int32_t cut_b;                // radix path: last histogram bucket below the cut
+    int32_t cut_n;                // radix path: entries up to and including it
     int64_t t_start;              // wall clock at the workgroup's start
     int64_t t_level1;             // wall clock after the 1-hop level
 };
@@ -230,38 +236,6 @@ __device__ __forceinline__ int run_add(int32_t* ctr, int key) {
     return __shfl(base, start, 64) + (lane - start);
 }
 
-// Bitonic sort of s[0,P) (P a power of two <= kSortMax) by the workgroup:
-// a stage's P/2 compare-exchange pairs are spread over the threads, each
-// loading all of its pairs before comparing.
-__device__ void bitonic_sort(int32_t* s, int P) {
-    constexpr int kPairs = kSortMax / (2 * kBT);
-    const int np = P >> 1;
-    for (int k = 2; k <= P; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            int32_t A[kPairs], B[kPairs];
-#pragma unroll
-            for (int r = 0; r < kPairs; ++r) {
-                const int t = threadIdx.x + r * kBT;
-                if (t < np) {
-                    const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-                    A[r] = s[i];
-                    B[r] = s[i + j];
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < kPairs; ++r) {
-                const int t = threadIdx.x + r * kBT;
-                if (t < np) {
-                    const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-                    const bool up = (i & k) == 0;
-                    if ((A[r] > B[r]) == up) { s[i] = B[r]; s[i + j] = A[r]; }
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
 struct Query {
     CsrView g;
     int32_t root;
@@ -315,39 +289,6 @@ __device__ __forceinline__ bool keep_id(const Query& q, int32_t x) {
     return true;
 }
 
-// Sort the window, drop duplicates and filtered ids, append to the output
-// (up to cap).  All threads call it.
-__device__ void flush_window(const Query& q, int nw, int cap) {
-    int P = 1;
-    while (P < nw) P <<= 1;
-    for (int k = nw + threadIdx.x; k < P; k += kBT) L.win[k] = INT_MAX;
-    __syncthreads();
-    if (nw > 1) bitonic_sort(L.win, P);
-    const int chunk = (nw + kBT - 1) / kBT;
-    const int s0 = threadIdx.x * chunk;
-    const int s1 = min(nw, s0 + chunk);
-    int mine = 0;
-    for (int k = s0; k < s1; ++k) {
-        int32_t x = L.win[k];
-        if ((k == 0 || L.win[k - 1] != x) && keep_id(q, x)) ++mine;
-    }
-    int64_t tot;
-    int64_t pos = block_excl_scan(mine, &tot) + L.nout;
-    for (int k = s0; k < s1; ++k) {
-        int32_t x = L.win[k];
-        if ((k == 0 || L.win[k - 1] != x) && keep_id(q, x)) {
-            if (pos < cap) put_out(q, (int)pos, x);
-            ++pos;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int64_t no = L.nout + tot;
-        L.nout = (int32_t)(no < cap ? no : cap);
-    }
-    __syncthreads();
-}
-
 // Appends the set bits of bm[0,nw) (ids lo + bit index) in ascending order
 // to the output after the L.nout already there, up to cap; updates L.nout.
 // Each thread takes a contiguous run of words.  All threads call it.
@@ -355,6 +296,9 @@ __device__ void flush_window(const Query& q, int nw, int cap) {
 // loads), so kWin / kXW threads cover a full window.
 constexpr int kXW = 16;
 static_assert(kWin % kXW == 0 && kWin / kXW <= kBT, "bitmap window per thread");
+static_assert(kRadixMax <= kWin && kRadixMax + 2 * kBT <= kMaxRows && kRadixMax % kBT == 0 &&
+                  kHistBuckets <= kRadixMax,
+              "radix path buffers");
 __device__ __forceinline__ void extract_bits(const Query& q, const uint32_t* bm, int nw, int32_t lo,
                                              int cap) {
     const int w0 = threadIdx.x * kXW;
@@ -543,13 +487,135 @@ __device__ __forceinline__ void stage_rows(const Query& q, int nr, int cap, bool
     }
 }
 
+// Radix path, step 1: the level's row entries (rows cut as in stage_rows)
+// below lim, the root dropped, appended to A[0, kRadixMax) in visit order;
+// returns their number (those past the capacity are counted, not stored).
+// With H (kHistBuckets zeroed counters), also counts them per bucket
+// x >> hsh.  All threads call it.
+__device__ __forceinline__ int radix_gather(const Query& q, int nr, int cap, bool long_ok, int nlong,
+                                            int npieces, int32_t lim, int32_t* A, int32_t* H,
+                                            int hsh) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (1ull << lane) - 1;
+    if (threadIdx.x == 0) L.fill = 0;
+    __syncthreads();
+    stage_rows(q, nr, cap, long_ok, nlong, npieces, [&](int32_t x, bool ok) {
+        const bool keep = ok && x < lim && x != q.root;
+        const uint64_t m = __ballot(keep);
+        if (m == 0) return;  // uniform over the wavefront
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&L.fill, __popcll(m));
+        base = __shfl(base, 0, 64);
+        const int at = base + __popcll(m & lt);
+        if (keep && at < kRadixMax) A[at] = x;
+        if (H) run_add<false>(H, keep ? x >> hsh : -1);
+    });
+    __syncthreads();
+    return L.fill;
+}
+
+// Radix path, step 2: stable LSD radix sort of A[0,M) (M <= kRadixMax),
+// 8-bit digits over the low `bits` bits, B[0,M) the other buffer; returns
+// the buffer with the result.  Wavefront w holds entries [w*64K, (w+1)*64K)
+// (K = ceil(M / kBT)), round r at lane l entry w*64K + 64r + l; an entry's
+// rank among its wavefront's entries of the same digit comes from eight
+// ballots (the lanes with equal digits) and a per-wavefront counter, the
+// counters (16-bit, digit-major over the wavefronts, two per word of C[0,
+// kBT*2)) are scanned, and each entry is stored at its digit's offset plus
+// its rank.  All threads call it.
+__device__ __forceinline__ int32_t* radix_sort(int32_t* A, int32_t* B, uint32_t* C, int M,
+                                               int bits) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int K = (M + kBT - 1) / kBT;
+    const int w0 = wid * 64 * K;
+    const uint64_t lt = (1ull << lane) - 1;
+    for (int shift = 0; shift < bits; shift += 8) {
+        C[threadIdx.x] = 0u;
+        C[threadIdx.x + kBT] = 0u;
+        __syncthreads();
+        int rk[kRadixRounds];  // the keys are read again for the scatter
+#pragma unroll
+        for (int r = 0; r < kRadixRounds; ++r) {
+            rk[r] = 0;
+            if (r < K) {  // uniform
+                const int idx = w0 + r * 64 + lane;
+                const bool valid = idx < M;
+                const uint32_t d = valid ? ((uint32_t)A[idx] >> shift) & 255u : 0u;
+                uint64_t peers = __ballot(valid);
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    const bool bit = (d >> b) & 1u;
+                    const uint64_t mb = __ballot(bit);
+                    peers &= bit ? mb : ~mb;
+                }
+                const int ci = (int)d * kWaves + wid;
+                const int hs = (ci & 1) * 16;
+                const int before = (int)((C[ci >> 1] >> hs) & 0xFFFFu);
+                rk[r] = before + __popcll(peers & lt);
+                if (valid && (peers & lt) == 0)
+                    atomicAdd(&C[ci >> 1], (uint32_t)__popcll(peers) << hs);
+            }
+        }
+        __syncthreads();
+        {
+            const uint32_t a = C[2 * threadIdx.x], b = C[2 * threadIdx.x + 1];
+            const int c0 = (int)(a & 0xFFFFu), c1 = (int)(a >> 16);
+            const int c2 = (int)(b & 0xFFFFu), c3 = (int)(b >> 16);
+            int tot;
+            const int p0 = block_excl_scan32(c0 + c1 + c2 + c3, &tot);
+            const int p1 = p0 + c0, p2 = p1 + c1, p3 = p2 + c2;
+            C[2 * threadIdx.x] = (uint32_t)p0 | ((uint32_t)p1 << 16);
+            C[2 * threadIdx.x + 1] = (uint32_t)p2 | ((uint32_t)p3 << 16);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kRadixRounds; ++r) {
+            const int idx = w0 + r * 64 + lane;
+            if (r < K && idx < M) {
+                const int32_t key = A[idx];
+                const int ci = (int)(((uint32_t)key >> shift) & 255u) * kWaves + wid;
+                B[(int)((C[ci >> 1] >> ((ci & 1) * 16)) & 0xFFFFu) + rk[r]] = key;
+            }
+        }
+        __syncthreads();
+        int32_t* t = A;
+        A = B;
+        B = t;
+    }
+    return A;
+}
+
+// Radix path, step 3: the distinct kept ids of the sorted Z[0,M), the
+// first cap of them to the output; returns how many there are (uncapped).
+// Each thread takes a contiguous run.  All threads call it.
+__device__ __forceinline__ int radix_emit(const Query& q, const int32_t* Z, int M, int cap) {
+    const int chunk = (M + kBT - 1) / kBT;
+    const int s0 = min(M, (int)threadIdx.x * chunk);
+    const int s1 = min(M, s0 + chunk);
+    int mine = 0;
+    for (int k = s0; k < s1; ++k) {
+        const int32_t x = Z[k];
+        if ((k == 0 || Z[k - 1] != x) && keep_id(q, x)) ++mine;
+    }
+    int tot;
+    int pos = block_excl_scan32(mine, &tot);
+    for (int k = s0; k < s1 && pos < cap; ++k) {
+        const int32_t x = Z[k];
+        if ((k == 0 || Z[k - 1] != x) && keep_id(q, x)) put_out(q, pos++, x);
+    }
+    __syncthreads();  // the output is complete
+    return tot;
+}
+
 // Collects the `cap` smallest distinct kept ids of the level's rows into the
 // output (level 1: L.f1, level 2: q.S; ascending); returns the count.  All
-// threads call it.  Up to kSortMax row entries are gathered and
-// bitonic-sorted in LDS.  Larger lists below the answer bound (from rows
-// longer than cap) are staged in a pool slot bucketed by id when they fit,
-// else swept through the id range in bitmap windows; either way ids come out
-// of LDS bitmaps in ascending order.  It has one call site (two_hop) in one
+// threads call it.  Row entries below the answer bound (from rows longer
+// than cap) are gathered into LDS and radix-sorted there: always up to
+// kSortMax row entries, and (q.st.radix) any list whose entries below the
+// bound, or below a tighter bound from their histogram, fit kRadixMax.
+// Other lists are staged in a pool slot bucketed by id when they fit, else
+// swept through the id range in bitmap windows; either way ids come out of
+// LDS bitmaps in ascending order.  It has one call site (two_hop) in one
 // kernel (k_bfs_root): out-of-line calls cost a scratch stack frame per lane.
 __device__ __forceinline__ int collect(Query& q, int cap) {
     const int32_t n = (int32_t)q.g.n;
@@ -593,6 +659,89 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
     if (threadIdx.x == 0) {
         L.nout = 0; L.tot[q.level] = total;
         if (L.nlong > kMaxLong) L.nlong = -1;  // list overflow: every row per thread
+        L.cut_b = -1;
+    }
+    __syncthreads();
+    if (total == 0) return 0;
+    const bool long_ok = L.nlong >= 0;
+    const int nlong = long_ok ? L.nlong : 0;
+    // long rows in pieces of kPiece entries: first piece, kind and length
+    // per long row (L.lpre), first entry (L.lbeg)
+    int npieces;
+    {
+        // (an opaque copy of the thread index: left visible, the LDS
+        // addresses below are hoisted out of the level loop and spilled)
+        int tid = (int)threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        int cnt = 0, pk = 0, bb = 0, ln = 0;
+        if (tid < nlong) {
+            const int r = L.lrow[tid];
+            g_cint* p; int b, e;
+            graph_row(q, r, p, b, e);
+            ln = min(e - b, cap + 1);
+            cnt = (ln + kPiece - 1) / kPiece;
+            pk = q.level == 1 ? r : (r - 1) & 1;
+            bb = b;
+        }
+        int64_t tp;
+        const int pre = (int)block_excl_scan(cnt, &tp);
+        if (tid < nlong) {
+            L.lpre[tid] = pre | (pk << 16) | (ln << 18);
+            L.lbeg[tid] = bb;
+        }
+        npieces = (int)tp;
+    }
+    if (total <= kSortMax || q.st.radix) {
+        // radix path: the entries below the bound gathered into L.win and
+        // sorted there.  When more than kRadixMax of them, a histogram of
+        // the first gather gives a lower bound holding cap + cap/4 entries
+        // and they are gathered again below it; kept ids short of cap
+        // below such a bound (duplicates) leave it to the paths below.
+        int32_t* A = L.win;
+        int32_t* B = L.cursor;
+        uint32_t* C = reinterpret_cast<uint32_t*>(L.cursor + kRadixMax);
+        int32_t lim = (int32_t)span;
+        const bool hist = total_cut > kRadixMax;
+        int hsh = 0;
+        while (((lim - 1) >> hsh) >= kHistBuckets) ++hsh;
+        if (hist)
+            for (int k = threadIdx.x; k < kHistBuckets; k += kBT) B[k] = 0;
+        int M = radix_gather(q, nr, cap, long_ok, nlong, npieces, lim, A, hist ? B : nullptr, hsh);
+        bool cut = false;
+        if (M > kRadixMax) {
+            static_assert(kHistBuckets == 2 * kBT, "two buckets per thread");
+            const int target = cap + cap / 4;
+            const int v0 = B[2 * threadIdx.x], v1 = B[2 * threadIdx.x + 1];
+            int tot;
+            const int p0 = block_excl_scan32(v0 + v1, &tot);
+            if (p0 < target && p0 + v0 + v1 >= target) {
+                const bool first = p0 + v0 >= target;
+                L.cut_b = 2 * (int)threadIdx.x + (first ? 0 : 1);
+                L.cut_n = first ? p0 + v0 : p0 + v0 + v1;
+            }
+            __syncthreads();
+            const int cb = L.cut_b;
+            if (cb >= 0 && L.cut_n <= kRadixMax) {
+                lim = (int32_t)min((int64_t)lim, (int64_t)(cb + 1) << hsh);
+                cut = true;
+                M = radix_gather(q, nr, cap, long_ok, nlong, npieces, lim, A, nullptr, 0);
+            }
+        }
+        if (M <= kRadixMax) {
+            const int bits = 32 - __builtin_clz((uint32_t)max(lim - 1, 1));
+            const int32_t* Z = radix_sort(A, B, C, M, bits);
+            const int kept = radix_emit(q, Z, M, cap);
+            if (!cut || kept >= cap) return min(kept, cap);
+        }
+        // the sweep's row cursors again
+        for (int r = threadIdx.x; r < nr; r += kBT) {
+            g_cint* p; int b, e;
+            row_bounds(q, r, p, b, e);
+            L.cursor[r] = b;
+        }
+    }
+    if (threadIdx.x == 0) {
+        L.nout = 0;
         if (want_stage && L.slot < 0) {
             // first free slot from a block-dependent start; one pass at most
             for (int t = 0; t < q.st.n_slots; ++t) {
@@ -603,36 +752,6 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
         }
     }
     __syncthreads();
-    if (total == 0) return 0;
-    if (total <= kSortMax) {
-        // one sorted window: the F1c list first, then the CSR rows, a
-        // wavefront's chunk placed by one LDS atomic
-        const int f1n = q.level == 2 ? q.n1c : 0;
-        for (int k = threadIdx.x; k < f1n; k += kBT) L.win[k] = L.f1[k];
-        if (threadIdx.x == 0) L.fill = f1n;
-        __syncthreads();
-        for (int r0 = wid * 64; r0 < nr; r0 += kBT) {
-            const int r = r0 + lane;
-            g_cint* p = GROW(q.g.fwd_adj);
-            int b = 0, e = 0;
-            if (r < nr && !is_f1_row(q, r)) graph_row(q, r, p, b, e);
-            int len = e - b;
-            int x = len;
-            for (int off = 1; off < 64; off <<= 1) x += __shfl_xor(x, off, 64);
-            if (x == 0) continue;
-            int base = 0;
-            if (lane == 0) base = atomicAdd(&L.fill, x);
-            base = __shfl(base, 0, 64);
-            chunk_rows(p, b, len, [&](int32_t v, int pp, bool ok) {
-                if (ok) L.win[base + pp] = v;
-            });
-        }
-        __syncthreads();
-        flush_window(q, (int)total, cap);
-        return L.nout;
-    }
-    const bool long_ok = L.nlong >= 0;
-    const int nlong = long_ok ? L.nlong : 0;
     uint32_t* bm = reinterpret_cast<uint32_t*>(L.win);
     if (want_stage && L.slot >= 0) {
         // staged: the entries below the bound are counted per id bucket
@@ -642,30 +761,6 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
         g_int* buf = (g_int*)(q.st.buf + (int64_t)L.slot * q.st.cap_entries);
         const int nbk = (int)((span - 1) >> sh) + 1;
         int32_t* E = L.cursor;  // bucket counts, then starts, then ends
-        int npieces;
-        {
-            // (an opaque copy of the thread index: left visible, the LDS
-            // addresses below are hoisted out of the level loop and spilled)
-            int tid = (int)threadIdx.x;
-            asm volatile("" : "+v"(tid));
-            int cnt = 0, pk = 0, bb = 0, ln = 0;
-            if (tid < nlong) {
-                const int r = L.lrow[tid];
-                g_cint* p; int b, e;
-                graph_row(q, r, p, b, e);
-                ln = min(e - b, cap + 1);
-                cnt = (ln + kPiece - 1) / kPiece;
-                pk = q.level == 1 ? r : (r - 1) & 1;
-                bb = b;
-            }
-            int64_t tp;
-            const int pre = (int)block_excl_scan(cnt, &tp);
-            if (tid < nlong) {
-                L.lpre[tid] = pre | (pk << 16) | (ln << 18);
-                L.lbeg[tid] = bb;
-            }
-            npieces = (int)tp;
-        }
         for (int k = threadIdx.x; k < nbk; k += kBT) E[k] = 0;
         for (int k = threadIdx.x; k < kWin; k += kBT) bm[k] = 0u;  // each window re-zeroes what it used
         __syncthreads();
@@ -1608,6 +1703,7 @@ void launch_nb_draw_batch(hipStream_t s, const Batch& b, int m, int cap, uint32_
 BfsStage bfs_stage_view(const BfsStage& pool, int mode) {
     BfsStage v = pool;
     v.max_total = mode == 1 ? 0 : mode == 2 ? INT64_MAX : kBfsStageDefaultMax;
+    v.radix = mode == 0 ? 1 : 0;
     return v;
 }
 

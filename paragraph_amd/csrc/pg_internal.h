@@ -123,6 +123,7 @@ struct BfsStage {
     int32_t n_slots = 0;
     int32_t cap_entries = 0;
     int64_t max_total = 0;   // larger candidate lists (row entries) are swept
+    int32_t radix = 0;       // 1: every list tries the in-LDS radix sort first
 };
 // Slots for one device: twice the resident workgroups of the path.
 int bfs_stage_slots(int device);

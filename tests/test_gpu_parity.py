@@ -193,6 +193,45 @@ def test_pa_bfs_tiers(cap, mode):
             assert tiers[2] > 0
 
 
+def cut_graph(n=40000, hubs=4, p=0.1, k=4, seed=17):
+    """Seed graph with 4 papers each cited by ~10% of the others (in-degree
+    ~4000, citers mostly distinct), every paper citing 4 random earlier
+    ones: a root citing two or more of them has 8000+ distinct 2-hop
+    candidates below the answer bound."""
+    rng = np.random.default_rng(seed)
+    birth = np.sort(rng.integers(-20, 1, n)).astype(np.int32)
+    src, dst = [], []
+    for i in range(hubs + 1, n):
+        t = rng.choice(i, size=min(k, i), replace=False)
+        hs = np.nonzero(rng.random(hubs) < p)[0]
+        hs = hs[~np.isin(hs, t)]
+        src += [i] * (t.size + hs.size)
+        dst += list(t) + list(hs)
+    return pg.SeedGraph(birth, np.array(src, np.int32), np.array(dst, np.int32),
+                        np.zeros(n, np.float32))
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("cap", [4096, 1500])
+def test_bfs_radix_cut(cap, mode):
+    """Workgroup path on lists whose entries below the answer bound exceed
+    the in-LDS sort (7168): cut at a histogram bound when the kept ids below
+    it reach cap, otherwise (many duplicates, the hub roots) handed to the
+    staged or swept path."""
+    g = cut_graph()
+    ctx = new_ctx(g, [1], 1, cap=cap)
+    ctx.set_tuning(bfs_mode=mode)
+    G = oracle_graph(ctx, g.logfit)
+    src = np.asarray(g.src)
+    two_hubs = np.nonzero(np.bincount(src[np.asarray(g.dst) < 4], minlength=g.n) >= 2)[0]
+    roots = np.concatenate([np.arange(0, 6), two_hubs[:40]]).astype(np.int32)
+    log(f"BFS cut graph cap={cap} mode={mode} roots={roots.size}")
+    got = ctx.bfs(roots)
+    for r, s in zip(roots, got):
+        ref = ob.bfs(G, int(r), cap)
+        assert np.array_equal(s, ref), f"root {r}: {s[:8]} vs {ref[:8]}"
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("cap", [4096, 64, 1])
 def test_bfs_sets(cap, mode):
