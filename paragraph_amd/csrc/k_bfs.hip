@@ -43,6 +43,7 @@ constexpr int kSortMax = 4096;              // totals up to this are always radi
 constexpr int kRadixMax = 7168;             // radix path: entries sorted in LDS (L.win)
 constexpr int kRadixRounds = kRadixMax / kBT;
 constexpr int kHistBuckets = 1024;          // radix path: buckets of the cut histogram
+constexpr int kWaveSortMax = 512;           // radix path: fewer entries sorted by one wavefront
 constexpr int kBmBits = kWin * 32;          // ids per bitmap window
 constexpr int kMaxRows = 1 + 2 * kBfsCapMax;
 constexpr int kLongRow = 192;               // rows longer than this use wave search
@@ -487,6 +488,84 @@ __device__ __forceinline__ void stage_rows(const Query& q, int nr, int cap, bool
     }
 }
 
+// Value of lane (lane ^ J) without the LDS crossbar: DPP quad permutes and
+// row mirrors for J < 16 (xor 4 = half-row mirror then xor 3; xor 8 = row
+// mirror then half-row mirror) and gfx950's row-pair swaps for 16 and 32.
+template <int J>
+__device__ __forceinline__ int32_t xor_lane(int32_t v) {
+    static_assert(J == 1 || J == 2 || J == 4 || J == 8 || J == 16 || J == 32, "lane xor");
+    if constexpr (J == 1) {
+        return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    } else if constexpr (J == 2) {
+        return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    } else if constexpr (J == 4) {
+        const int32_t t = __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);  // ^7
+        return __builtin_amdgcn_update_dpp(0, t, 0x1B, 0xF, 0xF, false);             // ^3
+    } else if constexpr (J == 8) {
+        const int32_t t = __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);  // ^15
+        return __builtin_amdgcn_update_dpp(0, t, 0x141, 0xF, 0xF, false);            // ^7
+    } else if constexpr (J == 16) {
+        // {v with odd rows <- even rows, v with even rows <- odd rows}
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return ((threadIdx.x >> 4) & 1) ? (int32_t)r[0] : (int32_t)r[1];
+    } else {
+        const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return ((threadIdx.x >> 5) & 1) ? (int32_t)r[0] : (int32_t)r[1];
+    }
+}
+
+// Bitonic sort of s[0,P) (P a power of two <= 4096) by one wavefront,
+// in registers: element i lives in register e = i / 64 of lane i % 64 (so
+// the loads and stores are conflict-free and the result lands back in
+// natural order).  Compare distances below 64 are lane exchanges (DPP and
+// row swaps, xor_lane), larger ones pairs of registers within a lane; the
+// network is fully unrolled for E = P / 64 registers per lane.
+// One compare-exchange step (distance J) of merge size K; recurses down to
+// J = 1.  Compile-time J and K, so every lane exchange is a fixed pattern.
+template <int E, int K, int J>
+__device__ __forceinline__ void bitonic_step(int32_t (&v)[E], int lane) {
+    if constexpr (J < 64) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int i = e * 64 + lane;
+            const int32_t o = xor_lane<J>(v[e]);
+            const bool up = (i & K) == 0;
+            const bool lower = (lane & J) == 0;
+            v[e] = (lower == up) ? min(v[e], o) : max(v[e], o);
+        }
+    } else {
+        constexpr int je = J / 64;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if ((e & je) == 0) {
+                const int i = e * 64 + lane;
+                const bool up = (i & K) == 0;
+                const int32_t x = v[e], y = v[e + je];
+                v[e] = up ? min(x, y) : max(x, y);
+                v[e + je] = up ? max(x, y) : min(x, y);
+            }
+        }
+    }
+    if constexpr (J > 1) bitonic_step<E, K, J / 2>(v, lane);
+}
+
+template <int E, int K>
+__device__ __forceinline__ void bitonic_merges(int32_t (&v)[E], int lane) {
+    bitonic_step<E, K, K / 2>(v, lane);
+    if constexpr (K < 64 * E) bitonic_merges<E, 2 * K>(v, lane);
+}
+
+template <int E>
+__device__ __forceinline__ void wave_sort_reg(int32_t* s) {
+    const int lane = threadIdx.x & 63;
+    int32_t v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = s[e * 64 + lane];
+    bitonic_merges<E, 2>(v, lane);
+#pragma unroll
+    for (int e = 0; e < E; ++e) s[e * 64 + lane] = v[e];
+}
+
 // Radix path, step 1: the level's row entries (rows cut as in stage_rows)
 // below lim, the root dropped, appended to A[0, kRadixMax) in visit order;
 // returns their number (those past the capacity are counted, not stored).
@@ -728,8 +807,26 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
             }
         }
         if (M <= kRadixMax) {
-            const int bits = 32 - __builtin_clz((uint32_t)max(lim - 1, 1));
-            const int32_t* Z = radix_sort(A, B, C, M, bits);
+            const int32_t* Z = A;
+            if (M <= kWaveSortMax) {
+                // few entries: one wavefront's register sort, no barriers
+                int P = 64;
+                while (P < M) P <<= 1;
+                for (int k = M + (int)threadIdx.x; k < P; k += kBT) A[k] = INT_MAX;
+                __syncthreads();
+                if (wid == 0) {
+                    switch (P) {
+                        case 64: wave_sort_reg<1>(A); break;
+                        case 128: wave_sort_reg<2>(A); break;
+                        case 256: wave_sort_reg<4>(A); break;
+                        default: wave_sort_reg<8>(A); break;
+                    }
+                }
+                __syncthreads();
+            } else {
+                const int bits = 32 - __builtin_clz((uint32_t)max(lim - 1, 1));
+                Z = radix_sort(A, B, C, M, bits);
+            }
             const int kept = radix_emit(q, Z, M, cap);
             if (!cut || kept >= cap) return min(kept, cap);
         }
@@ -1072,84 +1169,6 @@ struct WaveLds {
     int32_t f1[F1];
     int32_t win[WIN];
 };
-
-// Value of lane (lane ^ J) without the LDS crossbar: DPP quad permutes and
-// row mirrors for J < 16 (xor 4 = half-row mirror then xor 3; xor 8 = row
-// mirror then half-row mirror) and gfx950's row-pair swaps for 16 and 32.
-template <int J>
-__device__ __forceinline__ int32_t xor_lane(int32_t v) {
-    static_assert(J == 1 || J == 2 || J == 4 || J == 8 || J == 16 || J == 32, "lane xor");
-    if constexpr (J == 1) {
-        return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
-    } else if constexpr (J == 2) {
-        return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
-    } else if constexpr (J == 4) {
-        const int32_t t = __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);  // ^7
-        return __builtin_amdgcn_update_dpp(0, t, 0x1B, 0xF, 0xF, false);             // ^3
-    } else if constexpr (J == 8) {
-        const int32_t t = __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);  // ^15
-        return __builtin_amdgcn_update_dpp(0, t, 0x141, 0xF, 0xF, false);            // ^7
-    } else if constexpr (J == 16) {
-        // {v with odd rows <- even rows, v with even rows <- odd rows}
-        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
-        return ((threadIdx.x >> 4) & 1) ? (int32_t)r[0] : (int32_t)r[1];
-    } else {
-        const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
-        return ((threadIdx.x >> 5) & 1) ? (int32_t)r[0] : (int32_t)r[1];
-    }
-}
-
-// Bitonic sort of s[0,P) (P a power of two <= 4096) by one wavefront,
-// in registers: element i lives in register e = i / 64 of lane i % 64 (so
-// the loads and stores are conflict-free and the result lands back in
-// natural order).  Compare distances below 64 are lane exchanges (DPP and
-// row swaps, xor_lane), larger ones pairs of registers within a lane; the
-// network is fully unrolled for E = P / 64 registers per lane.
-// One compare-exchange step (distance J) of merge size K; recurses down to
-// J = 1.  Compile-time J and K, so every lane exchange is a fixed pattern.
-template <int E, int K, int J>
-__device__ __forceinline__ void bitonic_step(int32_t (&v)[E], int lane) {
-    if constexpr (J < 64) {
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int i = e * 64 + lane;
-            const int32_t o = xor_lane<J>(v[e]);
-            const bool up = (i & K) == 0;
-            const bool lower = (lane & J) == 0;
-            v[e] = (lower == up) ? min(v[e], o) : max(v[e], o);
-        }
-    } else {
-        constexpr int je = J / 64;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            if ((e & je) == 0) {
-                const int i = e * 64 + lane;
-                const bool up = (i & K) == 0;
-                const int32_t x = v[e], y = v[e + je];
-                v[e] = up ? min(x, y) : max(x, y);
-                v[e + je] = up ? max(x, y) : min(x, y);
-            }
-        }
-    }
-    if constexpr (J > 1) bitonic_step<E, K, J / 2>(v, lane);
-}
-
-template <int E, int K>
-__device__ __forceinline__ void bitonic_merges(int32_t (&v)[E], int lane) {
-    bitonic_step<E, K, K / 2>(v, lane);
-    if constexpr (K < 64 * E) bitonic_merges<E, 2 * K>(v, lane);
-}
-
-template <int E>
-__device__ __forceinline__ void wave_sort_reg(int32_t* s) {
-    const int lane = threadIdx.x & 63;
-    int32_t v[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = s[e * 64 + lane];
-    bitonic_merges<E, 2>(v, lane);
-#pragma unroll
-    for (int e = 0; e < E; ++e) s[e * 64 + lane] = v[e];
-}
 
 // s[0,P), P a power of two <= WIN: padded to 64 if smaller (with INT_MAX),
 // sorted.
