@@ -93,7 +93,19 @@ def cpu_baseline(G0, sched, m, n_nb, alpha, tau, seed, cap):
     dt = time.perf_counter() - t0
     return {"value": edges / dt, "unit": "citation edges/s", "cores": cores, "kind": "port",
             "sample": f"the full workload ({len(sched)} years, {edges} edges) on the C oracle "
-                      f"with {cores} OpenMP threads: {dt:.2f} s"}
+                      f"with {cores} OpenMP threads: {dt:.2f} s"}, G
+
+
+def degree_ks(gpu_in_deg, cpu_in_deg) -> dict:
+    """Two-sample Kolmogorov-Smirnov statistic between the final in-degree
+    distributions of the GPU run and the oracle's run of the same workload,
+    plus the number of papers whose in-degree differs (PA draws differ only
+    where a uniform fell within 1e-12 relative of a CDF boundary)."""
+    from scipy import stats
+    r = stats.ks_2samp(gpu_in_deg, cpu_in_deg)
+    return {"statistic": float(r.statistic), "pvalue": float(r.pvalue),
+            "papers": int(gpu_in_deg.size),
+            "papers_differing_in_degree": int(np.count_nonzero(gpu_in_deg != cpu_in_deg))}
 
 
 def er_bench(args) -> dict:
@@ -599,10 +611,15 @@ def main():
                                           if v["launches"]}},
         "roofline": roofline(dom, achieved, "pa_10y_2M_kernels.md", dom),
     }
+    gpu_in_deg = None
+    if not args.no_cpu_baseline:
+        one_step()  # untimed: the final graph for the degree comparison
+        gpu_in_deg = ctx.download()["in_deg"].copy()
     ctx.close()
     if not args.no_cpu_baseline:
         try:
-            out["cpu_baseline"] = cpu_baseline(G0, sched, m, n_nb, alpha, tau, seed, cap)
+            out["cpu_baseline"], G = cpu_baseline(G0, sched, m, n_nb, alpha, tau, seed, cap)
+            out["degree_ks"] = degree_ks(gpu_in_deg, G.in_deg)
         except Exception as exc:  # reported, never fatal for the GPU number
             eprint(f"cpu baseline failed: {exc}")
             out["cpu_baseline"] = None
