@@ -369,10 +369,13 @@ typedef struct {
                                   1, 2, 4, 8 with 8 * lanes >= citations per
                                   paper (0 = the fewest such) */
     int32_t bfs_mode;          /* workgroup 2-hop path for candidate lists
-                                  above 4096 entries: 0 = staged in id
-                                  buckets up to 65536 entries, swept above;
-                                  1 = always swept; 2 = staged whenever the
-                                  list fits a staging slot */
+                                  above 4096 entries: 0 = radix-sorted in LDS
+                                  when the entries below the answer bound
+                                  (or a histogram cut) fit 7168, else staged
+                                  in id buckets up to 65536 entries, else
+                                  swept; 1 = always swept; 2 = staged
+                                  whenever the list fits a staging slot
+                                  (lists up to 4096 entries: radix-sorted) */
 } pg_tuning;
 
 int pg_ensemble_create(const pg_ensemble_config* cfg, const pg_sim_params* sims,
