@@ -340,9 +340,21 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
     // second interval of a group within the same step (NB_DRAW: dedup, then draws)
     auto pbeg2 = [&](int k) { if (ctx->prof) (void)hipEventRecord(ctx->pev2[k][0], s); };
     auto pend2 = [&](int k) { if (ctx->prof) { (void)hipEventRecord(ctx->pev2[k][1], s); used2[k] = true; } };
+    // PA years whose bound D*m fits max_edges run without a host round trip
+    // mid-year: the append sorts D*m keys (the unfilled tail padded with key
+    // N+D, past every paper) and the merge reads its edge total on the device
+    const int64_t UB = D * (int64_t)std::max(p->m, 0);
+    const bool one_sync = p->model == PG_MODEL_PA && !ctx->seq_recording && D > 0 &&
+                          ctx->e + UB <= ctx->cfg.max_edges;
+    if (one_sync) {
+        int rc = ensure_year_edges(ctx, UB);
+        if (rc) return rc;
+    }
     PG_HIP(hipEventRecord(ctx->ev0, s));
     int64_t e_year = 0, near_total = 0, attempts = 0;
     double bfs_bytes = 0.0;
+    int64_t tot = 0;
+    unsigned long long hs[4] = {0, 0, 0, 0};
     if (D > 0) {
         launch_fill_i32(s, ctx->birth + N, p->year, D);
         unsigned long long* sums = ctx->d_sum;  // [0] near, [1] attempts, [2] BFS bytes
@@ -388,18 +400,13 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
         }
         launch_excl_scan_i32(s, ctx->cnt, D, ctx->new_off, ctx->scratch);
         PG_HIP(hipGetLastError());
-        int64_t tot = 0;
-        unsigned long long hs[4] = {0, 0, 0, 0};
-        PG_HIP(hipMemcpyAsync(&tot, ctx->new_off + D, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        PG_HIP(hipMemcpyAsync(hs, sums, sizeof(hs), hipMemcpyDeviceToHost, s));
-        PG_HIP(hipStreamSynchronize(s));
-        e_year = tot;
-        near_total = (int64_t)hs[0];
-        attempts = (int64_t)hs[1];
-        bfs_bytes = (double)hs[2];
-        if (ctx->e + e_year > ctx->cfg.max_edges)
-            return fail(ctx, PG_ECAP, "pg_year_step: edges would exceed max_edges");
-        {
+        if (!one_sync) {
+            PG_HIP(hipMemcpyAsync(&tot, ctx->new_off + D, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+            PG_HIP(hipMemcpyAsync(hs, sums, sizeof(hs), hipMemcpyDeviceToHost, s));
+            PG_HIP(hipStreamSynchronize(s));
+            e_year = tot;
+            if (ctx->e + e_year > ctx->cfg.max_edges)
+                return fail(ctx, PG_ECAP, "pg_year_step: edges would exceed max_edges");
             int rc = ensure_year_edges(ctx, e_year);
             if (rc) return rc;
         }
@@ -413,20 +420,27 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
             pend(PG_K_ER);
             pbeg(PG_K_CSR_APPEND);
         }
-        launch_radix_sort_pairs(s, reinterpret_cast<const uint32_t*>(ctx->fwd_adj + ctx->e),
-                                reinterpret_cast<const uint32_t*>(ctx->src_of), ctx->k_sorted,
-                                ctx->v_sorted, ctx->k_tmp, ctx->v_tmp, e_year, key_bits_for(N),
-                                ctx->scratch);
+        uint32_t* keys = reinterpret_cast<uint32_t*>(ctx->fwd_adj + ctx->e);
+        const int64_t n_keys = one_sync ? UB : e_year;
+        if (one_sync) launch_fill_tail_u32(s, keys, ctx->new_off + D, UB, (uint32_t)(N + D));
+        launch_radix_sort_pairs(s, keys, reinterpret_cast<const uint32_t*>(ctx->src_of),
+                                ctx->k_sorted, ctx->v_sorted, ctx->k_tmp, ctx->v_tmp, n_keys,
+                                key_bits_for(one_sync ? N + D + 1 : N), ctx->scratch);
         const int nxt = 1 - ctx->cur;
-        launch_bwd_offsets_sorted(s, N, D, ctx->bwd_off[ctx->cur], ctx->k_sorted, e_year,
+        launch_bwd_offsets_sorted(s, N, D, ctx->bwd_off[ctx->cur], ctx->k_sorted, n_keys,
                                   ctx->bwd_off[nxt], ctx->in_deg);
         // the scratch holds at least max_papers + 2 int64 (its scan part)
         launch_bwd_merge(s, N, N + D, ctx->bwd_off[ctx->cur], ctx->bwd_adj[ctx->cur],
-                         ctx->bwd_off[nxt], ctx->v_sorted, ctx->bwd_adj[nxt], ctx->e + e_year, 0,
+                         ctx->bwd_off[nxt], ctx->v_sorted, ctx->bwd_adj[nxt],
+                         one_sync ? -1 : ctx->e + e_year, one_sync ? ctx->e + UB : 0,
                          ctx->scratch, ctx->cfg.max_papers + 2);
         pend(PG_K_CSR_APPEND);
         PG_HIP(hipGetLastError());
         ctx->cur = nxt;
+        if (one_sync) {
+            PG_HIP(hipMemcpyAsync(&tot, ctx->new_off + D, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+            PG_HIP(hipMemcpyAsync(hs, sums, sizeof(hs), hipMemcpyDeviceToHost, s));
+        }
     }
     PG_HIP(hipEventRecord(ctx->ev1, s));
     {
@@ -434,6 +448,12 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
         if (rc) return rc;
     }
     PG_HIP(hipEventSynchronize(ctx->ev1));
+    if (D > 0) {
+        e_year = tot;
+        near_total = (int64_t)hs[0];
+        attempts = (int64_t)hs[1];
+        bfs_bytes = (double)hs[2];
+    }
     float ms = 0.0f;
     PG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     if (ctx->prof && D > 0) {
