@@ -238,6 +238,11 @@ int pg_checkpoint_load(pg_ctx* ctx, const char* path, void* user, int64_t user_c
 int pg_profile_enable(pg_ctx* ctx, int on);
 int pg_profile_read(pg_ctx* ctx, int n, double* ms, double* bytes, int64_t* launches);
 int pg_profile_reset(pg_ctx* ctx);
+/* Restricts the timing to the groups whose bit (1u << PG_K_...) is set in
+ * mask (default: all).  Each event pair between kernels costs the stream a
+ * few microseconds, so a caller timing whole steps may keep only the group
+ * it reports.  No reference counterpart (the reference times whole stages). */
+int pg_profile_groups(pg_ctx* ctx, uint32_t mask);
 
 /*
  * ---- Citation sequences and the PA likelihood ----

@@ -160,6 +160,7 @@ def lib() -> ctypes.CDLL:
             "pg_checkpoint_save": (ctypes.c_int, [vp, ctypes.c_char_p, vp, ctypes.c_int64]),
             "pg_checkpoint_load": (ctypes.c_int, [vp, ctypes.c_char_p, vp, ctypes.c_int64, i64p]),
             "pg_profile_enable": (ctypes.c_int, [vp, ctypes.c_int]),
+            "pg_profile_groups": (ctypes.c_int, [vp, ctypes.c_uint32]),
             "pg_profile_read": (ctypes.c_int, [vp, ctypes.c_int, f64p, f64p, i64p]),
             "pg_profile_reset": (ctypes.c_int, [vp]),
             "pg_seed_graph": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int32,
@@ -508,6 +509,11 @@ class Context:
 
     def profile(self, on: bool = True):
         self._check(lib().pg_profile_enable(self._h, 1 if on else 0), "pg_profile_enable")
+
+    def profile_groups(self, names=None):
+        """Time only the named kernel groups while profiling (None: all)."""
+        mask = 0xFFFFFFFF if names is None else sum(1 << KERNEL_GROUPS.index(n) for n in names)
+        self._check(lib().pg_profile_groups(self._h, mask), "pg_profile_groups")
 
     def profile_reset(self):
         self._check(lib().pg_profile_reset(self._h), "pg_profile_reset")

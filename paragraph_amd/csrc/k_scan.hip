@@ -278,12 +278,18 @@ __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
     for (; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-__global__ void k_sum_i32(const int32_t* __restrict__ p, int64_t n, unsigned long long* out) {
+// Sum into *out: one atomic per block (a wave-level atomic per 64 lanes
+// queued thousands of atomics on one address).
+__global__ __launch_bounds__(256) void k_sum_i32(const int32_t* __restrict__ p, int64_t n,
+                                                 unsigned long long* out) {
+    __shared__ unsigned long long part[4];
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long acc = 0;
     for (; i < n; i += (int64_t)gridDim.x * blockDim.x) acc += (unsigned long long)(int64_t)p[i];
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
 }
 
 __global__ void k_acc_i64(const int64_t* __restrict__ src, unsigned long long* dst) {
@@ -341,8 +347,8 @@ void launch_fill_i32(hipStream_t s, int32_t* p, int32_t v, int64_t n) {
 
 void launch_sum_i32(hipStream_t s, const int32_t* p, int64_t n, unsigned long long* out) {
     if (n <= 0) return;
-    int64_t nb = (n + 255) / 256;
-    if (nb > 1024) nb = 1024;
+    int64_t nb = (n + 1023) / 1024;
+    if (nb > 256) nb = 256;
     hipLaunchKernelGGL(k_sum_i32, dim3((uint32_t)nb), dim3(256), 0, s, p, n, out);
 }
 

@@ -335,11 +335,12 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
     const uint32_t yi = (uint32_t)p->year_index;
     bool used[PG_K_COUNT] = {};
     bool used2[PG_K_COUNT] = {};
-    auto pbeg = [&](int k) { if (ctx->prof) (void)hipEventRecord(ctx->pev[k][0], s); };
-    auto pend = [&](int k) { if (ctx->prof) { (void)hipEventRecord(ctx->pev[k][1], s); used[k] = true; } };
+    auto on = [&](int k) { return ctx->prof && ((ctx->prof_mask >> k) & 1u); };
+    auto pbeg = [&](int k) { if (on(k)) (void)hipEventRecord(ctx->pev[k][0], s); };
+    auto pend = [&](int k) { if (on(k)) { (void)hipEventRecord(ctx->pev[k][1], s); used[k] = true; } };
     // second interval of a group within the same step (NB_DRAW: dedup, then draws)
-    auto pbeg2 = [&](int k) { if (ctx->prof) (void)hipEventRecord(ctx->pev2[k][0], s); };
-    auto pend2 = [&](int k) { if (ctx->prof) { (void)hipEventRecord(ctx->pev2[k][1], s); used2[k] = true; } };
+    auto pbeg2 = [&](int k) { if (on(k)) (void)hipEventRecord(ctx->pev2[k][0], s); };
+    auto pend2 = [&](int k) { if (on(k)) { (void)hipEventRecord(ctx->pev2[k][1], s); used2[k] = true; } };
     // PA years whose bound D*m fits max_edges run without a host round trip
     // mid-year: the append sorts D*m keys (the unfilled tail padded with key
     // N+D, past every paper) and the merge reads its edge total on the device
@@ -565,6 +566,12 @@ int pg_snapshot_restore(pg_ctx* ctx) {
 int pg_profile_enable(pg_ctx* ctx, int on) {
     if (!ctx) return PG_EINVAL;
     ctx->prof = on != 0;
+    return PG_OK;
+}
+
+int pg_profile_groups(pg_ctx* ctx, uint32_t mask) {
+    if (!ctx) return PG_EINVAL;
+    ctx->prof_mask = mask;
     return PG_OK;
 }
 

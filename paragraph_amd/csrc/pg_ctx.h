@@ -104,6 +104,7 @@ struct pg_ctx {
     int last_m = 0, last_model = -1;
     // per-kernel-group profiling
     bool prof = false;
+    uint32_t prof_mask = 0xFFFFFFFFu;  // groups timed while prof is on
     hipEvent_t pev[PG_K_COUNT][2] = {};
     hipEvent_t pev2[PG_K_COUNT][2] = {};
     double prof_ms[PG_K_COUNT] = {};

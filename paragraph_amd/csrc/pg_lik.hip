@@ -246,7 +246,8 @@ int lik_eval(pg_ctx* ctx, const pg_lik_params* p, pg_lik_out* out, bool grad) {
     }
     hipStream_t s = p->stream_given ? static_cast<hipStream_t>(p->stream) : ctx->stream;
     const int grp = grad ? PG_K_LIK_GRAD : PG_K_LIK;
-    if (ctx->prof) PG_HIP(hipEventRecord(ctx->pev[grp][0], s));
+    const bool prof = ctx->prof && ((ctx->prof_mask >> grp) & 1u);
+    if (prof) PG_HIP(hipEventRecord(ctx->pev[grp][0], s));
     const LikSeq v = seq_view(q);
     LikArgs a;
     a.alpha = p->alpha;
@@ -268,7 +269,7 @@ int lik_eval(pg_ctx* ctx, const pg_lik_params* p, pg_lik_out* out, bool grad) {
     launch_lik_final(s, v.n_years, grad, nP, part_b, a, q.year_sums, q.d_scalars,
                      out->scalars_dev);
     PG_HIP(hipGetLastError());
-    if (ctx->prof) {
+    if (prof) {
         PG_HIP(hipEventRecord(ctx->pev[grp][1], s));
         PG_HIP(hipEventSynchronize(ctx->pev[grp][1]));
         float ms = 0.0f;
