@@ -1506,6 +1506,7 @@ template <bool BATCH>
 __global__ __launch_bounds__(256) void k_nb_draw(Batch bt, int64_t n_exist, int64_t n_new, int m,
                                                  int n_nb, int cap, uint64_t seed, uint32_t yi,
                                                  const double* __restrict__ cdf,
+                                                 const int32_t* __restrict__ cidx,
                                                  const int32_t* __restrict__ root_slot,
                                                  const int32_t* __restrict__ S_buf,
                                                  const uint64_t* __restrict__ Q_buf,
@@ -1638,13 +1639,14 @@ __global__ __launch_bounds__(256) void k_nb_draw(Batch bt, int64_t n_exist, int6
     if (live && gl == 0) {
         const double Zg = cdf[n_exist - 1];
         const double eps = kNearRel * Zg;
+        const double kscale = cdf_key_scale(Zg);
         const int a_max = kAttemptsPerCite * m;
         int a = anext[i];
         int nr = 0;
         while (c < m && a < a_max) {
             u32x4 r = philox(gid, (uint32_t)a, yi, TAG_PA, seed);
             const double t = u53(r.x, r.y) * Zg;
-            const int64_t k = cdf_upper_bound(cdf, n_exist, t);
+            const int64_t k = cdf_search(cdf, cidx, n_exist, t, kscale);
             if (cdf_near(cdf, n_exist, k, t, eps)) ++nr;
             ++a;
             const int32_t x = (int32_t)(k + base);
@@ -1698,14 +1700,15 @@ void launch_bfs_root(hipStream_t s, const CsrView& g, const float* w, const int3
 }
 
 void launch_nb_draw(hipStream_t s, int64_t n_exist, int64_t n_new, int m, int n_nb, int cap,
-                    uint64_t seed, uint32_t year_index, const double* cdf, const int32_t* root_slot,
-                    const int32_t* S_buf, const uint64_t* Q_buf, const int32_t* ns_buf,
-                    int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* near, int32_t* anext) {
+                    uint64_t seed, uint32_t year_index, const double* cdf, const int32_t* cdf_idx,
+                    const int32_t* root_slot, const int32_t* S_buf, const uint64_t* Q_buf,
+                    const int32_t* ns_buf, int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* near,
+                    int32_t* anext) {
     if (n_new <= 0 || n_nb <= 0) return;
     const int64_t nb = (n_new + 256 / kNbG - 1) / (256 / kNbG);
     hipLaunchKernelGGL(k_nb_draw<false>, dim3((uint32_t)nb), dim3(256), 0, s, Batch{}, n_exist,
-                       n_new, m, n_nb, cap, seed, year_index, cdf, root_slot, S_buf, Q_buf, ns_buf,
-                       cit, kind, cnt, near, anext);
+                       n_new, m, n_nb, cap, seed, year_index, cdf, cdf_idx, root_slot, S_buf,
+                       Q_buf, ns_buf, cit, kind, cnt, near, anext);
 }
 
 void launch_nb_draw_batch(hipStream_t s, const Batch& b, int m, int cap, uint32_t year_index,
@@ -1715,8 +1718,8 @@ void launch_nb_draw_batch(hipStream_t s, const Batch& b, int m, int cap, uint32_
     if (b.n_new <= 0) return;
     const int64_t nb = (b.n_new + 256 / kNbG - 1) / (256 / kNbG);
     hipLaunchKernelGGL(k_nb_draw<true>, dim3((uint32_t)nb), dim3(256), 0, s, b, (int64_t)0,
-                       b.n_new, m, 0, cap, (uint64_t)0, year_index, cdf, root_slot, S_buf, Q_buf,
-                       ns_buf, cit, kind, cnt, near, anext);
+                       b.n_new, m, 0, cap, (uint64_t)0, year_index, cdf, (const int32_t*)nullptr,
+                       root_slot, S_buf, Q_buf, ns_buf, cit, kind, cnt, near, anext);
 }
 
 BfsStage bfs_stage_view(const BfsStage& pool, int mode) {

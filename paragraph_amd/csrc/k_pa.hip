@@ -23,7 +23,8 @@ namespace {
 // and n_pa; picks are written as global ids (region base + local id).
 template <int G, bool BATCH>
 __global__ __launch_bounds__(256) void k_pa_draw(Batch bt, int64_t n_exist,
-                                                 const double* __restrict__ cdf, int64_t n_new,
+                                                 const double* __restrict__ cdf,
+                                                 const int32_t* __restrict__ cidx, int64_t n_new,
                                                  int m, int n_pa, uint64_t seed, uint32_t yi,
                                                  int a_max, int32_t* __restrict__ cit,
                                                  int8_t* __restrict__ kind, int32_t* __restrict__ cnt,
@@ -54,6 +55,7 @@ __global__ __launch_bounds__(256) void k_pa_draw(Batch bt, int64_t n_exist,
     const uint32_t gid = (uint32_t)(n_exist + local);
     const double Z = cdf[n_exist - 1];
     const double eps = kNearRel * Z;
+    const double kscale = cdf_key_scale(Z);
 
     int c = 0, a0 = 0, nr = 0, a_stop = a_max;
     bool need = live && n_pa > 0;
@@ -65,7 +67,7 @@ __global__ __launch_bounds__(256) void k_pa_draw(Batch bt, int64_t n_exist,
         if (valid) {
             u32x4 r = philox(gid, (uint32_t)a, yi, TAG_PA, seed);
             double t = u53(r.x, r.y) * Z;
-            int64_t k = cdf_upper_bound(cdf, n_exist, t);
+            int64_t k = cdf_search(cdf, cidx, n_exist, t, kscale);
             nf = cdf_near(cdf, n_exist, k, t, eps);
             pick = (int32_t)k;
         }
@@ -122,19 +124,20 @@ __global__ __launch_bounds__(256) void k_pa_draw(Batch bt, int64_t n_exist,
 
 template <bool BATCH>
 void pa_draw_g(hipStream_t s, int G, const Batch& bt, int64_t n_exist, const double* cdf,
-               int64_t n_new, int m, int n_pa, uint64_t seed, uint32_t year_index, int32_t* cit,
+               const int32_t* cidx, int64_t n_new, int m, int n_pa, uint64_t seed,
+               uint32_t year_index, int32_t* cit,
                int8_t* kind, int32_t* cnt, int32_t* near, int32_t* anext) {
     const int a_max = kAttemptsPerCite * m;
     const int64_t per = 256 / G;
     const dim3 grid((uint32_t)((n_new + per - 1) / per)), block(256);
     switch (G) {
-        case 8: hipLaunchKernelGGL((k_pa_draw<8, BATCH>), grid, block, 0, s, bt, n_exist, cdf, n_new,
+        case 8: hipLaunchKernelGGL((k_pa_draw<8, BATCH>), grid, block, 0, s, bt, n_exist, cdf, cidx, n_new,
                                    m, n_pa, seed, year_index, a_max, cit, kind, cnt, near, anext); break;
-        case 16: hipLaunchKernelGGL((k_pa_draw<16, BATCH>), grid, block, 0, s, bt, n_exist, cdf, n_new,
+        case 16: hipLaunchKernelGGL((k_pa_draw<16, BATCH>), grid, block, 0, s, bt, n_exist, cdf, cidx, n_new,
                                     m, n_pa, seed, year_index, a_max, cit, kind, cnt, near, anext); break;
-        case 32: hipLaunchKernelGGL((k_pa_draw<32, BATCH>), grid, block, 0, s, bt, n_exist, cdf, n_new,
+        case 32: hipLaunchKernelGGL((k_pa_draw<32, BATCH>), grid, block, 0, s, bt, n_exist, cdf, cidx, n_new,
                                     m, n_pa, seed, year_index, a_max, cit, kind, cnt, near, anext); break;
-        default: hipLaunchKernelGGL((k_pa_draw<64, BATCH>), grid, block, 0, s, bt, n_exist, cdf, n_new,
+        default: hipLaunchKernelGGL((k_pa_draw<64, BATCH>), grid, block, 0, s, bt, n_exist, cdf, cidx, n_new,
                                     m, n_pa, seed, year_index, a_max, cit, kind, cnt, near, anext); break;
     }
 }
@@ -143,20 +146,22 @@ void pa_draw_g(hipStream_t s, int G, const Batch& bt, int64_t n_exist, const dou
 
 int pa_draw_lanes(int n_pa) { return n_pa <= 8 ? 8 : n_pa <= 16 ? 16 : n_pa <= 32 ? 32 : 64; }
 
-void launch_pa_draw(hipStream_t s, int64_t n_exist, const double* cdf, int64_t n_new, int m,
-                    int n_pa, uint64_t seed, uint32_t year_index, int32_t* cit, int8_t* kind,
-                    int32_t* cnt, int32_t* near, int32_t* anext, int lanes) {
+void launch_pa_draw(hipStream_t s, int64_t n_exist, const double* cdf, const int32_t* cdf_idx,
+                    int64_t n_new, int m, int n_pa, uint64_t seed, uint32_t year_index,
+                    int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* near, int32_t* anext,
+                    int lanes) {
     if (n_new <= 0) return;
     const int G = lanes > 0 ? lanes : pa_draw_lanes(n_pa);
-    pa_draw_g<false>(s, G, Batch{}, n_exist, cdf, n_new, m, n_pa, seed, year_index, cit, kind, cnt,
-                     near, anext);
+    pa_draw_g<false>(s, G, Batch{}, n_exist, cdf, cdf_idx, n_new, m, n_pa, seed, year_index, cit,
+                     kind, cnt, near, anext);
 }
 
 void launch_pa_draw_batch(hipStream_t s, const Batch& b, int m, uint32_t year_index, int lanes,
                           const double* cdf, int32_t* cit, int8_t* kind, int32_t* cnt,
                           int32_t* near, int32_t* anext) {
     if (b.n_new <= 0) return;
-    pa_draw_g<true>(s, lanes, b, 0, cdf, b.n_new, m, 0, 0, year_index, cit, kind, cnt, near, anext);
+    pa_draw_g<true>(s, lanes, b, 0, cdf, nullptr, b.n_new, m, 0, 0, year_index, cit, kind, cnt,
+                    near, anext);
 }
 
 }  // namespace pg

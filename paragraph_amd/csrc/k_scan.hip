@@ -292,6 +292,19 @@ __global__ __launch_bounds__(256) void k_sum_i32(const int32_t* __restrict__ p, 
     if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
 }
 
+// Bucket index of a CDF (see cdf_search): thread i <= n writes idx[k] = i for
+// the buckets k in (key(cdf[i-1]), key(cdf[i])] (i = 0 from bucket 0, i = n
+// up to kCdfIdx + 1), so every bucket is written once.
+__global__ __launch_bounds__(256) void k_cdf_index(const double* __restrict__ cdf, int64_t n,
+                                                   int32_t* __restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    const double kscale = cdf_key_scale(cdf[n - 1]);
+    const int k0 = i == 0 ? 0 : cdf_key(cdf[i - 1], kscale) + 1;
+    const int k1 = i == n ? kCdfIdx + 1 : cdf_key(cdf[i], kscale);
+    for (int k = k0; k <= k1; ++k) idx[k] = (int32_t)i;
+}
+
 __global__ void k_acc_i64(const int64_t* __restrict__ src, unsigned long long* dst) {
     if (threadIdx.x == 0) *dst += (unsigned long long)*src;
 }
@@ -304,7 +317,7 @@ void launch_acc_i64(hipStream_t s, const int64_t* src, unsigned long long* dst) 
 
 void launch_weights_cdf(hipStream_t s, int64_t n, const int32_t* in_deg, const int32_t* birth,
                         const float* logfit, int32_t year, float alpha, float inv_tau, float* w,
-                        double* cdf, double* tile_scratch) {
+                        double* cdf, double* tile_scratch, int32_t* cdf_idx) {
     if (n <= 0) return;
     int64_t nb = (n + kTile - 1) / kTile;
     const Batch none{};
@@ -313,6 +326,9 @@ void launch_weights_cdf(hipStream_t s, int64_t n, const int32_t* in_deg, const i
     hipLaunchKernelGGL(k_tile_chain<false>, dim3(1), dim3(256), 0, s, none, tile_scratch, nb);
     hipLaunchKernelGGL(k_cdf_down<false>, dim3((uint32_t)nb), dim3(kScanThreads), 0, s, none, n, w,
                        tile_scratch, cdf);
+    if (cdf_idx)
+        hipLaunchKernelGGL(k_cdf_index, dim3((uint32_t)((n + 1 + 255) / 256)), dim3(256), 0, s, cdf,
+                           n, cdf_idx);
 }
 
 void launch_weights_cdf_batch(hipStream_t s, const Batch& b, const int32_t* in_deg,

@@ -33,7 +33,8 @@ std::string g_create_err;
 
 void release(pg_ctx* c) {
     void* ptrs[] = {c->birth, c->in_deg, c->logfit, c->fwd_off, c->fwd_adj, c->bwd_off[0],
-                    c->bwd_off[1], c->bwd_adj[0], c->bwd_adj[1], c->w, c->cdf, c->scan_agg,
+                    c->bwd_off[1], c->bwd_adj[0], c->bwd_adj[1], c->w, c->cdf, c->cdf_idx,
+                    c->scan_agg,
                     c->cit, c->kind, c->cnt,
                     c->near, c->anext, c->new_off, c->src_of, c->k_sorted,
                     c->v_sorted, c->k_tmp, c->v_tmp, c->scratch, c->d_sum, c->d_tmp_i32,
@@ -201,6 +202,7 @@ int pg_ctx_create(const pg_config* cfg, pg_ctx** out) {
     }
     CREATE_HIP(dalloc(&ctx->w, N));
     CREATE_HIP(dalloc(&ctx->cdf, N));
+    CREATE_HIP(dalloc(&ctx->cdf_idx, kCdfIdx + 2));
     CREATE_HIP(dalloc(&ctx->scan_agg, tiles));
     CREATE_HIP(dalloc(&ctx->cit, D * M));
     CREATE_HIP(dalloc(&ctx->kind, D * M));
@@ -364,11 +366,11 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
             const int m = p->m, n_nb = p->n_nb;
             pbeg(PG_K_WEIGHTS_CDF);
             launch_weights_cdf(s, N, ctx->in_deg, ctx->birth, ctx->logfit, p->year, p->alpha,
-                               p->inv_tau, ctx->w, ctx->cdf, ctx->scan_agg);
+                               p->inv_tau, ctx->w, ctx->cdf, ctx->scan_agg, ctx->cdf_idx);
             pend(PG_K_WEIGHTS_CDF);
             pbeg(PG_K_PA_DRAW);
-            launch_pa_draw(s, N, ctx->cdf, D, m, m - n_nb, seed, yi, ctx->cit, ctx->kind, ctx->cnt,
-                           ctx->near, ctx->anext, ctx->tune_pa_lanes);
+            launch_pa_draw(s, N, ctx->cdf, ctx->cdf_idx, D, m, m - n_nb, seed, yi, ctx->cit,
+                           ctx->kind, ctx->cnt, ctx->near, ctx->anext, ctx->tune_pa_lanes);
             pend(PG_K_PA_DRAW);
             if (n_nb > 0) {
                 {
@@ -389,7 +391,7 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
                 pend(PG_K_BFS);
                 pbeg2(PG_K_NB_DRAW);
                 launch_nb_draw(s, N, D, m, n_nb, ctx->cfg.bfs_cap, seed, yi, ctx->cdf,
-                               ctx->root_slot, ctx->S_buf, ctx->Q_buf, ctx->ns_buf, ctx->cit,
+                               ctx->cdf_idx, ctx->root_slot, ctx->S_buf, ctx->Q_buf, ctx->ns_buf, ctx->cit,
                                ctx->kind, ctx->cnt, ctx->near, ctx->anext);
                 pend2(PG_K_NB_DRAW);
             }

@@ -83,6 +83,7 @@ struct pg_ctx {
     int cur = 0;
     float* w = nullptr;
     double* cdf = nullptr;
+    int32_t* cdf_idx = nullptr;  // kCdfIdx + 2 buckets of the current CDF
     double* scan_agg = nullptr;
     int32_t* cit = nullptr;
     int8_t* kind = nullptr;

@@ -24,6 +24,38 @@ __device__ __forceinline__ int64_t cdf_upper_bound(const double* __restrict__ cd
     return lo < n ? lo : n - 1;
 }
 
+// CDF bucket index: bucket of a CDF value or target t is
+// floor(t * kCdfIdx / Z) clamped to [0, kCdfIdx], monotone in t (one
+// rounding of a product by a positive constant), and idx[k] is the first
+// paper whose bucket is >= k (idx[kCdfIdx + 1] = n).  Every paper below
+// idx[key(t)] has cdf <= t and paper idx[key(t) + 1] has cdf > t, so the
+// first paper with cdf > t lies in [idx[key(t)], idx[key(t) + 1]]: a
+// search there returns what the full binary search returns.
+constexpr int kCdfIdx = 1 << 16;
+
+__device__ __forceinline__ double cdf_key_scale(double Z) {
+    return Z > 0.0 ? (double)kCdfIdx / Z : 0.0;
+}
+
+__device__ __forceinline__ int cdf_key(double t, double kscale) {
+    const double x = t * kscale;
+    return x <= 0.0 ? 0 : (x >= (double)kCdfIdx ? kCdfIdx : (int)x);
+}
+
+// cdf_upper_bound through the index when idx is non-null.
+__device__ __forceinline__ int64_t cdf_search(const double* __restrict__ cdf,
+                                              const int32_t* __restrict__ idx, int64_t n, double t,
+                                              double kscale) {
+    if (!idx) return cdf_upper_bound(cdf, n, t);
+    const int k = cdf_key(t, kscale);
+    int64_t lo = idx[k], hi = idx[k + 1];
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (cdf[mid] > t) hi = mid; else lo = mid + 1;
+    }
+    return lo < n ? lo : n - 1;
+}
+
 __device__ __forceinline__ bool cdf_near(const double* __restrict__ cdf, int64_t n, int64_t k,
                                          double t, double eps) {
     return (k > 0 && t - cdf[k - 1] <= eps) || (k < n - 1 && cdf[k] - t <= eps);
@@ -75,9 +107,10 @@ __device__ __forceinline__ int batch_sim_of_tile(const Batch& b, int64_t t) {
 
 // w[i] = pa_weight(...) for i < n; cdf[i] = FP64 inclusive prefix of w.
 // tile_scratch: ceil(n/4096) doubles.
+// (cdf_idx: kCdfIdx + 2 int32, the bucket index of the new CDF; nullable)
 void launch_weights_cdf(hipStream_t s, int64_t n, const int32_t* in_deg, const int32_t* birth,
                         const float* logfit, int32_t year, float alpha, float inv_tau,
-                        float* w, double* cdf, double* tile_scratch);
+                        float* w, double* cdf, double* tile_scratch, int32_t* cdf_idx);
 
 // Batched form: per simulation of the year, w and the FP64 CDF over its
 // papers [base, base + n_exist), with its own alpha/inv_tau; tile_scratch:
@@ -94,9 +127,10 @@ void launch_excl_scan_i32(hipStream_t s, const int32_t* in, int64_t n, int64_t* 
 // PA phase 1: n_pa distinct draws per new paper from the global CDF.
 // lanes: attempts evaluated together per paper (8/16/32/64; 0 = by n_pa).
 int pa_draw_lanes(int n_pa);
-void launch_pa_draw(hipStream_t s, int64_t n_exist, const double* cdf, int64_t n_new, int m,
-                    int n_pa, uint64_t seed, uint32_t year_index, int32_t* cit, int8_t* kind,
-                    int32_t* cnt, int32_t* near, int32_t* anext, int lanes = 0);
+void launch_pa_draw(hipStream_t s, int64_t n_exist, const double* cdf, const int32_t* cdf_idx,
+                    int64_t n_new, int m, int n_pa, uint64_t seed, uint32_t year_index,
+                    int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* near, int32_t* anext,
+                    int lanes = 0);
 // Batched: every simulation's new papers of the year (cdf: the global FP64
 // CDF array; targets are written as global ids).
 void launch_pa_draw_batch(hipStream_t s, const Batch& b, int m, uint32_t year_index, int lanes,
@@ -155,9 +189,10 @@ void launch_bfs_root(hipStream_t s, const CsrView& g, const float* w, const int3
                      unsigned long long* bytes_out, const BfsStage& stage);
 // Per paper: neighbourhood draws from its root's S, then PA refill.
 void launch_nb_draw(hipStream_t s, int64_t n_exist, int64_t n_new, int m, int n_nb, int cap,
-                    uint64_t seed, uint32_t year_index, const double* cdf, const int32_t* root_slot,
-                    const int32_t* S_buf, const uint64_t* Q_buf, const int32_t* ns_buf,
-                    int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* near, int32_t* anext);
+                    uint64_t seed, uint32_t year_index, const double* cdf, const int32_t* cdf_idx,
+                    const int32_t* root_slot, const int32_t* S_buf, const uint64_t* Q_buf,
+                    const int32_t* ns_buf, int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* near,
+                    int32_t* anext);
 
 void launch_nb_draw_batch(hipStream_t s, const Batch& b, int m, int cap, uint32_t year_index,
                           const double* cdf, const int32_t* root_slot, const int32_t* S_buf,
