@@ -566,6 +566,30 @@ __device__ __forceinline__ void wave_sort_reg(int32_t* s) {
     for (int e = 0; e < E; ++e) s[e * 64 + lane] = v[e];
 }
 
+// One wavefront: the distinct ids of the sorted s[0,n) other than root, the
+// first limit of them into out (another array); returns their number,
+// capped.  Each lane reads only the elements it wrote in the sort.
+__device__ __forceinline__ int wave_unique_to(const int32_t* s, int n, int limit, int32_t root,
+                                              int32_t* out) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int cnt = 0;
+    int32_t carry = INT_MIN;  // last element of the previous chunk
+    for (int base = 0; base < n && cnt < limit; base += 64) {
+        const int i = base + lane;
+        const int32_t x = i < n ? s[i] : INT_MAX;
+        int32_t prev = __shfl_up(x, 1, 64);
+        if (lane == 0) prev = carry;
+        const bool k = i < n && x != prev && x != root;
+        const uint64_t mask = __ballot(k);
+        const int pos = cnt + __popcll(mask & below);
+        carry = __shfl(x, 63, 64);
+        if (k && pos < limit) out[pos] = x;
+        cnt += __popcll(mask);
+    }
+    return cnt < limit ? cnt : limit;
+}
+
 // Radix path, step 1: the level's row entries (rows cut as in stage_rows)
 // below lim, the root dropped, appended to A[0, kRadixMax) in visit order;
 // returns their number (those past the capacity are counted, not stored).
@@ -700,6 +724,38 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
     const int32_t n = (int32_t)q.g.n;
     const int nr = num_rows(q);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (q.level == 1) {
+        // a 1-hop list of at most kWaveSortMax entries: the first wavefront
+        // loads both rows, sorts them in registers and compacts them into
+        // L.f1; the others wait at one barrier
+        g_cint* fo = GROW(q.g.fwd_off);
+        g_cint* bo = GROW(q.g.bwd_off);
+        const int32_t fb = fo[q.root], fe = fo[q.root + 1], bb = bo[q.root], be = bo[q.root + 1];
+        const int df = fe - fb, d1 = df + (be - bb);
+        if (d1 <= kWaveSortMax) {
+            if (wid == 0) {
+                int P = 64;
+                while (P < d1) P <<= 1;
+                g_cint* fa = GROW(q.g.fwd_adj);
+                g_cint* ba = GROW(q.g.bwd_adj);
+                for (int k = lane; k < P; k += 64)
+                    L.win[k] = k < df ? fa[fb + k] : (k < d1 ? ba[bb + k - df] : INT_MAX);
+                switch (P) {
+                    case 64: wave_sort_reg<1>(L.win); break;
+                    case 128: wave_sort_reg<2>(L.win); break;
+                    case 256: wave_sort_reg<4>(L.win); break;
+                    default: wave_sort_reg<8>(L.win); break;
+                }
+                const int n1 = wave_unique_to(L.win, d1, cap, q.root, L.f1);
+                if (lane == 0) {
+                    L.nout = n1;
+                    L.tot[1] = d1;
+                }
+            }
+            __syncthreads();
+            return L.nout;
+        }
+    }
     if (threadIdx.x == 0) L.nlong = 0;
     __syncthreads();
     // row entries (bits 26 and up) and entries cut at cap+1 per row (bits
