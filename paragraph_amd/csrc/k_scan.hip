@@ -292,17 +292,19 @@ __global__ __launch_bounds__(256) void k_sum_i32(const int32_t* __restrict__ p, 
     if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
 }
 
-// Bucket index of a CDF (see cdf_search): thread i <= n writes idx[k] = i for
+// Bucket index of a CDF (see cdf_search): paper i <= n writes idx[k] = i for
 // the buckets k in (key(cdf[i-1]), key(cdf[i])] (i = 0 from bucket 0, i = n
-// up to kCdfIdx + 1), so every bucket is written once.
+// up to kCdfIdx + 1), so every bucket is written once.  A thread per paper;
+// cdf[i-1] comes from the neighbouring lane.
 __global__ __launch_bounds__(256) void k_cdf_index(const double* __restrict__ cdf, int64_t n,
                                                    int32_t* __restrict__ idx) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > n) return;
     const double kscale = cdf_key_scale(cdf[n - 1]);
-    const int k0 = i == 0 ? 0 : cdf_key(cdf[i - 1], kscale) + 1;
-    const int k1 = i == n ? kCdfIdx + 1 : cdf_key(cdf[i], kscale);
-    for (int k = k0; k <= k1; ++k) idx[k] = (int32_t)i;
+    const int kc = i < n ? cdf_key(cdf[i], kscale) : kCdfIdx + 1;
+    int kp = __shfl_up(kc, 1, 64);
+    if ((threadIdx.x & 63) == 0) kp = (i == 0 || i > n) ? -1 : cdf_key(cdf[i - 1], kscale);
+    if (i > n) return;
+    for (int k = kp + 1; k <= kc; ++k) idx[k] = (int32_t)i;
 }
 
 __global__ void k_acc_i64(const int64_t* __restrict__ src, unsigned long long* dst) {

@@ -44,6 +44,7 @@ void release(pg_ctx* c) {
                     c->root_slot, c->uroot, c->big_list, c->stage.buf, c->stage.busy};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    if (c->h_sum) (void)hipHostFree(c->h_sum);
     for (int k = 0; k < PG_K_COUNT; ++k)
         for (int j = 0; j < 2; ++j) {
             if (c->pev[k][j]) (void)hipEventDestroy(c->pev[k][j]);
@@ -210,7 +211,9 @@ int pg_ctx_create(const pg_config* cfg, pg_ctx** out) {
     CREATE_HIP(dalloc(&ctx->near, D));
     CREATE_HIP(dalloc(&ctx->anext, D));
     CREATE_HIP(dalloc(&ctx->new_off, D + 2));
-    CREATE_HIP(dalloc(&ctx->d_sum, 4));
+    CREATE_HIP(dalloc(&ctx->d_sum, 5));
+    CREATE_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->h_sum), 5 * sizeof(unsigned long long),
+                             hipHostMallocDefault));
     for (int k = 0; k < PG_K_COUNT; ++k)
         for (int j = 0; j < 2; ++j) {
             CREATE_HIP(hipEventCreate(&ctx->pev[k][j]));
@@ -360,8 +363,9 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
     unsigned long long hs[4] = {0, 0, 0, 0};
     if (D > 0) {
         launch_fill_i32(s, ctx->birth + N, p->year, D);
-        unsigned long long* sums = ctx->d_sum;  // [0] near, [1] attempts, [2] BFS bytes
-        PG_HIP(hipMemsetAsync(sums, 0, 4 * sizeof(unsigned long long), s));
+        // [0] near, [1] attempts, [2] BFS bytes, [3] tier counts, [4] edges
+        unsigned long long* sums = ctx->d_sum;
+        launch_fill_i32(s, reinterpret_cast<int32_t*>(sums), 0, 10);
         if (p->model == PG_MODEL_PA) {
             const int m = p->m, n_nb = p->n_nb;
             pbeg(PG_K_WEIGHTS_CDF);
@@ -404,9 +408,12 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
         launch_excl_scan_i32(s, ctx->cnt, D, ctx->new_off, ctx->scratch);
         PG_HIP(hipGetLastError());
         if (!one_sync) {
-            PG_HIP(hipMemcpyAsync(&tot, ctx->new_off + D, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-            PG_HIP(hipMemcpyAsync(hs, sums, sizeof(hs), hipMemcpyDeviceToHost, s));
+            launch_acc_i64(s, ctx->new_off + D, sums + 4);
+            PG_HIP(hipMemcpyAsync(ctx->h_sum, sums, 5 * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, s));
             PG_HIP(hipStreamSynchronize(s));
+            for (int k = 0; k < 4; ++k) hs[k] = ctx->h_sum[k];
+            tot = (int64_t)ctx->h_sum[4];
             e_year = tot;
             if (ctx->e + e_year > ctx->cfg.max_edges)
                 return fail(ctx, PG_ECAP, "pg_year_step: edges would exceed max_edges");
@@ -441,8 +448,9 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
         PG_HIP(hipGetLastError());
         ctx->cur = nxt;
         if (one_sync) {
-            PG_HIP(hipMemcpyAsync(&tot, ctx->new_off + D, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-            PG_HIP(hipMemcpyAsync(hs, sums, sizeof(hs), hipMemcpyDeviceToHost, s));
+            launch_acc_i64(s, ctx->new_off + D, sums + 4);
+            PG_HIP(hipMemcpyAsync(ctx->h_sum, sums, 5 * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, s));
         }
     }
     PG_HIP(hipEventRecord(ctx->ev1, s));
@@ -452,6 +460,10 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
     }
     PG_HIP(hipEventSynchronize(ctx->ev1));
     if (D > 0) {
+        if (one_sync) {
+            for (int k = 0; k < 4; ++k) hs[k] = ctx->h_sum[k];
+            tot = (int64_t)ctx->h_sum[4];
+        }
         e_year = tot;
         near_total = (int64_t)hs[0];
         attempts = (int64_t)hs[1];

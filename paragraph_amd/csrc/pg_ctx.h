@@ -99,6 +99,7 @@ struct pg_ctx {
     uint32_t* v_tmp = nullptr;
     int64_t* scratch = nullptr;
     unsigned long long* d_sum = nullptr;
+    unsigned long long* h_sum = nullptr;  // pinned host copy of d_sum
     int32_t* d_tmp_i32 = nullptr;
     int64_t d_tmp_cap = 0;
     int64_t last_n_new = 0, last_n_exist = 0;
