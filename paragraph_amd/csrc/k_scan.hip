@@ -273,6 +273,35 @@ __global__ __launch_bounds__(256) void k_scan_down(const int32_t* __restrict__ i
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = tile_off[gridDim.x];
 }
 
+// k_scan_tiles and k_scan_down in one launch for at most 256 tiles: each
+// block sums the tile sums before its own (one per thread) first.
+__global__ __launch_bounds__(256) void k_scan_down_fused(const int32_t* __restrict__ in, int64_t n,
+                                                         const int64_t* __restrict__ tile_sum,
+                                                         int64_t* __restrict__ out) {
+    int64_t off;
+    block_excl_scan_i64((int64_t)threadIdx.x < (int64_t)blockIdx.x ? tile_sum[threadIdx.x] : 0,
+                        &off);
+    const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * 16;
+    int32_t vals[16];
+    int64_t acc = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        int64_t i = base + j;
+        vals[j] = i < n ? in[i] : 0;
+        acc += vals[j];
+    }
+    int64_t tot;
+    int64_t ex = block_excl_scan_i64(acc, &tot);
+    int64_t run = off + ex;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        int64_t i = base + j;
+        if (i < n) out[i] = run;
+        run += vals[j];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = off + tot;
+}
+
 __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
@@ -352,6 +381,11 @@ void launch_excl_scan_i32(hipStream_t s, const int32_t* in, int64_t n, int64_t* 
     int64_t nt = (n + kTile - 1) / kTile;
     if (nt == 0) nt = 1;
     hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nt), dim3(256), 0, s, in, n, scratch);
+    if (nt <= 256) {
+        hipLaunchKernelGGL(k_scan_down_fused, dim3((uint32_t)nt), dim3(256), 0, s, in, n, scratch,
+                           out);
+        return;
+    }
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(256), 0, s, scratch, nt);
     hipLaunchKernelGGL(k_scan_down, dim3((uint32_t)nt), dim3(256), 0, s, in, n, scratch, out);
 }
