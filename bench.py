@@ -108,6 +108,47 @@ def degree_ks(gpu_in_deg, cpu_in_deg) -> dict:
             "papers_differing_in_degree": int(np.count_nonzero(gpu_in_deg != cpu_in_deg))}
 
 
+def abm_bench(args) -> dict:
+    """`abm` end to end on CSV files: a 2,000,000-paper Philox seed graph
+    (Poisson(10) citations) written as `node,year` and `citing,cited` CSVs,
+    then `abm` grows it 10 years at 3% (m=20 from the out-degree bag, 30%
+    neighbourhood) and writes the `#source,target` edgelist and the
+    15-column auxiliary CSV.  Reported: the process wall time and abm's own
+    read, growth and write times."""
+    import subprocess
+    import tempfile
+    import pandas as pd
+    with tempfile.TemporaryDirectory() as d:
+        g = pg.seed_graph(1, 2_000_000, 10.0)
+        pd.DataFrame({"a": np.arange(g.n) + 100, "b": g.birth.astype(np.int64) + 2000}).to_csv(
+            f"{d}/nodes.csv", header=False, index=False)
+        with open(f"{d}/edges.csv", "w") as f:
+            f.write("#source,target\n")
+        pd.DataFrame({"a": g.src.astype(np.int64) + 100, "b": g.dst.astype(np.int64) + 100}).to_csv(
+            f"{d}/edges.csv", header=False, index=False, mode="a")
+        open(f"{d}/bag.csv", "w").write("0,20\n")
+        open(f"{d}/rec.csv", "w").write("0,1\n")
+        cmd = [pg.ABM_PATH, "--edgelist", f"{d}/edges.csv", "--nodelist", f"{d}/nodes.csv",
+               "--out-degree-bag", f"{d}/bag.csv", "--recency-probabilities", f"{d}/rec.csv",
+               "--growth-rate", "0.03", "--num-cycles", "10", "--same-year-proportion", "0",
+               "--output-file", f"{d}/out.csv", "--auxiliary-information-file", f"{d}/aux.csv",
+               "--log-file", f"{d}/log.txt", "--seed", "1"]
+        t0 = time.perf_counter()
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - t0
+        if r.returncode:
+            raise RuntimeError(f"abm failed: {r.stderr[-500:]}")
+        out = {"metric": "abm wall time (2M-paper CSV seed, 10y PA, CSV outputs)", "value": wall,
+               "unit": "s", "higher_is_better": False,
+               "seed_edges": int(g.src.size), "edgelist_bytes": os.path.getsize(f"{d}/out.csv"),
+               "aux_bytes": os.path.getsize(f"{d}/aux.csv")}
+        for line in r.stdout.splitlines():
+            if line.startswith("Growth:") or line.startswith("I/O:"):
+                eprint(line)
+                out["growth_line" if line.startswith("Growth") else "io_line"] = line
+        return out
+
+
 def er_bench(args) -> dict:
     """ER G(n,p) growth: 1,000,000 isolated seed papers, 10 years at 3% per
     year, each new paper linked to each existing paper with p = 1e-5
@@ -517,6 +558,8 @@ def main():
     ap.add_argument("--bfs-mode", type=int, default=0,
                     help="2-hop workgroup path: 0 staged where it fits, 1 sweep only")
     ap.add_argument("--ens-reps", type=int, default=3)
+    ap.add_argument("--abm-only", action="store_true",
+                    help="only abm end to end on CSV files (its JSON object on stdout)")
     ap.add_argument("--lik-only", action="store_true",
                     help="only the likelihood part (its JSON object on stdout)")
     args = ap.parse_args()
@@ -531,6 +574,9 @@ def main():
         return
     if args.er_only:
         print(json.dumps(er_bench(args)), flush=True)
+        return
+    if args.abm_only:
+        print(json.dumps(abm_bench(args)), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

@@ -27,11 +27,13 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <exception>
 #include <map>
 #include <sstream>
 #include <stdexcept>
 #include <string>
-#include <unordered_map>
+#include <thread>
+#include <utility>
 #include <vector>
 
 #include "../../include/paragraph.h"
@@ -128,11 +130,30 @@ std::string slurp(const std::string& path) {
     return s;
 }
 
-// Splits `text` into lines and the first two delimiter-separated fields.
+// Host threads for parsing and formatting.
+int io_threads() {
+    const unsigned h = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(h, 16u));
+}
+
+// Runs fn(t) for t in [0, T) on T threads; rethrows the first failure in t
+// order.
 template <typename F>
-void for_each_pair(const std::string& text, char delim, F&& fn) {
-    const char* p = text.data();
-    const char* end = p + text.size();
+void parallel_for(int T, F&& fn) {
+    std::vector<std::exception_ptr> err(T);
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t)
+        th.emplace_back([&, t] {
+            try { fn(t); } catch (...) { err[t] = std::current_exception(); }
+        });
+    for (auto& x : th) x.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+}
+
+// Splits [p, end) into lines and the first two delimiter-separated fields.
+template <typename F>
+void for_each_pair(const char* p, const char* end, char delim, F&& fn) {
     while (p < end) {
         const char* le = p;
         while (le < end && *le != '\n' && *le != '\r') ++le;
@@ -152,6 +173,31 @@ void for_each_pair(const std::string& text, char delim, F&& fn) {
         p = le;
         while (p < end && (*p == '\n' || *p == '\r')) ++p;
     }
+}
+
+// The (first, second) field pairs of every line, in file order: T chunks cut
+// at line starts are parsed on T threads.
+std::vector<std::pair<long long, long long>> read_pairs(const std::string& path, int T) {
+    const char delim = detect_delimiter(path);
+    const std::string text = slurp(path);
+    std::vector<size_t> cut(T + 1, text.size());
+    cut[0] = 0;
+    for (int t = 1; t < T; ++t) {
+        size_t q = std::max(cut[t - 1], text.size() / T * t);
+        while (q < text.size() && q > 0 && text[q - 1] != '\n') ++q;
+        cut[t] = q;
+    }
+    std::vector<std::vector<std::pair<long long, long long>>> part(T);
+    parallel_for(T, [&](int t) {
+        for_each_pair(text.data() + cut[t], text.data() + cut[t + 1], delim,
+                      [&](long long a, long long b) { part[t].emplace_back(a, b); });
+    });
+    size_t tot = 0;
+    for (auto& v : part) tot += v.size();
+    std::vector<std::pair<long long, long long>> all;
+    all.reserve(tot);
+    for (auto& v : part) all.insert(all.end(), v.begin(), v.end());
+    return all;
 }
 
 struct Logger {
@@ -202,45 +248,64 @@ int main(int argc, char** argv) {
                 std::fprintf(stderr, "abm: %s is a parameter of the reference's score-mixing "
                                      "model and is not used by this growth model\n", k);
 
-        // ---- nodelist: dense ids in ascending original-id order ----
-        std::map<long long, int> node_year;
+        const int T = io_threads();
+        const auto t_read = Clock::now();
+        // ---- nodelist: dense ids in ascending original-id order (a repeated
+        // id keeps its last year) ----
+        std::vector<long long> orig;
+        std::vector<int32_t> birth;
         {
-            const std::string path = a.str("--nodelist");
-            const char d = detect_delimiter(path);
-            for_each_pair(slurp(path), d, [&](long long id, long long y) { node_year[id] = (int)y; });
+            auto nodes = read_pairs(a.str("--nodelist"), T);
+            std::stable_sort(nodes.begin(), nodes.end(),
+                             [](const auto& x, const auto& y) { return x.first < y.first; });
+            for (size_t i = 0; i < nodes.size(); ++i) {
+                if (i + 1 < nodes.size() && nodes[i + 1].first == nodes[i].first) continue;
+                orig.push_back(nodes[i].first);
+                birth.push_back((int32_t)nodes[i].second);
+            }
         }
-        if (node_year.empty()) throw std::runtime_error("empty nodelist");
-        const int64_t n0 = (int64_t)node_year.size();
-        std::vector<long long> orig(n0);
-        std::vector<int32_t> birth(n0);
-        std::unordered_map<long long, int32_t> seq;
-        seq.reserve((size_t)n0 * 2);
-        {
-            int32_t s = 0;
-            for (const auto& [id, y] : node_year) { orig[s] = id; birth[s] = y; seq[id] = s; ++s; }
-        }
+        if (orig.empty()) throw std::runtime_error("empty nodelist");
+        const int64_t n0 = (int64_t)orig.size();
         int max_year = birth[0];
         for (int32_t y : birth) max_year = std::max(max_year, y);
-        // ---- edgelist: edges between known papers ----
+        // dense index of an original id, -1 if unknown (direct when the ids
+        // form one range)
+        const bool dense = orig[n0 - 1] - orig[0] + 1 == n0;
+        auto index_of = [&](long long id) -> int32_t {
+            if (dense) return (id >= orig[0] && id <= orig[n0 - 1]) ? (int32_t)(id - orig[0]) : -1;
+            auto it = std::lower_bound(orig.begin(), orig.end(), id);
+            return (it != orig.end() && *it == id) ? (int32_t)(it - orig.begin()) : -1;
+        };
+        // ---- edgelist: edges between known papers, in file order ----
         std::vector<int32_t> src, dst;
         {
-            const std::string path = a.str("--edgelist");
-            const char d = detect_delimiter(path);
-            for_each_pair(slurp(path), d, [&](long long u, long long v) {
-                auto iu = seq.find(u), iv = seq.find(v);
-                if (iu != seq.end() && iv != seq.end() && iu->second != iv->second) {
-                    src.push_back(iu->second);
-                    dst.push_back(iv->second);
+            const auto edges = read_pairs(a.str("--edgelist"), T);
+            std::vector<std::vector<int32_t>> ps(T), pd(T);
+            parallel_for(T, [&](int t) {
+                const size_t b = edges.size() / T * t;
+                const size_t e = t + 1 == T ? edges.size() : edges.size() / T * (t + 1);
+                for (size_t k = b; k < e; ++k) {
+                    const int32_t iu = index_of(edges[k].first), iv = index_of(edges[k].second);
+                    if (iu >= 0 && iv >= 0 && iu != iv) {
+                        ps[t].push_back(iu);
+                        pd[t].push_back(iv);
+                    }
                 }
             });
+            for (int t = 0; t < T; ++t) {
+                src.insert(src.end(), ps[t].begin(), ps[t].end());
+                dst.insert(dst.end(), pd[t].begin(), pd[t].end());
+            }
         }
+        const double read_s = std::chrono::duration<double>(Clock::now() - t_read).count();
         // ---- citations per new paper (out-degree bag `idx,outdeg`) ----
         int m = (int)a.num("--citations-per-paper", -1);
         if (m < 0 && model == "pa") {
             std::vector<long long> bag;
             const std::string path = a.str("--out-degree-bag");
-            const char d = detect_delimiter(path);
-            for_each_pair(slurp(path), d, [&](long long, long long k) { bag.push_back(k); });
+            const std::string text = slurp(path);
+            for_each_pair(text.data(), text.data() + text.size(), detect_delimiter(path),
+                          [&](long long, long long k) { bag.push_back(k); });
             if (bag.empty()) throw std::runtime_error("empty out-degree bag");
             for (long long k : bag)
                 if (k != bag[0])
@@ -365,47 +430,68 @@ int main(int argc, char** argv) {
         long long next_id = orig[n0 - 1] + 1;
         orig.resize(nn);
         for (int64_t v = n0; v < nn; ++v) orig[v] = next_id++;
+        const auto t_write = Clock::now();
         {
-            std::string buf;
-            buf.reserve((size_t)ne * 16 + 32);
-            buf += "#source,target\n";
-            for (int64_t v = 0; v < nn; ++v) {
-                for (int32_t k = fo[v]; k < fo[v + 1]; ++k) {
-                    append_int(buf, orig[v]);
-                    buf.push_back(',');
-                    append_int(buf, orig[fa[k]]);
-                    buf.push_back('\n');
-                }
+            // rows of papers [pb[t], pb[t+1]) formatted by thread t, about
+            // ne / T edges each
+            std::vector<int64_t> pb(T + 1, nn);
+            pb[0] = 0;
+            for (int t = 1; t < T; ++t) {
+                const int64_t target = ne / T * t;
+                pb[t] = std::max<int64_t>(pb[t - 1],
+                                          std::upper_bound(fo.begin(), fo.begin() + nn, target) -
+                                              fo.begin() - 1);
             }
+            std::vector<std::string> part(T);
+            parallel_for(T, [&](int t) {
+                std::string& buf = part[t];
+                buf.reserve((size_t)(fo[pb[t + 1]] - fo[pb[t]]) * 16 + 16);
+                for (int64_t v = pb[t]; v < pb[t + 1]; ++v) {
+                    for (int32_t k = fo[v]; k < fo[v + 1]; ++k) {
+                        append_int(buf, orig[v]);
+                        buf.push_back(',');
+                        append_int(buf, orig[fa[k]]);
+                        buf.push_back('\n');
+                    }
+                }
+            });
             std::ofstream out(a.str("--output-file"), std::ios::binary);
-            out.write(buf.data(), (std::streamsize)buf.size());
+            out << "#source,target\n";
+            for (const auto& b : part) out.write(b.data(), (std::streamsize)b.size());
             if (!out) throw std::runtime_error("cannot write --output-file");
         }
         {
-            std::string buf;
-            buf += "node_id,type,year,pa_weight,rec_weight,fit_weight,fit_lag_duration,"
+            std::vector<std::string> part(T);
+            parallel_for(T, [&](int t) {
+                const int64_t b = nn / T * t, e = t + 1 == T ? nn : nn / T * (t + 1);
+                std::string& buf = part[t];
+                buf.reserve((size_t)(e - b) * 64);
+                char row[320];
+                for (int64_t v = b; v < e; ++v) {
+                    const int outd = fo[v + 1] - fo[v];
+                    int len;
+                    if (v < n0) {
+                        len = std::snprintf(row, sizeof(row), "%lld,seed,%d,-1,-1,-1,0,0,0,-1,%d,%d,-1,-1,no_generators\n",
+                                            orig[v], by[v], indeg[v], outd);
+                    } else {
+                        const int32_t r = roots[v - n0];
+                        const long long ro = r >= 0 ? orig[r] : -1;
+                        len = std::snprintf(row, sizeof(row), "%lld,agent,%d,%.10g,%.10g,%.10g,0,0,0,%.10g,%d,%d,%d,-1,%lld\n",
+                                            orig[v], by[v], alpha, tau, std::exp((double)logfit[v]),
+                                            f_nb, indeg[v], outd, m, ro);
+                    }
+                    buf.append(row, (size_t)len);
+                }
+            });
+            std::ofstream out(a.str("--auxiliary-information-file"), std::ios::binary);
+            out << "node_id,type,year,pa_weight,rec_weight,fit_weight,fit_lag_duration,"
                    "fit_peak_value,fit_peak_duration,alpha,in_degree,out_degree,"
                    "assigned_out_degree,planted_nodes_line_number,generator_node_string\n";
-            char row[320];
-            for (int64_t v = 0; v < nn; ++v) {
-                const int outd = fo[v + 1] - fo[v];
-                int len;
-                if (v < n0) {
-                    len = std::snprintf(row, sizeof(row), "%lld,seed,%d,-1,-1,-1,0,0,0,-1,%d,%d,-1,-1,no_generators\n",
-                                        orig[v], by[v], indeg[v], outd);
-                } else {
-                    const int32_t r = roots[v - n0];
-                    const long long ro = r >= 0 ? orig[r] : -1;
-                    len = std::snprintf(row, sizeof(row), "%lld,agent,%d,%.10g,%.10g,%.10g,0,0,0,%.10g,%d,%d,%d,-1,%lld\n",
-                                        orig[v], by[v], alpha, tau, std::exp((double)logfit[v]),
-                                        f_nb, indeg[v], outd, m, ro);
-                }
-                buf.append(row, (size_t)len);
-            }
-            std::ofstream out(a.str("--auxiliary-information-file"), std::ios::binary);
-            out.write(buf.data(), (std::streamsize)buf.size());
+            for (const auto& b : part) out.write(b.data(), (std::streamsize)b.size());
             if (!out) throw std::runtime_error("cannot write --auxiliary-information-file");
         }
+        const double write_s = std::chrono::duration<double>(Clock::now() - t_write).count();
+        std::printf("I/O: read %.3f s, wrote %.3f s (%d threads)\n", read_s, write_s, T);
         log.info("wrote " + std::to_string(ne) + " edges for " + std::to_string(nn) + " papers");
         return 0;
     } catch (const std::exception& e) {
