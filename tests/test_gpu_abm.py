@@ -69,3 +69,48 @@ def test_abm_writes_reference_formats_and_matches_oracle(tmp_path):
     assert row[1] == "seed" and row[-1] == "no_generators" and len(row) == 15
     row = aux[-1].split(",")
     assert row[1] == "agent" and len(row) == 15 and int(row[11]) == m
+
+
+def test_abm_input_forms_equal_clean_input(tmp_path):
+    """CRLF line ends, space delimiters, a repeated node (its last year
+    counts), a self-citation and an unknown id read as the clean files do:
+    both runs write the same bytes."""
+    d = str(tmp_path)
+    n0 = 2000
+    g = pg.seed_graph(8, n0, 5.0)
+    ids = np.arange(n0) + 50
+    j = lambda f: os.path.join(d, f)
+    with open(j("nodes.csv"), "w") as f:
+        for i in range(n0):
+            f.write(f"{ids[i]},{g.birth[i] + 2000}\n")
+    with open(j("edges.csv"), "w") as f:
+        for s, t in zip(g.src, g.dst):
+            f.write(f"{ids[s]},{ids[t]}\n")
+    with open(j("nodes_q.txt"), "w", newline="") as f:
+        f.write("# node year\r\n")
+        f.write(f"{ids[0]} 1990\r\n")                    # overridden below
+        for i in range(n0):
+            f.write(f"{ids[i]} {g.birth[i] + 2000}\r\n")
+    with open(j("edges_q.txt"), "w", newline="") as f:
+        f.write(f"{ids[10]} {ids[10]}\r\n")              # self-citation: dropped
+        for s, t in zip(g.src, g.dst):
+            f.write(f"{ids[s]} {ids[t]}\r\n")
+        f.write(f"5 {ids[0]}\r\n")                       # unknown id: dropped
+    open(j("bag.csv"), "w").write("0,6\n")
+    open(j("rec.csv"), "w").write("0,1\n")
+
+    def run(nodes, edges, tag):
+        args = [pg.ABM_PATH, "--edgelist", j(edges), "--nodelist", j(nodes),
+                "--out-degree-bag", j("bag.csv"), "--recency-probabilities", j("rec.csv"),
+                "--growth-rate", "0.05", "--num-cycles", "2", "--same-year-proportion", "0",
+                "--output-file", j(f"out_{tag}.csv"), "--auxiliary-information-file",
+                j(f"aux_{tag}.csv"), "--log-file", j(f"log_{tag}.txt"), "--seed", "3"]
+        print("running abm", tag, flush=True)
+        r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        assert "I/O: read" in r.stdout
+
+    run("nodes.csv", "edges.csv", "clean")
+    run("nodes_q.txt", "edges_q.txt", "quirky")
+    for f in ("out", "aux"):
+        assert open(j(f"{f}_clean.csv"), "rb").read() == open(j(f"{f}_quirky.csv"), "rb").read(), f
