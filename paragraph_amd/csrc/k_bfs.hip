@@ -495,15 +495,15 @@ template <int J>
 __device__ __forceinline__ int32_t xor_lane(int32_t v) {
     static_assert(J == 1 || J == 2 || J == 4 || J == 8 || J == 16 || J == 32, "lane xor");
     if constexpr (J == 1) {
-        return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+        return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
     } else if constexpr (J == 2) {
-        return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+        return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
     } else if constexpr (J == 4) {
-        const int32_t t = __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);  // ^7
-        return __builtin_amdgcn_update_dpp(0, t, 0x1B, 0xF, 0xF, false);             // ^3
+        const int32_t t = __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true);  // ^7
+        return __builtin_amdgcn_mov_dpp(t, 0x1B, 0xF, 0xF, true);             // ^3
     } else if constexpr (J == 8) {
-        const int32_t t = __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);  // ^15
-        return __builtin_amdgcn_update_dpp(0, t, 0x141, 0xF, 0xF, false);            // ^7
+        const int32_t t = __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true);  // ^15
+        return __builtin_amdgcn_mov_dpp(t, 0x141, 0xF, 0xF, true);            // ^7
     } else if constexpr (J == 16) {
         // {v with odd rows <- even rows, v with even rows <- odd rows}
         const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
@@ -522,16 +522,31 @@ __device__ __forceinline__ int32_t xor_lane(int32_t v) {
 // network is fully unrolled for E = P / 64 registers per lane.
 // One compare-exchange step (distance J) of merge size K; recurses down to
 // J = 1.  Compile-time J and K, so every lane exchange is a fixed pattern.
+// Median of three, written so that the compiler emits one v_med3_i32: with
+// c = INT_MIN it is min(a, b), with c = INT_MAX max(a, b).
+__device__ __forceinline__ int32_t med3_i32(int32_t a, int32_t b, int32_t c) {
+    return min(max(a, b), max(min(a, b), c));
+}
+
+// A lane exchange keeps min(v, o) where the lane is the lower one of its
+// pair in an ascending block, else max(v, o): one v_med3 against a per-lane
+// bound (INT_MIN or INT_MAX) chosen once per step.
 template <int E, int K, int J>
 __device__ __forceinline__ void bitonic_step(int32_t (&v)[E], int lane) {
     if constexpr (J < 64) {
+        const bool lower = (lane & J) == 0;
+        if constexpr (K < 64) {
+            const int32_t b = (lower == ((lane & K) == 0)) ? INT_MIN : INT_MAX;
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int i = e * 64 + lane;
-            const int32_t o = xor_lane<J>(v[e]);
-            const bool up = (i & K) == 0;
-            const bool lower = (lane & J) == 0;
-            v[e] = (lower == up) ? min(v[e], o) : max(v[e], o);
+            for (int e = 0; e < E; ++e) v[e] = med3_i32(v[e], xor_lane<J>(v[e]), b);
+        } else {
+            // ascending iff bit K of the element index e*64 + lane, that is
+            // of e*64, is clear
+            const int32_t b_up = lower ? INT_MIN : INT_MAX;
+            const int32_t b_dn = lower ? INT_MAX : INT_MIN;
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                v[e] = med3_i32(v[e], xor_lane<J>(v[e]), ((e * 64) & K) == 0 ? b_up : b_dn);
         }
     } else {
         constexpr int je = J / 64;
