@@ -1144,6 +1144,7 @@ __global__ __launch_bounds__(kBT, 4) void k_bfs_root(CsrView g, const float* __r
                                                   const int32_t* __restrict__ big_count, int cap,
                                                   int32_t* __restrict__ S_buf,
                                                   uint64_t* __restrict__ Q_buf,
+                                                  int32_t* __restrict__ q_idx,
                                                   int32_t* __restrict__ ns_buf,
                                                   unsigned long long* __restrict__ bytes_out,
                                                   const int32_t* __restrict__ only_roots,
@@ -1211,11 +1212,15 @@ __global__ __launch_bounds__(kBT, 4) void k_bfs_root(CsrView g, const float* __r
     int64_t tot;
     int64_t run = block_excl_scan(acc, &tot);
     uint64_t* Q = Q_buf + u * cap;
+    int32_t* qidx = q_idx + u * kQIdx;
+    const int qsh = q_shift((uint64_t)tot);
 #pragma unroll
     for (int e = 0; e < kMaxPer; ++e) {
         if (s0 + e < s1) {
+            const int64_t prev = run;
             run += (int64_t)nb_fixed(wv[e], scale);
             Q[s0 + e] = (uint64_t)run;
+            q_index_put(qidx, s0 + e, ns, (uint64_t)prev, (uint64_t)run, qsh);
         }
     }
 }
@@ -1300,6 +1305,7 @@ __device__ __forceinline__ void bfs_wave(CsrView g, const float* __restrict__ w,
                                          const int32_t* __restrict__ in_list,
                                          const int32_t* __restrict__ in_count, int cap,
                                          int32_t* __restrict__ S_buf, uint64_t* __restrict__ Q_buf,
+                                         int32_t* __restrict__ q_idx,
                                          int32_t* __restrict__ ns_buf,
                                          int32_t* __restrict__ out_list,
                                          int32_t* __restrict__ out_count,
@@ -1445,6 +1451,13 @@ __device__ __forceinline__ void bfs_wave(CsrView g, const float* __restrict__ w,
     }
     for (int off = 32; off > 0; off >>= 1) mb = max(mb, (uint32_t)__shfl_xor((int)mb, off, 64));
     const double scale = pow2d(49 - ((int)(mb >> 23) - 127));
+    // the CDF's total first (each lane reads back the weights it stored),
+    // for the bucket index written with the CDF
+    uint64_t zs = 0;
+    for (int s2 = lane; s2 < ns; s2 += 64) zs += nb_fixed(__uint_as_float((uint32_t)Ls.win[s2]), scale);
+    for (int off = 32; off > 0; off >>= 1) zs += __shfl_xor(zs, off, 64);
+    const int qsh = q_shift(zs);
+    int32_t* qidx = q_idx + u * kQIdx;
     __syncthreads();
     uint64_t carry = 0;
     for (int s0 = 0; s0 < ns; s0 += 256) {
@@ -1464,8 +1477,12 @@ __device__ __forceinline__ void bfs_wave(CsrView g, const float* __restrict__ w,
         }
         const uint64_t ex = carry + x - acc;
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
-            if (b0 + e < ns) Q[b0 + e] = ex + q[e];
+        for (int e = 0; e < 4; ++e) {
+            if (b0 + e < ns) {
+                Q[b0 + e] = ex + q[e];
+                q_index_put(qidx, b0 + e, ns, e == 0 ? ex : ex + q[e - 1], ex + q[e], qsh);
+            }
+        }
         carry += __shfl(x, 63, 64);
     }
     if (lane == 0) {
@@ -1478,20 +1495,21 @@ __device__ __forceinline__ void bfs_wave(CsrView g, const float* __restrict__ w,
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_bfs_small(
     CsrView g, const float* __restrict__ w, const int32_t* __restrict__ uroot,
     const int64_t* __restrict__ n_unique_ptr, int cap, int32_t* __restrict__ S_buf,
-    uint64_t* __restrict__ Q_buf, int32_t* __restrict__ ns_buf, int32_t* __restrict__ out_list,
-    int32_t* __restrict__ out_count, unsigned long long* __restrict__ bytes_out) {
+    uint64_t* __restrict__ Q_buf, int32_t* __restrict__ q_idx, int32_t* __restrict__ ns_buf,
+    int32_t* __restrict__ out_list, int32_t* __restrict__ out_count,
+    unsigned long long* __restrict__ bytes_out) {
     bfs_wave<kSmallF1, kSmallWin, false>(g, w, uroot, n_unique_ptr, nullptr, nullptr, cap, S_buf,
-                                         Q_buf, ns_buf, out_list, out_count, bytes_out);
+                                         Q_buf, q_idx, ns_buf, out_list, out_count, bytes_out);
 }
 
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_bfs_mid(
     CsrView g, const float* __restrict__ w, const int32_t* __restrict__ uroot,
     const int32_t* __restrict__ in_list, const int32_t* __restrict__ in_count, int cap,
-    int32_t* __restrict__ S_buf, uint64_t* __restrict__ Q_buf, int32_t* __restrict__ ns_buf,
-    int32_t* __restrict__ out_list, int32_t* __restrict__ out_count,
-    unsigned long long* __restrict__ bytes_out) {
+    int32_t* __restrict__ S_buf, uint64_t* __restrict__ Q_buf, int32_t* __restrict__ q_idx,
+    int32_t* __restrict__ ns_buf, int32_t* __restrict__ out_list,
+    int32_t* __restrict__ out_count, unsigned long long* __restrict__ bytes_out) {
     bfs_wave<kMidF1, kMidWin, true>(g, w, uroot, nullptr, in_list, in_count, cap, S_buf, Q_buf,
-                                    ns_buf, out_list, out_count, bytes_out);
+                                    q_idx, ns_buf, out_list, out_count, bytes_out);
 }
 
 // ---- root deduplication ----
@@ -1581,6 +1599,7 @@ __global__ __launch_bounds__(256) void k_nb_draw(Batch bt, int64_t n_exist, int6
                                                  const int32_t* __restrict__ root_slot,
                                                  const int32_t* __restrict__ S_buf,
                                                  const uint64_t* __restrict__ Q_buf,
+                                                 const int32_t* __restrict__ q_idx,
                                                  const int32_t* __restrict__ ns_buf,
                                                  int32_t* __restrict__ cit, int8_t* __restrict__ kind,
                                                  int32_t* __restrict__ cnt, int32_t* __restrict__ near,
@@ -1615,11 +1634,13 @@ __global__ __launch_bounds__(256) void k_nb_draw(Batch bt, int64_t n_exist, int6
     int ns = 0;
     const int32_t* S = S_buf;
     const uint64_t* Q = Q_buf;
+    const int32_t* qidx = q_idx;
     if (live) {
         const int64_t u = root_slot[i];
         ns = ns_buf[u];
         S = S_buf + u * cap;
         Q = Q_buf + u * cap;
+        qidx = q_idx + u * kQIdx;
         for (int j = gl; j < c0; j += G) s_row[grp][j] = row[j];
     }
     __syncthreads();
@@ -1666,6 +1687,7 @@ __global__ __launch_bounds__(256) void k_nb_draw(Batch bt, int64_t n_exist, int6
     // rejection draws over the neighbourhood's fixed-point CDF
     bool need = live && !take_all;
     const uint64_t Zq = (live && ns > 0) ? Q[ns - 1] : 0;
+    const int qsh = q_shift(Zq);
     const int b_max = kAttemptsPerCite * n_nb;
     int got = 0, b0 = 0;
     while (__syncthreads_or(need)) {
@@ -1676,7 +1698,8 @@ __global__ __launch_bounds__(256) void k_nb_draw(Batch bt, int64_t n_exist, int6
             const u32x4 r = philox(gid, (uint32_t)b, yi, TAG_NB, seed);
             const uint64_t v = ((uint64_t)r.x << 21) ^ (uint64_t)(r.y >> 11);
             const uint64_t t = (__umul64hi(v, Zq) << 11) | ((v * Zq) >> 53);
-            int lo = 0, hi = ns;
+            const int k = (int)(t >> qsh);  // t < Zq: k < kQBuckets
+            int lo = qidx[k], hi = qidx[k + 1];
             while (lo < hi) { int mid = (lo + hi) >> 1; if (Q[mid] > t) hi = mid; else lo = mid + 1; }
             if (lo >= ns) lo = ns - 1;
             pick = S[lo];
@@ -1756,41 +1779,42 @@ void launch_root_dedup(hipStream_t s, int64_t n_exist, int64_t n_new, int m, con
 
 void launch_bfs_root(hipStream_t s, const CsrView& g, const float* w, const int32_t* uroot,
                      const int64_t* n_unique, int64_t max_unique, int cap, int32_t* S_buf,
-                     uint64_t* Q_buf, int32_t* ns_buf, int32_t* big_list, int32_t* big_count,
-                     unsigned long long* bytes_out, const BfsStage& stage) {
+                     uint64_t* Q_buf, int32_t* q_idx, int32_t* ns_buf, int32_t* big_list,
+                     int32_t* big_count, unsigned long long* bytes_out, const BfsStage& stage) {
     if (max_unique <= 0) return;
     int32_t* mid_list = big_list + max_unique;
     hipLaunchKernelGGL(k_bfs_small, dim3((uint32_t)max_unique), dim3(64), 0, s, g, w, uroot,
-                       n_unique, cap, S_buf, Q_buf, ns_buf, big_list, big_count, bytes_out);
+                       n_unique, cap, S_buf, Q_buf, q_idx, ns_buf, big_list, big_count, bytes_out);
     hipLaunchKernelGGL(k_bfs_mid, dim3((uint32_t)max_unique), dim3(64), 0, s, g, w, uroot,
-                       big_list, big_count, cap, S_buf, Q_buf, ns_buf, mid_list, big_count + 1,
-                       bytes_out);
+                       big_list, big_count, cap, S_buf, Q_buf, q_idx, ns_buf, mid_list,
+                       big_count + 1, bytes_out);
     hipLaunchKernelGGL(k_bfs_root, dim3((uint32_t)max_unique), dim3(kBT), 0, s, g, w, uroot,
-                       mid_list, big_count + 1, cap, S_buf, Q_buf, ns_buf, bytes_out,
+                       mid_list, big_count + 1, cap, S_buf, Q_buf, q_idx, ns_buf, bytes_out,
                        (const int32_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr, stage);
 }
 
 void launch_nb_draw(hipStream_t s, int64_t n_exist, int64_t n_new, int m, int n_nb, int cap,
                     uint64_t seed, uint32_t year_index, const double* cdf, const int32_t* cdf_idx,
                     const int32_t* root_slot, const int32_t* S_buf, const uint64_t* Q_buf,
-                    const int32_t* ns_buf, int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* near,
-                    int32_t* anext) {
+                    const int32_t* q_idx, const int32_t* ns_buf, int32_t* cit, int8_t* kind,
+                    int32_t* cnt, int32_t* near, int32_t* anext) {
     if (n_new <= 0 || n_nb <= 0) return;
     const int64_t nb = (n_new + 256 / kNbG - 1) / (256 / kNbG);
     hipLaunchKernelGGL(k_nb_draw<false>, dim3((uint32_t)nb), dim3(256), 0, s, Batch{}, n_exist,
                        n_new, m, n_nb, cap, seed, year_index, cdf, cdf_idx, root_slot, S_buf,
-                       Q_buf, ns_buf, cit, kind, cnt, near, anext);
+                       Q_buf, q_idx, ns_buf, cit, kind, cnt, near, anext);
 }
 
 void launch_nb_draw_batch(hipStream_t s, const Batch& b, int m, int cap, uint32_t year_index,
                           const double* cdf, const int32_t* root_slot, const int32_t* S_buf,
-                          const uint64_t* Q_buf, const int32_t* ns_buf, int32_t* cit, int8_t* kind,
-                          int32_t* cnt, int32_t* near, int32_t* anext) {
+                          const uint64_t* Q_buf, const int32_t* q_idx, const int32_t* ns_buf,
+                          int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* near,
+                          int32_t* anext) {
     if (b.n_new <= 0) return;
     const int64_t nb = (b.n_new + 256 / kNbG - 1) / (256 / kNbG);
     hipLaunchKernelGGL(k_nb_draw<true>, dim3((uint32_t)nb), dim3(256), 0, s, b, (int64_t)0,
                        b.n_new, m, 0, cap, (uint64_t)0, year_index, cdf, (const int32_t*)nullptr,
-                       root_slot, S_buf, Q_buf, ns_buf, cit, kind, cnt, near, anext);
+                       root_slot, S_buf, Q_buf, q_idx, ns_buf, cit, kind, cnt, near, anext);
 }
 
 BfsStage bfs_stage_view(const BfsStage& pool, int mode) {
@@ -1813,7 +1837,7 @@ void launch_bfs_only(hipStream_t s, const CsrView& g, int64_t count, const int32
     if (count <= 0) return;
     hipLaunchKernelGGL(k_bfs_root, dim3((uint32_t)count), dim3(kBT), 0, s, g, (const float*)nullptr,
                        (const int32_t*)nullptr, (const int32_t*)nullptr, (const int32_t*)nullptr,
-                       cap, out, (uint64_t*)nullptr, (int32_t*)nullptr,
+                       cap, out, (uint64_t*)nullptr, (int32_t*)nullptr, (int32_t*)nullptr,
                        (unsigned long long*)nullptr, roots, out_len, ticks, stage);
 }
 

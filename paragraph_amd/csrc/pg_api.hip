@@ -39,7 +39,7 @@ void release(pg_ctx* c) {
                     c->near, c->anext, c->new_off, c->src_of, c->k_sorted,
                     c->v_sorted, c->k_tmp, c->v_tmp, c->scratch, c->d_sum, c->d_tmp_i32,
                     c->s_birth, c->s_in_deg, c->s_fwd_off, c->s_fwd_adj, c->s_bwd_off,
-                    c->s_bwd_adj, c->S_buf, c->Q_buf, c->ns_buf, c->root_bits, c->r_wcount,
+                    c->s_bwd_adj, c->S_buf, c->Q_buf, c->Q_idx, c->ns_buf, c->root_bits, c->r_wcount,
                     c->r_wprefix, c->r_excl,
                     c->root_slot, c->uroot, c->big_list, c->stage.buf, c->stage.busy};
     for (void* p : ptrs)
@@ -119,6 +119,7 @@ int ensure_nb_buffers(pg_ctx* ctx) {
     const int64_t cap = ctx->cfg.bfs_cap;
     hipError_t r = dalloc(&ctx->S_buf, D * cap);
     if (r == hipSuccess) r = dalloc(&ctx->Q_buf, D * cap);
+    if (r == hipSuccess) r = dalloc(&ctx->Q_idx, D * kQIdx);
     if (r == hipSuccess) r = dalloc(&ctx->ns_buf, D);
     const int64_t nw = ctx->cfg.max_papers / 32 + 1;
     if (r == hipSuccess) r = dalloc(&ctx->root_bits, nw);
@@ -389,13 +390,14 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
                 int32_t* big_count = reinterpret_cast<int32_t*>(sums + 3);
                 pbeg(PG_K_BFS);
                 launch_bfs_root(s, view(ctx), ctx->w, ctx->uroot, ctx->r_excl + D, D,
-                                ctx->cfg.bfs_cap, ctx->S_buf, ctx->Q_buf, ctx->ns_buf,
+                                ctx->cfg.bfs_cap, ctx->S_buf, ctx->Q_buf, ctx->Q_idx, ctx->ns_buf,
                                 ctx->big_list, big_count, sums + 2,
                                 bfs_stage_view(ctx->stage, ctx->tune_bfs_mode));
                 pend(PG_K_BFS);
                 pbeg2(PG_K_NB_DRAW);
                 launch_nb_draw(s, N, D, m, n_nb, ctx->cfg.bfs_cap, seed, yi, ctx->cdf,
-                               ctx->cdf_idx, ctx->root_slot, ctx->S_buf, ctx->Q_buf, ctx->ns_buf, ctx->cit,
+                               ctx->cdf_idx, ctx->root_slot, ctx->S_buf, ctx->Q_buf, ctx->Q_idx,
+                               ctx->ns_buf, ctx->cit,
                                ctx->kind, ctx->cnt, ctx->near, ctx->anext);
                 pend2(PG_K_NB_DRAW);
             }

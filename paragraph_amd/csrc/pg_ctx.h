@@ -128,6 +128,7 @@ struct pg_ctx {
     // neighbourhoods of the year's distinct roots
     int32_t* S_buf = nullptr;
     uint64_t* Q_buf = nullptr;
+    int32_t* Q_idx = nullptr;  // kQIdx bucket starts of each root's Q
     int32_t* ns_buf = nullptr;
     uint32_t* root_bits = nullptr; // [max_papers/32] roots of the year, zero between years
     int32_t* r_wcount = nullptr;   // [max_papers/32] popcounts of root_bits

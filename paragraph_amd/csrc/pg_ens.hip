@@ -107,10 +107,10 @@ void enqueue_year(pg_ensemble* ens, int y, hipStream_t s) {
         launch_fill_i32(s, big_count, 0, 2);
         const CsrView g{c->fwd_off, c->fwd_adj, c->bwd_off[c->cur], c->bwd_adj[c->cur], TOT};
         launch_bfs_root(s, g, c->w, c->uroot, c->r_excl + Dn, Dn, c->cfg.bfs_cap, c->S_buf,
-                        c->Q_buf, c->ns_buf, c->big_list, big_count, cnt + 2,
+                        c->Q_buf, c->Q_idx, c->ns_buf, c->big_list, big_count, cnt + 2,
                         bfs_stage_view(c->stage, c->tune_bfs_mode));
         launch_nb_draw_batch(s, b, m, c->cfg.bfs_cap, (uint32_t)y, c->cdf, c->root_slot,
-                             c->S_buf, c->Q_buf, c->ns_buf, c->cit, c->kind, c->cnt, c->near,
+                             c->S_buf, c->Q_buf, c->Q_idx, c->ns_buf, c->cit, c->kind, c->cnt, c->near,
                              c->anext);
     }
     launch_sum_i32(s, c->near, Dn, cnt + 0);

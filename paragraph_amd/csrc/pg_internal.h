@@ -24,6 +24,29 @@ __device__ __forceinline__ int64_t cdf_upper_bound(const double* __restrict__ cd
     return lo < n ? lo : n - 1;
 }
 
+// Bucket index of a root's fixed-point neighbourhood CDF Q[0,ns) (total Z):
+// bucket of a value x is x >> q_shift(Z) (at most 63), monotone in x, and
+// qidx[k] (k <= kQBuckets) is the first s whose bucket is >= k, ns if none;
+// the first s with Q[s] > t then lies in [qidx[key(t)], qidx[key(t) + 1]].
+constexpr int kQBuckets = 64;
+constexpr int kQIdx = kQBuckets + 1;  // int32 per root
+
+__device__ __forceinline__ int q_shift(uint64_t Z) {
+    const int bl = Z ? 64 - __clzll((long long)Z) : 0;
+    return bl > 6 ? bl - 6 : 0;
+}
+
+// Index entries of element s (inclusive prefix qs, previous prefix qp) of
+// a root's CDF of ns elements.
+__device__ __forceinline__ void q_index_put(int32_t* __restrict__ qidx, int s, int ns, uint64_t qp,
+                                            uint64_t qs, int sh) {
+    const int k0 = s == 0 ? 0 : (int)(qp >> sh) + 1;
+    const int k1 = (int)(qs >> sh);
+    for (int k = k0; k <= k1; ++k) qidx[k] = s;
+    if (s == ns - 1)
+        for (int k = k1 + 1; k <= kQBuckets; ++k) qidx[k] = ns;
+}
+
 // CDF bucket index: bucket of a CDF value or target t is
 // floor(t * kCdfIdx / Z) clamped to [0, kCdfIdx], monotone in t (one
 // rounding of a product by a positive constant), and idx[k] is the first
@@ -185,19 +208,21 @@ void launch_root_dedup(hipStream_t s, int64_t n_exist, int64_t n_new, int m, con
 // big_count[0..1] = 0 on entry); grids of max_unique blocks.
 void launch_bfs_root(hipStream_t s, const CsrView& g, const float* w, const int32_t* uroot,
                      const int64_t* n_unique, int64_t max_unique, int cap, int32_t* S_buf,
-                     uint64_t* Q_buf, int32_t* ns_buf, int32_t* big_list, int32_t* big_count,
+                     uint64_t* Q_buf, int32_t* q_idx, int32_t* ns_buf, int32_t* big_list,
+                     int32_t* big_count,
                      unsigned long long* bytes_out, const BfsStage& stage);
 // Per paper: neighbourhood draws from its root's S, then PA refill.
 void launch_nb_draw(hipStream_t s, int64_t n_exist, int64_t n_new, int m, int n_nb, int cap,
                     uint64_t seed, uint32_t year_index, const double* cdf, const int32_t* cdf_idx,
                     const int32_t* root_slot, const int32_t* S_buf, const uint64_t* Q_buf,
-                    const int32_t* ns_buf, int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* near,
-                    int32_t* anext);
+                    const int32_t* q_idx, const int32_t* ns_buf, int32_t* cit, int8_t* kind,
+                    int32_t* cnt, int32_t* near, int32_t* anext);
 
 void launch_nb_draw_batch(hipStream_t s, const Batch& b, int m, int cap, uint32_t year_index,
                           const double* cdf, const int32_t* root_slot, const int32_t* S_buf,
-                          const uint64_t* Q_buf, const int32_t* ns_buf, int32_t* cit, int8_t* kind,
-                          int32_t* cnt, int32_t* near, int32_t* anext);
+                          const uint64_t* Q_buf, const int32_t* q_idx, const int32_t* ns_buf,
+                          int32_t* cit, int8_t* kind, int32_t* cnt, int32_t* near,
+                          int32_t* anext);
 
 // Stand-alone 2-hop neighbourhoods (out[i*cap ...], out_len[i]) by the
 // workgroup path; ticks (nullable) gets per root the wall-clock ticks to the
