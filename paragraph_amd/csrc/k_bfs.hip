@@ -13,22 +13,24 @@
 //
 // The neighbourhood depends only on the root and on the graph, which is
 // frozen for the year, and PA roots concentrate on hubs (late in a 10-year
-// run thousands of papers share one root).  So the year's roots are sorted
-// and deduplicated, S and its CDF are built once per distinct root
-// (k_bfs_root), and each paper then draws from its root's S (k_nb_draw).
+// run thousands of papers share one root).  So the year's roots are
+// deduplicated (a bitmap over paper ids, numbered in id order), S and its
+// CDF are built once per distinct root, and each paper then draws from its
+// root's S (k_nb_draw).
 //
-// k_bfs_root, one 512-thread workgroup per distinct root: each level's
-// candidate ids are the rows of the CSR (sorted); the id range is swept in
-// ascending windows [lo,hi) sized so that a window holds at most 4096 row
-// entries, every row keeps a cursor in LDS, a row's part of a window is found
-// by galloping search, long parts are copied by whole wavefronts, and the
-// window is bitonic-sorted, deduplicated and filtered in LDS, until `cap`
-// ids are found.  Per-query work follows the adjacency touched below the
-// final threshold, not N.
+// S is built by three tiers, each handing the roots over its limits to the
+// next: one wavefront per root for small and medium neighbourhoods
+// (k_bfs_small, k_bfs_mid: rows gathered into LDS, sorted by a register
+// bitonic network, compacted by ballots), then one 512-thread workgroup per
+// root (k_bfs_root): the entries below the answer bound are radix-sorted in
+// LDS, or, when they do not fit, staged in id buckets in a device slot or
+// swept through the id range in bitmap windows.  Per-query work follows the
+// adjacency touched below the final threshold, not N.
 //
 // The neighbourhood CDF is exact 64-bit fixed point (weights scaled so the
-// largest maps below 2^50), built by an integer block scan, so it is
-// bit-identical to the oracle's sequential loop.
+// largest maps below 2^50), built by an integer scan, so it is
+// bit-identical to the oracle's sequential loop; a 64-bucket index of it
+// starts each draw's search.
 #include <climits>
 
 #include "pg_internal.h"
