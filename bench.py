@@ -579,8 +579,8 @@ def main():
         print(json.dumps(abm_bench(args)), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(every GPU box here has one GPU).", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path.",
+              file=sys.stderr)
         sys.exit(2)
     alpha, tau, seed, cap = 1.0, 5.0, 1, 4096
     n0, sched, m, n_nb = workload(args)

@@ -4,7 +4,8 @@
  *
  * The reference's own CPU path (cpu-model branch) is not available, so this
  * file restates the model from the reference's year-loop structure and the
- * BASELINE formulas.  Each function names the reference code it follows.
+ * growth model's formulas.  Each function names the reference code it
+ * follows.
  *
  * Parity pinning -- what checks each part against something independent of
  * this file (tests/test_oracle_*.py):
@@ -180,7 +181,7 @@ double or_log1p(double x) {
 /* ---------------------------------------------------------------------- */
 /* Weights and CDF.  Reference score prep: src/abm.cu:112-122 (in-degree),  */
 /* src/abm.cu:152-176 (fitness), src/abm.cu:324-338 (scores), frozen at the */
-/* start of the year (src/kernel.cu:1698-1706).  Formula from BASELINE.     */
+/* start of the year (src/kernel.cu:1698-1706).  Weight (k+1)^a*e^(-age/t)*f. */
 /* ---------------------------------------------------------------------- */
 static inline float weight_one(int32_t k, int32_t age, float logfit, float alpha,
                                float inv_tau) {

@@ -8,7 +8,7 @@
  *
  * It restates, in plain C with OpenMP over the papers of a year, the model
  * that the reference's year loop runs (src/kernel.cu:1682-1845) with the
- * BASELINE formulas (FP32 weights (k+1)^a * exp(-age/tau) * fitness, FP64
+ * growth model's formulas (FP32 weights (k+1)^a * exp(-age/tau) * fitness, FP64
  * CDF, binary-search draws without duplicates, 2-hop neighbourhood draws,
  * ER G(n,p) by geometric skips, CSR append with degree update).  Every
  * random number is a Philox4x32-10 output keyed by the simulation seed and

@@ -3,7 +3,7 @@
 // Replaces the reference's candidate sampling (MakePopulateCitations,
 // src/abm.cu:461-565, run from Stage 2/3, src/kernel.cu:170-344), which walked
 // a per-paper candidate bitmap with Efraimidis-Spirakis keys and a top-K
-// heap, with the BASELINE rule: draw u, take the first paper whose FP64 CDF
+// heap, with this model's rule: draw u, take the first paper whose FP64 CDF
 // value exceeds u*Z (binary search), and redraw on duplicates.
 //
 // Attempt a of paper g uses Philox counter (g, a, year index, TAG_PA), so the
