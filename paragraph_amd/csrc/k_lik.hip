@@ -144,13 +144,31 @@ __global__ __launch_bounds__(kLikBT) void k_lik_papers(LikSeq q, LikArgs a,
         }
     }
     double gL = 0.0, ga = 0.0;
-    for (int y = 0; y < q.n_years; ++y) {
+    // kLikYU years at a time: their log(k+1) loads are issued together
+    // before any is used (the sums still run year by year, in order)
+    constexpr int kLikYU = 4;
+    for (int y0 = 0; y0 < q.n_years; y0 += kLikYU) {
+    float Lv[kLikYU][kLikPPT];
+#pragma unroll
+    for (int yy = 0; yy < kLikYU; ++yy) {
+        const int y = y0 + yy;
+        const int64_t Ny = y < q.n_years ? q.n_exist[y] : 0;
+        const int64_t doff = y < q.n_years ? q.deg_off[y] : 0;
+#pragma unroll
+        for (int r = 0; r < kLikPPT; ++r) {
+            const int64_t j = j0 + r * kLikBT + threadIdx.x;
+            Lv[yy][r] = j < Ny ? q.Lpy[doff + j] : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int yy = 0; yy < kLikYU; ++yy) {
+        const int y = y0 + yy;
+        if (y >= q.n_years) break;
         const int64_t Ny = q.n_exist[y];
         if (j0 >= Ny) {                  // block-uniform: no paper of the block exists
             if (!BWD && threadIdx.x == 0) part[(int64_t)y * nblk + blockIdx.x] = 0.0;
             continue;
         }
-        const int64_t doff = q.deg_off[y];
         const int32_t year = q.year_of[y];
         const double Ay = BWD ? year_sums[y * kLikYearCols + 1] : 0.0;
         double zacc = 0.0;
@@ -158,7 +176,7 @@ __global__ __launch_bounds__(kLikBT) void k_lik_papers(LikSeq q, LikArgs a,
         for (int r = 0; r < kLikPPT; ++r) {
             const int64_t j = j0 + r * kLikBT + threadIdx.x;
             if (j < Ny) {
-                const float L = q.Lpy[doff + j];
+                const float L = Lv[yy][r];
                 const int32_t age = year - b[r];
                 float x;
                 const double wv = (double)lik_weight_la(L, (float)age, phi[r], s, x);
@@ -176,6 +194,7 @@ __global__ __launch_bounds__(kLikBT) void k_lik_papers(LikSeq q, LikArgs a,
             const double ws = wave_sum_d(zacc);
             if ((threadIdx.x & 63) == 0) s_red[wave][y] = ws;
         }
+    }
     }
     if (BWD) {
 #pragma unroll
