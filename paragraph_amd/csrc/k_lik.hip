@@ -267,6 +267,7 @@ __global__ __launch_bounds__(kLikBT) void k_lik_cite(LikSeq q, LikArgs a,
     const int c = i < Dy ? q.cnt[P] : 0;
     const int j0 = lg * K;                            // first slot of this lane
     float wr[K], Lr[K], ar[K];
+    int32_t pos[K];  // each slot's by-target position, loaded with its target
     uint32_t evm = 0, vm = 0;
     double ll = 0.0, gaacc = 0.0, gtacc = 0.0;
     if (c > 0) {
@@ -275,6 +276,7 @@ __global__ __launch_bounds__(kLikBT) void k_lik_cite(LikSeq q, LikArgs a,
             const int j = j0 + e;
             const bool v = j < c;
             const int64_t qi = (int64_t)(v ? j : 0) * NP + P;
+            if (GRAD) pos[e] = q.pos_t[qi];
             const int32_t t = q.cit_t[qi];
             const bool ev = v && q.kind_t[qi] != 1;
             const float2 la = q.la_t[qi];
@@ -342,8 +344,7 @@ __global__ __launch_bounds__(kLikBT) void k_lik_cite(LikSeq q, LikArgs a,
             const double wl = (double)wr[e] * later;
             sL += wl * (double)Lr[e];
             sa += wl * (double)ar[e];
-            if ((vm >> e) & 1)
-                share[q.pos_t[(int64_t)(j0 + e) * NP + P]] = (double)((evm >> e) & 1) + wl;
+            if ((vm >> e) & 1) share[pos[e]] = (double)((evm >> e) & 1) + wl;
             later += inv[e];
         }
         gaacc += sL;
