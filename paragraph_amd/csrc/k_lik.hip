@@ -437,75 +437,104 @@ __global__ __launch_bounds__(kLikBT) void k_lik_slot_major(int64_t T, int m, int
 }
 
 // Per-target sums of the slot shares (stored in by-target order) over sorted
-// positions [0, S) in chunks
-// of kLikSegChunk (kLikSegPer per thread).  A run of equal keys that starts
-// and ends inside the chunk is final: evt[key] = its sum.  A run that
-// continues from the previous chunk gives carry[2*chunk] (its part in this
-// chunk), one that continues into the next chunk gives carry[2*chunk+1].
+// positions [0, S) in chunks of kLikSegChunk (kLikSegPer consecutive
+// positions per thread).  A run of equal keys that starts and ends inside the
+// chunk is final: evt[key] = its sum.  A run that continues from the previous
+// chunk gives carry[2*chunk] (its part in this chunk), one that continues into
+// the next chunk gives carry[2*chunk+1].  The chunk is loaded coalesced into
+// LDS and each thread then takes its consecutive positions from there; the
+// segmented scan over the threads' trailing runs is a shuffle scan inside each
+// wave and a combine over the waves in order (fixed order: the sums are
+// reproducible).
 __global__ __launch_bounds__(kLikBT) void k_lik_segsum(int64_t S, const uint32_t* __restrict__ skey,
                                                        const double* __restrict__ share,
                                                        double* __restrict__ evt,
                                                        double* __restrict__ carry) {
-    __shared__ double s_val[kLikBT];
-    __shared__ int s_flag[kLikBT];
+    __shared__ uint32_t s_key[kLikSegChunk];
+    __shared__ double s_v[kLikSegChunk];
+    __shared__ double s_wt[kLikWaves];
+    __shared__ int s_wf[kLikWaves];
     const int64_t c0 = (int64_t)blockIdx.x * kLikSegChunk;
-    const int64_t c1 = c0 + kLikSegChunk < S ? c0 + kLikSegChunk : S;
-    const int64_t i0 = c0 + (int64_t)threadIdx.x * kLikSegPer;
+    const int n = (int)(S - c0 < kLikSegChunk ? S - c0 : kLikSegChunk);
+    {
+        uint32_t kk[kLikSegPer];
+        double vv[kLikSegPer];
+#pragma unroll
+        for (int k = 0; k < kLikSegPer; ++k) {
+            const int i = k * kLikBT + threadIdx.x;
+            kk[k] = i < n ? skey[c0 + i] : 0xFFFFFFFFu;
+            vv[k] = i < n ? share[c0 + i] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kLikSegPer; ++k) {
+            s_key[k * kLikBT + threadIdx.x] = kk[k];
+            s_v[k * kLikBT + threadIdx.x] = vv[k];
+        }
+    }
+    __syncthreads();
+    const int t0 = threadIdx.x * kLikSegPer;
     uint32_t key[kLikSegPer];
     double val[kLikSegPer];
-    const uint32_t kprev = (i0 > 0 && i0 - 1 < S) ? skey[i0 - 1] : 0xFFFFFFFFu;
 #pragma unroll
     for (int e = 0; e < kLikSegPer; ++e) {
-        const int64_t i = i0 + e;
-        key[e] = i < c1 ? skey[i] : 0xFFFFFFFFu;
-        val[e] = i < c1 ? share[i] : 0.0;
+        key[e] = s_key[t0 + e];
+        val[e] = s_v[t0 + e];
     }
-    const uint32_t knext = (i0 + kLikSegPer < c1) ? skey[i0 + kLikSegPer] : 0xFFFFFFFFu;
+    const uint32_t kprev = t0 > 0 ? s_key[t0 - 1] : 0xFFFFFFFFu;
+    const uint32_t knext = t0 + kLikSegPer < kLikSegChunk ? s_key[t0 + kLikSegPer] : 0xFFFFFFFFu;
     // thread aggregate: sum of the trailing run, and whether a run starts here
-    // (the chunk's first element always starts a run for the scan)
-    int head_any = (i0 == c0) ? 1 : 0;
+    // (the chunk's first position always starts a run for the scan)
+    int head_any = t0 == 0 ? 1 : 0;
     double tail = 0.0;
 #pragma unroll
     for (int e = 0; e < kLikSegPer; ++e) {
         const uint32_t kp = e ? key[e - 1] : kprev;
-        const bool head = (e == 0 && i0 == c0) || key[e] != kp;
+        const bool head = (e == 0 && t0 == 0) || key[e] != kp;
         if (head) {
             head_any = 1;
             tail = 0.0;
         }
         tail += val[e];
     }
-    // inclusive segmented scan of (head_any, tail) over the block's threads
-    // (Hillis-Steele in LDS, fixed order)
-    s_val[threadIdx.x] = tail;
-    s_flag[threadIdx.x] = head_any;
-    __syncthreads();
-    for (int d = 1; d < kLikBT; d <<= 1) {
-        double v = s_val[threadIdx.x];
-        int f = s_flag[threadIdx.x];
-        if ((int)threadIdx.x >= d && !f) {
-            v = s_val[threadIdx.x - d] + v;
-            f = s_flag[threadIdx.x - d];
+    // inclusive segmented scan of (head_any, tail): inside the wave...
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double v = tail;
+    int f = head_any;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double pv = __shfl_up(v, d, 64);
+        const int pf = __shfl_up(f, d, 64);
+        if (lane >= d && !f) {
+            v = pv + v;
+            f = pf;
         }
-        __syncthreads();
-        s_val[threadIdx.x] = v;
-        s_flag[threadIdx.x] = f;
-        __syncthreads();
     }
-    double run = threadIdx.x > 0 ? s_val[threadIdx.x - 1] : 0.0;  // carry into this thread
-    const uint32_t kfirst = skey[c0];
+    if (lane == 63) {
+        s_wt[wave] = v;
+        s_wf[wave] = f;
+    }
+    __syncthreads();
+    // ...then the earlier waves' trailing run, combined in wave order
+    double cw = 0.0;
+    for (int u = 0; u < wave; ++u) cw = s_wf[u] ? s_wt[u] : cw + s_wt[u];
+    const double incl = f ? v : cw + v;
+    const double up = __shfl_up(incl, 1, 64);
+    double run = lane ? up : cw;  // carry into this thread (0 for thread 0)
+    const uint32_t kfirst = s_key[0];
     const bool cont_in = c0 > 0 && skey[c0 - 1] == kfirst;
-    const uint32_t klast = skey[c1 - 1];
+    const uint32_t klast = s_key[n - 1];
+    const int64_t c1 = c0 + n;
     const bool cont_out = c1 < S && skey[c1] == klast;
+    const int64_t i0 = c0 + t0;
 #pragma unroll
     for (int e = 0; e < kLikSegPer; ++e) {
         const int64_t i = i0 + e;
         if (i >= c1) break;
         const uint32_t kp = e ? key[e - 1] : kprev;
-        if ((e == 0 && i0 == c0) || key[e] != kp) run = 0.0;
+        if ((e == 0 && t0 == 0) || key[e] != kp) run = 0.0;
         run += val[e];
         const uint32_t kn = e + 1 < kLikSegPer ? key[e + 1] : knext;
-        if (i + 1 == c1 || kn != key[e]) {            // last element of its run in the chunk
+        if (i + 1 == c1 || kn != key[e]) {            // last position of its run in the chunk
             const bool starts = !(cont_in && key[e] == kfirst);
             const bool ends = !(cont_out && key[e] == klast);
             if (starts && ends) evt[key[e]] = run;
