@@ -394,7 +394,9 @@ int or_pa_year(const or_csr* g, const int32_t* in_deg, const int32_t* birth,
     const double Z = C[N - 1];
     const double eps = NEAR_REL * Z;
     const int m = p->m, n_nb = p->n_nb, n_pa = m - n_nb;
-    const int a_max = PA_ATTEMPTS_PER_CITE * m;
+    /* no positive weight (every FP32 weight underflowed to zero): no PA
+     * attempts, so the year adds no citations */
+    const int a_max = Z > 0.0 ? PA_ATTEMPTS_PER_CITE * m : 0;
     const int b_max = PA_ATTEMPTS_PER_CITE * n_nb;
     const uint32_t yi = (uint32_t)p->year_index;
 

@@ -56,9 +56,13 @@ __global__ __launch_bounds__(256) void k_pa_draw(Batch bt, int64_t n_exist,
     const double Z = cdf[n_exist - 1];
     const double eps = kNearRel * Z;
     const double kscale = cdf_key_scale(Z);
+    // no positive weight (every FP32 weight underflowed to zero): no PA
+    // attempts, so the paper gets no citations (the neighbourhood draws and
+    // the refill need a first pick)
+    if (!(Z > 0.0)) a_max = 0;
 
     int c = 0, a0 = 0, nr = 0, a_stop = a_max;
-    bool need = live && n_pa > 0;
+    bool need = live && n_pa > 0 && a_max > 0;
     while (__syncthreads_or(need)) {
         const int a = a0 + gl;
         const bool valid = need && a < a_max;

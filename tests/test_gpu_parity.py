@@ -67,7 +67,7 @@ def check_pa_year(ctx, logfit_full, year, yi, D, m, n_nb, alpha, tau, seed, cap)
     for i in range(0, D, max(1, D // 200)):
         row = gres["cit"][i, :gres["cnt"][i]]
         assert np.unique(row).size == row.size
-        assert row.min() >= 0 and row.max() < N
+        assert row.size == 0 or (row.min() >= 0 and row.max() < N)
     # CSR append: oracle appends the GPU's draws
     off, tgt = ob.pa_compact(gres)
     G2 = ob.csr_append(G, off, tgt, logfit_full, year)
@@ -136,6 +136,16 @@ def test_pa_tiny_graph_exhaustion():
     ctx = new_ctx(g, [4], 5, cap=8, seed=2)
     st, _ = check_pa_year(ctx, g.logfit, 1, 0, 4, 5, 2, 1.0, 5.0, 2, 8)
     assert st.shortfall > 0
+
+
+def test_pa_year_without_positive_weight():
+    """Every weight underflows to zero (log-fitness -200): the year adds no
+    citations, as in the oracle."""
+    g = pg.seed_graph(4, 3000, 5.0, n_final=3200)
+    g.logfit[:] = -200.0
+    ctx = new_ctx(g, [200], 20, cap=4096, seed=5)
+    st, _ = check_pa_year(ctx, g.logfit, 1, 0, 200, 20, 6, 1.0, 5.0, 5, 4096)
+    assert st.edges_added == 0 and st.near_boundary == 0
 
 
 def tier_graph(n=12000, big=6, mid=40, seed=13):

@@ -160,3 +160,18 @@ def test_exhaustion_is_bounded():
     r = ob.pa_year(G, 1, 0, 1, 10, 5, 2, 1.0, 5.0)
     assert np.all(r["cnt"] == 3)
     assert np.all(r["cit"][:, 3:] == -1)
+
+
+def test_year_without_positive_weight_adds_nothing():
+    """Every weight underflows to zero (log-fitness -200): no PA attempt is
+    made and no paper gets a citation (no neighbourhood draw or refill
+    without a first pick).  One paper with a positive weight is then every
+    paper's first pick."""
+    lf = np.full(100004, -200.0, np.float32)
+    G = graph(4, [(1, 0), (2, 1), (3, 2)], logfit=lf)
+    r = ob.pa_year(G, 1, 0, 1, 50, 5, 2, 1.0, 5.0)
+    assert np.all(r["cnt"] == 0) and np.all(r["cit"] == -1) and np.all(r["near"] == 0)
+    lf[0] = 0.0
+    G = graph(4, [(1, 0), (2, 1), (3, 2)], logfit=lf)
+    r = ob.pa_year(G, 1, 0, 1, 50, 5, 2, 1.0, 5.0)
+    assert np.all(r["cnt"] >= 1) and np.all(r["cit"][:, 0] == 0)
