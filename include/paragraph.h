@@ -174,6 +174,22 @@ int pg_bfs_timed(pg_ctx* ctx, int64_t count, const int32_t* roots, int32_t* out,
                  int32_t* out_len, int64_t* ticks);
 
 /*
+ * pg_bfs without allocation, for callers that cannot allocate device memory
+ * (code under HIP stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  roots_dev, out_dev
+ * [count*bfs_cap] and out_len_dev [count] are DEVICE buffers; the call only
+ * enqueues work (on `stream` when stream_given != 0, NULL being the null
+ * stream; else on the context's stream) and returns without waiting.  The
+ * neighbourhoods equal pg_bfs's.  A root outside [0, n) gets out_len 0 (the
+ * roots are not read on the host).  Scratch is one static device array of
+ * about 24 MiB per GPU, held until the process ends: count above 1048576
+ * roots is refused (PG_ECAP).  Calls that overlap on one GPU (other streams,
+ * other contexts) share that array and race; calls on one stream do not.
+ */
+int pg_bfs_noalloc(pg_ctx* ctx, int64_t count, const int32_t* roots_dev, int32_t* out_dev,
+                   int32_t* out_len_dev, void* stream, int32_t stream_given);
+
+/*
  * Device-side snapshot of the graph state (CSR, degrees, birth years,
  * log-fitness): pg_snapshot_save copies the current state into a second
  * device buffer, pg_snapshot_restore copies it back.  Used to rerun a

@@ -153,6 +153,8 @@ def lib() -> ctypes.CDLL:
             "pg_year_weights": (ctypes.c_int, [vp, f32p, f64p]),
             "pg_bfs": (ctypes.c_int, [vp, ctypes.c_int64, i32p, i32p, i32p]),
             "pg_bfs_timed": (ctypes.c_int, [vp, ctypes.c_int64, i32p, i32p, i32p, i64p]),
+            "pg_bfs_noalloc": (ctypes.c_int, [vp, ctypes.c_int64, vp, vp, vp, vp,
+                                              ctypes.c_int32]),
             "pg_growth_count": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_double]),
             "pg_snapshot_save": (ctypes.c_int, [vp]),
             "pg_snapshot_restore": (ctypes.c_int, [vp]),
@@ -379,6 +381,15 @@ class Context:
                                  _p(out, ctypes.c_int32), _p(ln, ctypes.c_int32)), "pg_bfs")
         out = out.reshape(-1, cap)
         return [out[i, :ln[i]].copy() for i in range(roots.shape[0])]
+
+    def bfs_noalloc(self, count: int, roots_dev: int, out_dev: int, out_len_dev: int,
+                    stream: int | None = None):
+        """pg_bfs_noalloc on device buffers (pointers as ints): enqueues the
+        neighbourhoods of roots_dev[count] into out_dev[count*cap] and
+        out_len_dev[count] without allocating; stream None = the context's."""
+        self._check(lib().pg_bfs_noalloc(self._h, int(count), roots_dev or None, out_dev or None,
+                                         out_len_dev or None, stream or None,
+                                         0 if stream is None else 1), "pg_bfs_noalloc")
 
     # ---- citation sequence and PA likelihood ----
     def seq_reserve(self, m: int, max_paper_years: int, max_new_papers: int,

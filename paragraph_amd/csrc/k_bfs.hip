@@ -1183,8 +1183,9 @@ __global__ __launch_bounds__(kBT, 4) void k_bfs_root(CsrView g, const float* __r
         ns_buf[u] = ns;
         const int64_t b = 16 * (1 + (int64_t)L.n1c) + 4 * (L.tot[1] + L.tot[2] - L.n1c) +
                           4 * (int64_t)ns;
-        atomicAdd(bytes_out, (unsigned long long)b);
+        if (bytes_out) atomicAdd(bytes_out, (unsigned long long)b);
     }
+    if (!w) return;  // S only (no weights given): no CDF
     __syncthreads();
     // fixed-point CDF: scale from the largest weight's exponent field.  Each
     // thread's (at most kMaxPer) weights are gathered once, loads issued
@@ -1323,6 +1324,10 @@ __device__ __forceinline__ void bfs_wave(CsrView g, const float* __restrict__ w,
     }
     const int lane = threadIdx.x;
     const int32_t root = uroot[u];
+    if (root < 0 || root >= g.n) {  // caller-given roots: out of range = empty S
+        if (lane == 0) ns_buf[u] = 0;
+        return;
+    }
     const int32_t fb = g.fwd_off[root], fe = g.fwd_off[root + 1];
     const int32_t bb = g.bwd_off[root], be = g.bwd_off[root + 1];
     const int df = fe - fb, d1 = df + (be - bb);
@@ -1423,11 +1428,16 @@ __device__ __forceinline__ void bfs_wave(CsrView g, const float* __restrict__ w,
         }
         return true;
     });
+    int32_t* S = S_buf + u * cap;
+    if (!w) {  // S only (no weights given): no CDF
+        for (int s2 = lane; s2 < ns; s2 += 64) S[s2] = Ls.win[s2];
+        if (lane == 0) ns_buf[u] = ns;
+        return;
+    }
     // S, and its fixed-point CDF from the weights.  Pass 1 writes S and
     // gathers the weights (kWb per lane per round, loads issued together)
     // into the LDS slots of their ids; pass 2 reads four consecutive
     // weights per lane, sums them locally and scans the lane totals.
-    int32_t* S = S_buf + u * cap;
     uint64_t* Q = Q_buf + u * cap;
     constexpr int kWb = 16;
     uint32_t mb = 0;
@@ -1490,7 +1500,7 @@ __device__ __forceinline__ void bfs_wave(CsrView g, const float* __restrict__ w,
     if (lane == 0) {
         ns_buf[u] = ns;
         const int64_t by = 16 * (1 + (int64_t)n1c) + 4 * ((int64_t)d1 + tot2 - n1c) + 4 * (int64_t)ns;
-        atomicAdd(bytes_out, (unsigned long long)by);
+        if (bytes_out) atomicAdd(bytes_out, (unsigned long long)by);
     }
 }
 

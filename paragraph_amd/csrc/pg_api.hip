@@ -224,6 +224,7 @@ int pg_ctx_create(const pg_config* cfg, pg_ctx** out) {
         int rc = ensure_year_edges(ctx, D * M);
         if (rc) return bail(rc);
     }
+    (void)noalloc_setup(ctx);
 #undef CREATE_HIP
     *out = ctx;
     return PG_OK;

@@ -173,6 +173,10 @@ int ensure_nb_buffers(pg_ctx* ctx);
 // Staging pool of the workgroup 2-hop path (allocated once per context).
 int ensure_bfs_stage(pg_ctx* ctx);
 
+// Looks up the device address of pg_bfs_noalloc's static scratch on the
+// context's device (current), once per device (pg_noalloc.hip).
+int noalloc_setup(pg_ctx* ctx);
+
 // Sequence store (pg_lik.hip): frees its device memory; appends the PA year
 // that pg_year_step is about to run (in-degree snapshot) and has just drawn.
 void seq_release(pg_ctx* ctx);
