@@ -24,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import paragraph_amd as pg  # noqa: E402
+from paragraph_amd.workloads import ensemble_params  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
 
@@ -412,23 +413,6 @@ def likelihood_bench(args) -> dict:
     res["autotune"] = tune
     ctx.close()
     return res
-
-
-def ensemble_params(n_sims: int, m: int = 20):
-    """The ensemble's simulations: seed sizes log-uniform in [10k, 500k],
-    alpha ~ U[0.5, 1.5], tau ~ U[1, 10] years, neighbourhood fraction
-    ~ U[0, 0.5] (n_nb = round(f*m), at most m-1), Philox keys 0..n_sims-1.
-    Drawn from numpy's PCG64 with seed 3."""
-    rng = np.random.default_rng(3)
-    out = []
-    for s in range(n_sims):
-        n0 = int(round(math.exp(rng.uniform(math.log(1e4), math.log(5e5)))))
-        alpha = float(rng.uniform(0.5, 1.5))
-        tau = float(rng.uniform(1.0, 10.0))
-        f_nb = float(rng.uniform(0.0, 0.5))
-        out.append(dict(n0=n0, alpha=alpha, tau=tau, n_nb=min(int(math.floor(f_nb * m + 0.5)), m - 1),
-                        seed=s))
-    return out
 
 
 def ensemble_bench(args) -> dict:

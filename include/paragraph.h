@@ -414,8 +414,19 @@ int pg_ensemble_load(pg_ensemble* ens, int32_t sim, int64_t n, const int32_t* bi
  * captured once into a hipGraph, replayed on later runs. */
 int pg_ensemble_run(pg_ensemble* ens, int32_t use_graph, pg_ensemble_stats* stats);
 int pg_ensemble_restore(pg_ensemble* ens);
+/* One growth year (the next one) for all simulations, eagerly, from the seed
+ * state or after earlier steps; the same kernels as pg_ensemble_run, which
+ * needs the seed state.  Lets a caller inspect every year. */
+int pg_ensemble_step(pg_ensemble* ens, pg_ensemble_stats* stats);
+/* The last year's draws of simulation `sim` in its local ids, as
+ * pg_year_citations returns them (cit/kind [n_new*m], cnt/near [n_new]), and
+ * its FP32 weights and FP64 CDF [n_exist] of that year.  Any pointer may be
+ * NULL. */
+int pg_ensemble_year_draws(pg_ensemble* ens, int32_t sim, int32_t* cit, int8_t* kind,
+                           int32_t* cnt, int32_t* near_boundary, float* w, double* cdf);
 int pg_ensemble_sim_size(pg_ensemble* ens, int32_t sim, int64_t* n_papers, int64_t* n_edges);
-/* One simulation's CSR in its local ids (buffers sized from sim_size). */
+/* One simulation's CSR in its local ids (buffers sized from sim_size: the
+ * papers after the years done so far). */
 int pg_ensemble_download(pg_ensemble* ens, int32_t sim, int32_t* fwd_off, int32_t* fwd_adj,
                          int32_t* bwd_off, int32_t* bwd_adj, int32_t* in_deg,
                          int32_t* birth);

@@ -186,6 +186,9 @@ def lib() -> ctypes.CDLL:
                                                 ctypes.c_int64, i32p, i32p]),
             "pg_ensemble_run": (ctypes.c_int, [vp, ctypes.c_int32, P(EnsembleStats)]),
             "pg_ensemble_restore": (ctypes.c_int, [vp]),
+            "pg_ensemble_step": (ctypes.c_int, [vp, P(EnsembleStats)]),
+            "pg_ensemble_year_draws": (ctypes.c_int, [vp, ctypes.c_int32, i32p, i8p, i32p, i32p,
+                                                      f32p, f64p]),
             "pg_ensemble_sim_size": (ctypes.c_int, [vp, ctypes.c_int32, i64p, i64p]),
             "pg_ensemble_download": (ctypes.c_int, [vp, ctypes.c_int32, i32p, i32p, i32p, i32p,
                                                     i32p, i32p]),
@@ -597,6 +600,28 @@ class Ensemble:
 
     def restore(self):
         self._check(lib().pg_ensemble_restore(self._h), "pg_ensemble_restore")
+
+    def step(self) -> EnsembleStats:
+        """The next growth year for every simulation (eager)."""
+        st = EnsembleStats(ctypes.sizeof(EnsembleStats))
+        self._check(lib().pg_ensemble_step(self._h, ctypes.byref(st)), "pg_ensemble_step")
+        return st
+
+    def year_draws(self, sim: int, n_new: int, n_exist: int) -> dict:
+        """The last year's draws of `sim` (local ids) and its weights and CDF."""
+        m = self.cfg.m
+        cit = np.empty(n_new * m, np.int32)
+        kind = np.empty(n_new * m, np.int8)
+        cnt = np.empty(n_new, np.int32)
+        near = np.empty(n_new, np.int32)
+        w = np.empty(n_exist, np.float32)
+        cdf = np.empty(n_exist, np.float64)
+        self._check(lib().pg_ensemble_year_draws(
+            self._h, sim, _p(cit, ctypes.c_int32), _p(kind, ctypes.c_int8),
+            _p(cnt, ctypes.c_int32), _p(near, ctypes.c_int32), _p(w, ctypes.c_float),
+            _p(cdf, ctypes.c_double)), "pg_ensemble_year_draws")
+        return {"cit": cit.reshape(n_new, m), "kind": kind.reshape(n_new, m), "cnt": cnt,
+                "near": near, "w": w, "cdf": cdf}
 
     def set_tuning(self, pa_lanes: int = 0, lik_lanes: int = 0, bfs_mode: int = 0):
         """Launch shapes (speed only): PA lanes per paper, 2-hop path mode."""

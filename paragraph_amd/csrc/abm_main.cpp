@@ -316,8 +316,12 @@ int main(int argc, char** argv) {
         }
         if (model == "er-gnp") m = 0;
         if (m > 64) throw std::runtime_error("at most 64 citations per paper");
+        if (a.has("--alpha") && !a.has("--neighbourhood-fraction") && !(a.num("--alpha", 0.0) >= 0.0))
+            throw std::runtime_error("a negative --alpha (the reference's per-paper uniform "
+                                     "neighbourhood share) is not supported by this growth "
+                                     "model; pass a share in [0, 1]");
         const double f_nb = a.has("--neighbourhood-fraction") ? a.num("--neighbourhood-fraction", 0.3)
-                            : (a.num("--alpha", -1.0) >= 0.0 ? a.num("--alpha", -1.0) : 0.3);
+                            : a.num("--alpha", 0.3);
         int n_nb = (int)std::floor(f_nb * m + 0.5);
         if (m > 0 && n_nb > m - 1) n_nb = m - 1;
         if (n_nb < 0) n_nb = 0;

@@ -63,20 +63,41 @@ int64_t scratch_need(int64_t max_papers, int64_t max_new, int64_t year_edges) {
     return std::max(a, std::max(b, c));
 }
 
+// Grows the per-year edge buffers to hold `need` edges.  The new buffers are
+// allocated first and the old ones freed only after all of them succeeded,
+// so a failed growth leaves the context with its old, consistent buffers.
 int ensure_year_edges(pg_ctx* ctx, int64_t need) {
     if (need <= ctx->year_edge_cap) return PG_OK;
-    int64_t cap = std::max(need, ctx->year_edge_cap * 2);
-    (void)hipFree(ctx->src_of); (void)hipFree(ctx->k_sorted); (void)hipFree(ctx->v_sorted);
-    (void)hipFree(ctx->k_tmp); (void)hipFree(ctx->v_tmp); (void)hipFree(ctx->scratch);
-    ctx->src_of = nullptr; ctx->k_sorted = ctx->v_sorted = ctx->k_tmp = ctx->v_tmp = nullptr;
-    ctx->scratch = nullptr;
-    PG_HIP(dalloc(&ctx->src_of, cap));
-    PG_HIP(dalloc(&ctx->k_sorted, cap));
-    PG_HIP(dalloc(&ctx->v_sorted, cap));
-    PG_HIP(dalloc(&ctx->k_tmp, cap));
-    PG_HIP(dalloc(&ctx->v_tmp, cap));
-    PG_HIP(dalloc(&ctx->scratch,
-                  scratch_need(ctx->cfg.max_papers, ctx->cfg.max_new_per_year, cap)));
+    const int64_t cap = std::max(need, ctx->year_edge_cap * 2);
+    int32_t* src_of = nullptr;
+    uint32_t *k_sorted = nullptr, *v_sorted = nullptr, *k_tmp = nullptr, *v_tmp = nullptr;
+    int64_t* scratch = nullptr;
+    hipError_t r = dalloc(&src_of, cap);
+    if (r == hipSuccess) r = dalloc(&k_sorted, cap);
+    if (r == hipSuccess) r = dalloc(&v_sorted, cap);
+    if (r == hipSuccess) r = dalloc(&k_tmp, cap);
+    if (r == hipSuccess) r = dalloc(&v_tmp, cap);
+    if (r == hipSuccess)
+        r = dalloc(&scratch, scratch_need(ctx->cfg.max_papers, ctx->cfg.max_new_per_year, cap));
+    if (r != hipSuccess) {
+        void* got[] = {src_of, k_sorted, v_sorted, k_tmp, v_tmp, scratch};
+        for (void* q : got)
+            if (q) (void)hipFree(q);
+        return fail(ctx, r == hipErrorOutOfMemory ? PG_ENOMEM : PG_EHIP,
+                    std::string("per-year edge buffers for ") + std::to_string(cap) +
+                        " edges: " + hipGetErrorString(r));
+    }
+    // the old buffers may still be read by queued work
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    void* old[] = {ctx->src_of, ctx->k_sorted, ctx->v_sorted, ctx->k_tmp, ctx->v_tmp, ctx->scratch};
+    for (void* q : old)
+        if (q) (void)hipFree(q);
+    ctx->src_of = src_of;
+    ctx->k_sorted = k_sorted;
+    ctx->v_sorted = v_sorted;
+    ctx->k_tmp = k_tmp;
+    ctx->v_tmp = v_tmp;
+    ctx->scratch = scratch;
     ctx->year_edge_cap = cap;
     return PG_OK;
 }
@@ -282,8 +303,10 @@ int pg_graph_load(pg_ctx* ctx, int64_t n, const int32_t* birth, const float* log
             for (int32_t k = fo[v]; k < fo[v + 1]; ++k) badj[cur[adj[k]]++] = (int32_t)v;
     }
     PG_HIP(hipSetDevice(ctx->cfg.device));
-    if (ctx->logfit) { (void)hipFree(ctx->logfit); ctx->logfit = nullptr; }
-    PG_HIP(dalloc(&ctx->logfit, ctx->cfg.max_papers));
+    // the device state is overwritten below: until the load completes the
+    // context holds no graph
+    ctx->loaded = false;
+    if (!ctx->logfit) PG_HIP(dalloc(&ctx->logfit, ctx->cfg.max_papers));
     hipStream_t s = ctx->stream;
     PG_HIP(hipMemcpyAsync(ctx->birth, birth, n * sizeof(int32_t), hipMemcpyHostToDevice, s));
     PG_HIP(hipMemcpyAsync(ctx->logfit, logfit, n_logfit * sizeof(float), hipMemcpyHostToDevice, s));

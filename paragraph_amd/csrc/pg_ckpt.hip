@@ -144,6 +144,43 @@ int pg_checkpoint_save(pg_ctx* ctx, const char* path, const void* user, int64_t 
     return PG_OK;
 }
 
+// The checksum only catches accidental damage, so the state is checked for
+// what the kernels rely on before any of it reaches the device (O(n + e)):
+// offsets monotone from 0 to e, ids in [0, n), rows strictly ascending and
+// free of self-citations, the backward CSR the transpose of the forward one
+// (citers ascending), and in-degrees equal to the backward row lengths.
+// Returns null when consistent, else what failed.
+static const char* check_state(int64_t n, int64_t e, const std::vector<int32_t>& birth,
+                               const std::vector<int32_t>& indeg, const std::vector<int32_t>& fo,
+                               const std::vector<int32_t>& fa, const std::vector<int32_t>& bo,
+                               const std::vector<int32_t>& ba) {
+    (void)birth;
+    if (fo[0] != 0 || fo[n] != e || bo[0] != 0 || bo[n] != e) return "offset ends";
+    for (int64_t v = 0; v < n; ++v) {
+        if (fo[v + 1] < fo[v] || bo[v + 1] < bo[v]) return "offsets not monotone";
+        if (indeg[v] != bo[v + 1] - bo[v]) return "in-degree is not the backward row length";
+    }
+    // forward rows: ids in range, strictly ascending, no self-citation; count
+    // each target's citers for the transpose check
+    std::vector<int32_t> cur(bo.begin(), bo.end() - 1);
+    for (int64_t v = 0; v < n; ++v) {
+        for (int64_t k = fo[v]; k < fo[v + 1]; ++k) {
+            const int32_t t = fa[k];
+            if (t < 0 || t >= n) return "forward id out of range";
+            if (t == v) return "self-citation";
+            if (k > fo[v] && fa[k - 1] >= t) return "forward row not strictly ascending";
+            // citers of t arrive in ascending v, so they must fill t's
+            // backward row in order
+            if (cur[t] >= bo[t + 1] || ba[cur[t]] != (int32_t)v)
+                return "backward rows are not the transpose of the forward rows";
+            ++cur[t];
+        }
+    }
+    for (int64_t t = 0; t < n; ++t)
+        if (cur[t] != bo[t + 1]) return "backward rows are not the transpose of the forward rows";
+    return nullptr;
+}
+
 int pg_checkpoint_load(pg_ctx* ctx, const char* path, void* user, int64_t user_cap,
                        int64_t* user_bytes) {
     if (!ctx) return PG_EINVAL;
@@ -177,14 +214,16 @@ int pg_checkpoint_load(pg_ctx* ctx, const char* path, void* user, int64_t user_c
                     get(f.f, &sum, lf.data(), nl * 4) && get(f.f, &sum, ub.data(), ub.size());
     if (!ok) return fail(ctx, PG_EIO, std::string("pg_checkpoint_load: truncated file ") + path);
     if (sum.h != h.checksum) return fail(ctx, PG_EIO, "pg_checkpoint_load: checksum mismatch");
-    if (fo[0] != 0 || fo[n] != e || bo[0] != 0 || bo[n] != e)
-        return fail(ctx, PG_EIO, "pg_checkpoint_load: inconsistent CSR offsets");
+    if (const char* why = check_state(n, e, birth, indeg, fo, fa, bo, ba))
+        return fail(ctx, PG_EIO, std::string("pg_checkpoint_load: inconsistent graph state (") +
+                                     why + ")");
     PG_HIP(hipSetDevice(ctx->cfg.device));
     if (!ctx->logfit) {
         PG_HIP(dalloc(&ctx->logfit, ctx->cfg.max_papers));
         ctx->logfit_owned = false;
     }
     hipStream_t s = ctx->stream;
+    ctx->loaded = false;  // until the copies below complete
     ctx->cur = 0;
     PG_HIP(hipMemcpyAsync(ctx->birth, birth.data(), n * 4, hipMemcpyHostToDevice, s));
     PG_HIP(hipMemcpyAsync(ctx->in_deg, indeg.data(), n * 4, hipMemcpyHostToDevice, s));

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -162,10 +163,22 @@ inline int fail(pg_ctx* ctx, int code, const std::string& msg) {
     return code;
 }
 
+// Device allocation of `count` elements (at least one).  Test hook: with
+// PG_TEST_FAIL_ALLOC_BYTES=N in the environment, every request of N bytes
+// or more fails as out of memory, so tests can check that a failed growth
+// leaves a context consistent.
 template <typename T>
 inline hipError_t dalloc(T** p, int64_t count) {
     if (count <= 0) count = 1;
-    return hipMalloc(reinterpret_cast<void**>(p), (size_t)count * sizeof(T));
+    const size_t bytes = (size_t)count * sizeof(T);
+    if (const char* lim = std::getenv("PG_TEST_FAIL_ALLOC_BYTES")) {
+        const long long n = std::atoll(lim);
+        if (n > 0 && bytes >= (size_t)n) {
+            *p = nullptr;
+            return hipErrorOutOfMemory;
+        }
+    }
+    return hipMalloc(reinterpret_cast<void**>(p), bytes);
 }
 
 // Neighbourhood buffers sized by cfg.max_new_per_year and cfg.bfs_cap.

@@ -45,6 +45,7 @@ struct pg_ensemble {
     int64_t bwd_fill = 0;                  // backward entries loaded so far
     bool ready = false;                    // all loaded and snapshotted
     bool fresh = false;                    // state is the seed state
+    int next_year = 0;                     // growth years done since the seed state
     int pa_lanes = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -347,6 +348,66 @@ int pg_ensemble_restore(pg_ensemble* ens) {
     ens->ctx->n = ens->TOT;
     ens->ctx->e = ens->ETOT;
     ens->fresh = true;
+    ens->next_year = 0;
+    return PG_OK;
+}
+
+int pg_ensemble_step(pg_ensemble* ens, pg_ensemble_stats* stats) {
+    if (!ens) return PG_EINVAL;
+    if (stats && stats->size != sizeof(pg_ensemble_stats))
+        return efail(ens, PG_EINVAL, "pg_ensemble_step: pg_ensemble_stats.size mismatch");
+    if (!ens->ready) return efail(ens, PG_ESTATE, "pg_ensemble_step: not all simulations loaded");
+    if (ens->next_year >= ens->cfg.n_years)
+        return efail(ens, PG_ESTATE, "pg_ensemble_step: every growth year is done");
+    pg_ctx* c = ens->ctx;
+    EHIP(hipSetDevice(ens->cfg.device));
+    hipStream_t s = c->stream;
+    const int y = ens->next_year;
+    EHIP(hipEventRecord(c->ev0, s));
+    launch_fill_i32(s, reinterpret_cast<int32_t*>(ens->counters), 0, 16);
+    enqueue_year(ens, y, s);
+    EHIP(hipGetLastError());
+    EHIP(hipEventRecord(c->ev1, s));
+    EHIP(hipEventSynchronize(c->ev1));
+    ens->fresh = false;
+    ens->next_year = y + 1;
+    unsigned long long h[8];
+    EHIP(hipMemcpy(h, ens->counters, sizeof h, hipMemcpyDeviceToHost));
+    if (stats) {
+        float ms = 0.0f;
+        EHIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        stats->edges_added = (int64_t)h[4];
+        stats->near_boundary = (int64_t)h[0];
+        stats->pa_attempts = (int64_t)h[1];
+        stats->shortfall = ens->year_new[y] * ens->cfg.m - (int64_t)h[4];
+        stats->device_ms = ms;
+    }
+    return PG_OK;
+}
+
+int pg_ensemble_year_draws(pg_ensemble* ens, int32_t sim, int32_t* cit, int8_t* kind,
+                           int32_t* cnt, int32_t* near_boundary, float* w, double* cdf) {
+    if (!ens) return PG_EINVAL;
+    if (sim < 0 || sim >= ens->cfg.n_sims || !ens->ready)
+        return efail(ens, PG_EINVAL, "pg_ensemble_year_draws: bad simulation or not loaded");
+    if (ens->next_year < 1)
+        return efail(ens, PG_ESTATE, "pg_ensemble_year_draws: no growth year done");
+    const int y = ens->next_year - 1, m = ens->cfg.m;
+    const BatchSim& b = ens->host_tab[(size_t)y * ens->cfg.n_sims + sim];
+    pg_ctx* c = ens->ctx;
+    EHIP(hipSetDevice(ens->cfg.device));
+    const int64_t D = b.n_new, N = b.n_exist;
+    if (cit && D) {
+        EHIP(hipMemcpy(cit, c->cit + b.new_first * m, D * m * 4, hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < D * m; ++k)
+            if (cit[k] >= 0) cit[k] -= (int32_t)b.base;
+    }
+    if (kind && D) EHIP(hipMemcpy(kind, c->kind + b.new_first * m, D * m, hipMemcpyDeviceToHost));
+    if (cnt && D) EHIP(hipMemcpy(cnt, c->cnt + b.new_first, D * 4, hipMemcpyDeviceToHost));
+    if (near_boundary && D)
+        EHIP(hipMemcpy(near_boundary, c->near + b.new_first, D * 4, hipMemcpyDeviceToHost));
+    if (w && N) EHIP(hipMemcpy(w, c->w + b.base, N * 4, hipMemcpyDeviceToHost));
+    if (cdf && N) EHIP(hipMemcpy(cdf, c->cdf + b.base, N * 8, hipMemcpyDeviceToHost));
     return PG_OK;
 }
 
@@ -386,6 +447,7 @@ int pg_ensemble_run(pg_ensemble* ens, int32_t use_graph, pg_ensemble_stats* stat
     EHIP(hipEventRecord(c->ev1, s));
     EHIP(hipEventSynchronize(c->ev1));
     ens->fresh = false;
+    ens->next_year = ens->cfg.n_years;
     unsigned long long h[8];
     EHIP(hipMemcpy(h, ens->counters, sizeof h, hipMemcpyDeviceToHost));
     if (stats) {
@@ -406,7 +468,8 @@ int pg_ensemble_sim_size(pg_ensemble* ens, int32_t sim, int64_t* n_papers, int64
     if (!ens) return PG_EINVAL;
     if (sim < 0 || sim >= ens->cfg.n_sims || !ens->ready)
         return efail(ens, PG_EINVAL, "pg_ensemble_sim_size: bad simulation or not loaded");
-    const int64_t N = ens->fresh ? ens->sims[sim].n0 : ens->nfinal[sim];
+    int64_t N = ens->sims[sim].n0;
+    for (int y = 0; y < ens->next_year; ++y) N += ens->D[sim][y];
     int32_t b0 = 0, b1 = 0;
     EHIP(hipMemcpy(&b0, ens->ctx->fwd_off + ens->R[sim], 4, hipMemcpyDeviceToHost));
     EHIP(hipMemcpy(&b1, ens->ctx->fwd_off + ens->R[sim] + N, 4, hipMemcpyDeviceToHost));

@@ -1,0 +1,191 @@
+"""The four full-size workloads against the oracle (needs an MI355X).
+
+  - PA, 10 years at 3% from the 2,000,000-paper seed (m=20, 14 PA + 6
+    neighbourhood, cap 4096): every year from the GPU's own state -- weights
+    bit-exact, CDF within 1e-12, draws exact except flagged papers, CSR bytes
+    exact after the append; 13,756,740 edges.
+  - ER G(n,p), 1,000,000 isolated seeds, 10 years at 3%, p=1e-5: every
+    year's edge set and CSR exact.
+  - The 256-simulation ensemble: stepped year by year, eight simulations
+    (smallest and largest seed, alpha and tau extremes, no and most
+    neighbourhood draws) checked each year directly against the oracle from
+    the ensemble's own state (weights, CDF, draws, CSR); every simulation's
+    final graph equal to the captured-graph run and to a plain run.
+  - The likelihood of the 30-year sequence from a 1,000,000-paper seed
+    (28,545,720 slots): log L and the gradient against the same-arithmetic
+    oracle (1e-9) and the FP64 oracle (1e-5).
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+import paragraph_amd as pg
+from gpu_checks import check_pa_year, compare_append, compare_year, log, new_ctx, oracle_graph
+from oracle import binding as ob
+from paragraph_amd.workloads import ensemble_params
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(1200)]
+
+KEYS = ("fwd_off", "fwd_adj", "bwd_off", "bwd_adj", "in_deg", "birth")
+
+
+def test_pa_2m_seed_ten_years():
+    n0, m, n_nb = 2_000_000, 20, 6
+    sched = pg.growth_schedule(n0, 0.03, 10)
+    g = pg.seed_graph(1, n0, 10.0, n_final=n0 + sum(sched))
+    ctx = new_ctx(g, sched, m, cap=4096, seed=1)
+    _, e0 = ctx.size()
+    added, flagged_bad = 0, 0
+    for yi, D in enumerate(sched):
+        st, nb = check_pa_year(ctx, g.logfit, 1 + yi, yi, D, m, n_nb, 1.0, 5.0, 1, 4096)
+        assert st.shortfall == 0
+        added += st.edges_added
+        flagged_bad += nb
+    log(f"2M seed, 10 years: {added} edges, {flagged_bad} flagged papers differ")
+    assert added == 13_756_740
+    assert ctx.size() == (2_687_837, e0 + 13_756_740)
+    ctx.close()
+
+
+def test_er_1m_seed_ten_years():
+    n0, p = 1_000_000, 1e-5
+    sched = pg.growth_schedule(n0, 0.03, 10)
+    none = np.zeros(0, np.int32)
+    n_final = n0 + sum(sched)
+    g = pg.SeedGraph(np.zeros(n0, np.int32), none, none, np.zeros(n_final, np.float32))
+    ctx = pg.Context(n_final, 6_000_000, max(sched), max_cites=1, bfs_cap=16, seed=2)
+    ctx.load(g)
+    G = oracle_graph(ctx, g.logfit)
+    total = 0
+    for yi, D in enumerate(sched):
+        N = G.n
+        st = ctx.year_er(1 + yi, yi, D, p)
+        off, tgt = ob.er_year(N, D, p, 2, yi)
+        log(f"ER year {yi}: N={N} D={D} edges={st.edges_added} oracle={tgt.size}")
+        assert st.edges_added == tgt.size
+        d = ctx.download()
+        fo = d["fwd_off"]
+        assert np.array_equal(d["fwd_adj"][fo[N]:fo[N + D]], tgt)
+        assert np.array_equal(fo[N:N + D + 1] - fo[N], off)
+        G = ob.csr_append(G, off, tgt, g.logfit, 1 + yi)
+        for k in KEYS:
+            assert np.array_equal(d[k], getattr(G, k)), k
+        total += st.edges_added
+    log(f"ER total {total} edges")
+    assert 3_800_000 < total < 4_150_000  # mean 3.97M, sd about 2k
+    ctx.close()
+
+
+def state_hash(d: dict) -> str:
+    h = hashlib.sha1()
+    for k in KEYS:
+        h.update(np.ascontiguousarray(d[k]).tobytes())
+    return h.hexdigest()
+
+
+def watched_sims(sims) -> list:
+    n0 = [s["n0"] for s in sims]
+    a = [s["alpha"] for s in sims]
+    t = [s["tau"] for s in sims]
+    nb = [s["n_nb"] for s in sims]
+    pick = [int(np.argmin(n0)), int(np.argmax(n0)), int(np.argmin(a)), int(np.argmax(a)),
+            int(np.argmin(t)), int(np.argmax(t)), int(np.argmax(nb)),
+            int(np.nonzero(np.array(nb) == 0)[0][0])]
+    out = []
+    for s in pick:
+        if s not in out:
+            out.append(s)
+    return out
+
+
+def test_ensemble_256_against_oracle_and_plain_runs():
+    m, years, rate = 20, 10, 0.03
+    sims = ensemble_params(256, m)
+    graphs = []
+    for sp in sims:
+        sched = pg.growth_schedule(sp["n0"], rate, years)
+        g = pg.seed_graph(sp["seed"], sp["n0"], 10.0, n_final=sp["n0"] + sum(sched))
+        graphs.append((g, sched))
+        sp["seed_edges"] = int(g.src.size)
+    ens = pg.Ensemble(sims, years, m, rate)
+    for s, (g, _) in enumerate(graphs):
+        ens.load(s, g)
+    watch = watched_sims(sims)
+    log(f"ensemble: 256 simulations, watched {watch}")
+    stepped_edges = 0
+    for y in range(years):
+        before = {s: ens.download(s) for s in watch}
+        st = ens.step()
+        stepped_edges += st.edges_added
+        for s in watch:
+            sp, (g, sched) = sims[s], graphs[s]
+            G = ob.Graph.from_download(before[s], g.logfit)
+            D, N = sched[y], G.n
+            got = ens.year_draws(s, D, N)
+            _, bad = compare_year(G, got, got["w"], got["cdf"], 1 + y, y, D, m, sp["n_nb"],
+                                  sp["alpha"], sp["tau"], sp["seed"], 4096)
+            compare_append(G, got, ens.download(s), g.logfit, 1 + y)
+            log(f"  year {y} sim {s}: N={N} D={D} mismatching flagged papers {bad.size}")
+    stepped = [state_hash(ens.download(s)) for s in range(len(sims))]
+    ens.restore()
+    st = ens.run(use_graph=True)
+    assert st.edges_added == stepped_edges
+    for s in range(len(sims)):
+        assert state_hash(ens.download(s)) == stepped[s], f"sim {s}: graph run differs"
+    ens.close()
+    log(f"stepped and captured runs agree: {stepped_edges} edges")
+    plain_edges = 0
+    for s, (sp, (g, sched)) in enumerate(zip(sims, graphs)):
+        ctx = new_ctx(g, sched, m, cap=4096, seed=sp["seed"])
+        for yi, d in enumerate(sched):
+            plain_edges += ctx.year_pa(1 + yi, yi, d, m, sp["n_nb"], sp["alpha"],
+                                       sp["tau"]).edges_added
+        assert state_hash(ctx.download()) == stepped[s], f"sim {s}: plain run differs"
+        ctx.close()
+    assert plain_edges == stepped_edges
+    log(f"plain runs agree: {plain_edges} edges")
+
+
+def test_likelihood_30y_1m_seed():
+    torch = pytest.importorskip("torch")
+    n0, years, rate, m, n_nb = 1_000_000, 30, 0.03, 20, 6
+    sched = pg.growth_schedule(n0, rate, years)
+    n_final = n0 + sum(sched)
+    g = pg.seed_graph(4, n0, 8.0, n_final=n_final)
+    ctx = new_ctx(g, sched, m, cap=4096, seed=4)
+    paper_years = sum(n0 + sum(sched[:y]) for y in range(years))
+    ctx.seq_reserve(m, paper_years, sum(sched))
+    ctx.seq_record(True)
+    for yi, d in enumerate(sched):
+        ctx.year_pa(1 + yi, yi, d, m, n_nb, 1.2, 4.0)
+    ctx.seq_record(False)
+    info = ctx.seq_info()
+    assert info.n_papers == n_final and info.new_papers * m == 28_545_720
+    seq = ctx.seq_download()
+    phi = torch.from_numpy(np.ascontiguousarray(g.logfit[:n_final])).to("cuda")
+    grad = torch.empty(n_final, dtype=torch.float64, device="cuda")
+    for a, t in [(1.0, 5.0), (1.2, 4.0)]:
+        ll, ga, gt = ctx.pa_loglik_grad(a, t, phi.data_ptr(), grad.data_ptr())
+        gphi = grad.cpu().numpy()
+        kw = dict(n_exist=seq["n_exist"], n_new=seq["n_new"], year_of=seq["year_of"],
+                  deg_list=seq["deg"], cit_list=seq["cit"], kind_list=seq["kind"],
+                  cnt_list=seq["cnt"], m=m, birth=seq["birth"], logfit=g.logfit[:n_final],
+                  alpha=a, tau=t)
+        w = ob.loglik_grad(**kw, w32=True)
+        d = ob.loglik_grad(**kw, w32=False)
+        rel = lambda x, y: abs(x - y) / abs(y)
+        log(f"alpha={a} tau={t}: log L {ll:.12g} (same arith {w[0]:.12g}, fp64 {d[0]:.12g}); "
+            f"dalpha {ga:.10g} / {w[1]:.10g} / {d[1]:.10g}; dtau {gt:.10g} / {w[2]:.10g} / {d[2]:.10g}")
+        assert rel(ll, w[0]) <= 1e-9 and rel(ga, w[1]) <= 1e-9 and rel(gt, w[2]) <= 1e-9
+        # dL/dphi_j sums up to ~1e5 slot shares of a hub and 30 years of
+        # A_y*w_j in another FP64 order than the oracle: 1e-8 on max(1, |g|)
+        # (the measured worst paper is about 1.6e-9), 1e-10 over the vector
+        err = np.abs(gphi - w[3]) / np.maximum(1.0, np.abs(w[3]))
+        log(f"  dL/dphi vs same arithmetic: max {err.max():.2e}, 2-norm "
+            f"{np.linalg.norm(gphi - w[3]) / np.linalg.norm(w[3]):.2e}")
+        assert err.max() <= 1e-8
+        assert np.linalg.norm(gphi - w[3]) / np.linalg.norm(w[3]) <= 1e-10
+        assert rel(ll, d[0]) <= 1e-5 and rel(ga, d[1]) <= 1e-5 and rel(gt, d[2]) <= 1e-5
+        assert np.linalg.norm(gphi - d[3]) / np.linalg.norm(d[3]) <= 1e-5
+    ctx.close()
