@@ -92,7 +92,8 @@ enum {
     PG_K_CSR_APPEND = 5,       /* forward append, radix group, backward merge */
     PG_K_LIK = 6,              /* pg_pa_loglik evaluations (all passes) */
     PG_K_LIK_GRAD = 7,         /* pg_pa_loglik_grad evaluations (all passes) */
-    PG_K_COUNT = 8
+    PG_K_LIK_BATCH = 8,        /* pg_pa_loglik_grad_batch calls (all passes) */
+    PG_K_COUNT = 9
 };
 
 /* Returns PG_ABI_VERSION. */
@@ -250,6 +251,14 @@ int pg_checkpoint_load(pg_ctx* ctx, const char* path, void* user, int64_t user_c
  *                sum written and read, dL/dphi written) + 4 B per citing
  *                paper + 33 B per filled slot (17 B as LIK + 8 B share
  *                written + 8 B sorted position and share read back).
+ *   LIK_BATCH    per call at B points: shared by the points, 8 B per
+ *                paper-year (log(k+1) in both paper passes) + 12 B per paper
+ *                (birth twice, segment offset) + 4 B per citing paper + 21 B
+ *                per filled slot (target, kind, {log(k+1), age} record,
+ *                by-target position, sorted key); per point, 32 B per paper
+ *                (log-fitness twice, share sum written and read, dL/dphi) +
+ *                20 B per filled slot (log-fitness gathered, share written
+ *                and read back).
  */
 int pg_profile_enable(pg_ctx* ctx, int on);
 int pg_profile_read(pg_ctx* ctx, int n, double* ms, double* bytes, int64_t* launches);
@@ -338,6 +347,39 @@ int pg_pa_loglik(pg_ctx* ctx, const pg_lik_params* p, pg_lik_out* out);
  * sorted by target, built once per sequence), so repeated calls return the
  * same bits. */
 int pg_pa_loglik_grad(pg_ctx* ctx, const pg_lik_params* p, pg_lik_out* out);
+
+/*
+ * log L and its gradient at B parameter points in one pass over the
+ * sequence (B = 4, 8 or 16): point b has alpha_b, tau_b and its own
+ * log-fitness vector.  Each point's results equal pg_pa_loglik_grad's at
+ * that point up to FP64 summation order.  The sequence's streams are read
+ * once for all points and the per-point arrays interleave the points, so
+ * every gathered log-fitness line and every stored share line serves B
+ * evaluations.  No reference counterpart (the reference has no likelihood).
+ */
+typedef struct {
+    uint32_t size;
+    int32_t n_points;           /* B: 4, 8 or 16 (B * lanes per citing paper <= 64) */
+    const double* alpha_tau;    /* HOST double[2B] {alpha_b, tau_b}, used when
+                                   alpha_tau_dev is NULL */
+    const double* alpha_tau_dev;/* optional DEVICE double[2B] */
+    const float* logfit_dev;    /* DEVICE float[n_papers * B]: phi_b of paper j
+                                   at j*B + b */
+    void* stream;               /* as pg_lik_params */
+    int32_t stream_given;
+} pg_lik_batch_params;
+
+typedef struct {
+    uint32_t size;
+    double* scalars_dev;        /* optional DEVICE double[3B]: {log L, dL/dalpha,
+                                   dL/dtau} of point b at 3b */
+    double* grad_logfit_dev;    /* DEVICE double[n_papers * B]: dL/dphi_b of
+                                   paper j at j*B + b (overwritten) */
+    double* host;               /* optional HOST double[3B] as scalars_dev; when
+                                   set, the call waits for the result */
+} pg_lik_batch_out;
+
+int pg_pa_loglik_grad_batch(pg_ctx* ctx, const pg_lik_batch_params* p, pg_lik_batch_out* out);
 
 /*
  * ---- Batched ensembles ----

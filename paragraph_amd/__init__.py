@@ -69,6 +69,18 @@ class LikOut(ctypes.Structure):
                 ("grad_logfit_dev", ctypes.c_void_p), ("host3", ctypes.c_void_p)]
 
 
+class LikBatchParams(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("n_points", ctypes.c_int32),
+                ("alpha_tau", ctypes.c_void_p), ("alpha_tau_dev", ctypes.c_void_p),
+                ("logfit_dev", ctypes.c_void_p), ("stream", ctypes.c_void_p),
+                ("stream_given", ctypes.c_int32)]
+
+
+class LikBatchOut(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("scalars_dev", ctypes.c_void_p),
+                ("grad_logfit_dev", ctypes.c_void_p), ("host", ctypes.c_void_p)]
+
+
 class EnsembleConfig(ctypes.Structure):
     _fields_ = [("size", ctypes.c_uint32), ("device", ctypes.c_int32), ("n_sims", ctypes.c_int32),
                 ("n_years", ctypes.c_int32), ("m", ctypes.c_int32), ("bfs_cap", ctypes.c_int32),
@@ -112,7 +124,7 @@ class TuneReport(ctypes.Structure):
 
 
 KERNEL_GROUPS = ["weights_cdf", "pa_draw", "bfs", "nb_draw", "er", "csr_append", "lik",
-                 "lik_grad"]
+                 "lik_grad", "lik_batch"]
 
 
 _lib = None
@@ -198,6 +210,7 @@ def lib() -> ctypes.CDLL:
             "pg_lik_autotune": (ctypes.c_int, [vp, P(LikParams), ctypes.c_int32, P(TuneReport)]),
             "pg_pa_loglik": (ctypes.c_int, [vp, P(LikParams), P(LikOut)]),
             "pg_pa_loglik_grad": (ctypes.c_int, [vp, P(LikParams), P(LikOut)]),
+            "pg_pa_loglik_grad_batch": (ctypes.c_int, [vp, P(LikBatchParams), P(LikBatchOut)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -475,6 +488,25 @@ class Context:
     def pa_loglik_grad(self, alpha: float, tau: float, logfit_dev: int, grad_dev: int, **kw):
         """log L, dL/dalpha, dL/dtau (host=True) and dL/dphi into grad_dev (FP64)."""
         return self._lik("pg_pa_loglik_grad", alpha, tau, logfit_dev, grad_dev=grad_dev, **kw)
+
+    def pa_loglik_grad_batch(self, alpha_tau, logfit_dev: int, grad_dev: int, *,
+                             alpha_tau_dev: int = 0, scalars_dev: int = 0,
+                             stream: int | None = None, host: bool = True):
+        """log L and its gradient at B = len(alpha_tau) points in one pass.
+        logfit_dev: device float[n_papers*B] (point b of paper j at j*B+b);
+        grad_dev: device double[n_papers*B].  Returns a (B, 3) array of
+        {log L, dL/dalpha, dL/dtau} when host=True."""
+        at = np.ascontiguousarray(np.asarray(alpha_tau, np.float64).reshape(-1, 2))
+        B = at.shape[0]
+        p = LikBatchParams(ctypes.sizeof(LikBatchParams), B, at.ctypes.data,
+                           alpha_tau_dev or None, logfit_dev or None, stream or None,
+                           0 if stream is None else 1)
+        h = np.zeros((B, 3), np.float64)
+        o = LikBatchOut(ctypes.sizeof(LikBatchOut), scalars_dev or None, grad_dev or None,
+                        h.ctypes.data if host else None)
+        self._check(lib().pg_pa_loglik_grad_batch(self._h, ctypes.byref(p), ctypes.byref(o)),
+                    "pg_pa_loglik_grad_batch")
+        return h if host else None
 
     def lik_autotune(self, alpha: float, tau: float, logfit_dev: int, reps: int = 3) -> dict:
         """Times the likelihood's lanes-per-citing-paper candidates on this

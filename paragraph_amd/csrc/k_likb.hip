@@ -1,0 +1,611 @@
+// k_likb.hip -- the PA log-likelihood and its gradient at B parameter points
+// (alpha_b, tau_b, phi_b) in one pass over a recorded citation sequence.  No
+// reference counterpart.  Same model, arithmetic and pass structure as
+// k_lik.hip; what changes is the layout, so that every stream of the
+// sequence (log(k+1) per paper-year, the per-slot target, kind, record and
+// sorted position) is read once for all B points and every random access
+// moves B contiguous values:
+//   phi      float [paper][B]     log-fitness of point b at j*B + b
+//   grad     double[paper][B]     dL/dphi
+//   share    double[sorted slot][B]  per-slot dL/dphi shares (by target)
+//   partials double[...][B], per-year sums double[year][col][B]
+// A lane owns one point b; the B lanes of a paper (or slot group) are
+// consecutive, b fastest, so the loads of phi and the stores of the shares
+// are contiguous runs of B values.  Sums keep a fixed order (point b's sums
+// do not depend on the other points), so results are bit-reproducible.
+#include <hip/hip_runtime.h>
+
+#include "pg_internal.h"
+#include "pg_math.h"
+
+namespace pg {
+
+namespace {
+
+constexpr int kBT = 256;          // threads per block, every pass
+constexpr int kWv = kBT / 64;
+constexpr int kYChunk = 8;        // paper passes: years per LDS reduction round
+
+struct PScal {
+    float alpha_f, inv_tau_f;
+    double itau2;
+};
+
+// Point b's alpha and 1/tau as the simulator's weight arithmetic takes them.
+__device__ inline PScal point_scal(const double* __restrict__ at, int b) {
+    const double alpha = at[2 * b], tau = at[2 * b + 1];
+    PScal s;
+    s.alpha_f = (float)alpha;
+    s.inv_tau_f = (float)(1.0 / tau);
+    s.itau2 = 1.0 / (tau * tau);
+    return s;
+}
+
+__device__ inline float weight_la(float L, float age, float phi, const PScal& s, float& x) {
+    const float t1 = __builtin_fmaf(-age, s.inv_tau_f, phi);
+    x = __builtin_fmaf(s.alpha_f, L, t1);
+    return det_expf(x);
+}
+
+// Sum over the lanes of a wave that hold the same point (lane % B): the
+// result is in every lane, fixed order (butterflies over the offsets >= B).
+template <int B>
+__device__ inline double wave_sum_b(double v) {
+#pragma unroll
+    for (int o = 32; o >= B; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum over the block of the threads with the same point (threadIdx.x % B):
+// returned in the threads t < B (for point t); fixed order.
+template <int B>
+__device__ inline double block_sum_b(double v, double* red /* [kWv * B] */) {
+    v = wave_sum_b<B>(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane < B) red[wave * B + lane] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x < B) {
+#pragma unroll
+        for (int w = 0; w < kWv; ++w) t += red[w * B + threadIdx.x];
+    }
+    return t;
+}
+
+template <int B>
+constexpr int seg_chunk() { return kLikBSegPer * (kBT / B); }
+
+// Slot shares of paper j for point b from the segment sums.
+template <int B>
+__device__ inline double slot_share_b(int64_t j, int b, const int32_t* __restrict__ seg_off,
+                                      const double* __restrict__ evt,
+                                      const double* __restrict__ carry) {
+    const int64_t o0 = seg_off[j], o1 = seg_off[j + 1];
+    if (o0 == o1) return 0.0;
+    constexpr int C = seg_chunk<B>();
+    const int64_t c0 = o0 / C, c1 = (o1 - 1) / C;
+    if (c0 == c1) return evt[j * B + b];
+    double e = carry[(2 * c0 + 1) * B + b];
+    for (int64_t c = c0 + 1; c < c1; ++c) e += carry[2 * c * B + b];
+    return e + carry[2 * c1 * B + b];
+}
+
+// Paper-major passes over tiles of PPB = kLikBPPT*kBT/B papers: block blk
+// takes tiles blk, blk + nblk, ...; in a tile, thread t holds point
+// b = t % B and papers j = tile*PPB + r*(kBT/B) + t/B (r < kLikBPPT).
+//   BWD = false: part[(y*nblk + blk)*B + b] = sum of the block's w_j(y) at b
+//                (per-year totals kept in LDS across the block's tiles:
+//                dynamic shared memory of n_years*B doubles).
+//   BWD = true : grad[j*B + b] = slot shares - sum_y A_y w_j(y);
+//                part[(c*nblk + blk)*B + b], c = 0, 1: sums of A w L, A w age.
+// Every sum runs in a fixed order (tiles in order, waves in order).
+template <bool BWD, int B>
+__global__ __launch_bounds__(kBT) void k_likb_papers(LikSeq q, const double* __restrict__ at,
+                                                     const float* __restrict__ logfit,
+                                                     const double* __restrict__ year_sums,
+                                                     LikSegs sg, double* __restrict__ grad,
+                                                     double* __restrict__ part) {
+    constexpr int R = kBT / B;                     // papers per row
+    constexpr int PPB = kLikBPPT * R;
+    __shared__ double s_red[kYChunk][kWv * B];
+    extern __shared__ double s_tot[];              // [n_years * B] (forward pass)
+    const int b = threadIdx.x % B;
+    const int pr = threadIdx.x / B;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const PScal s = point_scal(at, b);
+    const int64_t nblk = gridDim.x;
+    const int64_t ntile = (q.n_papers + PPB - 1) / PPB;
+    if (!BWD)
+        for (int k = threadIdx.x; k < q.n_years * B; k += kBT) s_tot[k] = 0.0;
+    double gL = 0.0, ga = 0.0;
+    for (int64_t tile = blockIdx.x; tile < ntile; tile += nblk) {
+        const int64_t j0 = tile * PPB;
+        float phi[kLikBPPT];
+        int32_t bi[kLikBPPT];
+        double g[kLikBPPT];
+#pragma unroll
+        for (int r = 0; r < kLikBPPT; ++r) {
+            const int64_t j = j0 + r * R + pr;
+            phi[r] = 0.0f;
+            bi[r] = 0;
+            g[r] = 0.0;
+            if (j < q.n_papers) {
+                phi[r] = logfit[j * B + b];
+                bi[r] = q.birth[j];
+            }
+        }
+        for (int yc = 0; yc < q.n_years; yc += kYChunk) {
+            const int ny = q.n_years - yc < kYChunk ? q.n_years - yc : kYChunk;
+            if (j0 >= q.n_exist[yc + ny - 1]) continue;   // tile-uniform: no paper exists yet
+#pragma unroll
+            for (int y4 = 0; y4 < kYChunk; y4 += 4) {
+                // four years of log(k+1) loads issued before any is used
+                float Lv[4][kLikBPPT];
+#pragma unroll
+                for (int yy = 0; yy < 4; ++yy) {
+                    const int y = yc + y4 + yy;
+                    const int64_t Ny = y4 + yy < ny ? q.n_exist[y] : 0;
+                    const int64_t doff = y4 + yy < ny ? q.deg_off[y] : 0;
+#pragma unroll
+                    for (int r = 0; r < kLikBPPT; ++r) {
+                        const int64_t j = j0 + r * R + pr;
+                        Lv[yy][r] = j < Ny ? q.Lpy[doff + j] : 0.0f;
+                    }
+                }
+#pragma unroll
+                for (int yy = 0; yy < 4; ++yy) {
+                    const int yl = y4 + yy;
+                    if (yl >= ny) break;
+                    const int y = yc + yl;
+                    const int64_t Ny = q.n_exist[y];
+                    const int32_t year = q.year_of[y];
+                    const double Ay = BWD ? year_sums[(y * kLikYearCols + 1) * B + b] : 0.0;
+                    double zacc = 0.0;
+#pragma unroll
+                    for (int r = 0; r < kLikBPPT; ++r) {
+                        const int64_t j = j0 + r * R + pr;
+                        if (j < Ny) {
+                            const float L = Lv[yy][r];
+                            const int32_t age = year - bi[r];
+                            float x;
+                            const double wv = (double)weight_la(L, (float)age, phi[r], s, x);
+                            if (BWD) {
+                                const double aw = Ay * wv;
+                                g[r] += aw;
+                                gL += aw * (double)L;
+                                ga += aw * (double)age;
+                            } else {
+                                zacc += wv;
+                            }
+                        }
+                    }
+                    if (!BWD) {
+                        const double ws = wave_sum_b<B>(zacc);
+                        if (lane < B) s_red[yl][wave * B + lane] = ws;
+                    }
+                }
+            }
+            if (!BWD) {
+                __syncthreads();
+                // the tile's total per (year, point), waves in order, added
+                // to the block's running totals
+                for (int k = threadIdx.x; k < ny * B; k += kBT) {
+                    const int yl = k / B, bb = k % B;
+                    double t = 0.0;
+#pragma unroll
+                    for (int w = 0; w < kWv; ++w) t += s_red[yl][w * B + bb];
+                    s_tot[(yc + yl) * B + bb] += t;
+                }
+                __syncthreads();
+            }
+        }
+        if (BWD) {
+#pragma unroll
+            for (int r = 0; r < kLikBPPT; ++r) {
+                const int64_t j = j0 + r * R + pr;
+                if (j < q.n_papers)
+                    grad[j * B + b] = slot_share_b<B>(j, b, sg.seg_off, sg.evt, sg.carry) - g[r];
+            }
+        }
+    }
+    if (BWD) {
+        __shared__ double red[kWv * B];
+        const double tL = block_sum_b<B>(gL, red);
+        const double tA = block_sum_b<B>(ga, red);
+        if (threadIdx.x < B) {
+            part[(0 * nblk + blockIdx.x) * B + threadIdx.x] = tL;
+            part[(1 * nblk + blockIdx.x) * B + threadIdx.x] = tA;
+        }
+    } else {
+        __syncthreads();
+        for (int k = threadIdx.x; k < q.n_years * B; k += kBT)
+            part[((int64_t)(k / B) * nblk + blockIdx.x) * B + (k % B)] = s_tot[k];
+    }
+}
+
+// Z_y at every point: sum over the paper blocks, fixed order.
+template <int B>
+__global__ __launch_bounds__(kBT) void k_likb_year_z(int64_t nblk, const double* __restrict__ part,
+                                                      double* __restrict__ year_sums) {
+    __shared__ double red[kWv * B];
+    const int y = blockIdx.x;
+    const int b = threadIdx.x % B;
+    double v = 0.0;
+    for (int64_t k = threadIdx.x / B; k < nblk; k += kBT / B)
+        v += part[((int64_t)y * nblk + k) * B + b];
+    const double t = block_sum_b<B>(v, red);
+    if (threadIdx.x < B) year_sums[(y * kLikYearCols + 0) * B + threadIdx.x] = t;
+}
+
+// Slot pass: G*B lanes per citing paper (lane gl = lg*B + b), lane (lg, b)
+// owning slots [lg*K, lg*K + K) of its paper at point b.  Block blk takes
+// kLikBCitePapers consecutive citing papers of year ev_block[2*blk] from
+// ev_block[2*blk+1], kBT/(G*B) of them per round.  As k_lik_cite, for every
+// point: D = Z_y - P per PA slot (P chained over the lanes in slot order),
+// log L, 1/D, `later`, and each slot's share [PA slot] + w_t * later stored
+// at share[pos*B + b].  Partials per block, column c at
+// part[(c*nblk + blk)*B + b]: 0 log L, 1 A, 2 alpha terms, 3 tau terms
+// (rounds in order, then the waves in order).
+template <int G, int K, int B>
+__global__ __launch_bounds__(kBT) void k_likb_cite(LikSeq q, const double* __restrict__ at,
+                                                   const float* __restrict__ logfit,
+                                                   const int32_t* __restrict__ ev_block,
+                                                   const double* __restrict__ year_sums,
+                                                   double* __restrict__ share,
+                                                   double* __restrict__ part) {
+    static_assert(G * B <= 64, "a paper's lanes must lie in one wavefront");
+    constexpr int PPR = kBT / (G * B);            // citing papers per round
+    constexpr int ROUNDS = kLikBCitePapers / PPR;
+    static_assert(ROUNDS * PPR == kLikBCitePapers, "rounds must tile the block's papers");
+    __shared__ double red[4][kWv * B];
+    const int gl = threadIdx.x % (G * B);
+    const int b = gl % B, lg = gl / B;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const PScal s = point_scal(at, b);
+    const int64_t nblk = gridDim.x;
+    const int y = ev_block[2 * blockIdx.x];
+    const int64_t first = ev_block[2 * blockIdx.x + 1];
+    const int64_t Dy = q.n_new[y];
+    const double Z = year_sums[(y * kLikYearCols + 0) * B + b];
+    const int64_t NP = q.n_cite;
+    double ll = 0.0, Aacc = 0.0, gaacc = 0.0, gtacc = 0.0;
+    for (int rd = 0; rd < ROUNDS; ++rd) {
+        const int64_t i = first + rd * PPR + threadIdx.x / (G * B);
+        if (first + rd * PPR >= Dy) break;        // block-uniform: no paper left
+        const int64_t P = q.new_off[y] + i;
+        const int c = i < Dy ? q.cnt[P] : 0;
+        const int j0 = lg * K;
+        float wr[K], Lr[K], ar[K];
+        int32_t pos[K];
+        uint32_t evm = 0, vm = 0;
+        double ga1 = 0.0, gt1 = 0.0;
+        if (c > 0) {
+#pragma unroll
+            for (int e = 0; e < K; ++e) {
+                const int j = j0 + e;
+                const bool v = j < c;
+                const int64_t qi = (int64_t)(v ? j : 0) * NP + P;
+                pos[e] = q.pos_t[qi];
+                const int32_t t = q.cit_t[qi];
+                const bool ev = v && q.kind_t[qi] != 1;
+                const float2 la = q.la_t[qi];
+                float x;
+                const float w = weight_la(la.x, la.y, logfit[(int64_t)t * B + b], s, x);
+                wr[e] = v ? w : 0.0f;
+                Lr[e] = la.x;
+                ar[e] = la.y;
+                evm |= uint32_t(ev) << e;
+                vm |= uint32_t(v) << e;
+                ll += ev ? (double)x : 0.0;
+                ga1 += ev ? (double)la.x : 0.0;
+                gt1 += ev ? (double)la.y : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < K; ++e) {
+                wr[e] = 0.0f;
+                Lr[e] = 0.0f;
+                ar[e] = 0.0f;
+                pos[e] = 0;
+            }
+        }
+        // prefix of the weights in slot order, chained across the paper's
+        // lanes of the same point (lane stride B)
+        double Pw[K];
+        double start = 0.0;
+#pragma unroll
+        for (int r = 0; r < G; ++r) {
+            double p = start;
+#pragma unroll
+            for (int e = 0; e < K; ++e) {
+                if (lg == r) Pw[e] = p;
+                p += (double)wr[e];
+            }
+            if (G > 1) {
+                const double nxt = __shfl(p, lane - (lg - r) * B, 64);
+                if (lg == r + 1) start = nxt;
+            }
+        }
+        double inv[K];
+        double tot = 0.0;
+#pragma unroll
+        for (int e = 0; e < K; ++e) {
+            const bool ev = (evm >> e) & 1;
+            const double Dv = Z - Pw[e];
+            ll -= ev ? det_log(Dv) : 0.0;
+            inv[e] = ev ? 1.0 / Dv : 0.0;
+            tot += inv[e];
+        }
+        Aacc += tot;
+        // sum of tot over the paper's later lanes of the same point
+        double suf = tot;
+#pragma unroll
+        for (int d = 1; d < G; d <<= 1) {
+            const double u = __shfl_down(suf, d * B, G * B);
+            if (lg + d < G) suf += u;
+        }
+        double later = suf - tot;
+        double sL = 0.0, sa = 0.0;
+#pragma unroll
+        for (int e = K - 1; e >= 0; --e) {
+            const double wl = (double)wr[e] * later;
+            sL += wl * (double)Lr[e];
+            sa += wl * (double)ar[e];
+            if ((vm >> e) & 1) share[(int64_t)pos[e] * B + b] = (double)((evm >> e) & 1) + wl;
+            later += inv[e];
+        }
+        gaacc += ga1 + sL;
+        gtacc += (gt1 + sa) * s.itau2;
+    }
+    // one reduction of the four columns: waves, then the waves in order
+    const double v0 = wave_sum_b<B>(ll), v1 = wave_sum_b<B>(Aacc);
+    const double v2 = wave_sum_b<B>(gaacc), v3 = wave_sum_b<B>(gtacc);
+    if (lane < B) {
+        red[0][wave * B + lane] = v0;
+        red[1][wave * B + lane] = v1;
+        red[2][wave * B + lane] = v2;
+        red[3][wave * B + lane] = v3;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 * B) {
+        const int c = threadIdx.x / B, bb = threadIdx.x % B;
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWv; ++w) t += red[c][w * B + bb];
+        part[((int64_t)c * nblk + blockIdx.x) * B + bb] = t;
+    }
+}
+
+// Per year: the slot blocks [year_blk[y], year_blk[y+1]) summed in order for
+// the four columns -> year_sums: log L [2], A [1], alpha terms [3], tau [4].
+template <int B>
+__global__ __launch_bounds__(kBT) void k_likb_year_ev(int64_t nblk, const int32_t* __restrict__ year_blk,
+                                                       const double* __restrict__ part,
+                                                       double* __restrict__ year_sums) {
+    __shared__ double red[kWv * B];
+    const int y = blockIdx.x;
+    const int b = threadIdx.x % B;
+    const int64_t b0 = year_blk[y], b1 = year_blk[y + 1];
+    for (int c = 0; c < 4; ++c) {
+        double v = 0.0;
+        for (int64_t k = b0 + threadIdx.x / B; k < b1; k += kBT / B)
+            v += part[((int64_t)c * nblk + k) * B + b];
+        const double t = block_sum_b<B>(v, red);
+        const int dst = c == 0 ? 2 : (c == 1 ? 1 : c + 1);
+        if (threadIdx.x < B) year_sums[(y * kLikYearCols + dst) * B + threadIdx.x] = t;
+    }
+}
+
+// Per-target sums of the shares over sorted positions [0, S), in chunks of
+// seg_chunk<B>() positions: lane group pr (B lanes, one per point) takes
+// kLikBSegPer consecutive positions.  Runs inside a chunk are final
+// (evt[key*B + b]); a run continuing from the previous chunk gives
+// carry[(2*chunk)*B + b], one continuing into the next chunk
+// carry[(2*chunk + 1)*B + b].  The segmented scan over the groups runs as
+// k_lik_segsum's over its threads (shuffles at stride B, then the waves in
+// order): fixed order.
+template <int B>
+__global__ __launch_bounds__(kBT) void k_likb_segsum(int64_t S, const uint32_t* __restrict__ skey,
+                                                      const double* __restrict__ share,
+                                                      double* __restrict__ evt,
+                                                      double* __restrict__ carry) {
+    constexpr int C = seg_chunk<B>();
+    constexpr int NG = kBT / B;                    // lane groups per block
+    constexpr int GPW = 64 / B;                    // lane groups per wave
+    __shared__ double s_wt[kWv][B];
+    __shared__ int s_wf[kWv];
+    const int b = threadIdx.x % B, pr = threadIdx.x / B;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int gw = lane / B;                       // group within the wave
+    const int64_t c0 = (int64_t)blockIdx.x * C;
+    const int n = (int)(S - c0 < C ? S - c0 : C);
+    const int t0 = pr * kLikBSegPer;
+    uint32_t key[kLikBSegPer];
+    double val[kLikBSegPer];
+#pragma unroll
+    for (int e = 0; e < kLikBSegPer; ++e) {
+        const int k = t0 + e;
+        key[e] = k < n ? skey[c0 + k] : 0xFFFFFFFFu;
+        val[e] = k < n ? share[(c0 + k) * B + b] : 0.0;
+    }
+    // (a group past the chunk's end holds no position: no neighbour reads)
+    const uint32_t kprev = t0 > 0 && t0 <= n ? skey[c0 + t0 - 1] : 0xFFFFFFFFu;
+    const uint32_t knext = t0 + kLikBSegPer < n ? skey[c0 + t0 + kLikBSegPer] : 0xFFFFFFFFu;
+    int head_any = t0 == 0 ? 1 : 0;
+    double tail = 0.0;
+#pragma unroll
+    for (int e = 0; e < kLikBSegPer; ++e) {
+        const uint32_t kp = e ? key[e - 1] : kprev;
+        const bool head = (e == 0 && t0 == 0) || key[e] != kp;
+        if (head) {
+            head_any = 1;
+            tail = 0.0;
+        }
+        tail += val[e];
+    }
+    // inclusive segmented scan of (head_any, tail) over the groups of the
+    // wave (stride B), then over the waves in order
+    double v = tail;
+    int f = head_any;
+#pragma unroll
+    for (int d = 1; d < GPW; d <<= 1) {
+        const double pv = __shfl_up(v, d * B, 64);
+        const int pf = __shfl_up(f, d * B, 64);
+        if (gw >= d && !f) {
+            v = pv + v;
+            f = pf;
+        }
+    }
+    if (gw == GPW - 1) {
+        s_wt[wave][b] = v;
+        if (b == 0) s_wf[wave] = f;
+    }
+    __syncthreads();
+    double cw = 0.0;
+    for (int u = 0; u < wave; ++u) cw = s_wf[u] ? s_wt[u][b] : cw + s_wt[u][b];
+    const double incl = f ? v : cw + v;
+    const double up = __shfl_up(incl, B, 64);
+    double run = gw ? up : cw;
+    (void)NG;
+    const uint32_t kfirst = skey[c0];
+    const bool cont_in = c0 > 0 && skey[c0 - 1] == kfirst;
+    const uint32_t klast = skey[c0 + n - 1];
+    const int64_t c1 = c0 + n;
+    const bool cont_out = c1 < S && skey[c1] == klast;
+    const int64_t i0 = c0 + t0;
+#pragma unroll
+    for (int e = 0; e < kLikBSegPer; ++e) {
+        const int64_t i = i0 + e;
+        if (i >= c1) break;
+        const uint32_t kp = e ? key[e - 1] : kprev;
+        if ((e == 0 && t0 == 0) || key[e] != kp) run = 0.0;
+        run += val[e];
+        const uint32_t kn = e + 1 < kLikBSegPer ? key[e + 1] : knext;
+        if (i + 1 == c1 || kn != key[e]) {
+            const bool starts = !(cont_in && key[e] == kfirst);
+            const bool ends = !(cont_out && key[e] == klast);
+            if (starts && ends) evt[(int64_t)key[e] * B + b] = run;
+            if (!starts) carry[(2 * (int64_t)blockIdx.x) * B + b] = run;
+            if (!ends) carry[(2 * (int64_t)blockIdx.x + 1) * B + b] = run;
+        }
+    }
+}
+
+// One block: per point, log L, dL/dalpha, dL/dtau into out[3*b + c].
+template <int B>
+__global__ __launch_bounds__(kBT) void k_likb_final(int n_years, int64_t nblk_p,
+                                                     const double* __restrict__ part_p,
+                                                     const double* __restrict__ at,
+                                                     const double* __restrict__ year_sums,
+                                                     double* __restrict__ out,
+                                                     double* __restrict__ out2) {
+    __shared__ double red[kWv * B];
+    const int b = threadIdx.x % B;
+    double v = 0.0, u = 0.0;
+    for (int64_t k = threadIdx.x / B; k < nblk_p; k += kBT / B) {
+        v += part_p[(0 * nblk_p + k) * B + b];
+        u += part_p[(1 * nblk_p + k) * B + b];
+    }
+    const double sL = block_sum_b<B>(v, red);
+    const double sA = block_sum_b<B>(u, red);
+    if (threadIdx.x < B) {
+        const PScal s = point_scal(at, b);
+        double ll = 0.0, ga = 0.0, gt = 0.0;
+        for (int y = 0; y < n_years; ++y) {
+            ll += year_sums[(y * kLikYearCols + 2) * B + b];
+            ga += year_sums[(y * kLikYearCols + 3) * B + b];
+            gt += year_sums[(y * kLikYearCols + 4) * B + b];
+        }
+        const double r[3] = {ll, ga - sL, gt - sA * s.itau2};
+        for (int c = 0; c < 3; ++c) {
+            out[3 * b + c] = r[c];
+            if (out2) out2[3 * b + c] = r[c];
+        }
+    }
+}
+
+template <int B>
+void run_batch(hipStream_t st, const LikSeq& q, const LikBatchBufs& bb, LikCiteShape sh) {
+    const int64_t nP = bb.nblk_p, nE = bb.nblk_e;
+    double* part_p = bb.part;
+    double* part_e = bb.part + (int64_t)q.n_years * nP * B;
+    double* part_b = part_e + 4 * nE * B;
+    if (nP > 0)
+        k_likb_papers<false, B><<<(unsigned)nP, kBT, (size_t)q.n_years * B * sizeof(double), st>>>(
+            q, bb.at, bb.logfit, bb.year_sums, LikSegs{}, nullptr, part_p);
+    if (q.n_years > 0) k_likb_year_z<B><<<q.n_years, kBT, 0, st>>>(nP, part_p, bb.year_sums);
+    if (nE > 0) {
+#define PG_CITE(GG, KK)                                                                   \
+    k_likb_cite<GG, KK, B><<<(unsigned)nE, kBT, 0, st>>>(q, bb.at, bb.logfit, bb.ev_block, \
+                                                         bb.year_sums, bb.share, part_e)
+#define PG_CITE_K(GG)                                 \
+    switch (sh.K) {                                   \
+        case 4: PG_CITE(GG, 4); break;                \
+        case 5: PG_CITE(GG, 5); break;                \
+        case 6: PG_CITE(GG, 6); break;                \
+        case 7: PG_CITE(GG, 7); break;                \
+        default: PG_CITE(GG, 8); break;               \
+    }
+        switch (sh.G) {
+            case 1: PG_CITE_K(1); break;
+            case 2: PG_CITE_K(2); break;
+            case 4: if constexpr (4 * B <= 64) { PG_CITE_K(4); } break;
+            default: if constexpr (8 * B <= 64) { PG_CITE_K(8); } break;
+        }
+#undef PG_CITE_K
+#undef PG_CITE
+    }
+    if (q.n_years > 0)
+        k_likb_year_ev<B><<<q.n_years, kBT, 0, st>>>(nE, bb.year_blk, part_e, bb.year_sums);
+    if (bb.S > 0)
+        k_likb_segsum<B><<<(unsigned)((bb.S + seg_chunk<B>() - 1) / seg_chunk<B>()), kBT, 0, st>>>(
+            bb.S, bb.skey, bb.share, bb.evt, bb.carry);
+    if (nP > 0)
+        k_likb_papers<true, B><<<(unsigned)nP, kBT, 0, st>>>(
+            q, bb.at, bb.logfit, bb.year_sums, LikSegs{bb.seg_off, bb.evt, bb.carry}, bb.grad,
+            part_b);
+    k_likb_final<B><<<1, kBT, 0, st>>>(q.n_years, nP, part_b, bb.at, bb.year_sums, bb.out,
+                                       bb.out2);
+}
+
+}  // namespace
+
+bool lik_batch_ok(int B) { return B == 4 || B == 8 || B == 16; }
+
+int64_t lik_batch_paper_blocks(int64_t n_papers, int B) {
+    const int64_t ppb = (int64_t)kLikBPPT * (kBT / B);
+    const int64_t tiles = (n_papers + ppb - 1) / ppb;
+    return tiles < kLikBPaperBlocks ? tiles : kLikBPaperBlocks;
+}
+
+int64_t lik_batch_seg_chunks(int64_t S, int B) {
+    const int64_t c = (int64_t)kLikBSegPer * (kBT / B);
+    return (S + c - 1) / c;
+}
+
+LikCiteShape lik_batch_cite_shape(int m, int B) {
+    LikCiteShape sh = lik_cite_shape(m, 0);
+    while (sh.G * B > 64 && sh.K * 2 <= 8) {  // keep a paper's lanes in one wavefront
+        sh.G >>= 1;
+        sh.K = (m + sh.G - 1) / sh.G;
+        if (sh.K < 4) sh.K = 4;
+    }
+    return sh;
+}
+
+bool lik_batch_shape_ok(int m, int B) {
+    const LikCiteShape sh = lik_batch_cite_shape(m, B);
+    return lik_batch_ok(B) && sh.G * B <= 64 && sh.K <= 8 && sh.G * sh.K >= m;
+}
+
+void launch_lik_batch(hipStream_t st, int B, const LikSeq& q, const LikBatchBufs& bb,
+                      LikCiteShape sh) {
+    switch (B) {
+        case 4: run_batch<4>(st, q, bb, sh); break;
+        case 8: run_batch<8>(st, q, bb, sh); break;
+        default: run_batch<16>(st, q, bb, sh); break;
+    }
+}
+
+}  // namespace pg

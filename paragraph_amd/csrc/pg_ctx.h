@@ -64,6 +64,21 @@ struct pg_seq_store {
     double* evt = nullptr;           // [cap_papers]
     double* carry = nullptr;         // [2 * chunks]
     int64_t carry_cap = 0;
+    // batched evaluation at bB points (pg_pa_loglik_grad_batch); built for
+    // the sequence of size b_key = {years, citing papers, filled slots, papers}
+    int32_t bB = 0;
+    int64_t b_key[4] = {-1, -1, -1, -1};
+    pg::LikCiteShape b_shape{1, 4};
+    int64_t b_nE = 0, b_nP = 0;
+    int32_t* b_ev_block = nullptr;   // slot-pass block table of the batched shape
+    int32_t* b_year_blk = nullptr;
+    double* b_part = nullptr;
+    double* b_year_sums = nullptr;
+    double* b_share = nullptr;       // [filled * bB]
+    double* b_evt = nullptr;         // [n_papers * bB]
+    double* b_carry = nullptr;
+    double* b_at = nullptr;          // [2 * bB] alpha/tau copied from the host
+    double* b_out = nullptr;         // [3 * bB]
 };
 
 struct pg_ctx {

@@ -374,4 +374,39 @@ void launch_lik_final(hipStream_t s, int n_years, bool grad, int64_t nblk_p,
                       const double* part_p, const LikArgs& a, const double* year_sums,
                       double* out, double* out2);
 
+// ---- PA likelihood at B parameter points per pass (k_likb.hip) ----
+constexpr int kLikBPPT = 8;      // papers per thread in the batched paper passes
+constexpr int kLikBPaperBlocks = 2048;  // batched paper passes: blocks (each takes many tiles)
+constexpr int kLikBCitePapers = 64;     // batched slot pass: citing papers per block
+constexpr int kLikBSegPer = 16;  // sorted slots per lane group in the batched segment sums
+
+// Buffers of one batched evaluation; per-point arrays interleave the B
+// points (point b of item i at i*B + b).
+struct LikBatchBufs {
+    const double* at;          // {alpha_b, tau_b} [2B]
+    const float* logfit;       // phi [n_papers * B]
+    double* year_sums;         // [n_years * kLikYearCols * B]
+    double* part;              // (n_years + 2) * nblk_p * B + 4 * nblk_e * B
+    const int32_t* ev_block;   // slot-pass block table for the batched shape
+    const int32_t* year_blk;
+    int64_t nblk_p, nblk_e;
+    double* share;             // [S * B] per-slot shares, by-target order
+    int64_t S;                 // filled slots
+    const uint32_t* skey;
+    const int32_t* seg_off;
+    double* evt;               // [n_papers * B]
+    double* carry;             // [2 * chunks * B]
+    double* grad;              // dL/dphi [n_papers * B]
+    double* out;               // {log L, dL/dalpha, dL/dtau} of point b at 3b [3B]
+    double* out2;              // optional copy of out
+};
+bool lik_batch_ok(int B);                                 // B in {4, 8, 16}
+bool lik_batch_shape_ok(int m, int B);                    // m slots fit the batched slot pass
+LikCiteShape lik_batch_cite_shape(int m, int B);         // G lanes per (paper, point), K slots
+int64_t lik_batch_paper_blocks(int64_t n_papers, int B);
+int64_t lik_batch_seg_chunks(int64_t S, int B);
+// The whole evaluation (both paper passes, slot pass, segment sums, final).
+void launch_lik_batch(hipStream_t s, int B, const LikSeq& q, const LikBatchBufs& bb,
+                      LikCiteShape sh);
+
 }  // namespace pg

@@ -26,7 +26,18 @@ void free_ptr(void* p) {
     if (p) (void)hipFree(p);
 }
 
+void seq_free_batch(pg_seq_store& q) {
+    free_ptr(q.b_ev_block); free_ptr(q.b_year_blk); free_ptr(q.b_part); free_ptr(q.b_year_sums);
+    free_ptr(q.b_share); free_ptr(q.b_evt); free_ptr(q.b_carry); free_ptr(q.b_at);
+    free_ptr(q.b_out);
+    q.b_ev_block = q.b_year_blk = nullptr;
+    q.b_part = q.b_year_sums = q.b_share = q.b_evt = q.b_carry = q.b_at = q.b_out = nullptr;
+    q.bB = 0;
+    for (int k = 0; k < 4; ++k) q.b_key[k] = -1;
+}
+
 void seq_free_tables(pg_seq_store& q) {
+    seq_free_batch(q);
     free_ptr(q.d_n_exist); free_ptr(q.d_n_new); free_ptr(q.d_deg_off); free_ptr(q.d_new_off);
     free_ptr(q.d_year_of); free_ptr(q.d_ev_block); free_ptr(q.d_ev_year_blk);
     free_ptr(q.part); free_ptr(q.year_sums); free_ptr(q.d_scalars);
@@ -221,6 +232,63 @@ double lik_bytes(const pg_seq_store& q, bool grad) {
                  S = (double)q.filled;
     return grad ? 8.0 * PY + 44.0 * P + 4.0 * NP + 33.0 * S
                 : 4.0 * PY + 8.0 * P + 4.0 * NP + 17.0 * S;
+}
+
+// Buffers and the slot-pass block table of the batched evaluation at B
+// points for the current sequence (seq_prepare done).
+int batch_prepare(pg_ctx* ctx, int B) {
+    pg_seq_store& q = ctx->seq;
+    const int64_t Y = (int64_t)q.n_exist.size();
+    const int64_t key[4] = {Y, q.new_total, q.filled, q.n_papers};
+    if (q.bB == B && std::equal(key, key + 4, q.b_key)) return PG_OK;
+    seq_free_batch(q);
+    const LikCiteShape sh = lik_batch_cite_shape(q.m, B);
+    const int64_t per_block = kLikBCitePapers;
+    std::vector<int32_t> ev_block, year_blk(Y + 1, 0);
+    for (int64_t y = 0; y < Y; ++y) {
+        year_blk[y] = (int32_t)(ev_block.size() / 2);
+        for (int64_t f = 0; f < q.n_new[y]; f += per_block) {
+            ev_block.push_back((int32_t)y);
+            ev_block.push_back((int32_t)f);
+        }
+    }
+    year_blk[Y] = (int32_t)(ev_block.size() / 2);
+    const int64_t nE = (int64_t)ev_block.size() / 2;
+    const int64_t nP = lik_batch_paper_blocks(q.n_papers, B);
+    const int64_t part = ((Y + 2) * nP + 4 * nE) * B + 1;
+    hipError_t r = dalloc(&q.b_ev_block, std::max<int64_t>(2 * nE, 1));
+    if (r == hipSuccess) r = dalloc(&q.b_year_blk, Y + 1);
+    if (r == hipSuccess) r = dalloc(&q.b_part, part);
+    if (r == hipSuccess) r = dalloc(&q.b_year_sums, std::max<int64_t>(Y, 1) * kLikYearCols * B);
+    if (r == hipSuccess) r = dalloc(&q.b_share, std::max<int64_t>(q.filled, 1) * B);
+    if (r == hipSuccess) r = dalloc(&q.b_evt, std::max<int64_t>(q.n_papers, 1) * B);
+    if (r == hipSuccess) r = dalloc(&q.b_carry, 2 * std::max<int64_t>(lik_batch_seg_chunks(q.filled, B), 1) * B);
+    if (r == hipSuccess) r = dalloc(&q.b_at, 2 * B);
+    if (r == hipSuccess) r = dalloc(&q.b_out, 3 * B);
+    if (r == hipSuccess && nE > 0)
+        r = hipMemcpy(q.b_ev_block, ev_block.data(), nE * 8, hipMemcpyHostToDevice);
+    if (r == hipSuccess)
+        r = hipMemcpy(q.b_year_blk, year_blk.data(), (Y + 1) * 4, hipMemcpyHostToDevice);
+    if (r != hipSuccess) {
+        seq_free_batch(q);
+        return fail(ctx, r == hipErrorOutOfMemory ? PG_ENOMEM : PG_EHIP,
+                    std::string("pg_pa_loglik_grad_batch: buffers for ") + std::to_string(B) +
+                        " points: " + hipGetErrorString(r));
+    }
+    q.bB = B;
+    std::copy(key, key + 4, q.b_key);
+    q.b_shape = sh;
+    q.b_nE = nE;
+    q.b_nP = nP;
+    return PG_OK;
+}
+
+// Algorithmic HBM bytes of one batched evaluation at B points (the formula
+// in paragraph.h).
+double lik_batch_bytes(const pg_seq_store& q, int B) {
+    const double PY = (double)q.paper_years, P = (double)q.n_papers, NP = (double)q.new_total,
+                 S = (double)q.filled;
+    return 8.0 * PY + 12.0 * P + 4.0 * NP + 21.0 * S + (double)B * (32.0 * P + 20.0 * S);
 }
 
 int lik_eval(pg_ctx* ctx, const pg_lik_params* p, pg_lik_out* out, bool grad) {
@@ -469,6 +537,81 @@ int pg_seq_download(pg_ctx* ctx, int64_t* n_exist, int64_t* n_new, int32_t* year
     if (cnt && q.new_total) PG_HIP(hipMemcpy(cnt, q.cnt, q.new_total * 4, hipMemcpyDeviceToHost));
     if (birth && q.n_papers)
         PG_HIP(hipMemcpy(birth, q.birth, q.n_papers * 4, hipMemcpyDeviceToHost));
+    return PG_OK;
+}
+
+int pg_pa_loglik_grad_batch(pg_ctx* ctx, const pg_lik_batch_params* p, pg_lik_batch_out* out) {
+    const char* fn = "pg_pa_loglik_grad_batch";
+    if (!ctx) return PG_EINVAL;
+    if (!p || p->size != sizeof(pg_lik_batch_params) || !out ||
+        out->size != sizeof(pg_lik_batch_out))
+        return fail(ctx, PG_EINVAL, std::string(fn) + ": null argument or struct size mismatch");
+    pg_seq_store& q = ctx->seq;
+    if (!q.reserved) return fail(ctx, PG_ESTATE, std::string(fn) + ": no sequence");
+    const int B = p->n_points;
+    if (!lik_batch_ok(B))
+        return fail(ctx, PG_EINVAL, std::string(fn) + ": n_points must be 4, 8 or 16");
+    if (!lik_batch_shape_ok(q.m, B))
+        return fail(ctx, PG_EINVAL, std::string(fn) + ": " + std::to_string(q.m) +
+                                        " slots per paper do not fit the slot pass at " +
+                                        std::to_string(B) + " points (use fewer points)");
+    if (!p->alpha_tau_dev) {
+        if (!p->alpha_tau) return fail(ctx, PG_EINVAL, std::string(fn) + ": no alpha/tau given");
+        for (int b = 0; b < B; ++b)
+            if (!(std::isfinite(p->alpha_tau[2 * b]) && p->alpha_tau[2 * b + 1] > 0.0 &&
+                  std::isfinite(p->alpha_tau[2 * b + 1])))
+                return fail(ctx, PG_EINVAL, std::string(fn) + ": alpha must be finite and tau > 0");
+    }
+    if (q.n_papers > 0 && (!p->logfit_dev || !out->grad_logfit_dev))
+        return fail(ctx, PG_EINVAL, std::string(fn) + ": logfit_dev or grad_logfit_dev is NULL");
+    for (size_t y = 0; y < q.n_exist.size(); ++y)
+        if (q.n_exist[y] + q.n_new[y] > q.n_papers)
+            return fail(ctx, PG_ESTATE, std::string(fn) + ": births not set for every paper");
+    PG_HIP(hipSetDevice(ctx->cfg.device));
+    if (int rc = seq_prepare(ctx)) return rc;
+    if (int rc = batch_prepare(ctx, B)) return rc;
+    hipStream_t s = p->stream_given ? static_cast<hipStream_t>(p->stream) : ctx->stream;
+    const double* at = p->alpha_tau_dev;
+    if (!at) {
+        PG_HIP(hipMemcpyAsync(q.b_at, p->alpha_tau, 2 * B * sizeof(double), hipMemcpyHostToDevice,
+                              s));
+        at = q.b_at;
+    }
+    const bool prof = ctx->prof && ((ctx->prof_mask >> PG_K_LIK_BATCH) & 1u);
+    if (prof) PG_HIP(hipEventRecord(ctx->pev[PG_K_LIK_BATCH][0], s));
+    LikBatchBufs bb;
+    bb.at = at;
+    bb.logfit = p->logfit_dev;
+    bb.year_sums = q.b_year_sums;
+    bb.part = q.b_part;
+    bb.ev_block = q.b_ev_block;
+    bb.year_blk = q.b_year_blk;
+    bb.nblk_p = q.b_nP;
+    bb.nblk_e = q.b_nE;
+    bb.share = q.b_share;
+    bb.S = q.filled;
+    bb.skey = q.skey;
+    bb.seg_off = q.seg_off;
+    bb.evt = q.b_evt;
+    bb.carry = q.b_carry;
+    bb.grad = out->grad_logfit_dev;
+    bb.out = q.b_out;
+    bb.out2 = out->scalars_dev;
+    launch_lik_batch(s, B, seq_view(q), bb, q.b_shape);
+    PG_HIP(hipGetLastError());
+    if (prof) {
+        PG_HIP(hipEventRecord(ctx->pev[PG_K_LIK_BATCH][1], s));
+        PG_HIP(hipEventSynchronize(ctx->pev[PG_K_LIK_BATCH][1]));
+        float ms = 0.0f;
+        PG_HIP(hipEventElapsedTime(&ms, ctx->pev[PG_K_LIK_BATCH][0], ctx->pev[PG_K_LIK_BATCH][1]));
+        ctx->prof_ms[PG_K_LIK_BATCH] += ms;
+        ctx->prof_bytes[PG_K_LIK_BATCH] += lik_batch_bytes(q, B);
+        ctx->prof_calls[PG_K_LIK_BATCH] += 1;
+    }
+    if (out->host) {
+        PG_HIP(hipMemcpyAsync(out->host, q.b_out, 3 * B * sizeof(double), hipMemcpyDeviceToHost, s));
+        PG_HIP(hipStreamSynchronize(s));
+    }
     return PG_OK;
 }
 

@@ -91,6 +91,62 @@ def test_gpu_matches_oracle(case):
         assert np.linalg.norm(gphi - d[3]) <= 1e-5 * np.linalg.norm(d[3])
 
 
+@pytest.mark.parametrize("B,case", [
+    (16, dict(n0=2000, years=3, rate=0.05, m=20, n_nb=6, alpha=1.2, tau=4.0, seed=5)),
+    (8, dict(n0=400, years=2, rate=0.1, m=40, n_nb=10, alpha=1.0, tau=6.0, seed=3)),
+    (4, dict(n0=300, years=3, rate=0.1, m=5, n_nb=2, alpha=1.2, tau=4.0, seed=7)),
+    (16, dict(n0=500, years=4, rate=0.08, m=3, n_nb=0, alpha=0.7, tau=2.0, seed=11)),
+])
+def test_batched_points_match_oracle(B, case):
+    """B points (alpha_b, tau_b, phi_b) in one call: each point against the
+    same-arithmetic oracle (1e-12 / 1e-9 as the single-point entry), the FP64
+    oracle (1e-5), and the single-point GPU entry; repeated calls give the
+    same bits."""
+    import torch
+    log("batched likelihood", B, case)
+    seq, birth, logfit, m = simulate(**case)
+    ctx = seq_ctx(seq, birth, m)
+    rng = np.random.default_rng(B)
+    at = np.stack([rng.uniform(0.5, 1.5, B), rng.uniform(1.0, 10.0, B)], 1)
+    phis = [(logfit + (rng.standard_normal(logfit.size) * 0.1 if b else 0)).astype(np.float32)
+            for b in range(B)]
+    n = logfit.size
+    inter = np.ascontiguousarray(np.stack(phis, 1).reshape(-1))
+    phi_d = torch.as_tensor(inter, device="cuda")
+    grad_d = torch.empty(n * B, dtype=torch.float64, device="cuda")
+    h = ctx.pa_loglik_grad_batch(at, phi_d.data_ptr(), grad_d.data_ptr())
+    g = grad_d.cpu().numpy().reshape(n, B)
+    h2 = ctx.pa_loglik_grad_batch(at, phi_d.data_ptr(), grad_d.data_ptr())
+    assert np.array_equal(h, h2) and np.array_equal(g, grad_d.cpu().numpy().reshape(n, B))
+    for b in range(B):
+        a, t = float(at[b, 0]), float(at[b, 1])
+        r = oracle_eval(seq, birth, phis[b], m, a, t, w32=True)
+        assert rel(h[b, 0], r[0]) <= 1e-12, (b, h[b, 0], r[0])
+        assert abs(h[b, 1] - r[1]) <= 1e-9 * max(abs(r[1]), 1.0)
+        assert abs(h[b, 2] - r[2]) <= 1e-9 * max(abs(r[2]), 1.0)
+        assert np.all(np.abs(g[:, b] - r[3]) <= 1e-9 * np.maximum(1.0, np.abs(r[3])))
+        d = oracle_eval(seq, birth, phis[b], m, a, t, w32=False)
+        assert rel(h[b, 1], d[1]) <= 1e-5 and rel(h[b, 2], d[2]) <= 1e-5
+        assert np.linalg.norm(g[:, b] - d[3]) <= 1e-5 * np.linalg.norm(d[3])
+        one = gpu_eval(ctx, a, t, phis[b])
+        assert rel(h[b, 0], one[0]) <= 1e-12
+        assert np.all(np.abs(g[:, b] - one[3]) <= 1e-10 * np.maximum(1.0, np.abs(one[3])))
+
+
+def test_batched_points_refused_shapes():
+    import torch
+    seq, birth, logfit, m = simulate(n0=400, years=2, rate=0.1, m=40, n_nb=10, seed=3)
+    ctx = seq_ctx(seq, birth, m)
+    phi = torch.zeros(logfit.size * 16, dtype=torch.float32, device="cuda")
+    grad = torch.empty(logfit.size * 16, dtype=torch.float64, device="cuda")
+    with pytest.raises(pg.PGError, match="EINVAL"):     # 40 slots need 8 lanes: 8*16 > 64
+        ctx.pa_loglik_grad_batch([(1.0, 5.0)] * 16, phi.data_ptr(), grad.data_ptr())
+    with pytest.raises(pg.PGError, match="EINVAL"):     # B must be 4, 8 or 16
+        ctx.pa_loglik_grad_batch([(1.0, 5.0)] * 5, phi.data_ptr(), grad.data_ptr())
+    with pytest.raises(pg.PGError, match="EINVAL"):     # tau <= 0
+        ctx.pa_loglik_grad_batch([(1.0, 5.0)] * 3 + [(1.0, 0.0)], phi.data_ptr(), grad.data_ptr())
+
+
 def test_uniform_closed_form_and_reproducible():
     seq, birth, logfit, m = simulate(n0=300, years=3, rate=0.1, m=5, n_nb=2)
     ctx = seq_ctx(seq, birth, m)
