@@ -323,8 +323,19 @@ __global__ __launch_bounds__(kLikBT) void k_lik_cite(LikSeq q, LikArgs a,
     for (int e = 0; e < K; ++e) {
         const bool ev = (evm >> e) & 1;
         const double Dv = Z - Pw[e];
-        ll -= ev ? det_log(Dv) : 0.0;
-        inv[e] = (GRAD && ev) ? 1.0 / Dv : 0.0;
+        // D is a difference of sums of FP32 values: 0, negative (never for a
+        // drawn pick) or at least 2^-149, a normal double, where det_log_pos
+        // is det_log; the other two cases as det_log gives them.  Computed
+        // for every slot and selected: no branch.
+        const double lgD = Dv > 0.0 ? det_log_pos(Dv)
+                                    : (Dv == 0.0 ? -__builtin_huge_val() : __builtin_nan(""));
+        ll -= ev ? lgD : 0.0;
+        if (GRAD) {
+            const double rD = 1.0 / Dv;
+            inv[e] = ev ? rD : 0.0;
+        } else {
+            inv[e] = 0.0;
+        }
         tot += inv[e];
     }
     double Aacc = 0.0;

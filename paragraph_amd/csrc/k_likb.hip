@@ -76,19 +76,11 @@ __device__ inline double block_sum_b(double v, double* red /* [kWv * B] */) {
 template <int B>
 constexpr int seg_chunk() { return kLikBSegPer * (kBT / B); }
 
-// Slot shares of paper j for point b from the segment sums.
+// Slot shares of paper j for point b: its segment total (0 without slots).
 template <int B>
 __device__ inline double slot_share_b(int64_t j, int b, const int32_t* __restrict__ seg_off,
-                                      const double* __restrict__ evt,
-                                      const double* __restrict__ carry) {
-    const int64_t o0 = seg_off[j], o1 = seg_off[j + 1];
-    if (o0 == o1) return 0.0;
-    constexpr int C = seg_chunk<B>();
-    const int64_t c0 = o0 / C, c1 = (o1 - 1) / C;
-    if (c0 == c1) return evt[j * B + b];
-    double e = carry[(2 * c0 + 1) * B + b];
-    for (int64_t c = c0 + 1; c < c1; ++c) e += carry[2 * c * B + b];
-    return e + carry[2 * c1 * B + b];
+                                      const double* __restrict__ evt) {
+    return seg_off[j] == seg_off[j + 1] ? 0.0 : evt[j * B + b];
 }
 
 // Paper-major passes over tiles of PPB = kLikBPPT*kBT/B papers: block blk
@@ -119,21 +111,20 @@ __global__ __launch_bounds__(kBT) void k_likb_papers(LikSeq q, const double* __r
     if (!BWD)
         for (int k = threadIdx.x; k < q.n_years * B; k += kBT) s_tot[k] = 0.0;
     double gL = 0.0, ga = 0.0;
+    // paper ids stay below 2^31 (32-bit index arithmetic); a paper that does
+    // not exist in a year is computed like the others and masked off
     for (int64_t tile = blockIdx.x; tile < ntile; tile += nblk) {
-        const int64_t j0 = tile * PPB;
+        const int32_t j0 = (int32_t)(tile * PPB);
         float phi[kLikBPPT];
         int32_t bi[kLikBPPT];
         double g[kLikBPPT];
 #pragma unroll
         for (int r = 0; r < kLikBPPT; ++r) {
-            const int64_t j = j0 + r * R + pr;
-            phi[r] = 0.0f;
-            bi[r] = 0;
+            const int32_t j = j0 + r * R + pr;
+            const bool in = j < (int32_t)q.n_papers;
+            phi[r] = in ? logfit[(uint32_t)j * B + b] : 0.0f;
+            bi[r] = in ? q.birth[j] : 0;
             g[r] = 0.0;
-            if (j < q.n_papers) {
-                phi[r] = logfit[j * B + b];
-                bi[r] = q.birth[j];
-            }
         }
         for (int yc = 0; yc < q.n_years; yc += kYChunk) {
             const int ny = q.n_years - yc < kYChunk ? q.n_years - yc : kYChunk;
@@ -145,12 +136,13 @@ __global__ __launch_bounds__(kBT) void k_likb_papers(LikSeq q, const double* __r
 #pragma unroll
                 for (int yy = 0; yy < 4; ++yy) {
                     const int y = yc + y4 + yy;
-                    const int64_t Ny = y4 + yy < ny ? q.n_exist[y] : 0;
-                    const int64_t doff = y4 + yy < ny ? q.deg_off[y] : 0;
+                    const bool yin = y4 + yy < ny;
+                    const int32_t Ny = yin ? (int32_t)q.n_exist[y] : 0;
+                    const float* Lrow = Ny > 0 ? q.Lpy + q.deg_off[y] : q.Lpy;
 #pragma unroll
                     for (int r = 0; r < kLikBPPT; ++r) {
-                        const int64_t j = j0 + r * R + pr;
-                        Lv[yy][r] = j < Ny ? q.Lpy[doff + j] : 0.0f;
+                        const int32_t j = j0 + r * R + pr;
+                        Lv[yy][r] = Lrow[j < Ny ? j : 0];
                     }
                 }
 #pragma unroll
@@ -158,26 +150,25 @@ __global__ __launch_bounds__(kBT) void k_likb_papers(LikSeq q, const double* __r
                     const int yl = y4 + yy;
                     if (yl >= ny) break;
                     const int y = yc + yl;
-                    const int64_t Ny = q.n_exist[y];
+                    const int32_t Ny = (int32_t)q.n_exist[y];
                     const int32_t year = q.year_of[y];
                     const double Ay = BWD ? year_sums[(y * kLikYearCols + 1) * B + b] : 0.0;
                     double zacc = 0.0;
 #pragma unroll
                     for (int r = 0; r < kLikBPPT; ++r) {
-                        const int64_t j = j0 + r * R + pr;
-                        if (j < Ny) {
-                            const float L = Lv[yy][r];
-                            const int32_t age = year - bi[r];
-                            float x;
-                            const double wv = (double)weight_la(L, (float)age, phi[r], s, x);
-                            if (BWD) {
-                                const double aw = Ay * wv;
-                                g[r] += aw;
-                                gL += aw * (double)L;
-                                ga += aw * (double)age;
-                            } else {
-                                zacc += wv;
-                            }
+                        const int32_t j = j0 + r * R + pr;
+                        const bool in = j < Ny;
+                        const float L = Lv[yy][r];
+                        const int32_t age = year - bi[r];
+                        float x;
+                        const double wv = (double)weight_la(L, (float)age, phi[r], s, x);
+                        if (BWD) {
+                            const double aw = in ? Ay * wv : 0.0;
+                            g[r] += aw;
+                            gL += aw * (double)L;
+                            ga += aw * (double)age;
+                        } else {
+                            zacc += in ? wv : 0.0;
                         }
                     }
                     if (!BWD) {
@@ -203,9 +194,9 @@ __global__ __launch_bounds__(kBT) void k_likb_papers(LikSeq q, const double* __r
         if (BWD) {
 #pragma unroll
             for (int r = 0; r < kLikBPPT; ++r) {
-                const int64_t j = j0 + r * R + pr;
-                if (j < q.n_papers)
-                    grad[j * B + b] = slot_share_b<B>(j, b, sg.seg_off, sg.evt, sg.carry) - g[r];
+                const int32_t j = j0 + r * R + pr;
+                if (j < (int32_t)q.n_papers)
+                    grad[(size_t)j * B + b] = slot_share_b<B>(j, b, sg.seg_off, sg.evt) - g[r];
             }
         }
     }
@@ -266,66 +257,65 @@ __global__ __launch_bounds__(kBT) void k_likb_cite(LikSeq q, const double* __res
     const int64_t nblk = gridDim.x;
     const int y = ev_block[2 * blockIdx.x];
     const int64_t first = ev_block[2 * blockIdx.x + 1];
-    const int64_t Dy = q.n_new[y];
+    const int32_t Dy = (int32_t)q.n_new[y];
     const double Z = year_sums[(y * kLikYearCols + 0) * B + b];
-    const int64_t NP = q.n_cite;
+    const uint32_t NP = (uint32_t)q.n_cite;       // slots of a sequence stay below 2^31
+    const uint32_t P0 = (uint32_t)q.new_off[y];
     double ll = 0.0, Aacc = 0.0, gaacc = 0.0, gtacc = 0.0;
     for (int rd = 0; rd < ROUNDS; ++rd) {
-        const int64_t i = first + rd * PPR + threadIdx.x / (G * B);
-        if (first + rd * PPR >= Dy) break;        // block-uniform: no paper left
-        const int64_t P = q.new_off[y] + i;
+        const int32_t i = (int32_t)first + rd * PPR + threadIdx.x / (G * B);
+        if ((int32_t)first + rd * PPR >= Dy) break;   // block-uniform: no paper left
+        // a lane past the year's papers reads the year's first paper, masked
+        const uint32_t P = P0 + (uint32_t)(i < Dy ? i : 0);
         const int c = i < Dy ? q.cnt[P] : 0;
         const int j0 = lg * K;
         float wr[K], Lr[K], ar[K];
-        int32_t pos[K];
+        uint32_t pos[K];
         uint32_t evm = 0, vm = 0;
         double ga1 = 0.0, gt1 = 0.0;
-        if (c > 0) {
+        // every slot is loaded (slot 0 of the paper stands in for the empty
+        // ones, which exist for every paper) and masked: no branches
 #pragma unroll
-            for (int e = 0; e < K; ++e) {
-                const int j = j0 + e;
-                const bool v = j < c;
-                const int64_t qi = (int64_t)(v ? j : 0) * NP + P;
-                pos[e] = q.pos_t[qi];
-                const int32_t t = q.cit_t[qi];
-                const bool ev = v && q.kind_t[qi] != 1;
-                const float2 la = q.la_t[qi];
-                float x;
-                const float w = weight_la(la.x, la.y, logfit[(int64_t)t * B + b], s, x);
-                wr[e] = v ? w : 0.0f;
-                Lr[e] = la.x;
-                ar[e] = la.y;
-                evm |= uint32_t(ev) << e;
-                vm |= uint32_t(v) << e;
-                ll += ev ? (double)x : 0.0;
-                ga1 += ev ? (double)la.x : 0.0;
-                gt1 += ev ? (double)la.y : 0.0;
-            }
-        } else {
-#pragma unroll
-            for (int e = 0; e < K; ++e) {
-                wr[e] = 0.0f;
-                Lr[e] = 0.0f;
-                ar[e] = 0.0f;
-                pos[e] = 0;
-            }
+        for (int e = 0; e < K; ++e) {
+            const int j = j0 + e;
+            const bool v = j < c;
+            const uint32_t qi = (uint32_t)(v ? j : 0) * NP + P;
+            pos[e] = q.pos_t[qi];
+            const int32_t t = q.cit_t[qi];
+            const bool ev = v && q.kind_t[qi] != 1;
+            const float2 la = q.la_t[qi];
+            float x;
+            const float w = weight_la(la.x, la.y, logfit[(uint32_t)t * B + b], s, x);
+            wr[e] = v ? w : 0.0f;
+            Lr[e] = la.x;
+            ar[e] = la.y;
+            evm |= uint32_t(ev) << e;
+            vm |= uint32_t(v) << e;
+            ll += ev ? (double)x : 0.0;
+            ga1 += ev ? (double)la.x : 0.0;
+            gt1 += ev ? (double)la.y : 0.0;
         }
-        // prefix of the weights in slot order, chained across the paper's
-        // lanes of the same point (lane stride B)
+        // prefix of the weights in slot order: the lane's own prefix plus the
+        // sum of the paper's earlier lanes at the same point (an inclusive
+        // scan of the lane totals at lane stride B); a fixed association
         double Pw[K];
-        double start = 0.0;
+        double acc = 0.0;
 #pragma unroll
-        for (int r = 0; r < G; ++r) {
-            double p = start;
+        for (int e = 0; e < K; ++e) {
+            Pw[e] = acc;
+            acc += (double)wr[e];
+        }
+        if (G > 1) {
+            double incl = acc;
 #pragma unroll
-            for (int e = 0; e < K; ++e) {
-                if (lg == r) Pw[e] = p;
-                p += (double)wr[e];
+            for (int d = 1; d < G; d <<= 1) {
+                const double u = __shfl_up(incl, d * B, G * B);
+                if (lg >= d) incl += u;
             }
-            if (G > 1) {
-                const double nxt = __shfl(p, lane - (lg - r) * B, 64);
-                if (lg == r + 1) start = nxt;
-            }
+            const double up = __shfl_up(incl, B, G * B);
+            const double start = lg > 0 ? up : 0.0;
+#pragma unroll
+            for (int e = 0; e < K; ++e) Pw[e] += start;
         }
         double inv[K];
         double tot = 0.0;
@@ -333,8 +323,14 @@ __global__ __launch_bounds__(kBT) void k_likb_cite(LikSeq q, const double* __res
         for (int e = 0; e < K; ++e) {
             const bool ev = (evm >> e) & 1;
             const double Dv = Z - Pw[e];
-            ll -= ev ? det_log(Dv) : 0.0;
-            inv[e] = ev ? 1.0 / Dv : 0.0;
+            // D is a difference of sums of FP32 values: 0, negative (never
+            // for a drawn pick) or at least 2^-149, a normal double, where
+            // det_log_pos is det_log; the other two cases as det_log gives them
+            const double lgD = Dv > 0.0 ? det_log_pos(Dv)
+                                        : (Dv == 0.0 ? -__builtin_huge_val() : __builtin_nan(""));
+            const double rD = 1.0 / Dv;
+            ll -= ev ? lgD : 0.0;
+            inv[e] = ev ? rD : 0.0;
             tot += inv[e];
         }
         Aacc += tot;
@@ -352,7 +348,7 @@ __global__ __launch_bounds__(kBT) void k_likb_cite(LikSeq q, const double* __res
             const double wl = (double)wr[e] * later;
             sL += wl * (double)Lr[e];
             sa += wl * (double)ar[e];
-            if ((vm >> e) & 1) share[(int64_t)pos[e] * B + b] = (double)((evm >> e) & 1) + wl;
+            if ((vm >> e) & 1) share[(size_t)pos[e] * B + b] = (double)((evm >> e) & 1) + wl;
             later += inv[e];
         }
         gaacc += ga1 + sL;
@@ -397,20 +393,25 @@ __global__ __launch_bounds__(kBT) void k_likb_year_ev(int64_t nblk, const int32_
     }
 }
 
-// Per-target sums of the shares over sorted positions [0, S), in chunks of
-// seg_chunk<B>() positions: lane group pr (B lanes, one per point) takes
-// kLikBSegPer consecutive positions.  Runs inside a chunk are final
-// (evt[key*B + b]); a run continuing from the previous chunk gives
-// carry[(2*chunk)*B + b], one continuing into the next chunk
-// carry[(2*chunk + 1)*B + b].  The segmented scan over the groups runs as
-// k_lik_segsum's over its threads (shuffles at stride B, then the waves in
-// order): fixed order.
+// Per-target sums of the shares over the sorted positions, one block per
+// job; jobs are cut at paper boundaries (built once per sequence), so no run
+// crosses a job.
+//   job_tag < 0: the papers of positions [job_start, job_start+1), at most
+//     seg_chunk<B>() positions; lane group pr (B lanes, one per point) takes
+//     kLikBSegPer consecutive positions, and the segmented scan over the
+//     groups (shuffles at stride B, then the waves in order) gives each run's
+//     total: evt[key*B + b].
+//   job_tag >= 0: a piece of at most kLikBBigPiece positions of one paper with
+//     more than seg_chunk<B>() positions: its total goes to
+//     bigpart[tag*B + b] (groups strided over the piece, then the groups in
+//     a fixed order), which k_likb_bigsum adds up per paper.
 template <int B>
-__global__ __launch_bounds__(kBT) void k_likb_segsum(int64_t S, const uint32_t* __restrict__ skey,
+__global__ __launch_bounds__(kBT) void k_likb_segjob(const int64_t* __restrict__ job_start,
+                                                      const int32_t* __restrict__ job_tag,
+                                                      const uint32_t* __restrict__ skey,
                                                       const double* __restrict__ share,
                                                       double* __restrict__ evt,
-                                                      double* __restrict__ carry) {
-    constexpr int C = seg_chunk<B>();
+                                                      double* __restrict__ bigpart) {
     constexpr int NG = kBT / B;                    // lane groups per block
     constexpr int GPW = 64 / B;                    // lane groups per wave
     __shared__ double s_wt[kWv][B];
@@ -418,8 +419,33 @@ __global__ __launch_bounds__(kBT) void k_likb_segsum(int64_t S, const uint32_t* 
     const int b = threadIdx.x % B, pr = threadIdx.x / B;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int gw = lane / B;                       // group within the wave
-    const int64_t c0 = (int64_t)blockIdx.x * C;
-    const int n = (int)(S - c0 < C ? S - c0 : C);
+    const int64_t c0 = job_start[blockIdx.x];
+    const int n = (int)(job_start[blockIdx.x + 1] - c0);
+    const int tag = job_tag[blockIdx.x];
+    if (tag >= 0) {                                // block-uniform
+        // positions pr, pr + NG, ... added in order, eight loads in flight
+        double v = 0.0;
+        for (int k0 = 0; k0 < n; k0 += 8 * NG) {
+            double x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = k0 + u * NG + pr;
+                x[u] = k < n ? share[(c0 + k) * B + b] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v += x[u];
+        }
+        v = wave_sum_b<B>(v);
+        if (lane < B) s_wt[wave][lane] = v;
+        __syncthreads();
+        if (threadIdx.x < B) {
+            double t = 0.0;
+#pragma unroll
+            for (int w = 0; w < kWv; ++w) t += s_wt[w][threadIdx.x];
+            bigpart[(int64_t)tag * B + threadIdx.x] = t;
+        }
+        return;
+    }
     const int t0 = pr * kLikBSegPer;
     uint32_t key[kLikBSegPer];
     double val[kLikBSegPer];
@@ -429,7 +455,7 @@ __global__ __launch_bounds__(kBT) void k_likb_segsum(int64_t S, const uint32_t* 
         key[e] = k < n ? skey[c0 + k] : 0xFFFFFFFFu;
         val[e] = k < n ? share[(c0 + k) * B + b] : 0.0;
     }
-    // (a group past the chunk's end holds no position: no neighbour reads)
+    // (a group past the job's end holds no position: no neighbour reads)
     const uint32_t kprev = t0 > 0 && t0 <= n ? skey[c0 + t0 - 1] : 0xFFFFFFFFu;
     const uint32_t knext = t0 + kLikBSegPer < n ? skey[c0 + t0 + kLikBSegPer] : 0xFFFFFFFFu;
     int head_any = t0 == 0 ? 1 : 0;
@@ -467,29 +493,37 @@ __global__ __launch_bounds__(kBT) void k_likb_segsum(int64_t S, const uint32_t* 
     const double incl = f ? v : cw + v;
     const double up = __shfl_up(incl, B, 64);
     double run = gw ? up : cw;
-    (void)NG;
-    const uint32_t kfirst = skey[c0];
-    const bool cont_in = c0 > 0 && skey[c0 - 1] == kfirst;
-    const uint32_t klast = skey[c0 + n - 1];
-    const int64_t c1 = c0 + n;
-    const bool cont_out = c1 < S && skey[c1] == klast;
-    const int64_t i0 = c0 + t0;
 #pragma unroll
     for (int e = 0; e < kLikBSegPer; ++e) {
-        const int64_t i = i0 + e;
-        if (i >= c1) break;
+        if (t0 + e >= n) break;
         const uint32_t kp = e ? key[e - 1] : kprev;
         if ((e == 0 && t0 == 0) || key[e] != kp) run = 0.0;
         run += val[e];
         const uint32_t kn = e + 1 < kLikBSegPer ? key[e + 1] : knext;
-        if (i + 1 == c1 || kn != key[e]) {
-            const bool starts = !(cont_in && key[e] == kfirst);
-            const bool ends = !(cont_out && key[e] == klast);
-            if (starts && ends) evt[(int64_t)key[e] * B + b] = run;
-            if (!starts) carry[(2 * (int64_t)blockIdx.x) * B + b] = run;
-            if (!ends) carry[(2 * (int64_t)blockIdx.x + 1) * B + b] = run;
-        }
+        if (t0 + e + 1 == n || kn != key[e]) evt[(int64_t)key[e] * B + b] = run;
     }
+}
+
+// Totals of the papers with more than seg_chunk<B>() positions: thread
+// (paper k, point b) adds its pieces in order.
+template <int B>
+__global__ __launch_bounds__(kBT) void k_likb_bigsum(int32_t n_big, const int32_t* __restrict__ big_paper,
+                                                      const int32_t* __restrict__ big_first,
+                                                      const double* __restrict__ bigpart,
+                                                      double* __restrict__ evt) {
+    const int64_t t = (int64_t)blockIdx.x * kBT + threadIdx.x;
+    if (t >= (int64_t)n_big * B) return;
+    const int k = (int)(t / B), b = (int)(t % B);
+    const int i0 = big_first[k], i1 = big_first[k + 1];
+    double v = 0.0;
+    for (int i = i0; i < i1; i += 8) {             // in order, eight loads in flight
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = i + u < i1 ? bigpart[(int64_t)(i + u) * B + b] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v += x[u];
+    }
+    evt[(int64_t)big_paper[k] * B + b] = v;
 }
 
 // One block: per point, log L, dL/dalpha, dL/dtau into out[3*b + c].
@@ -558,12 +592,15 @@ void run_batch(hipStream_t st, const LikSeq& q, const LikBatchBufs& bb, LikCiteS
     }
     if (q.n_years > 0)
         k_likb_year_ev<B><<<q.n_years, kBT, 0, st>>>(nE, bb.year_blk, part_e, bb.year_sums);
-    if (bb.S > 0)
-        k_likb_segsum<B><<<(unsigned)((bb.S + seg_chunk<B>() - 1) / seg_chunk<B>()), kBT, 0, st>>>(
-            bb.S, bb.skey, bb.share, bb.evt, bb.carry);
+    if (bb.n_jobs > 0)
+        k_likb_segjob<B><<<(unsigned)bb.n_jobs, kBT, 0, st>>>(bb.job_start, bb.job_tag, bb.skey,
+                                                               bb.share, bb.evt, bb.bigpart);
+    if (bb.n_big > 0)
+        k_likb_bigsum<B><<<(unsigned)(((int64_t)bb.n_big * B + kBT - 1) / kBT), kBT, 0, st>>>(
+            bb.n_big, bb.big_paper, bb.big_first, bb.bigpart, bb.evt);
     if (nP > 0)
         k_likb_papers<true, B><<<(unsigned)nP, kBT, 0, st>>>(
-            q, bb.at, bb.logfit, bb.year_sums, LikSegs{bb.seg_off, bb.evt, bb.carry}, bb.grad,
+            q, bb.at, bb.logfit, bb.year_sums, LikSegs{bb.seg_off, bb.evt, nullptr}, bb.grad,
             part_b);
     k_likb_final<B><<<1, kBT, 0, st>>>(q.n_years, nP, part_b, bb.at, bb.year_sums, bb.out,
                                        bb.out2);
@@ -579,10 +616,7 @@ int64_t lik_batch_paper_blocks(int64_t n_papers, int B) {
     return tiles < kLikBPaperBlocks ? tiles : kLikBPaperBlocks;
 }
 
-int64_t lik_batch_seg_chunks(int64_t S, int B) {
-    const int64_t c = (int64_t)kLikBSegPer * (kBT / B);
-    return (S + c - 1) / c;
-}
+int lik_batch_small_job(int B) { return kLikBSegPer * (kBT / B); }
 
 LikCiteShape lik_batch_cite_shape(int m, int B) {
     LikCiteShape sh = lik_cite_shape(m, 0);

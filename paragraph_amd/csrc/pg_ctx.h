@@ -76,7 +76,13 @@ struct pg_seq_store {
     double* b_year_sums = nullptr;
     double* b_share = nullptr;       // [filled * bB]
     double* b_evt = nullptr;         // [n_papers * bB]
-    double* b_carry = nullptr;
+    int64_t* b_job_start = nullptr;  // segment-sum jobs (cut at paper boundaries)
+    int32_t* b_job_tag = nullptr;
+    int64_t b_n_jobs = 0;
+    double* b_bigpart = nullptr;     // [pieces * bB]
+    int32_t* b_big_paper = nullptr;
+    int32_t* b_big_first = nullptr;
+    int32_t b_n_big = 0;
     double* b_at = nullptr;          // [2 * bB] alpha/tau copied from the host
     double* b_out = nullptr;         // [3 * bB]
 };

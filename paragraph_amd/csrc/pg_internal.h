@@ -394,8 +394,18 @@ struct LikBatchBufs {
     int64_t S;                 // filled slots
     const uint32_t* skey;
     const int32_t* seg_off;
-    double* evt;               // [n_papers * B]
-    double* carry;             // [2 * chunks * B]
+    double* evt;               // [n_papers * B] per-paper share totals
+    // segment-sum jobs over the sorted positions, cut at paper boundaries:
+    // job k covers [job_start[k], job_start[k+1]); job_tag[k] < 0 for a run
+    // of whole papers (at most lik_batch_small_job(B) positions), else the
+    // index of a piece (at most kLikBBigPiece positions) of one larger paper
+    const int64_t* job_start;
+    const int32_t* job_tag;
+    int64_t n_jobs;
+    double* bigpart;           // [pieces * B]
+    const int32_t* big_paper;  // [n_big] the larger papers
+    const int32_t* big_first;  // [n_big + 1] their first piece
+    int32_t n_big;
     double* grad;              // dL/dphi [n_papers * B]
     double* out;               // {log L, dL/dalpha, dL/dtau} of point b at 3b [3B]
     double* out2;              // optional copy of out
@@ -404,7 +414,8 @@ bool lik_batch_ok(int B);                                 // B in {4, 8, 16}
 bool lik_batch_shape_ok(int m, int B);                    // m slots fit the batched slot pass
 LikCiteShape lik_batch_cite_shape(int m, int B);         // G lanes per (paper, point), K slots
 int64_t lik_batch_paper_blocks(int64_t n_papers, int B);
-int64_t lik_batch_seg_chunks(int64_t S, int B);
+int lik_batch_small_job(int B);                            // positions of a whole-paper job
+constexpr int kLikBBigPiece = 4096;                        // positions of a larger paper's piece
 // The whole evaluation (both paper passes, slot pass, segment sums, final).
 void launch_lik_batch(hipStream_t s, int B, const LikSeq& q, const LikBatchBufs& bb,
                       LikCiteShape sh);

@@ -28,10 +28,16 @@ void free_ptr(void* p) {
 
 void seq_free_batch(pg_seq_store& q) {
     free_ptr(q.b_ev_block); free_ptr(q.b_year_blk); free_ptr(q.b_part); free_ptr(q.b_year_sums);
-    free_ptr(q.b_share); free_ptr(q.b_evt); free_ptr(q.b_carry); free_ptr(q.b_at);
-    free_ptr(q.b_out);
+    free_ptr(q.b_share); free_ptr(q.b_evt); free_ptr(q.b_at); free_ptr(q.b_out);
+    free_ptr(q.b_job_start); free_ptr(q.b_job_tag); free_ptr(q.b_bigpart);
+    free_ptr(q.b_big_paper); free_ptr(q.b_big_first);
     q.b_ev_block = q.b_year_blk = nullptr;
-    q.b_part = q.b_year_sums = q.b_share = q.b_evt = q.b_carry = q.b_at = q.b_out = nullptr;
+    q.b_part = q.b_year_sums = q.b_share = q.b_evt = q.b_at = q.b_out = nullptr;
+    q.b_job_start = nullptr;
+    q.b_job_tag = q.b_big_paper = q.b_big_first = nullptr;
+    q.b_bigpart = nullptr;
+    q.b_n_jobs = 0;
+    q.b_n_big = 0;
     q.bB = 0;
     for (int k = 0; k < 4; ++k) q.b_key[k] = -1;
 }
@@ -256,13 +262,62 @@ int batch_prepare(pg_ctx* ctx, int B) {
     const int64_t nE = (int64_t)ev_block.size() / 2;
     const int64_t nP = lik_batch_paper_blocks(q.n_papers, B);
     const int64_t part = ((Y + 2) * nP + 4 * nE) * B + 1;
+    // segment-sum jobs from the per-paper offsets of the sorted slots: runs
+    // of whole papers up to `small` positions, larger papers in pieces
+    std::vector<int32_t> so(q.n_papers + 1);
+    if (q.n_papers >= 0)
+        PG_HIP(hipMemcpy(so.data(), q.seg_off, (q.n_papers + 1) * 4, hipMemcpyDeviceToHost));
+    const int64_t small = lik_batch_small_job(B);
+    std::vector<int64_t> job_start;
+    std::vector<int32_t> job_tag, big_paper, big_first;
+    int32_t pieces = 0;
+    int64_t cur = 0, cur_len = 0;
+    for (int64_t j = 0; j < q.n_papers; ++j) {
+        const int64_t len = (int64_t)so[j + 1] - so[j];
+        if (len == 0) continue;
+        if (len > small) {
+            if (cur_len > 0) { job_start.push_back(cur); job_tag.push_back(-1); }
+            big_paper.push_back((int32_t)j);
+            big_first.push_back(pieces);
+            for (int64_t o = 0; o < len; o += kLikBBigPiece) {
+                job_start.push_back(so[j] + o);
+                job_tag.push_back(pieces++);
+            }
+            cur = so[j + 1];
+            cur_len = 0;
+        } else {
+            if (cur_len + len > small) {
+                job_start.push_back(cur);
+                job_tag.push_back(-1);
+                cur = so[j];
+                cur_len = 0;
+            }
+            cur_len += len;
+        }
+    }
+    if (cur_len > 0) { job_start.push_back(cur); job_tag.push_back(-1); }
+    const int64_t n_jobs = (int64_t)job_tag.size();
+    job_start.push_back(q.filled);
+    big_first.push_back(pieces);
     hipError_t r = dalloc(&q.b_ev_block, std::max<int64_t>(2 * nE, 1));
     if (r == hipSuccess) r = dalloc(&q.b_year_blk, Y + 1);
     if (r == hipSuccess) r = dalloc(&q.b_part, part);
     if (r == hipSuccess) r = dalloc(&q.b_year_sums, std::max<int64_t>(Y, 1) * kLikYearCols * B);
     if (r == hipSuccess) r = dalloc(&q.b_share, std::max<int64_t>(q.filled, 1) * B);
     if (r == hipSuccess) r = dalloc(&q.b_evt, std::max<int64_t>(q.n_papers, 1) * B);
-    if (r == hipSuccess) r = dalloc(&q.b_carry, 2 * std::max<int64_t>(lik_batch_seg_chunks(q.filled, B), 1) * B);
+    if (r == hipSuccess) r = dalloc(&q.b_job_start, n_jobs + 1);
+    if (r == hipSuccess) r = dalloc(&q.b_job_tag, std::max<int64_t>(n_jobs, 1));
+    if (r == hipSuccess) r = dalloc(&q.b_bigpart, (int64_t)std::max<int32_t>(pieces, 1) * B);
+    if (r == hipSuccess) r = dalloc(&q.b_big_paper, std::max<int64_t>((int64_t)big_paper.size(), 1));
+    if (r == hipSuccess) r = dalloc(&q.b_big_first, (int64_t)big_first.size());
+    if (r == hipSuccess)
+        r = hipMemcpy(q.b_job_start, job_start.data(), (n_jobs + 1) * 8, hipMemcpyHostToDevice);
+    if (r == hipSuccess && n_jobs > 0)
+        r = hipMemcpy(q.b_job_tag, job_tag.data(), n_jobs * 4, hipMemcpyHostToDevice);
+    if (r == hipSuccess && !big_paper.empty())
+        r = hipMemcpy(q.b_big_paper, big_paper.data(), big_paper.size() * 4, hipMemcpyHostToDevice);
+    if (r == hipSuccess)
+        r = hipMemcpy(q.b_big_first, big_first.data(), big_first.size() * 4, hipMemcpyHostToDevice);
     if (r == hipSuccess) r = dalloc(&q.b_at, 2 * B);
     if (r == hipSuccess) r = dalloc(&q.b_out, 3 * B);
     if (r == hipSuccess && nE > 0)
@@ -280,6 +335,8 @@ int batch_prepare(pg_ctx* ctx, int B) {
     q.b_shape = sh;
     q.b_nE = nE;
     q.b_nP = nP;
+    q.b_n_jobs = n_jobs;
+    q.b_n_big = (int32_t)big_paper.size();
     return PG_OK;
 }
 
@@ -555,6 +612,8 @@ int pg_pa_loglik_grad_batch(pg_ctx* ctx, const pg_lik_batch_params* p, pg_lik_ba
         return fail(ctx, PG_EINVAL, std::string(fn) + ": " + std::to_string(q.m) +
                                         " slots per paper do not fit the slot pass at " +
                                         std::to_string(B) + " points (use fewer points)");
+    if ((q.n_papers + 1) * (int64_t)B >= (int64_t(1) << 31))
+        return fail(ctx, PG_ECAP, std::string(fn) + ": papers times points must stay below 2^31");
     if (!p->alpha_tau_dev) {
         if (!p->alpha_tau) return fail(ctx, PG_EINVAL, std::string(fn) + ": no alpha/tau given");
         for (int b = 0; b < B; ++b)
@@ -593,7 +652,13 @@ int pg_pa_loglik_grad_batch(pg_ctx* ctx, const pg_lik_batch_params* p, pg_lik_ba
     bb.skey = q.skey;
     bb.seg_off = q.seg_off;
     bb.evt = q.b_evt;
-    bb.carry = q.b_carry;
+    bb.job_start = q.b_job_start;
+    bb.job_tag = q.b_job_tag;
+    bb.n_jobs = q.b_n_jobs;
+    bb.bigpart = q.b_bigpart;
+    bb.big_paper = q.b_big_paper;
+    bb.big_first = q.b_big_first;
+    bb.n_big = q.b_n_big;
     bb.grad = out->grad_logfit_dev;
     bb.out = q.b_out;
     bb.out2 = out->scalars_dev;

@@ -57,11 +57,15 @@ PG_HD float u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
 PG_HD uint64_t d2u(double x) { return __builtin_bit_cast(uint64_t, x); }
 PG_HD double u2d(uint64_t u) { return __builtin_bit_cast(double, u); }
 
+// exp(x) in FP32: Cody-Waite reduction by ln2 (hi/lo split), degree-7
+// Taylor polynomial on [-ln2/2, ln2/2], scaled by 2^n.  Written without
+// branches (the special cases select over the polynomial's result, which is
+// then discarded): every lane runs the same instructions.  The scaling is
+// one ldexp: for the n this range reaches (-126..128) p*2^n is a single
+// correctly rounded scaling, the same value as scaling by two halves of n.
+// Results below the smallest normal are 0; a NaN input gives a NaN.
 PG_HD float det_expf(float x) {
-    if (x != x) return x;
-    if (x > 88.7228317f) return __builtin_huge_valf();
-    if (x < -87.3365402f) return 0.0f;
-    float kf = __builtin_rintf(x * 0x1.715476p+0f);
+    const float kf = __builtin_rintf(x * 0x1.715476p+0f);
     float r = __builtin_fmaf(kf, -0x1.62e4p-1f, x);
     r = __builtin_fmaf(kf, -0x1.7f7d1cp-20f, r);
     float p = 1.0f / 5040.0f;
@@ -72,12 +76,12 @@ PG_HD float det_expf(float x) {
     p = __builtin_fmaf(p, r, 0.5f);
     p = __builtin_fmaf(p, r, 1.0f);
     p = __builtin_fmaf(p, r, 1.0f);
-    int n = (int)kf;
-    int n1 = n / 2;
-    int n2 = n - n1;
-    float s1 = u2f((uint32_t)(n1 + 127) << 23);
-    float s2 = u2f((uint32_t)(n2 + 127) << 23);
-    return (p * s1) * s2;
+    // kf clamped (a NaN goes to a bound) so the conversion is defined; a NaN
+    // x carries through the polynomial, out-of-range x are selected below
+    const int n = (int)__builtin_fminf(__builtin_fmaxf(kf, -200.0f), 200.0f);
+    float v = __builtin_ldexpf(p, n);
+    v = x < -87.3365402f ? 0.0f : v;
+    return x > 88.7228317f ? __builtin_huge_valf() : v;
 }
 
 PG_HD float det_logf(float x) {
@@ -137,6 +141,21 @@ PG_HD double det_log(double x) {
     if (mm > 1.4142135623730951) { mm = mm * 0.5; e += 1; }
     double lm = det_log_core(mm - 1.0);
     double ef = (double)e;
+    return __builtin_fma(ef, 6.93147180369123816490e-01,
+                         __builtin_fma(ef, 1.90821492927058770002e-10, lm));
+}
+
+// det_log for a positive, normal, finite x: the same operations on that
+// path, without the special cases (a partition sum minus earlier picks).
+PG_HD double det_log_pos(double x) {
+    const uint64_t b = d2u(x);
+    int e = (int)(b >> 52) - 1023;
+    double mm = u2d((b & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
+    const bool hi = mm > 1.4142135623730951;
+    mm = hi ? mm * 0.5 : mm;
+    e += hi ? 1 : 0;
+    const double lm = det_log_core(mm - 1.0);
+    const double ef = (double)e;
     return __builtin_fma(ef, 6.93147180369123816490e-01,
                          __builtin_fma(ef, 1.90821492927058770002e-10, lm));
 }

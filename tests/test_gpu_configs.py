@@ -165,6 +165,18 @@ def test_likelihood_30y_1m_seed():
     seq = ctx.seq_download()
     phi = torch.from_numpy(np.ascontiguousarray(g.logfit[:n_final])).to("cuda")
     grad = torch.empty(n_final, dtype=torch.float64, device="cuda")
+    # scale of the scalar gradients' terms: dL/dalpha and dL/dtau are
+    # differences of two sums of this size (near the generating point they
+    # nearly cancel), so their errors are measured against it
+    sum_L = sum_age = 0.0
+    n_cited = np.zeros(n_final, np.float64)  # PA slots citing each paper: dL/dphi's term scale
+    for y in range(years):
+        c, k = seq["cit"][y], seq["kind"][y]
+        pa = (np.arange(m)[None, :] < seq["cnt"][y][:, None]) & (k != 1)
+        tg = c[pa]
+        n_cited += np.bincount(tg, minlength=n_final)
+        sum_L += float(np.log1p(seq["deg"][y][tg].astype(np.float64)).sum())
+        sum_age += float((seq["year_of"][y] - seq["birth"][tg]).astype(np.float64).sum())
     for a, t in [(1.0, 5.0), (1.2, 4.0)]:
         ll, ga, gt = ctx.pa_loglik_grad(a, t, phi.data_ptr(), grad.data_ptr())
         gphi = grad.cpu().numpy()
@@ -177,7 +189,12 @@ def test_likelihood_30y_1m_seed():
         rel = lambda x, y: abs(x - y) / abs(y)
         log(f"alpha={a} tau={t}: log L {ll:.12g} (same arith {w[0]:.12g}, fp64 {d[0]:.12g}); "
             f"dalpha {ga:.10g} / {w[1]:.10g} / {d[1]:.10g}; dtau {gt:.10g} / {w[2]:.10g} / {d[2]:.10g}")
-        assert rel(ll, w[0]) <= 1e-9 and rel(ga, w[1]) <= 1e-9 and rel(gt, w[2]) <= 1e-9
+        sa, st = sum_L, sum_age / (t * t)
+        log(f"  scalar errors / term scale: same arithmetic {abs(ga - w[1]) / sa:.1e}, "
+            f"{abs(gt - w[2]) / st:.1e}; fp64 {abs(ga - d[1]) / sa:.1e}, {abs(gt - d[2]) / st:.1e}")
+        assert rel(ll, w[0]) <= 1e-9
+        assert abs(ga - w[1]) <= 1e-9 * max(abs(w[1]), sa)
+        assert abs(gt - w[2]) <= 1e-9 * max(abs(w[2]), st)
         # dL/dphi_j sums up to ~1e5 slot shares of a hub and 30 years of
         # A_y*w_j in another FP64 order than the oracle: 1e-8 on max(1, |g|)
         # (the measured worst paper is about 1.6e-9), 1e-10 over the vector
@@ -186,6 +203,11 @@ def test_likelihood_30y_1m_seed():
             f"{np.linalg.norm(gphi - w[3]) / np.linalg.norm(w[3]):.2e}")
         assert err.max() <= 1e-8
         assert np.linalg.norm(gphi - w[3]) / np.linalg.norm(w[3]) <= 1e-10
-        assert rel(ll, d[0]) <= 1e-5 and rel(ga, d[1]) <= 1e-5 and rel(gt, d[2]) <= 1e-5
-        assert np.linalg.norm(gphi - d[3]) / np.linalg.norm(d[3]) <= 1e-5
+        # FP32 weights against FP64 ones: 1e-5 relative, or of the terms'
+        # scale where the scalar gradient nearly cancels
+        assert rel(ll, d[0]) <= 1e-5
+        assert abs(ga - d[1]) <= 1e-5 * max(abs(d[1]), sa)
+        assert abs(gt - d[2]) <= 1e-5 * max(abs(d[2]), st)
+        assert np.linalg.norm(gphi - d[3]) <= 1e-5 * max(np.linalg.norm(d[3]),
+                                                          np.linalg.norm(n_cited))
     ctx.close()
