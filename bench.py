@@ -33,14 +33,20 @@ GROUP_KERNELS = {
     "bfs": ["k_bfs_small", "k_bfs_mid", "k_bfs_root"],
     "lik_grad": ["k_lik_papers<false>", "k_lik_year_z", "k_lik_cite<true, 4, 5>", "k_lik_year_ev",
                  "k_lik_segsum", "k_lik_papers<true>", "k_lik_final"],
+    "lik_batch16": ["k_likb_papers<false, 16>", "k_likb_year_z<16>", "k_likb_cite<4, 5, 16>",
+                    "k_likb_year_ev<16>", "k_likb_segjob<16>", "k_likb_bigsum<16>",
+                    "k_likb_papers<true, 16>", "k_likb_final<16>"],
+    "er": ["k_er_count", "k_er_write"],
+    "ens_bfs": ["k_bfs_small", "k_bfs_mid", "k_bfs_root"],
 }
 
 
 def measured_traffic(summary: str, group: str):
     """Memory-side bytes per launch of a kernel group: the sum over its
-    kernels of the copy-calibrated FETCH_SIZE + WRITE_SIZE per dispatch in
-    the committed rocprofv3 summary profiles/<summary> (two --pmc runs of the
-    same workload).  None when the summary or a kernel row is missing."""
+    kernels of the read and write bytes per dispatch in the committed
+    rocprofv3 summary profiles/<summary>: L2-to-fabric requests counted by
+    size (TCC_EA0_RDREQ_32B/64B/128B, TCC_EA0_WRREQ/_64B; two --pmc runs of
+    the same workload).  None when the summary or a kernel row is missing."""
     path = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(path):
         return None
@@ -63,8 +69,8 @@ def roofline(kernel: str, achieved: float, summary: str, group: str) -> dict:
     return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": PEAK_HBM_GBS,
             "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": t,
             "traffic_note": None if t is None else
-            f"bytes per launch of {', '.join(GROUP_KERNELS[group])}: FETCH_SIZE + WRITE_SIZE "
-            f"per dispatch from profiles/{summary} (rocprofv3 --pmc, copy-calibrated)"}
+            f"bytes per launch of {', '.join(GROUP_KERNELS[group])}: L2-to-fabric read and "
+            f"write requests by size per dispatch from profiles/{summary} (rocprofv3 --pmc)"}
 
 
 def eprint(*a):
@@ -92,9 +98,35 @@ def cpu_baseline(G0, sched, m, n_nb, alpha, tau, seed, cap):
         G = ob.csr_append(G, off, tgt, G0.logfit, 1 + yi)
         edges += int(res["cnt"].sum())
     dt = time.perf_counter() - t0
+    # one thread on a bounded sample: the first two years
+    G1, e1 = G0, 0
+    t1 = time.perf_counter()
+    for yi, D in enumerate(sched[:2]):
+        res = ob.pa_year(G1, seed, yi, 1 + yi, D, m, n_nb, alpha, tau, cap, threads=1)
+        off, tgt = ob.pa_compact(res)
+        G1 = ob.csr_append(G1, off, tgt, G0.logfit, 1 + yi)
+        e1 += int(res["cnt"].sum())
+    dt1 = time.perf_counter() - t1
     return {"value": edges / dt, "unit": "citation edges/s", "cores": cores, "kind": "port",
             "sample": f"the full workload ({len(sched)} years, {edges} edges) on the C oracle "
-                      f"with {cores} OpenMP threads: {dt:.2f} s"}, G
+                      f"with {cores} OpenMP threads: {dt:.2f} s",
+            "host_cpu": host_cpu(),
+            "one_thread": {"value": e1 / dt1, "unit": "citation edges/s", "cores": 1,
+                           "sample": f"years 0-1 ({e1} edges, draws + CSR append) on one "
+                                     f"thread: {dt1:.2f} s"}}, G
+
+
+def host_cpu() -> str:
+    """The host CPU model (lscpu's model name) and its logical CPU count."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs, {len(os.sched_getaffinity(0))} usable"
 
 
 def degree_ks(gpu_in_deg, cpu_in_deg) -> dict:
@@ -212,10 +244,10 @@ def er_bench(args) -> dict:
            "unit": "citation edges/s", "ms_per_step": wall * 1e3, "edges_per_step": edges,
            "year0_equals_oracle": ok,
            "kernel_ms_per_step": {k: v["ms"] / len(walls) for k, v in prof.items() if v["launches"]},
-           "roofline": {"bound": "hbm", "kernel": "er", "achieved": achieved, "peak": PEAK_HBM_GBS,
-                        "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                        "traffic_note": "4 B per target + 4 B per new paper: about 16 MB per run, "
-                                        "latency- and FP64-log-bound"}}
+           "roofline": dict(roofline("er (k_er_count + k_er_write)", achieved,
+                                     "er_10y_1M_kernels.md", "er"),
+                            bound_note="4 B per target + 4 B per new paper: about 16 MB per "
+                                       "run; the kernels are latency- and FP64-log-bound")}
     if not args.no_cpu_baseline:
         cores = len(os.sched_getaffinity(0))
         t0 = time.perf_counter()
@@ -315,39 +347,43 @@ def likelihood_bench(args) -> dict:
     eprint(f"fused gradient: {evals_s:.1f} evals/s ({1e3 / evals_s:.3f} ms wall, {dev_ms:.3f} ms "
            f"device, {bytes_eval / 1e9:.3f} GB algorithmic, {achieved:.0f} GB/s); "
            f"log L {ll:.10g} dL/dalpha {ga:.8g} dL/dtau {gt:.8g}")
-    # torch.autograd.Function (forward + backward, device-resident parameters)
-    at = torch.tensor([a0, t0_], dtype=torch.float64, device=dev, requires_grad=True)
-    phr = phi.clone().requires_grad_(True)
-    K2 = max(K // 4, 5)
-    for i in range(K2 + 2):
-        if i == 2:
-            torch.cuda.synchronize()
-            t2 = time.perf_counter()
-        at.grad = None
-        phr.grad = None
-        PALogLikelihood.apply(ctx, at, phr).backward()
-    torch.cuda.synchronize()
-    autograd_s = K2 / (time.perf_counter() - t2)
-    # plain PyTorch autograd, FP64
+    batched, g_b0 = batched_likelihood(ctx, phi, n_final, a0, t0_, stream, args,
+                                       (ll, ga, gt), grad)
     seq = ctx.seq_download()
-    ps = PlainSequence(seq, dev)
-    at2 = torch.tensor([a0, t0_], dtype=torch.float64, device=dev, requires_grad=True)
-    ph2 = phi.to(torch.float64).requires_grad_(True)
-    K3 = 3
-    for i in range(K3 + 1):
-        if i == 1:
-            torch.cuda.synchronize()
-            t3 = time.perf_counter()
-        at2.grad = None
-        ph2.grad = None
-        ll2 = loglik_plain_torch(ps, at2, ph2)
-        ll2.backward()
-    torch.cuda.synchronize()
-    plain_s = K3 / (time.perf_counter() - t3)
-    g_plain = ph2.grad.cpu().numpy()
-    ga2, gt2 = at2.grad.cpu().numpy()
-    eprint(f"autograd.Function: {autograd_s:.1f} evals/s; plain PyTorch autograd (FP64): "
-           f"{plain_s:.2f} evals/s; plain log L {float(ll2.detach()):.10g}")
+    autograd_s = plain_s = None
+    if not args.lik_no_torch:
+        # torch.autograd.Function (forward + backward, device-resident parameters)
+        at = torch.tensor([a0, t0_], dtype=torch.float64, device=dev, requires_grad=True)
+        phr = phi.clone().requires_grad_(True)
+        K2 = max(K // 4, 5)
+        for i in range(K2 + 2):
+            if i == 2:
+                torch.cuda.synchronize()
+                t2 = time.perf_counter()
+            at.grad = None
+            phr.grad = None
+            PALogLikelihood.apply(ctx, at, phr).backward()
+        torch.cuda.synchronize()
+        autograd_s = K2 / (time.perf_counter() - t2)
+        # plain PyTorch autograd, FP64
+        ps = PlainSequence(seq, dev)
+        at2 = torch.tensor([a0, t0_], dtype=torch.float64, device=dev, requires_grad=True)
+        ph2 = phi.to(torch.float64).requires_grad_(True)
+        K3 = 3
+        for i in range(K3 + 1):
+            if i == 1:
+                torch.cuda.synchronize()
+                t3 = time.perf_counter()
+            at2.grad = None
+            ph2.grad = None
+            ll2 = loglik_plain_torch(ps, at2, ph2)
+            ll2.backward()
+        torch.cuda.synchronize()
+        plain_s = K3 / (time.perf_counter() - t3)
+        g_plain = ph2.grad.cpu().numpy()
+        ga2, gt2 = at2.grad.cpu().numpy()
+        eprint(f"autograd.Function: {autograd_s:.1f} evals/s; plain PyTorch autograd (FP64): "
+               f"{plain_s:.2f} evals/s; plain log L {float(ll2.detach()):.10g}")
     g_gpu = grad.cpu().numpy()
     res = {
         "metric": "likelihood-grad evals/s (30y sequence, 1M seed)",
@@ -355,7 +391,7 @@ def likelihood_bench(args) -> dict:
         "unit": "gradient evaluations/s",
         "ms_per_eval": 1e3 / evals_s,
         "device_ms_per_eval": dev_ms,
-        "vs_plain": evals_s / plain_s,
+        "vs_plain": evals_s / plain_s if plain_s else None,
         "plain_torch_evals_per_s": plain_s,
         "autograd_function_evals_per_s": autograd_s,
         "dtype": "f32 weights, f64 partition sums and gradients",
@@ -366,7 +402,8 @@ def likelihood_bench(args) -> dict:
                    "slots": info.new_papers * m, "params": info.n_papers + 2},
         "roofline": roofline("pg_pa_loglik_grad (all passes)", achieved,
                              "likelihood_30y_1M_kernels.md", "lik_grad"),
-        "vs_plain_torch_rel_err": {
+        "batched": batched,
+        "vs_plain_torch_rel_err": None if plain_s is None else {
             "dalpha": abs(ga - ga2) / abs(ga2), "dtau": abs(gt - gt2) / abs(gt2),
             "dphi_l2": float(np.linalg.norm(g_gpu - g_plain) / np.linalg.norm(g_plain))},
     }
@@ -392,6 +429,16 @@ def likelihood_bench(args) -> dict:
             "loglik": abs(ll - w[0]) / abs(w[0]), "dalpha": abs(ga - w[1]) / abs(w[1]),
             "dtau": abs(gt - w[2]) / abs(w[2]),
             "dphi_max": float(np.max(np.abs(g_gpu - w[3]) / np.maximum(1.0, np.abs(w[3]))))}
+        # the batched call's point 0 is this evaluation's (alpha, tau, phi)
+        h0 = batched["point0"]
+        batched["point0_vs_oracle_fp64_rel_err"] = {
+            "loglik": abs(h0[0] - d[0]) / abs(d[0]), "dalpha": abs(h0[1] - d[1]) / abs(d[1]),
+            "dtau": abs(h0[2] - d[2]) / abs(d[2]),
+            "dphi_l2": float(np.linalg.norm(g_b0 - d[3]) / np.linalg.norm(d[3]))}
+        batched["point0_vs_oracle_same_arith_rel_err"] = {
+            "loglik": abs(h0[0] - w[0]) / abs(w[0]), "dalpha": abs(h0[1] - w[1]) / abs(w[1]),
+            "dtau": abs(h0[2] - w[2]) / abs(w[2]),
+            "dphi_max": float(np.max(np.abs(g_b0 - w[3]) / np.maximum(1.0, np.abs(w[3]))))}
         eprint(f"oracle: {cpu_s:.2f} s per FP64 evaluation; errors {res['vs_oracle_fp64_rel_err']}"
                f" (FP64), {res['vs_oracle_same_arith_rel_err']} (same arithmetic)")
     # Adam from alpha=1, tau=5 (log-fitness held at its true values)
@@ -413,6 +460,70 @@ def likelihood_bench(args) -> dict:
     res["autotune"] = tune
     ctx.close()
     return res
+
+
+def batched_likelihood(ctx, phi, n_final, a0, t0_, stream, args, plain3, plain_grad):
+    """pg_pa_loglik_grad_batch on the same sequence at B = 4, 8, 16 points
+    per pass (point b at alpha = a0 + 0.02 b, tau = t0 - 0.1 b, the same
+    log-fitness): evaluations/s, device time and algorithmic bytes per call.
+    Point 0 is the single-point evaluation's and must agree with it."""
+    import torch
+    dev = phi.device
+    per_b, best = [], None
+    g0 = None
+    for B in (4, 8, 16):
+        pb = phi.repeat_interleave(B).contiguous()
+        gb = torch.empty(n_final * B, dtype=torch.float64, device=dev)
+        sb = torch.empty(3 * B, dtype=torch.float64, device=dev)
+        at = [(a0 + 0.02 * b, t0_ - 0.1 * b) for b in range(B)]
+
+        def call():
+            ctx.pa_loglik_grad_batch(at, pb.data_ptr(), gb.data_ptr(), scalars_dev=sb.data_ptr(),
+                                     stream=stream, host=False)
+
+        for _ in range(2):
+            call()
+        torch.cuda.synchronize()
+        K = max(args.lik_evals // B, 5)
+        t1 = time.perf_counter()
+        for _ in range(K):
+            call()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        ctx.profile(True)
+        ctx.profile_reset()
+        for _ in range(10):
+            call()
+        prof = ctx.profile_read()["lik_batch"]
+        ctx.profile(False)
+        ms = prof["ms"] / prof["launches"]
+        by = prof["bytes"] / prof["launches"]
+        ach = by / (ms * 1e-3) / 1e9
+        h = sb.cpu().numpy().reshape(B, 3)
+        gcol = gb.view(n_final, B)[:, 0].cpu().numpy()
+        rel = max(abs(h[0, c] - plain3[c]) / max(abs(plain3[c]), 1e-300) for c in range(3))
+        gref = plain_grad.cpu().numpy()
+        gerr = float(np.max(np.abs(gcol - gref) / np.maximum(1.0, np.abs(gref))))
+        row = {"points": B, "evals_per_s": B * K / dt, "ms_per_call": 1e3 * dt / K,
+               "device_ms_per_call": ms, "gb_per_call": by / 1e9, "achieved_GBs": ach,
+               "frac": ach / PEAK_HBM_GBS, "point0_vs_single_max_rel": rel,
+               "point0_dphi_vs_single_max": gerr}
+        eprint(f"batched B={B}: {row['evals_per_s']:.1f} evals/s ({ms:.3f} ms device per call, "
+               f"{by / 1e9:.3f} GB algorithmic, {ach:.0f} GB/s = {ach / PEAK_HBM_GBS:.3f} of "
+               f"peak); point 0 vs single-point: scalars {rel:.1e}, dL/dphi {gerr:.1e}")
+        if rel > 1e-12 or gerr > 1e-10:
+            raise RuntimeError(f"batched point 0 differs from the single-point evaluation (B={B})")
+        per_b.append(row)
+        if B == 16:
+            best = (row, h[0].tolist(), ach)
+            g0 = gcol
+        del pb, gb
+    row16, h0, ach16 = best
+    return {"metric": "likelihood-grad evals/s, 16 parameter points per pass (30y sequence)",
+            "value": row16["evals_per_s"], "unit": "gradient evaluations/s",
+            "per_points": per_b, "point0": h0,
+            "roofline": roofline("pg_pa_loglik_grad_batch, 16 points (all passes)", ach16,
+                                 "likelihood_30y_1M_kernels.md", "lik_batch16")}, g0
 
 
 def ensemble_bench(args) -> dict:
@@ -441,7 +552,10 @@ def ensemble_bench(args) -> dict:
         ens.load(s, g)
     eprint(f"ensemble loaded ({time.perf_counter() - t1:.1f} s)")
     tune = None
-    if not args.no_tune:
+    if args.ens_shape:
+        lanes, mode = (int(x) for x in args.ens_shape.split(","))
+        ens.set_tuning(pa_lanes=lanes, bfs_mode=mode)
+    elif not args.no_tune:
         t4 = time.perf_counter()
         tune = ens.autotune(reps=args.tune_reps)   # leaves the seed state
         eprint(f"autotune ({time.perf_counter() - t4:.1f} s): chosen {tune['chosen']} "
@@ -465,10 +579,21 @@ def ensemble_bench(args) -> dict:
     eprint(f"ensemble run: {edges} edges, {wall * 1e3:.1f} ms wall ({dev:.1f} ms device; eager "
            f"{eager_ms:.1f} ms, first graph run {capture_ms:.1f} ms), {edges / wall:.3e} edges/s, "
            f"near-boundary {st.near_boundary}, shortfall {st.shortfall}")
+    # the dominant group (2-hop kernels) timed in eager year steps
+    ens.restore()
+    bfs_ms = bfs_bytes = step_ms = 0.0
+    for _ in range(years):
+        sst = ens.step()
+        bfs_ms += sst.bfs_ms
+        bfs_bytes += sst.bfs_bytes
+        step_ms += sst.device_ms
+    ens_ach = bfs_bytes / years / (bfs_ms / years * 1e-3) / 1e9
+    eprint(f"ensemble 2-hop kernels: {bfs_ms:.1f} of {step_ms:.1f} ms over the {years} eager "
+           f"steps, {bfs_bytes / 1e9:.2f} GB algorithmic, {ens_ach:.0f} GB/s")
     ens.close()
     # the same simulations as plain runs, one after another
     plain_s, plain_edges = 0.0, 0
-    for sp, (g, sched) in zip(sims, graphs):
+    for sp, (g, sched) in ([] if args.ens_no_plain else zip(sims, graphs)):
         ctx = pg.Context(g.n + sum(sched), g.src.size + sum(sched) * m + 16, max(sched),
                          max_cites=m, bfs_cap=4096, seed=sp["seed"])
         ctx.load(g)
@@ -477,23 +602,29 @@ def ensemble_bench(args) -> dict:
             plain_edges += ctx.year_pa(1 + yi, yi, d, m, sp["n_nb"], sp["alpha"], sp["tau"]).edges_added
         plain_s += time.perf_counter() - t3
         ctx.close()
-    eprint(f"plain runs: {plain_edges} edges in {plain_s:.3f} s ({plain_edges / plain_s:.3e} edges/s)")
-    if plain_edges != edges:
-        raise RuntimeError(f"ensemble added {edges} edges, plain runs {plain_edges}")
+    if not args.ens_no_plain:
+        eprint(f"plain runs: {plain_edges} edges in {plain_s:.3f} s "
+               f"({plain_edges / plain_s:.3e} edges/s)")
+        if plain_edges != edges:
+            raise RuntimeError(f"ensemble added {edges} edges, plain runs {plain_edges}")
+    plain_rate = plain_edges / plain_s if plain_s else None
     return {
         "metric": f"ensemble citation edges/s ({len(sims)} simulations, 10y)",
         "value": edges / wall,
         "unit": "citation edges/s",
         "ms_per_run": wall * 1e3,
         "device_ms_per_run": dev,
-        "plain_sequential_edges_per_s": plain_edges / plain_s,
-        "plain_sequential_s": plain_s,
-        "vs_plain": (edges / wall) / (plain_edges / plain_s),
+        "plain_sequential_edges_per_s": plain_rate,
+        "plain_sequential_s": plain_s or None,
+        "vs_plain": (edges / wall) / plain_rate if plain_rate else None,
         "config": {"workload": f"{len(sims)} PA simulations, seeds log-uniform in [1e4, 5e5] "
                                f"papers, alpha~U[0.5,1.5], tau~U[1,10], f_nb~U[0,0.5], m=20, "
                                f"10y at 3%/y, Philox keys 0..{len(sims) - 1}",
                    "seed_papers": n_seed, "new_papers": n_new, "edges_added": edges,
                    "near_boundary_attempts": st.near_boundary, "hipgraph": True},
+        "bfs_ms_per_run": bfs_ms, "eager_step_ms_per_run": step_ms,
+        "roofline": roofline("2-hop kernels per year launch (k_bfs_small, k_bfs_mid, "
+                             "k_bfs_root)", ens_ach, "ensemble_256x10y_kernels.md", "ens_bfs"),
         "autotune": tune,
     }
 
@@ -529,6 +660,8 @@ def main():
     ap.add_argument("--lik-evals", type=int, default=200,
                     help="timed likelihood-gradient evaluations")
     ap.add_argument("--fit-steps", type=int, default=100)
+    ap.add_argument("--lik-no-torch", action="store_true",
+                    help="skip the autograd.Function and plain-PyTorch comparisons")
     ap.add_argument("--no-lik", action="store_true", help="skip the likelihood part")
     ap.add_argument("--no-ens", action="store_true", help="skip the ensemble part")
     ap.add_argument("--no-er", action="store_true", help="skip the ER part")
@@ -536,6 +669,10 @@ def main():
     ap.add_argument("--ens-only", action="store_true",
                     help="only the ensemble part (its JSON object on stdout)")
     ap.add_argument("--ens-sims", type=int, default=256)
+    ap.add_argument("--ens-shape", default=None,
+                    help="ensemble launch shape 'PA_LANES,BFS_MODE' instead of the autotuner")
+    ap.add_argument("--ens-no-plain", action="store_true",
+                    help="skip the ensemble's sequential plain-run comparison")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuners (ensemble, likelihood)")
     ap.add_argument("--tune-reps", type=int, default=2)

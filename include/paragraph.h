@@ -421,6 +421,10 @@ typedef struct {
     int64_t shortfall;
     int64_t pa_attempts;
     double device_ms;          /* GPU time of the run (hipEvents) */
+    double bfs_ms;             /* pg_ensemble_step: GPU time of the 2-hop kernels
+                                  (hipEvents around them); 0 for pg_ensemble_run */
+    double bfs_bytes;          /* algorithmic bytes of the 2-hop kernels (the
+                                  BFS formula of pg_profile_read) */
 } pg_ensemble_stats;
 
 /* Launch shapes that change speed, never results. */

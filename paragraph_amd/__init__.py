@@ -96,7 +96,8 @@ class SimParams(ctypes.Structure):
 class EnsembleStats(ctypes.Structure):
     _fields_ = [("size", ctypes.c_uint32), ("edges_added", ctypes.c_int64),
                 ("near_boundary", ctypes.c_int64), ("shortfall", ctypes.c_int64),
-                ("pa_attempts", ctypes.c_int64), ("device_ms", ctypes.c_double)]
+                ("pa_attempts", ctypes.c_int64), ("device_ms", ctypes.c_double),
+                ("bfs_ms", ctypes.c_double), ("bfs_bytes", ctypes.c_double)]
 
 
 class Tuning(ctypes.Structure):

@@ -86,7 +86,7 @@ Batch batch_of(const pg_ensemble* ens, int y) {
 
 // Enqueues growth year y for every simulation (no host synchronisation, no
 // memset: capturable).
-void enqueue_year(pg_ensemble* ens, int y, hipStream_t s) {
+void enqueue_year(pg_ensemble* ens, int y, hipStream_t s, hipEvent_t* bfs_ev = nullptr) {
     pg_ctx* c = ens->ctx;
     const Batch b = batch_of(ens, y);
     const int m = ens->cfg.m;
@@ -107,9 +107,11 @@ void enqueue_year(pg_ensemble* ens, int y, hipStream_t s) {
         int32_t* big_count = reinterpret_cast<int32_t*>(cnt + 3);
         launch_fill_i32(s, big_count, 0, 2);
         const CsrView g{c->fwd_off, c->fwd_adj, c->bwd_off[c->cur], c->bwd_adj[c->cur], TOT};
+        if (bfs_ev) (void)hipEventRecord(bfs_ev[0], s);
         launch_bfs_root(s, g, c->w, c->uroot, c->r_excl + Dn, Dn, c->cfg.bfs_cap, c->S_buf,
                         c->Q_buf, c->Q_idx, c->ns_buf, c->big_list, big_count, cnt + 2,
                         bfs_stage_view(c->stage, c->tune_bfs_mode));
+        if (bfs_ev) (void)hipEventRecord(bfs_ev[1], s);
         launch_nb_draw_batch(s, b, m, c->cfg.bfs_cap, (uint32_t)y, c->cdf, c->root_slot,
                              c->S_buf, c->Q_buf, c->Q_idx, c->ns_buf, c->cit, c->kind, c->cnt, c->near,
                              c->anext);
@@ -365,7 +367,10 @@ int pg_ensemble_step(pg_ensemble* ens, pg_ensemble_stats* stats) {
     const int y = ens->next_year;
     EHIP(hipEventRecord(c->ev0, s));
     launch_fill_i32(s, reinterpret_cast<int32_t*>(ens->counters), 0, 16);
-    enqueue_year(ens, y, s);
+    bool any_nb = false;
+    for (int si = 0; si < ens->cfg.n_sims; ++si) any_nb |= ens->sims[si].n_nb > 0;
+    hipEvent_t* bev = any_nb && ens->year_new[y] > 0 ? c->pev[PG_K_BFS] : nullptr;
+    enqueue_year(ens, y, s, bev);
     EHIP(hipGetLastError());
     EHIP(hipEventRecord(c->ev1, s));
     EHIP(hipEventSynchronize(c->ev1));
@@ -381,6 +386,10 @@ int pg_ensemble_step(pg_ensemble* ens, pg_ensemble_stats* stats) {
         stats->pa_attempts = (int64_t)h[1];
         stats->shortfall = ens->year_new[y] * ens->cfg.m - (int64_t)h[4];
         stats->device_ms = ms;
+        float bms = 0.0f;
+        if (bev) EHIP(hipEventElapsedTime(&bms, bev[0], bev[1]));
+        stats->bfs_ms = bms;
+        stats->bfs_bytes = (double)h[2];
     }
     return PG_OK;
 }
@@ -460,6 +469,8 @@ int pg_ensemble_run(pg_ensemble* ens, int32_t use_graph, pg_ensemble_stats* stat
         stats->pa_attempts = (int64_t)h[1];
         stats->shortfall = slots - (int64_t)h[4];
         stats->device_ms = ms;
+        stats->bfs_ms = 0.0;
+        stats->bfs_bytes = (double)h[2];
     }
     return PG_OK;
 }
