@@ -75,7 +75,9 @@ const char* kKnown[] = {
     "--pa-exponent", "--age-decay", "--neighbourhood-fraction", "--citations-per-paper",
     "--seed", "--bfs-cap", "--fitness-sigma", "--device",
     // exact year-granular checkpoint / resume (no reference counterpart)
-    "--checkpoint-file", "--resume-from"};
+    "--checkpoint-file", "--resume-from",
+    // `node,year` list of the grown graph, the nodelist input of a chained run
+    "--output-nodelist"};
 
 const char* kRequired[] = {"--edgelist", "--nodelist", "--out-degree-bag",
                            "--recency-probabilities", "--growth-rate", "--num-cycles",
@@ -90,7 +92,8 @@ void usage() {
                  "           [--model pa|er-gnp] [--er-probability p] [--er-edges-per-node k]\n"
                  "           [--pa-exponent a] [--age-decay tau] [--neighbourhood-fraction f | --alpha f]\n"
                  "           [--citations-per-paper m] [--seed s] [--bfs-cap c] [--fitness-sigma s]\n"
-                 "           [--checkpoint-file F (written after every year)] [--resume-from F]\n");
+                 "           [--checkpoint-file F (written after every year)] [--resume-from F]\n"
+                 "           [--output-nodelist F (node,year of every paper, for chaining runs)]\n");
 }
 
 Args parse(int argc, char** argv) {
@@ -493,6 +496,27 @@ int main(int argc, char** argv) {
                    "assigned_out_degree,planted_nodes_line_number,generator_node_string\n";
             for (const auto& b : part) out.write(b.data(), (std::streamsize)b.size());
             if (!out) throw std::runtime_error("cannot write --auxiliary-information-file");
+        }
+        if (a.has("--output-nodelist")) {
+            // `node,year` per paper in ascending original id (the order the
+            // nodelist parser assigns dense ids in, src/graph.cu:336-373), so
+            // this file with --output-file is the input of a chained run
+            std::vector<std::string> part(T);
+            parallel_for(T, [&](int t) {
+                const int64_t b = nn / T * t, e = t + 1 == T ? nn : nn / T * (t + 1);
+                std::string& buf = part[t];
+                buf.reserve((size_t)(e - b) * 16);
+                for (int64_t v = b; v < e; ++v) {
+                    append_int(buf, orig[v]);
+                    buf.push_back(',');
+                    append_int(buf, by[v]);
+                    buf.push_back('\n');
+                }
+            });
+            std::ofstream out(a.str("--output-nodelist"), std::ios::binary);
+            out << "#node,year\n";
+            for (const auto& b : part) out.write(b.data(), (std::streamsize)b.size());
+            if (!out) throw std::runtime_error("cannot write --output-nodelist");
         }
         const double write_s = std::chrono::duration<double>(Clock::now() - t_write).count();
         std::printf("I/O: read %.3f s, wrote %.3f s (%d threads)\n", read_s, write_s, T);

@@ -114,3 +114,55 @@ def test_abm_input_forms_equal_clean_input(tmp_path):
     run("nodes_q.txt", "edges_q.txt", "quirky")
     for f in ("out", "aux"):
         assert open(j(f"{f}_clean.csv"), "rb").read() == open(j(f"{f}_quirky.csv"), "rb").read(), f
+
+
+def test_abm_nodelist_output_chains_runs(tmp_path):
+    """--output-nodelist writes `node,year` for every paper: with the edgelist
+    it is the input of a chained run.  A zero-year run on those files writes
+    the same bytes back, and a run that grows on from them keeps the first
+    run's graph as the prefix of its edgelist."""
+    d = str(tmp_path)
+    j = lambda f: os.path.join(d, f)
+    n0 = 1500
+    g = pg.seed_graph(9, n0, 5.0)
+    ids = np.arange(n0) * 2 + 7
+    with open(j("nodes.csv"), "w") as f:
+        for i in range(n0):
+            f.write(f"{ids[i]},{g.birth[i] + 2000}\n")
+    with open(j("edges.csv"), "w") as f:
+        for s, t in zip(g.src, g.dst):
+            f.write(f"{ids[s]},{ids[t]}\n")
+    open(j("bag.csv"), "w").write("0,6\n")
+    open(j("rec.csv"), "w").write("0,1\n")
+
+    def run(nodes, edges, years, tag):
+        args = [pg.ABM_PATH, "--edgelist", j(edges), "--nodelist", j(nodes),
+                "--out-degree-bag", j("bag.csv"), "--recency-probabilities", j("rec.csv"),
+                "--growth-rate", "0.05", "--num-cycles", str(years), "--same-year-proportion", "0",
+                "--output-file", j(f"out_{tag}.csv"), "--auxiliary-information-file",
+                j(f"aux_{tag}.csv"), "--log-file", j(f"log_{tag}.txt"), "--seed", "5",
+                "--output-nodelist", j(f"nodes_{tag}.csv")]
+        print("running abm", tag, flush=True)
+        r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+
+    run("nodes.csv", "edges.csv", 2, "a")
+    lines = open(j("nodes_a.csv")).read().splitlines()
+    assert lines[0] == "#node,year"
+    rows = np.array([list(map(int, l.split(","))) for l in lines[1:]], np.int64)
+    sched = pg.growth_schedule(n0, 0.05, 2)
+    assert rows.shape[0] == n0 + sum(sched)
+    assert np.array_equal(rows[:n0, 0], ids) and np.array_equal(rows[:n0, 1], g.birth + 2000)
+    new_ids = rows[n0:, 0]
+    assert np.array_equal(new_ids, ids[-1] + 1 + np.arange(sum(sched)))
+    y0 = int(g.birth.max()) + 2000 + 1
+    assert np.array_equal(rows[n0:, 1], np.repeat([y0, y0 + 1], sched))
+    # zero years on the first run's outputs: the same bytes back
+    run("nodes_a.csv", "out_a.csv", 0, "b")
+    for f in ("out", "nodes"):
+        assert open(j(f"{f}_a.csv"), "rb").read() == open(j(f"{f}_b.csv"), "rb").read(), f
+    # growing on: the first run's graph is the prefix of the edgelist
+    run("nodes_a.csv", "out_a.csv", 1, "c")
+    a_txt = open(j("out_a.csv")).read()
+    c_txt = open(j("out_c.csv")).read()
+    assert c_txt.startswith(a_txt) and len(c_txt) > len(a_txt)
