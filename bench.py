@@ -546,11 +546,16 @@ def ensemble_bench(args) -> dict:
     n_new = sum(sum(sc) for _, sc in graphs)
     eprint(f"ensemble: {len(sims)} simulations, {n_seed} seed papers, {n_new} new papers "
            f"(seed graphs {time.perf_counter() - t0:.1f} s)")
+    host_gen_s = time.perf_counter() - t0
+    # the ensemble's seed graphs and log-fitness generated on the device (the
+    # same bits as the host graphs above, which the plain runs load)
     t1 = time.perf_counter()
     ens = pg.Ensemble(sims, years, m, rate)
-    for s, (g, _) in enumerate(graphs):
-        ens.load(s, g)
-    eprint(f"ensemble loaded ({time.perf_counter() - t1:.1f} s)")
+    t1b = time.perf_counter()
+    ens.generate(10.0)
+    gen_s = time.perf_counter() - t1b
+    eprint(f"ensemble created ({t1b - t1:.1f} s) and its seed graphs generated on the device "
+           f"({gen_s:.2f} s; the host generator took {host_gen_s:.1f} s)")
     tune = None
     if args.ens_shape:
         lanes, mode = (int(x) for x in args.ens_shape.split(","))
@@ -623,6 +628,7 @@ def ensemble_bench(args) -> dict:
                    "seed_papers": n_seed, "new_papers": n_new, "edges_added": edges,
                    "near_boundary_attempts": st.near_boundary, "hipgraph": True},
         "bfs_ms_per_run": bfs_ms, "eager_step_ms_per_run": step_ms,
+        "seed_generation_s": {"device": gen_s, "host": host_gen_s},
         "roofline": roofline("2-hop kernels per year launch (k_bfs_small, k_bfs_mid, "
                              "k_bfs_root)", ens_ach, "ensemble_256x10y_kernels.md", "ens_bfs"),
         "autotune": tune,

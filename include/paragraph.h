@@ -456,6 +456,14 @@ const char* pg_ensemble_error(const pg_ensemble* ens);
 int pg_ensemble_load(pg_ensemble* ens, int32_t sim, int64_t n, const int32_t* birth,
                      const float* logfit, int64_t n_edges, const int32_t* src,
                      const int32_t* dst);
+/* Instead of pg_ensemble_load: every simulation's seed graph and log-fitness
+ * generated on the device, bit for bit what pg_seed_graph(key, n0, year_lo,
+ * year_hi, mean_cites) and pg_seed_logfit(key, 0, final papers, mu, sigma)
+ * give for the simulation's Philox key (its pg_sim_params.seed).  seed_edges
+ * must bound each simulation's draws (PG_ECAP otherwise).  No reference
+ * counterpart (the reference reads its seed graph from files). */
+int pg_ensemble_generate(pg_ensemble* ens, int32_t year_lo, int32_t year_hi, double mean_cites,
+                         double mu, double sigma);
 /* All years for all simulations, from the seed state.  use_graph != 0:
  * captured once into a hipGraph, replayed on later runs. */
 int pg_ensemble_run(pg_ensemble* ens, int32_t use_graph, pg_ensemble_stats* stats);

@@ -198,6 +198,9 @@ def lib() -> ctypes.CDLL:
             "pg_ensemble_load": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int64, i32p, f32p,
                                                 ctypes.c_int64, i32p, i32p]),
             "pg_ensemble_run": (ctypes.c_int, [vp, ctypes.c_int32, P(EnsembleStats)]),
+            "pg_ensemble_generate": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32,
+                                                    ctypes.c_double, ctypes.c_double,
+                                                    ctypes.c_double]),
             "pg_ensemble_restore": (ctypes.c_int, [vp]),
             "pg_ensemble_step": (ctypes.c_int, [vp, P(EnsembleStats)]),
             "pg_ensemble_year_draws": (ctypes.c_int, [vp, ctypes.c_int32, i32p, i8p, i32p, i32p,
@@ -624,6 +627,14 @@ class Ensemble:
                                            _p(logfit, ctypes.c_float), src.shape[0],
                                            _p(src, ctypes.c_int32), _p(dst, ctypes.c_int32)),
                     "pg_ensemble_load")
+
+    def generate(self, mean_cites: float, year_lo: int = -20, year_hi: int = 0,
+                 mu: float = 0.0, sigma: float = 0.5):
+        """Every simulation's seed graph and log-fitness generated on the
+        device (instead of load): what seed_graph(sim seed, n0, mean_cites,
+        n_final) gives, bit for bit."""
+        self._check(lib().pg_ensemble_generate(self._h, year_lo, year_hi, float(mean_cites),
+                                               float(mu), float(sigma)), "pg_ensemble_generate")
 
     def run(self, use_graph: bool = True) -> EnsembleStats:
         st = EnsembleStats(ctypes.sizeof(EnsembleStats))

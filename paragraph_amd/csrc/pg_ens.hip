@@ -342,6 +342,119 @@ int pg_ensemble_load(pg_ensemble* ens, int32_t sim, int64_t n, const int32_t* bi
     return PG_OK;
 }
 
+int pg_ensemble_generate(pg_ensemble* ens, int32_t year_lo, int32_t year_hi, double mean_cites,
+                         double mu, double sigma) {
+    if (!ens) return PG_EINVAL;
+    if (ens->loaded_upto != 0)
+        return efail(ens, PG_ESTATE, "pg_ensemble_generate: simulations already loaded");
+    const int64_t span = (int64_t)year_hi - year_lo + 1;
+    if (span < 1 || span > 4096 || !(mean_cites >= 0.0) || !(sigma >= 0.0) || !std::isfinite(mu))
+        return efail(ens, PG_EINVAL, "pg_ensemble_generate: bad year range or law parameters");
+    const int S = ens->cfg.n_sims;
+    std::vector<GenSim> tab(S);
+    int64_t n_flat = 0, n_lf = 0;
+    for (int s = 0; s < S; ++s) {
+        tab[s] = GenSim{ens->R[s], n_flat, ens->sims[s].n0, n_lf, ens->EB[s], ens->sims[s].seed};
+        n_flat += ens->sims[s].n0;
+        n_lf += ens->nfinal[s];
+    }
+    pg_ctx* c = ens->ctx;
+    EHIP(hipSetDevice(ens->cfg.device));
+    hipStream_t st = c->stream;
+    GenSim* d_tab = nullptr;
+    int32_t *hist = nullptr, *cnt = nullptr, *zero = nullptr;
+    int64_t *per_year = nullptr, *off = nullptr, *scan_scr = nullptr, *sim_edges = nullptr;
+    uint32_t *keys = nullptr, *vals = nullptr, *k_tmp = nullptr, *v_tmp = nullptr, *k_sorted = nullptr;
+    int64_t* rscr = nullptr;
+    auto release_tmp = [&]() {
+        void* ps[] = {d_tab, hist, cnt, zero, per_year, off, scan_scr, sim_edges,
+                      keys, vals, k_tmp, v_tmp, k_sorted, rscr};
+        for (void* q : ps)
+            if (q) (void)hipFree(q);
+    };
+    auto bail = [&](int code, const std::string& msg) {
+        (void)hipStreamSynchronize(st);
+        release_tmp();
+        return efail(ens, code, "pg_ensemble_generate: " + msg);
+    };
+    hipError_t r = dalloc(&d_tab, S);
+    if (r == hipSuccess) r = dalloc(&hist, (int64_t)S * span);
+    if (r == hipSuccess) r = dalloc(&per_year, (int64_t)S * (span + 1));
+    if (r == hipSuccess) r = dalloc(&cnt, n_flat);
+    if (r == hipSuccess) r = dalloc(&off, n_flat + 1);
+    if (r == hipSuccess) r = dalloc(&scan_scr, scan_scratch_elems(n_flat));
+    if (r == hipSuccess) r = dalloc(&sim_edges, 2 * S);
+    if (r == hipSuccess) r = dalloc(&zero, 1);
+    if (r != hipSuccess) return bail(PG_ENOMEM, "device allocation failed");
+    r = hipMemcpy(d_tab, tab.data(), sizeof(GenSim) * S, hipMemcpyHostToDevice);
+    if (r == hipSuccess) r = hipMemset(hist, 0, sizeof(int32_t) * S * span);
+    if (r == hipSuccess) r = hipMemset(zero, 0, sizeof(int32_t));
+    if (r != hipSuccess) return bail(PG_EHIP, hipGetErrorString(r));
+    // birth-year histogram -> per-year prefix (host: S x span values)
+    launch_gen_hist(st, d_tab, S, n_flat, (int32_t)span, hist);
+    std::vector<int32_t> h((size_t)S * span);
+    r = hipMemcpyAsync(h.data(), hist, h.size() * 4, hipMemcpyDeviceToHost, st);
+    if (r == hipSuccess) r = hipStreamSynchronize(st);
+    if (r != hipSuccess) return bail(PG_EHIP, hipGetErrorString(r));
+    std::vector<int64_t> py((size_t)S * (span + 1));
+    for (int s = 0; s < S; ++s) {
+        py[(size_t)s * (span + 1)] = 0;
+        for (int64_t y = 0; y < span; ++y)
+            py[(size_t)s * (span + 1) + y + 1] = py[(size_t)s * (span + 1) + y] + h[(size_t)s * span + y];
+    }
+    r = hipMemcpy(per_year, py.data(), py.size() * 8, hipMemcpyHostToDevice);
+    if (r != hipSuccess) return bail(PG_EHIP, hipGetErrorString(r));
+    // counts, offsets, per-simulation edge totals (checked against seed_edges)
+    launch_gen_counts(st, d_tab, S, n_flat, (int32_t)span, year_lo, per_year, mean_cites,
+                      std::exp(-mean_cites), c->birth, cnt);
+    launch_excl_scan_i32(st, cnt, n_flat, off, scan_scr);
+    launch_gen_sim_edges(st, d_tab, S, off, sim_edges);
+    std::vector<int64_t> se(2 * (size_t)S);
+    r = hipGetLastError();
+    if (r == hipSuccess) r = hipMemcpyAsync(se.data(), sim_edges, se.size() * 8, hipMemcpyDeviceToHost, st);
+    if (r == hipSuccess) r = hipStreamSynchronize(st);
+    if (r != hipSuccess) return bail(PG_EHIP, hipGetErrorString(r));
+    int64_t E = 0;
+    for (int s = 0; s < S; ++s) {
+        if (se[2 * s + 1] > ens->sims[s].seed_edges)
+            return bail(PG_ECAP, "simulation " + std::to_string(s) + " draws " +
+                                     std::to_string(se[2 * s + 1]) + " seed citations, more than "
+                                     "its seed_edges");
+        E += se[2 * s + 1];
+    }
+    r = dalloc(&keys, E);
+    if (r == hipSuccess) r = dalloc(&vals, E);
+    if (r == hipSuccess) r = dalloc(&k_tmp, E);
+    if (r == hipSuccess) r = dalloc(&v_tmp, E);
+    if (r == hipSuccess) r = dalloc(&k_sorted, E);
+    if (r == hipSuccess) r = dalloc(&rscr, radix_scratch_elems(E));
+    if (r != hipSuccess) return bail(PG_ENOMEM, "device allocation for the transpose failed");
+    // forward rows, padding, backward CSR (stable sort of (target, citer)),
+    // in-degrees, log-fitness
+    launch_gen_rows(st, d_tab, S, n_flat, (int32_t)span, year_lo, per_year, c->birth, cnt, off,
+                    sim_edges, c->fwd_off, c->fwd_adj, keys, vals);
+    launch_gen_pad(st, d_tab, S, ens->TOT, sim_edges, c->birth, c->fwd_off);
+    if (E > 0)
+        launch_radix_sort_pairs(st, keys, vals, k_sorted, reinterpret_cast<uint32_t*>(c->bwd_adj[0]),
+                                k_tmp, v_tmp, E, key_bits(ens->TOT), rscr);
+    launch_bwd_offsets_sorted(st, 0, ens->TOT, zero, k_sorted, E, c->bwd_off[0], c->in_deg);
+    launch_gen_indeg(st, ens->TOT, c->bwd_off[0], c->in_deg);
+    launch_gen_logfit(st, d_tab, S, n_lf, mu, sigma, c->logfit);
+    r = hipGetLastError();
+    if (r == hipSuccess) r = hipStreamSynchronize(st);
+    if (r != hipSuccess) return bail(PG_EHIP, hipGetErrorString(r));
+    release_tmp();
+    ens->bwd_fill = E;
+    ens->loaded_upto = S;
+    c->loaded = true;
+    c->cur = 0;
+    int rc = pg_snapshot_save(c);
+    if (rc) return efail(ens, rc, std::string("pg_ensemble_generate: ") + c->err);
+    ens->ready = true;
+    ens->fresh = true;
+    return PG_OK;
+}
+
 int pg_ensemble_restore(pg_ensemble* ens) {
     if (!ens) return PG_EINVAL;
     if (!ens->ready) return efail(ens, PG_ESTATE, "pg_ensemble_restore: not all simulations loaded");

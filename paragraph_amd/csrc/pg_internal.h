@@ -284,6 +284,31 @@ void launch_sum_i32(hipStream_t s, const int32_t* p, int64_t n, unsigned long lo
 // *dst += *src (one thread).
 void launch_acc_i64(hipStream_t s, const int64_t* src, unsigned long long* dst);
 
+// ---- seed graphs generated on the device for an ensemble (k_gen.hip) ----
+// One row per simulation: its first global id, its first flat paper index
+// (papers of all simulations numbered in order), its seed papers, its first
+// flat log-fitness index (its final papers), its edge region and its key.
+struct GenSim {
+    int64_t base, pfx, n0, lfx, ebase;
+    uint64_t seed;
+};
+void launch_gen_hist(hipStream_t s, const GenSim* t, int n_sims, int64_t n_flat, int32_t span,
+                     int32_t* hist);
+void launch_gen_counts(hipStream_t s, const GenSim* t, int n_sims, int64_t n_flat, int32_t span,
+                       int32_t year_lo, const int64_t* per_year, double lambda, double e0,
+                       int32_t* birth, int32_t* cnt);
+void launch_gen_sim_edges(hipStream_t s, const GenSim* t, int n_sims, const int64_t* off,
+                          int64_t* sim_edges);
+void launch_gen_rows(hipStream_t s, const GenSim* t, int n_sims, int64_t n_flat, int32_t span,
+                     int32_t year_lo, const int64_t* per_year, const int32_t* birth,
+                     const int32_t* cnt, const int64_t* off, const int64_t* sim_edges,
+                     int32_t* fwd_off, int32_t* fwd_adj, uint32_t* keys, uint32_t* vals);
+void launch_gen_pad(hipStream_t s, const GenSim* t, int n_sims, int64_t tot,
+                    const int64_t* sim_edges, int32_t* birth, int32_t* fwd_off);
+void launch_gen_indeg(hipStream_t s, int64_t tot, const int32_t* bwd_off, int32_t* in_deg);
+void launch_gen_logfit(hipStream_t s, const GenSim* t, int n_sims, int64_t n_flat, double mu,
+                       double sigma, float* logfit);
+
 // ---- PA likelihood (k_lik.hip) ----
 constexpr int kLikMaxYears = 256;   // years per sequence (LDS per-year partials)
 constexpr int kLikYearCols = 5;     // per year: Z, A, log L, alpha terms, tau terms

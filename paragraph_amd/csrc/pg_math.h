@@ -165,6 +165,60 @@ PG_HD double det_log1p(double x) {
     return det_log(1.0 + x);
 }
 
+// ---- synthetic seed graphs (host generator and device generator) ----
+// Draws keyed by (paper, attempt, kSeedYear, tag) under the graph's seed.
+constexpr uint32_t kSeedYear = 0xFFFFFFFFu;
+
+// Raw birth-year offset in [0, span) of draw j.
+PG_HD int64_t seed_birth_offset(uint64_t seed, uint32_t j, int64_t span) {
+    const u32x4 r = philox(j, 0u, kSeedYear, TAG_SEED_BIRTH, seed);
+    int64_t y = (int64_t)__builtin_floor(u53(r.x, r.y) * (double)span);
+    return y >= span ? span - 1 : y;
+}
+
+// Poisson(lambda) by inversion of uniform u: the smallest k with F(k) > u;
+// e0 = exp(-lambda), computed once by the caller.
+PG_HD int32_t poisson_inv(double lambda, double e0, double u) {
+    double p = e0, F = p;
+    int32_t k = 0;
+    while (u >= F && k < 100000) {
+        ++k;
+        p *= lambda / k;
+        F += p;
+        if (p == 0.0) break;
+    }
+    return k;
+}
+
+PG_HD int32_t seed_count(uint64_t seed, uint32_t i, double lambda, double e0, int64_t older) {
+    const u32x4 r = philox(i, 0u, kSeedYear, TAG_SEED_COUNT, seed);
+    const int64_t k = poisson_inv(lambda, e0, u53(r.x, r.y));
+    return (int32_t)(k < older ? k : older);
+}
+
+// Attempt a of paper i's citation draws: uniform among its `older` papers.
+PG_HD int32_t seed_target(uint64_t seed, uint32_t i, uint32_t a, int64_t older) {
+    const u32x4 r = philox(i, a, kSeedYear, TAG_SEED_CITE, seed);
+    int64_t t = (int64_t)__builtin_floor(u53(r.x, r.y) * (double)older);
+    return (int32_t)(t >= older ? older - 1 : t);
+}
+
+// Log-fitness of paper g: mu + sigma * z, z standard normal by the polar
+// method (Marsaglia & Bray, 1964) on successive Philox draws.
+PG_HD float seed_logfit(uint64_t seed, uint32_t g, double mu, double sigma) {
+    double z = 0.0;
+    for (uint32_t a = 0;; ++a) {
+        const u32x4 r = philox(g, a, kSeedYear, TAG_FITNESS, seed);
+        const double v1 = 2.0 * u53(r.x, r.y) - 1.0;
+        const double v2 = 2.0 * u53(r.z, r.w) - 1.0;
+        const double s = v1 * v1 + v2 * v2;
+        if (s >= 1.0 || s == 0.0) continue;
+        z = v1 * __builtin_sqrt(-2.0 * det_log(s) / s);
+        break;
+    }
+    return (float)(mu + sigma * z);
+}
+
 // PA weight of one paper: exp(alpha*log(k+1) - age*inv_tau + logfit), FP32.
 PG_HD float pa_weight(int32_t k, int32_t age, float logfit, float alpha, float inv_tau) {
     float lk = det_logf((float)(k + 1));

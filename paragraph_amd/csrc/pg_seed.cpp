@@ -21,8 +21,6 @@ using namespace pg;
 
 namespace {
 
-constexpr uint32_t kSeedYear = 0xFFFFFFFFu;  // counter word 2 for seed-graph draws
-
 template <typename F>
 void parallel_for(int64_t n, F&& f) {
     unsigned hw = std::thread::hardware_concurrency();
@@ -38,19 +36,6 @@ void parallel_for(int64_t n, F&& f) {
     for (auto& x : th) x.join();
 }
 
-int32_t poisson(double lambda, double u) {
-    // inversion: smallest k with F(k) > u
-    double p = std::exp(-lambda), F = p;
-    int32_t k = 0;
-    while (u >= F && k < 100000) {
-        ++k;
-        p *= lambda / k;
-        F += p;
-        if (p == 0.0) break;
-    }
-    return k;
-}
-
 }  // namespace
 
 extern "C" int pg_seed_graph(uint64_t seed, int64_t n, int32_t year_lo, int32_t year_hi,
@@ -63,10 +48,7 @@ extern "C" int pg_seed_graph(uint64_t seed, int64_t n, int32_t year_lo, int32_t 
     // birth years drawn per draw index, then ids sorted by year (stable)
     std::vector<int32_t> raw(n);
     parallel_for(n, [&](int64_t j) {
-        u32x4 r = philox((uint32_t)j, 0u, kSeedYear, TAG_SEED_BIRTH, seed);
-        int64_t y = (int64_t)std::floor(u53(r.x, r.y) * (double)span);
-        if (y >= span) y = span - 1;
-        raw[j] = (int32_t)(year_lo + y);
+        raw[j] = (int32_t)(year_lo + seed_birth_offset(seed, (uint32_t)j, span));
     });
     std::vector<int64_t> per_year(span + 1, 0);
     for (int64_t j = 0; j < n; ++j) per_year[raw[j] - year_lo + 1]++;
@@ -75,11 +57,10 @@ extern "C" int pg_seed_graph(uint64_t seed, int64_t n, int32_t year_lo, int32_t 
         for (int64_t i = per_year[y]; i < per_year[y + 1]; ++i) birth[i] = (int32_t)(year_lo + y);
     // citation counts
     std::vector<int32_t> cnt(n);
+    const double e0 = std::exp(-mean_cites);
     parallel_for(n, [&](int64_t i) {
         const int64_t older = per_year[birth[i] - year_lo];
-        u32x4 r = philox((uint32_t)i, 0u, kSeedYear, TAG_SEED_COUNT, seed);
-        int64_t k = poisson(mean_cites, u53(r.x, r.y));
-        cnt[i] = (int32_t)std::min<int64_t>(k, older);
+        cnt[i] = seed_count(seed, (uint32_t)i, mean_cites, e0, older);
     });
     std::vector<int64_t> off(n + 1, 0);
     for (int64_t i = 0; i < n; ++i) off[i + 1] = off[i] + cnt[i];
@@ -92,9 +73,7 @@ extern "C" int pg_seed_graph(uint64_t seed, int64_t n, int32_t year_lo, int32_t 
         int32_t* out = dst + off[i];
         int32_t got = 0;
         for (uint32_t a = 0; got < c; ++a) {
-            u32x4 r = philox((uint32_t)i, a, kSeedYear, TAG_SEED_CITE, seed);
-            int64_t t = (int64_t)std::floor(u53(r.x, r.y) * (double)older);
-            if (t >= older) t = older - 1;
+            const int32_t t = seed_target(seed, (uint32_t)i, a, older);
             bool dup = false;
             for (int32_t q = 0; q < got && !dup; ++q) dup = (out[q] == (int32_t)t);
             if (!dup) out[got++] = (int32_t)t;
@@ -108,18 +87,7 @@ extern "C" int pg_seed_logfit(uint64_t seed, int64_t first, int64_t count, doubl
                               float* out) {
     if (first < 0 || count < 0 || (count > 0 && !out) || !(sigma >= 0.0)) return PG_EINVAL;
     parallel_for(count, [&](int64_t k) {
-        const uint32_t g = (uint32_t)(first + k);
-        double z = 0.0;
-        for (uint32_t a = 0;; ++a) {
-            u32x4 r = philox(g, a, kSeedYear, TAG_FITNESS, seed);
-            double v1 = 2.0 * u53(r.x, r.y) - 1.0;
-            double v2 = 2.0 * u53(r.z, r.w) - 1.0;
-            double s = v1 * v1 + v2 * v2;
-            if (s >= 1.0 || s == 0.0) continue;
-            z = v1 * std::sqrt(-2.0 * det_log(s) / s);
-            break;
-        }
-        out[k] = (float)(mu + sigma * z);
+        out[k] = seed_logfit(seed, (uint32_t)(first + k), mu, sigma);
     });
     return PG_OK;
 }

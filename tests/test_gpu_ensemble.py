@@ -136,3 +136,44 @@ def test_ensemble_errors():
     ens.run(use_graph=False)
     with pytest.raises(pg.PGError, match="ESTATE"):   # must restore first
         ens.run()
+
+
+def test_device_seed_generation_equals_host_graphs():
+    """pg_ensemble_generate builds every simulation's seed graph and
+    log-fitness on the device: the same graph as pg_seed_graph with the
+    simulation's key (rows, transposed rows, in-degrees, births), and runs
+    from it equal runs from the host graphs (which also needs the same
+    log-fitness bits: the weights of the last year are compared)."""
+    graphs = []
+    for sp in SIMS:
+        sched = pg.growth_schedule(sp["n0"], RATE, YEARS)
+        graphs.append((pg.seed_graph(sp["seed"], sp["n0"], 8.0, n_final=sp["n0"] + sum(sched)),
+                       sched))
+    host = make_ensemble(SIMS, graphs)
+    bound = [dict(sp, seed_edges=int(sp["n0"] * 8 + 10 * np.sqrt(sp["n0"] * 8) + 100))
+             for sp in SIMS]
+    gen = pg.Ensemble(bound, YEARS, M, RATE)
+    log("generating", len(SIMS), "seed graphs on the device")
+    gen.generate(8.0)
+    check_same(gen, [host.download(k) for k in range(len(SIMS))])
+    a, b = host.run(use_graph=False), gen.run(use_graph=False)
+    assert a.edges_added == b.edges_added and a.pa_attempts == b.pa_attempts
+    check_same(gen, [host.download(k) for k in range(len(SIMS))])
+    for k, sp in enumerate(SIMS):
+        g, sched = graphs[k]
+        n_last = g.n + sum(sched[:-1])
+        wa = host.year_draws(k, sched[-1], n_last)["w"]
+        wb = gen.year_draws(k, sched[-1], n_last)["w"]
+        assert np.array_equal(wa.view(np.uint32), wb.view(np.uint32)), f"sim {k}: weights"
+    host.close()
+    gen.close()
+    # refusals: a bound below the draws, and generation after loading
+    small = [dict(sp, seed_edges=10) for sp in SIMS]
+    e = pg.Ensemble(small, YEARS, M, RATE)
+    with pytest.raises(pg.PGError, match="ECAP"):
+        e.generate(8.0)
+    e.close()
+    e = make_ensemble(SIMS[:1], graphs[:1])
+    with pytest.raises(pg.PGError, match="ESTATE"):
+        e.generate(8.0)
+    e.close()
