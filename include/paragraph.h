@@ -43,7 +43,11 @@ enum {
     PG_EIO = 7       /* checkpoint file unreadable, unwritable or malformed */
 };
 
-enum { PG_MODEL_PA = 0, PG_MODEL_ER = 1 };
+enum {
+    PG_MODEL_PA = 0,   /* preferential attachment with ageing and fitness */
+    PG_MODEL_ER = 1,   /* Erdos-Renyi G(n,p) */
+    PG_MODEL_ABM = 2   /* the reference's scoring model (pg_abm_configure) */
+};
 
 typedef struct pg_ctx pg_ctx;
 
@@ -60,7 +64,7 @@ typedef struct {
 
 typedef struct {
     uint32_t size;
-    int32_t model;             /* PG_MODEL_PA or PG_MODEL_ER */
+    int32_t model;             /* PG_MODEL_PA, PG_MODEL_ER or PG_MODEL_ABM */
     int32_t year;              /* calendar year given to the new papers */
     int32_t year_index;        /* 0-based growth year (Philox counter word) */
     int64_t n_new;             /* papers added this year */
@@ -93,7 +97,8 @@ enum {
     PG_K_LIK = 6,              /* pg_pa_loglik evaluations (all passes) */
     PG_K_LIK_GRAD = 7,         /* pg_pa_loglik_grad evaluations (all passes) */
     PG_K_LIK_BATCH = 8,        /* pg_pa_loglik_grad_batch calls (all passes) */
-    PG_K_COUNT = 9
+    PG_K_ABM = 9,              /* scoring-model year: scores, same-year, selection, rows */
+    PG_K_COUNT = 10
 };
 
 /* Returns PG_ABI_VERSION. */
@@ -127,6 +132,61 @@ int pg_graph_load(pg_ctx* ctx, int64_t n, const int32_t* birth,
  * degree update (src/kernel.cu:1766-1815).  stats may be NULL.
  */
 int pg_year_step(pg_ctx* ctx, const pg_year_params* params, pg_year_stats* stats);
+
+/* ---- the reference's scoring model ----
+ *
+ * The agent-based model of the reference (src/abm.cu, year loop
+ * src/kernel.cu:1682-1845): each new paper gets an out-degree k from the
+ * bag, a neighbourhood share alpha and three score weights; it cites a
+ * uniformly drawn existing paper (the generator), one other paper of the
+ * same year if it is in the year's same-year subset (floor(D * same_year)
+ * papers), then ceil(rem * alpha) papers of the generator's 1-hop
+ * neighbourhood and the rest of rem = k - 1 - same-year - floor(k *
+ * fully_random) from its 2-hop neighbourhood, both by weighted sampling
+ * without replacement on score = w_pa * P + w_rec * R + w_fit * F (P: share
+ * of in-degree^3 + 1, R: recency table share of the paper's age, F: share
+ * of fitness^3 + 1, fitness a static power-law peak value), and fills the
+ * remainder with distinct uniform papers.  Replaces the reference's score
+ * preparation (src/abm.cu:112-208, :324-338), per-paper draws
+ * (src/abm.cu:241-290), Stage 1-4 kernels (src/kernel.cu:436-573 and
+ * src/abm.cu:376-1000) and generator assignment (src/abm.cu:567-574).
+ * Year steps run it with pg_year_params.model = PG_MODEL_ABM (n_new, year,
+ * year_index; the PA and ER fields are ignored).
+ */
+typedef struct {
+    uint32_t size;
+    const int32_t* bag;        /* out-degree bag (src/abm.cu:70-88), values in [0, 250] */
+    int32_t n_bag;
+    const double* rec_prob;    /* p(year difference d) for d in [0, n_rec) (src/abm.cu:91-110) */
+    int32_t n_rec;             /* <= 1024; differences outside have p = 0 */
+    double alpha;              /* 1-hop share in [0,1]; < 0: uniform per paper */
+    double w_pa, w_rec, w_fit; /* >= 0 with a positive sum, or any -1: uniform per paper */
+    double fully_random;       /* share of k reserved for uniform papers, [0,1] */
+    double same_year;          /* share of a year's papers citing the same year, [0,1] */
+} pg_abm_model;
+
+/* Configures the scoring model on ctx (after pg_graph_load or
+ * pg_checkpoint_load; kept across years).  Copies the bag and the table. */
+int pg_abm_configure(pg_ctx* ctx, const pg_abm_model* model);
+
+/* Per new paper of the last PG_MODEL_ABM year, info[6*i .. 6*i+5]:
+ * generator, out-degree, same-year target (-1 if none), 1-hop picks,
+ * 2-hop picks, uniform picks. */
+int pg_abm_year_info(pg_ctx* ctx, int32_t* info);
+
+/* Score components of the last PG_MODEL_ABM year's existing papers:
+ * comp[4*c .. 4*c+3] = P, R, F, 0 for c in [0, n_exist). */
+int pg_abm_scores(pg_ctx* ctx, float* comp);
+
+/* Static draws of papers [first, first+count) under the configured model
+ * (host computation): out-degree, alpha, weights[3*i..3*i+2], fitness peak
+ * value; any pointer may be NULL. */
+int pg_abm_paper_draws(pg_ctx* ctx, int64_t first, int64_t count, int32_t* out_degree,
+                       double* alpha, double* weights, int32_t* fitness);
+
+/* The 1000 cumulative fitness-peak weights f^-1.634 (f = 1..1000) the
+ * model draws fitness from (includes/abm.cuh:469-484). */
+void pg_abm_fitness_table(double* cdf);
 
 /* Current paper and edge counts. */
 int pg_graph_size(const pg_ctx* ctx, int64_t* n_papers, int64_t* n_edges);

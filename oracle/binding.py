@@ -38,6 +38,13 @@ class ErParams(ctypes.Structure):
                 ("n_exist", ctypes.c_int64), ("n_new", ctypes.c_int64), ("p", ctypes.c_double)]
 
 
+class AbmModelC(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("bag", _i32p), ("n_bag", ctypes.c_int32),
+                ("rec_prob", _f64p), ("n_rec", ctypes.c_int32), ("alpha", ctypes.c_double),
+                ("w_pa", ctypes.c_double), ("w_rec", ctypes.c_double), ("w_fit", ctypes.c_double),
+                ("frc", ctypes.c_double), ("syp", ctypes.c_double)]
+
+
 _lib = None
 
 
@@ -83,6 +90,18 @@ def lib() -> ctypes.CDLL:
                                                      _i64p, _i32p, _i8p, _i64p, _i32p, _i64p,
                                                      ctypes.c_int32, _i32p, _f32p, ctypes.c_double,
                                                      ctypes.c_double, _f64p, _f64p, _f64p, _f64p]),
+            "or_abm_fitness_cdf": (None, [_f64p]),
+            "or_abm_fitness": (ctypes.c_int32, [ctypes.c_uint64, ctypes.c_int64, _f64p]),
+            "or_abm_paper": (None, [P(AbmModelC), ctypes.c_int64, _i32p, _f64p, _f64p]),
+            "or_abm_scores": (ctypes.c_int, [ctypes.c_int64, _i32p, _i32p, ctypes.c_int32,
+                                             P(AbmModelC), _f32p]),
+            "or_abm_key": (ctypes.c_double, [P(AbmModelC), ctypes.c_uint32, ctypes.c_int32,
+                                             ctypes.c_uint32, _f32p, _f64p]),
+            "or_abm_same_year": (ctypes.c_int, [P(AbmModelC), ctypes.c_int64, ctypes.c_int64,
+                                                ctypes.c_uint32, _i32p]),
+            "or_abm_year": (ctypes.c_int64, [P(Csr), _i32p, _i32p, P(AbmModelC), ctypes.c_int32,
+                                             ctypes.c_int32, ctypes.c_int64, _i32p, _i32p,
+                                             ctypes.c_int64, _f32p, _i32p, ctypes.c_int]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -291,3 +310,84 @@ def vec(fn: str, x: np.ndarray) -> np.ndarray:
         y = np.empty_like(x)
         getattr(lib(), "or_vec_" + fn)(x.size, _p(x, ctypes.c_double), _p(y, ctypes.c_double))
     return y
+
+
+# ---- the reference's scoring model (abm_model.c) ----
+
+class AbmModel:
+    """Parameters of the scoring model; keeps the bag and recency arrays alive."""
+
+    def __init__(self, seed: int, bag, rec_prob, alpha: float = 0.5, w_pa: float = 1 / 3,
+                 w_rec: float = 1 / 3, w_fit: float = 1 / 3, frc: float = 0.05,
+                 syp: float = 0.12):
+        self.bag = np.ascontiguousarray(np.minimum(np.asarray(bag, np.int64), 250), np.int32)
+        self.rec = np.ascontiguousarray(rec_prob, np.float64)
+        self.seed, self.alpha, self.frc, self.syp = seed, alpha, frc, syp
+        self.w = (w_pa, w_rec, w_fit)
+        self.c = AbmModelC(seed, _p(self.bag, ctypes.c_int32), self.bag.size,
+                           _p(self.rec, ctypes.c_double), self.rec.size, alpha, w_pa, w_rec,
+                           w_fit, frc, syp)
+
+
+def abm_fitness_cdf() -> np.ndarray:
+    c = np.empty(1000, np.float64)
+    lib().or_abm_fitness_cdf(_p(c, ctypes.c_double))
+    return c
+
+
+def abm_fitness(seed: int, ids) -> np.ndarray:
+    c = abm_fitness_cdf()
+    return np.array([lib().or_abm_fitness(seed, int(i), _p(c, ctypes.c_double)) for i in ids],
+                    np.int32)
+
+
+def abm_paper(m: AbmModel, pid: int) -> tuple[int, float, np.ndarray]:
+    k = ctypes.c_int32()
+    a = ctypes.c_double()
+    w = np.empty(3, np.float64)
+    lib().or_abm_paper(ctypes.byref(m.c), pid, ctypes.byref(k), ctypes.byref(a),
+                       _p(w, ctypes.c_double))
+    return k.value, a.value, w
+
+
+def abm_scores(n: int, in_deg: np.ndarray, birth: np.ndarray, year: int,
+               m: AbmModel) -> np.ndarray:
+    comp = np.empty((n, 4), np.float32)
+    rc = lib().or_abm_scores(n, _p(np.ascontiguousarray(in_deg[:n], np.int32), ctypes.c_int32),
+                             _p(np.ascontiguousarray(birth[:n], np.int32), ctypes.c_int32),
+                             year, ctypes.byref(m.c), _p(comp, ctypes.c_float))
+    if rc:
+        raise RuntimeError(f"or_abm_scores failed: {rc}")
+    return comp
+
+
+def abm_key(m: AbmModel, v: int, c: int, yi: int, comp: np.ndarray, w: np.ndarray) -> float:
+    comp = np.ascontiguousarray(comp, np.float32)
+    w = np.ascontiguousarray(w, np.float64)
+    return lib().or_abm_key(ctypes.byref(m.c), v, c, yi, _p(comp, ctypes.c_float),
+                            _p(w, ctypes.c_double))
+
+
+def abm_same_year(m: AbmModel, n_exist: int, n_new: int, yi: int) -> np.ndarray:
+    t = np.empty(max(n_new, 1), np.int32)
+    if lib().or_abm_same_year(ctypes.byref(m.c), n_exist, n_new, yi, _p(t, ctypes.c_int32)):
+        raise RuntimeError("or_abm_same_year failed")
+    return t[:n_new].copy()
+
+
+def abm_year(g: Graph, m: AbmModel, year: int, year_index: int, n_new: int,
+             threads: int = 0) -> dict:
+    """One year of the scoring model: rows (off, tgt), per-paper info, scores."""
+    off = np.empty(n_new + 1, np.int32)
+    info = np.empty((max(n_new, 1), 6), np.int32)
+    comp = np.empty((g.n, 4), np.float32)
+    cap = int(n_new) * 252 + 1
+    tgt = np.empty(cap, np.int32)
+    c = g.csr()
+    e = lib().or_abm_year(ctypes.byref(c), _p(g.in_deg, ctypes.c_int32),
+                          _p(g.birth, ctypes.c_int32), ctypes.byref(m.c), year, year_index,
+                          n_new, _p(off, ctypes.c_int32), _p(tgt, ctypes.c_int32), cap,
+                          _p(comp, ctypes.c_float), _p(info, ctypes.c_int32), threads)
+    if e < 0:
+        raise RuntimeError(f"or_abm_year failed: {e}")
+    return {"off": off, "tgt": tgt[:e].copy(), "info": info[:n_new], "comp": comp}

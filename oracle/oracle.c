@@ -555,11 +555,13 @@ int or_csr_append(const or_csr* g, int64_t n_new, const int32_t* new_off,
     }
     /* backward: counting placement, citers in ascending id order */
     int64_t* add = (int64_t*)calloc((size_t)(n + n_new), sizeof(int64_t));
-    for (int64_t e = 0; e < new_off[n_new]; ++e) {
-        int32_t t = new_tgt[e];
-        if (t < 0 || t >= n) { free(add); return 2; }
-        add[t]++;
-    }
+    for (int64_t i = 0; i < n_new; ++i)
+        for (int64_t e = new_off[i]; e < new_off[i + 1]; ++e) {
+            /* existing papers, or (same-year citations) other new papers */
+            int32_t t = new_tgt[e];
+            if (t < 0 || t >= n + n_new || t == n + i) { free(add); return 2; }
+            add[t]++;
+        }
     bwd_off_out[0] = 0;
     for (int64_t v = 0; v < n + n_new; ++v) {
         int64_t old = v < n ? (int64_t)(g->bwd_off[v + 1] - g->bwd_off[v]) : 0;
@@ -567,9 +569,10 @@ int or_csr_append(const or_csr* g, int64_t n_new, const int32_t* new_off,
         in_deg_out[v] = (int32_t)(old + add[v]);
     }
     int64_t* cur = add;
-    for (int64_t v = 0; v < n; ++v) {
-        int64_t old = g->bwd_off[v + 1] - g->bwd_off[v];
-        memcpy(bwd_adj_out + bwd_off_out[v], g->bwd_adj + g->bwd_off[v], (size_t)old * sizeof(int32_t));
+    for (int64_t v = 0; v < n + n_new; ++v) {
+        int64_t old = v < n ? (int64_t)(g->bwd_off[v + 1] - g->bwd_off[v]) : 0;
+        if (old)
+            memcpy(bwd_adj_out + bwd_off_out[v], g->bwd_adj + g->bwd_off[v], (size_t)old * sizeof(int32_t));
         cur[v] = bwd_off_out[v] + old;
     }
     for (int64_t i = 0; i < n_new; ++i)

@@ -102,7 +102,8 @@ int64_t or_er_year(const or_er_params* p, int32_t* off_out, int32_t* tgt_out,
                    int64_t cap_out, int nthreads);
 
 /* CSR append at a year boundary.  New paper i (id n_old+i) cites
- * new_tgt[new_off[i] .. new_off[i+1]) (distinct, any order).  Output arrays
+ * new_tgt[new_off[i] .. new_off[i+1]) (distinct, any order; existing papers
+ * or other new papers of the year).  Output arrays
  * are sized for n_old+n_new rows and E_old+E_added entries; in_deg_out gets
  * the new backward row lengths. */
 int or_csr_append(const or_csr* g, int64_t n_new, const int32_t* new_off,
@@ -139,6 +140,44 @@ int or_pa_loglik_grad_w32(int32_t n_years, const int64_t* n_exist,
                           const int32_t* birth, const float* logfit, double alpha,
                           double tau, double* loglik, double* grad_alpha,
                           double* grad_tau, double* grad_phi);
+
+/* ---- the reference's scoring model (abm_model.c) ---- */
+typedef struct {
+    uint64_t seed;
+    const int32_t* bag;        /* out-degree bag values, each in [0, 250] */
+    int32_t n_bag;
+    const double* rec_prob;    /* probability of year difference d, d in [0, n_rec) */
+    int32_t n_rec;
+    double alpha;              /* neighbourhood share in [0,1]; < 0: per-paper uniform */
+    double w_pa, w_rec, w_fit; /* score weights; any of them -1: per-paper uniform */
+    double frc;                /* fully random share in [0,1] */
+    double syp;                /* same-year proportion in [0,1] */
+} or_abm_model;
+
+/* Cumulative fitness-peak probabilities of f = 1..1000 (1000 doubles). */
+void or_abm_fitness_cdf(double* cdf);
+/* Static fitness peak value of paper id, in [1,1000]. */
+int32_t or_abm_fitness(uint64_t seed, int64_t id, const double* cdf);
+/* Out-degree, neighbourhood share and normalised weights (w[3]) of paper id. */
+void or_abm_paper(const or_abm_model* m, int64_t id, int32_t* k, double* alpha, double* w);
+/* Score components of papers [0,n) in a year: comp[4c..4c+3] = P, R, F, 0.
+ * Returns 0, or 2 if an in-degree reaches 2^21. */
+int or_abm_scores(int64_t n, const int32_t* in_deg, const int32_t* birth, int32_t year,
+                  const or_abm_model* m, float* comp);
+/* Selection key (a float, as double) of candidate c for citing paper v. */
+double or_abm_key(const or_abm_model* m, uint32_t v, int32_t c, uint32_t yi, const float* comp,
+                  const double* w);
+/* Same-year citation target of each of the year's D papers, -1 for none. */
+int or_abm_same_year(const or_abm_model* m, int64_t N, int64_t D, uint32_t yi, int32_t* target);
+/* One year of the model on graph g (N = g->n papers): new paper l's row is
+ * tgt_out[off_out[l] .. off_out[l+1]) ascending; info[6l..6l+5] = generator,
+ * out-degree, same-year target (-1), F1 picks, F2 picks, uniform picks;
+ * comp_out (optional, [4N]) gets the score components.  Returns the edge
+ * count, -1 on failure or if cap_out is too small, -2 if N < 1. */
+int64_t or_abm_year(const or_csr* g, const int32_t* in_deg, const int32_t* birth,
+                    const or_abm_model* m, int32_t year, int32_t year_index, int64_t n_new,
+                    int32_t* off_out, int32_t* tgt_out, int64_t cap_out, float* comp_out,
+                    int32_t* info, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -20,6 +20,7 @@ ABM_PATH = os.path.join(_HERE, "abm")
 
 PG_MODEL_PA = 0
 PG_MODEL_ER = 1
+PG_MODEL_ABM = 2
 
 _ERRORS = {1: "EINVAL", 2: "ENOMEM", 3: "EHIP", 4: "ECAP", 5: "ESTATE", 6: "ENODEV", 7: "EIO"}
 
@@ -43,6 +44,14 @@ class YearParams(ctypes.Structure):
                 ("n_new", ctypes.c_int64), ("m", ctypes.c_int32), ("n_nb", ctypes.c_int32),
                 ("alpha", ctypes.c_float), ("inv_tau", ctypes.c_float),
                 ("er_p", ctypes.c_double)]
+
+
+class AbmModel(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_uint32), ("bag", ctypes.POINTER(ctypes.c_int32)),
+                ("n_bag", ctypes.c_int32), ("rec_prob", ctypes.POINTER(ctypes.c_double)),
+                ("n_rec", ctypes.c_int32), ("alpha", ctypes.c_double), ("w_pa", ctypes.c_double),
+                ("w_rec", ctypes.c_double), ("w_fit", ctypes.c_double),
+                ("fully_random", ctypes.c_double), ("same_year", ctypes.c_double)]
 
 
 class YearStats(ctypes.Structure):
@@ -125,7 +134,7 @@ class TuneReport(ctypes.Structure):
 
 
 KERNEL_GROUPS = ["weights_cdf", "pa_draw", "bfs", "nb_draw", "er", "csr_append", "lik",
-                 "lik_grad", "lik_batch"]
+                 "lik_grad", "lik_batch", "abm"]
 
 
 _lib = None
@@ -160,6 +169,12 @@ def lib() -> ctypes.CDLL:
             "pg_graph_load": (ctypes.c_int, [vp, ctypes.c_int64, i32p, f32p, ctypes.c_int64,
                                              ctypes.c_int64, i32p, i32p]),
             "pg_year_step": (ctypes.c_int, [vp, P(YearParams), P(YearStats)]),
+            "pg_abm_configure": (ctypes.c_int, [vp, P(AbmModel)]),
+            "pg_abm_year_info": (ctypes.c_int, [vp, i32p]),
+            "pg_abm_scores": (ctypes.c_int, [vp, f32p]),
+            "pg_abm_paper_draws": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int64, i32p, f64p,
+                                                  f64p, i32p]),
+            "pg_abm_fitness_table": (None, [f64p]),
             "pg_graph_size": (ctypes.c_int, [vp, i64p, i64p]),
             "pg_graph_download": (ctypes.c_int, [vp, i32p, i32p, i32p, i32p, i32p, i32p, f32p]),
             "pg_year_citations": (ctypes.c_int, [vp, i32p, i8p, i32p, i32p]),
@@ -243,6 +258,13 @@ def growth_schedule(n0: int, rate: float, years: int) -> list[int]:
         out.append(d)
         n += d
     return out
+
+
+def abm_fitness_table() -> np.ndarray:
+    """The 1000 cumulative fitness-peak weights the scoring model draws from."""
+    c = np.empty(1000, np.float64)
+    lib().pg_abm_fitness_table(_p(c, ctypes.c_double))
+    return c
 
 
 @dataclass
@@ -341,6 +363,50 @@ class Context:
         self._check(lib().pg_year_step(self._h, ctypes.byref(p), ctypes.byref(st)),
                     "pg_year_step")
         return st
+
+    # ---- the reference's scoring model ----
+    def abm_configure(self, bag, rec_prob, alpha: float = 0.5, w_pa: float = 1 / 3,
+                      w_rec: float = 1 / 3, w_fit: float = 1 / 3, fully_random: float = 0.05,
+                      same_year: float = 0.12):
+        """pg_abm_configure: out-degree bag, recency table p(d) for d = 0..,
+        neighbourhood share (< 0: per paper), weights (any -1: per paper)."""
+        bag = np.ascontiguousarray(bag, np.int32)
+        rec = np.ascontiguousarray(rec_prob, np.float64)
+        m = AbmModel(ctypes.sizeof(AbmModel), _p(bag, ctypes.c_int32), bag.shape[0],
+                     _p(rec, ctypes.c_double) if rec.size else None, rec.shape[0], alpha, w_pa,
+                     w_rec, w_fit, fully_random, same_year)
+        self._check(lib().pg_abm_configure(self._h, ctypes.byref(m)), "pg_abm_configure")
+
+    def year_abm(self, year: int, year_index: int, n_new: int) -> YearStats:
+        p = YearParams(ctypes.sizeof(YearParams), PG_MODEL_ABM, year, year_index, n_new, 0, 0,
+                       0.0, 0.0, 0.0)
+        st = YearStats(ctypes.sizeof(YearStats))
+        self._check(lib().pg_year_step(self._h, ctypes.byref(p), ctypes.byref(st)),
+                    "pg_year_step")
+        return st
+
+    def abm_year_info(self, n_new: int) -> np.ndarray:
+        """Per new paper of the last scoring-model year: generator, out-degree,
+        same-year target, 1-hop picks, 2-hop picks, uniform picks."""
+        info = np.empty((max(n_new, 1), 6), np.int32)
+        self._check(lib().pg_abm_year_info(self._h, _p(info, ctypes.c_int32)),
+                    "pg_abm_year_info")
+        return info[:n_new]
+
+    def abm_scores(self, n_exist: int) -> np.ndarray:
+        comp = np.empty((max(n_exist, 1), 4), np.float32)
+        self._check(lib().pg_abm_scores(self._h, _p(comp, ctypes.c_float)), "pg_abm_scores")
+        return comp[:n_exist]
+
+    def abm_paper_draws(self, first: int, count: int) -> dict:
+        k = np.empty(count, np.int32)
+        a = np.empty(count, np.float64)
+        w = np.empty((count, 3), np.float64)
+        f = np.empty(count, np.int32)
+        self._check(lib().pg_abm_paper_draws(self._h, first, count, _p(k, ctypes.c_int32),
+                                             _p(a, ctypes.c_double), _p(w, ctypes.c_double),
+                                             _p(f, ctypes.c_int32)), "pg_abm_paper_draws")
+        return {"out_degree": k, "alpha": a, "weights": w, "fitness": f}
 
     def size(self) -> tuple[int, int]:
         n, e = ctypes.c_int64(), ctypes.c_int64()

@@ -137,7 +137,8 @@ __global__ __launch_bounds__(kRT) void k_radix_scatter(const uint32_t* __restric
 }
 
 // New backward offsets and in-degrees from the year's targets sorted
-// ascending (keys[0, n_keys); padding keys >= n_total allowed): the new
+// ascending (keys[0, n_keys), existing or new papers; padding keys >=
+// n_total allowed): the new
 // citations of v are the run of v in keys, so excl[v] = lower_bound(v) and
 // the in-degree grows by lower_bound(v+1) - lower_bound(v).  No atomics:
 // hub targets receive thousands of citations a year.  Each thread takes
@@ -178,8 +179,10 @@ __global__ __launch_bounds__(256) void k_bwd_offsets_sorted(int64_t n_old, int64
             }
             q = l;
         }
+        // new papers start from their citations within their own year
+        // (same-year citations of the scoring model; none in PA and ER years)
         if (v < n_old) in_deg[v] += (int32_t)(q - p);
-        else if (v < n_total) in_deg[v] = 0;
+        else if (v < n_total) in_deg[v] = (int32_t)(q - p);
         p = q;
     }
 }

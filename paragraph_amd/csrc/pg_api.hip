@@ -54,6 +54,7 @@ void release(pg_ctx* c) {
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     seq_release(c);
+    abm_release(c);
 }
 
 int64_t scratch_need(int64_t max_papers, int64_t max_new, int64_t year_edges) {
@@ -62,6 +63,8 @@ int64_t scratch_need(int64_t max_papers, int64_t max_new, int64_t year_edges) {
     int64_t c = radix_scratch_elems(year_edges);
     return std::max(a, std::max(b, c));
 }
+
+}  // namespace
 
 // Grows the per-year edge buffers to hold `need` edges.  The new buffers are
 // allocated first and the old ones freed only after all of them succeeded,
@@ -112,8 +115,6 @@ int key_bits_for(int64_t n) {
 CsrView view(const pg_ctx* c) {
     return CsrView{c->fwd_off, c->fwd_adj, c->bwd_off[c->cur], c->bwd_adj[c->cur], c->n};
 }
-
-}  // namespace
 
 int ensure_bfs_stage(pg_ctx* ctx) {
     if (ctx->stage.buf) return PG_OK;
@@ -343,6 +344,7 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
     if (D > ctx->cfg.max_new_per_year) return fail(ctx, PG_ECAP, "pg_year_step: n_new > max_new_per_year");
     if (N + D > ctx->cfg.max_papers) return fail(ctx, PG_ECAP, "pg_year_step: graph would exceed max_papers");
     if (N + D > ctx->n_logfit) return fail(ctx, PG_ECAP, "pg_year_step: log-fitness not supplied for the new papers");
+    if (p->model == PG_MODEL_ABM) return abm_year_step(ctx, p, stats);
     if (p->model != PG_MODEL_PA && p->model != PG_MODEL_ER)
         return fail(ctx, PG_EINVAL, "pg_year_step: unknown model");
     if (p->model == PG_MODEL_PA) {

@@ -87,6 +87,33 @@ struct pg_seq_store {
     double* b_out = nullptr;         // [3 * bB]
 };
 
+// The reference's scoring model (pg_abm.hip): configuration, its device
+// copies, and the per-year buffers.
+struct pg_abm_store {
+    bool on = false;
+    int32_t n_bag = 0, n_rec = 0;
+    double alpha = 0, w_pa = 0, w_rec = 0, w_fit = 0, frc = 0, syp = 0;
+    std::vector<int32_t> h_bag;     // host copy (pg_abm_paper_draws)
+    std::vector<double> h_cdf;      // fitness table
+    int32_t* bag = nullptr;
+    double* rec = nullptr;          // [kAbmRecMax]
+    double* cdf = nullptr;          // [kAbmFitMax]
+    int16_t* fit = nullptr;         // [max_papers] fitness peak values
+    int64_t fit_n = 0;              // fit holds ids [0, fit_n)
+    float4* comp = nullptr;         // [max_papers] P, R, F of the year
+    uint64_t* parts = nullptr;      // [3 * kAbmMaxParts]
+    int32_t* cnt = nullptr;         // [kAbmRecMax + 1]: recency counts, then the 2^21 flag
+    float* rt = nullptr;            // [kAbmRecMax]
+    double* tot = nullptr;          // [2]
+    int32_t* syt = nullptr;         // [max_new] same-year targets
+    uint32_t* sy_keys = nullptr;    // [max_new]
+    uint32_t* sy_vals = nullptr;    // [max_new]
+    int32_t* sel = nullptr;         // [2 * max_new * kAbmSlots] selected ids per paper and level
+    int32_t* nsel = nullptr;        // [2 * max_new]
+    int32_t* info = nullptr;        // [6 * max_new]
+    int64_t last_n = -1, last_d = 0;  // the last year's existing and new papers
+};
+
 struct pg_ctx {
     pg_config cfg{};
     hipStream_t stream = nullptr;
@@ -167,6 +194,7 @@ struct pg_ctx {
     // citation-sequence store for the likelihood; recording appends each PA year
     pg_seq_store seq;
     bool seq_recording = false;
+    pg_abm_store abm;
 };
 
 // Host helpers shared by the ABI source files.
@@ -202,6 +230,13 @@ inline hipError_t dalloc(T** p, int64_t count) {
     return hipMalloc(reinterpret_cast<void**>(p), bytes);
 }
 
+// Per-year edge buffers (src_of, radix buffers, scratch) for `need` edges.
+int ensure_year_edges(pg_ctx* ctx, int64_t need);
+// Radix key bits covering ids below n.
+int key_bits_for(int64_t n);
+// The current CSR as kernels read it.
+pg::CsrView view(const pg_ctx* c);
+
 // Neighbourhood buffers sized by cfg.max_new_per_year and cfg.bfs_cap.
 int ensure_nb_buffers(pg_ctx* ctx);
 // Staging pool of the workgroup 2-hop path (allocated once per context).
@@ -210,6 +245,10 @@ int ensure_bfs_stage(pg_ctx* ctx);
 // Looks up the device address of pg_bfs_noalloc's static scratch on the
 // context's device (current), once per device (pg_noalloc.hip).
 int noalloc_setup(pg_ctx* ctx);
+
+// Scoring model (pg_abm.hip): frees its device memory; runs one year.
+void abm_release(pg_ctx* ctx);
+int abm_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats);
 
 // Sequence store (pg_lik.hip): frees its device memory; appends the PA year
 // that pg_year_step is about to run (in-degree snapshot) and has just drawn.

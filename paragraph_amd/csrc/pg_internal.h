@@ -168,6 +168,42 @@ struct CsrView {
     int64_t n;
 };
 
+// ---- the reference's scoring model (k_abm.hip) ----
+constexpr int kAbmRecMax = 1024;   // recency table entries (year differences 0..1023)
+constexpr int kAbmMaxParts = 2048; // blocks of the score sums
+constexpr int kAbmSlots = 256;     // per paper and level: selected ids (at most 249)
+
+// One year of the model: N existing papers, D new; the bag and the
+// per-paper parameter rules of the configured model.
+struct AbmYear {
+    uint64_t seed;
+    int64_t N, D;
+    uint32_t yi;
+    int32_t year;
+    const int32_t* bag;
+    int32_t n_bag;
+    double alpha, w_pa, w_rec, w_fit, frc;
+};
+
+void launch_abm_fit(hipStream_t s, int64_t lo, int64_t hi, uint64_t seed, const double* cdf,
+                    int16_t* fit);
+int abm_sum_parts(int64_t N);
+void launch_abm_scores(hipStream_t s, int64_t N, const int32_t* in_deg, const int32_t* birth,
+                       const int16_t* fit, int32_t year, int32_t n_rec, const double* prob,
+                       uint64_t* parts, int32_t* cnt, int32_t* flag, double* tot, float* rt,
+                       float4* comp);
+void launch_abm_same_keys(hipStream_t s, int64_t N, int64_t D, uint64_t seed, uint32_t yi,
+                          uint32_t* keys, uint32_t* vals, int32_t* syt);
+void launch_abm_same_mark(hipStream_t s, int64_t N, int64_t D, int64_t K, uint64_t seed,
+                          uint32_t yi, const uint32_t* order, int32_t* syt);
+void launch_abm_select(hipStream_t s, const AbmYear& y, const CsrView& g, const float4* comp,
+                       const int32_t* syt, int32_t* sel, int32_t* nsel);
+void launch_abm_count(hipStream_t s, const AbmYear& y, const int32_t* syt, const int32_t* nsel,
+                      int32_t* cnt, int32_t* info);
+void launch_abm_write(hipStream_t s, const AbmYear& y, const int32_t* syt, const int32_t* sel,
+                      const int32_t* info, const int64_t* new_off, int64_t fwd_base,
+                      int32_t* fwd_off, int32_t* fwd_adj, int32_t* src_of);
+
 // Staging pool of the workgroup 2-hop path: n_slots slots of cap_entries
 // int32 ids in buf; busy[k] is 0 while slot k is free (zeroed once at
 // allocation; each workgroup that takes a slot clears its flag when done).

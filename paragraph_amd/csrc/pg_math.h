@@ -20,7 +20,12 @@
 namespace pg {
 
 enum : uint32_t { TAG_PA = 0, TAG_NB = 1, TAG_ER = 2, TAG_SEED_BIRTH = 3,
-                  TAG_FITNESS = 4, TAG_SEED_COUNT = 5, TAG_SEED_CITE = 6 };
+                  TAG_FITNESS = 4, TAG_SEED_COUNT = 5, TAG_SEED_CITE = 6,
+                  // the reference's scoring model: fitness peak value, paper
+                  // properties, generator, same-year citation, selection
+                  // key, uniform fill
+                  TAG_ABM_FIT = 7, TAG_ABM_PAPER = 8, TAG_ABM_GEN = 9, TAG_ABM_SAME = 10,
+                  TAG_ABM_KEY = 11, TAG_ABM_UNI = 12 };
 
 struct u32x4 { uint32_t x, y, z, w; };
 
@@ -217,6 +222,113 @@ PG_HD float seed_logfit(uint64_t seed, uint32_t g, double mu, double sigma) {
         break;
     }
     return (float)(mu + sigma * z);
+}
+
+// ---- the reference's scoring model (agent-based model, src/abm.cu) ----
+// Per-paper properties are keyed by (paper id, word, kSeedYear, tag); the
+// year's draws by (citing paper, word, year index, tag).
+constexpr int kAbmMaxOut = 250;   // the reference's citation buffer (src/kernel.cu:1151)
+constexpr int kAbmFitMax = 1000;  // fitness peak values 1..1000 (includes/abm.cuh:543-544)
+
+// floor(u * n) for the 53-bit uniform of (a, b), clamped to n-1.
+PG_HD int64_t abm_pick(uint32_t a, uint32_t b, int64_t n) {
+    const int64_t x = (int64_t)__builtin_floor(u53(a, b) * (double)n);
+    return x >= n ? n - 1 : x;
+}
+
+// Static fitness peak value of paper id: the first f with cdf[f-1] > u*total
+// (cdf: the 1000 cumulative weights f^-1.634, includes/abm.cuh:469-484).
+PG_HD int32_t abm_fitness(uint64_t seed, uint32_t id, const double* cdf) {
+    const u32x4 r = philox(id, 0u, kSeedYear, TAG_ABM_FIT, seed);
+    const double t = u53(r.x, r.y) * cdf[kAbmFitMax - 1];
+    int lo = 0, hi = kAbmFitMax - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cdf[mid] > t) hi = mid; else lo = mid + 1;
+    }
+    return lo + 1;
+}
+
+struct AbmPaper {
+    int32_t k;        // out-degree drawn from the bag
+    double alpha;     // neighbourhood share
+    double w[3];      // normalised preferential, recency, fitness weights
+};
+
+// Out-degree, share and weights of paper id (PopulateOutDegreeArr,
+// PopulateAlphaArr, PopulateWeightArrs: src/abm.cu:241-290).  alpha < 0
+// draws a uniform share; any weight equal to -1 draws three uniform weights.
+PG_HD AbmPaper abm_paper(uint64_t seed, uint32_t id, const int32_t* bag, int32_t n_bag,
+                         double alpha, double w_pa, double w_rec, double w_fit) {
+    AbmPaper p;
+    const u32x4 r = philox(id, 0u, kSeedYear, TAG_ABM_PAPER, seed);
+    p.k = bag[abm_pick(r.x, r.y, n_bag)];
+    p.alpha = alpha >= 0.0 ? alpha : u53(r.z, r.w);
+    double a = w_pa, b = w_rec, c = w_fit;
+    if (w_pa == -1.0 || w_rec == -1.0 || w_fit == -1.0) {
+        const u32x4 q = philox(id, 1u, kSeedYear, TAG_ABM_PAPER, seed);
+        const u32x4 q2 = philox(id, 2u, kSeedYear, TAG_ABM_PAPER, seed);
+        a = u53(q.x, q.y);
+        b = u53(q.z, q.w);
+        c = u53(q2.x, q2.y);
+    }
+    const double s = a + b + c;
+    if (s > 0.0) {
+        p.w[0] = a / s;
+        p.w[1] = b / s;
+        p.w[2] = c / s;
+    } else {
+        p.w[0] = p.w[1] = p.w[2] = 1.0 / 3.0;
+    }
+    return p;
+}
+
+// Citation budgets of one new paper (src/kernel.cu:244-255, :383-386):
+// reserved = floor(frc*k), rem = k-1-sy-reserved, the first-level list
+// size ceil(rem*alpha) (both products rounded to float before floor/ceil,
+// as __float2int_rd/_ru take floats), the second-level cap max(rem, 0).
+struct AbmBudget {
+    int32_t rem, ins_raw, t1, t2;
+};
+
+PG_HD AbmBudget abm_budget(int32_t k, int32_t sy, double alpha, double frc) {
+    AbmBudget b;
+    const int32_t reserved = (int32_t)__builtin_floorf((float)(frc * (double)k));
+    b.rem = k - 1 - sy - reserved;
+    b.ins_raw = (int32_t)__builtin_ceilf((float)((double)b.rem * alpha));
+    b.t1 = b.ins_raw > 0 ? b.ins_raw : 0;
+    b.t2 = b.rem > 0 ? b.rem : 0;
+    return b;
+}
+
+// Picks and uniform fill of a paper from its list sizes c1 (first level,
+// at most t1) and c2 (second level, at most t2): p1, p2, n_uniform.
+PG_HD void abm_counts(int32_t k, int32_t sy, const AbmBudget& b, int32_t c1, int32_t c2,
+                      int64_t n_exist, int32_t* p1, int32_t* p2, int32_t* nuni) {
+    const int32_t inside = b.ins_raw > 0 ? c1 : b.ins_raw;
+    const int32_t rem3 = b.rem - inside;
+    const int32_t outside = rem3 > 0 ? (rem3 < c2 ? rem3 : c2) : rem3;
+    const int32_t fully = k - 1 - sy - inside - outside;
+    *p1 = inside > 0 ? inside : 0;
+    *p2 = outside > 0 ? outside : 0;
+    const int64_t avail = n_exist - 1 - *p1 - *p2;
+    *nuni = fully > 0 ? (int32_t)(fully < avail ? fully : avail) : 0;
+}
+
+// Selection key of candidate c for citing paper v (src/abm.cu:495-545):
+// log(u) / score in FP64, truncated to float, u a float in (0,1]; score 0
+// gives the reference's INT_MIN.  Returned as a 64-bit value whose ascending
+// order is (key descending, id ascending): the smallest values are kept.
+PG_HD uint64_t abm_key(uint64_t seed, uint32_t v, int32_t c, uint32_t yi, float P, float R, float F,
+                       const double* w) {
+    const u32x4 r = philox(v, (uint32_t)c, yi, TAG_ABM_KEY, seed);
+    const float u = (float)((r.x >> 8) + 1u) * 0x1p-24f;
+    const float lg = det_logf(u);
+    const double s = ((double)P * w[0] + (double)R * w[1]) + (double)F * w[2];
+    const float key = s != 0.0 ? (float)((double)lg / s) : -2147483648.0f;
+    const uint32_t kb = f2u(key);
+    const uint32_t asc = (kb & 0x80000000u) ? ~kb : (kb | 0x80000000u);
+    return ((uint64_t)(~asc) << 32) | (uint32_t)c;
 }
 
 // PA weight of one paper: exp(alpha*log(k+1) - age*inv_tau + logfit), FP32.
