@@ -12,12 +12,18 @@
 // max_id+1, ... (src/kernel.cu:1638, 1711-1719), and each year adds
 // ceil(N * growth_rate) papers (src/abm.cu:233-240).
 //
-// The growth model is the PA/ER model of this library (weights
-// (k+1)^a * exp(-age/tau) * fitness, FP64 CDF, 2-hop neighbourhood draws, or
-// ER G(n,p)); flags that drive the reference's score-mixing model
-// (--preferential-weight, --recency-weight, --fitness-weight,
-// --same-year-proportion, --fully-random-citations, --planted-nodes,
-// --recency-probabilities contents) are parsed and reported as not used.
+// Models: `--model pa` (the default) and `er-gnp` are this library's PA/ER
+// growth models (weights (k+1)^a * exp(-age/tau) * fitness, FP64 CDF, 2-hop
+// neighbourhood draws, or ER G(n,p)); with them the flags of the reference's
+// score-mixing model are parsed and reported as not used.  `--model
+// reference` runs the reference's scoring model itself (PG_MODEL_ABM): the
+// out-degree bag (src/abm.cu:70-88, values above 250 clamped), the recency
+// table (src/abm.cu:91-110), --alpha (negative: per-paper uniform,
+// src/abm.cu:241-254), --preferential/recency/fitness-weight (-1: per-paper
+// uniform, src/abm.cu:256-280), --fully-random-citations and
+// --same-year-proportion, with the reference's defaults (src/main.cu:61-85);
+// its auxiliary CSV carries the reference's column meanings.  Planted nodes
+// (--planted-nodes, src/abm.cu:44-68) are not modelled.
 // The process exits non-zero on any failure (the reference always exits 0,
 // src/main.cu:169).
 #include <algorithm>
@@ -89,7 +95,7 @@ void usage() {
                  "usage: abm --edgelist F --nodelist F --out-degree-bag F --recency-probabilities F\n"
                  "           --growth-rate R --num-cycles Y --same-year-proportion P\n"
                  "           --output-file F --auxiliary-information-file F --log-file F\n"
-                 "           [--model pa|er-gnp] [--er-probability p] [--er-edges-per-node k]\n"
+                 "           [--model pa|er-gnp|reference] [--er-probability p] [--er-edges-per-node k]\n"
                  "           [--pa-exponent a] [--age-decay tau] [--neighbourhood-fraction f | --alpha f]\n"
                  "           [--citations-per-paper m] [--seed s] [--bfs-cap c] [--fitness-sigma s]\n"
                  "           [--checkpoint-file F (written after every year)] [--resume-from F]\n"
@@ -131,6 +137,28 @@ std::string slurp(const std::string& path) {
     f.seekg(0);
     f.read(&s[0], (std::streamsize)s.size());
     return s;
+}
+
+// `key,value` lines of a small table (first field an integer, second a
+// number), `#` lines skipped, delimiter detected as for the other inputs.
+std::vector<std::pair<long long, double>> read_table(const std::string& path) {
+    const char delim = detect_delimiter(path);
+    std::ifstream f(path);
+    if (!f) throw std::runtime_error("cannot open " + path);
+    std::vector<std::pair<long long, double>> out;
+    std::string line;
+    while (std::getline(f, line)) {
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        if (line.empty() || line[0] == '#') continue;
+        const size_t c = line.find(delim);
+        if (c == std::string::npos) throw std::runtime_error("malformed line in " + path + ": " + line);
+        try {
+            out.emplace_back(std::stoll(line.substr(0, c)), std::stod(line.substr(c + 1)));
+        } catch (const std::exception&) {
+            throw std::runtime_error("malformed line in " + path + ": " + line);
+        }
+    }
+    return out;
 }
 
 // Host threads for parsing and formatting.
@@ -237,19 +265,34 @@ int main(int argc, char** argv) {
         Logger log;
         log.level = (int)a.num("--log-level", 1) - 1;
         log.f.open(a.str("--log-file"));
-        const std::string model = a.str("--model", "pa");
-        if (model != "pa" && model != "er-gnp")
-            throw std::runtime_error("--model must be pa or er-gnp");
+        std::string model = a.str("--model", "pa");
+        if (model == "abm") model = "reference";
+        if (model != "pa" && model != "er-gnp" && model != "reference")
+            throw std::runtime_error("--model must be pa, er-gnp or reference");
+        const bool ref_model = model == "reference";
         const double rate = a.num("--growth-rate", 0.0);
         const int years = (int)a.num("--num-cycles", 0);
         if (!(rate >= 0.0) || years < 0) throw std::runtime_error("bad --growth-rate/--num-cycles");
         for (const char* k : {"--preferential-weight", "--recency-weight", "--fitness-weight",
                               "--same-year-proportion", "--fully-random-citations", "--planted-nodes",
                               "--use-warp-bfs", "--use-batching", "--max-batch-size",
-                              "--use-multistage-kernel", "--num-processors"})
-            if (a.has(k))
+                              "--use-multistage-kernel", "--num-processors"}) {
+            const bool exec_knob = std::strcmp(k, "--use-warp-bfs") == 0 ||
+                                   std::strcmp(k, "--use-batching") == 0 ||
+                                   std::strcmp(k, "--max-batch-size") == 0 ||
+                                   std::strcmp(k, "--use-multistage-kernel") == 0 ||
+                                   std::strcmp(k, "--num-processors") == 0;
+            if (!a.has(k)) continue;
+            if (!ref_model)
                 std::fprintf(stderr, "abm: %s is a parameter of the reference's score-mixing "
                                      "model and is not used by this growth model\n", k);
+            else if (exec_knob)
+                std::fprintf(stderr, "abm: %s selects the reference's kernel schedule and has no "
+                                     "effect here\n", k);
+        }
+        if (ref_model && !a.str("--planted-nodes").empty())
+            std::fprintf(stderr, "abm: --planted-nodes is not modelled; planted papers get drawn "
+                                 "fitness like the others\n");
 
         const int T = io_threads();
         const auto t_read = Clock::now();
@@ -303,6 +346,35 @@ int main(int argc, char** argv) {
         const double read_s = std::chrono::duration<double>(Clock::now() - t_read).count();
         // ---- citations per new paper (out-degree bag `idx,outdeg`) ----
         int m = (int)a.num("--citations-per-paper", -1);
+        // ---- the reference's scoring model: bag and recency table ----
+        std::vector<int32_t> bag;
+        std::vector<double> rec;
+        int32_t max_out = 0;
+        if (ref_model) {
+            const std::string path = a.str("--out-degree-bag");
+            const std::string text = slurp(path);
+            int64_t clamped = 0;
+            for_each_pair(text.data(), text.data() + text.size(), detect_delimiter(path),
+                          [&](long long, long long k) {
+                              if (k < 0) throw std::runtime_error("negative out-degree in the bag");
+                              if (k > 250) ++clamped;
+                              bag.push_back((int32_t)std::min<long long>(k, 250));
+                          });
+            if (bag.empty()) throw std::runtime_error("empty out-degree bag");
+            if (clamped)
+                std::fprintf(stderr, "abm: %lld out-degree bag values above 250 (the reference's "
+                                     "citation buffer) clamped to 250\n", (long long)clamped);
+            for (int32_t k : bag) max_out = std::max(max_out, k);
+            for (const auto& [d, prob] : read_table(a.str("--recency-probabilities"))) {
+                if (d < 0 || d >= 1024)
+                    throw std::runtime_error("recency table: year differences must be in [0, 1023]");
+                if (!(prob >= 0.0)) throw std::runtime_error("recency table: negative probability");
+                if ((size_t)d >= rec.size()) rec.resize((size_t)d + 1, 0.0);
+                rec[(size_t)d] = prob;  // a repeated difference keeps its last value
+            }
+            if (a.num("--alpha", -1.0) > 1.0) throw std::runtime_error("--alpha must be at most 1");
+            m = 0;
+        }
         if (m < 0 && model == "pa") {
             std::vector<long long> bag;
             const std::string path = a.str("--out-degree-bag");
@@ -319,7 +391,8 @@ int main(int argc, char** argv) {
         }
         if (model == "er-gnp") m = 0;
         if (m > 64) throw std::runtime_error("at most 64 citations per paper");
-        if (a.has("--alpha") && !a.has("--neighbourhood-fraction") && !(a.num("--alpha", 0.0) >= 0.0))
+        if (!ref_model && a.has("--alpha") && !a.has("--neighbourhood-fraction") &&
+            !(a.num("--alpha", 0.0) >= 0.0))
             throw std::runtime_error("a negative --alpha (the reference's per-paper uniform "
                                      "neighbourhood share) is not supported by this growth "
                                      "model; pass a share in [0, 1]");
@@ -350,6 +423,8 @@ int main(int argc, char** argv) {
         int64_t e_cap = (int64_t)src.size() + (n_final - n0) * (int64_t)std::max(m, 1) + 16;
         if (model == "er-gnp")
             e_cap = (int64_t)src.size() + (int64_t)(1.2 * p_er * (double)n_final * (double)(n_final - n0)) + 1000000;
+        if (ref_model)  // at most max(k, 2 + floor(k)) edges per paper (generator and same-year included)
+            e_cap = (int64_t)src.size() + (n_final - n0) * (int64_t)(max_out + 2) + 16;
         std::vector<float> logfit(n_final);
         if (pg_seed_logfit(seed, 0, n_final, 0.0, a.num("--fitness-sigma", 0.5), logfit.data()))
             throw std::runtime_error("fitness draw failed");
@@ -385,13 +460,28 @@ int main(int argc, char** argv) {
             y_start = done;
             log.info("resumed after " + std::to_string(done) + " years");
         }
+        if (ref_model) {
+            pg_abm_model am{};
+            am.size = sizeof(am);
+            am.bag = bag.data();
+            am.n_bag = (int32_t)bag.size();
+            am.rec_prob = rec.empty() ? nullptr : rec.data();
+            am.n_rec = (int32_t)rec.size();
+            am.alpha = a.num("--alpha", -1.0);
+            am.w_pa = a.num("--preferential-weight", -1.0);
+            am.w_rec = a.num("--recency-weight", -1.0);
+            am.w_fit = a.num("--fitness-weight", -1.0);
+            am.fully_random = a.num("--fully-random-citations", 0.05);
+            am.same_year = a.num("--same-year-proportion", 0.0);
+            check(pg_abm_configure(ctx, &am), ctx, "pg_abm_configure");
+        }
         // ---- year loop ----
         const auto t_loop = Clock::now();
         int64_t total_edges = 0;
         for (int y = y_start; y < years; ++y) {
             pg_year_params p{};
             p.size = sizeof(p);
-            p.model = model == "pa" ? PG_MODEL_PA : PG_MODEL_ER;
+            p.model = ref_model ? PG_MODEL_ABM : model == "pa" ? PG_MODEL_PA : PG_MODEL_ER;
             p.year = max_year + 1 + y;
             p.year_index = y;
             p.n_new = sched[y];
@@ -405,6 +495,11 @@ int main(int argc, char** argv) {
             int64_t nb_before = 0, e_unused = 0;
             pg_graph_size(ctx, &nb_before, &e_unused);
             check(pg_year_step(ctx, &p, &st), ctx, "pg_year_step");
+            if (ref_model && sched[y] > 0) {
+                std::vector<int32_t> info((size_t)sched[y] * 6);
+                check(pg_abm_year_info(ctx, info.data()), ctx, "pg_abm_year_info");
+                for (int64_t i = 0; i < sched[y]; ++i) roots[nb_before - n0 + i] = info[(size_t)i * 6];
+            }
             if (model == "pa" && m > 0) {
                 std::vector<int32_t> cit((size_t)sched[y] * m);
                 check(pg_year_citations(ctx, cit.data(), nullptr, nullptr, nullptr), ctx,
@@ -433,6 +528,21 @@ int main(int argc, char** argv) {
         std::vector<int32_t> fo(nn + 1), fa(ne), bo(nn + 1), indeg(nn), by(nn);
         check(pg_graph_download(ctx, fo.data(), fa.data(), bo.data(), nullptr, indeg.data(),
                                 by.data(), nullptr), ctx, "pg_graph_download");
+        // the scoring model's static draws of every paper (aux columns)
+        std::vector<int32_t> r_k, r_fit;
+        std::vector<double> r_alpha, r_w;
+        if (ref_model) {
+            r_k.resize(nn);
+            r_fit.resize(nn);
+            r_alpha.resize(nn);
+            r_w.resize((size_t)nn * 3);
+            parallel_for(T, [&](int t) {
+                const int64_t b = nn / T * t, e = t + 1 == T ? nn : nn / T * (t + 1);
+                check(pg_abm_paper_draws(ctx, b, e - b, r_k.data() + b, r_alpha.data() + b,
+                                         r_w.data() + 3 * b, r_fit.data() + b), ctx,
+                      "pg_abm_paper_draws");
+            });
+        }
         pg_ctx_destroy(ctx);
         long long next_id = orig[n0 - 1] + 1;
         orig.resize(nn);
@@ -477,7 +587,18 @@ int main(int argc, char** argv) {
                 for (int64_t v = b; v < e; ++v) {
                     const int outd = fo[v + 1] - fo[v];
                     int len;
-                    if (v < n0) {
+                    if (ref_model && v < n0) {
+                        // the reference's seed rows: static fitness (lag 0,
+                        // peak duration 1000, src/abm.cu:124-150)
+                        len = std::snprintf(row, sizeof(row), "%lld,seed,%d,-1,-1,-1,0,%d,1000,-1,%d,%d,-1,-1,no_generators\n",
+                                            orig[v], by[v], r_fit[v], indeg[v], outd);
+                    } else if (ref_model) {
+                        const int32_t r = roots[v - n0];
+                        const long long ro = r >= 0 ? orig[r] : -1;
+                        len = std::snprintf(row, sizeof(row), "%lld,agent,%d,%.10g,%.10g,%.10g,0,%d,1000,%.10g,%d,%d,%d,-1,%lld\n",
+                                            orig[v], by[v], r_w[3 * v], r_w[3 * v + 1], r_w[3 * v + 2],
+                                            r_fit[v], r_alpha[v], indeg[v], outd, r_k[v], ro);
+                    } else if (v < n0) {
                         len = std::snprintf(row, sizeof(row), "%lld,seed,%d,-1,-1,-1,0,0,0,-1,%d,%d,-1,-1,no_generators\n",
                                             orig[v], by[v], indeg[v], outd);
                     } else {

@@ -68,3 +68,29 @@ def test_variable_out_degree_bag_is_rejected(tmp_path):
     open(os.path.join(d, "bag.csv"), "w").write("0,3\n1,5\n")
     r = run(base_args(d))
     assert r.returncode == 1 and "citations-per-paper" in r.stderr
+
+
+def test_reference_model_accepts_variable_bag(tmp_path):
+    """--model reference takes the reference's multi-value bag and its
+    defaults (src/main.cu:61-85); only the device is missing on the CPU."""
+    d = str(tmp_path)
+    write_inputs(d)
+    open(os.path.join(d, "bag.csv"), "w").write("0,3\n1,5\n2,300\n")
+    open(os.path.join(d, "rec.csv"), "w").write("#d,p\n1,0.5\n2,0.25\n")
+    r = run(base_args(d) + ["--model", "reference", "--use-warp-bfs", "false"])
+    assert "clamped to 250" in r.stderr and "has no effect here" in r.stderr
+    if os.path.exists("/dev/kfd"):
+        assert r.returncode == 0, r.stderr
+    else:
+        assert r.returncode == 1 and "device" in r.stderr, r.stderr
+
+
+def test_reference_model_rejects_bad_tables(tmp_path):
+    d = str(tmp_path)
+    write_inputs(d)
+    open(os.path.join(d, "rec.csv"), "w").write("2000,0.5\n")
+    r = run(base_args(d) + ["--model", "reference"])
+    assert r.returncode == 1 and "recency table" in r.stderr
+    open(os.path.join(d, "rec.csv"), "w").write("1,0.5\n")
+    r = run(base_args(d) + ["--model", "reference", "--alpha", "1.5"])
+    assert r.returncode == 1 and "--alpha" in r.stderr

@@ -166,3 +166,82 @@ def test_abm_nodelist_output_chains_runs(tmp_path):
     a_txt = open(j("out_a.csv")).read()
     c_txt = open(j("out_c.csv")).read()
     assert c_txt.startswith(a_txt) and len(c_txt) > len(a_txt)
+
+
+def test_abm_reference_model_matches_oracle(tmp_path):
+    """`--model reference` with the flags of drivers/ablation_parallel_gpu_10y.sh:
+    a multi-value out-degree bag (values above 250 clamped), a recency table,
+    equal weights, alpha 0.5, 5% fully random, 12% same-year: the edgelist
+    equals the oracle's replay of the scoring model, and the auxiliary CSV
+    carries each paper's out-degree, generator, weights, share and fitness."""
+    d = str(tmp_path)
+    n0, years, rate, seed = 4000, 3, 0.04, 9
+    g = pg.seed_graph(8, n0, 6.0)
+    ids = np.arange(n0) * 2 + 50
+    with open(os.path.join(d, "nodes.csv"), "w") as f:
+        f.write("#node,year\n")
+        for i in range(n0):
+            f.write(f"{ids[i]},{g.birth[i] + 1990}\n")
+    with open(os.path.join(d, "edges.csv"), "w") as f:
+        for s, t in zip(g.src, g.dst):
+            f.write(f"{ids[s]},{ids[t]}\n")
+    bag = [5, 6, 8, 12, 20, 33, 47, 90, 251, 400]
+    with open(os.path.join(d, "bag.csv"), "w") as f:
+        f.write("#idx,outdeg\n")
+        for i, k in enumerate(bag):
+            f.write(f"{i},{k}\n")
+    rec = {1: 0.3, 2: 0.25, 3: 0.15, 5: 0.1, 10: 0.05, 20: 0.02}
+    with open(os.path.join(d, "rec.csv"), "w") as f:
+        f.write("#year_diff,probability\n")
+        for k, v in rec.items():
+            f.write(f"{k},{v}\n")
+    j = lambda f: os.path.join(d, f)
+    args = [pg.ABM_PATH, "--edgelist", j("edges.csv"), "--nodelist", j("nodes.csv"),
+            "--out-degree-bag", j("bag.csv"), "--recency-probabilities", j("rec.csv"),
+            "--alpha", "0.5", "--preferential-weight", "0.33", "--recency-weight", "0.33",
+            "--fitness-weight", "0.33", "--growth-rate", str(rate), "--fully-random-citations",
+            "0.05", "--num-cycles", str(years), "--same-year-proportion", "0.12",
+            "--output-file", j("out.csv"), "--auxiliary-information-file", j("aux.csv"),
+            "--log-file", j("log.txt"), "--num-processors", "16", "--log-level", "1",
+            "--use-warp-bfs", "true", "--use-batching", "true", "--max-batch-size", "20000",
+            "--use-multistage-kernel", "true", "--model", "reference", "--seed", str(seed)]
+    print("running abm --model reference", flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "2 out-degree bag values above 250" in r.stderr
+    lines = open(j("out.csv")).read().splitlines()
+    e = np.array([list(map(int, l.split(","))) for l in lines[1:]], np.int64)
+    birth = (g.birth + 1990).astype(np.int32)
+    sched = pg.growth_schedule(n0, rate, years)
+    lf = np.zeros(n0 + sum(sched), np.float32)
+    G = ob.Graph.from_edges(n0, g.src, g.dst, birth, lf)
+    table = np.zeros(21)
+    for k, v in rec.items():
+        table[k] = v
+    m = ob.AbmModel(seed, bag, table, alpha=0.5, w_pa=0.33, w_rec=0.33, w_fit=0.33, frc=0.05,
+                    syp=0.12)
+    infos = []
+    for y, D in enumerate(sched):
+        yr = int(birth.max()) + 1 + y
+        res = ob.abm_year(G, m, yr, y, D)
+        infos.append(res["info"])
+        G = ob.csr_append(G, res["off"], res["tgt"], lf, yr)
+    orig = np.concatenate([ids, ids[-1] + 1 + np.arange(G.n - n0)])
+    src = np.repeat(np.arange(G.n), np.diff(G.fwd_off))
+    ref = np.stack([orig[src], orig[G.fwd_adj]], 1)
+    assert np.array_equal(e, ref)
+    info = np.concatenate(infos)
+    aux = [l.split(",") for l in open(j("aux.csv")).read().splitlines()[1:]]
+    assert len(aux) == G.n
+    fit = ob.abm_fitness(seed, range(G.n))
+    for v in range(G.n):
+        row = aux[v]
+        assert int(row[7]) == fit[v] and row[6] == "0" and row[8] == "1000"
+        assert int(row[10]) == G.in_deg[v] and int(row[11]) == G.fwd_off[v + 1] - G.fwd_off[v]
+        if v < n0:
+            assert row[1] == "seed" and row[14] == "no_generators"
+        else:
+            gen, k = info[v - n0, 0], info[v - n0, 1]
+            assert row[1] == "agent" and int(row[12]) == k and int(row[14]) == orig[gen]
+            assert [float(x) for x in row[3:6]] == [float(f"{1 / 3:.10g}")] * 3
+            assert float(row[9]) == 0.5
