@@ -38,6 +38,7 @@ GROUP_KERNELS = {
                     "k_likb_papers<true, 16>", "k_likb_final<16>"],
     "er": ["k_er_count", "k_er_write"],
     "ens_bfs": ["k_bfs_small", "k_bfs_mid", "k_bfs_root"],
+    "abm_select": ["k_abm_select"],
 }
 
 
@@ -264,6 +265,119 @@ def er_bench(args) -> dict:
                                          f"(no CSR append)"}
         if ce != edges:
             raise RuntimeError(f"ER oracle {ce} edges, GPU {edges}")
+    return res
+
+
+def ref_model_bench(args) -> dict:
+    """The reference's scoring model (PG_MODEL_ABM, `abm --model reference`):
+    the headline workload's 2,000,000-paper seed graph grown 10 years at 3%
+    per year with the inputs of workloads.reference_model_inputs (out-degree
+    bag with mean about 19, recency table, alpha 0.5, equal weights, 5% fully
+    random, 12% same-year), Philox seed 1.  Timed: the 10 device year steps
+    (scores, same-year subset, 1-hop/2-hop selection, rows, CSR append), the
+    seed restored from a device snapshot between steps outside the timed
+    region.  Year 0 is compared bit for bit with the oracle (scores, per-paper
+    draws and counts, rows), and the oracle times the workload on the host."""
+    from oracle import binding as ob
+    from paragraph_amd.workloads import reference_model_inputs
+    n0, years, rate, seed = args.n0, args.years, 0.03, 1
+    sched = pg.growth_schedule(n0, rate, years)
+    n_final = n0 + sum(sched)
+    g = pg.seed_graph(seed, n0, 10.0, n_final=n_final)
+    inp = reference_model_inputs()
+    e_cap = int(g.src.size + sum(sched) * (int(inp["bag"].max()) + 2) + 16)
+    ctx = pg.Context(n_final, e_cap, max(sched), max_cites=1, bfs_cap=16, seed=seed)
+    ctx.load(g)
+    cfg = dict(alpha=inp["alpha"], w_pa=inp["w_pa"], w_rec=inp["w_rec"], w_fit=inp["w_fit"],
+               fully_random=inp["fully_random"], same_year=inp["same_year"])
+    ctx.abm_configure(inp["bag"], inp["rec"], **cfg)
+    ctx.snapshot_save()
+    year0 = 1   # calendar year of the first new papers (seed births are -20..0)
+
+    def run():
+        e, cand = 0, [0, 0]
+        for yi, d in enumerate(sched):
+            e += ctx.year_abm(year0 + yi, yi, d).edges_added
+            c1, c2 = ctx.abm_year_counts()
+            cand[0] += c1
+            cand[1] += c2
+        return e, cand
+
+    run()                      # warm-up
+    ctx.snapshot_restore()
+    # year 0 against the oracle, bit for bit
+    G0 = ob.Graph.from_download(ctx.download(), g.logfit)
+    m = ob.AbmModel(seed, inp["bag"], inp["rec"], alpha=inp["alpha"], w_pa=inp["w_pa"],
+                    w_rec=inp["w_rec"], w_fit=inp["w_fit"], frc=inp["fully_random"],
+                    syp=inp["same_year"])
+    cores = len(os.sched_getaffinity(0))
+    ctx.year_abm(year0, 0, sched[0])
+    ref = ob.abm_year(G0, m, year0, 0, sched[0], threads=cores)
+    after = ctx.download()
+    fo = after["fwd_off"]
+    ok = (np.array_equal(ctx.abm_year_info(sched[0]), ref["info"]) and
+          np.array_equal(ctx.abm_scores(n0).view(np.uint32), ref["comp"].view(np.uint32)) and
+          np.array_equal(fo[n0:n0 + sched[0] + 1] - fo[n0], ref["off"]) and
+          np.array_equal(after["fwd_adj"][fo[n0]:fo[n0 + sched[0]]], ref["tgt"]))
+    if not ok:
+        raise RuntimeError("scoring-model year 0 differs from the oracle")
+    ctx.snapshot_restore()
+    ctx.profile_reset()
+    ctx.profile(True)
+    walls, edges, cand = [], 0, [0, 0]
+    for _ in range(max(args.steps, 1)):
+        t0 = time.perf_counter()
+        edges, cand = run()
+        walls.append(time.perf_counter() - t0)
+        ctx.snapshot_restore()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    ctx.close()
+    wall = float(np.median(walls))
+    steps = len(walls)
+    groups = ["abm_scores", "abm_select", "abm_rows", "csr_append"]
+    dom = max(groups, key=lambda k: prof[k]["ms"])
+    dv = prof[dom]
+    achieved = (dv["bytes"] / dv["launches"]) / (dv["ms"] / dv["launches"] * 1e-3) / 1e9
+    eprint(f"reference model: {edges} edges in {wall * 1e3:.2f} ms ({edges / wall:.3e} edges/s); "
+           + ", ".join(f"{k} {prof[k]['ms'] / steps:.2f} ms" for k in groups) +
+           f" per run; candidates keyed {cand[0]} (1-hop) + {cand[1]} (2-hop); year 0 equals "
+           f"the oracle")
+    res = {"metric": "citation edges/s (reference scoring model, 10y, 2M seed)",
+           "value": edges / wall, "unit": "citation edges/s", "ms_per_step": wall * 1e3,
+           "edges_per_step": edges, "year0_equals_oracle": ok,
+           "candidates_per_step": {"one_hop": cand[0], "two_hop": cand[1]},
+           "config": {"workload": f"reference scoring model {years}y at 3%/y from a {n0}-paper "
+                                  f"seed, bag mean {float(inp['bag'].mean()):.1f}, alpha 0.5, "
+                                  f"weights 0.33, 5% fully random, 12% same-year"},
+           "kernel_ms_per_step": {k: v["ms"] / steps for k, v in prof.items() if v["launches"]},
+           "roofline": dict(roofline(dom, achieved, "reference_10y_2M_kernels.md", dom),
+                            bound_note="selection: 4 B per candidate entry plus its 16 B score "
+                                       "record (and 16 B row offsets per 1-hop entry); each "
+                                       "entry also costs a Philox draw, a float log and an "
+                                       "FP64 divide")}
+    if not args.no_cpu_baseline:
+        G = G0
+        ce = 0
+        t0 = time.perf_counter()
+        for yi, d in enumerate(sched):
+            r = ob.abm_year(G, m, year0 + yi, yi, d, threads=cores)
+            G = ob.csr_append(G, r["off"], r["tgt"], g.logfit, year0 + yi)
+            ce += r["tgt"].size
+        dt = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        r1 = ob.abm_year(G0, m, year0, 0, sched[0], threads=1)
+        dt1 = time.perf_counter() - t1
+        res["cpu_baseline"] = {
+            "value": ce / dt, "unit": "citation edges/s", "cores": cores, "kind": "port",
+            "sample": f"the full workload ({years} years, {ce} edges, selection + CSR append) on "
+                      f"the C oracle with {cores} OpenMP threads: {dt:.2f} s",
+            "host_cpu": host_cpu(),
+            "one_thread": {"value": r1["tgt"].size / dt1, "unit": "citation edges/s", "cores": 1,
+                           "sample": f"year 0 ({r1['tgt'].size} edges, no append) on one thread: "
+                                     f"{dt1:.2f} s"}}
+        if ce != edges:
+            raise RuntimeError(f"scoring model: oracle {ce} edges, GPU {edges}")
     return res
 
 
@@ -672,6 +786,9 @@ def main():
     ap.add_argument("--no-ens", action="store_true", help="skip the ensemble part")
     ap.add_argument("--no-er", action="store_true", help="skip the ER part")
     ap.add_argument("--er-only", action="store_true", help="only the ER part")
+    ap.add_argument("--no-ref", action="store_true", help="skip the reference scoring model part")
+    ap.add_argument("--ref-only", action="store_true",
+                    help="only the reference scoring model part")
     ap.add_argument("--ens-only", action="store_true",
                     help="only the ensemble part (its JSON object on stdout)")
     ap.add_argument("--ens-sims", type=int, default=256)
@@ -701,6 +818,9 @@ def main():
         return
     if args.er_only:
         print(json.dumps(er_bench(args)), flush=True)
+        return
+    if args.ref_only:
+        print(json.dumps(ref_model_bench(args)), flush=True)
         return
     if args.abm_only:
         print(json.dumps(abm_bench(args)), flush=True)
@@ -807,6 +927,8 @@ def main():
         out["ensemble"] = ensemble_bench(args)
     if not args.no_er:
         out["er"] = er_bench(args)
+    if not args.no_ref:
+        out["reference_model"] = ref_model_bench(args)
     print(json.dumps(out), flush=True)
 
 

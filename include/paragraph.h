@@ -97,8 +97,10 @@ enum {
     PG_K_LIK = 6,              /* pg_pa_loglik evaluations (all passes) */
     PG_K_LIK_GRAD = 7,         /* pg_pa_loglik_grad evaluations (all passes) */
     PG_K_LIK_BATCH = 8,        /* pg_pa_loglik_grad_batch calls (all passes) */
-    PG_K_ABM = 9,              /* scoring-model year: scores, same-year, selection, rows */
-    PG_K_COUNT = 10
+    PG_K_ABM_SCORES = 9,       /* scoring model: fitness, score sums and components, same-year subset */
+    PG_K_ABM_SELECT = 10,      /* scoring model: 1-hop and 2-hop weighted selection */
+    PG_K_ABM_ROWS = 11,        /* scoring model: row counts, scan, uniform fill, row writes */
+    PG_K_COUNT = 12
 };
 
 /* Returns PG_ABI_VERSION. */
@@ -173,6 +175,10 @@ int pg_abm_configure(pg_ctx* ctx, const pg_abm_model* model);
  * generator, out-degree, same-year target (-1 if none), 1-hop picks,
  * 2-hop picks, uniform picks. */
 int pg_abm_year_info(pg_ctx* ctx, int32_t* info);
+
+/* Candidates keyed by the last PG_MODEL_ABM year's selection:
+ * counts[0] = 1-hop entries, counts[1] = 2-hop entries (with repeats). */
+int pg_abm_year_counts(pg_ctx* ctx, int64_t* counts);
 
 /* Score components of the last PG_MODEL_ABM year's existing papers:
  * comp[4*c .. 4*c+3] = P, R, F, 0 for c in [0, n_exist). */

@@ -134,7 +134,7 @@ class TuneReport(ctypes.Structure):
 
 
 KERNEL_GROUPS = ["weights_cdf", "pa_draw", "bfs", "nb_draw", "er", "csr_append", "lik",
-                 "lik_grad", "lik_batch", "abm"]
+                 "lik_grad", "lik_batch", "abm_scores", "abm_select", "abm_rows"]
 
 
 _lib = None
@@ -172,6 +172,7 @@ def lib() -> ctypes.CDLL:
             "pg_abm_configure": (ctypes.c_int, [vp, P(AbmModel)]),
             "pg_abm_year_info": (ctypes.c_int, [vp, i32p]),
             "pg_abm_scores": (ctypes.c_int, [vp, f32p]),
+            "pg_abm_year_counts": (ctypes.c_int, [vp, i64p]),
             "pg_abm_paper_draws": (ctypes.c_int, [vp, ctypes.c_int64, ctypes.c_int64, i32p, f64p,
                                                   f64p, i32p]),
             "pg_abm_fitness_table": (None, [f64p]),
@@ -392,6 +393,12 @@ class Context:
         self._check(lib().pg_abm_year_info(self._h, _p(info, ctypes.c_int32)),
                     "pg_abm_year_info")
         return info[:n_new]
+
+    def abm_year_counts(self) -> tuple[int, int]:
+        """1-hop and 2-hop candidate entries keyed in the last scoring-model year."""
+        c = np.zeros(2, np.int64)
+        self._check(lib().pg_abm_year_counts(self._h, _p(c, ctypes.c_int64)), "pg_abm_year_counts")
+        return int(c[0]), int(c[1])
 
     def abm_scores(self, n_exist: int) -> np.ndarray:
         comp = np.empty((max(n_exist, 1), 4), np.float32)

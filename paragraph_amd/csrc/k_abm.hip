@@ -35,7 +35,17 @@ namespace {
 constexpr int kAT = 256;        // threads of the paper passes
 constexpr int kAbmBuf = 512;    // selection buffer (entries) per wavefront
 
-// ---- wave-level helpers (64-thread blocks: __syncthreads is a wave barrier) ----
+// ---- wave-level helpers ----
+
+// Orders the LDS accesses of one wavefront around this point (the LDS
+// executes a wavefront's operations in issue order; the fences keep the
+// compiler from moving them across).  Wavefronts of a block never wait for
+// each other: each works on its own item with its own trip counts.
+__device__ __forceinline__ void wsync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // Ascending bitonic sort of s[0, 64*E) held in LDS, in registers.
 template <typename T, int E>
@@ -84,12 +94,12 @@ __device__ void wave_sort_n(T* s, int n, T pad) {
     int P = 64;
     while (P < n) P <<= 1;
     for (int i = n + lane; i < P; i += 64) s[i] = pad;
-    __syncthreads();
+    wsync();
     if (P == 64) wave_sort<T, 1>(s);
     else if (P == 128) wave_sort<T, 2>(s);
     else if (P == 256) wave_sort<T, 4>(s);
     else if constexpr (EMAX >= 8) wave_sort<T, 8>(s);
-    __syncthreads();
+    wsync();
 }
 
 __device__ __forceinline__ uint64_t lanes_below() {
@@ -97,8 +107,8 @@ __device__ __forceinline__ uint64_t lanes_below() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// Sorted s[0,n): keeps the first copy of each value, at most `limit`, in
-// place; returns the count.
+// Sorted s[0,n): keeps the first copy of each value other than ~0 (a
+// dropped entry), at most `limit`, in place; returns the count.
 __device__ int wave_unique(uint64_t* s, int n, int limit) {
     const int lane = threadIdx.x & 63;
     const uint64_t below = lanes_below();
@@ -109,13 +119,13 @@ __device__ int wave_unique(uint64_t* s, int n, int limit) {
         const uint64_t x = i < n ? s[i] : ~0ull;
         uint64_t prev = __shfl_up(x, 1, 64);
         if (lane == 0) prev = carry;
-        const bool k = i < n && x != prev;
+        const bool k = i < n && x != prev && x != ~0ull;
         const uint64_t mask = __ballot(k);
         const int pos = out + __popcll(mask & below);
         carry = __shfl(x, 63, 64);
-        __syncthreads();
+        wsync();
         if (k && pos < limit) s[pos] = x;
-        __syncthreads();
+        wsync();
         out += __popcll(mask);
     }
     return out < limit ? out : limit;
@@ -299,69 +309,122 @@ __global__ __launch_bounds__(kAT) void k_abm_same_mark(int64_t N, int64_t D, int
 
 // ---- selection: one wavefront per (paper, level) ----
 
+// The selection buffer of one wavefront: buf[0, nl) is the current list
+// (sorted, distinct, checked), buf[nl, nb) entries appended since.  A
+// compaction drops the appended entries of the generator's own rows (level
+// 2: checked here, for the few entries that got in, rather than for every
+// candidate), sorts, deduplicates and keeps the best t; once t are kept,
+// only keys below the t-th enter.
 struct SelState {
     uint64_t* buf;
+    int nl;          // checked entries at the front
     int nb;          // entries in buf
     int t;           // entries kept
     uint64_t thr;    // keys must be below this to enter (all, until t are kept)
 };
 
-__device__ __forceinline__ void sel_compact(SelState& st) {
+__device__ __forceinline__ void sel_compact(SelState& st, bool check, const CsrView& g,
+                                            int32_t fa, int32_t fb, int32_t ba, int32_t bb) {
+    if (check) {
+        for (int i = st.nl + (int)(threadIdx.x & 63); i < st.nb; i += 64) {
+            const int32_t c = (int32_t)(uint32_t)st.buf[i];
+            if (row_has(g.fwd_adj, fa, fb, c) || row_has(g.bwd_adj, ba, bb, c)) st.buf[i] = ~0ull;
+        }
+    }
     wave_sort_n<uint64_t, kAbmBuf / 64>(st.buf, st.nb, ~0ull);
     st.nb = wave_unique(st.buf, st.nb, st.t);
-    __syncthreads();
-    st.thr = st.nb == st.t ? st.buf[st.t - 1] : ~0ull;
+    st.nl = st.nb;
+    wsync();
+    st.thr = (st.t > 0 && st.nb == st.t) ? st.buf[st.t - 1] : ~0ull;
 }
 
-__global__ __launch_bounds__(64) void k_abm_select(AbmYear y, CsrView g,
+constexpr int kSelWaves = 4;   // wavefronts (papers) per block
+
+// One wavefront per new paper: the 1-hop list (best t1 of the generator's
+// rows) and then the 2-hop list (best t2 of the rows of every 1-hop entry,
+// the generator's own rows excluded), through one LDS buffer.  kc[l] and
+// kc[D + l] get the entries keyed per level (with repeats).
+__global__ __launch_bounds__(64 * kSelWaves) void k_abm_select(AbmYear y, CsrView g,
                                                    const float4* __restrict__ comp,
                                                    const int32_t* __restrict__ syt,
                                                    int32_t* __restrict__ sel,
-                                                   int32_t* __restrict__ nsel) {
-    __shared__ uint64_t buf[kAbmBuf];
-    const int64_t l = blockIdx.x;
-    const int lev = blockIdx.y;
-    const int lane = threadIdx.x;
+                                                   int32_t* __restrict__ nsel,
+                                                   int32_t* __restrict__ kc) {
+    __shared__ uint64_t bufs[kSelWaves][kAbmBuf];
+    const int wid = threadIdx.x >> 6;
+    uint64_t* buf = bufs[wid];
+    const int64_t l = (int64_t)blockIdx.x * kSelWaves + wid;
+    const int lane = threadIdx.x & 63;
+    if (l >= y.D) return;
     const AbmSetup s = abm_setup(y, l, syt);
-    const int t = lev == 0 ? s.b.t1 : s.b.t2;
-    int32_t* out = sel + ((int64_t)lev * y.D + l) * kAbmSlots;
-    if (t <= 0) {
-        if (lane == 0) nsel[(int64_t)lev * y.D + l] = 0;
-        return;
-    }
     const int32_t gn = s.gen;
     const int32_t fa = g.fwd_off[gn], fb = g.fwd_off[gn + 1];
     const int32_t ba = g.bwd_off[gn], bb = g.bwd_off[gn + 1];
     const int32_t na = fb - fa, n1 = na + (bb - ba);
     const uint64_t below = lanes_below();
-    SelState st{buf, 0, t, ~0ull};
-    auto offer = [&](bool valid, int32_t c) {
-        bool pass = false;
+    SelState st{buf, 0, 0, 0, ~0ull};
+    bool check = false;
+    // key of candidate c (valid lanes other than the generator) and whether
+    // it enters: below the threshold at the time of the call
+    auto key_of = [&](bool valid, int32_t c, bool& pass) -> uint64_t {
+        pass = false;
         uint64_t kv = 0;
         if (valid && c != gn) {
             const float4 q = comp[c];
             kv = abm_key(y.seed, s.v, c, y.yi, q.x, q.y, q.z, s.p.w);
             pass = kv < st.thr;
-            if (pass && lev == 1)
-                pass = !row_has(g.fwd_adj, fa, fb, c) && !row_has(g.bwd_adj, ba, bb, c);
         }
+        return kv;
+    };
+    // more: candidates follow this batch (an early compaction only pays then)
+    auto offer = [&](bool pass, uint64_t kv, bool more) {
         const uint64_t mask = __ballot(pass);
         if (mask) {
             if (pass) buf[st.nb + __popcll(mask & below)] = kv;
             st.nb += __popcll(mask);
-            __syncthreads();
-            if (st.nb > kAbmBuf - 64) sel_compact(st);
+            wsync();
+            // first compaction as soon as t are held (sets the threshold),
+            // later ones when the buffer fills
+            if ((more && st.thr == ~0ull && st.nb >= st.t) || st.nb > kAbmBuf - 64)
+                sel_compact(st, check, g, fa, fb, ba, bb);
         }
     };
-    if (lev == 0) {
-        for (int32_t base = 0; base < n1; base += 64) {
-            const int32_t p = base + lane;
-            const bool valid = p < n1;
-            int32_t c = 0;
-            if (valid) c = p < na ? g.fwd_adj[fa + p] : g.bwd_adj[ba + (p - na)];
-            offer(valid, c);
+    auto finish = [&](int lev) {
+        if (st.t > 0) sel_compact(st, check, g, fa, fb, ba, bb);
+        else st.nb = 0;
+        int32_t* out = sel + ((int64_t)lev * y.D + l) * kAbmSlots;
+        for (int i = lane; i < st.nb; i += 64) out[i] = (int32_t)(uint32_t)buf[i];
+        if (lane == 0) nsel[(int64_t)lev * y.D + l] = st.nb;
+        wsync();
+    };
+    // Two batches of 64 candidates per iteration: their loads and keys are
+    // independent, so each wavefront keeps twice the memory requests and
+    // arithmetic in flight.  A key compared with the threshold of before the
+    // first batch's offer may enter needlessly; it is dropped at the next
+    // compaction (the threshold only falls).
+    auto entry = [&](int32_t p) -> int32_t {
+        return p < na ? g.fwd_adj[fa + p] : g.bwd_adj[ba + (p - na)];
+    };
+    // ---- level 1 ----
+    st = SelState{buf, 0, 0, s.b.t1, ~0ull};
+    check = false;
+    if (st.t > 0) {
+        for (int32_t base = 0; base < n1; base += 128) {
+            const int32_t p0 = base + lane, p1 = p0 + 64;
+            const bool v0 = p0 < n1, v1 = p1 < n1;
+            const int32_t c0 = v0 ? entry(p0) : 0, c1 = v1 ? entry(p1) : 0;
+            bool s0, s1;
+            const uint64_t k0 = key_of(v0, c0, s0), k1 = key_of(v1, c1, s1);
+            offer(s0, k0, base + 64 < n1);
+            if (base + 64 < n1) offer(s1 && k1 < st.thr, k1, base + 128 < n1);
         }
-    } else {
+    }
+    finish(0);
+    // ---- level 2 ----
+    int64_t keyed = 0;  // 2-hop entries streamed (with repeats)
+    st = SelState{buf, 0, 0, s.b.t2, ~0ull};
+    check = true;
+    if (st.t > 0) {
         for (int32_t base = 0; base < n1; base += 64) {
             // 64 level-1 entries: their rows, and the inclusive prefix of the
             // rows' lengths over the lanes
@@ -369,7 +432,7 @@ __global__ __launch_bounds__(64) void k_abm_select(AbmYear y, CsrView g,
             int32_t u = -1, uf = 0, ul = 0, ub = 0;
             int64_t len = 0;
             if (q < n1) {
-                u = q < na ? g.fwd_adj[fa + q] : g.bwd_adj[ba + (q - na)];
+                u = entry(q);
                 if (u != gn) {
                     uf = g.fwd_off[u];
                     ul = g.fwd_off[u + 1] - uf;
@@ -384,9 +447,9 @@ __global__ __launch_bounds__(64) void k_abm_select(AbmYear y, CsrView g,
                 if (lane >= o) incl += x;
             }
             const int64_t total = __shfl(incl, 63, 64);
-            for (int64_t off = 0; off < total; off += 64) {
-                const int64_t P = off + lane;
-                const bool valid = P < total;
+            keyed += total;
+            // the candidate at position P of the chunk's rows
+            auto locate = [&](int64_t P) -> int32_t {
                 int j = 0, hi = 63;  // first lane whose prefix exceeds P
 #pragma unroll
                 for (int it = 0; it < 6; ++it) {
@@ -396,18 +459,28 @@ __global__ __launch_bounds__(64) void k_abm_select(AbmYear y, CsrView g,
                 const int64_t start = __shfl(incl, j, 64) - __shfl(len, j, 64);
                 const int32_t juf = __shfl(uf, j, 64), jul = __shfl(ul, j, 64);
                 const int32_t jub = __shfl(ub, j, 64);
-                int32_t c = 0;
-                if (valid) {
-                    const int32_t r = (int32_t)(P - start);
-                    c = r < jul ? g.fwd_adj[juf + r] : g.bwd_adj[jub + (r - jul)];
-                }
-                offer(valid, c);
+                const int32_t r = (int32_t)(P - start);
+                if (P >= total) return 0;
+                return r < jul ? g.fwd_adj[juf + r] : g.bwd_adj[jub + (r - jul)];
+            };
+            const bool last_chunk = base + 64 >= n1;
+            for (int64_t off = 0; off < total; off += 128) {
+                const int64_t P0 = off + lane, P1 = P0 + 64;
+                const bool v0 = P0 < total, v1 = P1 < total;
+                const int32_t c0 = locate(P0), c1 = locate(P1);
+                bool s0, s1;
+                const uint64_t k0 = key_of(v0, c0, s0), k1 = key_of(v1, c1, s1);
+                offer(s0, k0, !last_chunk || off + 64 < total);
+                if (off + 64 < total)
+                    offer(s1 && k1 < st.thr, k1, !last_chunk || off + 128 < total);
             }
         }
     }
-    sel_compact(st);
-    for (int i = lane; i < st.nb; i += 64) out[i] = (int32_t)(uint32_t)buf[i];
-    if (lane == 0) nsel[(int64_t)lev * y.D + l] = st.nb;
+    finish(1);
+    if (lane == 0) {
+        kc[l] = s.b.t1 > 0 ? n1 : 0;
+        kc[y.D + l] = (int32_t)min(keyed, (int64_t)INT32_MAX);
+    }
 }
 
 // ---- rows ----
@@ -466,17 +539,17 @@ __global__ __launch_bounds__(64) void k_abm_write(AbmYear y, const int32_t* __re
         }
         const uint64_t mask = __ballot(ok);
         const int rank = __popcll(mask & below);
-        __syncthreads();
+        wsync();
         if (ok && rank < need) row[n + rank] = c;
         const int took = min(__popcll(mask), need);
         n += took;
         need -= took;
-        __syncthreads();
+        wsync();
         if (took) wave_sort_n<int32_t, kAbmSlots / 64>(row, n, INT_MAX);
     }
     if (sy_t >= 0 && lane == 0) row[n] = sy_t;  // a new paper: past every existing id
     n += sy_t >= 0 ? 1 : 0;
-    __syncthreads();
+    wsync();
     const int64_t b = new_off[l];
     for (int i = lane; i < n; i += 64) {
         fwd_adj[fwd_base + b + i] = row[i];
@@ -520,8 +593,10 @@ void launch_abm_same_mark(hipStream_t s, int64_t N, int64_t D, int64_t K, uint64
 }
 
 void launch_abm_select(hipStream_t s, const AbmYear& y, const CsrView& g, const float4* comp,
-                       const int32_t* syt, int32_t* sel, int32_t* nsel) {
-    if (y.D > 0) k_abm_select<<<dim3((uint32_t)y.D, 2), 64, 0, s>>>(y, g, comp, syt, sel, nsel);
+                       const int32_t* syt, int32_t* sel, int32_t* nsel, int32_t* kc) {
+    if (y.D > 0)
+        k_abm_select<<<(uint32_t)((y.D + kSelWaves - 1) / kSelWaves), 64 * kSelWaves, 0, s>>>(
+            y, g, comp, syt, sel, nsel, kc);
 }
 
 void launch_abm_count(hipStream_t s, const AbmYear& y, const int32_t* syt, const int32_t* nsel,

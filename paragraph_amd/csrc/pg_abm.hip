@@ -44,7 +44,7 @@ void abm_release(pg_ctx* c) {
     pg_abm_store& a = c->abm;
     dfree(a.bag); dfree(a.rec); dfree(a.cdf); dfree(a.fit); dfree(a.comp); dfree(a.parts);
     dfree(a.cnt); dfree(a.rt); dfree(a.tot); dfree(a.syt); dfree(a.sy_keys); dfree(a.sy_vals);
-    dfree(a.sel); dfree(a.nsel); dfree(a.info);
+    dfree(a.sel); dfree(a.nsel); dfree(a.info); dfree(a.cand); dfree(a.kc);
     a.on = false;
 }
 
@@ -58,18 +58,30 @@ int abm_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
     if (N + D > (int64_t)INT32_MAX) return fail(ctx, PG_ECAP, "pg_year_step: ids exceed int32");
     PG_HIP(hipSetDevice(ctx->cfg.device));
     hipStream_t s = ctx->stream;
-    const bool on = ctx->prof && ((ctx->prof_mask >> PG_K_ABM) & 1u);
+    bool used[PG_K_COUNT] = {}, used2[PG_K_COUNT] = {};
+    auto on = [&](int k) { return ctx->prof && ((ctx->prof_mask >> k) & 1u); };
+    auto ev = [&](int k, int j) -> hipError_t {
+        if (!on(k)) return hipSuccess;
+        if (j == 1) used[k] = true;
+        return hipEventRecord(ctx->pev[k][j], s);
+    };
+    auto ev2 = [&](int k, int j) -> hipError_t {
+        if (!on(k)) return hipSuccess;
+        if (j == 1) used2[k] = true;
+        return hipEventRecord(ctx->pev2[k][j], s);
+    };
     PG_HIP(hipEventRecord(ctx->ev0, s));
-    if (on) PG_HIP(hipEventRecord(ctx->pev[PG_K_ABM][0], s));
     int64_t e_year = 0;
     if (D > 0) {
         if (int rc = ensure_year_edges(ctx, D)) return rc;
+        PG_HIP(ev(PG_K_ABM_SCORES, 0));
         if (a.fit_n < N + D) {
             launch_abm_fit(s, a.fit_n, N + D, ctx->cfg.seed, a.cdf, a.fit);
             a.fit_n = N + D;
         }
         launch_fill_i32(s, ctx->birth + N, p->year, D);
         launch_fill_i32(s, a.cnt, 0, kAbmRecMax + 1);
+        launch_fill_i32(s, reinterpret_cast<int32_t*>(a.cand), 0, 4);
         launch_abm_scores(s, N, ctx->in_deg, ctx->birth, a.fit, p->year, a.n_rec, a.rec, a.parts,
                           a.cnt, a.cnt + kAbmRecMax, a.tot, a.rt, a.comp);
         const uint32_t yi = (uint32_t)p->year_index;
@@ -80,16 +92,27 @@ int abm_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
                                     ctx->k_tmp, ctx->v_tmp, D, 32, ctx->scratch);
             launch_abm_same_mark(s, N, D, K, ctx->cfg.seed, yi, ctx->v_sorted, a.syt);
         }
+        PG_HIP(ev(PG_K_ABM_SCORES, 1));
         AbmYear y{ctx->cfg.seed, N, D, yi, p->year, a.bag, a.n_bag, a.alpha, a.w_pa, a.w_rec,
                   a.w_fit, a.frc};
-        launch_abm_select(s, y, view(ctx), a.comp, a.syt, a.sel, a.nsel);
+        PG_HIP(ev(PG_K_ABM_SELECT, 0));
+        launch_abm_select(s, y, view(ctx), a.comp, a.syt, a.sel, a.nsel, a.kc);
+        launch_sum_i32(s, a.kc, D, a.cand);
+        launch_sum_i32(s, a.kc + D, D, a.cand + 1);
+        PG_HIP(ev(PG_K_ABM_SELECT, 1));
+        PG_HIP(ev(PG_K_ABM_ROWS, 0));
         launch_abm_count(s, y, a.syt, a.nsel, ctx->cnt, a.info);
         launch_excl_scan_i32(s, ctx->cnt, D, ctx->new_off, ctx->scratch);
+        PG_HIP(ev(PG_K_ABM_ROWS, 1));
         PG_HIP(hipGetLastError());
         PG_HIP(hipMemcpyAsync(ctx->h_sum, ctx->new_off + D, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         PG_HIP(hipMemcpyAsync(ctx->h_sum + 1, a.cnt + kAbmRecMax, sizeof(int32_t),
                               hipMemcpyDeviceToHost, s));
+        PG_HIP(hipMemcpyAsync(ctx->h_sum + 2, a.cand, 2 * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, s));
         PG_HIP(hipStreamSynchronize(s));
+        a.last_cand[0] = (int64_t)ctx->h_sum[2];
+        a.last_cand[1] = (int64_t)ctx->h_sum[3];
         e_year = (int64_t)ctx->h_sum[0];
         if ((int32_t)ctx->h_sum[1] != 0)
             return fail(ctx, PG_ECAP, "pg_year_step: an in-degree reached 2^21 (k^3 overflows)");
@@ -98,12 +121,11 @@ int abm_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
         if (ctx->e + e_year > (int64_t)INT32_MAX)
             return fail(ctx, PG_ECAP, "pg_year_step: edge offsets exceed int32");
         if (int rc = ensure_year_edges(ctx, std::max(e_year, D))) return rc;
+        PG_HIP(ev2(PG_K_ABM_ROWS, 0));
         launch_abm_write(s, y, a.syt, a.sel, a.info, ctx->new_off, ctx->e, ctx->fwd_off,
                          ctx->fwd_adj, ctx->src_of);
-        if (on) {
-            PG_HIP(hipEventRecord(ctx->pev[PG_K_ABM][1], s));
-            PG_HIP(hipEventRecord(ctx->pev[PG_K_CSR_APPEND][0], s));
-        }
+        PG_HIP(ev2(PG_K_ABM_ROWS, 1));
+        PG_HIP(ev(PG_K_CSR_APPEND, 0));
         uint32_t* keys = reinterpret_cast<uint32_t*>(ctx->fwd_adj + ctx->e);
         launch_radix_sort_pairs(s, keys, reinterpret_cast<const uint32_t*>(ctx->src_of),
                                 ctx->k_sorted, ctx->v_sorted, ctx->k_tmp, ctx->v_tmp, e_year,
@@ -114,7 +136,7 @@ int abm_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
         launch_bwd_merge(s, N, N + D, ctx->bwd_off[ctx->cur], ctx->bwd_adj[ctx->cur],
                          ctx->bwd_off[nxt], ctx->v_sorted, ctx->bwd_adj[nxt], ctx->e + e_year, 0,
                          ctx->scratch, ctx->cfg.max_papers + 2);
-        if (on) PG_HIP(hipEventRecord(ctx->pev[PG_K_CSR_APPEND][1], s));
+        PG_HIP(ev(PG_K_CSR_APPEND, 1));
         PG_HIP(hipGetLastError());
         ctx->cur = nxt;
     }
@@ -122,19 +144,34 @@ int abm_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
     PG_HIP(hipEventSynchronize(ctx->ev1));
     float ms = 0.0f;
     PG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    if (on && D > 0) {
-        float k1 = 0.0f, k2 = 0.0f;
-        PG_HIP(hipEventElapsedTime(&k1, ctx->pev[PG_K_ABM][0], ctx->pev[PG_K_ABM][1]));
-        PG_HIP(hipEventElapsedTime(&k2, ctx->pev[PG_K_CSR_APPEND][0], ctx->pev[PG_K_CSR_APPEND][1]));
+    if (ctx->prof && D > 0) {
+        // algorithmic bytes: scores read in_deg, birth (4 B) and fitness (2 B)
+        // in each of two passes and write P,R,F (16 B); the same-year keys are
+        // 8 B per paper plus four radix passes of 16 B; the selection reads
+        // each 1-hop entry twice (4 B, once per level), gathers its P,R,F
+        // (16 B) and its row offsets (16 B), and each 2-hop entry (4 B) with
+        // its P,R,F (16 B); rows read the counts and picks and write 8 B per
+        // edge (target, source for the append)
         const double e_old = (double)ctx->e;
-        ctx->prof_ms[PG_K_ABM] += k1;
-        // scores: in_deg, birth, fit read twice, comp written; rows: 8 B per edge
-        ctx->prof_bytes[PG_K_ABM] += 2.0 * 10.0 * N + 16.0 * N + 8.0 * e_year;
-        ctx->prof_calls[PG_K_ABM] += 1;
-        ctx->prof_ms[PG_K_CSR_APPEND] += k2;
-        ctx->prof_bytes[PG_K_CSR_APPEND] += 4.0 * (e_year + D) + 4.0 * (e_old + N) +
-                                            4.0 * (e_old + e_year + N + D) + 8.0 * e_year;
-        ctx->prof_calls[PG_K_CSR_APPEND] += 1;
+        double bytes[PG_K_COUNT] = {};
+        bytes[PG_K_ABM_SCORES] = 36.0 * N + 72.0 * D;
+        bytes[PG_K_ABM_SELECT] = 40.0 * (double)a.last_cand[0] + 20.0 * (double)a.last_cand[1];
+        bytes[PG_K_ABM_ROWS] = 12.0 * e_year + 80.0 * D;
+        bytes[PG_K_CSR_APPEND] = 4.0 * (e_year + D) + 4.0 * (e_old + N) +
+                                 4.0 * (e_old + e_year + N + D) + 8.0 * e_year;
+        for (int k = 0; k < PG_K_COUNT; ++k) {
+            if (!used[k]) continue;
+            float kms = 0.0f;
+            PG_HIP(hipEventElapsedTime(&kms, ctx->pev[k][0], ctx->pev[k][1]));
+            if (used2[k]) {
+                float k2 = 0.0f;
+                PG_HIP(hipEventElapsedTime(&k2, ctx->pev2[k][0], ctx->pev2[k][1]));
+                kms += k2;
+            }
+            ctx->prof_ms[k] += kms;
+            ctx->prof_bytes[k] += bytes[k];
+            ctx->prof_calls[k] += 1;
+        }
     }
     if (stats) {
         stats->n_before = N;
@@ -207,6 +244,8 @@ int pg_abm_configure(pg_ctx* ctx, const pg_abm_model* m) {
     if (r == hipSuccess) r = dalloc(&a.sel, 2 * DN * kAbmSlots);
     if (r == hipSuccess) r = dalloc(&a.nsel, 2 * DN);
     if (r == hipSuccess) r = dalloc(&a.info, 6 * DN);
+    if (r == hipSuccess) r = dalloc(&a.cand, 2);
+    if (r == hipSuccess) r = dalloc(&a.kc, 2 * DN);
     if (r != hipSuccess) {
         abm_release(ctx);
         return fail(ctx, PG_ENOMEM, "pg_abm_configure: device allocation failed");
@@ -242,6 +281,17 @@ int pg_abm_year_info(pg_ctx* ctx, int32_t* info) {
     PG_HIP(hipSetDevice(ctx->cfg.device));
     if (a.last_d > 0)
         PG_HIP(hipMemcpy(info, a.info, (size_t)a.last_d * 6 * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return PG_OK;
+}
+
+int pg_abm_year_counts(pg_ctx* ctx, int64_t* counts) {
+    if (!ctx) return PG_EINVAL;
+    const pg_abm_store& a = ctx->abm;
+    if (!a.on || a.last_n < 0 || ctx->last_model != PG_MODEL_ABM)
+        return fail(ctx, PG_ESTATE, "pg_abm_year_counts: the last year was not a PG_MODEL_ABM year");
+    if (!counts) return fail(ctx, PG_EINVAL, "pg_abm_year_counts: null buffer");
+    counts[0] = a.last_cand[0];
+    counts[1] = a.last_cand[1];
     return PG_OK;
 }
 

@@ -111,7 +111,10 @@ struct pg_abm_store {
     int32_t* sel = nullptr;         // [2 * max_new * kAbmSlots] selected ids per paper and level
     int32_t* nsel = nullptr;        // [2 * max_new]
     int32_t* info = nullptr;        // [6 * max_new]
+    unsigned long long* cand = nullptr;  // [2] candidates keyed per level this year
+    int32_t* kc = nullptr;          // [2 * max_new] candidates keyed per paper and level
     int64_t last_n = -1, last_d = 0;  // the last year's existing and new papers
+    int64_t last_cand[2] = {0, 0};
 };
 
 struct pg_ctx {

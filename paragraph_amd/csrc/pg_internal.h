@@ -196,8 +196,9 @@ void launch_abm_same_keys(hipStream_t s, int64_t N, int64_t D, uint64_t seed, ui
                           uint32_t* keys, uint32_t* vals, int32_t* syt);
 void launch_abm_same_mark(hipStream_t s, int64_t N, int64_t D, int64_t K, uint64_t seed,
                           uint32_t yi, const uint32_t* order, int32_t* syt);
+// kc[l], kc[D + l]: candidate entries keyed per level for paper l (with repeats).
 void launch_abm_select(hipStream_t s, const AbmYear& y, const CsrView& g, const float4* comp,
-                       const int32_t* syt, int32_t* sel, int32_t* nsel);
+                       const int32_t* syt, int32_t* sel, int32_t* nsel, int32_t* kc);
 void launch_abm_count(hipStream_t s, const AbmYear& y, const int32_t* syt, const int32_t* nsel,
                       int32_t* cnt, int32_t* info);
 void launch_abm_write(hipStream_t s, const AbmYear& y, const int32_t* syt, const int32_t* sel,

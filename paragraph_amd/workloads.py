@@ -26,3 +26,20 @@ def ensemble_params(n_sims: int, m: int = 20) -> list[dict]:
         out.append(dict(n0=n0, alpha=alpha, tau=tau,
                         n_nb=min(int(math.floor(f_nb * m + 0.5)), m - 1), seed=s))
     return out
+
+
+def reference_model_inputs(n_bag: int = 10_000) -> dict:
+    """Inputs of the reference's scoring model for the benchmark, shaped like
+    the reference's own runs (drivers/ablation_parallel_gpu_10y.sh: alpha
+    0.5, equal weights, 5% fully random, 12% same-year; an out-degree bag of
+    papers with at least five references): bag values 5 + floor(lognormal(
+    2.3, 0.8)) capped at 250 (mean about 19), drawn from numpy's PCG64 with
+    seed 4; recency probabilities p(d) proportional to d * exp(-d / 4) for
+    year differences d = 1..30."""
+    rng = np.random.default_rng(4)
+    bag = np.minimum(5 + np.floor(rng.lognormal(2.3, 0.8, n_bag)), 250).astype(np.int32)
+    d = np.arange(31, dtype=np.float64)
+    rec = d * np.exp(-d / 4.0)
+    rec /= rec.sum()
+    return dict(bag=bag, rec=rec, alpha=0.5, w_pa=0.33, w_rec=0.33, w_fit=0.33,
+                fully_random=0.05, same_year=0.12)
