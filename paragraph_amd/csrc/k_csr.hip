@@ -187,148 +187,17 @@ __global__ __launch_bounds__(256) void k_bwd_offsets_sorted(int64_t n_old, int64
     }
 }
 
-// outputs per block (4 per thread); measured against 512 and 2048.  4096
-// gave rows that differ from the oracle on the GPU although a CPU emulation of
-// this indexing merges correctly at 4096 (the cause lies in how that build
-// executes, not yet found): keep kMergeOut at 2048 or below, a multiple of 256
+// outputs per block (4 per thread); measured against 512 and 2048.  A 4096
+// build once gave rows that differ from the oracle on the GPU.  The kernel
+// source (k_merge.inc) run on the host with real barriers, wavefront
+// exchanges and LDS atomics merges correctly at 1024, 2048 and 4096
+// (oracle/xcheck/merge_host.cpp), and the 4096 build uses no scratch
+// (49,200 B of LDS, 30 VGPRs), so the cause lies outside the source's logic
+// and is not found; keep kMergeOut at 2048 or below, a multiple of 256
 constexpr int kMergeOut = 1024;
 static_assert(kMergeOut % 256 == 0 && kMergeOut >= 256 && kMergeOut <= 2048, "kMergeOut");
 
-__device__ __forceinline__ int64_t row_of_pos(const int32_t* off, int64_t lo, int64_t hi, int64_t p) {
-    // largest v in [lo, hi) with off[v] <= p
-    while (hi - lo > 1) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (off[mid] <= p) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
-// Rows spanned by each merge block: part[2b] = row of the block's first
-// position, part[2b+1] = row of its last (one thread per block, so the
-// binary searches run in parallel instead of at the head of every block).
-__global__ __launch_bounds__(256) void k_merge_part(int64_t n_total, const int32_t* __restrict__ new_off,
-                                                    int64_t e_new, int64_t nblk,
-                                                    int64_t* __restrict__ part) {
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nblk) return;
-    if (e_new < 0) e_new = new_off[n_total];
-    const int64_t pb = b * kMergeOut;
-    if (pb >= e_new) {
-        part[2 * b] = part[2 * b + 1] = 0;
-        return;
-    }
-    const int64_t pe = (pb + kMergeOut < e_new ? pb + kMergeOut : e_new) - 1;
-    const int64_t r0 = row_of_pos(new_off, 0, n_total + 1, pb);
-    part[2 * b] = r0;
-    part[2 * b + 1] = row_of_pos(new_off, r0, n_total + 1, pe);
-}
-
-// Each block writes kMergeOut consecutive positions of the new backward
-// adjacency.  The new and old offsets of the rows those positions span are
-// staged in LDS; every position's row comes from a row-start table (each
-// row marks its first position, a prefix maximum fills the rest), and then
-// consecutive threads write consecutive positions, so reads of old rows,
-// of the sorted new citers and the writes are all coalesced.  Blocks whose
-// positions span more than kMergeOut rows (long runs of empty rows) search
-// the offsets per position instead.  part (nullable) holds the blocks' row
-// ranges from k_merge_part.
-__global__ __launch_bounds__(256) void k_bwd_merge(int64_t n_old, int64_t n_total,
-                                                   const int32_t* __restrict__ old_off,
-                                                   const int32_t* __restrict__ old_adj,
-                                                   const int32_t* __restrict__ new_off,
-                                                   const uint32_t* __restrict__ sorted_src,
-                                                   int32_t* __restrict__ out, int64_t e_new,
-                                                   const int64_t* __restrict__ part) {
-    __shared__ int32_t s_new[kMergeOut + 1];
-    __shared__ int32_t s_old[kMergeOut + 1];
-    __shared__ int32_t s_row[kMergeOut];
-    __shared__ int32_t s_max[4];
-    __shared__ int64_t s_rlo, s_nrows;
-    if (e_new < 0) e_new = new_off[n_total];  // total only known on the device
-    const int64_t pb = (int64_t)blockIdx.x * kMergeOut;
-    if (pb >= e_new) return;
-    const int64_t pe = (pb + kMergeOut < e_new ? pb + kMergeOut : e_new) - 1;  // last position
-    const int cnt = (int)(pe - pb + 1);
-    if (threadIdx.x == 0) {
-        int64_t r0, r1;
-        if (part) {
-            r0 = part[2 * blockIdx.x];
-            r1 = part[2 * blockIdx.x + 1];
-        } else {
-            r0 = row_of_pos(new_off, 0, n_total + 1, pb);
-            r1 = row_of_pos(new_off, r0, n_total + 1, pe);
-        }
-        s_rlo = r0;
-        s_nrows = r1 - r0 + 1;
-    }
-    __syncthreads();
-    const int64_t rlo = s_rlo, nrows = s_nrows;
-    if (nrows <= kMergeOut) {
-        for (int64_t k = threadIdx.x; k <= nrows; k += 256) {
-            s_new[k] = new_off[rlo + k];
-            const int64_t v = rlo + k;
-            s_old[k] = old_off[v < n_old ? v : n_old];
-        }
-        for (int j = threadIdx.x; j < kMergeOut; j += 256) s_row[j] = 0;
-        __syncthreads();
-        // row k starts at local position s_new[k] - pb; of several rows
-        // starting there (empty ones), the last holds the position
-        for (int64_t k = threadIdx.x + 1; k < nrows; k += 256) {
-            const int64_t st = s_new[k] - pb;
-            if (st >= 0 && st < cnt) atomicMax(&s_row[st], (int)k);
-        }
-        __syncthreads();
-        // prefix maximum over the positions: kPer per thread, across the
-        // lanes of a wavefront, then across the four wavefronts
-        constexpr int kPer = kMergeOut / 256;
-        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-        const int j0 = threadIdx.x * kPer;
-        int loc[kPer];
-        int mx = 0;
-#pragma unroll
-        for (int q = 0; q < kPer; ++q) {
-            loc[q] = s_row[j0 + q];
-            mx = max(mx, loc[q]);
-        }
-        int inc = mx;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int u = __shfl_up(inc, d, 64);
-            if (lane >= d) inc = max(inc, u);
-        }
-        if (lane == 63) s_max[wid] = inc;
-        __syncthreads();
-        int run = __shfl_up(inc, 1, 64);
-        if (lane == 0) run = 0;
-        for (int w = 0; w < wid; ++w) run = max(run, s_max[w]);
-#pragma unroll
-        for (int q = 0; q < kPer; ++q) {
-            run = max(run, loc[q]);
-            s_row[j0 + q] = run;
-        }
-        __syncthreads();
-        for (int j = threadIdx.x; j < cnt; j += 256) {
-            const int64_t p = pb + j;
-            const int v = s_row[j];
-            const int32_t row_beg = s_new[v];
-            const int64_t k = p - row_beg;
-            const int64_t ob = s_old[v];
-            const int64_t olen = s_old[v + 1] - ob;
-            out[p] = k < olen ? old_adj[ob + k] : (int32_t)sorted_src[(row_beg - ob) + (k - olen)];
-        }
-    } else {
-        for (int j = threadIdx.x; j < cnt; j += 256) {
-            const int64_t p = pb + j;
-            const int64_t v = row_of_pos(new_off, rlo, rlo + nrows, p);
-            const int32_t row_beg = new_off[v];
-            const int64_t k = p - row_beg;
-            const int64_t ob = old_off[v < n_old ? v : n_old];
-            const int64_t olen = old_off[v + 1 < n_old ? v + 1 : n_old] - ob;
-            out[p] = k < olen ? old_adj[ob + k] : (int32_t)sorted_src[(row_beg - ob) + (k - olen)];
-        }
-    }
-}
-
+#include "k_merge.inc"
 }  // namespace
 
 void launch_pa_compact(hipStream_t s, int64_t n_exist, int64_t n_new, int m, const int32_t* cit,
@@ -417,11 +286,11 @@ void launch_bwd_merge(hipStream_t s, int64_t n_old, int64_t n_total, const int32
     if (e_grid <= 0) return;
     int64_t nb = (e_grid + kMergeOut - 1) / kMergeOut;
     if (part && 2 * nb <= part_cap)
-        hipLaunchKernelGGL(k_merge_part, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, s, n_total,
+        hipLaunchKernelGGL(k_merge_part<kMergeOut>, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, s, n_total,
                            bwd_off_new, e_new, nb, part);
     else
         part = nullptr;
-    hipLaunchKernelGGL(k_bwd_merge, dim3((uint32_t)nb), dim3(256), 0, s, n_old, n_total, bwd_off_old,
+    hipLaunchKernelGGL(k_bwd_merge<kMergeOut>, dim3((uint32_t)nb), dim3(256), 0, s, n_old, n_total, bwd_off_old,
                        bwd_adj_old, bwd_off_new, sorted_src, bwd_adj_new, e_new,
                        (const int64_t*)part);
 }
