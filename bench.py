@@ -840,6 +840,14 @@ def main():
     ctx.set_tuning(bfs_mode=args.bfs_mode)
     ctx.load(g)
     ctx.snapshot_save()
+    tune = None
+    if not args.no_tune:
+        # launch shapes timed on this workload (results identical by check)
+        years = [dict(year=1 + yi, year_index=yi, n_new=D, m=m, n_nb=n_nb, alpha=alpha, tau=tau)
+                 for yi, D in enumerate(sched)]
+        tune = ctx.year_autotune(years, reps=args.tune_reps)
+        eprint(f"year autotune: chosen {tune['chosen']} {tune['chosen_ms']:.2f} ms per 10 years "
+               f"(default shapes {tune['default_ms']:.2f} ms)")
     G0 = None
     if not args.no_cpu_baseline:
         from oracle import binding as ob
@@ -909,6 +917,9 @@ def main():
                                f"(14 PA + 6 neighbourhood), cap 4096, alpha=1, tau=5",
                    "edges_per_step": total_edges // max(args.steps, 1),
                    "near_boundary_attempts": total_near,
+                   "autotune": None if tune is None else {
+                       "chosen": tune["chosen"], "chosen_ms": tune["chosen_ms"],
+                       "default_ms": tune["default_ms"], "trials": len(tune["trials"])},
                    "kernel_ms_per_step": {k: v["ms"] for k, v in prof.items()
                                           if v["launches"]}},
         "roofline": roofline(dom, achieved, "pa_10y_2M_kernels.md", dom),

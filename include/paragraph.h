@@ -509,6 +509,14 @@ typedef struct {
                                   swept; 1 = always swept; 2 = staged
                                   whenever the list fits a staging slot
                                   (lists up to 4096 entries: radix-sorted) */
+    int32_t bfs_small_max;     /* waves per 2-hop query: the most candidate
+                                  entries a root may bring to the one-wavefront
+                                  small tier, 256/512/1024/2048 (0 = 2048) */
+    int32_t bfs_mid_max;       /* ... to the one-wavefront medium tier,
+                                  512/1024/2048/4096 (0 = 4096); roots past
+                                  it take a 512-thread workgroup */
+    int32_t merge_out;         /* backward-CSR merge tile: output positions
+                                  per block, 256/512/1024/2048 (0 = 1024) */
 } pg_tuning;
 
 int pg_ensemble_create(const pg_ensemble_config* cfg, const pg_sim_params* sims,
@@ -589,6 +597,16 @@ int pg_ensemble_autotune(pg_ensemble* ens, int32_t reps, pg_tune_report* report)
  * 8 * lanes >= m), each trial one pg_pa_loglik_grad at p's parameters on the
  * context's stream.  Leaves the context with the chosen shape. */
 int pg_lik_autotune(pg_ctx* ctx, const pg_lik_params* p, int32_t reps, pg_tune_report* report);
+/* Year steps of a plain context, on the caller's schedule years[0..n_years)
+ * run from the context's current state: PA lanes (8, 16, 32, 64), then the
+ * 2-hop modes, the wavefront tiers' candidate limits (small 512, 1024; medium
+ * 2048) and the merge tile (512, 2048), each step keeping the best so far.
+ * Every trial restores the state (pg_snapshot_save is called first and
+ * replaces the context's snapshot) and must reproduce the same edge,
+ * attempt and near-boundary counts (PG_ESTATE otherwise).  Leaves the
+ * context in the state it was called in, with the chosen shapes. */
+int pg_year_autotune(pg_ctx* ctx, const pg_year_params* years, int32_t n_years, int32_t reps,
+                     pg_tune_report* report);
 
 /* Ceil growth rule of the reference (src/abm.cu:233-240,
  * src/kernel.cu:1690): ceil(n * rate) in double precision. */

@@ -111,10 +111,14 @@ class EnsembleStats(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("size", ctypes.c_uint32), ("pa_lanes", ctypes.c_int32),
-                ("lik_lanes", ctypes.c_int32), ("bfs_mode", ctypes.c_int32)]
+                ("lik_lanes", ctypes.c_int32), ("bfs_mode", ctypes.c_int32),
+                ("bfs_small_max", ctypes.c_int32), ("bfs_mid_max", ctypes.c_int32),
+                ("merge_out", ctypes.c_int32)]
 
     def as_dict(self) -> dict:
-        return {"pa_lanes": self.pa_lanes, "lik_lanes": self.lik_lanes, "bfs_mode": self.bfs_mode}
+        return {"pa_lanes": self.pa_lanes, "lik_lanes": self.lik_lanes, "bfs_mode": self.bfs_mode,
+                "bfs_small_max": self.bfs_small_max, "bfs_mid_max": self.bfs_mid_max,
+                "merge_out": self.merge_out}
 
 
 TUNE_MAX_TRIALS = 16
@@ -228,6 +232,8 @@ def lib() -> ctypes.CDLL:
             "pg_set_tuning": (ctypes.c_int, [vp, P(Tuning)]),
             "pg_ensemble_autotune": (ctypes.c_int, [vp, ctypes.c_int32, P(TuneReport)]),
             "pg_lik_autotune": (ctypes.c_int, [vp, P(LikParams), ctypes.c_int32, P(TuneReport)]),
+            "pg_year_autotune": (ctypes.c_int, [vp, P(YearParams), ctypes.c_int32, ctypes.c_int32,
+                                                P(TuneReport)]),
             "pg_pa_loglik": (ctypes.c_int, [vp, P(LikParams), P(LikOut)]),
             "pg_pa_loglik_grad": (ctypes.c_int, [vp, P(LikParams), P(LikOut)]),
             "pg_pa_loglik_grad_batch": (ctypes.c_int, [vp, P(LikBatchParams), P(LikBatchOut)]),
@@ -596,12 +602,31 @@ class Context:
                     "pg_lik_autotune")
         return r.as_dict()
 
-    def set_tuning(self, pa_lanes: int = 0, lik_lanes: int = 0, bfs_mode: int = 0):
+    def set_tuning(self, pa_lanes: int = 0, lik_lanes: int = 0, bfs_mode: int = 0,
+                   bfs_small_max: int = 0, bfs_mid_max: int = 0, merge_out: int = 0):
         """Launch shapes (speed only): PA lanes per paper, likelihood lanes
         per citing paper, 2-hop workgroup path mode (0 staged up to 65536
-        candidate entries, 1 always swept, 2 staged whenever it fits)."""
-        t = Tuning(ctypes.sizeof(Tuning), pa_lanes, lik_lanes, bfs_mode)
+        candidate entries, 1 always swept, 2 staged whenever it fits), the
+        one-wavefront 2-hop tiers' candidate limits, the merge tile."""
+        t = Tuning(ctypes.sizeof(Tuning), pa_lanes, lik_lanes, bfs_mode, bfs_small_max,
+                   bfs_mid_max, merge_out)
         self._check(lib().pg_set_tuning(self._h, ctypes.byref(t)), "pg_set_tuning")
+
+    def year_autotune(self, years: list[dict], reps: int = 2) -> dict:
+        """pg_year_autotune over PA year steps given as dicts (year,
+        year_index, n_new, m, n_nb, alpha, tau); the context's state is kept,
+        the chosen shapes are set; returns the report."""
+        arr = (YearParams * len(years))()
+        for k, y in enumerate(years):
+            tau = y["tau"]
+            arr[k] = YearParams(ctypes.sizeof(YearParams), PG_MODEL_PA, y["year"], y["year_index"],
+                                y["n_new"], y["m"], y["n_nb"], y["alpha"],
+                                0.0 if math.isinf(tau) else 1.0 / tau, 0.0)
+        r = TuneReport()
+        r.size = ctypes.sizeof(TuneReport)
+        self._check(lib().pg_year_autotune(self._h, arr, len(years), int(reps), ctypes.byref(r)),
+                    "pg_year_autotune")
+        return r.as_dict()
 
     def snapshot_save(self):
         self._check(lib().pg_snapshot_save(self._h), "pg_snapshot_save")
@@ -740,9 +765,12 @@ class Ensemble:
         return {"cit": cit.reshape(n_new, m), "kind": kind.reshape(n_new, m), "cnt": cnt,
                 "near": near, "w": w, "cdf": cdf}
 
-    def set_tuning(self, pa_lanes: int = 0, lik_lanes: int = 0, bfs_mode: int = 0):
-        """Launch shapes (speed only): PA lanes per paper, 2-hop path mode."""
-        t = Tuning(ctypes.sizeof(Tuning), pa_lanes, lik_lanes, bfs_mode)
+    def set_tuning(self, pa_lanes: int = 0, lik_lanes: int = 0, bfs_mode: int = 0,
+                   bfs_small_max: int = 0, bfs_mid_max: int = 0, merge_out: int = 0):
+        """Launch shapes (speed only): PA lanes per paper, 2-hop path mode,
+        the one-wavefront 2-hop tiers' candidate limits, the merge tile."""
+        t = Tuning(ctypes.sizeof(Tuning), pa_lanes, lik_lanes, bfs_mode, bfs_small_max,
+                   bfs_mid_max, merge_out)
         self._check(lib().pg_ensemble_set_tuning(self._h, ctypes.byref(t)),
                     "pg_ensemble_set_tuning")
 

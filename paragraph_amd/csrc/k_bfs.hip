@@ -1239,9 +1239,9 @@ __global__ __launch_bounds__(kBT, 4) void k_bfs_root(CsrView g, const float* __r
 // workgroup path: sort, drop duplicates (ballot/popcount compaction),
 // filter, keep the cap smallest.
 constexpr int kSmallF1 = 512;
-constexpr int kSmallWin = 2048;
+constexpr int kSmallWin = kBfsSmallWinMax;
 constexpr int kMidF1 = 1024;
-constexpr int kMidWin = 4096;
+constexpr int kMidWin = kBfsMidWinMax;
 
 template <int F1, int WIN>
 struct WaveLds {
@@ -1312,7 +1312,8 @@ __device__ __forceinline__ void bfs_wave(CsrView g, const float* __restrict__ w,
                                          int32_t* __restrict__ ns_buf,
                                          int32_t* __restrict__ out_list,
                                          int32_t* __restrict__ out_count,
-                                         unsigned long long* __restrict__ bytes_out) {
+                                         unsigned long long* __restrict__ bytes_out,
+                                         int win_max) {
     __shared__ WaveLds<F1, WIN> Ls;
     int64_t u;
     if (FROM_LIST) {
@@ -1358,7 +1359,7 @@ __device__ __forceinline__ void bfs_wave(CsrView g, const float* __restrict__ w,
     }
     for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
     const int64_t tot2 = t + n1c;
-    if (tot2 > WIN) {
+    if (tot2 > win_max) {
         if (lane == 0) out_list[atomicAdd(out_count, 1)] = (int32_t)u;
         return;
     }
@@ -1509,9 +1510,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const int64_t* __restrict__ n_unique_ptr, int cap, int32_t* __restrict__ S_buf,
     uint64_t* __restrict__ Q_buf, int32_t* __restrict__ q_idx, int32_t* __restrict__ ns_buf,
     int32_t* __restrict__ out_list, int32_t* __restrict__ out_count,
-    unsigned long long* __restrict__ bytes_out) {
+    unsigned long long* __restrict__ bytes_out, int win_max) {
     bfs_wave<kSmallF1, kSmallWin, false>(g, w, uroot, n_unique_ptr, nullptr, nullptr, cap, S_buf,
-                                         Q_buf, q_idx, ns_buf, out_list, out_count, bytes_out);
+                                         Q_buf, q_idx, ns_buf, out_list, out_count, bytes_out,
+                                         win_max);
 }
 
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_bfs_mid(
@@ -1519,9 +1521,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const int32_t* __restrict__ in_list, const int32_t* __restrict__ in_count, int cap,
     int32_t* __restrict__ S_buf, uint64_t* __restrict__ Q_buf, int32_t* __restrict__ q_idx,
     int32_t* __restrict__ ns_buf, int32_t* __restrict__ out_list,
-    int32_t* __restrict__ out_count, unsigned long long* __restrict__ bytes_out) {
+    int32_t* __restrict__ out_count, unsigned long long* __restrict__ bytes_out, int win_max) {
     bfs_wave<kMidF1, kMidWin, true>(g, w, uroot, nullptr, in_list, in_count, cap, S_buf, Q_buf,
-                                    q_idx, ns_buf, out_list, out_count, bytes_out);
+                                    q_idx, ns_buf, out_list, out_count, bytes_out, win_max);
 }
 
 // ---- root deduplication ----
@@ -1795,11 +1797,14 @@ void launch_bfs_root(hipStream_t s, const CsrView& g, const float* w, const int3
                      int32_t* big_count, unsigned long long* bytes_out, const BfsStage& stage) {
     if (max_unique <= 0) return;
     int32_t* mid_list = big_list + max_unique;
+    const int small_max = stage.small_max > 0 ? std::min(stage.small_max, kSmallWin) : kSmallWin;
+    const int mid_max = stage.mid_max > 0 ? std::min(stage.mid_max, kMidWin) : kMidWin;
     hipLaunchKernelGGL(k_bfs_small, dim3((uint32_t)max_unique), dim3(64), 0, s, g, w, uroot,
-                       n_unique, cap, S_buf, Q_buf, q_idx, ns_buf, big_list, big_count, bytes_out);
+                       n_unique, cap, S_buf, Q_buf, q_idx, ns_buf, big_list, big_count, bytes_out,
+                       small_max);
     hipLaunchKernelGGL(k_bfs_mid, dim3((uint32_t)max_unique), dim3(64), 0, s, g, w, uroot,
                        big_list, big_count, cap, S_buf, Q_buf, q_idx, ns_buf, mid_list,
-                       big_count + 1, bytes_out);
+                       big_count + 1, bytes_out, mid_max);
     hipLaunchKernelGGL(k_bfs_root, dim3((uint32_t)max_unique), dim3(kBT), 0, s, g, w, uroot,
                        mid_list, big_count + 1, cap, S_buf, Q_buf, q_idx, ns_buf, bytes_out,
                        (const int32_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr, stage);
@@ -1829,10 +1834,12 @@ void launch_nb_draw_batch(hipStream_t s, const Batch& b, int m, int cap, uint32_
                        root_slot, S_buf, Q_buf, q_idx, ns_buf, cit, kind, cnt, near, anext);
 }
 
-BfsStage bfs_stage_view(const BfsStage& pool, int mode) {
+BfsStage bfs_stage_view(const BfsStage& pool, int mode, int small_max, int mid_max) {
     BfsStage v = pool;
     v.max_total = mode == 1 ? 0 : mode == 2 ? INT64_MAX : kBfsStageDefaultMax;
     v.radix = mode == 0 ? 1 : 0;
+    v.small_max = small_max;
+    v.mid_max = mid_max;
     return v;
 }
 

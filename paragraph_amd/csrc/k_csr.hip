@@ -278,21 +278,46 @@ void launch_bwd_offsets_sorted(hipStream_t s, int64_t n_old, int64_t n_new,
 }
 
 
+template <int kOut>
+void bwd_merge_at(hipStream_t s, int64_t n_old, int64_t n_total, const int32_t* bwd_off_old,
+                  const int32_t* bwd_adj_old, const int32_t* bwd_off_new,
+                  const uint32_t* sorted_src, int32_t* bwd_adj_new, int64_t e_new, int64_t e_grid,
+                  int64_t* part, int64_t part_cap) {
+    static_assert(kOut % 256 == 0 && kOut >= 256 && kOut <= 2048, "positions per block");
+    const int64_t nb = (e_grid + kOut - 1) / kOut;
+    if (part && 2 * nb <= part_cap)
+        hipLaunchKernelGGL(k_merge_part<kOut>, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, s,
+                           n_total, bwd_off_new, e_new, nb, part);
+    else
+        part = nullptr;
+    hipLaunchKernelGGL(k_bwd_merge<kOut>, dim3((uint32_t)nb), dim3(256), 0, s, n_old, n_total,
+                       bwd_off_old, bwd_adj_old, bwd_off_new, sorted_src, bwd_adj_new, e_new,
+                       (const int64_t*)part);
+}
+
 void launch_bwd_merge(hipStream_t s, int64_t n_old, int64_t n_total, const int32_t* bwd_off_old,
                       const int32_t* bwd_adj_old, const int32_t* bwd_off_new,
                       const uint32_t* sorted_src, int32_t* bwd_adj_new, int64_t e_new,
-                      int64_t e_bound, int64_t* part, int64_t part_cap) {
+                      int64_t e_bound, int64_t* part, int64_t part_cap, int merge_out) {
     const int64_t e_grid = e_new >= 0 ? e_new : e_bound;
     if (e_grid <= 0) return;
-    int64_t nb = (e_grid + kMergeOut - 1) / kMergeOut;
-    if (part && 2 * nb <= part_cap)
-        hipLaunchKernelGGL(k_merge_part<kMergeOut>, dim3((uint32_t)((nb + 255) / 256)), dim3(256), 0, s, n_total,
-                           bwd_off_new, e_new, nb, part);
-    else
-        part = nullptr;
-    hipLaunchKernelGGL(k_bwd_merge<kMergeOut>, dim3((uint32_t)nb), dim3(256), 0, s, n_old, n_total, bwd_off_old,
-                       bwd_adj_old, bwd_off_new, sorted_src, bwd_adj_new, e_new,
-                       (const int64_t*)part);
+    switch (merge_out) {
+        case 256:
+            bwd_merge_at<256>(s, n_old, n_total, bwd_off_old, bwd_adj_old, bwd_off_new, sorted_src,
+                              bwd_adj_new, e_new, e_grid, part, part_cap);
+            break;
+        case 512:
+            bwd_merge_at<512>(s, n_old, n_total, bwd_off_old, bwd_adj_old, bwd_off_new, sorted_src,
+                              bwd_adj_new, e_new, e_grid, part, part_cap);
+            break;
+        case 2048:
+            bwd_merge_at<2048>(s, n_old, n_total, bwd_off_old, bwd_adj_old, bwd_off_new,
+                               sorted_src, bwd_adj_new, e_new, e_grid, part, part_cap);
+            break;
+        default:
+            bwd_merge_at<kMergeOut>(s, n_old, n_total, bwd_off_old, bwd_adj_old, bwd_off_new,
+                                    sorted_src, bwd_adj_new, e_new, e_grid, part, part_cap);
+    }
 }
 
 }  // namespace pg

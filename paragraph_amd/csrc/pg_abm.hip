@@ -135,7 +135,7 @@ int abm_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
                                   ctx->bwd_off[nxt], ctx->in_deg);
         launch_bwd_merge(s, N, N + D, ctx->bwd_off[ctx->cur], ctx->bwd_adj[ctx->cur],
                          ctx->bwd_off[nxt], ctx->v_sorted, ctx->bwd_adj[nxt], ctx->e + e_year, 0,
-                         ctx->scratch, ctx->cfg.max_papers + 2);
+                         ctx->scratch, ctx->cfg.max_papers + 2, ctx->tune_merge_out);
         PG_HIP(ev(PG_K_CSR_APPEND, 1));
         PG_HIP(hipGetLastError());
         ctx->cur = nxt;

@@ -418,7 +418,8 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
                 launch_bfs_root(s, view(ctx), ctx->w, ctx->uroot, ctx->r_excl + D, D,
                                 ctx->cfg.bfs_cap, ctx->S_buf, ctx->Q_buf, ctx->Q_idx, ctx->ns_buf,
                                 ctx->big_list, big_count, sums + 2,
-                                bfs_stage_view(ctx->stage, ctx->tune_bfs_mode));
+                                bfs_stage_view(ctx->stage, ctx->tune_bfs_mode,
+                                               ctx->tune_small_max, ctx->tune_mid_max));
                 pend(PG_K_BFS);
                 pbeg2(PG_K_NB_DRAW);
                 launch_nb_draw(s, N, D, m, n_nb, ctx->cfg.bfs_cap, seed, yi, ctx->cdf,
@@ -471,7 +472,7 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
         launch_bwd_merge(s, N, N + D, ctx->bwd_off[ctx->cur], ctx->bwd_adj[ctx->cur],
                          ctx->bwd_off[nxt], ctx->v_sorted, ctx->bwd_adj[nxt],
                          one_sync ? -1 : ctx->e + e_year, one_sync ? ctx->e + UB : 0,
-                         ctx->scratch, ctx->cfg.max_papers + 2);
+                         ctx->scratch, ctx->cfg.max_papers + 2, ctx->tune_merge_out);
         pend(PG_K_CSR_APPEND);
         PG_HIP(hipGetLastError());
         ctx->cur = nxt;
@@ -697,19 +698,115 @@ int pg_set_tuning(pg_ctx* ctx, const pg_tuning* t) {
     if (!ctx) return PG_EINVAL;
     if (!t || t->size != sizeof(pg_tuning))
         return fail(ctx, PG_EINVAL, "pg_set_tuning: null or struct size mismatch");
-    if (t->pa_lanes != 0 && t->pa_lanes != 8 && t->pa_lanes != 16 && t->pa_lanes != 32 &&
-        t->pa_lanes != 64)
-        return fail(ctx, PG_EINVAL, "pg_set_tuning: pa_lanes must be 0, 8, 16, 32 or 64");
-    if (t->lik_lanes != 0 && t->lik_lanes != 1 && t->lik_lanes != 2 && t->lik_lanes != 4 &&
-        t->lik_lanes != 8)
-        return fail(ctx, PG_EINVAL, "pg_set_tuning: lik_lanes must be 0, 1, 2, 4 or 8");
-    if (t->bfs_mode < 0 || t->bfs_mode > 2)
-        return fail(ctx, PG_EINVAL, "pg_set_tuning: bfs_mode must be 0, 1 or 2");
+    if (const char* e = tuning_error(t)) return fail(ctx, PG_EINVAL, std::string("pg_set_tuning: ") + e);
     if (t->lik_lanes != ctx->tune_lik_lanes) ctx->seq.tables_dirty = true;
     ctx->tune_pa_lanes = t->pa_lanes;
     ctx->tune_lik_lanes = t->lik_lanes;
     ctx->tune_bfs_mode = t->bfs_mode;
+    ctx->tune_small_max = t->bfs_small_max;
+    ctx->tune_mid_max = t->bfs_mid_max;
+    ctx->tune_merge_out = t->merge_out;
     return PG_OK;
+}
+
+int pg_year_autotune(pg_ctx* ctx, const pg_year_params* years, int32_t n_years, int32_t reps,
+                     pg_tune_report* report) {
+    if (!ctx) return PG_EINVAL;
+    if (report && report->size != sizeof(pg_tune_report))
+        return fail(ctx, PG_EINVAL, "pg_year_autotune: pg_tune_report.size mismatch");
+    if (!years || n_years < 1 || reps < 1 || reps > 64)
+        return fail(ctx, PG_EINVAL, "pg_year_autotune: need years and reps in [1, 64]");
+    if (!ctx->loaded) return fail(ctx, PG_ESTATE, "pg_year_autotune: no graph loaded");
+    if (ctx->seq_recording) return fail(ctx, PG_ESTATE, "pg_year_autotune: sequence recording is on");
+    if (int rc = pg_snapshot_save(ctx)) return rc;
+    const pg_tuning before{sizeof(pg_tuning), ctx->tune_pa_lanes, ctx->tune_lik_lanes,
+                           ctx->tune_bfs_mode, ctx->tune_small_max, ctx->tune_mid_max,
+                           ctx->tune_merge_out};
+    std::vector<pg_tuning> cand;
+    std::vector<double> ms;
+    int64_t ref[3] = {0, 0, 0};
+    bool have_ref = false;
+    // one candidate: an untimed pass over the years, then reps timed ones,
+    // each from the saved state
+    auto trial = [&](const pg_tuning& t) -> int {
+        if (int rc = pg_set_tuning(ctx, &t)) return rc;
+        std::vector<double> v;
+        for (int r = 0; r <= reps; ++r) {
+            if (int rc = pg_snapshot_restore(ctx)) return rc;
+            int64_t got[3] = {0, 0, 0};
+            double dms = 0.0;
+            for (int32_t y = 0; y < n_years; ++y) {
+                pg_year_stats st{};
+                st.size = sizeof st;
+                if (int rc = pg_year_step(ctx, &years[y], &st)) return rc;
+                got[0] += st.edges_added;
+                got[1] += st.near_boundary;
+                got[2] += st.pa_attempts;
+                dms += st.device_ms;
+            }
+            if (!have_ref) {
+                for (int k = 0; k < 3; ++k) ref[k] = got[k];
+                have_ref = true;
+            }
+            if (got[0] != ref[0] || got[1] != ref[1] || got[2] != ref[2])
+                return fail(ctx, PG_ESTATE, "pg_year_autotune: a launch shape changed the run");
+            if (r > 0) v.push_back(dms);
+        }
+        std::sort(v.begin(), v.end());
+        cand.push_back(t);
+        ms.push_back(v.size() % 2 ? v[v.size() / 2] : 0.5 * (v[v.size() / 2 - 1] + v[v.size() / 2]));
+        return PG_OK;
+    };
+    auto run_all = [&]() -> int {
+        int m = 0;
+        for (int32_t y = 0; y < n_years; ++y) m = std::max(m, (int)years[y].m - (int)years[y].n_nb);
+        const int def_lanes = pa_draw_lanes(m);
+        double default_ms = 0.0;
+        size_t best = 0;
+        for (int l : {8, 16, 32, 64}) {
+            pg_tuning t = before;
+            t.pa_lanes = l;
+            t.bfs_mode = 0;
+            t.bfs_small_max = t.bfs_mid_max = t.merge_out = 0;
+            if (int rc = trial(t)) return rc;
+            if (l == def_lanes) default_ms = ms.back();
+        }
+        auto pick = [&]() { best = std::min_element(ms.begin(), ms.end()) - ms.begin(); };
+        pick();
+        auto vary = [&](int which, std::initializer_list<int> vals) -> int {
+            for (int v : vals) {
+                pg_tuning t = cand[best];
+                (which == 0 ? t.bfs_mode : which == 1 ? t.bfs_small_max
+                                         : which == 2 ? t.bfs_mid_max : t.merge_out) = v;
+                if (int rc = trial(t)) return rc;
+            }
+            pick();
+            return PG_OK;
+        };
+        if (int rc = vary(0, {1, 2})) return rc;
+        if (int rc = vary(1, {512, 1024})) return rc;
+        if (int rc = vary(2, {2048})) return rc;
+        if (int rc = vary(3, {512, 2048})) return rc;
+        const pg_tuning chosen = cand[best];
+        if (int rc = pg_set_tuning(ctx, &chosen)) return rc;
+        if (report) {
+            report->n_trials = (int32_t)std::min<size_t>(cand.size(), PG_TUNE_MAX_TRIALS);
+            for (int k = 0; k < report->n_trials; ++k) {
+                report->trial[k] = cand[k];
+                report->trial_ms[k] = ms[k];
+            }
+            report->chosen = chosen;
+            report->chosen_ms = ms[best];
+            report->default_ms = default_ms;
+            report->max_rel_dev = 0.0;
+        }
+        return PG_OK;
+    };
+    const int rc = run_all();
+    if (rc) (void)pg_set_tuning(ctx, &before);
+    // back to the state of the call (kept on failure too, when restorable)
+    const int rr = pg_snapshot_restore(ctx);
+    return rc ? rc : rr;
 }
 
 int pg_bfs_timed(pg_ctx* ctx, int64_t count, const int32_t* roots, int32_t* out, int32_t* out_len,
@@ -754,6 +851,28 @@ int pg_bfs_timed(pg_ctx* ctx, int64_t count, const int32_t* roots, int32_t* out,
 int pg_bfs(pg_ctx* ctx, int64_t count, const int32_t* roots, int32_t* out, int32_t* out_len) {
     return pg_bfs_timed(ctx, count, roots, out, out_len, nullptr);
 }
+
+}  // extern "C"
+
+const char* tuning_error(const pg_tuning* t) {
+    auto in = [](int v, std::initializer_list<int> ok) {
+        for (int x : ok)
+            if (v == x) return true;
+        return false;
+    };
+    if (!in(t->pa_lanes, {0, 8, 16, 32, 64})) return "pa_lanes must be 0, 8, 16, 32 or 64";
+    if (!in(t->lik_lanes, {0, 1, 2, 4, 8})) return "lik_lanes must be 0, 1, 2, 4 or 8";
+    if (t->bfs_mode < 0 || t->bfs_mode > 2) return "bfs_mode must be 0, 1 or 2";
+    if (!in(t->bfs_small_max, {0, 256, 512, 1024, 2048}))
+        return "bfs_small_max must be 0, 256, 512, 1024 or 2048";
+    if (!in(t->bfs_mid_max, {0, 512, 1024, 2048, 4096}))
+        return "bfs_mid_max must be 0, 512, 1024, 2048 or 4096";
+    if (!in(t->merge_out, {0, 256, 512, 1024, 2048}))
+        return "merge_out must be 0, 256, 512, 1024 or 2048";
+    return nullptr;
+}
+
+extern "C" {
 
 int64_t pg_growth_count(int64_t n, double rate) {
     return (int64_t)std::ceil((double)n * rate);

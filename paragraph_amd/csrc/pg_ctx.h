@@ -194,6 +194,9 @@ struct pg_ctx {
     int tune_pa_lanes = 0;
     int tune_lik_lanes = 0;
     int tune_bfs_mode = 0;
+    int tune_small_max = 0;
+    int tune_mid_max = 0;
+    int tune_merge_out = 0;
     // citation-sequence store for the likelihood; recording appends each PA year
     pg_seq_store seq;
     bool seq_recording = false;
@@ -239,6 +242,9 @@ int ensure_year_edges(pg_ctx* ctx, int64_t need);
 int key_bits_for(int64_t n);
 // The current CSR as kernels read it.
 pg::CsrView view(const pg_ctx* c);
+
+// Why a pg_tuning is invalid (nullptr: valid), shared by the set_tuning calls.
+const char* tuning_error(const pg_tuning* t);
 
 // Neighbourhood buffers sized by cfg.max_new_per_year and cfg.bfs_cap.
 int ensure_nb_buffers(pg_ctx* ctx);

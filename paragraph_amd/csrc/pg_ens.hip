@@ -110,7 +110,8 @@ void enqueue_year(pg_ensemble* ens, int y, hipStream_t s, hipEvent_t* bfs_ev = n
         if (bfs_ev) (void)hipEventRecord(bfs_ev[0], s);
         launch_bfs_root(s, g, c->w, c->uroot, c->r_excl + Dn, Dn, c->cfg.bfs_cap, c->S_buf,
                         c->Q_buf, c->Q_idx, c->ns_buf, c->big_list, big_count, cnt + 2,
-                        bfs_stage_view(c->stage, c->tune_bfs_mode));
+                        bfs_stage_view(c->stage, c->tune_bfs_mode, c->tune_small_max,
+                                       c->tune_mid_max));
         if (bfs_ev) (void)hipEventRecord(bfs_ev[1], s);
         launch_nb_draw_batch(s, b, m, c->cfg.bfs_cap, (uint32_t)y, c->cdf, c->root_slot,
                              c->S_buf, c->Q_buf, c->Q_idx, c->ns_buf, c->cit, c->kind, c->cnt, c->near,
@@ -132,7 +133,8 @@ void enqueue_year(pg_ensemble* ens, int y, hipStream_t s, hipEvent_t* bfs_ev = n
     launch_bwd_offsets_sorted(s, TOT, 0, c->bwd_off[c->cur], c->k_sorted, UB, c->bwd_off[nxt],
                               c->in_deg);
     launch_bwd_merge(s, TOT, TOT, c->bwd_off[c->cur], c->bwd_adj[c->cur], c->bwd_off[nxt],
-                     c->v_sorted, c->bwd_adj[nxt], -1, ens->ETOT, c->scratch, c->cfg.max_papers + 2);
+                     c->v_sorted, c->bwd_adj[nxt], -1, ens->ETOT, c->scratch, c->cfg.max_papers + 2,
+                     c->tune_merge_out);
     c->cur = nxt;
 }
 
@@ -633,14 +635,18 @@ int pg_ensemble_download(pg_ensemble* ens, int32_t sim, int32_t* fwd_off, int32_
 int pg_ensemble_set_tuning(pg_ensemble* ens, const pg_tuning* t) {
     if (!ens || !t || t->size != sizeof(pg_tuning))
         return efail(ens, PG_EINVAL, "pg_ensemble_set_tuning: null or struct size mismatch");
-    if (t->pa_lanes != 0 && t->pa_lanes != 8 && t->pa_lanes != 16 && t->pa_lanes != 32 &&
-        t->pa_lanes != 64)
-        return efail(ens, PG_EINVAL, "pg_ensemble_set_tuning: pa_lanes must be 0, 8, 16, 32 or 64");
-    if (t->bfs_mode < 0 || t->bfs_mode > 2)
-        return efail(ens, PG_EINVAL, "pg_ensemble_set_tuning: bfs_mode must be 0, 1 or 2");
-    if (t->pa_lanes != ens->pa_lanes || t->bfs_mode != ens->ctx->tune_bfs_mode) drop_graph(ens);
+    if (const char* e = tuning_error(t))
+        return efail(ens, PG_EINVAL, std::string("pg_ensemble_set_tuning: ") + e);
+    pg_ctx* c = ens->ctx;
+    if (t->pa_lanes != ens->pa_lanes || t->bfs_mode != c->tune_bfs_mode ||
+        t->bfs_small_max != c->tune_small_max || t->bfs_mid_max != c->tune_mid_max ||
+        t->merge_out != c->tune_merge_out)
+        drop_graph(ens);  // the captured graph holds the old launches
     ens->pa_lanes = t->pa_lanes;
-    ens->ctx->tune_bfs_mode = t->bfs_mode;
+    c->tune_bfs_mode = t->bfs_mode;
+    c->tune_small_max = t->bfs_small_max;
+    c->tune_mid_max = t->bfs_mid_max;
+    c->tune_merge_out = t->merge_out;
     return PG_OK;
 }
 
@@ -707,6 +713,20 @@ int pg_ensemble_autotune(pg_ensemble* ens, int32_t reps, pg_tune_report* report)
     for (int mode : {1, 2})
         if (int rc = trial(shape(best_lanes, mode))) return rc;
     best = std::min_element(ms.begin(), ms.end()) - ms.begin();
+    // then the wavefront tiers' limits and the merge tile, one knob at a
+    // time from the best so far
+    auto vary = [&](int which, std::initializer_list<int> vals) -> int {
+        for (int v : vals) {
+            pg_tuning t = cand[best];
+            (which == 0 ? t.bfs_small_max : which == 1 ? t.bfs_mid_max : t.merge_out) = v;
+            if (int rc = trial(t)) return rc;
+        }
+        best = std::min_element(ms.begin(), ms.end()) - ms.begin();
+        return PG_OK;
+    };
+    if (int rc = vary(0, {512, 1024})) return rc;
+    if (int rc = vary(1, {2048})) return rc;
+    if (int rc = vary(2, {512, 2048})) return rc;
     const pg_tuning chosen = cand[best];
     if (int rc = pg_ensemble_set_tuning(ens, &chosen)) return rc;
     if (int rc = pg_ensemble_restore(ens)) return rc;

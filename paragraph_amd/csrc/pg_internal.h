@@ -218,6 +218,11 @@ struct BfsStage {
     int32_t cap_entries = 0;
     int64_t max_total = 0;   // larger candidate lists (row entries) are swept
     int32_t radix = 0;       // 1: every list tries the in-LDS radix sort first
+    // one-wavefront tiers: the most 2-hop candidates a root may bring to the
+    // small and to the medium tier (0: the tier's LDS capacity); roots past
+    // them go on to the next tier.  Speed only: every tier builds the same S.
+    int32_t small_max = 0;
+    int32_t mid_max = 0;
 };
 // Slots for one device: twice the resident workgroups of the path.
 int bfs_stage_slots(int device);
@@ -228,7 +233,10 @@ constexpr int32_t kBfsStageEntries = 1 << 17;
 constexpr int64_t kBfsStageDefaultMax = 1 << 16;
 // The pool view that a launch gets for tuning mode m (0 default bound,
 // 1 never staged, 2 staged whenever the list fits a slot).
-BfsStage bfs_stage_view(const BfsStage& pool, int mode);
+BfsStage bfs_stage_view(const BfsStage& pool, int mode, int small_max = 0, int mid_max = 0);
+// Candidate limits a tuning may give the wavefront tiers (their LDS windows).
+constexpr int kBfsSmallWinMax = 2048;
+constexpr int kBfsMidWinMax = 4096;
 
 // PA phases 2-3.  Roots (first PA pick of each paper) are deduplicated and
 // numbered in ascending id order: root_slot[i] = distinct-root index u of
@@ -293,10 +301,11 @@ void launch_bwd_offsets_sorted(hipStream_t s, int64_t n_old, int64_t n_new,
 // grouped by target in ascending citing order).  e_new < 0: the edge total
 // is read on the device (bwd_off_new[n_total]) and e_bound sizes the grid.
 // part (nullable, part_cap int64): scratch for the blocks' row ranges.
+// merge_out: output positions per block, 256, 512, 1024 or 2048 (0: 1024).
 void launch_bwd_merge(hipStream_t s, int64_t n_old, int64_t n_total, const int32_t* bwd_off_old,
                       const int32_t* bwd_adj_old, const int32_t* bwd_off_new,
                       const uint32_t* sorted_src, int32_t* bwd_adj_new, int64_t e_new,
-                      int64_t e_bound, int64_t* part, int64_t part_cap);
+                      int64_t e_bound, int64_t* part, int64_t part_cap, int merge_out = 0);
 
 // Batched forward append (rows at the end of each simulation's region; the
 // year's (target, citing) pairs also listed contiguously in stage/src_of).

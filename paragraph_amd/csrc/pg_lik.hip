@@ -762,6 +762,9 @@ int pg_lik_autotune(pg_ctx* ctx, const pg_lik_params* p, int32_t reps, pg_tune_r
             t.pa_lanes = ctx->tune_pa_lanes;
             t.lik_lanes = cand[k];
             t.bfs_mode = ctx->tune_bfs_mode;
+            t.bfs_small_max = ctx->tune_small_max;
+            t.bfs_mid_max = ctx->tune_mid_max;
+            t.merge_out = ctx->tune_merge_out;
             report->trial[k] = t;
             report->trial_ms[k] = ms[k];
         }

@@ -103,16 +103,17 @@ def test_ensemble_tuning_does_not_change_results():
 
 
 def test_ensemble_autotune_keeps_results():
-    """The autotuner times 4 PA lane counts and 2 more 2-hop modes on whole
-    captured runs, keeps the fastest, and leaves the seed state: a run with
-    the chosen shapes reproduces the plain runs bit for bit."""
+    """The autotuner times 4 PA lane counts, 2 more 2-hop modes, 3 wavefront
+    tier limits and 2 merge tiles on whole captured runs, keeps the fastest,
+    and leaves the seed state: a run with the chosen shapes reproduces the
+    plain runs bit for bit."""
     sims = SIMS[:4]
     graphs = seed_graphs(sims)
     refs = [plain_run(sp, g, sched)[0] for sp, (g, sched) in zip(sims, graphs)]
     ens = make_ensemble(sims, graphs)
     rep = ens.autotune(reps=2)
     log("autotune", rep["chosen"], f"{rep['chosen_ms']:.3f} ms (default {rep['default_ms']:.3f} ms)")
-    assert len(rep["trials"]) == 6
+    assert len(rep["trials"]) == 11
     assert rep["chosen_ms"] == min(t["ms"] for t in rep["trials"])
     assert rep["chosen"]["pa_lanes"] in (8, 16, 32, 64) and rep["chosen"]["bfs_mode"] in (0, 1, 2)
     assert rep["default_ms"] > 0
