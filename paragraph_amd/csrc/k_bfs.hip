@@ -55,8 +55,8 @@ constexpr int kMaxBucketShift = 17;         // a bucket must fit one bitmap wind
 constexpr int kPiece = 256;                 // staged path: long-row part per wave step
 
 struct BfsLds {
-    int32_t f1[kBfsCapMax];       // capped 1-hop list (ascending)
-    int32_t win[kWin];            // sort window, or bitmap of one id window
+    alignas(16) int32_t f1[kBfsCapMax];       // capped 1-hop list (ascending)
+    alignas(16) int32_t win[kWin];           // sort window, or bitmap of one id window
     int32_t cursor[kMaxRows];     // per-row position, (staged path) bucket ends, or
                                   // (radix path) the second buffer and the counters
     int32_t lrow[kMaxLong];       // rows searched/copied by whole wavefronts
@@ -516,52 +516,69 @@ __device__ __forceinline__ int32_t xor_lane(int32_t v) {
     }
 }
 
-// Bitonic sort of s[0,P) (P a power of two <= 4096) by one wavefront,
-// in registers: element i lives in register e = i / 64 of lane i % 64 (so
-// the loads and stores are conflict-free and the result lands back in
-// natural order).  Compare distances below 64 are lane exchanges (DPP and
-// row swaps, xor_lane), larger ones pairs of registers within a lane; the
-// network is fully unrolled for E = P / 64 registers per lane.
-// One compare-exchange step (distance J) of merge size K; recurses down to
-// J = 1.  Compile-time J and K, so every lane exchange is a fixed pattern.
 // Median of three, written so that the compiler emits one v_med3_i32: with
 // c = INT_MIN it is min(a, b), with c = INT_MAX max(a, b).
 __device__ __forceinline__ int32_t med3_i32(int32_t a, int32_t b, int32_t c) {
     return min(max(a, b), max(min(a, b), c));
 }
 
-// A lane exchange keeps min(v, o) where the lane is the lower one of its
-// pair in an ascending block, else max(v, o): one v_med3 against a per-lane
-// bound (INT_MIN or INT_MAX) chosen once per step.
+// Hides x's value (a per-lane INT_MIN/INT_MAX bound) from the compiler, so
+// that a v_med3 against it stays one instruction instead of a min, a max
+// and a select.
+__device__ __forceinline__ void opaque(int32_t& x) { asm("" : "+v"(x)); }
+
+// Orders this wavefront's LDS accesses before the barrier against its
+// accesses after it, across lanes (a compiler barrier; the LDS serves one
+// wavefront's instructions in order).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Bitonic sort of s[0,P) (P = 64E a power of two <= 4096) by one wavefront,
+// in registers, lane-major: lane l holds elements [lE, lE+E) in registers
+// 0..E-1.  A compare distance J below E pairs two registers of one lane (a
+// min and a max, or two v_med3 where the direction depends on the lane); a
+// distance J >= E pairs register e of lanes l and l ^ J/E (xor_lane: DPP and
+// row swaps) with one v_med3 against a per-lane bound.  Every merge ends
+// with its log2(E) in-lane steps, so for E >= 8 most of the network needs no
+// lane exchange.  Loads and stores are 16-byte pieces (E >= 4); they touch
+// other lanes' natural-order elements, hence the fences on both sides.
+// One compare-exchange step (distance J) of merge size K, recursing down to
+// J = 1; compile-time J and K, so every lane exchange is a fixed pattern.
 template <int E, int K, int J>
 __device__ __forceinline__ void bitonic_step(int32_t (&v)[E], int lane) {
-    if constexpr (J < 64) {
-        const bool lower = (lane & J) == 0;
-        if constexpr (K < 64) {
-            const int32_t b = (lower == ((lane & K) == 0)) ? INT_MIN : INT_MAX;
-#pragma unroll
-            for (int e = 0; e < E; ++e) v[e] = med3_i32(v[e], xor_lane<J>(v[e]), b);
-        } else {
-            // ascending iff bit K of the element index e*64 + lane, that is
-            // of e*64, is clear
-            const int32_t b_up = lower ? INT_MIN : INT_MAX;
-            const int32_t b_dn = lower ? INT_MAX : INT_MIN;
-#pragma unroll
-            for (int e = 0; e < E; ++e)
-                v[e] = med3_i32(v[e], xor_lane<J>(v[e]), ((e * 64) & K) == 0 ? b_up : b_dn);
-        }
-    } else {
-        constexpr int je = J / 64;
+    if constexpr (J < E) {
+        // element lE+e ascends iff bit K of it is clear: of e for K < E,
+        // of the lane (bit K/E) otherwise
+        const bool up_l = (lane & (K >= E ? K / E : 0)) == 0;
+        int32_t b_lo = up_l ? INT_MIN : INT_MAX, b_hi = up_l ? INT_MAX : INT_MIN;
+        opaque(b_lo);
+        opaque(b_hi);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            if ((e & je) == 0) {
-                const int i = e * 64 + lane;
-                const bool up = (i & K) == 0;
-                const int32_t x = v[e], y = v[e + je];
-                v[e] = up ? min(x, y) : max(x, y);
-                v[e + je] = up ? max(x, y) : min(x, y);
+            if ((e & J) == 0) {
+                const int32_t x = v[e], y = v[e + J];
+                if constexpr (K < E) {
+                    const bool up = (e & K) == 0;
+                    v[e] = up ? min(x, y) : max(x, y);
+                    v[e + J] = up ? max(x, y) : min(x, y);
+                } else {
+                    v[e] = med3_i32(x, y, b_lo);
+                    v[e + J] = med3_i32(x, y, b_hi);
+                }
             }
         }
+    } else {
+        constexpr int d = J / E;
+        // keep the minimum where "lower of the pair" equals "ascending"
+        const bool lower = (lane & d) == 0;
+        const bool up = (lane & (K / E)) == 0;
+        int32_t b = (lower == up) ? INT_MIN : INT_MAX;
+        opaque(b);
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = med3_i32(v[e], xor_lane<d>(v[e]), b);
     }
     if constexpr (J > 1) bitonic_step<E, K, J / 2>(v, lane);
 }
@@ -576,16 +593,33 @@ template <int E>
 __device__ __forceinline__ void wave_sort_reg(int32_t* s) {
     const int lane = threadIdx.x & 63;
     int32_t v[E];
+    int32_t* mine = s + lane * E;
+    wave_lds_sync();
+    if constexpr (E % 4 == 0) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = s[e * 64 + lane];
+        for (int e = 0; e < E; e += 4) {
+            const int4 q = *reinterpret_cast<const int4*>(mine + e);
+            v[e] = q.x; v[e + 1] = q.y; v[e + 2] = q.z; v[e + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = mine[e];
+    }
     bitonic_merges<E, 2>(v, lane);
+    if constexpr (E % 4 == 0) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) s[e * 64 + lane] = v[e];
+        for (int e = 0; e < E; e += 4)
+            *reinterpret_cast<int4*>(mine + e) = make_int4(v[e], v[e + 1], v[e + 2], v[e + 3]);
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) mine[e] = v[e];
+    }
+    wave_lds_sync();
 }
 
 // One wavefront: the distinct ids of the sorted s[0,n) other than root, the
 // first limit of them into out (another array); returns their number,
-// capped.  Each lane reads only the elements it wrote in the sort.
+// capped.
 __device__ __forceinline__ int wave_unique_to(const int32_t* s, int n, int limit, int32_t root,
                                               int32_t* out) {
     const int lane = threadIdx.x & 63;
@@ -1245,8 +1279,8 @@ constexpr int kMidWin = kBfsMidWinMax;
 
 template <int F1, int WIN>
 struct WaveLds {
-    int32_t f1[F1];
-    int32_t win[WIN];
+    alignas(16) int32_t f1[F1];
+    alignas(16) int32_t win[WIN];
 };
 
 // s[0,P), P a power of two <= WIN: padded to 64 if smaller (with INT_MAX),
