@@ -137,14 +137,51 @@ def test_ensemble_256_against_oracle_and_plain_runs():
     log(f"stepped and captured runs agree: {stepped_edges} edges")
     plain_edges = 0
     for s, (sp, (g, sched)) in enumerate(zip(sims, graphs)):
-        ctx = new_ctx(g, sched, m, cap=4096, seed=sp["seed"])
-        for yi, d in enumerate(sched):
-            plain_edges += ctx.year_pa(1 + yi, yi, d, m, sp["n_nb"], sp["alpha"],
-                                       sp["tau"]).edges_added
-        assert state_hash(ctx.download()) == stepped[s], f"sim {s}: plain run differs"
-        ctx.close()
+        edges, h, draws = plain_run(sp, g, sched, m)
+        plain_edges += edges
+        if h != stepped[s]:
+            pytest.fail(diagnose_plain(s, sp, g, sched, m, draws, stepped[s]))
     assert plain_edges == stepped_edges
     log(f"plain runs agree: {plain_edges} edges")
+
+
+def plain_run(sp, g, sched, m, checked=False):
+    """One simulation as a plain context: (edges, final state hash, each
+    year's draws); with checked, every year against the oracle."""
+    ctx = new_ctx(g, sched, m, cap=4096, seed=sp["seed"])
+    edges, draws = 0, []
+    for yi, d in enumerate(sched):
+        if checked:
+            st, _ = check_pa_year(ctx, g.logfit, 1 + yi, yi, d, m, sp["n_nb"], sp["alpha"],
+                                  sp["tau"], sp["seed"], 4096)
+        else:
+            st = ctx.year_pa(1 + yi, yi, d, m, sp["n_nb"], sp["alpha"], sp["tau"])
+        edges += st.edges_added
+        draws.append(ctx.citations(d, m))
+    h = state_hash(ctx.download())
+    ctx.close()
+    return edges, h, draws
+
+
+def diagnose_plain(s, sp, g, sched, m, draws, want) -> str:
+    """A plain run whose final state differs from the ensemble's: a second
+    run, every year checked against the oracle, and the first year where the
+    two plain runs' draws part."""
+    msg = f"sim {s} {sp}: plain run differs from the ensemble"
+    try:
+        _, h2, draws2 = plain_run(sp, g, sched, m, checked=True)
+    except AssertionError as e:
+        return msg + f"; a second run, checked against the oracle year by year, fails too: {e}"
+    msg += "; a second run matches the oracle every year and " + \
+        ("equals" if h2 == want else "differs from") + " the ensemble"
+    for y, (a, b) in enumerate(zip(draws, draws2)):
+        bad = np.nonzero(np.any(a["cit"] != b["cit"], axis=1) | (a["cnt"] != b["cnt"]))[0]
+        if bad.size:
+            i = int(bad[0])
+            return msg + (f"; the first run's draws part from it in year {y} at {bad.size} "
+                          f"papers, e.g. paper {i}: {a['cit'][i].tolist()} kinds "
+                          f"{a['kind'][i].tolist()} against {b['cit'][i].tolist()}")
+    return msg + "; both runs' draws are equal every year (the difference is in the append)"
 
 
 def test_likelihood_30y_1m_seed():
