@@ -433,7 +433,19 @@ typedef struct {
                                    at j*B + b */
     void* stream;               /* as pg_lik_params */
     int32_t stream_given;
+    int32_t arith;              /* PG_LIK_ARITH_SIM or PG_LIK_ARITH_FAST */
 } pg_lik_batch_params;
+
+/* Arithmetic of a batched evaluation.  SIM: the weights are the simulator's
+ * own (det_expf, the bits the sequence was drawn with), log D and 1/D are
+ * the correctly rounded FP64 routines; each point equals pg_pa_loglik_grad
+ * up to FP64 summation order.  FAST: the hardware exponential, logarithm
+ * and reciprocal (weights within a few ulp, log D within about 1e-7, 1/D
+ * within an ulp after one Newton step); the partition sums stay FP64.  FAST
+ * agrees with pg_pa_loglik_grad to about 1e-7 relative and is bound by
+ * memory traffic rather than by the weight arithmetic. */
+#define PG_LIK_ARITH_SIM 0
+#define PG_LIK_ARITH_FAST 1
 
 typedef struct {
     uint32_t size;

@@ -21,6 +21,8 @@ ABM_PATH = os.path.join(_HERE, "abm")
 PG_MODEL_PA = 0
 PG_MODEL_ER = 1
 PG_MODEL_ABM = 2
+PG_LIK_ARITH_SIM = 0
+PG_LIK_ARITH_FAST = 1
 
 _ERRORS = {1: "EINVAL", 2: "ENOMEM", 3: "EHIP", 4: "ECAP", 5: "ESTATE", 6: "ENODEV", 7: "EIO"}
 
@@ -82,7 +84,7 @@ class LikBatchParams(ctypes.Structure):
     _fields_ = [("size", ctypes.c_uint32), ("n_points", ctypes.c_int32),
                 ("alpha_tau", ctypes.c_void_p), ("alpha_tau_dev", ctypes.c_void_p),
                 ("logfit_dev", ctypes.c_void_p), ("stream", ctypes.c_void_p),
-                ("stream_given", ctypes.c_int32)]
+                ("stream_given", ctypes.c_int32), ("arith", ctypes.c_int32)]
 
 
 class LikBatchOut(ctypes.Structure):
@@ -574,16 +576,19 @@ class Context:
 
     def pa_loglik_grad_batch(self, alpha_tau, logfit_dev: int, grad_dev: int, *,
                              alpha_tau_dev: int = 0, scalars_dev: int = 0,
-                             stream: int | None = None, host: bool = True):
+                             stream: int | None = None, host: bool = True, fast: bool = False):
         """log L and its gradient at B = len(alpha_tau) points in one pass.
         logfit_dev: device float[n_papers*B] (point b of paper j at j*B+b);
         grad_dev: device double[n_papers*B].  Returns a (B, 3) array of
-        {log L, dL/dalpha, dL/dtau} when host=True."""
+        {log L, dL/dalpha, dL/dtau} when host=True.  fast: the hardware
+        exp/log/reciprocal (PG_LIK_ARITH_FAST) instead of the simulator's
+        arithmetic."""
         at = np.ascontiguousarray(np.asarray(alpha_tau, np.float64).reshape(-1, 2))
         B = at.shape[0]
         p = LikBatchParams(ctypes.sizeof(LikBatchParams), B, at.ctypes.data,
                            alpha_tau_dev or None, logfit_dev or None, stream or None,
-                           0 if stream is None else 1)
+                           0 if stream is None else 1,
+                           PG_LIK_ARITH_FAST if fast else PG_LIK_ARITH_SIM)
         h = np.zeros((B, 3), np.float64)
         o = LikBatchOut(ctypes.sizeof(LikBatchOut), scalars_dev or None, grad_dev or None,
                         h.ctypes.data if host else None)

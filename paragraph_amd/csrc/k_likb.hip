@@ -25,6 +25,8 @@ namespace {
 constexpr int kBT = 256;          // threads per block, every pass
 constexpr int kWv = kBT / 64;
 constexpr int kYChunk = 8;        // paper passes: years per LDS reduction round
+constexpr int kPts = 4;           // paper passes: points per thread (at most)
+constexpr int kFwdRows = 8;       // forward paper pass: papers per thread
 
 struct PScal {
     float alpha_f, inv_tau_f;
@@ -41,10 +43,54 @@ __device__ inline PScal point_scal(const double* __restrict__ at, int b) {
     return s;
 }
 
+// The weight and its exponent x.  FAST = false: the simulator's arithmetic
+// (det_expf, the same bits as the weights the sequence was drawn with).
+// FAST = true: the hardware exponential, exp(x) = 2^(x*log2(e)) with the
+// product's rounding error and log2(e)'s low part folded back in to first
+// order (a few ulp of the exact weight).
+template <bool FAST>
 __device__ inline float weight_la(float L, float age, float phi, const PScal& s, float& x) {
     const float t1 = __builtin_fmaf(-age, s.inv_tau_f, phi);
     x = __builtin_fmaf(s.alpha_f, L, t1);
-    return det_expf(x);
+    if constexpr (FAST) {
+        constexpr float kL2E = 1.44269502162933349609375f;       // log2(e) rounded to float
+        constexpr float kL2ELo = 1.925963033500011079013348e-8f;  // log2(e) - kL2E
+        constexpr float kLn2 = 0.693147180559945309417232f;
+        const float y = x * kL2E;
+        const float r = __builtin_fmaf(x, kL2E, -y) + x * kL2ELo;
+        const float p = __builtin_amdgcn_exp2f(y);
+        return __builtin_fmaf(p, r * kLn2, p);
+    } else {
+        return det_expf(x);
+    }
+}
+
+// log D for D > 0.  FAST: the exponent times ln 2 plus log2 of the mantissa
+// (in [0.5, 1), rounded to float) by the hardware logarithm; absolute error
+// about 1e-7 per term.  Otherwise the simulator's det_log_pos.
+template <bool FAST>
+__device__ inline double log_pos(double D) {
+    if constexpr (FAST) {
+        constexpr double kLn2d = 0.6931471805599453094172321;
+        const double m = __builtin_amdgcn_frexp_mant(D);
+        const int e = __builtin_amdgcn_frexp_exp(D);
+        const float l2 = __builtin_amdgcn_logf((float)m);
+        return __builtin_fma((double)e, kLn2d, (double)l2 * kLn2d);
+    } else {
+        return det_log_pos(D);
+    }
+}
+
+// 1/D for D > 0.  FAST: the hardware reciprocal and one Newton step.
+template <bool FAST>
+__device__ inline double recip_pos(double D) {
+    if constexpr (FAST) {
+        const double r = __builtin_amdgcn_rcp(D);
+        const double e = __builtin_fma(-D, r, 1.0);
+        return __builtin_fma(r, e, r);
+    } else {
+        return 1.0 / D;
+    }
 }
 
 // Sum over the lanes of a wave that hold the same point (lane % B): the
@@ -83,48 +129,74 @@ __device__ inline double slot_share_b(int64_t j, int b, const int32_t* __restric
     return seg_off[j] == seg_off[j + 1] ? 0.0 : evt[j * B + b];
 }
 
-// Paper-major passes over tiles of PPB = kLikBPPT*kBT/B papers: block blk
-// takes tiles blk, blk + nblk, ...; in a tile, thread t holds point
-// b = t % B and papers j = tile*PPB + r*(kBT/B) + t/B (r < kLikBPPT).
+// Points per thread of the paper passes: a thread evaluates P points of each
+// of its papers, so a paper-year's log(k+1) load, age and mask serve P
+// weights; T = B/P threads share a paper.  Rows: papers per thread.  The
+// forward pass reduces every year's sums over the wavefront, so it takes
+// more papers per thread; the backward pass keeps per-paper gradients in
+// registers, so it takes fewer.
+template <int B, bool BWD>
+constexpr int papers_pts() { return B >= kPts ? kPts : B; }
+template <int B, bool BWD>
+constexpr int papers_rows() { return BWD ? kLikBPPT / papers_pts<B, BWD>() : kFwdRows; }
+
+// Paper-major passes over tiles of PPB = NR*kBT/T papers: block blk takes
+// tiles blk, blk + nblk, ...; in a tile, thread t holds the points
+// b0..b0+P-1 (b0 = (t % T)*P) of the papers j = tile*PPB + r*(kBT/T) + t/T
+// (r < NR).  Measured on the 30-year sequence at 16 points: 4 points by 8
+// papers per thread forward and 4 by 2 backward (424 and 545 us with the
+// fast arithmetic) against one point by 8 papers (624 and 731 us), 2 points
+// (486 and 594 us) and a 4-paper forward pass (505 us).
 //   BWD = false: part[(y*nblk + blk)*B + b] = sum of the block's w_j(y) at b
 //                (per-year totals kept in LDS across the block's tiles:
 //                dynamic shared memory of n_years*B doubles).
 //   BWD = true : grad[j*B + b] = slot shares - sum_y A_y w_j(y);
 //                part[(c*nblk + blk)*B + b], c = 0, 1: sums of A w L, A w age.
-// Every sum runs in a fixed order (tiles in order, waves in order).
-template <bool BWD, int B>
+// Every sum runs in a fixed order (tiles in order, rows of a thread in
+// order, lanes by fixed butterflies, waves in order).
+template <bool BWD, int B, bool FAST>
 __global__ __launch_bounds__(kBT) void k_likb_papers(LikSeq q, const double* __restrict__ at,
                                                      const float* __restrict__ logfit,
                                                      const double* __restrict__ year_sums,
                                                      LikSegs sg, double* __restrict__ grad,
                                                      double* __restrict__ part) {
-    constexpr int R = kBT / B;                     // papers per row
-    constexpr int PPB = kLikBPPT * R;
+    constexpr int P = papers_pts<B, BWD>();
+    constexpr int T = B / P;                       // threads per paper
+    constexpr int R = kBT / T;                     // papers per row
+    constexpr int NR = papers_rows<B, BWD>();      // rows (papers) per thread
+    constexpr int PPB = NR * R;
     __shared__ double s_red[kYChunk][kWv * B];
     extern __shared__ double s_tot[];              // [n_years * B] (forward pass)
-    const int b = threadIdx.x % B;
-    const int pr = threadIdx.x / B;
+    const int pg = threadIdx.x % T, b0 = pg * P;
+    const int pr = threadIdx.x / T;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const PScal s = point_scal(at, b);
+    PScal s[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) s[k] = point_scal(at, b0 + k);
     const int64_t nblk = gridDim.x;
     const int64_t ntile = (q.n_papers + PPB - 1) / PPB;
     if (!BWD)
         for (int k = threadIdx.x; k < q.n_years * B; k += kBT) s_tot[k] = 0.0;
-    double gL = 0.0, ga = 0.0;
+    double gL[P], ga[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) gL[k] = ga[k] = 0.0;
     // paper ids stay below 2^31 (32-bit index arithmetic); a paper that does
     // not exist in a year is computed like the others and masked off
     for (int64_t tile = blockIdx.x; tile < ntile; tile += nblk) {
         const int32_t j0 = (int32_t)(tile * PPB);
-        float phi[kLikBPPT];
-        int32_t bi[kLikBPPT];
-        double g[kLikBPPT];
+        float phi[NR][P];
+        int32_t bi[NR];
+        double g[NR][P];
 #pragma unroll
-        for (int r = 0; r < kLikBPPT; ++r) {
+        for (int r = 0; r < NR; ++r) {
             const int32_t j = j0 + r * R + pr;
             const bool in = j < (int32_t)q.n_papers;
-            phi[r] = in ? logfit[(uint32_t)j * B + b] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                phi[r][k] = in ? logfit[(uint32_t)j * B + b0 + k] : 0.0f;
+                g[r][k] = 0.0;
+            }
             bi[r] = in ? q.birth[j] : 0;
-            g[r] = 0.0;
         }
         for (int yc = 0; yc < q.n_years; yc += kYChunk) {
             const int ny = q.n_years - yc < kYChunk ? q.n_years - yc : kYChunk;
@@ -132,7 +204,7 @@ __global__ __launch_bounds__(kBT) void k_likb_papers(LikSeq q, const double* __r
 #pragma unroll
             for (int y4 = 0; y4 < kYChunk; y4 += 4) {
                 // four years of log(k+1) loads issued before any is used
-                float Lv[4][kLikBPPT];
+                float Lv[4][NR];
 #pragma unroll
                 for (int yy = 0; yy < 4; ++yy) {
                     const int y = yc + y4 + yy;
@@ -140,7 +212,7 @@ __global__ __launch_bounds__(kBT) void k_likb_papers(LikSeq q, const double* __r
                     const int32_t Ny = yin ? (int32_t)q.n_exist[y] : 0;
                     const float* Lrow = Ny > 0 ? q.Lpy + q.deg_off[y] : q.Lpy;
 #pragma unroll
-                    for (int r = 0; r < kLikBPPT; ++r) {
+                    for (int r = 0; r < NR; ++r) {
                         const int32_t j = j0 + r * R + pr;
                         Lv[yy][r] = Lrow[j < Ny ? j : 0];
                     }
@@ -152,28 +224,40 @@ __global__ __launch_bounds__(kBT) void k_likb_papers(LikSeq q, const double* __r
                     const int y = yc + yl;
                     const int32_t Ny = (int32_t)q.n_exist[y];
                     const int32_t year = q.year_of[y];
-                    const double Ay = BWD ? year_sums[(y * kLikYearCols + 1) * B + b] : 0.0;
-                    double zacc = 0.0;
+                    double Ay[P], zacc[P];
 #pragma unroll
-                    for (int r = 0; r < kLikBPPT; ++r) {
+                    for (int k = 0; k < P; ++k) {
+                        Ay[k] = BWD ? year_sums[(y * kLikYearCols + 1) * B + b0 + k] : 0.0;
+                        zacc[k] = 0.0;
+                    }
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) {
                         const int32_t j = j0 + r * R + pr;
                         const bool in = j < Ny;
                         const float L = Lv[yy][r];
                         const int32_t age = year - bi[r];
-                        float x;
-                        const double wv = (double)weight_la(L, (float)age, phi[r], s, x);
-                        if (BWD) {
-                            const double aw = in ? Ay * wv : 0.0;
-                            g[r] += aw;
-                            gL += aw * (double)L;
-                            ga += aw * (double)age;
-                        } else {
-                            zacc += in ? wv : 0.0;
+                        const float agef = (float)age;
+#pragma unroll
+                        for (int k = 0; k < P; ++k) {
+                            float x;
+                            const float wf = weight_la<FAST>(L, agef, phi[r][k], s[k], x);
+                            const double wv = (double)(in ? wf : 0.0f);
+                            if (BWD) {
+                                const double aw = in ? Ay[k] * wv : 0.0;
+                                g[r][k] += aw;
+                                gL[k] += aw * (double)L;
+                                ga[k] += aw * (double)age;
+                            } else {
+                                zacc[k] += wv;
+                            }
                         }
                     }
                     if (!BWD) {
-                        const double ws = wave_sum_b<B>(zacc);
-                        if (lane < B) s_red[yl][wave * B + lane] = ws;
+#pragma unroll
+                        for (int k = 0; k < P; ++k) {
+                            const double ws = wave_sum_b<T>(zacc[k]);
+                            if (lane < T) s_red[yl][wave * B + lane * P + k] = ws;
+                        }
                     }
                 }
             }
@@ -193,20 +277,27 @@ __global__ __launch_bounds__(kBT) void k_likb_papers(LikSeq q, const double* __r
         }
         if (BWD) {
 #pragma unroll
-            for (int r = 0; r < kLikBPPT; ++r) {
+            for (int r = 0; r < NR; ++r) {
                 const int32_t j = j0 + r * R + pr;
-                if (j < (int32_t)q.n_papers)
-                    grad[(size_t)j * B + b] = slot_share_b<B>(j, b, sg.seg_off, sg.evt) - g[r];
+                if (j < (int32_t)q.n_papers) {
+#pragma unroll
+                    for (int k = 0; k < P; ++k)
+                        grad[(size_t)j * B + b0 + k] =
+                            slot_share_b<B>(j, b0 + k, sg.seg_off, sg.evt) - g[r][k];
+                }
             }
         }
     }
     if (BWD) {
         __shared__ double red[kWv * B];
-        const double tL = block_sum_b<B>(gL, red);
-        const double tA = block_sum_b<B>(ga, red);
-        if (threadIdx.x < B) {
-            part[(0 * nblk + blockIdx.x) * B + threadIdx.x] = tL;
-            part[(1 * nblk + blockIdx.x) * B + threadIdx.x] = tA;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const double tL = block_sum_b<T>(gL[k], red);
+            const double tA = block_sum_b<T>(ga[k], red);
+            if (threadIdx.x < T) {
+                part[(0 * nblk + blockIdx.x) * B + threadIdx.x * P + k] = tL;
+                part[(1 * nblk + blockIdx.x) * B + threadIdx.x * P + k] = tA;
+            }
         }
     } else {
         __syncthreads();
@@ -238,7 +329,7 @@ __global__ __launch_bounds__(kBT) void k_likb_year_z(int64_t nblk, const double*
 // at share[pos*B + b].  Partials per block, column c at
 // part[(c*nblk + blk)*B + b]: 0 log L, 1 A, 2 alpha terms, 3 tau terms
 // (rounds in order, then the waves in order).
-template <int G, int K, int B>
+template <int G, int K, int B, bool FAST>
 __global__ __launch_bounds__(kBT) void k_likb_cite(LikSeq q, const double* __restrict__ at,
                                                    const float* __restrict__ logfit,
                                                    const int32_t* __restrict__ ev_block,
@@ -285,7 +376,7 @@ __global__ __launch_bounds__(kBT) void k_likb_cite(LikSeq q, const double* __res
             const bool ev = v && q.kind_t[qi] != 1;
             const float2 la = q.la_t[qi];
             float x;
-            const float w = weight_la(la.x, la.y, logfit[(uint32_t)t * B + b], s, x);
+            const float w = weight_la<FAST>(la.x, la.y, logfit[(uint32_t)t * B + b], s, x);
             wr[e] = v ? w : 0.0f;
             Lr[e] = la.x;
             ar[e] = la.y;
@@ -326,9 +417,9 @@ __global__ __launch_bounds__(kBT) void k_likb_cite(LikSeq q, const double* __res
             // D is a difference of sums of FP32 values: 0, negative (never
             // for a drawn pick) or at least 2^-149, a normal double, where
             // det_log_pos is det_log; the other two cases as det_log gives them
-            const double lgD = Dv > 0.0 ? det_log_pos(Dv)
+            const double lgD = Dv > 0.0 ? log_pos<FAST>(Dv)
                                         : (Dv == 0.0 ? -__builtin_huge_val() : __builtin_nan(""));
-            const double rD = 1.0 / Dv;
+            const double rD = Dv > 0.0 ? recip_pos<FAST>(Dv) : 1.0 / Dv;
             ll -= ev ? lgD : 0.0;
             inv[e] = ev ? rD : 0.0;
             tot += inv[e];
@@ -559,19 +650,19 @@ __global__ __launch_bounds__(kBT) void k_likb_final(int n_years, int64_t nblk_p,
     }
 }
 
-template <int B>
+template <int B, bool FAST>
 void run_batch(hipStream_t st, const LikSeq& q, const LikBatchBufs& bb, LikCiteShape sh) {
     const int64_t nP = bb.nblk_p, nE = bb.nblk_e;
     double* part_p = bb.part;
     double* part_e = bb.part + (int64_t)q.n_years * nP * B;
     double* part_b = part_e + 4 * nE * B;
     if (nP > 0)
-        k_likb_papers<false, B><<<(unsigned)nP, kBT, (size_t)q.n_years * B * sizeof(double), st>>>(
+        k_likb_papers<false, B, FAST><<<(unsigned)nP, kBT, (size_t)q.n_years * B * sizeof(double), st>>>(
             q, bb.at, bb.logfit, bb.year_sums, LikSegs{}, nullptr, part_p);
     if (q.n_years > 0) k_likb_year_z<B><<<q.n_years, kBT, 0, st>>>(nP, part_p, bb.year_sums);
     if (nE > 0) {
 #define PG_CITE(GG, KK)                                                                   \
-    k_likb_cite<GG, KK, B><<<(unsigned)nE, kBT, 0, st>>>(q, bb.at, bb.logfit, bb.ev_block, \
+    k_likb_cite<GG, KK, B, FAST><<<(unsigned)nE, kBT, 0, st>>>(q, bb.at, bb.logfit, bb.ev_block, \
                                                          bb.year_sums, bb.share, part_e)
 #define PG_CITE_K(GG)                                 \
     switch (sh.K) {                                   \
@@ -599,7 +690,7 @@ void run_batch(hipStream_t st, const LikSeq& q, const LikBatchBufs& bb, LikCiteS
         k_likb_bigsum<B><<<(unsigned)(((int64_t)bb.n_big * B + kBT - 1) / kBT), kBT, 0, st>>>(
             bb.n_big, bb.big_paper, bb.big_first, bb.bigpart, bb.evt);
     if (nP > 0)
-        k_likb_papers<true, B><<<(unsigned)nP, kBT, 0, st>>>(
+        k_likb_papers<true, B, FAST><<<(unsigned)nP, kBT, 0, st>>>(
             q, bb.at, bb.logfit, bb.year_sums, LikSegs{bb.seg_off, bb.evt, nullptr}, bb.grad,
             part_b);
     k_likb_final<B><<<1, kBT, 0, st>>>(q.n_years, nP, part_b, bb.at, bb.year_sums, bb.out,
@@ -634,11 +725,19 @@ bool lik_batch_shape_ok(int m, int B) {
 }
 
 void launch_lik_batch(hipStream_t st, int B, const LikSeq& q, const LikBatchBufs& bb,
-                      LikCiteShape sh) {
+                      LikCiteShape sh, bool fast) {
+    if (fast) {
+        switch (B) {
+            case 4: run_batch<4, true>(st, q, bb, sh); break;
+            case 8: run_batch<8, true>(st, q, bb, sh); break;
+            default: run_batch<16, true>(st, q, bb, sh); break;
+        }
+        return;
+    }
     switch (B) {
-        case 4: run_batch<4>(st, q, bb, sh); break;
-        case 8: run_batch<8>(st, q, bb, sh); break;
-        default: run_batch<16>(st, q, bb, sh); break;
+        case 4: run_batch<4, false>(st, q, bb, sh); break;
+        case 8: run_batch<8, false>(st, q, bb, sh); break;
+        default: run_batch<16, false>(st, q, bb, sh); break;
     }
 }
 

@@ -487,8 +487,9 @@ LikCiteShape lik_batch_cite_shape(int m, int B);         // G lanes per (paper, 
 int64_t lik_batch_paper_blocks(int64_t n_papers, int B);
 int lik_batch_small_job(int B);                            // positions of a whole-paper job
 constexpr int kLikBBigPiece = 4096;                        // positions of a larger paper's piece
-// The whole evaluation (both paper passes, slot pass, segment sums, final).
+// The whole evaluation (both paper passes, slot pass, segment sums, final);
+// fast: hardware exp/log/reciprocal instead of the simulator's arithmetic.
 void launch_lik_batch(hipStream_t s, int B, const LikSeq& q, const LikBatchBufs& bb,
-                      LikCiteShape sh);
+                      LikCiteShape sh, bool fast);
 
 }  // namespace pg

@@ -614,6 +614,8 @@ int pg_pa_loglik_grad_batch(pg_ctx* ctx, const pg_lik_batch_params* p, pg_lik_ba
                                         std::to_string(B) + " points (use fewer points)");
     if ((q.n_papers + 1) * (int64_t)B >= (int64_t(1) << 31))
         return fail(ctx, PG_ECAP, std::string(fn) + ": papers times points must stay below 2^31");
+    if (p->arith != PG_LIK_ARITH_SIM && p->arith != PG_LIK_ARITH_FAST)
+        return fail(ctx, PG_EINVAL, std::string(fn) + ": arith must be PG_LIK_ARITH_SIM or _FAST");
     if (!p->alpha_tau_dev) {
         if (!p->alpha_tau) return fail(ctx, PG_EINVAL, std::string(fn) + ": no alpha/tau given");
         for (int b = 0; b < B; ++b)
@@ -662,7 +664,7 @@ int pg_pa_loglik_grad_batch(pg_ctx* ctx, const pg_lik_batch_params* p, pg_lik_ba
     bb.grad = out->grad_logfit_dev;
     bb.out = q.b_out;
     bb.out2 = out->scalars_dev;
-    launch_lik_batch(s, B, seq_view(q), bb, q.b_shape);
+    launch_lik_batch(s, B, seq_view(q), bb, q.b_shape, p->arith == PG_LIK_ARITH_FAST);
     PG_HIP(hipGetLastError());
     if (prof) {
         PG_HIP(hipEventRecord(ctx->pev[PG_K_LIK_BATCH][1], s));

@@ -133,6 +133,51 @@ def test_batched_points_match_oracle(B, case):
         assert np.all(np.abs(g[:, b] - one[3]) <= 1e-10 * np.maximum(1.0, np.abs(one[3])))
 
 
+@pytest.mark.parametrize("B,case", [
+    (16, dict(n0=2000, years=3, rate=0.05, m=20, n_nb=6, alpha=1.2, tau=4.0, seed=5)),
+    (8, dict(n0=400, years=2, rate=0.1, m=40, n_nb=10, alpha=1.0, tau=6.0, seed=3)),
+    (4, dict(n0=500, years=4, rate=0.08, m=3, n_nb=0, alpha=0.7, tau=2.0, seed=11)),
+])
+def test_batched_fast_arith(B, case):
+    """PG_LIK_ARITH_FAST (hardware exp, log and reciprocal): every point
+    within 1e-5 of the FP64 oracle gradient (pinned by finite differences)
+    and of the single-point entry in the simulator's arithmetic (the plain
+    path), log L within 1e-6; repeated calls give the same bits."""
+    import torch
+    log("batched likelihood, fast arithmetic", B, case)
+    seq, birth, logfit, m = simulate(**case)
+    ctx = seq_ctx(seq, birth, m)
+    rng = np.random.default_rng(100 + B)
+    at = np.stack([rng.uniform(0.5, 1.5, B), rng.uniform(1.0, 10.0, B)], 1)
+    phis = [(logfit + (rng.standard_normal(logfit.size) * 0.1 if b else 0)).astype(np.float32)
+            for b in range(B)]
+    n = logfit.size
+    phi_d = torch.as_tensor(np.ascontiguousarray(np.stack(phis, 1).reshape(-1)), device="cuda")
+    grad_d = torch.empty(n * B, dtype=torch.float64, device="cuda")
+    h = ctx.pa_loglik_grad_batch(at, phi_d.data_ptr(), grad_d.data_ptr(), fast=True)
+    g = grad_d.cpu().numpy().reshape(n, B)
+    h2 = ctx.pa_loglik_grad_batch(at, phi_d.data_ptr(), grad_d.data_ptr(), fast=True)
+    assert np.array_equal(h, h2) and np.array_equal(g, grad_d.cpu().numpy().reshape(n, B))
+    worst = 0.0
+    for b in range(B):
+        a, t = float(at[b, 0]), float(at[b, 1])
+        d = oracle_eval(seq, birth, phis[b], m, a, t, w32=False)
+        one = gpu_eval(ctx, a, t, phis[b])
+        for ref in (d, one):
+            assert rel(h[b, 0], ref[0]) <= 1e-6, (b, h[b, 0], ref[0])
+            assert rel(h[b, 1], ref[1]) <= 1e-5, (b, h[b, 1], ref[1])
+            assert rel(h[b, 2], ref[2]) <= 1e-5, (b, h[b, 2], ref[2])
+            assert np.linalg.norm(g[:, b] - ref[3]) <= 1e-5 * np.linalg.norm(ref[3])
+        worst = max(worst, rel(h[b, 1], one[1]), rel(h[b, 2], one[2]),
+                    float(np.linalg.norm(g[:, b] - one[3]) / np.linalg.norm(one[3])))
+    log(f"  largest relative difference from the plain path: {worst:.2e}")
+    # a planted mistake must show: the simulator-arithmetic batch differs
+    # from the fast one, but by far less than the tolerance above
+    h_sim = ctx.pa_loglik_grad_batch(at, phi_d.data_ptr(), grad_d.data_ptr())
+    assert not np.array_equal(h_sim, h)
+    assert np.all(np.abs(h_sim - h) <= 1e-6 * np.maximum(1.0, np.abs(h_sim)))
+
+
 def test_batched_points_refused_shapes():
     import torch
     seq, birth, logfit, m = simulate(n0=400, years=2, rate=0.1, m=40, n_nb=10, seed=3)
@@ -145,6 +190,13 @@ def test_batched_points_refused_shapes():
         ctx.pa_loglik_grad_batch([(1.0, 5.0)] * 5, phi.data_ptr(), grad.data_ptr())
     with pytest.raises(pg.PGError, match="EINVAL"):     # tau <= 0
         ctx.pa_loglik_grad_batch([(1.0, 5.0)] * 3 + [(1.0, 0.0)], phi.data_ptr(), grad.data_ptr())
+    # an unknown arithmetic selector
+    import ctypes
+    at = np.array([1.0, 5.0] * 4, np.float64)
+    p = pg.LikBatchParams(ctypes.sizeof(pg.LikBatchParams), 4, at.ctypes.data, None,
+                          phi.data_ptr(), None, 0, 2)
+    o = pg.LikBatchOut(ctypes.sizeof(pg.LikBatchOut), None, grad.data_ptr(), None)
+    assert pg.lib().pg_pa_loglik_grad_batch(ctx._h, ctypes.byref(p), ctypes.byref(o)) == 1  # PG_EINVAL
 
 
 def test_uniform_closed_form_and_reproducible():
