@@ -77,6 +77,17 @@ def roofline(kernel: str, achieved: float, summary: str, group: str) -> dict:
             f"write requests by size per dispatch from profiles/{summary} (rocprofv3 --pmc)"}
 
 
+def state_digest(ctx) -> str:
+    """SHA-1 of the context's graph (both CSRs, in-degrees, births): every
+    timed step starts from the same snapshot, so all must agree."""
+    import hashlib
+    d = ctx.download()
+    h = hashlib.sha1()
+    for k in ("fwd_off", "fwd_adj", "bwd_off", "bwd_adj", "in_deg", "birth"):
+        h.update(np.ascontiguousarray(d[k]).tobytes())
+    return h.hexdigest()
+
+
 def eprint(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -904,6 +915,7 @@ def main():
     ctx.profile_groups(["bfs"])
     ctx.profile_reset()
     total_edges, total_near, total_s = 0, 0, 0.0
+    digests = []
     for k in range(args.steps):
         t1 = time.perf_counter()
         e, nr = one_step()
@@ -911,9 +923,12 @@ def main():
         total_edges += e
         total_near += nr
         total_s += t2 - t1
+        digests.append(state_digest(ctx))  # after the timed region
         eprint(f"step {k}: {e} edges in {t2 - t1:.3f} s ({e / (t2 - t1):.3e} edges/s), "
-               f"near-boundary attempts {nr}")
+               f"near-boundary attempts {nr}, state {digests[-1][:12]}")
         ctx.snapshot_restore()
+    if len(set(digests)) != 1:
+        eprint(f"WARNING: the timed steps ended in {len(set(digests))} different graphs")
     prof_timed = ctx.profile_read()
     ctx.profile_groups(None)
     ctx.profile_reset()
@@ -948,6 +963,7 @@ def main():
                                f"(14 PA + 6 neighbourhood), cap 4096, alpha=1, tau=5",
                    "edges_per_step": total_edges // max(args.steps, 1),
                    "near_boundary_attempts": total_near,
+                   "steps_identical_graphs": len(set(digests)) == 1,
                    "autotune": None if tune is None else {
                        "chosen": tune["chosen"], "chosen_ms": tune["chosen_ms"],
                        "default_ms": tune["default_ms"], "trials": len(tune["trials"])},
