@@ -321,19 +321,22 @@ __global__ __launch_bounds__(256) void k_sum_i32(const int32_t* __restrict__ p, 
     if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
 }
 
-// Bucket index of a CDF (see cdf_search): paper i <= n writes idx[k] = i for
-// the buckets k in (key(cdf[i-1]), key(cdf[i])] (i = 0 from bucket 0, i = n
-// up to kCdfIdx + 1), so every bucket is written once.  A thread per paper;
-// cdf[i-1] comes from the neighbouring lane.
+// Bucket index of a CDF (see cdf_search): idx[k] = the first paper i with
+// bucket(cdf[i]) >= k, n if none (k = 0 .. kCdfIdx + 1).  One thread per
+// bucket, a binary search over the CDF for its boundary: a heavy paper spans
+// many buckets, and a thread per paper writing its span made that paper's
+// thread the launch's tail.
 __global__ __launch_bounds__(256) void k_cdf_index(const double* __restrict__ cdf, int64_t n,
                                                    int32_t* __restrict__ idx) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > kCdfIdx + 1) return;
     const double kscale = cdf_key_scale(cdf[n - 1]);
-    const int kc = i < n ? cdf_key(cdf[i], kscale) : kCdfIdx + 1;
-    int kp = __shfl_up(kc, 1, 64);
-    if ((threadIdx.x & 63) == 0) kp = (i == 0 || i > n) ? -1 : cdf_key(cdf[i - 1], kscale);
-    if (i > n) return;
-    for (int k = kp + 1; k <= kc; ++k) idx[k] = (int32_t)i;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (cdf_key(cdf[mid], kscale) >= k) hi = mid; else lo = mid + 1;
+    }
+    idx[k] = (int32_t)lo;
 }
 
 __global__ void k_acc_i64(const int64_t* __restrict__ src, unsigned long long* dst) {
@@ -358,8 +361,8 @@ void launch_weights_cdf(hipStream_t s, int64_t n, const int32_t* in_deg, const i
     hipLaunchKernelGGL(k_cdf_down<false>, dim3((uint32_t)nb), dim3(kScanThreads), 0, s, none, n, w,
                        tile_scratch, cdf);
     if (cdf_idx)
-        hipLaunchKernelGGL(k_cdf_index, dim3((uint32_t)((n + 1 + 255) / 256)), dim3(256), 0, s, cdf,
-                           n, cdf_idx);
+        hipLaunchKernelGGL(k_cdf_index, dim3((uint32_t)((kCdfIdx + 2 + 255) / 256)), dim3(256), 0, s,
+                           cdf, n, cdf_idx);
 }
 
 void launch_weights_cdf_batch(hipStream_t s, const Batch& b, const int32_t* in_deg,
