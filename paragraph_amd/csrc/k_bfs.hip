@@ -53,6 +53,7 @@ constexpr int kMaxLong = 256;
 constexpr int kMaxBuckets = 2048;           // staged path: id buckets (in cursor[])
 constexpr int kMaxBucketShift = 17;         // a bucket must fit one bitmap window
 constexpr int kPiece = 256;                 // staged path: long-row part per wave step
+constexpr int kMaxBound = 4;                // radix path: rows longer than cap merged, not sorted
 
 struct BfsLds {
     alignas(16) int32_t f1[kBfsCapMax];       // capped 1-hop list (ascending)
@@ -62,6 +63,10 @@ struct BfsLds {
     int32_t lrow[kMaxLong];       // rows searched/copied by whole wavefronts
     int32_t lpre[kMaxLong + 1];   // staged path: long rows' first piece; bits 16-17 the row kind
     int32_t lbeg[kMaxLong];       // staged path: long rows' first entry
+    int32_t bnd_b[kMaxBound];     // radix path, level 2: rows longer than cap (first entry,
+    int32_t bnd_n[kMaxBound];     // length, kind: 0 forward, 1 backward)
+    int32_t bnd_k[kMaxBound];
+    int32_t nbnd;
     int64_t red[kWaves];
     int64_t tot[3];               // row entries per level
     int32_t nlong;
@@ -438,7 +443,8 @@ __device__ __forceinline__ int extract_bits_staged(const Query& q, uint32_t* bm,
 // wavefronts, ok telling the lanes that hold an entry.
 template <typename Visit>
 __device__ __forceinline__ void stage_rows(const Query& q, int nr, int cap, bool long_ok,
-                                           int nlong, int npieces, Visit visit) {
+                                           int nlong, int npieces, Visit visit,
+                                           bool skip_bound = false) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (q.level == 2)
         for (int k0 = 0; k0 < q.n1c; k0 += kBT) {
@@ -452,6 +458,7 @@ __device__ __forceinline__ void stage_rows(const Query& q, int nr, int cap, bool
         if (r < nr && !is_f1_row(q, r)) graph_row(q, r, p, b, e);
         int len = min(e - b, cap + 1);
         if (long_ok && e - b > kLongRow) len = 0;
+        if (skip_bound && e - b > cap) len = 0;  // merged afterwards (merge_emit)
         chunk_rows(p, b, len, [&](int32_t x, int, bool ok) { visit(x, ok); });
     }
     // kPP pieces per wavefront step: kPP * kPiece / 64 loads in flight per lane
@@ -589,9 +596,13 @@ __device__ __forceinline__ void bitonic_merges(int32_t (&v)[E], int lane) {
     if constexpr (K < 64 * E) bitonic_merges<E, 2 * K>(v, lane);
 }
 
-template <int E>
+// OPQ: the lane index is opaque, so the per-lane bounds are computed where
+// they are used; otherwise the compiler hoists them to the kernel's start,
+// which in the workgroup kernel (128 VGPRs, two call sites) spills them.
+template <int E, bool OPQ = false>
 __device__ __forceinline__ void wave_sort_reg(int32_t* s) {
-    const int lane = threadIdx.x & 63;
+    int lane = (int)threadIdx.x & 63;
+    if constexpr (OPQ) asm volatile("" : "+v"(lane));
     int32_t v[E];
     int32_t* mine = s + lane * E;
     wave_lds_sync();
@@ -648,7 +659,7 @@ __device__ __forceinline__ int wave_unique_to(const int32_t* s, int n, int limit
 // x >> hsh.  All threads call it.
 __device__ __forceinline__ int radix_gather(const Query& q, int nr, int cap, bool long_ok, int nlong,
                                             int npieces, int32_t lim, int32_t* A, int32_t* H,
-                                            int hsh) {
+                                            int hsh, bool skip_bound) {
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1;
     if (threadIdx.x == 0) L.fill = 0;
@@ -663,7 +674,7 @@ __device__ __forceinline__ int radix_gather(const Query& q, int nr, int cap, boo
         const int at = base + __popcll(m & lt);
         if (keep && at < kRadixMax) A[at] = x;
         if (H) run_add<false>(H, keep ? x >> hsh : -1);
-    });
+    }, skip_bound);
     __syncthreads();
     return L.fill;
 }
@@ -674,9 +685,12 @@ __device__ __forceinline__ int radix_gather(const Query& q, int nr, int cap, boo
 // (K = ceil(M / kBT)), round r at lane l entry w*64K + 64r + l; an entry's
 // rank among its wavefront's entries of the same digit comes from eight
 // ballots (the lanes with equal digits) and a per-wavefront counter, the
-// counters (16-bit, digit-major over the wavefronts, two per word of C[0,
-// kBT*2)) are scanned, and each entry is stored at its digit's offset plus
-// its rank.  All threads call it.
+// counters are scanned in (digit, wavefront) order, and each entry is
+// stored at its digit's offset plus its rank.  The counters are 16-bit, two
+// per word of C[0, kBT*2), stored wavefront-major (wavefront w's counter
+// for digit d at half d & 1 of word 128w + d/2): a wavefront's lanes then
+// spread over 128 consecutive words, where digit-major storage (word
+// 4d + w/2) put all of them on one LDS bank in four.  All threads call it.
 __device__ __forceinline__ int32_t* radix_sort(int32_t* A, int32_t* B, uint32_t* C, int M,
                                                int bits) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -702,24 +716,37 @@ __device__ __forceinline__ int32_t* radix_sort(int32_t* A, int32_t* B, uint32_t*
                     const uint64_t mb = __ballot(bit);
                     peers &= bit ? mb : ~mb;
                 }
-                const int ci = (int)d * kWaves + wid;
-                const int hs = (ci & 1) * 16;
-                const int before = (int)((C[ci >> 1] >> hs) & 0xFFFFu);
+                const int wi = wid * 128 + (int)(d >> 1);
+                const int hs = (int)(d & 1u) * 16;
+                const int before = (int)((C[wi] >> hs) & 0xFFFFu);
                 rk[r] = before + __popcll(peers & lt);
                 if (valid && (peers & lt) == 0)
-                    atomicAdd(&C[ci >> 1], (uint32_t)__popcll(peers) << hs);
+                    atomicAdd(&C[wi], (uint32_t)__popcll(peers) << hs);
             }
         }
         __syncthreads();
         {
-            const uint32_t a = C[2 * threadIdx.x], b = C[2 * threadIdx.x + 1];
-            const int c0 = (int)(a & 0xFFFFu), c1 = (int)(a >> 16);
-            const int c2 = (int)(b & 0xFFFFu), c3 = (int)(b >> 16);
+            // thread t scans digit t/2's counters of wavefronts 4(t&1) to
+            // 4(t&1)+3 (all in one half of their words); the thread index
+            // is opaque so the four addresses are not hoisted and spilled
+            int tid = (int)threadIdx.x;
+            asm volatile("" : "+v"(tid));
+            const int dg = tid >> 1, w0c = (tid & 1) * 4;
+            const int hs = (dg & 1) * 16;
+            int c[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) c[k] = (int)((C[(w0c + k) * 128 + (dg >> 1)] >> hs) & 0xFFFFu);
             int tot;
-            const int p0 = block_excl_scan32(c0 + c1 + c2 + c3, &tot);
-            const int p1 = p0 + c0, p2 = p1 + c1, p3 = p2 + c2;
-            C[2 * threadIdx.x] = (uint32_t)p0 | ((uint32_t)p1 << 16);
-            C[2 * threadIdx.x + 1] = (uint32_t)p2 | ((uint32_t)p3 << 16);
+            int pk = block_excl_scan32(c[0] + c[1] + c[2] + c[3], &tot);
+            // a word's two digits belong to threads t and t ^ 2 (one
+            // wavefront, which alone reads the word): the even digit's
+            // thread writes both offsets
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int other = __shfl_xor(pk, 2, 64);
+                if (!hs) C[(w0c + k) * 128 + (dg >> 1)] = (uint32_t)pk | ((uint32_t)other << 16);
+                pk += c[k];
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -727,8 +754,8 @@ __device__ __forceinline__ int32_t* radix_sort(int32_t* A, int32_t* B, uint32_t*
             const int idx = w0 + r * 64 + lane;
             if (r < K && idx < M) {
                 const int32_t key = A[idx];
-                const int ci = (int)(((uint32_t)key >> shift) & 255u) * kWaves + wid;
-                B[(int)((C[ci >> 1] >> ((ci & 1) * 16)) & 0xFFFFu) + rk[r]] = key;
+                const uint32_t d = ((uint32_t)key >> shift) & 255u;
+                B[(int)((C[wid * 128 + (int)(d >> 1)] >> ((d & 1u) * 16)) & 0xFFFFu) + rk[r]] = key;
             }
         }
         __syncthreads();
@@ -740,9 +767,11 @@ __device__ __forceinline__ int32_t* radix_sort(int32_t* A, int32_t* B, uint32_t*
 }
 
 // Radix path, step 3: the distinct kept ids of the sorted Z[0,M), the
-// first cap of them to the output; returns how many there are (uncapped).
-// Each thread takes a contiguous run.  All threads call it.
-__device__ __forceinline__ int radix_emit(const Query& q, const int32_t* Z, int M, int cap) {
+// first cap of them to the output (or to dst in LDS, another array);
+// returns how many there are (uncapped).  Each thread takes a contiguous
+// run.  All threads call it.
+__device__ __forceinline__ int radix_emit(const Query& q, const int32_t* Z, int M, int cap,
+                                          int32_t* dst = nullptr) {
     const int chunk = (M + kBT - 1) / kBT;
     const int s0 = min(M, (int)threadIdx.x * chunk);
     const int s1 = min(M, s0 + chunk);
@@ -755,10 +784,110 @@ __device__ __forceinline__ int radix_emit(const Query& q, const int32_t* Z, int 
     int pos = block_excl_scan32(mine, &tot);
     for (int k = s0; k < s1 && pos < cap; ++k) {
         const int32_t x = Z[k];
-        if ((k == 0 || Z[k - 1] != x) && keep_id(q, x)) put_out(q, pos++, x);
+        if ((k == 0 || Z[k - 1] != x) && keep_id(q, x)) {
+            if (dst) dst[pos] = x; else put_out(q, pos, x);
+            ++pos;
+        }
     }
     __syncthreads();  // the output is complete
     return tot;
+}
+
+// One step of merge_bound: Y[0, min(result, cap)) = the distinct kept ids
+// of the ascending X[0,np) and F[0,nf) together; returns their number
+// (uncapped).  Each thread takes a contiguous run of the merged order,
+// found by a merge-path search (X first among equal entries), walks it
+// once to count and once to write.  All threads call it.
+__device__ __forceinline__ int merge_step(const Query& q, const int32_t* X, int np,
+                                          const int32_t* F, int nf, int cap, int32_t* Y) {
+    const int T = np + nf;
+    const int per = (T + kBT - 1) / kBT;
+    const int d0 = min(T, (int)threadIdx.x * per), d1 = min(T, d0 + per);
+    int a = max(0, d0 - nf), z = min(d0, np);
+    while (a < z) {
+        const int mid = (a + z) >> 1;
+        if (X[mid] <= F[d0 - 1 - mid]) a = mid + 1; else z = mid;
+    }
+    const int i0 = a, j0 = d0 - a;
+    // the merged entry before the run: the larger of the last ones taken
+    const int32_t p0 = max(i0 > 0 ? X[i0 - 1] : INT_MIN, j0 > 0 ? F[j0 - 1] : INT_MIN);
+    int i = i0, j = j0, mine = 0;
+    int32_t prev = p0;
+    for (int d = d0; d < d1; ++d) {
+        const bool tx = j >= nf || (i < np && X[i] <= F[j]);
+        const int32_t x = tx ? X[i] : F[j];
+        i += tx; j += !tx;
+        if (x != prev && keep_id(q, x)) ++mine;
+        prev = x;
+    }
+    int tot;
+    int pos = block_excl_scan32(mine, &tot);
+    i = i0; j = j0; prev = p0;
+    for (int d = d0; d < d1 && pos < cap; ++d) {
+        const bool tx = j >= nf || (i < np && X[i] <= F[j]);
+        const int32_t x = tx ? X[i] : F[j];
+        i += tx; j += !tx;
+        if (x != prev && keep_id(q, x)) Y[pos++] = x;
+        prev = x;
+    }
+    __syncthreads();  // Y is complete
+    return tot;
+}
+
+// Radix path at level 2 with the rows longer than cap (L.bnd_*, nbnd of
+// them) left out of the gather: the distinct kept ids of the sorted short
+// entries Z[0,M) (in L.win or L.cursor), then each long row's entries below
+// the bound merged in; the first cap to q.S; returns their number (capped).
+// A row longer than cap is ascending in memory, so it needs no sort, and
+// only its entries below the bound (at most cap+1) are read.  The running
+// list alternates between L.win[0,cap) and L.cursor[0,cap); a row's entries
+// are staged at L.cursor[kBfsCapMax, kBfsCapMax + cap + 1).  All threads
+// call it.
+static_assert(2 * kBfsCapMax + 1 <= kMaxRows && kBfsCapMax <= kWin, "merge_bound buffers");
+__device__ __forceinline__ int merge_bound(const Query& q, const int32_t* Z, int M, int cap,
+                                           int32_t lim, int nbnd) {
+    int32_t* X = (Z == L.win) ? L.cursor : L.win;
+    int32_t* Y = (Z == L.win) ? L.win : L.cursor;
+    int32_t* F = L.cursor + kBfsCapMax;
+    // a row's entries are loaded into registers one row ahead, so the loads
+    // are in flight during the previous row's merge
+    constexpr int kFP = (kBfsCapMax + kBT) / kBT;  // cap + 1 entries per row at most
+    int32_t fv[kFP];
+    auto fetch = [&](int k) {
+        g_cint* p = GROW(L.bnd_k[k] ? q.g.bwd_adj : q.g.fwd_adj) + L.bnd_b[k];
+        const int len = L.bnd_n[k];
+#pragma unroll
+        for (int u = 0; u < kFP; ++u) {
+            const int t = (int)threadIdx.x + u * kBT;
+            fv[u] = t < len ? p[t] : 0;
+        }
+    };
+    fetch(0);
+    int np = min(radix_emit(q, Z, M, cap, X), cap);
+    for (int k = 0; k < nbnd; ++k) {
+        // ids above the cap-th of the running list cannot enter it
+        const int32_t hi = np == cap ? min(lim, X[cap - 1]) : lim;
+        const int len = L.bnd_n[k];
+#pragma unroll
+        for (int u = 0; u < kFP; ++u) {
+            const int t = (int)threadIdx.x + u * kBT;
+            if (t < len) F[t] = fv[u];
+        }
+        __syncthreads();
+        if (k + 1 < nbnd) fetch(k + 1);
+        int a = 0, z = len;  // entries below hi
+        while (a < z) {
+            const int mid = (a + z) >> 1;
+            if (F[mid] < hi) a = mid + 1; else z = mid;
+        }
+        np = min(merge_step(q, X, np, F, a, cap, Y), cap);
+        int32_t* t = X;
+        X = Y;
+        Y = t;
+    }
+    for (int t = threadIdx.x; t < np; t += kBT) q.S[t] = X[t];
+    __syncthreads();  // the output is complete
+    return np;
 }
 
 // Collects the `cap` smallest distinct kept ids of the level's rows into the
@@ -792,10 +921,10 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
                 for (int k = lane; k < P; k += 64)
                     L.win[k] = k < df ? fa[fb + k] : (k < d1 ? ba[bb + k - df] : INT_MAX);
                 switch (P) {
-                    case 64: wave_sort_reg<1>(L.win); break;
-                    case 128: wave_sort_reg<2>(L.win); break;
-                    case 256: wave_sort_reg<4>(L.win); break;
-                    default: wave_sort_reg<8>(L.win); break;
+                    case 64: wave_sort_reg<1, true>(L.win); break;
+                    case 128: wave_sort_reg<2, true>(L.win); break;
+                    case 256: wave_sort_reg<4, true>(L.win); break;
+                    default: wave_sort_reg<8, true>(L.win); break;
                 }
                 const int n1 = wave_unique_to(L.win, d1, cap, q.root, L.f1);
                 if (lane == 0) {
@@ -807,7 +936,7 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
             return L.nout;
         }
     }
-    if (threadIdx.x == 0) L.nlong = 0;
+    if (threadIdx.x == 0) { L.nlong = 0; L.nbnd = 0; }
     __syncthreads();
     // row entries (bits 26 and up) and entries cut at cap+1 per row (bits
     // 0-25: at most kMaxRows * (kBfsCapMax + 1) < 2^26)
@@ -822,7 +951,17 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
         // a row with more than cap entries holds at least cap kept ids (only
         // the root can be dropped from it) among its first cap+1, so no id
         // above its entry [cap] can be in the answer
-        if (e - b > cap) tb = min(tb, p[b + cap]);
+        if (e - b > cap) {
+            tb = min(tb, p[b + cap]);
+            if (q.level == 2) {
+                const int s = atomicAdd(&L.nbnd, 1);
+                if (s < kMaxBound) {
+                    L.bnd_b[s] = b;
+                    L.bnd_n[s] = cap + 1;
+                    L.bnd_k[s] = (r - 1) & 1;
+                }
+            }
+        }
         if (e - b > kLongRow) {
             const int s = atomicAdd(&L.nlong, 1);
             if (s < kMaxLong) L.lrow[s] = r;
@@ -851,10 +990,15 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
     if (total == 0) return 0;
     const bool long_ok = L.nlong >= 0;
     const int nlong = long_ok ? L.nlong : 0;
+    const bool radix = total <= kSortMax || q.st.radix;
+    // level 2 on the radix path: rows longer than cap (at most kMaxBound of
+    // them) are merged into the sorted short entries, not gathered
+    const int nbnd = L.nbnd;
+    const bool merge = radix && q.level == 2 && nbnd > 0 && nbnd <= kMaxBound;
     // long rows in pieces of kPiece entries: first piece, kind and length
-    // per long row (L.lpre), first entry (L.lbeg)
-    int npieces;
-    {
+    // per long row (L.lpre), first entry (L.lbeg); with skip, the rows
+    // longer than cap get none
+    auto set_pieces = [&](bool skip) {
         // (an opaque copy of the thread index: left visible, the LDS
         // addresses below are hoisted out of the level loop and spilled)
         int tid = (int)threadIdx.x;
@@ -864,7 +1008,7 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
             const int r = L.lrow[tid];
             g_cint* p; int b, e;
             graph_row(q, r, p, b, e);
-            ln = min(e - b, cap + 1);
+            ln = (skip && e - b > cap) ? 0 : min(e - b, cap + 1);
             cnt = (ln + kPiece - 1) / kPiece;
             pk = q.level == 1 ? r : (r - 1) & 1;
             bb = b;
@@ -875,24 +1019,28 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
             L.lpre[tid] = pre | (pk << 16) | (ln << 18);
             L.lbeg[tid] = bb;
         }
-        npieces = (int)tp;
-    }
-    if (total <= kSortMax || q.st.radix) {
+        return (int)tp;
+    };
+    int npieces = set_pieces(merge);
+    if (radix) {
         // radix path: the entries below the bound gathered into L.win and
         // sorted there.  When more than kRadixMax of them, a histogram of
         // the first gather gives a lower bound holding cap + cap/4 entries
         // and they are gathered again below it; kept ids short of cap
-        // below such a bound (duplicates) leave it to the paths below.
+        // below such a bound (duplicates) leave it to the paths below.  In
+        // merge mode only the short rows' entries are gathered and sorted
+        // (merge_bound adds the long rows').
         int32_t* A = L.win;
         int32_t* B = L.cursor;
         uint32_t* C = reinterpret_cast<uint32_t*>(L.cursor + kRadixMax);
         int32_t lim = (int32_t)span;
-        const bool hist = total_cut > kRadixMax;
+        const bool hist = total_cut - (merge ? (int64_t)nbnd * (cap + 1) : 0) > kRadixMax;
         int hsh = 0;
         while (((lim - 1) >> hsh) >= kHistBuckets) ++hsh;
         if (hist)
             for (int k = threadIdx.x; k < kHistBuckets; k += kBT) B[k] = 0;
-        int M = radix_gather(q, nr, cap, long_ok, nlong, npieces, lim, A, hist ? B : nullptr, hsh);
+        int M = radix_gather(q, nr, cap, long_ok, nlong, npieces, lim, A, hist ? B : nullptr, hsh,
+                             merge);
         bool cut = false;
         if (M > kRadixMax) {
             static_assert(kHistBuckets == 2 * kBT, "two buckets per thread");
@@ -910,7 +1058,7 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
             if (cb >= 0 && L.cut_n <= kRadixMax) {
                 lim = (int32_t)min((int64_t)lim, (int64_t)(cb + 1) << hsh);
                 cut = true;
-                M = radix_gather(q, nr, cap, long_ok, nlong, npieces, lim, A, nullptr, 0);
+                M = radix_gather(q, nr, cap, long_ok, nlong, npieces, lim, A, nullptr, 0, merge);
             }
         }
         if (M <= kRadixMax) {
@@ -923,10 +1071,10 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
                 __syncthreads();
                 if (wid == 0) {
                     switch (P) {
-                        case 64: wave_sort_reg<1>(A); break;
-                        case 128: wave_sort_reg<2>(A); break;
-                        case 256: wave_sort_reg<4>(A); break;
-                        default: wave_sort_reg<8>(A); break;
+                        case 64: wave_sort_reg<1, true>(A); break;
+                        case 128: wave_sort_reg<2, true>(A); break;
+                        case 256: wave_sort_reg<4, true>(A); break;
+                        default: wave_sort_reg<8, true>(A); break;
                     }
                 }
                 __syncthreads();
@@ -934,9 +1082,15 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
                 const int bits = 32 - __builtin_clz((uint32_t)max(lim - 1, 1));
                 Z = radix_sort(A, B, C, M, bits);
             }
-            const int kept = radix_emit(q, Z, M, cap);
-            if (!cut || kept >= cap) return min(kept, cap);
+            if (merge) {
+                const int kept = merge_bound(q, Z, M, cap, lim, nbnd);
+                if (!cut || kept >= cap) return kept;
+            } else {
+                const int kept = radix_emit(q, Z, M, cap);
+                if (!cut || kept >= cap) return min(kept, cap);
+            }
         }
+        if (merge) npieces = set_pieces(false);
         // the sweep's row cursors again
         for (int r = threadIdx.x; r < nr; r += kBT) {
             g_cint* p; int b, e;

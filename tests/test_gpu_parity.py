@@ -307,3 +307,48 @@ def test_bfs_stage_overflow():
     for r, s in zip(roots, ctx.bfs(roots)):
         ref = ob.bfs(G, int(r), 4096)
         assert np.array_equal(s, ref), f"root {r}: {s[:8]} vs {ref[:8]} ({s.size} vs {ref.size})"
+
+def long_row_graph(hubs, mids, shared, n=60_000, hub_deg=6000, mid_deg=400, seed=21):
+    """Paper n-1 cites `hubs` hubs (ids 0..hubs-1, ~hub_deg citers each) and
+    `mids` papers after them (~mid_deg citers each).  With `shared` the hubs'
+    citers come from one pool and the mids' from a small one, so the rows
+    repeat each other's ids."""
+    rng = np.random.default_rng(seed + 7 * hubs + mids)
+    lo = hubs + mids
+    pool_h = rng.choice(np.arange(lo, n - 1), size=int(hub_deg * 1.25), replace=False)
+    pool_m = rng.choice(np.arange(lo, n - 1), size=3 * mid_deg, replace=False)
+    src, dst = [], []
+    for h in range(lo):
+        deg = hub_deg if h < hubs else mid_deg
+        if shared:
+            cit = rng.choice(pool_h if h < hubs else pool_m, size=deg, replace=False)
+        else:
+            cit = rng.choice(np.arange(lo, n - 1), size=deg, replace=False)
+        src.append(np.sort(cit)); dst.append(np.full(deg, h))
+    src.append(np.full(lo, n - 1)); dst.append(np.arange(lo))
+    for i in range(lo + 1, n - 1, 5):   # some ordinary citations too
+        src.append(np.array([i])); dst.append(np.array([rng.integers(lo, i)]))
+    src = np.concatenate(src).astype(np.int32)
+    dst = np.concatenate(dst).astype(np.int32)
+    return pg.SeedGraph(np.zeros(n, np.int32), src, dst, np.zeros(n + 10, np.float32))
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("cap", [4096, 1000, 64])
+@pytest.mark.parametrize("hubs,mids,shared", [(1, 0, False), (3, 0, True), (4, 0, False),
+                                              (5, 0, False), (4, 40, False), (4, 40, True)])
+def test_bfs_long_rows_merged(hubs, mids, shared, cap, mode):
+    """Level-2 lists holding rows longer than the cap: on the in-LDS sort
+    path up to 4 such rows are merged into the sorted short entries (one,
+    several with repeated ids, exactly 4; 5 take the plain sort), also
+    when the short entries alone need the histogram cut, and when that cut
+    keeps fewer than cap distinct ids (repeated ids) and hands the list on."""
+    g = long_row_graph(hubs, mids, shared)
+    ctx = new_ctx(g, [1], 1, cap=cap)
+    ctx.set_tuning(bfs_mode=mode)
+    G = oracle_graph(ctx, g.logfit)
+    roots = np.array([g.n - 1, 0, hubs, hubs + mids + 3, 4321], np.int32)
+    log(f"long-row BFS hubs={hubs} mids={mids} shared={shared} cap={cap} mode={mode}")
+    for r, s in zip(roots, ctx.bfs(roots)):
+        ref = ob.bfs(G, int(r), cap)
+        assert np.array_equal(s, ref), f"root {r}: {s[:8]} vs {ref[:8]} ({s.size} vs {ref.size})"
