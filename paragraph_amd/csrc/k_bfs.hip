@@ -444,7 +444,7 @@ __device__ __forceinline__ int extract_bits_staged(const Query& q, uint32_t* bm,
 template <typename Visit>
 __device__ __forceinline__ void stage_rows(const Query& q, int nr, int cap, bool long_ok,
                                            int nlong, int npieces, Visit visit,
-                                           bool skip_bound = false) {
+                                           int skip_over = INT_MAX) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (q.level == 2)
         for (int k0 = 0; k0 < q.n1c; k0 += kBT) {
@@ -458,7 +458,7 @@ __device__ __forceinline__ void stage_rows(const Query& q, int nr, int cap, bool
         if (r < nr && !is_f1_row(q, r)) graph_row(q, r, p, b, e);
         int len = min(e - b, cap + 1);
         if (long_ok && e - b > kLongRow) len = 0;
-        if (skip_bound && e - b > cap) len = 0;  // merged afterwards (merge_emit)
+        if (e - b > skip_over) len = 0;  // merged afterwards (merge_bound)
         chunk_rows(p, b, len, [&](int32_t x, int, bool ok) { visit(x, ok); });
     }
     // kPP pieces per wavefront step: kPP * kPiece / 64 loads in flight per lane
@@ -659,7 +659,7 @@ __device__ __forceinline__ int wave_unique_to(const int32_t* s, int n, int limit
 // x >> hsh.  All threads call it.
 __device__ __forceinline__ int radix_gather(const Query& q, int nr, int cap, bool long_ok, int nlong,
                                             int npieces, int32_t lim, int32_t* A, int32_t* H,
-                                            int hsh, bool skip_bound) {
+                                            int hsh, int skip_over) {
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1;
     if (threadIdx.x == 0) L.fill = 0;
@@ -674,7 +674,7 @@ __device__ __forceinline__ int radix_gather(const Query& q, int nr, int cap, boo
         const int at = base + __popcll(m & lt);
         if (keep && at < kRadixMax) A[at] = x;
         if (H) run_add<false>(H, keep ? x >> hsh : -1);
-    }, skip_bound);
+    }, skip_over);
     __syncthreads();
     return L.fill;
 }
@@ -834,12 +834,13 @@ __device__ __forceinline__ int merge_step(const Query& q, const int32_t* X, int 
     return tot;
 }
 
-// Radix path at level 2 with the rows longer than cap (L.bnd_*, nbnd of
-// them) left out of the gather: the distinct kept ids of the sorted short
-// entries Z[0,M) (in L.win or L.cursor), then each long row's entries below
-// the bound merged in; the first cap to q.S; returns their number (capped).
-// A row longer than cap is ascending in memory, so it needs no sort, and
-// only its entries below the bound (at most cap+1) are read.  The running
+// Radix path with the long rows (L.bnd_*, nbnd of them: at level 2 those
+// longer than cap, at level 1 those above kWaveSortMax) left out of the
+// gather: the distinct kept ids of the sorted other entries Z[0,M) (in
+// L.win or L.cursor), then each long row's entries below the bound merged
+// in; the first cap to the output; returns their number (capped).  A row
+// is ascending in memory, so it needs no sort, and only its entries below
+// the bound among its first cap+1 are read.  The running
 // list alternates between L.win[0,cap) and L.cursor[0,cap); a row's entries
 // are staged at L.cursor[kBfsCapMax, kBfsCapMax + cap + 1).  All threads
 // call it.
@@ -885,7 +886,7 @@ __device__ __forceinline__ int merge_bound(const Query& q, const int32_t* Z, int
         X = Y;
         Y = t;
     }
-    for (int t = threadIdx.x; t < np; t += kBT) q.S[t] = X[t];
+    for (int t = threadIdx.x; t < np; t += kBT) put_out(q, t, X[t]);
     __syncthreads();  // the output is complete
     return np;
 }
@@ -942,6 +943,11 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
     // 0-25: at most kMaxRows * (kBfsCapMax + 1) < 2^26)
     int64_t mine = 0;
     int32_t tb = INT_MAX;
+    // rows longer than mthr are merged (merge_bound) when there are at most
+    // kMaxBound of them: at level 2 those longer than cap (they bound the
+    // answer); at level 1 (two rows, more than kWaveSortMax entries
+    // together) any row above kWaveSortMax
+    const int mthr = q.level == 1 ? kWaveSortMax : cap;
     for (int r = threadIdx.x; r < nr; r += kBT) {
         g_cint* p; int b, e;
         row_bounds(q, r, p, b, e);
@@ -951,15 +957,13 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
         // a row with more than cap entries holds at least cap kept ids (only
         // the root can be dropped from it) among its first cap+1, so no id
         // above its entry [cap] can be in the answer
-        if (e - b > cap) {
-            tb = min(tb, p[b + cap]);
-            if (q.level == 2) {
-                const int s = atomicAdd(&L.nbnd, 1);
-                if (s < kMaxBound) {
-                    L.bnd_b[s] = b;
-                    L.bnd_n[s] = cap + 1;
-                    L.bnd_k[s] = (r - 1) & 1;
-                }
+        if (e - b > cap) tb = min(tb, p[b + cap]);
+        if (e - b > mthr) {
+            const int s = atomicAdd(&L.nbnd, 1);
+            if (s < kMaxBound) {
+                L.bnd_b[s] = b;
+                L.bnd_n[s] = min(e - b, cap + 1);
+                L.bnd_k[s] = q.level == 1 ? r : (r - 1) & 1;
             }
         }
         if (e - b > kLongRow) {
@@ -991,10 +995,15 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
     const bool long_ok = L.nlong >= 0;
     const int nlong = long_ok ? L.nlong : 0;
     const bool radix = total <= kSortMax || q.st.radix;
-    // level 2 on the radix path: rows longer than cap (at most kMaxBound of
-    // them) are merged into the sorted short entries, not gathered
+    // the rows longer than mthr (at most kMaxBound of them) are merged into
+    // the sorted other entries, not gathered: at level 1 always, at level 2
+    // on the radix path
     const int nbnd = L.nbnd;
-    const bool merge = radix && q.level == 2 && nbnd > 0 && nbnd <= kMaxBound;
+    const bool merge = (q.level == 1 || radix) && nbnd > 0 && nbnd <= kMaxBound;
+    const int skip_over = merge ? mthr : INT_MAX;
+    int64_t merged_cut = 0;  // their entries, cut at cap+1
+    if (merge)
+        for (int k = 0; k < nbnd; ++k) merged_cut += L.bnd_n[k];
     // long rows in pieces of kPiece entries: first piece, kind and length
     // per long row (L.lpre), first entry (L.lbeg); with skip, the rows
     // longer than cap get none
@@ -1008,7 +1017,7 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
             const int r = L.lrow[tid];
             g_cint* p; int b, e;
             graph_row(q, r, p, b, e);
-            ln = (skip && e - b > cap) ? 0 : min(e - b, cap + 1);
+            ln = (skip && e - b > mthr) ? 0 : min(e - b, cap + 1);
             cnt = (ln + kPiece - 1) / kPiece;
             pk = q.level == 1 ? r : (r - 1) & 1;
             bb = b;
@@ -1022,7 +1031,7 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
         return (int)tp;
     };
     int npieces = set_pieces(merge);
-    if (radix) {
+    if (radix || merge) {
         // radix path: the entries below the bound gathered into L.win and
         // sorted there.  When more than kRadixMax of them, a histogram of
         // the first gather gives a lower bound holding cap + cap/4 entries
@@ -1034,13 +1043,13 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
         int32_t* B = L.cursor;
         uint32_t* C = reinterpret_cast<uint32_t*>(L.cursor + kRadixMax);
         int32_t lim = (int32_t)span;
-        const bool hist = total_cut - (merge ? (int64_t)nbnd * (cap + 1) : 0) > kRadixMax;
+        const bool hist = total_cut - merged_cut > kRadixMax;
         int hsh = 0;
         while (((lim - 1) >> hsh) >= kHistBuckets) ++hsh;
         if (hist)
             for (int k = threadIdx.x; k < kHistBuckets; k += kBT) B[k] = 0;
         int M = radix_gather(q, nr, cap, long_ok, nlong, npieces, lim, A, hist ? B : nullptr, hsh,
-                             merge);
+                             skip_over);
         bool cut = false;
         if (M > kRadixMax) {
             static_assert(kHistBuckets == 2 * kBT, "two buckets per thread");
@@ -1058,7 +1067,7 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
             if (cb >= 0 && L.cut_n <= kRadixMax) {
                 lim = (int32_t)min((int64_t)lim, (int64_t)(cb + 1) << hsh);
                 cut = true;
-                M = radix_gather(q, nr, cap, long_ok, nlong, npieces, lim, A, nullptr, 0, merge);
+                M = radix_gather(q, nr, cap, long_ok, nlong, npieces, lim, A, nullptr, 0, skip_over);
             }
         }
         if (M <= kRadixMax) {

@@ -352,3 +352,45 @@ def test_bfs_long_rows_merged(hubs, mids, shared, cap, mode):
     for r, s in zip(roots, ctx.bfs(roots)):
         ref = ob.bfs(G, int(r), cap)
         assert np.array_equal(s, ref), f"root {r}: {s[:8]} vs {ref[:8]} ({s.size} vs {ref.size})"
+
+
+def wide_root_graph(n=5000, seed=5):
+    """Paper 2500 cites 700 earlier papers and is cited by 900 later ones;
+    paper n-1 cites 600 (paper 2500 among them); every other paper cites
+    three earlier ones.  Both 1-hop rows of paper 2500, and the forward row
+    of paper n-1, are longer than one wavefront sorts."""
+    rng = np.random.default_rng(seed)
+    src, dst = [], []
+    for i in range(1, n - 1):
+        if i == 2500:
+            t = rng.choice(2500, size=700, replace=False)
+        else:
+            t = rng.choice(i, size=min(3, i), replace=False)
+        src.append(np.full(t.size, i)); dst.append(t)
+    citers = rng.choice(np.arange(2501, n - 1), size=900, replace=False)
+    src.append(citers); dst.append(np.full(citers.size, 2500))
+    t = np.unique(np.r_[rng.choice(n - 1, size=599, replace=False), 2500])[:600]
+    src.append(np.full(t.size, n - 1)); dst.append(t)
+    src = np.concatenate(src).astype(np.int32)
+    dst = np.concatenate(dst).astype(np.int32)
+    keep = np.unique(src.astype(np.int64) * n + dst, return_index=True)[1]
+    return pg.SeedGraph(np.zeros(n, np.int32), src[keep], dst[keep], np.zeros(n + 10, np.float32))
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("cap", [4096, 1000, 300, 1])
+def test_bfs_wide_roots_merged(cap, mode):
+    """Roots whose 1-hop rows exceed one wavefront's sort: the rows are
+    merged (both rows long, only the backward or only the forward one)."""
+    g = wide_root_graph()
+    ctx = new_ctx(g, [1], 1, cap=cap)
+    G = oracle_graph(ctx, g.logfit)
+    roots = np.array([2500, g.n - 1, 0, 17, 2501, 4000], np.int32)
+    refs = [ob.bfs(G, int(r), cap) for r in roots]
+    # default tier limits, then the smallest ones (paper n-1, 600 1-hop
+    # entries, takes the workgroup path)
+    for small_max, mid_max in [(0, 0), (256, 512)]:
+        ctx.set_tuning(bfs_mode=mode, bfs_small_max=small_max, bfs_mid_max=mid_max)
+        log(f"wide-root BFS cap={cap} mode={mode} tier limits {small_max}/{mid_max}")
+        for r, s, ref in zip(roots, ctx.bfs(roots), refs):
+            assert np.array_equal(s, ref), f"root {r}: {s[:8]} vs {ref[:8]} ({s.size} vs {ref.size})"
