@@ -1234,6 +1234,7 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
         // a thread's rows kRB at a time: their offsets, then their next
         // entries, are loaded together before any row is walked
         constexpr int kRB = 2;
+        constexpr int kWalk = 8;
         for (int r0 = threadIdx.x; r0 < nr; r0 += kBT * kRB) {
             g_cint* pp[kRB];
             int cc[kRB], ee[kRB];
@@ -1260,11 +1261,27 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
                 if (!mine[k]) continue;
                 int c = cc[k];
                 int32_t x = x0[k];
-                while (c < ee[k]) {
+                bool more = c < ee[k];
+                while (more) {
                     if (x >= hi) { nxt = min(nxt, x); break; }
                     atomicOr(&bm[(x - lo) >> 5], 1u << ((x - lo) & 31));
                     ++c;
-                    if (c < ee[k]) x = pp[k][c];
+                    more = c < ee[k];
+                    if (!more) break;
+                    // the next kWalk entries loaded together: one memory
+                    // round trip per kWalk entries of the walk
+                    int32_t nx[kWalk];
+#pragma unroll
+                    for (int j = 0; j < kWalk; ++j) nx[j] = c + j < ee[k] ? pp[k][c + j] : INT_MAX;
+                    x = nx[0];
+#pragma unroll
+                    for (int j = 1; j < kWalk; ++j) {
+                        if (x >= hi) break;
+                        atomicOr(&bm[(x - lo) >> 5], 1u << ((x - lo) & 31));
+                        ++c;
+                        x = nx[j];
+                    }
+                    more = c < ee[k];
                 }
                 L.cursor[r0 + k * kBT] = c;
             }
