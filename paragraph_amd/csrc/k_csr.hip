@@ -92,26 +92,49 @@ __global__ __launch_bounds__(kRT) void k_radix_hist(const uint32_t* __restrict__
     hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = (int32_t)h[threadIdx.x];
 }
 
+// One tile of kTile (key, value) pairs per block, stably ordered by the
+// digit (key >> shift) & 255 in LDS and then written out, so that each
+// digit's run of the tile goes to consecutive global positions (consecutive
+// threads, consecutive addresses) instead of one scattered word per pair.
+// Wavefront w ranks the tile's pairs [w*kPerWave, (w+1)*kPerWave), 64 per
+// round: a pair's rank among the round's lanes with its digit comes from
+// eight ballots, its rank before the round from the wavefront's digit
+// counters (LDS, one wavefront each); the tile's digit starts are a scan of
+// the counters, and offs (the exclusive scan of the digit-major per-tile
+// histogram) gives each digit's global start for this tile.
+constexpr int kPerWave = kTile / (kRT / 64);  // 1024
+constexpr int kWR = kPerWave / 64;            // rounds per wavefront
 __global__ __launch_bounds__(kRT) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                        const uint32_t* __restrict__ vin,
                                                        uint32_t* __restrict__ kout,
                                                        uint32_t* __restrict__ vout, int64_t n,
                                                        int shift, const int64_t* __restrict__ offs,
                                                        int64_t ntiles) {
-    __shared__ int64_t s_base[256];
+    __shared__ uint32_t s_key[kTile];
+    __shared__ uint32_t s_val[kTile];
     __shared__ uint32_t s_cnt[kRT / 64][256];
+    __shared__ int32_t s_lb[256];
+    __shared__ int64_t s_gb[256];
+    __shared__ int32_t s_wsum[kRT / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    s_base[threadIdx.x] = offs[(int64_t)threadIdx.x * ntiles + blockIdx.x];
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+    const int tn = (int)(n - base < kTile ? n - base : kTile);
     for (int w = 0; w < kRT / 64; ++w) s_cnt[w][threadIdx.x] = 0;
+    s_gb[threadIdx.x] = offs[(int64_t)threadIdx.x * ntiles + blockIdx.x];
     __syncthreads();
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int64_t base = (int64_t)blockIdx.x * kTile;
-    for (int r = 0; r < kRounds; ++r) {
-        const int64_t idx = base + r * kRT + threadIdx.x;
-        const bool valid = idx < n;
-        const uint32_t k = valid ? kin[idx] : 0u;
-        const uint32_t v = valid ? vin[idx] : 0u;
-        const uint32_t d = (k >> shift) & 255u;
+    uint32_t kk[kWR], vv[kWR];
+    int rk[kWR];
+#pragma unroll
+    for (int r = 0; r < kWR; ++r) {
+        const int j = wid * kPerWave + r * 64 + lane;
+        kk[r] = j < tn ? kin[base + j] : 0u;
+        vv[r] = j < tn ? vin[base + j] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kWR; ++r) {
+        const bool valid = wid * kPerWave + r * 64 + lane < tn;
+        const uint32_t d = (kk[r] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -119,20 +142,47 @@ __global__ __launch_bounds__(kRT) void k_radix_scatter(const uint32_t* __restric
             const uint64_t mb = __ballot(bit);
             peers &= bit ? mb : ~mb;
         }
-        const int rank = __popcll(peers & below);
-        if (valid && rank == 0) s_cnt[wid][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            int64_t pos = s_base[d] + rank;
-            for (int w = 0; w < wid; ++w) pos += s_cnt[w][d];
-            kout[pos] = k;
-            vout[pos] = v;
+        const int before = (int)s_cnt[wid][d];
+        rk[r] = before + __popcll(peers & below);
+        if (valid && (peers & below) == 0) atomicAdd(&s_cnt[wid][d], (uint32_t)__popcll(peers));
+    }
+    __syncthreads();
+    // digit d = thread: its count over the wavefronts (the counters become
+    // the wavefronts' exclusive prefixes) and the tile-local start
+    int tot = 0;
+#pragma unroll
+    for (int w = 0; w < kRT / 64; ++w) {
+        const int c = (int)s_cnt[w][threadIdx.x];
+        s_cnt[w][threadIdx.x] = (uint32_t)tot;
+        tot += c;
+    }
+    int x = tot;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_wsum[wid] = x;
+    __syncthreads();
+    int pre = x - tot;
+    for (int w = 0; w < wid; ++w) pre += s_wsum[w];
+    s_lb[threadIdx.x] = pre;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kWR; ++r) {
+        if (wid * kPerWave + r * 64 + lane < tn) {
+            const uint32_t d = (kk[r] >> shift) & 255u;
+            const int pos = s_lb[d] + (int)s_cnt[wid][d] + rk[r];
+            s_key[pos] = kk[r];
+            s_val[pos] = vv[r];
         }
-        __syncthreads();
-        uint32_t t = 0;
-        for (int w = 0; w < kRT / 64; ++w) { t += s_cnt[w][threadIdx.x]; s_cnt[w][threadIdx.x] = 0; }
-        s_base[threadIdx.x] += t;
-        __syncthreads();
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < tn; j += kRT) {
+        const uint32_t k = s_key[j];
+        const uint32_t d = (k >> shift) & 255u;
+        const int64_t g = s_gb[d] + (j - s_lb[d]);
+        kout[g] = k;
+        vout[g] = s_val[j];
     }
 }
 
