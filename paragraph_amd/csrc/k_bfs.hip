@@ -23,9 +23,11 @@
 // (k_bfs_small, k_bfs_mid: rows gathered into LDS, sorted by a register
 // bitonic network, compacted by ballots), then one 512-thread workgroup per
 // root (k_bfs_root): the entries below the answer bound are radix-sorted in
-// LDS, or, when they do not fit, staged in id buckets in a device slot or
-// swept through the id range in bitmap windows.  Per-query work follows the
-// adjacency touched below the final threshold, not N.
+// LDS -- up to four rows longer than the cap (level 2) or than one
+// wavefront's sort (level 1) are merged into the sorted list instead, rows
+// being ascending -- or, when they do not fit, staged in id buckets in a
+// device slot or swept through the id range in bitmap windows.  Per-query
+// work follows the adjacency touched below the final threshold, not N.
 //
 // The neighbourhood CDF is exact 64-bit fixed point (weights scaled so the
 // largest maps below 2^50), built by an integer scan, so it is
