@@ -78,6 +78,7 @@ struct BfsLds {
     int32_t n1c;
     int32_t cut_b;                // radix path: last histogram bucket below the cut
     int32_t cut_n;                // radix path: entries up to and including it
+    int32_t s_lds;                // level 2: where S also is in LDS (0 none, 1 win, 2 cursor, 3 f1)
     int64_t t_start;              // wall clock at the workgroup's start
     int64_t t_level1;             // wall clock after the 1-hop level
 };
@@ -773,7 +774,7 @@ __device__ __forceinline__ int32_t* radix_sort(int32_t* A, int32_t* B, uint32_t*
 // returns how many there are (uncapped).  Each thread takes a contiguous
 // run.  All threads call it.
 __device__ __forceinline__ int radix_emit(const Query& q, const int32_t* Z, int M, int cap,
-                                          int32_t* dst = nullptr) {
+                                          int32_t* dst = nullptr, int32_t* also = nullptr) {
     const int chunk = (M + kBT - 1) / kBT;
     const int s0 = min(M, (int)threadIdx.x * chunk);
     const int s1 = min(M, s0 + chunk);
@@ -788,6 +789,7 @@ __device__ __forceinline__ int radix_emit(const Query& q, const int32_t* Z, int 
         const int32_t x = Z[k];
         if ((k == 0 || Z[k - 1] != x) && keep_id(q, x)) {
             if (dst) dst[pos] = x; else put_out(q, pos, x);
+            if (also) also[pos] = x;
             ++pos;
         }
     }
@@ -889,6 +891,7 @@ __device__ __forceinline__ int merge_bound(const Query& q, const int32_t* Z, int
         Y = t;
     }
     for (int t = threadIdx.x; t < np; t += kBT) put_out(q, t, X[t]);
+    if (threadIdx.x == 0) L.s_lds = X == L.win ? 1 : 2;
     __syncthreads();  // the output is complete
     return np;
 }
@@ -939,7 +942,7 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
             return L.nout;
         }
     }
-    if (threadIdx.x == 0) { L.nlong = 0; L.nbnd = 0; }
+    if (threadIdx.x == 0) { L.nlong = 0; L.nbnd = 0; L.s_lds = 0; }
     __syncthreads();
     // row entries (bits 26 and up) and entries cut at cap+1 per row (bits
     // 0-25: at most kMaxRows * (kBfsCapMax + 1) < 2^26)
@@ -1097,11 +1100,15 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
                 const int kept = merge_bound(q, Z, M, cap, lim, nbnd);
                 if (!cut || kept >= cap) return kept;
             } else {
-                const int kept = radix_emit(q, Z, M, cap);
+                // (a copy of S in the buffer Z is not in, for the CDF)
+                int32_t* also = q.level == 2 ? (Z == L.win ? L.cursor : L.win) : nullptr;
+                if (threadIdx.x == 0 && also) L.s_lds = Z == L.win ? 2 : 1;
+                const int kept = radix_emit(q, Z, M, cap, nullptr, also);
                 if (!cut || kept >= cap) return min(kept, cap);
             }
         }
         if (merge) npieces = set_pieces(false);
+        if (threadIdx.x == 0) L.s_lds = 0;  // the paths below write S afresh
         // the sweep's row cursors again
         for (int r = threadIdx.x; r < nr; r += kBT) {
             g_cint* p; int b, e;
@@ -1205,6 +1212,7 @@ __device__ __forceinline__ int collect(Query& q, int cap) {
             int tid = (int)threadIdx.x;  // opaque, as above
             asm volatile("" : "+v"(tid));
             for (int k = tid; k < no; k += kBT) q.S[k] = L.f1[k];
+            if (tid == 0) L.s_lds = 3;
             __syncthreads();
             q.out_lds = false;
         }
@@ -1411,8 +1419,11 @@ __global__ __launch_bounds__(kBT, 4) void k_bfs_root(CsrView g, const float* __r
     const int s0 = threadIdx.x * per, s1 = min(ns, s0 + per);
     int32_t xs[kMaxPer];
     float wv[kMaxPer];
+    // S as the level left it in LDS, when it did (else the global copy)
+    const int sl = L.s_lds;
+    const int32_t* Sl = sl == 1 ? L.win : sl == 2 ? L.cursor : sl == 3 ? L.f1 : nullptr;
 #pragma unroll
-    for (int e = 0; e < kMaxPer; ++e) xs[e] = s0 + e < s1 ? S[s0 + e] : 0;
+    for (int e = 0; e < kMaxPer; ++e) xs[e] = s0 + e < s1 ? (Sl ? Sl[s0 + e] : S[s0 + e]) : 0;
 #pragma unroll
     for (int e = 0; e < kMaxPer; ++e) wv[e] = s0 + e < s1 ? w[xs[e]] : 0.0f;
     uint32_t mb = 0;
@@ -1420,10 +1431,10 @@ __global__ __launch_bounds__(kBT, 4) void k_bfs_root(CsrView g, const float* __r
     for (int e = 0; e < kMaxPer; ++e) mb = max(mb, f2u(wv[e]));
     for (int off = 32; off > 0; off >>= 1) mb = max(mb, (uint32_t)__shfl_down((int)mb, off, 64));
     __syncthreads();
-    if ((threadIdx.x & 63) == 0) L.win[threadIdx.x >> 6] = (int32_t)mb;
+    if ((threadIdx.x & 63) == 0) L.red[threadIdx.x >> 6] = (int64_t)mb;
     __syncthreads();
     uint32_t gmb = 0;
-    for (int k = 0; k < kWaves; ++k) gmb = max(gmb, (uint32_t)L.win[k]);
+    for (int k = 0; k < kWaves; ++k) gmb = max(gmb, (uint32_t)L.red[k]);
     const double scale = pow2d(49 - ((int)(gmb >> 23) - 127));
     int64_t acc = 0;
 #pragma unroll
