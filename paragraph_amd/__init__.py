@@ -610,9 +610,11 @@ class Context:
     def set_tuning(self, pa_lanes: int = 0, lik_lanes: int = 0, bfs_mode: int = 0,
                    bfs_small_max: int = 0, bfs_mid_max: int = 0, merge_out: int = 0):
         """Launch shapes (speed only): PA lanes per paper, likelihood lanes
-        per citing paper, 2-hop workgroup path mode (0 staged up to 65536
-        candidate entries, 1 always swept, 2 staged whenever it fits), the
-        one-wavefront 2-hop tiers' candidate limits, the merge tile."""
+        per citing paper, 2-hop workgroup path mode for lists above 4096
+        candidate entries (0 radix-sorted in LDS when they fit, else staged
+        up to 65536 entries, else swept; 1 always swept; 2 staged whenever
+        it fits), the one-wavefront 2-hop tiers' candidate limits, the
+        merge tile."""
         t = Tuning(ctypes.sizeof(Tuning), pa_lanes, lik_lanes, bfs_mode, bfs_small_max,
                    bfs_mid_max, merge_out)
         self._check(lib().pg_set_tuning(self._h, ctypes.byref(t)), "pg_set_tuning")
