@@ -1884,7 +1884,10 @@ __global__ __launch_bounds__(256) void k_nb_draw(Batch bt, int64_t n_exist, int6
         qidx = q_idx + u * kQIdx;
         for (int j = gl; j < c0; j += G) s_row[grp][j] = row[j];
     }
-    __syncthreads();
+    // a paper's lanes (one wavefront) are all that read its LDS rows and
+    // picks, so the rounds below synchronise wavefronts, not the block: a
+    // wavefront stops with its own papers, not with the block's slowest
+    wave_lds_sync();
     // PA picks inside S (only needed when taking all of S \ row is possible)
     int inS = 0;
     if (live && ns <= c0 + n_nb) {
@@ -1901,7 +1904,7 @@ __global__ __launch_bounds__(256) void k_nb_draw(Batch bt, int64_t n_exist, int6
     // take all of S \ row, in S order
     bool act = take_all;
     int sb = 0;
-    while (__syncthreads_or(act)) {
+    while (__ballot(act) != 0) {
         bool keep = false;
         int32_t x = 0;
         if (act) {
@@ -1925,13 +1928,14 @@ __global__ __launch_bounds__(256) void k_nb_draw(Batch bt, int64_t n_exist, int6
             if (sb >= ns) act = false;
         }
     }
+    wave_lds_sync();  // the take-all rows are written
     // rejection draws over the neighbourhood's fixed-point CDF
     bool need = live && !take_all;
     const uint64_t Zq = (live && ns > 0) ? Q[ns - 1] : 0;
     const int qsh = q_shift(Zq);
     const int b_max = kAttemptsPerCite * n_nb;
     int got = 0, b0 = 0;
-    while (__syncthreads_or(need)) {
+    while (__ballot(need) != 0) {
         const int b = b0 + gl;
         const bool valid = need && b < b_max;
         int32_t pick = -1;
@@ -1946,7 +1950,7 @@ __global__ __launch_bounds__(256) void k_nb_draw(Batch bt, int64_t n_exist, int6
             pick = S[lo];
         }
         s_pick[threadIdx.x] = pick;
-        __syncthreads();
+        wave_lds_sync();
         bool dup = !valid;
         if (valid) {
             const int g0 = grp * G;
@@ -1956,7 +1960,7 @@ __global__ __launch_bounds__(256) void k_nb_draw(Batch bt, int64_t n_exist, int6
         const uint64_t acc = __ballot(!dup) & gmask;
         const int pos = got + __popcll(acc & below);
         const bool take = !dup && pos < n_nb;
-        __syncthreads();
+        wave_lds_sync();
         if (take) {
             row[c + pos - got] = pick;
             kr[c + pos - got] = 1;
@@ -1970,6 +1974,7 @@ __global__ __launch_bounds__(256) void k_nb_draw(Batch bt, int64_t n_exist, int6
             if (got >= n_nb || b0 >= b_max) need = false;
         }
     }
+    wave_lds_sync();  // the drawn rows are written
     // refill with PA draws, continuing the attempt sequence
     if (live && gl == 0) {
         const double Zg = cdf[n_exist - 1];
