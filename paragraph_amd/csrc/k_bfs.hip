@@ -537,14 +537,6 @@ __device__ __forceinline__ int32_t med3_i32(int32_t a, int32_t b, int32_t c) {
 // and a select.
 __device__ __forceinline__ void opaque(int32_t& x) { asm("" : "+v"(x)); }
 
-// Orders this wavefront's LDS accesses before the barrier against its
-// accesses after it, across lanes (a compiler barrier; the LDS serves one
-// wavefront's instructions in order).
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Bitonic sort of s[0,P) (P = 64E a power of two <= 4096) by one wavefront,
 // in registers, lane-major: lane l holds elements [lE, lE+E) in registers

@@ -63,7 +63,10 @@ __global__ __launch_bounds__(256) void k_pa_draw(Batch bt, int64_t n_exist,
 
     int c = 0, a0 = 0, nr = 0, a_stop = a_max;
     bool need = live && n_pa > 0 && a_max > 0;
-    while (__syncthreads_or(need)) {
+    // a paper's G lanes and its LDS rows lie in one wavefront (G <= 64), so
+    // the rounds synchronise wavefronts: a wavefront stops with its own
+    // papers, not with the slowest paper of the block
+    while (__ballot(need) != 0) {
         const int a = a0 + gl;
         const bool valid = need && a < a_max;
         int32_t pick = -1;
@@ -76,7 +79,7 @@ __global__ __launch_bounds__(256) void k_pa_draw(Batch bt, int64_t n_exist,
             pick = (int32_t)k;
         }
         s_pick[threadIdx.x] = pick;
-        __syncthreads();
+        wave_lds_sync();
         bool dup = !valid;
         if (valid) {
             const int g0 = grp * G;
@@ -116,7 +119,7 @@ __global__ __launch_bounds__(256) void k_pa_draw(Batch bt, int64_t n_exist,
                 if (c < n_pa) a_stop = a_max;
             }
         }
-        __syncthreads();
+        wave_lds_sync();
     }
     if (live && gl == 0) {
         for (int j = c; j < m; ++j) { cit[i * m + j] = -1; kind[i * m + j] = -1; }

@@ -24,6 +24,15 @@ __device__ __forceinline__ int64_t cdf_upper_bound(const double* __restrict__ cd
     return lo < n ? lo : n - 1;
 }
 
+// Orders this wavefront's LDS accesses before the barrier against its
+// accesses after it, across lanes (a compiler barrier; the LDS serves one
+// wavefront's instructions in order).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Bucket index of a root's fixed-point neighbourhood CDF Q[0,ns) (total Z):
 // bucket of a value x is x >> q_shift(Z) (at most 63), monotone in x, and
 // qidx[k] (k <= kQBuckets) is the first s whose bucket is >= k, ns if none;
