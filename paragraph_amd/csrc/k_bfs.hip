@@ -1399,7 +1399,7 @@ __global__ __launch_bounds__(kBT, 4) void k_bfs_root(CsrView g, const float* __r
         ns_buf[u] = ns;
         const int64_t b = 16 * (1 + (int64_t)L.n1c) + 4 * (L.tot[1] + L.tot[2] - L.n1c) +
                           4 * (int64_t)ns;
-        if (bytes_out) atomicAdd(bytes_out, (unsigned long long)b);
+        if (bytes_out) atomicAdd(&bytes_out[(blockIdx.x % kByteSlots) * kByteSlotStride], (unsigned long long)b);
     }
     if (!w) return;  // S only (no weights given): no CDF
     __syncthreads();
@@ -1720,7 +1720,7 @@ __device__ __forceinline__ void bfs_wave(CsrView g, const float* __restrict__ w,
     if (lane == 0) {
         ns_buf[u] = ns;
         const int64_t by = 16 * (1 + (int64_t)n1c) + 4 * ((int64_t)d1 + tot2 - n1c) + 4 * (int64_t)ns;
-        if (bytes_out) atomicAdd(bytes_out, (unsigned long long)by);
+        if (bytes_out) atomicAdd(&bytes_out[(blockIdx.x % kByteSlots) * kByteSlotStride], (unsigned long long)by);
     }
 }
 
@@ -1995,6 +1995,19 @@ __global__ __launch_bounds__(256) void k_nb_draw(Batch bt, int64_t n_exist, int6
     }
 }
 
+// The 2-hop kernels' algorithmic-byte counts, spread over kByteSlots
+// counters a cache line apart (one same-address atomic per root had queued
+// tens of thousands of atomics on one L2 channel), summed into *out and
+// cleared for the next launch.
+__global__ __launch_bounds__(64) void k_fold_bytes(unsigned long long* __restrict__ slots,
+                                                   unsigned long long* __restrict__ out) {
+    static_assert(kByteSlots == 64, "one lane per counter");
+    unsigned long long v = slots[threadIdx.x * kByteSlotStride];
+    slots[threadIdx.x * kByteSlotStride] = 0;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (threadIdx.x == 0) atomicAdd(out, v);
+}
+
 }  // namespace
 
 void launch_root_dedup(hipStream_t s, int64_t n_exist, int64_t n_new, int m, const int32_t* cit,
@@ -2018,7 +2031,11 @@ void launch_root_dedup(hipStream_t s, int64_t n_exist, int64_t n_new, int m, con
 void launch_bfs_root(hipStream_t s, const CsrView& g, const float* w, const int32_t* uroot,
                      const int64_t* n_unique, int64_t max_unique, int cap, int32_t* S_buf,
                      uint64_t* Q_buf, int32_t* q_idx, int32_t* ns_buf, int32_t* big_list,
-                     int32_t* big_count, unsigned long long* bytes_out, const BfsStage& stage) {
+                     int32_t* big_count, unsigned long long* bytes_out, const BfsStage& stage,
+                     unsigned long long* byte_slots) {
+    // the kernels count into the spread slots, folded into bytes_out after
+    unsigned long long* const out = bytes_out;
+    bytes_out = out ? byte_slots : nullptr;
     if (max_unique <= 0) return;
     int32_t* mid_list = big_list + max_unique;
     const int small_max = stage.small_max > 0 ? std::min(stage.small_max, kSmallWin) : kSmallWin;
@@ -2032,6 +2049,7 @@ void launch_bfs_root(hipStream_t s, const CsrView& g, const float* w, const int3
     hipLaunchKernelGGL(k_bfs_root, dim3((uint32_t)max_unique), dim3(kBT), 0, s, g, w, uroot,
                        mid_list, big_count + 1, cap, S_buf, Q_buf, q_idx, ns_buf, bytes_out,
                        (const int32_t*)nullptr, (int32_t*)nullptr, (int64_t*)nullptr, stage);
+    if (out && byte_slots) hipLaunchKernelGGL(k_fold_bytes, dim3(1), dim3(64), 0, s, byte_slots, out);
 }
 
 void launch_nb_draw(hipStream_t s, int64_t n_exist, int64_t n_new, int m, int n_nb, int cap,

@@ -39,7 +39,7 @@ void release(pg_ctx* c) {
                     c->near, c->anext, c->new_off, c->src_of, c->k_sorted,
                     c->v_sorted, c->k_tmp, c->v_tmp, c->scratch, c->d_sum, c->d_tmp_i32,
                     c->s_birth, c->s_in_deg, c->s_fwd_off, c->s_fwd_adj, c->s_bwd_off,
-                    c->s_bwd_adj, c->S_buf, c->Q_buf, c->Q_idx, c->ns_buf, c->root_bits, c->r_wcount,
+                    c->s_bwd_adj, c->S_buf, c->Q_buf, c->Q_idx, c->ns_buf, c->byte_slots, c->root_bits, c->r_wcount,
                     c->r_wprefix, c->r_excl,
                     c->root_slot, c->uroot, c->big_list, c->stage.buf, c->stage.busy};
     for (void* p : ptrs)
@@ -143,6 +143,9 @@ int ensure_nb_buffers(pg_ctx* ctx) {
     if (r == hipSuccess) r = dalloc(&ctx->Q_buf, D * cap);
     if (r == hipSuccess) r = dalloc(&ctx->Q_idx, D * kQIdx);
     if (r == hipSuccess) r = dalloc(&ctx->ns_buf, D);
+    if (r == hipSuccess) r = dalloc(&ctx->byte_slots, (int64_t)kByteSlots * kByteSlotStride);
+    if (r == hipSuccess)
+        r = hipMemset(ctx->byte_slots, 0, (size_t)kByteSlots * kByteSlotStride * sizeof(unsigned long long));
     const int64_t nw = ctx->cfg.max_papers / 32 + 1;
     if (r == hipSuccess) r = dalloc(&ctx->root_bits, nw);
     if (r == hipSuccess) r = hipMemset(ctx->root_bits, 0, (size_t)nw * sizeof(uint32_t));
@@ -419,7 +422,8 @@ int pg_year_step(pg_ctx* ctx, const pg_year_params* p, pg_year_stats* stats) {
                                 ctx->cfg.bfs_cap, ctx->S_buf, ctx->Q_buf, ctx->Q_idx, ctx->ns_buf,
                                 ctx->big_list, big_count, sums + 2,
                                 bfs_stage_view(ctx->stage, ctx->tune_bfs_mode,
-                                               ctx->tune_small_max, ctx->tune_mid_max));
+                                               ctx->tune_small_max, ctx->tune_mid_max),
+                                ctx->byte_slots);
                 pend(PG_K_BFS);
                 pbeg2(PG_K_NB_DRAW);
                 launch_nb_draw(s, N, D, m, n_nb, ctx->cfg.bfs_cap, seed, yi, ctx->cdf,

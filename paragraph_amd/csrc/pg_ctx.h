@@ -182,6 +182,7 @@ struct pg_ctx {
     uint64_t* Q_buf = nullptr;
     int32_t* Q_idx = nullptr;  // kQIdx bucket starts of each root's Q
     int32_t* ns_buf = nullptr;
+    unsigned long long* byte_slots = nullptr;  // [kByteSlots * kByteSlotStride] 2-hop byte counts, zero between launches
     uint32_t* root_bits = nullptr; // [max_papers/32] roots of the year, zero between years
     int32_t* r_wcount = nullptr;   // [max_papers/32] popcounts of root_bits
     int64_t* r_wprefix = nullptr;  // [max_papers/32 + 1] their exclusive prefix

@@ -111,7 +111,8 @@ void enqueue_year(pg_ensemble* ens, int y, hipStream_t s, hipEvent_t* bfs_ev = n
         launch_bfs_root(s, g, c->w, c->uroot, c->r_excl + Dn, Dn, c->cfg.bfs_cap, c->S_buf,
                         c->Q_buf, c->Q_idx, c->ns_buf, c->big_list, big_count, cnt + 2,
                         bfs_stage_view(c->stage, c->tune_bfs_mode, c->tune_small_max,
-                                       c->tune_mid_max));
+                                       c->tune_mid_max),
+                        c->byte_slots);
         if (bfs_ev) (void)hipEventRecord(bfs_ev[1], s);
         launch_nb_draw_batch(s, b, m, c->cfg.bfs_cap, (uint32_t)y, c->cdf, c->root_slot,
                              c->S_buf, c->Q_buf, c->Q_idx, c->ns_buf, c->cit, c->kind, c->cnt, c->near,

@@ -38,6 +38,10 @@ __device__ __forceinline__ void wave_lds_sync() {
 // qidx[k] (k <= kQBuckets) is the first s whose bucket is >= k, ns if none;
 // the first s with Q[s] > t then lies in [qidx[key(t)], qidx[key(t) + 1]].
 constexpr int kQBuckets = 64;
+// Spread counters for the 2-hop kernels' byte counts (u64, kByteSlotStride
+// apart: one cache line each).
+constexpr int kByteSlots = 64;
+constexpr int kByteSlotStride = 16;
 constexpr int kQIdx = kQBuckets + 1;  // int32 per root
 
 __device__ __forceinline__ int q_shift(uint64_t Z) {
@@ -264,7 +268,8 @@ void launch_bfs_root(hipStream_t s, const CsrView& g, const float* w, const int3
                      const int64_t* n_unique, int64_t max_unique, int cap, int32_t* S_buf,
                      uint64_t* Q_buf, int32_t* q_idx, int32_t* ns_buf, int32_t* big_list,
                      int32_t* big_count,
-                     unsigned long long* bytes_out, const BfsStage& stage);
+                     unsigned long long* bytes_out, const BfsStage& stage,
+                     unsigned long long* byte_slots);
 // Per paper: neighbourhood draws from its root's S, then PA refill.
 void launch_nb_draw(hipStream_t s, int64_t n_exist, int64_t n_new, int m, int n_nb, int cap,
                     uint64_t seed, uint32_t year_index, const double* cdf, const int32_t* cdf_idx,

@@ -114,7 +114,8 @@ extern "C" int pg_bfs_noalloc(pg_ctx* ctx, int64_t count, const int32_t* roots_d
                     reinterpret_cast<const int64_t*>(counters + 8), count, ctx->cfg.bfs_cap,
                     out_dev, nullptr, nullptr, out_len_dev, lists, counters, nullptr,
                     bfs_stage_view(pool, ctx->tune_bfs_mode, ctx->tune_small_max,
-                                   ctx->tune_mid_max));
+                                   ctx->tune_mid_max),
+                    nullptr);
     PG_HIP(hipGetLastError());
     return PG_OK;
 }
