@@ -123,11 +123,14 @@ int ensure_bfs_stage(pg_ctx* ctx) {
     BfsStage st;
     hipError_t r = dalloc(&st.buf, (int64_t)slots * kBfsStageEntries);
     if (r == hipSuccess) r = dalloc(&st.busy, slots);
-    if (r == hipSuccess) r = hipMemset(st.busy, 0, (size_t)slots * sizeof(int32_t));
     if (r != hipSuccess) {
         (void)hipFree(st.buf); (void)hipFree(st.busy);
         return fail(ctx, PG_ENOMEM, "2-hop staging pool: device allocation failed");
     }
+    // Cleared on the context stream: a null-stream hipMemset is not ordered
+    // with the non-blocking context stream, so the year's kernels could see
+    // the flags before the clear lands.
+    launch_fill_i32(ctx->stream, st.busy, 0, slots);
     st.n_slots = slots;
     st.cap_entries = kBfsStageEntries;
     ctx->stage = st;
@@ -144,11 +147,8 @@ int ensure_nb_buffers(pg_ctx* ctx) {
     if (r == hipSuccess) r = dalloc(&ctx->Q_idx, D * kQIdx);
     if (r == hipSuccess) r = dalloc(&ctx->ns_buf, D);
     if (r == hipSuccess) r = dalloc(&ctx->byte_slots, (int64_t)kByteSlots * kByteSlotStride);
-    if (r == hipSuccess)
-        r = hipMemset(ctx->byte_slots, 0, (size_t)kByteSlots * kByteSlotStride * sizeof(unsigned long long));
     const int64_t nw = ctx->cfg.max_papers / 32 + 1;
     if (r == hipSuccess) r = dalloc(&ctx->root_bits, nw);
-    if (r == hipSuccess) r = hipMemset(ctx->root_bits, 0, (size_t)nw * sizeof(uint32_t));
     if (r == hipSuccess) r = dalloc(&ctx->r_wcount, nw);
     if (r == hipSuccess) r = dalloc(&ctx->r_wprefix, nw + 1);
     if (r == hipSuccess) r = dalloc(&ctx->r_excl, D + 1);
@@ -157,6 +157,13 @@ int ensure_nb_buffers(pg_ctx* ctx) {
     if (r == hipSuccess) r = dalloc(&ctx->big_list, 2 * D);
     if (r != hipSuccess)
         return fail(ctx, PG_ENOMEM, "neighbourhood buffers: device allocation failed");
+    // Zeroed on the context stream, ahead of the kernels that use them (this
+    // runs between launches of a year step, and that stream does not wait on
+    // the null stream).
+    launch_fill_i32(ctx->stream, reinterpret_cast<int32_t*>(ctx->byte_slots), 0,
+                    2 * (int64_t)kByteSlots * kByteSlotStride);
+    launch_fill_i32(ctx->stream, reinterpret_cast<int32_t*>(ctx->root_bits), 0, nw);
+    PG_HIP(hipGetLastError());
     return PG_OK;
 }
 

@@ -390,9 +390,11 @@ int pg_ensemble_generate(pg_ensemble* ens, int32_t year_lo, int32_t year_hi, dou
     if (r == hipSuccess) r = dalloc(&zero, 1);
     if (r != hipSuccess) return bail(PG_ENOMEM, "device allocation failed");
     r = hipMemcpy(d_tab, tab.data(), sizeof(GenSim) * S, hipMemcpyHostToDevice);
-    if (r == hipSuccess) r = hipMemset(hist, 0, sizeof(int32_t) * S * span);
-    if (r == hipSuccess) r = hipMemset(zero, 0, sizeof(int32_t));
     if (r != hipSuccess) return bail(PG_EHIP, hipGetErrorString(r));
+    // Cleared on st itself: it is non-blocking, so a null-stream memset
+    // would not be ordered before the histogram kernel.
+    launch_fill_i32(st, hist, 0, (int64_t)S * span);
+    launch_fill_i32(st, zero, 0, 1);
     // birth-year histogram -> per-year prefix (host: S x span values)
     launch_gen_hist(st, d_tab, S, n_flat, (int32_t)span, hist);
     std::vector<int32_t> h((size_t)S * span);
