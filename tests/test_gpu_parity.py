@@ -73,6 +73,24 @@ def test_pa_param_edges(params):
     check_pa_year(ctx, g.logfit, 1, 0, 300, m, n_nb, alpha, tau, 11, 4096)
 
 
+def test_first_year_on_dirty_device_memory():
+    """The buffers a context allocates in its first PA year with
+    neighbourhood draws (root-dedup bitmap, byte counters, staging-slot
+    flags) are cleared on the context's stream before the kernels that use
+    them: after 1 GiB of device memory full of set bits has been freed back
+    to the runtime, the first year still equals the oracle."""
+    import torch
+    junk = torch.full((1 << 28,), -1, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    del junk
+    torch.cuda.empty_cache()
+    sched = pg.growth_schedule(50000, 0.03, 1)
+    g = pg.seed_graph(3, 50000, 8.0, n_final=50000 + sum(sched))
+    ctx = new_ctx(g, sched, 10)
+    st, bad = check_pa_year(ctx, g.logfit, 1, 0, sched[0], 10, 3, 1.0, 5.0, 1, 4096)
+    assert st.shortfall == 0
+
+
 def test_pa_tiny_graph_exhaustion():
     """Fewer eligible papers than citations: attempt budget ends the draws."""
     g = pg.SeedGraph(np.array([-3, -2, -1], np.int32), np.array([1, 2], np.int32),
